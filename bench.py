@@ -1,0 +1,260 @@
+#!/usr/bin/env python3
+"""Benchmark: the config-2 stencil workload on one MI355X.
+
+Workload ("laplacian512_fused_residual"): DIM=3, FP64, one box of 512^3
+interior cells with 2 ghost layers; u = sin sin sin + 1e-3 U (seed 1), f = U
+(seed 1); one step computes, for the 7-point Laplacian() and the 27-point
+Laplacian_27() with scale 1/h^2 (h = 1/512), r = f - L(u) (Proto's forall
+axpy) plus absMax(r) and sum(r) -- both operators in one pass over u and f.
+
+value = algorithmic bytes of the step (u read once over its ghosted box, f
+read once, r7 and r27 written once: 4 320 330 240 B) / step time, in GB/s.
+Inputs are resident in HBM before the timed region.
+
+roofline: the fused kernel's own average duration (HIP events on its stream,
+launches without the tiny final-reduction kernel), against 8 TB/s.
+cpu_baseline: Proto's own CPU path (oracle/_ref, Release flags, one thread)
+running the same program on a bounded z-slab of the same field.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-planes", type=int, default=128, help="z-planes of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--only", default="", help="profile helper: run only this kernel family")
+    return ap.parse_args()
+
+
+def stencils(dtype):
+    import numpy as np
+    import proto_amd as pa
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    # Proto's Laplacian() and Laplacian_27() coefficient lists and orders
+    # (Proto_StencilDefs.H:77-87, 130-149), built the way the library does.
+    c7 = np.array([-6.0, 1, 1, 1, 1, 1, 1], dtype=npdt)
+    o7 = [[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]]
+    c27 = [npdt(-64.0 / 15.0)]
+    o27 = [[0, 0, 0]]
+    w = {1: 7.0 / 15.0, 2: 1.0 / 10.0, 3: 1.0 / 30.0}
+    for z in (-1, 0, 1):
+        for y in (-1, 0, 1):
+            for x in (-1, 0, 1):
+                s = abs(x) + abs(y) + abs(z)
+                if s:
+                    c27.append(npdt(w[s]))
+                    o27.append([x, y, z])
+    return pa.Stencil(c7, o7, dtype), pa.Stencil(np.array(c27, dtype=npdt), o27, dtype)
+
+
+def time_launches(fn, reps, stream):
+    """Average duration (ms) of fn() launched reps times, by HIP events on stream."""
+    from proto_amd.device import Event
+    fn()
+    stream.sync()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.sync()
+    return e0.elapsed_ms(e1) / reps
+
+
+def cpu_baseline(n, g, planes):
+    """Proto's CPU path on a z-slab of `planes` planes of the same config-2
+    field (both operators, forall residual, absMax, sum)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import util
+    lib = util.ref(3)
+    kind = "reference"
+    x = (np.arange(n) + 0.5) / n
+    s = np.sin(2 * np.pi * x)
+    nz = planes
+    noise = util.uniform(1, 0, n * n * nz).reshape(nz, n, n)
+    smooth = (s[None, None, :] * s[None, :, None]) * s[:nz, None, None]
+    ui = smooth + 1e-3 * noise
+    u = np.pad(ui, ((g, g), (g, g), (g, g)), mode="wrap")  # slab-local ghost fill
+    u = np.ascontiguousarray(u.reshape(-1))
+    f = util.uniform(1, 1, n * n * nz)
+    r = np.zeros(n * n * nz)
+    ulo, uhi = util.i3([-g, -g, -g]), util.i3([n - 1 + g, n - 1 + g, nz - 1 + g])
+    lo, hi = util.i3([0, 0, 0]), util.i3([n - 1, n - 1, nz - 1])
+    h = 1.0 / n
+    am, sm = ctypes.c_double(), ctypes.c_double()
+    t0 = time.perf_counter()
+    if lib is not None:
+        for which in (0, 1):
+            lib.ref_config2_step(which, util.ptr(u), util.ptr(ulo), util.ptr(uhi), util.ptr(f), util.ptr(r),
+                                 util.ptr(lo), util.ptr(hi), ctypes.c_double(1.0 / (h * h)), ctypes.byref(am),
+                                 ctypes.byref(sm))
+    else:
+        kind = "port"
+        o = util.oracle()
+        for name in ("Laplacian", "Laplacian_27"):
+            c, off, _, _, _ = util.lib_stencil(name)
+            o.orc_residual_f64(len(c), util.ptr(c), util.ptr(off.reshape(-1).copy()), util.ptr(u), util.ptr(ulo),
+                               util.ptr(uhi), util.ptr(f), util.ptr(r), util.ptr(lo), util.ptr(hi),
+                               ctypes.c_double(1.0 / (h * h)))
+            o.orc_reduce_f64(2, util.ptr(r), r.size)
+            o.orc_reduce_f64(3, util.ptr(r), r.size)
+    dt = time.perf_counter() - t0
+    nbytes = 8 * ((n + 2 * g) * (n + 2 * g) * (nz + 2 * g) + 3 * n * n * nz)
+    return {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
+            "sample": "config-2 step (7- and 27-point r = f - L(u), absMax, sum) on a %dx%dx%d z-slab, "
+                      "%.2f s single-threaded Release build" % (n, n, nz, dt)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path is a single-GPU kernel); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import proto_amd as pa
+    from proto_amd.device import DeviceArray, Event, Stream, gpu_available
+    from proto_amd.inputs import config2_inputs
+
+    assert gpu_available(), "bench.py needs a GPU"
+    n, g = args.n, 2
+    h = 1.0 / n
+    sc = 1.0 / (h * h)
+    stream = Stream()
+    sp = stream.h
+    s7, s27 = stencils(pa.PM_F64)
+    u, f = config2_inputs(n, g, np.float64, sp)
+    r1 = DeviceArray(n ** 3)
+    r2 = DeviceArray(n ** 3)
+    red = DeviceArray.zeros(4)
+    ub = pa.Box.of([-g] * 3, [n - 1 + g] * 3)
+    rb = pa.Box.of([0] * 3, [n - 1] * 3)
+    stream.sync()
+
+    def step():
+        pa.call("pm_stencil_residual2", s7.h, s27.h, u.ptr, ctypes.byref(ub), f.ptr, r1.ptr, r2.ptr,
+                ctypes.byref(rb), sc, sc, red.ptr, sp)
+
+    def kernel_only():
+        pa.call("pm_stencil_residual2", s7.h, s27.h, u.ptr, ctypes.byref(ub), f.ptr, r1.ptr, r2.ptr,
+                ctypes.byref(rb), sc, sc, None, sp)
+
+    bytes_u = 8 * (n + 2 * g) ** 3
+    bytes_f = 8 * n ** 3
+    step_bytes = bytes_u + bytes_f + 2 * bytes_f
+
+    if args.only:
+        fns = {"residual2": kernel_only, "step": step}
+        for _ in range(args.warmup + args.steps):
+            fns[args.only]()
+        stream.sync()
+        print(json.dumps({"only": args.only, "steps": args.steps}))
+        return
+
+    for _ in range(args.warmup):
+        step()
+    stream.sync()
+    t0 = time.perf_counter()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    e1.sync()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_ms(e1) / args.steps
+    value = step_bytes / (ms_step * 1e-3) / 1e9
+
+    k_ms = time_launches(kernel_only, max(5, args.steps), stream)
+    achieved = step_bytes / (k_ms * 1e-3) / 1e9
+    result = {
+        "metric": "stencil apply GB/s (% HBM roofline); MG solves/s + p99 latency vs plain FP64",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "laplacian512_fused_residual", "box": [n, n, n], "ghost": g,
+                   "operators": ["Laplacian (7-pt)", "Laplacian_27 (27-pt)"],
+                   "fused": "r = f - L(u), absMax(r), sum(r) for both operators, one pass",
+                   "step_bytes": step_bytes},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "k_zmarch<double,7,3,2> (7+27-point fused residual)", "kernel_ms": round(k_ms, 4)},
+        "wall_s": round(wall, 3),
+    }
+
+    if not args.no_extra:
+        extra = {}
+        dst = DeviceArray(n ** 3)
+        apply_bytes = bytes_u + bytes_f
+        for name, st in (("apply_L7_f64", s7), ("apply_L27_f64", s27)):
+            ms = time_launches(lambda: pa.call("pm_stencil_apply", st.h, u.ptr, ctypes.byref(ub), dst.ptr,
+                                               ctypes.byref(rb), ctypes.byref(rb), 1, 1, sc, sp), 10, stream)
+            gbs = apply_bytes / (ms * 1e-3) / 1e9
+            extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
+                           "bytes": apply_bytes}
+        for name, st in (("residual_L7_f64", s7), ("residual_L27_f64", s27)):
+            ms = time_launches(lambda: pa.call("pm_stencil_residual", st.h, u.ptr, ctypes.byref(ub), f.ptr,
+                                               dst.ptr, ctypes.byref(rb), sc, None, sp), 10, stream)
+            b = apply_bytes + bytes_f
+            gbs = b / (ms * 1e-3) / 1e9
+            extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
+                           "bytes": b}
+        dst.free()
+        # FP32 variants of the plain applies
+        u32, f32 = config2_inputs(n, g, np.float32, sp)
+        d32 = DeviceArray(n ** 3, np.float32)
+        t7, t27 = stencils(pa.PM_F32)
+        for name, st in (("apply_L7_f32", t7), ("apply_L27_f32", t27)):
+            ms = time_launches(lambda: pa.call("pm_stencil_apply", st.h, u32.ptr, ctypes.byref(ub), d32.ptr,
+                                               ctypes.byref(rb), ctypes.byref(rb), 1, 1, sc, sp), 10, stream)
+            b = apply_bytes // 2
+            gbs = b / (ms * 1e-3) / 1e9
+            extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
+                           "bytes": b}
+        u32.free()
+        f32.free()
+        d32.free()
+        # plain device copy of the same size (achievable-bandwidth probe)
+        a = DeviceArray(n ** 3)
+        box_all = pa.Box.of([0] * 3, [n - 1] * 3)
+        ms = time_launches(lambda: pa.call("pm_boxdata_copy", pa.PM_F64, r1.ptr, ctypes.byref(box_all), a.ptr,
+                                           ctypes.byref(box_all), ctypes.byref(box_all), pa.i3([0, 0, 0]), 1, sp),
+                           10, stream)
+        gbs = 2 * bytes_f / (ms * 1e-3) / 1e9
+        extra["copy_probe_f64"] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
+                                   "bytes": 2 * bytes_f}
+        a.free()
+        result["extra"] = extra
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(n, g, args.cpu_planes)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,302 @@
+/* protomi -- C ABI of the MI355X-native Proto hot path.
+ *
+ * Proto's device boundary is the template launch
+ *   protoLaunchKernelMemAsyncT<MEM, Ker>(blocks, threads, smem, stream, args...)
+ * (include/base/Proto_Macros.H:155-172) plus proto_malloc / proto_free /
+ * proto_memcpy<MEM> (include/base/implem/Proto_MemoryImplem.H:45-123).  Every
+ * Ker::gpu that the hot path launches is replaced by one entry point below,
+ * written for gfx950 and called by the C++ API in proto_amd/cpp (Proto's
+ * Stencil / BoxData / forall / Reduction / LevelBoxData / multigrid names).
+ *
+ * Conventions
+ *   - Every function returns 0 (PM_OK) or a negative PM_ERR_* code; the text of
+ *     the last failure on the calling thread is pm_last_error().  No C++
+ *     exception crosses this boundary.
+ *   - Boxes are inclusive [lo, hi] index triples with x fastest (Box::sindex,
+ *     Proto_BoxImplem.H:76-86); 2-D data uses lo[2] = hi[2] = 0.
+ *   - Arrays are BoxData storage: component c of a box B starts at
+ *     c * |B| elements (SoA, Proto_BoxDataImplem.H:697-717).  Pointers are
+ *     device pointers unless a name says host.
+ *   - dtype is PM_F32 or PM_F64; "T" below is that type.  Scalars (scale,
+ *     values) travel as double and are converted to T once.
+ *   - stream is a hipStream_t (NULL = the null stream).  Work is asynchronous
+ *     unless a function says it synchronises.  Handles are caller-owned; a
+ *     handle may not be used from two threads at once.
+ */
+#ifndef PROTOMI_H
+#define PROTOMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_OK 0
+#define PM_ERR_INVALID (-1)     /* bad argument: shape, box, null pointer, dtype */
+#define PM_ERR_HIP (-2)         /* HIP runtime error (message says which call)   */
+#define PM_ERR_NOMEM (-3)       /* device allocation failed / over budget        */
+#define PM_ERR_UNSUPPORTED (-4) /* valid request this build does not implement   */
+#define PM_ERR_NOT_CONVERGED (-5)
+#define PM_ERR_CANCELLED (-6)
+
+enum pm_dtype { PM_F32 = 0, PM_F64 = 1 };
+
+/* Reduction operators, in the order of Proto's enum Operation
+ * (include/base/Proto_Reduction.H:15). */
+enum pm_red_op { PM_RED_MAX = 0, PM_RED_MIN = 1, PM_RED_ABS = 2, PM_RED_SUM = 3,
+                 PM_RED_SUMABS = 4, PM_RED_SUMSQ = 5 };
+
+typedef struct pm_box { int lo[3]; int hi[3]; } pm_box;
+typedef void* pm_stream_t;
+
+/* ---------------------------------------------------------------- runtime */
+const char* pm_last_error(void);
+const char* pm_version(void);
+/* Number of visible devices; 0 when HIP finds none (no error). */
+int pm_device_count(int* count);
+
+/* proto_malloc<DEVICE> / proto_free<DEVICE> (Proto_MemoryImplem.H:77-123). */
+int pm_malloc(void** ptr, size_t bytes);
+int pm_free(void* ptr);
+/* proto_memcpy<SRC,DST> (Proto_MemoryImplem.H:45-75); kind: 0 H2D, 1 D2H, 2 D2D.
+ * Asynchronous on `stream`; pageable host memory makes it synchronous. */
+int pm_memcpy(void* dst, const void* src, size_t bytes, int kind, pm_stream_t stream);
+int pm_memset_zero(void* ptr, size_t bytes, pm_stream_t stream);
+int pm_stream_create(pm_stream_t* stream);
+int pm_stream_destroy(pm_stream_t stream);
+int pm_stream_sync(pm_stream_t stream);
+int pm_device_sync(void);
+/* HIP events for timing work on a stream (hipEventRecord / hipEventElapsedTime). */
+typedef void* pm_event_t;
+int pm_event_create(pm_event_t* ev);
+int pm_event_destroy(pm_event_t ev);
+int pm_event_record(pm_event_t ev, pm_stream_t stream);
+int pm_event_sync(pm_event_t ev);
+int pm_event_elapsed_ms(pm_event_t start, pm_event_t stop, float* ms);
+
+/* --------------------------------------------------------------- Stencil */
+typedef struct pm_stencil pm_stencil;
+
+/* Stencil<T> after construction (coefficients merged by addCoef,
+ * Proto_StencilImplem.H:237-276) plus closeForDevice (539-556): uploads the
+ * coefficient list (T, in Proto's order) and the offsets (ncoef x 3 ints).
+ * src_ratio / dst_ratio / dst_shift: Stencil::srcRatio/destRatio/destShift. */
+int pm_stencil_create(pm_stencil** out, int dtype, int dim, int ncoef, const void* coefs_host,
+                      const int* offsets_host, const int src_ratio[3], const int dst_ratio[3],
+                      const int dst_shift[3]);
+int pm_stencil_destroy(pm_stencil* st);
+/* Which kernel family serves this stencil: 0 generic, 7 the 7-point z-march,
+ * 27 the 27-point z-march, 5 the 2-D 5-point tile (for tests and tools). */
+int pm_stencil_kind(const pm_stencil* st, int* kind);
+
+/* Stencil::apply -> protoApply<DEVICE> -> sIndexer3::gpu
+ * (Proto_StencilImplem.H:412-431, 844-895, 565-614 and hostApply 725-828):
+ *   for p in idx_box:  dst[p*dR + dS] = (replace ? 0 : dst) + sum_j c'_j src[p*sR + o_j]
+ * accumulated in coefficient order with one rounding per multiply and per add,
+ * c'_j = c_j * scale when scale != 1 (bit-identical to hostApply).  scale == 0
+ * only zeroes (replace) or does nothing (accumulate).  Box inference
+ * (indexDomain(dst) & indexRange(src)) is the caller's (the C++ API's). */
+int pm_stencil_apply(const pm_stencil* st, const void* src, const pm_box* src_box, void* dst,
+                     const pm_box* dst_box, const pm_box* idx_box, int ncomp, int replace,
+                     double scale, pm_stream_t stream);
+
+/* Fused form of   Lu |= L(u, scale);  r = forall(f - Lu);  r.absMax(); r.sum()
+ * (config-2 program: Proto_StencilImplem.H:456-470, Proto_ForallImplem.H:195-213,
+ * Proto_BoxDataImplem.H:1375-1414).  r, f share r_box; u covers the stencil's
+ * domain of idx_box = r_box.  red_out (device, may be NULL) receives
+ * {absMax(r), sum(r)} as two T values; the sum is a fixed-shape tree, so it is
+ * deterministic but not in Proto's sequential order (bound: n*eps*sum|r|). */
+int pm_stencil_residual(const pm_stencil* st, const void* u, const pm_box* u_box, const void* f,
+                        void* r, const pm_box* r_box, double scale, void* red_out,
+                        pm_stream_t stream);
+
+/* The same for two operators in one pass over u and f (config 2 computes the
+ * 7-point and the 27-point residual of the same u): r1 = f - L1(u, scale1),
+ * r2 = f - L2(u, scale2); red_out = {absMax(r1), sum(r1), absMax(r2), sum(r2)}. */
+int pm_stencil_residual2(const pm_stencil* st1, const pm_stencil* st2, const void* u,
+                         const pm_box* u_box, const void* f, void* r1, void* r2,
+                         const pm_box* r_box, double scale1, double scale2, void* red_out,
+                         pm_stream_t stream);
+
+/* ------------------------------------------------------------- Reduction */
+/* Reduction<T,OP,DEVICE>::reduce (Proto_ReductionImplem.H:157-192) over n
+ * contiguous values.  The result (one T) goes to out_dev; with combine != 0
+ * it is folded into the value already there (Proto's m_deviTotal), else it
+ * replaces it.  Deterministic for a given n. */
+int pm_reduce(int op, int dtype, const void* data, size_t n, void* out_dev, int combine,
+              pm_stream_t stream);
+/* The same over the cells of `region` of component `comp` of an array on `box`
+ * (LevelBoxData::reduce's per-box valid region, Proto_LevelBoxDataImplem.H:595-617). */
+int pm_reduce_box(int op, int dtype, const void* data, const pm_box* box, const pm_box* region,
+                  int comp, void* out_dev, int combine, pm_stream_t stream);
+/* Reduction::init (Proto_ReductionImplem.H:64-76) written to out_dev. */
+int pm_reduce_init(int op, int dtype, void* out_dev, pm_stream_t stream);
+
+/* --------------------------------------------------- BoxData pointwise ops */
+/* BoxData::copyTo(dst, region, shift) (Proto_BoxDataImplem.H:490-640):
+ * dst[p + shift] = src[p] for p in region (source coordinates), all ncomp. */
+int pm_boxdata_copy(int dtype, const void* src, const pm_box* src_box, void* dst,
+                    const pm_box* dst_box, const pm_box* region, const int shift[3], int ncomp,
+                    pm_stream_t stream);
+/* dst op= src on region (BoxData operatorT, Proto_BoxDataImplem.H:937-983).
+ * op: 0 add, 1 subtract, 2 multiply, 3 divide. */
+int pm_boxdata_binop(int op, int dtype, void* dst, const pm_box* dst_box, const void* src,
+                     const pm_box* src_box, const pm_box* region, int ncomp, pm_stream_t stream);
+/* dst op= value on region: op 0 add, 1 subtract, 2 multiply, 3 divide, 4 set
+ * (scalarIndexer / setVal, Proto_BoxDataImplem.H:430-486, 1260-1278). */
+int pm_boxdata_scalar(int op, int dtype, void* dst, const pm_box* dst_box, const pm_box* region,
+                      double value, int ncomp, pm_stream_t stream);
+/* BoxData::increment: dst += src * scale (Proto_BoxDataImplem.H:1219-1234). */
+int pm_boxdata_increment(int dtype, void* dst, const pm_box* dst_box, const void* src,
+                         const pm_box* src_box, const pm_box* region, double scale, int ncomp,
+                         pm_stream_t stream);
+
+/* ------------------------------------------------------------------ level */
+/* A DisjointBoxLayout of equal boxes on a ProblemDomain [0, domain-1]
+ * (Proto_DisjointBoxLayout.H:52-110, Proto_ProblemDomainImplem.H:1-75): box
+ * size bs divides the domain; nboxes boxes given by their low corners in
+ * layout order (holes allowed).  Level data (LevelBoxData, one component set
+ * per box) is one device allocation: box b, component c starts at
+ * (b * ncomp + c) * |box grown by ghost| elements. */
+typedef struct pm_layout pm_layout;
+int pm_layout_create(pm_layout** out, int dim, const int domain[3], const int box_size[3],
+                     const int periodic[3], int nboxes, const int* box_lows_host);
+int pm_layout_destroy(pm_layout* layout);
+/* Elements of one box's ghosted storage (one component). */
+int pm_layout_box_elems(const pm_layout* layout, const int ghost[3], size_t* elems);
+
+/* LevelBoxData::exchange (Proto_LevelBoxDataImplem.H:313-341, motion plans
+ * of LevelExchangeCopier 898-951, periodic images ProblemDomainImplem.H:61-75):
+ * every ghost cell that is the (periodic image of a) valid cell of some box
+ * receives that cell's value.  One launch for the whole level. */
+int pm_level_exchange(const pm_layout* layout, int dtype, void* data, const int ghost[3], int ncomp,
+                      pm_stream_t stream);
+
+/* Red-black Gauss-Seidel half sweep on the 2*DIM+1-point Laplacian at the
+ * cells of one colour ((x+y+z) mod 2 == color, global indices), as
+ * examples/_archived/AMRFAS_old/src/LaplaceOp.H:56-73, 95-123 composes it:
+ *   t = 0 + sum_j c_j phi(p+o_j), c_j = Laplacian()_j * (-1/(dx*dx));
+ *   t = t + rhs(p);  phi(p) = phi(p) + lambda*t,  lambda = 2*(-(dx*dx)/(4*DIM)).
+ * phi's ghost cells must hold the neighbours (exchange first). */
+int pm_level_gsrb(const pm_layout* layout, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
+                  const int rhs_ghost[3], double dx, int color, pm_stream_t stream);
+
+/* Damped point Jacobi of examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:99-123:
+ * tmp = L(phi, 1/(4 DIM)) + (-(dx*dx)/(4 DIM)) rhs on all valid cells, then
+ * phi += tmp.  tmp: level data without ghosts. */
+int pm_level_jacobi(const pm_layout* layout, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
+                    const int rhs_ghost[3], void* tmp, double dx, pm_stream_t stream);
+
+/* resnorm of LevelMultigrid.cpp:75-97: max over valid cells of
+ * |(0 - rhs) + sum_j (c_j/dx^2) phi(p+o_j)| written to out_dev (one T);
+ * combine != 0 folds into the value there. */
+int pm_level_resnorm(const pm_layout* layout, int dtype, const void* phi, const int phi_ghost[3],
+                     const void* rhs, const int rhs_ghost[3], double dx, void* out_dev, int combine,
+                     pm_stream_t stream);
+
+/* coarseResidual of LevelMultigrid.cpp:51-73 fused with the copy into the
+ * coarse layout: res = (0 + rhs) + sum_j (c_j * (-1/dx^2)) phi(p+o_j) on the
+ * fine level, resc(I) = 0 + sum_{k in Box(2), x fastest} (1/8) res(2I+k)
+ * (AvgDown(2), Proto_StencilDefs.H:302-327), written straight into the boxes
+ * of `crse` (LevelBoxData::copyTo, 343-386).  crse covers fine.coarsen(2). */
+int pm_level_restrict_residual(const pm_layout* fine, const pm_layout* crse, int dtype, const void* phi,
+                               const int phi_ghost[3], const void* rhs, const int rhs_ghost[3], double dx,
+                               void* resc, const int resc_ghost[3], pm_stream_t stream);
+
+/* fineInterp of LevelMultigrid.cpp:125-140: phi += I(delta) with
+ * InterpStencil::Constant (linear = 0) or ::Linear (linear = 1) of ratio 2,
+ * staged x, y, z as Proto_InterpStencilImplem.H:83-112 does; delta lives on
+ * the coarse layout (periodic images read across boxes, so no exchange is
+ * needed). */
+int pm_level_prolong_add(const pm_layout* fine, const pm_layout* crse, int dtype, const void* delta,
+                         const int delta_ghost[3], void* phi, const int phi_ghost[3], int linear,
+                         pm_stream_t stream);
+
+/* setVal on every cell (ghosts included) of level data. */
+int pm_level_setval(const pm_layout* layout, int dtype, void* data, const int ghost[3], int ncomp,
+                    double value, pm_stream_t stream);
+
+/* LevelBoxData::copyTo between two layouts of the same domain
+ * (Proto_LevelBoxDataImplem.H:343-386): every valid cell of dst receives
+ * the valid cell of src at the same index. */
+int pm_level_copy(const pm_layout* src_layout, const void* src, const int src_ghost[3],
+                  const pm_layout* dst_layout, void* dst, const int dst_ghost[3], int dtype, int ncomp,
+                  pm_stream_t stream);
+
+/* Residual of the correction scheme, r = (0 + rhs) + sum_j (c_j * (-1/dx^2)) phi(p+o_j)
+ * (the order of coarseResidual, LevelMultigrid.cpp:64-66) on every valid cell,
+ * written as `out_dtype` into out (level data, out_ghost), with max|r| to
+ * norm_out (one value of phi's dtype; |r| equals resnorm's value bit for bit).
+ * One pass serves both the norm test and the next right-hand side of the
+ * FP32-preconditioned refinement.  phi must be exchanged. */
+int pm_level_residual(const pm_layout* layout, int dtype, const void* phi, const int phi_ghost[3],
+                      const void* rhs, const int rhs_ghost[3], double dx, int out_dtype, void* out,
+                      const int out_ghost[3], void* norm_out, pm_stream_t stream);
+/* phi += (T)e on valid cells (e of dtype e_dtype): the FP64 update of the
+ * refinement with an FP32 correction. */
+int pm_level_add_correction(const pm_layout* layout, int dtype, void* phi, const int phi_ghost[3], int e_dtype,
+                            const void* e, const int e_ghost[3], pm_stream_t stream);
+
+/* ------------------------------------------------------------- multigrid */
+/* Correction-scheme V-cycle solver on a periodic level
+ * (examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:13-160 with the
+ * GSRB relaxation of examples/_archived/AMRFAS_old/src/LaplaceOp.H:95-123, or
+ * its own damped Jacobi), solving Laplacian(phi)/dx^2 = rhs.  Coarse levels
+ * follow LevelMultigrid::define (21-49): domain halves; box size kept when it
+ * divides the coarse domain, else one box.  All level buffers are allocated
+ * at create and stay resident. */
+typedef struct pm_mg pm_mg;
+typedef struct pm_mg_options {
+    int relax;      /* 0 GSRB (pre/post/bottom sweeps), 1 damped Jacobi (2*DIM, 2*DIM, 10) */
+    int pre, post, bottom;
+    int interp;     /* 0 InterpStencil::Constant, 1 InterpStencil::Linear */
+    int precision;  /* 0 plain FP64 V-cycles; 1 FP32 V-cycle preconditioner inside FP64
+                       iterative refinement (residual, update and norm in FP64) */
+    int max_iter;
+    double tol;     /* stop when resnorm < tol * resnorm_0 (LevelMultigridSolverImplem.H:129) */
+    int use_graph;  /* capture the V-cycle into a hipGraph and replay it */
+} pm_mg_options;
+void pm_mg_default_options(pm_mg_options* opt);
+/* nlevels = number of coarsenings (the numLevels argument of LevelMultigrid). */
+int pm_mg_create(pm_mg** out, const pm_layout* fine, int nlevels, double dx, const pm_mg_options* opt);
+int pm_mg_destroy(pm_mg* mg);
+/* Device bytes the hierarchy holds (for admission by memory budget). */
+int pm_mg_bytes(const pm_mg* mg, size_t* bytes);
+/* Layout of level l (0 = finest) owned by the solver. */
+int pm_mg_level_layout(const pm_mg* mg, int level, const pm_layout** layout);
+/* Solve in place: phi (FP64 level data, ghost 1), rhs (FP64, ghost 0).
+ * hist_host (max_iter + 1 doubles, may be NULL) gets the max-norm residual
+ * before the first cycle and after each; *iters the cycles done.  Returns
+ * PM_ERR_NOT_CONVERGED if max_iter cycles did not reach tol (phi holds the
+ * last iterate).  Synchronises `stream` once per cycle (the norm test). */
+int pm_mg_solve(pm_mg* mg, void* phi, const void* rhs, double* hist_host, int* iters, pm_stream_t stream);
+/* One plain V-cycle on (phi, rhs) of the finest level, asynchronous. */
+int pm_mg_vcycle(pm_mg* mg, void* phi, const void* rhs, pm_stream_t stream);
+/* resnorm (LevelMultigrid.cpp:75-97) of (phi, rhs); synchronises. */
+int pm_mg_resnorm(pm_mg* mg, void* phi, const void* rhs, double* out_host, pm_stream_t stream);
+
+/* ----------------------------------------------------------------- inputs */
+/* Seeded inputs shared with the tests: value i of the SplitMix64 stream
+ * (seed, stream) mapped to [-1, 1); written to data[i] for i < n, as
+ * a*u + b when affine (used for config 2's 1e-3 noise). */
+int pm_fill_uniform(int dtype, void* data, size_t n, uint64_t seed, uint64_t stream_id,
+                    double a, double b, pm_stream_t stream);
+
+/* Initialise a field the way Proto examples do with forallInPlace_p
+ * (examples/_archived/LevelMultigrid/exec/main.cpp:18-25): for every cell p
+ * of `region` of an array on `box`, with w = p folded into [0, period)
+ * (period[d] > 0; use a large period for no folding),
+ *   data[p] = (tx[w.x] * ty[w.y]) * tz[w.z]  +  amp * U(seed, stream_id, w.x + P.x (w.y + P.y w.z))
+ * where tx/ty/tz are host tables of period[d] values (tz ignored in 2-D: use
+ * a table of one 1.0) and U is the pm_fill_uniform stream.  amp = 0 skips
+ * the noise.  Ghost cells so filled hold the periodic images. */
+int pm_fill_separable(int dtype, void* data, const pm_box* box, const pm_box* region, const int period[3],
+                      const double* tx_host, const double* ty_host, const double* tz_host, double amp,
+                      uint64_t seed, uint64_t stream_id, pm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PROTOMI_H */

@@ -178,6 +178,13 @@ void level_to_global(const LevelBoxData<double>& L, double* g, int n)
     }
 }
 
+PROTO_KERNEL_START
+void axpyResidualF(Var<double>& a_r, const Var<double>& a_f, const Var<double>& a_lu)
+{
+    a_r(0) = a_f(0) - a_lu(0);
+}
+PROTO_KERNEL_END(axpyResidualF, axpyResidual)
+
 DisjointBoxLayout periodic_layout(int n, int boxSize)
 {
     Box domain = Box::Cube(n);
@@ -476,6 +483,31 @@ int ref_mg_gsrb_solve(int n, int boxSize, int numLevels, int pre, int post, int 
     }
     level_to_global(P, phi, n);
     return it;
+}
+
+// The config-2 program as a Proto user writes it (one operator):
+//   BoxData<double> Lu(box); Lu |= L(u, scale);          (Stencil apply)
+//   forallInPlace(axpyResidual, r, f, Lu);               (r = f - Lu)
+//   amax = r.absMax(); sum = r.sum();                    (Reduction)
+// which = 0: Laplacian(), 1: Laplacian_27().  Used as the CPU baseline.
+void ref_config2_step(int which, const double* u, const int* ulo, const int* uhi, const double* f,
+                      double* r, const int* lo, const int* hi, double scale, double* amax, double* sum)
+{
+    Stencil<double> L;
+#if DIM == 3
+    L = which ? Stencil<double>::Laplacian_27() : Stencil<double>::Laplacian();
+#else
+    L = Stencil<double>::Laplacian();
+#endif
+    Box b = mkbox(lo, hi);
+    BoxData<double, 1, HOST> U(u, mkbox(ulo, uhi));
+    BoxData<double, 1, HOST> F(f, b);
+    BoxData<double, 1, HOST> R(r, b);
+    BoxData<double, 1, HOST> Lu(b);
+    Lu |= L(U, scale);
+    forallInPlace(axpyResidual, R, F, Lu);
+    *amax = R.absMax();
+    *sum = R.sum();
 }
 
 // One GSRB relaxation pass (niter iterations) on a single level, no coarse grids.

@@ -1,0 +1,182 @@
+"""proto_amd: the MI355X-native Proto hot path.
+
+The product is the native library proto_amd/libprotomi.so (C ABI declared in
+include/protomi.h, kernels in proto_amd/csrc) and the C++ API over it in
+proto_amd/cpp.  This module only loads that library with ctypes for the
+Python tests and the benchmark; it has no compute path of its own and raises
+if the library is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libprotomi.so")
+
+PM_F32, PM_F64 = 0, 1
+RED_MAX, RED_MIN, RED_ABS, RED_SUM, RED_SUMABS, RED_SUMSQ = range(6)
+
+# Every symbol declared in include/protomi.h with its ctypes signature.
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_i = ctypes.c_int
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+_u64 = ctypes.c_uint64
+_dpp = ctypes.POINTER(ctypes.c_double)
+
+
+class Box(ctypes.Structure):
+    """pm_box: inclusive [lo, hi] with x fastest (2-D: z = 0)."""
+    _fields_ = [("lo", ctypes.c_int * 3), ("hi", ctypes.c_int * 3)]
+
+    @classmethod
+    def of(cls, lo, hi):
+        lo = list(lo) + [0] * (3 - len(lo))
+        hi = list(hi) + [0] * (3 - len(hi))
+        b = cls()
+        for d in range(3):
+            b.lo[d] = int(lo[d])
+            b.hi[d] = int(hi[d])
+        return b
+
+    def size(self):
+        n = 1
+        for d in range(3):
+            n *= max(0, self.hi[d] - self.lo[d] + 1)
+        return n
+
+
+_BP = ctypes.POINTER(Box)
+
+
+class MgOptions(ctypes.Structure):
+    """pm_mg_options."""
+    _fields_ = [("relax", ctypes.c_int), ("pre", ctypes.c_int), ("post", ctypes.c_int), ("bottom", ctypes.c_int),
+                ("interp", ctypes.c_int), ("precision", ctypes.c_int), ("max_iter", ctypes.c_int),
+                ("tol", ctypes.c_double), ("use_graph", ctypes.c_int)]
+
+
+_OP = ctypes.POINTER(MgOptions)
+
+SIGNATURES = {
+    "pm_last_error": (ctypes.c_char_p, []),
+    "pm_version": (ctypes.c_char_p, []),
+    "pm_device_count": (_i, [_ip]),
+    "pm_malloc": (_i, [ctypes.POINTER(_vp), _sz]),
+    "pm_free": (_i, [_vp]),
+    "pm_memcpy": (_i, [_vp, _vp, _sz, _i, _vp]),
+    "pm_memset_zero": (_i, [_vp, _sz, _vp]),
+    "pm_stream_create": (_i, [ctypes.POINTER(_vp)]),
+    "pm_stream_destroy": (_i, [_vp]),
+    "pm_stream_sync": (_i, [_vp]),
+    "pm_device_sync": (_i, []),
+    "pm_stencil_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _ip, _ip, _ip, _ip]),
+    "pm_stencil_destroy": (_i, [_vp]),
+    "pm_stencil_kind": (_i, [_vp, _ip]),
+    "pm_stencil_apply": (_i, [_vp, _vp, _BP, _vp, _BP, _BP, _i, _i, _d, _vp]),
+    "pm_stencil_residual": (_i, [_vp, _vp, _BP, _vp, _vp, _BP, _d, _vp, _vp]),
+    "pm_stencil_residual2": (_i, [_vp, _vp, _vp, _BP, _vp, _vp, _vp, _BP, _d, _d, _vp, _vp]),
+    "pm_reduce": (_i, [_i, _i, _vp, _sz, _vp, _i, _vp]),
+    "pm_reduce_box": (_i, [_i, _i, _vp, _BP, _BP, _i, _vp, _i, _vp]),
+    "pm_reduce_init": (_i, [_i, _i, _vp, _vp]),
+    "pm_boxdata_copy": (_i, [_i, _vp, _BP, _vp, _BP, _BP, _ip, _i, _vp]),
+    "pm_boxdata_binop": (_i, [_i, _i, _vp, _BP, _vp, _BP, _BP, _i, _vp]),
+    "pm_boxdata_scalar": (_i, [_i, _i, _vp, _BP, _BP, _d, _i, _vp]),
+    "pm_boxdata_increment": (_i, [_i, _vp, _BP, _vp, _BP, _BP, _d, _i, _vp]),
+    "pm_fill_uniform": (_i, [_i, _vp, _sz, _u64, _u64, _d, _d, _vp]),
+    "pm_fill_separable": (_i, [_i, _vp, _BP, _BP, _ip, _dpp, _dpp, _dpp, _d, _u64, _u64, _vp]),
+    "pm_event_create": (_i, [ctypes.POINTER(_vp)]),
+    "pm_event_destroy": (_i, [_vp]),
+    "pm_event_record": (_i, [_vp, _vp]),
+    "pm_event_sync": (_i, [_vp]),
+    "pm_event_elapsed_ms": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "pm_layout_create": (_i, [ctypes.POINTER(_vp), _i, _ip, _ip, _ip, _i, _ip]),
+    "pm_layout_destroy": (_i, [_vp]),
+    "pm_layout_box_elems": (_i, [_vp, _ip, ctypes.POINTER(_sz)]),
+    "pm_level_exchange": (_i, [_vp, _i, _vp, _ip, _i, _vp]),
+    "pm_level_gsrb": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _d, _i, _vp]),
+    "pm_level_jacobi": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _vp, _d, _vp]),
+    "pm_level_resnorm": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _d, _vp, _i, _vp]),
+    "pm_level_restrict_residual": (_i, [_vp, _vp, _i, _vp, _ip, _vp, _ip, _d, _vp, _ip, _vp]),
+    "pm_level_prolong_add": (_i, [_vp, _vp, _i, _vp, _ip, _vp, _ip, _i, _vp]),
+    "pm_level_setval": (_i, [_vp, _i, _vp, _ip, _i, _d, _vp]),
+    "pm_level_copy": (_i, [_vp, _vp, _ip, _vp, _vp, _ip, _i, _i, _vp]),
+    "pm_level_residual": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _d, _i, _vp, _ip, _vp, _vp]),
+    "pm_level_add_correction": (_i, [_vp, _i, _vp, _ip, _i, _vp, _ip, _vp]),
+    "pm_mg_default_options": (None, [_OP]),
+    "pm_mg_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _d, _OP]),
+    "pm_mg_destroy": (_i, [_vp]),
+    "pm_mg_bytes": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "pm_mg_level_layout": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
+    "pm_mg_solve": (_i, [_vp, _vp, _vp, _dpp, _ip, _vp]),
+    "pm_mg_vcycle": (_i, [_vp, _vp, _vp, _vp]),
+    "pm_mg_resnorm": (_i, [_vp, _vp, _vp, _dpp, _vp]),
+}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError("proto_amd native library missing: %s (run __graft_entry__.build())" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().pm_last_error().decode(errors="replace")
+        raise NativeError("%s failed (rc=%d): %s" % (what, rc, msg))
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args), name)
+
+
+def i3(v, fill=0):
+    v = list(v) + [fill] * (3 - len(v))
+    return (ctypes.c_int * 3)(*[int(x) for x in v])
+
+
+class Stencil:
+    """A device Stencil handle (pm_stencil)."""
+
+    def __init__(self, coefs, offsets, dtype=PM_F64, dim=3, src_ratio=(1, 1, 1), dst_ratio=(1, 1, 1),
+                 dst_shift=(0, 0, 0)):
+        import numpy as np
+        npdt = np.float64 if dtype == PM_F64 else np.float32
+        self.coefs = np.ascontiguousarray(np.asarray(coefs, dtype=npdt))
+        offs = np.zeros((len(self.coefs), 3), dtype=np.int32)
+        o = np.asarray(offsets, dtype=np.int32).reshape(len(self.coefs), -1)
+        offs[:, :o.shape[1]] = o
+        self.offsets = np.ascontiguousarray(offs)
+        self.dtype = dtype
+        h = _vp()
+        call("pm_stencil_create", ctypes.byref(h), dtype, dim, len(self.coefs),
+             self.coefs.ctypes.data_as(_vp), self.offsets.ctypes.data_as(_ip), i3(src_ratio, 1),
+             i3(dst_ratio, 1), i3(dst_shift, 0))
+        self.h = h
+
+    def kind(self):
+        k = ctypes.c_int()
+        call("pm_stencil_kind", self.h, ctypes.byref(k))
+        return k.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pm_stencil_destroy(self.h)
+            self.h = None

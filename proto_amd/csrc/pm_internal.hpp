@@ -1,0 +1,172 @@
+// Internal helpers shared by the protomi HIP sources (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "protomi.h"
+
+namespace pm {
+
+// Records the message of the last failure on this host thread.
+void set_error(const char* fmt, ...);
+
+// Grow-only device scratch private to one stream (per-block partials of the
+// reductions).  Kernels on different streams never share it; growing it
+// synchronises that stream first.  Not for use inside hipGraph capture unless
+// already large enough.
+int workspace(hipStream_t stream, size_t bytes, void** out);
+
+// Shape of a layout (pm_level.hip): dim, domain, box size, periodic flags, box count.
+int layout_shape(const pm_layout* L, int* dim, int* domain, int* bs, int* periodic, int* nboxes);
+// Box low corners in layout order (3 ints per box).
+int layout_lows(const pm_layout* L, int* lows);
+
+#define PM_CHECK_HIP(expr)                                                        \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            ::pm::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                  \
+            return PM_ERR_HIP;                                                    \
+        }                                                                         \
+    } while (0)
+
+#define PM_REQUIRE(cond, ...)                   \
+    do {                                        \
+        if (!(cond)) {                          \
+            ::pm::set_error(__VA_ARGS__);       \
+            return PM_ERR_INVALID;              \
+        }                                       \
+    } while (0)
+
+// Launch-error check after a kernel launch.
+#define PM_CHECK_LAUNCH() PM_CHECK_HIP(hipGetLastError())
+
+inline hipStream_t S(pm_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Inclusive box with x fastest, as Proto's Box (Proto_BoxImplem.H:76-86).
+struct Box3 {
+    int lo[3];
+    int hi[3];
+};
+
+inline Box3 box3(const pm_box* b)
+{
+    Box3 r;
+    for (int d = 0; d < 3; d++) { r.lo[d] = b->lo[d]; r.hi[d] = b->hi[d]; }
+    return r;
+}
+
+__host__ __device__ inline long box_nx(const Box3& b) { return (long)b.hi[0] - b.lo[0] + 1; }
+__host__ __device__ inline long box_ny(const Box3& b) { return (long)b.hi[1] - b.lo[1] + 1; }
+__host__ __device__ inline long box_nz(const Box3& b) { return (long)b.hi[2] - b.lo[2] + 1; }
+__host__ __device__ inline long box_size(const Box3& b)
+{
+    if (b.hi[0] < b.lo[0] || b.hi[1] < b.lo[1] || b.hi[2] < b.lo[2]) { return 0; }
+    return box_nx(b) * box_ny(b) * box_nz(b);
+}
+__host__ __device__ inline bool box_empty(const Box3& b) { return box_size(b) == 0; }
+__host__ __device__ inline bool box_contains(const Box3& b, int x, int y, int z)
+{
+    return x >= b.lo[0] && x <= b.hi[0] && y >= b.lo[1] && y <= b.hi[1] && z >= b.lo[2] && z <= b.hi[2];
+}
+__host__ __device__ inline bool box_contains_box(const Box3& outer, const Box3& inner)
+{
+    return box_empty(inner) ||
+           (box_contains(outer, inner.lo[0], inner.lo[1], inner.lo[2]) &&
+            box_contains(outer, inner.hi[0], inner.hi[1], inner.hi[2]));
+}
+// Linear index of (x,y,z) in box b (Box::sindex).
+__host__ __device__ inline long box_index(const Box3& b, int x, int y, int z)
+{
+    return (long)(x - b.lo[0]) + box_nx(b) * ((long)(y - b.lo[1]) + box_ny(b) * (long)(z - b.lo[2]));
+}
+
+inline bool box_valid(const pm_box* b)
+{
+    return b != nullptr;
+}
+
+// XCD-aware block remap: MI355X dispatches workgroup i to XCD (i mod 8); this
+// hands each XCD a contiguous run of logical tiles so that tiles sharing halo
+// rows share that XCD's L2.
+__device__ inline unsigned xcd_remap(unsigned b, unsigned n)
+{
+    const unsigned full = (n / 8u) * 8u;
+    if (b >= full) { return b; }
+    return (b % 8u) * (n / 8u) + b / 8u;
+}
+
+template<typename T> struct dtype_of;
+template<> struct dtype_of<float> { static constexpr int value = PM_F32; };
+template<> struct dtype_of<double> { static constexpr int value = PM_F64; };
+
+inline size_t dtype_size(int dt) { return dt == PM_F32 ? 4 : 8; }
+
+// ---------------------------------------------------------------- reductions
+// Reduction operators of Proto_Reduction.H:15 with the init/update rules of
+// Proto_ReductionImplem.H:64-102.
+template<typename T> __host__ __device__ inline T red_init(int op)
+{
+    switch (op) {
+        case PM_RED_MAX: return sizeof(T) == 8 ? (T)2.2250738585072014e-308 : (T)1.17549435e-38f;
+        case PM_RED_MIN: return sizeof(T) == 8 ? (T)1.7976931348623157e308 : (T)3.40282347e38f;
+        default: return (T)0;
+    }
+}
+
+template<typename T> __device__ inline T red_update(int op, T v, T x)
+{
+    switch (op) {
+        case PM_RED_MAX: return (v < x) ? x : v;
+        case PM_RED_MIN: return (x < v) ? x : v;
+        case PM_RED_ABS: { T a = (x > 0) ? x : -x; return (v < a) ? a : v; }
+        case PM_RED_SUM: return v + x;
+        case PM_RED_SUMABS: return v + ((x > 0) ? x : -x);
+        case PM_RED_SUMSQ: return v + x * x;
+        default: return v;
+    }
+}
+
+// Combining two partial values of the same operator.
+template<typename T> __device__ inline T red_combine(int op, T a, T b)
+{
+    switch (op) {
+        case PM_RED_MAX: return (a < b) ? b : a;
+        case PM_RED_MIN: return (b < a) ? b : a;
+        case PM_RED_ABS: return (a < b) ? b : a;
+        default: return a + b;
+    }
+}
+
+// Wave64 reduction with a fixed butterfly: the result is the same on every
+// run for the same inputs (the combination tree depends only on lane ids).
+template<typename T> __device__ inline T wave_reduce(int op, T v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        T o = __shfl_xor(v, off, 64);
+        v = red_combine(op, v, o);
+    }
+    return v;
+}
+
+// Block reduction for up to 16 waves; every thread returns the block value.
+template<typename T> __device__ inline T block_reduce(int op, T v, T* lds /* >= 16 */)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    v = wave_reduce(op, v);
+    __syncthreads();
+    if (lane == 0) { lds[w] = v; }
+    __syncthreads();
+    T r = lds[0];
+    for (int i = 1; i < nw; i++) { r = red_combine(op, r, lds[i]); }
+    return r;
+}
+
+}  // namespace pm

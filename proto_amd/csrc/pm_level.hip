@@ -1,0 +1,889 @@
+// Level (LevelBoxData over a DisjointBoxLayout) kernels: periodic exchange,
+// GSRB / Jacobi relaxation, residual norm, fused residual + restriction into
+// the coarse layout, prolongation, copy between layouts.
+//
+// Every box of a layout has the same size and lies on the box grid of the
+// domain, so the owner of any cell is one table lookup; each launch covers all
+// boxes of the level (blockIdx.y = box), instead of one launch per box and
+// per motion item as Proto's Copier/LevelCopierOp does.
+#include <algorithm>
+#include <vector>
+
+#include "pm_internal.hpp"
+
+struct pm_layout {
+    int dim;
+    int domain[3], bs[3], periodic[3], nb[3];
+    int nboxes;
+    std::vector<int> lows;
+    int* d_lows;
+    int* d_table;
+};
+
+namespace pm {
+
+struct LayoutDev {
+    int dim, nboxes;
+    int domain[3], bs[3], periodic[3], nb[3];
+    const int* lows;
+    const int* table;
+};
+
+static LayoutDev ldev(const pm_layout* L)
+{
+    LayoutDev d;
+    d.dim = L->dim;
+    d.nboxes = L->nboxes;
+    for (int k = 0; k < 3; k++) {
+        d.domain[k] = L->domain[k];
+        d.bs[k] = L->bs[k];
+        d.periodic[k] = L->periodic[k];
+        d.nb[k] = L->nb[k];
+    }
+    d.lows = L->d_lows;
+    d.table = L->d_table;
+    return d;
+}
+
+struct G3 {
+    int g[3];
+};
+
+static G3 g3(const int* g)
+{
+    G3 r;
+    for (int k = 0; k < 3; k++) { r.g[k] = g ? g[k] : 0; }
+    return r;
+}
+
+// elements of one ghosted box
+__host__ __device__ inline long gvol(const LayoutDev& L, const G3& g)
+{
+    return (long)(L.bs[0] + 2 * g.g[0]) * (L.bs[1] + 2 * g.g[1]) * (L.bs[2] + 2 * g.g[2]);
+}
+
+// index of the cell at box-relative (x, y, z) (0 = box low) inside the ghosted storage
+__device__ inline long rel_index(const LayoutDev& L, const G3& g, int x, int y, int z)
+{
+    const long X = L.bs[0] + 2 * g.g[0], Y = L.bs[1] + 2 * g.g[1];
+    return (long)(x + g.g[0]) + X * ((long)(y + g.g[1]) + Y * (long)(z + g.g[2]));
+}
+
+// Box owning global cell p (periodic images folded in); -1 if none.  On
+// return p holds the image inside the domain.
+__device__ inline int owner(const LayoutDev& L, int p[3])
+{
+    int b[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+        if (p[k] < 0 || p[k] >= L.domain[k]) {
+            if (!L.periodic[k]) { return -1; }
+            p[k] = ((p[k] % L.domain[k]) + L.domain[k]) % L.domain[k];
+        }
+        b[k] = p[k] / L.bs[k];
+    }
+    return L.table[b[0] + L.nb[0] * (b[1] + L.nb[1] * b[2])];
+}
+
+// global cell value of level data (valid cells of the owner; periodic images)
+template<typename T>
+__device__ inline T level_value(const LayoutDev& L, const G3& g, const T* data, int x, int y, int z)
+{
+    int p[3] = {x, y, z};
+    const int o = owner(L, p);
+    const int* lo = L.lows + 3 * o;
+    return data[(long)o * gvol(L, g) + rel_index(L, g, p[0] - lo[0], p[1] - lo[1], p[2] - lo[2])];
+}
+
+// ------------------------------------------------------------ exchange
+template<typename T>
+__global__ __launch_bounds__(256) void k_exchange(LayoutDev L, G3 g, T* data, int ncomp)
+{
+    const int b = blockIdx.y;
+    const int X = L.bs[0] + 2 * g.g[0], Y = L.bs[1] + 2 * g.g[1];
+    const long A = (long)X * Y * g.g[2];
+    const long B = (long)X * g.g[1] * L.bs[2];
+    const long C = (long)g.g[0] * L.bs[1] * L.bs[2];
+    const long n = 2 * (A + B + C);
+    const int* lo = L.lows + 3 * b;
+    const long vol = gvol(L, g);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        int x, y, z;
+        long u = t;
+        if (u < 2 * A) {
+            const bool hi = u >= A;
+            if (hi) { u -= A; }
+            x = (int)(u % X) - g.g[0];
+            y = (int)((u / X) % Y) - g.g[1];
+            const int zz = (int)(u / ((long)X * Y));
+            z = hi ? L.bs[2] + zz : zz - g.g[2];
+        } else if ((u -= 2 * A) < 2 * B) {
+            const bool hi = u >= B;
+            if (hi) { u -= B; }
+            x = (int)(u % X) - g.g[0];
+            const int yy = (int)((u / X) % g.g[1]);
+            z = (int)(u / ((long)X * g.g[1]));
+            y = hi ? L.bs[1] + yy : yy - g.g[1];
+        } else {
+            u -= 2 * B;
+            const bool hi = u >= C;
+            if (hi) { u -= C; }
+            const int xx = (int)(u % g.g[0]);
+            y = (int)((u / g.g[0]) % L.bs[1]);
+            z = (int)(u / ((long)g.g[0] * L.bs[1]));
+            x = hi ? L.bs[0] + xx : xx - g.g[0];
+        }
+        int p[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
+        const int o = owner(L, p);
+        if (o < 0) { continue; }
+        const int* olo = L.lows + 3 * o;
+        const long si = rel_index(L, g, p[0] - olo[0], p[1] - olo[1], p[2] - olo[2]);
+        const long di = rel_index(L, g, x, y, z);
+        for (int c = 0; c < ncomp; c++) {
+            data[((long)b * ncomp + c) * vol + di] = data[((long)o * ncomp + c) * vol + si];
+        }
+    }
+}
+
+// ------------------------------------------------------------ GSRB
+// Coefficients of Laplacian() (Proto_StencilDefs.H:77-87) in order.
+__host__ __device__ constexpr int lapo(int j, int d)
+{
+    constexpr int t[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    return t[j][d];
+}
+
+template<typename T>
+struct Lap {
+    T c[7];
+};
+
+template<typename T, int DIMN>
+__global__ __launch_bounds__(256) void k_gsrb(LayoutDev L, G3 gp, G3 gr, T* __restrict__ phi,
+                                              const T* __restrict__ rhs, Lap<T> c, T lambda, int color)
+{
+    constexpr int NC = 2 * DIMN + 1;
+    const int b = blockIdx.y;
+    const int* lo = L.lows + 3 * b;
+    const int hx = (L.bs[0] + 1) / 2;
+    const long n = (long)hx * L.bs[1] * L.bs[2];
+    T* ph = phi + (long)b * gvol(L, gp);
+    const T* rh = rhs + (long)b * gvol(L, gr);
+    const long X = L.bs[0] + 2 * gp.g[0], XY = X * (L.bs[1] + 2 * gp.g[1]);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int xh = (int)(t % hx);
+        const long r = t / hx;
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const int par = (color + lo[0] + lo[1] + y + lo[2] + z) & 1;
+        const int x = 2 * xh + par;
+        if (x >= L.bs[0]) { continue; }
+        const long i = rel_index(L, gp, x, y, z);
+        T s = 0;
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            s = s + c.c[j] * ph[i + lapo(j, 0) + lapo(j, 1) * X + lapo(j, 2) * XY];
+        }
+        s = s + rh[rel_index(L, gr, x, y, z)];
+        ph[i] = ph[i] + lambda * s;
+    }
+}
+
+// ------------------------------------------------------------ Jacobi
+template<typename T, int DIMN>
+__global__ __launch_bounds__(256) void k_jacobi_tmp(LayoutDev L, G3 gp, G3 gr, const T* __restrict__ phi,
+                                                    const T* __restrict__ rhs, T* __restrict__ tmp, Lap<T> c,
+                                                    T dcoef)
+{
+    constexpr int NC = 2 * DIMN + 1;
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    const T* ph = phi + (long)b * gvol(L, gp);
+    const T* rh = rhs + (long)b * gvol(L, gr);
+    T* tm = tmp + (long)b * n;
+    const long X = L.bs[0] + 2 * gp.g[0], XY = X * (L.bs[1] + 2 * gp.g[1]);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const long i = rel_index(L, gp, x, y, z);
+        T s = 0;
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            s = s + c.c[j] * ph[i + lapo(j, 0) + lapo(j, 1) * X + lapo(j, 2) * XY];
+        }
+        s = s + dcoef * rh[rel_index(L, gr, x, y, z)];
+        tm[t] = s;
+    }
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_jacobi_add(LayoutDev L, G3 gp, T* __restrict__ phi, const T* __restrict__ tmp)
+{
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    T* ph = phi + (long)b * gvol(L, gp);
+    const T* tm = tmp + (long)b * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const long i = rel_index(L, gp, x, y, z);
+        ph[i] = ph[i] + tm[t];
+    }
+}
+
+// ------------------------------------------------------------ residual norm
+template<typename T, int DIMN>
+__global__ __launch_bounds__(256) void k_resnorm(LayoutDev L, G3 gp, G3 gr, const T* __restrict__ phi,
+                                                 const T* __restrict__ rhs, Lap<T> c, T* __restrict__ partials)
+{
+    constexpr int NC = 2 * DIMN + 1;
+    __shared__ T red[16];
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    const T* ph = phi + (long)b * gvol(L, gp);
+    const T* rh = rhs + (long)b * gvol(L, gr);
+    const long X = L.bs[0] + 2 * gp.g[0], XY = X * (L.bs[1] + 2 * gp.g[1]);
+    T m = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const long i = rel_index(L, gp, x, y, z);
+        T s = 0;
+        s = s - rh[rel_index(L, gr, x, y, z)];
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            s = s + c.c[j] * ph[i + lapo(j, 0) + lapo(j, 1) * X + lapo(j, 2) * XY];
+        }
+        const T a = s > 0 ? s : -s;
+        m = m < a ? a : m;
+    }
+    m = block_reduce(PM_RED_ABS, m, red);
+    if (threadIdx.x == 0) { partials[(long)b * gridDim.x + blockIdx.x] = m; }
+}
+
+// ------------------------------------------------------------ restriction
+template<typename T, int DIMN>
+__global__ __launch_bounds__(256) void k_restrict_residual(LayoutDev F, LayoutDev C, G3 gp, G3 gr, G3 gc,
+                                                           const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                           T* __restrict__ resc, Lap<T> c, T avg)
+{
+    constexpr int NC = 2 * DIMN + 1;
+    const int b = blockIdx.y;
+    const int* clo = C.lows + 3 * b;
+    const long n = (long)C.bs[0] * C.bs[1] * C.bs[2];
+    const long FX = F.bs[0] + 2 * gp.g[0], FXY = FX * (F.bs[1] + 2 * gp.g[1]);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % C.bs[0]);
+        const long r = t / C.bs[0];
+        const int y = (int)(r % C.bs[1]);
+        const int z = (int)(r / C.bs[1]);
+        int p[3] = {2 * (clo[0] + x), 2 * (clo[1] + y), DIMN == 3 ? 2 * (clo[2] + z) : 0};
+        const int fb = owner(F, p);
+        const int* flo = F.lows + 3 * fb;
+        const T* ph = phi + (long)fb * gvol(F, gp);
+        const T* rh = rhs + (long)fb * gvol(F, gr);
+        const int fx = p[0] - flo[0], fy = p[1] - flo[1], fz = p[2] - flo[2];
+        T acc = 0;
+#pragma unroll
+        for (int kz = 0; kz < (DIMN == 3 ? 2 : 1); kz++)
+#pragma unroll
+            for (int ky = 0; ky < 2; ky++)
+#pragma unroll
+                for (int kx = 0; kx < 2; kx++) {
+                    const long i = rel_index(F, gp, fx + kx, fy + ky, fz + kz);
+                    T s = 0;
+                    s = s + rh[rel_index(F, gr, fx + kx, fy + ky, fz + kz)];
+#pragma unroll
+                    for (int j = 0; j < NC; j++) {
+                        s = s + c.c[j] * ph[i + lapo(j, 0) + lapo(j, 1) * FX + lapo(j, 2) * FXY];
+                    }
+                    acc = acc + avg * s;
+                }
+        resc[(long)b * gvol(C, gc) + rel_index(C, gc, x, y, z)] = acc;
+    }
+}
+
+// ------------------------------------------------------------ prolongation
+template<typename T, int DIMN, int LINEAR>
+__global__ __launch_bounds__(256) void k_prolong_add(LayoutDev F, LayoutDev C, G3 gd, G3 gp,
+                                                     const T* __restrict__ delta, T* __restrict__ phi)
+{
+    const int b = blockIdx.y;
+    const int* flo = F.lows + 3 * b;
+    const long n = (long)F.bs[0] * F.bs[1] * F.bs[2];
+    T* ph = phi + (long)b * gvol(F, gp);
+    const T one = 1, half = 0.5, zero = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % F.bs[0]);
+        const long r = t / F.bs[0];
+        const int y = (int)(r % F.bs[1]);
+        const int z = (int)(r / F.bs[1]);
+        const int fx = flo[0] + x, fy = flo[1] + y, fz = flo[2] + z;
+        const int I = fx >> 1, J = fy >> 1, K = (DIMN == 3) ? (fz >> 1) : 0;
+        T v;
+        if (!LINEAR) {
+            T s0 = 0;
+            s0 = s0 + one * level_value(C, gd, delta, I, J, K);
+            T s1 = 0;
+            s1 = s1 + one * s0;
+            v = s1;
+            if (DIMN == 3) {
+                T s2 = 0;
+                s2 = s2 + one * s1;
+                v = s2;
+            }
+        } else {
+            // vertex-style weights of InterpStencil::Linear (ratio 2): shift 0 -> (1, 0), shift 1 -> (1/2, 1/2)
+            const T qx = (fx & 1) ? half : one, px = (fx & 1) ? half : zero;
+            const T qy = (fy & 1) ? half : one, py = (fy & 1) ? half : zero;
+            const T qz = (fz & 1) ? half : one, pz = (fz & 1) ? half : zero;
+            T s1v[2];
+#pragma unroll
+            for (int kk = 0; kk < (DIMN == 3 ? 2 : 1); kk++) {
+                T s0v[2];
+#pragma unroll
+                for (int jj = 0; jj < 2; jj++) {
+                    T a = 0;
+                    a = a + qx * level_value(C, gd, delta, I, J + jj, K + kk);
+                    a = a + px * level_value(C, gd, delta, I + 1, J + jj, K + kk);
+                    s0v[jj] = a;
+                }
+                T a = 0;
+                a = a + qy * s0v[0];
+                a = a + py * s0v[1];
+                s1v[kk] = a;
+            }
+            if (DIMN == 3) {
+                T a = 0;
+                a = a + qz * s1v[0];
+                a = a + pz * s1v[1];
+                v = a;
+            } else {
+                v = s1v[0];
+            }
+        }
+        const long i = rel_index(F, gp, x, y, z);
+        ph[i] = ph[i] + one * v;
+    }
+}
+
+// ------------------------------------------------------------ copy / setval
+template<typename T>
+__global__ __launch_bounds__(256) void k_level_copy(LayoutDev S, G3 gs, const T* __restrict__ src, LayoutDev D, G3 gd,
+                                                    T* __restrict__ dst, int ncomp)
+{
+    const int b = blockIdx.y;
+    const int* lo = D.lows + 3 * b;
+    const long n = (long)D.bs[0] * D.bs[1] * D.bs[2];
+    const long dv = gvol(D, gd), sv = gvol(S, gs);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % D.bs[0]);
+        const long r = t / D.bs[0];
+        const int y = (int)(r % D.bs[1]);
+        const int z = (int)(r / D.bs[1]);
+        int p[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
+        const int o = owner(S, p);
+        if (o < 0) { continue; }
+        const int* slo = S.lows + 3 * o;
+        const long si = rel_index(S, gs, p[0] - slo[0], p[1] - slo[1], p[2] - slo[2]);
+        const long di = rel_index(D, gd, x, y, z);
+        for (int c = 0; c < ncomp; c++) {
+            dst[((long)b * ncomp + c) * dv + di] = src[((long)o * ncomp + c) * sv + si];
+        }
+    }
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ data, long n, T v)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        data[t] = v;
+    }
+}
+
+template<typename T>
+__global__ __launch_bounds__(1024) void k_max_partials(const T* __restrict__ partials, long n, T* out, int combine)
+{
+    __shared__ T red[16];
+    T v = 0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) { v = red_combine(PM_RED_ABS, v, partials[i]); }
+    v = block_reduce(PM_RED_ABS, v, red);
+    if (threadIdx.x == 0) { *out = combine ? red_combine(PM_RED_ABS, *out, v) : v; }
+}
+
+// ------------------------------------------------------------ host helpers
+static unsigned bx_blocks(long n, long cap = 1024)
+{
+    long b = (n + 255) / 256;
+    return (unsigned)std::max(1L, std::min(b, cap));
+}
+
+template<typename T>
+static Lap<T> lap_coefs(int dim, double m)
+{
+    // Laplacian() coefficients (-2 DIM, then 1s) times m, rounded once in T
+    Lap<T> c;
+    for (int j = 0; j < 7; j++) { c.c[j] = 0; }
+    for (int j = 0; j < 2 * dim + 1; j++) {
+        const T lj = (T)(j == 0 ? -2.0 * dim : 1.0);
+        c.c[j] = lj * (T)m;
+    }
+    return c;
+}
+
+int layout_shape(const pm_layout* L, int* dim, int* domain, int* bs, int* periodic, int* nboxes)
+{
+    PM_REQUIRE(L, "null layout");
+    *dim = L->dim;
+    *nboxes = L->nboxes;
+    for (int k = 0; k < 3; k++) {
+        domain[k] = L->domain[k];
+        bs[k] = L->bs[k];
+        periodic[k] = L->periodic[k];
+    }
+    return PM_OK;
+}
+
+int layout_lows(const pm_layout* L, int* lows)
+{
+    PM_REQUIRE(L && lows, "null argument");
+    for (size_t i = 0; i < L->lows.size(); i++) { lows[i] = L->lows[i]; }
+    return PM_OK;
+}
+
+static bool same_layout_shape(const pm_layout* a, const pm_layout* b)
+{
+    for (int k = 0; k < 3; k++) {
+        if (a->domain[k] != b->domain[k]) { return false; }
+    }
+    return a->dim == b->dim;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_layout_create(pm_layout** out, int dim, const int domain[3], const int box_size[3], const int periodic[3],
+                     int nboxes, const int* box_lows_host)
+{
+    PM_REQUIRE(out && domain && box_size, "pm_layout_create: null argument");
+    *out = nullptr;
+    PM_REQUIRE(dim == 2 || dim == 3, "pm_layout_create: dim must be 2 or 3");
+    PM_REQUIRE(nboxes >= 1 && box_lows_host, "pm_layout_create: no boxes");
+    pm_layout* L = new pm_layout();
+    L->dim = dim;
+    L->nboxes = nboxes;
+    long ncell = 1;
+    for (int k = 0; k < 3; k++) {
+        L->domain[k] = (k < dim) ? domain[k] : 1;
+        L->bs[k] = (k < dim) ? box_size[k] : 1;
+        L->periodic[k] = (k < dim && periodic) ? (periodic[k] != 0) : 0;
+        if (L->bs[k] < 1 || L->domain[k] < 1 || L->domain[k] % L->bs[k] != 0) {
+            delete L;
+            set_error("pm_layout_create: box size must divide the domain");
+            return PM_ERR_INVALID;
+        }
+        L->nb[k] = L->domain[k] / L->bs[k];
+        ncell *= L->nb[k];
+    }
+    std::vector<int> table(ncell, -1);
+    L->lows.resize(3 * nboxes);
+    for (int b = 0; b < nboxes; b++) {
+        int gi[3];
+        for (int k = 0; k < 3; k++) {
+            const int lo = (k < dim) ? box_lows_host[3 * b + k] : 0;
+            L->lows[3 * b + k] = lo;
+            if (lo < 0 || lo >= L->domain[k] || lo % L->bs[k] != 0) {
+                delete L;
+                set_error("pm_layout_create: box %d low corner not on the box grid", b);
+                return PM_ERR_INVALID;
+            }
+            gi[k] = lo / L->bs[k];
+        }
+        int& slot = table[gi[0] + (long)L->nb[0] * (gi[1] + (long)L->nb[1] * gi[2])];
+        if (slot >= 0) {
+            delete L;
+            set_error("pm_layout_create: boxes %d and %d overlap", slot, b);
+            return PM_ERR_INVALID;
+        }
+        slot = b;
+    }
+    if (hipMalloc((void**)&L->d_lows, sizeof(int) * 3 * nboxes) != hipSuccess ||
+        hipMalloc((void**)&L->d_table, sizeof(int) * ncell) != hipSuccess) {
+        delete L;
+        set_error("pm_layout_create: device allocation failed");
+        return PM_ERR_NOMEM;
+    }
+    if (hipMemcpy(L->d_lows, L->lows.data(), sizeof(int) * 3 * nboxes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(L->d_table, table.data(), sizeof(int) * ncell, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(L->d_lows);
+        hipFree(L->d_table);
+        delete L;
+        set_error("pm_layout_create: upload failed");
+        return PM_ERR_HIP;
+    }
+    *out = L;
+    return PM_OK;
+}
+
+int pm_layout_destroy(pm_layout* L)
+{
+    if (!L) { return PM_OK; }
+    hipFree(L->d_lows);
+    hipFree(L->d_table);
+    delete L;
+    return PM_OK;
+}
+
+int pm_layout_box_elems(const pm_layout* L, const int ghost[3], size_t* elems)
+{
+    PM_REQUIRE(L && elems, "pm_layout_box_elems: null argument");
+    *elems = (size_t)gvol(ldev(L), g3(ghost));
+    return PM_OK;
+}
+
+#define PM_DISPATCH_T(dtype, CALL)                                         \
+    do {                                                                   \
+        if ((dtype) == PM_F64) { using T = double; CALL; }                 \
+        else if ((dtype) == PM_F32) { using T = float; CALL; }             \
+        else { PM_REQUIRE(false, "bad dtype %d", (int)(dtype)); }          \
+    } while (0)
+
+int pm_level_exchange(const pm_layout* L, int dtype, void* data, const int ghost[3], int ncomp, pm_stream_t stream)
+{
+    PM_REQUIRE(L && data && ncomp >= 1, "pm_level_exchange: bad argument");
+    const G3 g = g3(ghost);
+    if (g.g[0] == 0 && g.g[1] == 0 && g.g[2] == 0) { return PM_OK; }
+    for (int k = 0; k < 3; k++) {
+        PM_REQUIRE(g.g[k] >= 0 && g.g[k] <= L->bs[k], "pm_level_exchange: ghost wider than a box");
+        PM_REQUIRE(k < L->dim || g.g[k] == 0, "pm_level_exchange: ghost beyond dim");
+    }
+    const LayoutDev D = ldev(L);
+    const long X = L->bs[0] + 2 * g.g[0], Y = L->bs[1] + 2 * g.g[1];
+    const long nghost = 2 * (X * Y * g.g[2] + X * g.g[1] * L->bs[2] + (long)g.g[0] * L->bs[1] * L->bs[2]);
+    dim3 grid(bx_blocks(nghost, 256), L->nboxes);
+    PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_exchange<T>, grid, dim3(256), 0, S(stream), D, g, (T*)data, ncomp));
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_gsrb(const pm_layout* L, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
+                  const int rhs_ghost[3], double dx, int color, pm_stream_t stream)
+{
+    PM_REQUIRE(L && phi && rhs && (color == 0 || color == 1), "pm_level_gsrb: bad argument");
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost);
+    for (int k = 0; k < L->dim; k++) { PM_REQUIRE(gp.g[k] >= 1, "pm_level_gsrb: phi needs a ghost layer"); }
+    const LayoutDev D = ldev(L);
+    const long n = (long)((L->bs[0] + 1) / 2) * L->bs[1] * L->bs[2];
+    dim3 grid(bx_blocks(n), L->nboxes);
+    // LaplaceOp::init (LaplaceOp.H:56-73): lambda = 2 * (-(dx*dx)/(4 DIM)); coefficients Lap_j * (-1/(dx*dx))
+    double lam = -(dx * dx) / (4.0 * L->dim);
+    lam *= 2.0;
+    const double m = -1.0 / (dx * dx);
+    PM_DISPATCH_T(dtype, {
+        Lap<T> c = lap_coefs<T>(L->dim, m);
+        if (dtype == PM_F64) {
+            for (int j = 0; j < 2 * L->dim + 1; j++) { c.c[j] = (T)((j == 0 ? -2.0 * L->dim : 1.0) * m); }
+        }
+        if (L->dim == 3) {
+            hipLaunchKernelGGL((k_gsrb<T, 3>), grid, dim3(256), 0, S(stream), D, gp, gr, (T*)phi, (const T*)rhs, c,
+                               (T)(1.0 * lam), color);
+        } else {
+            hipLaunchKernelGGL((k_gsrb<T, 2>), grid, dim3(256), 0, S(stream), D, gp, gr, (T*)phi, (const T*)rhs, c,
+                               (T)(1.0 * lam), color);
+        }
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_jacobi(const pm_layout* L, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
+                    const int rhs_ghost[3], void* tmp, double dx, pm_stream_t stream)
+{
+    PM_REQUIRE(L && phi && rhs && tmp, "pm_level_jacobi: bad argument");
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost);
+    const LayoutDev D = ldev(L);
+    const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
+    dim3 grid(bx_blocks(n), L->nboxes);
+    // pointRelax (LevelMultigrid.cpp:99-123): weight 1/(4 DIM), lambda = dx^2/(4 DIM)
+    const double weight = 1.0 / (4 * L->dim);
+    const double lambda = dx * dx / (4 * L->dim);
+    PM_DISPATCH_T(dtype, {
+        Lap<T> c;
+        for (int j = 0; j < 7; j++) { c.c[j] = 0; }
+        for (int j = 0; j < 2 * L->dim + 1; j++) { c.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)weight; }
+        if (L->dim == 3) {
+            hipLaunchKernelGGL((k_jacobi_tmp<T, 3>), grid, dim3(256), 0, S(stream), D, gp, gr, (const T*)phi,
+                               (const T*)rhs, (T*)tmp, c, (T)(-lambda));
+        } else {
+            hipLaunchKernelGGL((k_jacobi_tmp<T, 2>), grid, dim3(256), 0, S(stream), D, gp, gr, (const T*)phi,
+                               (const T*)rhs, (T*)tmp, c, (T)(-lambda));
+        }
+        hipLaunchKernelGGL(k_jacobi_add<T>, grid, dim3(256), 0, S(stream), D, gp, (T*)phi, (const T*)tmp);
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_resnorm(const pm_layout* L, int dtype, const void* phi, const int phi_ghost[3], const void* rhs,
+                     const int rhs_ghost[3], double dx, void* out_dev, int combine, pm_stream_t stream)
+{
+    PM_REQUIRE(L && phi && rhs && out_dev, "pm_level_resnorm: bad argument");
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost);
+    const LayoutDev D = ldev(L);
+    const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
+    const unsigned bpb = bx_blocks(n, 64);
+    dim3 grid(bpb, L->nboxes);
+    const double hsqinv = 1. / (dx * dx);
+    void* ws = nullptr;
+    PM_DISPATCH_T(dtype, {
+        int rc = workspace(S(stream), sizeof(T) * (size_t)bpb * L->nboxes, &ws);
+        if (rc) { return rc; }
+        Lap<T> c;
+        for (int j = 0; j < 7; j++) { c.c[j] = 0; }
+        for (int j = 0; j < 2 * L->dim + 1; j++) { c.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)hsqinv; }
+        if (L->dim == 3) {
+            hipLaunchKernelGGL((k_resnorm<T, 3>), grid, dim3(256), 0, S(stream), D, gp, gr, (const T*)phi,
+                               (const T*)rhs, c, (T*)ws);
+        } else {
+            hipLaunchKernelGGL((k_resnorm<T, 2>), grid, dim3(256), 0, S(stream), D, gp, gr, (const T*)phi,
+                               (const T*)rhs, c, (T*)ws);
+        }
+        hipLaunchKernelGGL(k_max_partials<T>, dim3(1), dim3(1024), 0, S(stream), (const T*)ws,
+                           (long)bpb * L->nboxes, (T*)out_dev, combine);
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_restrict_residual(const pm_layout* F, const pm_layout* C, int dtype, const void* phi,
+                               const int phi_ghost[3], const void* rhs, const int rhs_ghost[3], double dx,
+                               void* resc, const int resc_ghost[3], pm_stream_t stream)
+{
+    PM_REQUIRE(F && C && phi && rhs && resc, "pm_level_restrict_residual: bad argument");
+    PM_REQUIRE(F->dim == C->dim, "pm_level_restrict_residual: dim mismatch");
+    for (int k = 0; k < F->dim; k++) {
+        PM_REQUIRE(C->domain[k] * 2 == F->domain[k], "pm_level_restrict_residual: coarse domain is not fine/2");
+        PM_REQUIRE(F->bs[k] % 2 == 0, "pm_level_restrict_residual: fine boxes must have even size");
+    }
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost), gc = g3(resc_ghost);
+    const LayoutDev DF = ldev(F), DC = ldev(C);
+    const long n = (long)C->bs[0] * C->bs[1] * C->bs[2];
+    dim3 grid(bx_blocks(n), C->nboxes);
+    const double hsqinv = 1. / (dx * dx);
+    PM_DISPATCH_T(dtype, {
+        Lap<T> c;
+        for (int j = 0; j < 7; j++) { c.c[j] = 0; }
+        for (int j = 0; j < 2 * F->dim + 1; j++) { c.c[j] = (T)(j == 0 ? -2.0 * F->dim : 1.0) * (T)(-hsqinv); }
+        const T avg = (T)(1.0 / (F->dim == 3 ? 8 : 4));
+        if (F->dim == 3) {
+            hipLaunchKernelGGL((k_restrict_residual<T, 3>), grid, dim3(256), 0, S(stream), DF, DC, gp, gr, gc,
+                               (const T*)phi, (const T*)rhs, (T*)resc, c, avg);
+        } else {
+            hipLaunchKernelGGL((k_restrict_residual<T, 2>), grid, dim3(256), 0, S(stream), DF, DC, gp, gr, gc,
+                               (const T*)phi, (const T*)rhs, (T*)resc, c, avg);
+        }
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_prolong_add(const pm_layout* F, const pm_layout* C, int dtype, const void* delta, const int delta_ghost[3],
+                         void* phi, const int phi_ghost[3], int linear, pm_stream_t stream)
+{
+    PM_REQUIRE(F && C && delta && phi, "pm_level_prolong_add: bad argument");
+    PM_REQUIRE(F->dim == C->dim, "pm_level_prolong_add: dim mismatch");
+    for (int k = 0; k < F->dim; k++) {
+        PM_REQUIRE(C->domain[k] * 2 == F->domain[k], "pm_level_prolong_add: coarse domain is not fine/2");
+        PM_REQUIRE(!linear || C->periodic[k], "pm_level_prolong_add: Linear needs a periodic coarse level");
+    }
+    const G3 gd = g3(delta_ghost), gp = g3(phi_ghost);
+    const LayoutDev DF = ldev(F), DC = ldev(C);
+    const long n = (long)F->bs[0] * F->bs[1] * F->bs[2];
+    dim3 grid(bx_blocks(n), F->nboxes);
+    PM_DISPATCH_T(dtype, {
+        if (F->dim == 3) {
+            if (linear) {
+                hipLaunchKernelGGL((k_prolong_add<T, 3, 1>), grid, dim3(256), 0, S(stream), DF, DC, gd, gp,
+                                   (const T*)delta, (T*)phi);
+            } else {
+                hipLaunchKernelGGL((k_prolong_add<T, 3, 0>), grid, dim3(256), 0, S(stream), DF, DC, gd, gp,
+                                   (const T*)delta, (T*)phi);
+            }
+        } else {
+            if (linear) {
+                hipLaunchKernelGGL((k_prolong_add<T, 2, 1>), grid, dim3(256), 0, S(stream), DF, DC, gd, gp,
+                                   (const T*)delta, (T*)phi);
+            } else {
+                hipLaunchKernelGGL((k_prolong_add<T, 2, 0>), grid, dim3(256), 0, S(stream), DF, DC, gd, gp,
+                                   (const T*)delta, (T*)phi);
+            }
+        }
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_setval(const pm_layout* L, int dtype, void* data, const int ghost[3], int ncomp, double value,
+                    pm_stream_t stream)
+{
+    PM_REQUIRE(L && data && ncomp >= 1, "pm_level_setval: bad argument");
+    const long n = gvol(ldev(L), g3(ghost)) * ncomp * L->nboxes;
+    PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_fill<T>, dim3(bx_blocks(n, 1 << 16)), dim3(256), 0, S(stream), (T*)data,
+                                            n, (T)value));
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_copy(const pm_layout* SL, const void* src, const int src_ghost[3], const pm_layout* DL, void* dst,
+                  const int dst_ghost[3], int dtype, int ncomp, pm_stream_t stream)
+{
+    PM_REQUIRE(SL && DL && src && dst && ncomp >= 1, "pm_level_copy: bad argument");
+    PM_REQUIRE(same_layout_shape(SL, DL), "pm_level_copy: layouts of different domains");
+    const LayoutDev DS = ldev(SL), DD = ldev(DL);
+    const long n = (long)DL->bs[0] * DL->bs[1] * DL->bs[2];
+    dim3 grid(bx_blocks(n), DL->nboxes);
+    PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_level_copy<T>, grid, dim3(256), 0, S(stream), DS, g3(src_ghost),
+                                            (const T*)src, DD, g3(dst_ghost), (T*)dst, ncomp));
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Residual into any dtype with max|r|, and the FP64 += FP32 correction.
+namespace pm {
+
+template<typename T, typename TO, int DIMN>
+__global__ __launch_bounds__(256) void k_residual(LayoutDev L, G3 gp, G3 gr, G3 go, const T* __restrict__ phi,
+                                                  const T* __restrict__ rhs, TO* __restrict__ out, Lap<T> c,
+                                                  T* __restrict__ partials)
+{
+    constexpr int NC = 2 * DIMN + 1;
+    __shared__ T red[16];
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    const T* ph = phi + (long)b * gvol(L, gp);
+    const T* rh = rhs + (long)b * gvol(L, gr);
+    TO* ot = out + (long)b * gvol(L, go);
+    const long X = L.bs[0] + 2 * gp.g[0], XY = X * (L.bs[1] + 2 * gp.g[1]);
+    T m = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const long i = rel_index(L, gp, x, y, z);
+        T s = 0;
+        s = s + rh[rel_index(L, gr, x, y, z)];
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            s = s + c.c[j] * ph[i + lapo(j, 0) + lapo(j, 1) * X + lapo(j, 2) * XY];
+        }
+        ot[rel_index(L, go, x, y, z)] = (TO)s;
+        const T a = s > 0 ? s : -s;
+        m = m < a ? a : m;
+    }
+    m = block_reduce(PM_RED_ABS, m, red);
+    if (threadIdx.x == 0) { partials[(long)b * gridDim.x + blockIdx.x] = m; }
+}
+
+template<typename T, typename TE>
+__global__ __launch_bounds__(256) void k_add_correction(LayoutDev L, G3 gp, G3 ge, T* __restrict__ phi,
+                                                        const TE* __restrict__ e)
+{
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    T* ph = phi + (long)b * gvol(L, gp);
+    const TE* ee = e + (long)b * gvol(L, ge);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const long i = rel_index(L, gp, x, y, z);
+        ph[i] = ph[i] + (T)ee[rel_index(L, ge, x, y, z)];
+    }
+}
+
+}  // namespace pm
+
+extern "C" {
+
+int pm_level_residual(const pm_layout* L, int dtype, const void* phi, const int phi_ghost[3], const void* rhs,
+                      const int rhs_ghost[3], double dx, int out_dtype, void* out, const int out_ghost[3],
+                      void* norm_out, pm_stream_t stream)
+{
+    PM_REQUIRE(L && phi && rhs && out && norm_out, "pm_level_residual: bad argument");
+    PM_REQUIRE(dtype == PM_F64 || (dtype == PM_F32 && out_dtype == PM_F32), "pm_level_residual: unsupported dtypes");
+    PM_REQUIRE(out_dtype == PM_F32 || out_dtype == PM_F64, "pm_level_residual: bad out dtype");
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost), go = g3(out_ghost);
+    const LayoutDev D = ldev(L);
+    const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
+    const unsigned bpb = bx_blocks(n, 64);
+    dim3 grid(bpb, L->nboxes);
+    const double hsqinv = 1. / (dx * dx);
+    void* ws = nullptr;
+    PM_DISPATCH_T(dtype, {
+        int rc = workspace(S(stream), sizeof(T) * (size_t)bpb * L->nboxes, &ws);
+        if (rc) { return rc; }
+        Lap<T> c;
+        for (int j = 0; j < 7; j++) { c.c[j] = 0; }
+        for (int j = 0; j < 2 * L->dim + 1; j++) { c.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)(-hsqinv); }
+        if (out_dtype == PM_F32) {
+            if (L->dim == 3) {
+                hipLaunchKernelGGL((k_residual<T, float, 3>), grid, dim3(256), 0, S(stream), D, gp, gr, go,
+                                   (const T*)phi, (const T*)rhs, (float*)out, c, (T*)ws);
+            } else {
+                hipLaunchKernelGGL((k_residual<T, float, 2>), grid, dim3(256), 0, S(stream), D, gp, gr, go,
+                                   (const T*)phi, (const T*)rhs, (float*)out, c, (T*)ws);
+            }
+        } else {
+            if (L->dim == 3) {
+                hipLaunchKernelGGL((k_residual<T, double, 3>), grid, dim3(256), 0, S(stream), D, gp, gr, go,
+                                   (const T*)phi, (const T*)rhs, (double*)out, c, (T*)ws);
+            } else {
+                hipLaunchKernelGGL((k_residual<T, double, 2>), grid, dim3(256), 0, S(stream), D, gp, gr, go,
+                                   (const T*)phi, (const T*)rhs, (double*)out, c, (T*)ws);
+            }
+        }
+        hipLaunchKernelGGL(k_max_partials<T>, dim3(1), dim3(1024), 0, S(stream), (const T*)ws,
+                           (long)bpb * L->nboxes, (T*)norm_out, 0);
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_add_correction(const pm_layout* L, int dtype, void* phi, const int phi_ghost[3], int e_dtype,
+                            const void* e, const int e_ghost[3], pm_stream_t stream)
+{
+    PM_REQUIRE(L && phi && e, "pm_level_add_correction: bad argument");
+    const G3 gp = g3(phi_ghost), ge = g3(e_ghost);
+    const LayoutDev D = ldev(L);
+    const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
+    dim3 grid(bx_blocks(n), L->nboxes);
+    if (dtype == PM_F64 && e_dtype == PM_F32) {
+        hipLaunchKernelGGL((k_add_correction<double, float>), grid, dim3(256), 0, S(stream), D, gp, ge, (double*)phi,
+                           (const float*)e);
+    } else if (dtype == PM_F64 && e_dtype == PM_F64) {
+        hipLaunchKernelGGL((k_add_correction<double, double>), grid, dim3(256), 0, S(stream), D, gp, ge,
+                           (double*)phi, (const double*)e);
+    } else if (dtype == PM_F32 && e_dtype == PM_F32) {
+        hipLaunchKernelGGL((k_add_correction<float, float>), grid, dim3(256), 0, S(stream), D, gp, ge, (float*)phi,
+                           (const float*)e);
+    } else {
+        PM_REQUIRE(false, "pm_level_add_correction: unsupported dtypes");
+    }
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+}  // extern "C"

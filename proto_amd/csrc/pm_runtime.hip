@@ -1,0 +1,303 @@
+// Runtime entry points: errors, device memory, streams, seeded inputs.
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <algorithm>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+struct WsEntry {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+static std::mutex g_ws_mu;
+static std::map<hipStream_t, WsEntry>* g_ws = nullptr;
+
+int workspace(hipStream_t stream, size_t bytes, void** out)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    if (!g_ws) { g_ws = new std::map<hipStream_t, WsEntry>(); }
+    WsEntry& w = (*g_ws)[stream];
+    if (bytes > w.bytes) {
+        if (w.ptr) {
+            PM_CHECK_HIP(hipStreamSynchronize(stream));
+            PM_CHECK_HIP(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        size_t nb = bytes < ((size_t)1 << 16) ? ((size_t)1 << 16) : bytes;
+        hipError_t e = hipMalloc(&w.ptr, nb);
+        if (e != hipSuccess) {
+            set_error("workspace hipMalloc(%zu): %s", nb, hipGetErrorString(e));
+            w.ptr = nullptr;
+            return PM_ERR_NOMEM;
+        }
+        w.bytes = nb;
+    }
+    *out = w.ptr;
+    return PM_OK;
+}
+
+// Drops the scratch of a stream that is about to be destroyed.
+static void workspace_release(hipStream_t stream)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    if (!g_ws) { return; }
+    auto it = g_ws->find(stream);
+    if (it == g_ws->end()) { return; }
+    if (it->second.ptr) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(it->second.ptr);
+    }
+    g_ws->erase(it);
+}
+
+// SplitMix64 (Steele, Lea, Flood 2014) in counter form: z_i = mix(s0 + (i+1) G),
+// s0 = seed + stream * 0x632BE59BD9B4E019.  u = ((z >> 11) 2^-53) 2 - 1.
+__device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+template<typename T>
+__global__ void k_fill_uniform(T* __restrict__ out, size_t n, uint64_t s0, double a, double b, int affine)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t z = mix64(s0 + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL);
+        double u = ((double)(z >> 11) * 0x1.0p-53) * 2.0 - 1.0;
+        if (affine) { u = a * u + b; }
+        out[i] = (T)u;
+    }
+}
+
+template<typename T>
+__global__ void k_fill_separable(T* __restrict__ out, Box3 box, Box3 region, int px, int py, int pz,
+                                 const double* __restrict__ tx, const double* __restrict__ ty,
+                                 const double* __restrict__ tz, double amp, uint64_t s0)
+{
+    const long nx = box_nx(region), ny = box_ny(region);
+    const long n = box_size(region);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = region.lo[0] + (int)(t % nx);
+        const long r = t / nx;
+        const int y = region.lo[1] + (int)(r % ny);
+        const int z = region.lo[2] + (int)(r / ny);
+        const int wx = ((x % px) + px) % px, wy = ((y % py) + py) % py, wz = ((z % pz) + pz) % pz;
+        double v = (tx[wx] * ty[wy]) * tz[wz];
+        if (amp != 0.0) {
+            const uint64_t i = (uint64_t)wx + (uint64_t)px * ((uint64_t)wy + (uint64_t)py * (uint64_t)wz);
+            const uint64_t zz = mix64(s0 + (i + 1) * 0x9E3779B97F4A7C15ULL);
+            const double u = ((double)(zz >> 11) * 0x1.0p-53) * 2.0 - 1.0;
+            v = v + amp * u;
+        }
+        out[box_index(box, x, y, z)] = (T)v;
+    }
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+const char* pm_last_error(void) { return g_err; }
+const char* pm_version(void) { return "protomi 0.1 (gfx950)"; }
+
+int pm_device_count(int* count)
+{
+    PM_REQUIRE(count, "pm_device_count: null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return PM_OK;
+}
+
+int pm_malloc(void** ptr, size_t bytes)
+{
+    PM_REQUIRE(ptr, "pm_malloc: null ptr");
+    *ptr = nullptr;
+    if (bytes == 0) { return PM_OK; }
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+        *ptr = nullptr;
+        return e == hipErrorOutOfMemory ? PM_ERR_NOMEM : PM_ERR_HIP;
+    }
+    return PM_OK;
+}
+
+int pm_free(void* ptr)
+{
+    if (!ptr) { return PM_OK; }
+    PM_CHECK_HIP(hipFree(ptr));
+    return PM_OK;
+}
+
+int pm_memcpy(void* dst, const void* src, size_t bytes, int kind, pm_stream_t stream)
+{
+    if (bytes == 0) { return PM_OK; }
+    PM_REQUIRE(dst && src, "pm_memcpy: null pointer");
+    hipMemcpyKind k;
+    switch (kind) {
+        case 0: k = hipMemcpyHostToDevice; break;
+        case 1: k = hipMemcpyDeviceToHost; break;
+        case 2: k = hipMemcpyDeviceToDevice; break;
+        default: PM_REQUIRE(false, "pm_memcpy: bad kind %d", kind);
+    }
+    PM_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, k, S(stream)));
+    return PM_OK;
+}
+
+int pm_memset_zero(void* ptr, size_t bytes, pm_stream_t stream)
+{
+    if (bytes == 0) { return PM_OK; }
+    PM_REQUIRE(ptr, "pm_memset_zero: null pointer");
+    PM_CHECK_HIP(hipMemsetAsync(ptr, 0, bytes, S(stream)));
+    return PM_OK;
+}
+
+int pm_stream_create(pm_stream_t* stream)
+{
+    PM_REQUIRE(stream, "pm_stream_create: null out");
+    hipStream_t s;
+    PM_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (pm_stream_t)s;
+    return PM_OK;
+}
+
+int pm_stream_destroy(pm_stream_t stream)
+{
+    if (!stream) { return PM_OK; }
+    workspace_release(S(stream));
+    PM_CHECK_HIP(hipStreamDestroy(S(stream)));
+    return PM_OK;
+}
+
+int pm_stream_sync(pm_stream_t stream)
+{
+    PM_CHECK_HIP(hipStreamSynchronize(S(stream)));
+    return PM_OK;
+}
+
+int pm_device_sync(void)
+{
+    PM_CHECK_HIP(hipDeviceSynchronize());
+    return PM_OK;
+}
+
+int pm_event_create(pm_event_t* ev)
+{
+    PM_REQUIRE(ev, "pm_event_create: null out");
+    hipEvent_t e;
+    PM_CHECK_HIP(hipEventCreate(&e));
+    *ev = (pm_event_t)e;
+    return PM_OK;
+}
+
+int pm_event_destroy(pm_event_t ev)
+{
+    if (!ev) { return PM_OK; }
+    PM_CHECK_HIP(hipEventDestroy((hipEvent_t)ev));
+    return PM_OK;
+}
+
+int pm_event_record(pm_event_t ev, pm_stream_t stream)
+{
+    PM_REQUIRE(ev, "pm_event_record: null event");
+    PM_CHECK_HIP(hipEventRecord((hipEvent_t)ev, S(stream)));
+    return PM_OK;
+}
+
+int pm_event_sync(pm_event_t ev)
+{
+    PM_REQUIRE(ev, "pm_event_sync: null event");
+    PM_CHECK_HIP(hipEventSynchronize((hipEvent_t)ev));
+    return PM_OK;
+}
+
+int pm_event_elapsed_ms(pm_event_t start, pm_event_t stop, float* ms)
+{
+    PM_REQUIRE(start && stop && ms, "pm_event_elapsed_ms: null argument");
+    PM_CHECK_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return PM_OK;
+}
+
+int pm_fill_separable(int dtype, void* data, const pm_box* box, const pm_box* region, const int period[3],
+                      const double* tx_host, const double* ty_host, const double* tz_host, double amp,
+                      uint64_t seed, uint64_t stream_id, pm_stream_t stream)
+{
+    PM_REQUIRE(data && box && region && period && tx_host && ty_host && tz_host, "pm_fill_separable: null argument");
+    PM_REQUIRE(dtype == PM_F32 || dtype == PM_F64, "pm_fill_separable: bad dtype %d", dtype);
+    for (int d = 0; d < 3; d++) { PM_REQUIRE(period[d] >= 1, "pm_fill_separable: period must be >= 1"); }
+    const Box3 b = box3(box), r = box3(region);
+    PM_REQUIRE(box_contains_box(b, r), "pm_fill_separable: region outside the box");
+    if (box_empty(r)) { return PM_OK; }
+    const size_t nt = (size_t)period[0] + period[1] + period[2];
+    double* dt = nullptr;
+    PM_CHECK_HIP(hipMalloc(&dt, sizeof(double) * nt));
+    hipError_t e1 = hipMemcpy(dt, tx_host, sizeof(double) * period[0], hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(dt + period[0], ty_host, sizeof(double) * period[1], hipMemcpyHostToDevice);
+    hipError_t e3 = hipMemcpy(dt + period[0] + period[1], tz_host, sizeof(double) * period[2], hipMemcpyHostToDevice);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        hipFree(dt);
+        set_error("pm_fill_separable: table upload failed");
+        return PM_ERR_HIP;
+    }
+    const uint64_t s0 = seed + stream_id * 0x632BE59BD9B4E019ULL;
+    size_t blocks = std::min<size_t>(((size_t)box_size(r) + 255) / 256, 65536);
+    if (dtype == PM_F64) {
+        hipLaunchKernelGGL(k_fill_separable<double>, dim3((unsigned)blocks), dim3(256), 0, S(stream), (double*)data,
+                           b, r, period[0], period[1], period[2], dt, dt + period[0], dt + period[0] + period[1],
+                           amp, s0);
+    } else {
+        hipLaunchKernelGGL(k_fill_separable<float>, dim3((unsigned)blocks), dim3(256), 0, S(stream), (float*)data,
+                           b, r, period[0], period[1], period[2], dt, dt + period[0], dt + period[0] + period[1],
+                           amp, s0);
+    }
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(S(stream));
+    hipFree(dt);
+    if (le != hipSuccess || se != hipSuccess) {
+        set_error("pm_fill_separable: %s", hipGetErrorString(le != hipSuccess ? le : se));
+        return PM_ERR_HIP;
+    }
+    return PM_OK;
+}
+
+int pm_fill_uniform(int dtype, void* data, size_t n, uint64_t seed, uint64_t stream_id, double a,
+                    double b, pm_stream_t stream)
+{
+    if (n == 0) { return PM_OK; }
+    PM_REQUIRE(data, "pm_fill_uniform: null data");
+    PM_REQUIRE(dtype == PM_F32 || dtype == PM_F64, "pm_fill_uniform: bad dtype %d", dtype);
+    uint64_t s0 = seed + stream_id * 0x632BE59BD9B4E019ULL;
+    int affine = !(a == 1.0 && b == 0.0);
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) { blocks = 65536; }
+    if (dtype == PM_F64) {
+        hipLaunchKernelGGL(k_fill_uniform<double>, dim3((unsigned)blocks), dim3(256), 0, S(stream),
+                           (double*)data, n, s0, a, b, affine);
+    } else {
+        hipLaunchKernelGGL(k_fill_uniform<float>, dim3((unsigned)blocks), dim3(256), 0, S(stream),
+                           (float*)data, n, s0, a, b, affine);
+    }
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+}  // extern "C"

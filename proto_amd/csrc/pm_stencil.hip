@@ -1,0 +1,655 @@
+// Stencil<T> on the device: handle, generic kernel, and the z-marching
+// LDS-tiled kernels for the 7-point and 27-point Laplacian shapes.
+//
+// Parity: every kernel accumulates in Proto's coefficient order with one
+// rounding per multiply and per add (the library is compiled with
+// -ffp-contract=off), starting from 0 (replace) or the old dst (accumulate),
+// with coefficients pre-multiplied by scale when scale != 1 -- the arithmetic
+// of Stencil::hostApply (Proto_StencilImplem.H:725-828), so results are
+// bit-identical to Proto's CPU path.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pm_internal.hpp"
+
+struct pm_stencil {
+    int dtype, dim, ncoef, kind;
+    std::vector<double> coefs;  // exact T values held as double
+    std::vector<int> offs;      // ncoef x 3
+    int sr[3], dr[3], ds[3];
+    int span_lo[3], span_hi[3];
+    void* d_coefs;  // T[ncoef]
+    int* d_offs;    // int[3 ncoef]
+};
+
+namespace pm {
+
+// ------------------------------------------------------------ shapes
+// Offsets of Stencil<T>::Laplacian() in its coefficient order
+// (Proto_StencilDefs.H:77-87): centre, +x, -x, +y, -y, +z, -z.
+__host__ __device__ constexpr int lap7_off(int j, int d)
+{
+    constexpr int t[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    return t[j][d];
+}
+// Offsets of Stencil<T>::Laplacian_27() (Proto_StencilDefs.H:130-149): the
+// centre, then Box::Cube(3) shifted by -1 in x-fastest order without it.
+__host__ __device__ constexpr int lap27_off(int j, int d)
+{
+    int k = (j == 0) ? 13 : ((j - 1) >= 13 ? j : j - 1);
+    int v = (d == 0) ? (k % 3) : (d == 1 ? (k / 3) % 3 : k / 9);
+    return v - 1;
+}
+template<int SHAPE> __host__ __device__ constexpr int shape_off(int j, int d)
+{
+    return SHAPE == 7 ? lap7_off(j, d) : lap27_off(j, d);
+}
+
+static int detect_kind(const pm_stencil* s)
+{
+    bool unit = true;
+    for (int d = 0; d < 3; d++) {
+        if (s->sr[d] != 1 || s->dr[d] != 1 || s->ds[d] != 0) { unit = false; }
+    }
+    if (!unit || s->dim != 3) { return 0; }
+    if (s->ncoef == 7) {
+        for (int j = 0; j < 7; j++)
+            for (int d = 0; d < 3; d++)
+                if (s->offs[3 * j + d] != lap7_off(j, d)) { return 0; }
+        return 7;
+    }
+    if (s->ncoef == 27) {
+        for (int j = 0; j < 27; j++)
+            for (int d = 0; d < 3; d++)
+                if (s->offs[3 * j + d] != lap27_off(j, d)) { return 0; }
+        return 27;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------ generic kernel
+template<typename T>
+struct GenArgs {
+    const T* src;
+    T* dst;
+    Box3 sbox, dbox, ibox;
+    const T* coefs;
+    const int* offs;
+    int ncoef, ncomp, replace, apply_scale;
+    T scale;
+    int sr[3], dr[3], ds[3];
+};
+
+// One thread per index point; the destination pencil start must lie in dst
+// (hostApply skips the pencil otherwise, Proto_StencilImplem.H:811).
+template<typename T>
+__global__ __launch_bounds__(256) void k_stencil_generic(GenArgs<T> a)
+{
+    const long nx = box_nx(a.ibox), ny = box_ny(a.ibox);
+    const long n = box_size(a.ibox);
+    const long sx = box_nx(a.sbox), sxy = sx * box_ny(a.sbox);
+    const long ssz = box_size(a.sbox), dsz = box_size(a.dbox);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int i = a.ibox.lo[0] + (int)(t % nx);
+        const long r = t / nx;
+        const int j = a.ibox.lo[1] + (int)(r % ny);
+        const int k = a.ibox.lo[2] + (int)(r / ny);
+        const int px = i * a.dr[0] + a.ds[0], py = j * a.dr[1] + a.ds[1], pz = k * a.dr[2] + a.ds[2];
+        if (!box_contains(a.dbox, a.ibox.lo[0] * a.dr[0] + a.ds[0], py, pz)) { continue; }
+        if (!box_contains(a.dbox, px, py, pz)) { continue; }
+        const long di = box_index(a.dbox, px, py, pz);
+        const long si = box_index(a.sbox, i * a.sr[0], j * a.sr[1], k * a.sr[2]);
+        for (int c = 0; c < a.ncomp; c++) {
+            T acc = a.replace ? (T)0 : a.dst[di + c * dsz];
+            for (int jj = 0; jj < a.ncoef; jj++) {
+                T cc = a.coefs[jj];
+                if (a.apply_scale) { cc = cc * a.scale; }
+                const long o = a.offs[3 * jj] + a.offs[3 * jj + 1] * sx + a.offs[3 * jj + 2] * sxy;
+                acc = acc + cc * a.src[si + c * ssz + o];
+            }
+            a.dst[di + c * dsz] = acc;
+        }
+    }
+}
+
+// ------------------------------------------------------------ z-march kernels
+// Block = 4 wave64s over a 64 x (4 RY) tile of one (x, y) column range; the
+// block walks its z range plane by plane.  Each plane (with a 1-cell halo in
+// x and y) is staged once from HBM into a double-buffered LDS tile; each
+// thread keeps the 3 x 3 neighbourhoods of its RY rows for planes z-1, z,
+// z+1 in registers (register queue along z), so one LDS pass per plane feeds
+// all 27 taps.  Global loads of plane z+1 are issued before the barrier of
+// plane z so their latency overlaps the arithmetic.
+enum { ZM_REPLACE = 0, ZM_ACCUM = 1, ZM_RESIDUAL = 2, ZM_RESIDUAL2 = 3 };
+
+template<typename T>
+struct ZArgs {
+    const T* src;   // u on sbox
+    T* dst;         // dst on dbox (or r on ibox for residual modes)
+    T* dst2;        // second residual (ZM_RESIDUAL2), on ibox
+    const T* f;     // f on ibox (residual modes)
+    T* partials;    // per block {amax, sum} (x2 for ZM_RESIDUAL2)
+    Box3 sbox, dbox, ibox;
+    int ntx, nty, tz;
+    T c[27];        // coefficients, pre-scaled (first operator)
+    T c2[27];       // second operator (ZM_RESIDUAL2, always the 27-point shape)
+};
+
+template<typename T, int SHAPE, int MODE, int RY>
+__global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
+{
+    constexpr int TX = 64, TY = 4 * RY, LX = TX + 2, LY = TY + 2;
+    constexpr int NE = LX * LY, NL = (NE + 255) / 256;
+    constexpr int NC = (SHAPE == 7) ? 7 : 27;
+    __shared__ T sm[2][LY][LX];
+    __shared__ T red[16];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = b % a.ntx;
+    b /= a.ntx;
+    const int ty = b % a.nty;
+    const int tzi = b / a.nty;
+    const int x0 = a.ibox.lo[0] + tx * TX, y0 = a.ibox.lo[1] + ty * TY;
+    const int zb = a.ibox.lo[2] + tzi * a.tz;
+    const int ze = min(zb + a.tz, a.ibox.hi[2] + 1);  // outputs z in [zb, ze)
+
+    const long sx = box_nx(a.sbox), sxy = sx * box_ny(a.sbox);
+    // per-thread plane element offsets (constant along z)
+    long eoff[NL];
+    bool eok[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = tid + 256 * k;
+        const int row = e / LX, col = e - (e / LX) * LX;
+        const int gx = x0 - 1 + col, gy = y0 - 1 + row;
+        eok[k] = (e < NE) && gx >= a.sbox.lo[0] && gx <= a.sbox.hi[0] && gy >= a.sbox.lo[1] && gy <= a.sbox.hi[1];
+        eoff[k] = (long)(gx - a.sbox.lo[0]) + sx * (long)(gy - a.sbox.lo[1]);
+    }
+    T pre[NL];
+    auto load = [&](int z) {
+        const T* base = a.src + sxy * (long)(z - a.sbox.lo[2]);
+#pragma unroll
+        for (int k = 0; k < NL; k++) { pre[k] = eok[k] ? base[eoff[k]] : (T)0; }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const int e = tid + 256 * k;
+            if (e < NE) { (&sm[buf][0][0])[e] = pre[k]; }
+        }
+    };
+
+    T q[3][RY + 2][3];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int r = 0; r < RY + 2; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) q[p][r][c] = (T)0;
+
+    T amax = 0, sum = 0, amax2 = 0, sum2 = 0;
+    const int gx = x0 + lane;
+    const bool xok = gx <= a.ibox.hi[0];
+
+    int buf = 0;
+    load(zb - 1);
+    for (int zz = zb - 1; zz <= ze; ++zz) {
+        store(buf);
+        if (zz + 1 <= ze) { load(zz + 1); }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RY + 2; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                q[0][r][c] = q[1][r][c];
+                q[1][r][c] = q[2][r][c];
+                q[2][r][c] = sm[buf][w * RY + r][lane + c];
+            }
+        if (zz >= zb + 1) {
+            const int z = zz - 1;
+#pragma unroll
+            for (int o = 0; o < RY; o++) {
+                const int gy = y0 + w * RY + o;
+                if (!(xok && gy <= a.ibox.hi[1])) { continue; }
+                T acc = 0;
+                if (MODE == ZM_ACCUM) { acc = a.dst[box_index(a.dbox, gx, gy, z)]; }
+#pragma unroll
+                for (int j = 0; j < NC; j++) {
+                    const int dx = shape_off<SHAPE>(j, 0), dy = shape_off<SHAPE>(j, 1), dz = shape_off<SHAPE>(j, 2);
+                    acc = acc + a.c[j] * q[1 + dz][o + 1 + dy][1 + dx];
+                }
+                if (MODE == ZM_REPLACE || MODE == ZM_ACCUM) {
+                    a.dst[box_index(a.dbox, gx, gy, z)] = acc;
+                } else {
+                    const long ii = box_index(a.ibox, gx, gy, z);
+                    const T fv = a.f[ii];
+                    const T rv = fv - acc;
+                    a.dst[ii] = rv;
+                    const T av = rv > 0 ? rv : -rv;
+                    amax = amax < av ? av : amax;
+                    sum = sum + rv;
+                    if (MODE == ZM_RESIDUAL2) {
+                        T acc2 = 0;
+#pragma unroll
+                        for (int j = 0; j < 27; j++) {
+                            const int dx = lap27_off(j, 0), dy = lap27_off(j, 1), dz = lap27_off(j, 2);
+                            acc2 = acc2 + a.c2[j] * q[1 + dz][o + 1 + dy][1 + dx];
+                        }
+                        const T rv2 = fv - acc2;
+                        a.dst2[ii] = rv2;
+                        const T av2 = rv2 > 0 ? rv2 : -rv2;
+                        amax2 = amax2 < av2 ? av2 : amax2;
+                        sum2 = sum2 + rv2;
+                    }
+                }
+            }
+        }
+        buf ^= 1;
+    }
+    if (MODE == ZM_RESIDUAL || MODE == ZM_RESIDUAL2) {
+        const unsigned pb = blockIdx.x;
+        const int per = (MODE == ZM_RESIDUAL2) ? 4 : 2;
+        T v = block_reduce(PM_RED_ABS, amax, red);
+        if (tid == 0) { a.partials[per * pb + 0] = v; }
+        v = block_reduce(PM_RED_SUM, sum, red);
+        if (tid == 0) { a.partials[per * pb + 1] = v; }
+        if (MODE == ZM_RESIDUAL2) {
+            v = block_reduce(PM_RED_ABS, amax2, red);
+            if (tid == 0) { a.partials[per * pb + 2] = v; }
+            v = block_reduce(PM_RED_SUM, sum2, red);
+            if (tid == 0) { a.partials[per * pb + 3] = v; }
+        }
+    }
+}
+
+// Deterministic second stage over per-block {amax, sum} pairs: one block, a
+// fixed strided split and a fixed tree.
+template<typename T>
+__global__ __launch_bounds__(1024) void k_finish_pairs(const T* __restrict__ partials, int nblocks, int per,
+                                                       T* __restrict__ out)
+{
+    __shared__ T red[16];
+    for (int q = 0; q < per; q++) {
+        const int op = (q % 2 == 0) ? PM_RED_ABS : PM_RED_SUM;
+        T v = 0;
+        for (int i = threadIdx.x; i < nblocks; i += blockDim.x) { v = red_combine(op, v, partials[per * i + q]); }
+        v = block_reduce(op, v, red);
+        if (threadIdx.x == 0) { out[q] = v; }
+        __syncthreads();
+    }
+}
+
+// r = f - r followed by {absMax, sum} partials (generic residual path).
+template<typename T>
+__global__ __launch_bounds__(256) void k_f_minus(const T* __restrict__ f, T* __restrict__ r, long n,
+                                                 T* __restrict__ partials)
+{
+    __shared__ T red[16];
+    T amax = 0, sum = 0;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const T v = f[i] - r[i];
+        r[i] = v;
+        const T av = v > 0 ? v : -v;
+        amax = amax < av ? av : amax;
+        sum = sum + v;
+    }
+    T v = block_reduce(PM_RED_ABS, amax, red);
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x] = v; }
+    v = block_reduce(PM_RED_SUM, sum, red);
+    if (threadIdx.x == 0) { partials[2 * blockIdx.x + 1] = v; }
+}
+
+// ------------------------------------------------------------ host side
+static bool covers(const pm_stencil* st, const Box3& sbox, const Box3& dbox, const Box3& ibox)
+{
+    // destination: p*dR + dS over idx box; source: p*sR + span
+    for (int d = 0; d < 3; d++) {
+        const long dlo = (long)ibox.lo[d] * st->dr[d] + st->ds[d];
+        const long dhi = (long)ibox.hi[d] * st->dr[d] + st->ds[d];
+        if (dlo < dbox.lo[d] || dhi > dbox.hi[d]) { return false; }
+        const long slo = (long)ibox.lo[d] * st->sr[d] + st->span_lo[d];
+        const long shi = (long)ibox.hi[d] * st->sr[d] + st->span_hi[d];
+        if (slo < sbox.lo[d] || shi > sbox.hi[d]) { return false; }
+    }
+    return true;
+}
+
+template<typename T>
+static int launch_generic(const pm_stencil* st, const void* src, const Box3& sbox, void* dst, const Box3& dbox,
+                          const Box3& ibox, int ncomp, int replace, T scale, int ncoef, hipStream_t s)
+{
+    GenArgs<T> a;
+    a.src = (const T*)src;
+    a.dst = (T*)dst;
+    a.sbox = sbox;
+    a.dbox = dbox;
+    a.ibox = ibox;
+    a.coefs = (const T*)st->d_coefs;
+    a.offs = st->d_offs;
+    a.ncoef = ncoef;
+    a.ncomp = ncomp;
+    a.replace = replace;
+    a.apply_scale = (scale != (T)1);
+    a.scale = scale;
+    for (int d = 0; d < 3; d++) { a.sr[d] = st->sr[d]; a.dr[d] = st->dr[d]; a.ds[d] = st->ds[d]; }
+    const long n = box_size(ibox);
+    long blocks = (n + 255) / 256;
+    blocks = std::min(blocks, 1L << 20);
+    hipLaunchKernelGGL(k_stencil_generic<T>, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+struct ZPlan {
+    int ntx, nty, ntz, tz;
+    unsigned blocks;
+};
+
+static ZPlan zplan(const Box3& ibox, int RY)
+{
+    ZPlan p;
+    const int nx = (int)box_nx(ibox), ny = (int)box_ny(ibox), nz = (int)box_nz(ibox);
+    p.ntx = (nx + 63) / 64;
+    p.nty = (ny + 4 * RY - 1) / (4 * RY);
+    // enough blocks to fill 256 CUs several times over, but long z runs
+    int tz = nz;
+    while (tz > 16 && (long)p.ntx * p.nty * ((nz + tz - 1) / tz) < 4096) { tz = (tz + 1) / 2; }
+    p.tz = tz;
+    p.ntz = (nz + tz - 1) / tz;
+    p.blocks = (unsigned)((long)p.ntx * p.nty * p.ntz);
+    return p;
+}
+
+template<typename T> static int zmarch_ry() { return sizeof(T) == 8 ? 2 : 4; }
+
+template<typename T, int SHAPE, int MODE>
+static int launch_zmarch(ZArgs<T>& a, const Box3& ibox, hipStream_t s, unsigned* nblocks)
+{
+    constexpr int RY = sizeof(T) == 8 ? 2 : 4;
+    ZPlan p = zplan(ibox, RY);
+    a.ntx = p.ntx;
+    a.nty = p.nty;
+    a.tz = p.tz;
+    a.ibox = ibox;
+    *nblocks = p.blocks;
+    hipLaunchKernelGGL((k_zmarch<T, SHAPE, MODE, RY>), dim3(p.blocks), dim3(256), 0, s, a);
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+template<typename T>
+static void scaled_coefs(const pm_stencil* st, double scale, T* out)
+{
+    const T sc = (T)scale;
+    for (int j = 0; j < st->ncoef; j++) {
+        T c = (T)st->coefs[j];
+        if (sc != (T)1) { c = c * sc; }
+        out[j] = c;
+    }
+}
+
+template<typename T>
+static int apply_t(const pm_stencil* st, const void* src, const Box3& sbox, void* dst, const Box3& dbox,
+                   const Box3& ibox, int ncomp, int replace, double scale, hipStream_t s)
+{
+    const T sc = (T)scale;
+    if (sc == (T)0) {
+        if (!replace) { return PM_OK; }
+        return launch_generic<T>(st, src, sbox, dst, dbox, ibox, ncomp, 1, sc, 0, s);
+    }
+    if ((st->kind == 7 || st->kind == 27) && ncomp >= 1) {
+        const long ssz = box_size(sbox), dsz = box_size(dbox);
+        for (int c = 0; c < ncomp; c++) {
+            ZArgs<T> a;
+            std::memset(&a, 0, sizeof(a));
+            a.src = (const T*)src + c * ssz;
+            a.dst = (T*)dst + c * dsz;
+            a.sbox = sbox;
+            a.dbox = dbox;
+            scaled_coefs<T>(st, scale, a.c);
+            unsigned nb;
+            int rc;
+            if (st->kind == 7) {
+                rc = replace ? launch_zmarch<T, 7, ZM_REPLACE>(a, ibox, s, &nb) : launch_zmarch<T, 7, ZM_ACCUM>(a, ibox, s, &nb);
+            } else {
+                rc = replace ? launch_zmarch<T, 27, ZM_REPLACE>(a, ibox, s, &nb) : launch_zmarch<T, 27, ZM_ACCUM>(a, ibox, s, &nb);
+            }
+            if (rc) { return rc; }
+        }
+        return PM_OK;
+    }
+    return launch_generic<T>(st, src, sbox, dst, dbox, ibox, ncomp, replace, sc, st->ncoef, s);
+}
+
+template<typename T>
+static int residual_t(const pm_stencil* st, const void* u, const Box3& ubox, const void* f, void* r,
+                      const Box3& rbox, double scale, void* red_out, hipStream_t s)
+{
+    const T sc = (T)scale;
+    if ((st->kind == 7 || st->kind == 27) && sc != (T)0) {
+        ZArgs<T> a;
+        std::memset(&a, 0, sizeof(a));
+        a.src = (const T*)u;
+        a.dst = (T*)r;
+        a.f = (const T*)f;
+        a.sbox = ubox;
+        a.dbox = rbox;
+        scaled_coefs<T>(st, scale, a.c);
+        ZPlan p = zplan(rbox, zmarch_ry<T>());
+        void* ws = nullptr;
+        int rc = workspace(s, sizeof(T) * 2 * (size_t)p.blocks, &ws);
+        if (rc) { return rc; }
+        a.partials = (T*)ws;
+        unsigned nb;
+        rc = st->kind == 7 ? launch_zmarch<T, 7, ZM_RESIDUAL>(a, rbox, s, &nb)
+                           : launch_zmarch<T, 27, ZM_RESIDUAL>(a, rbox, s, &nb);
+        if (rc) { return rc; }
+        if (red_out) {
+            hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)nb, 2, (T*)red_out);
+            PM_CHECK_LAUNCH();
+        }
+        return PM_OK;
+    }
+    // generic path: Lu |= L(u) into r, then r = f - r with partials
+    int rc = apply_t<T>(st, u, ubox, r, rbox, rbox, 1, 1, scale, s);
+    if (rc) { return rc; }
+    const long n = box_size(rbox);
+    long blocks = std::min((n + 255) / 256, 2048L);
+    if (blocks < 1) { blocks = 1; }
+    void* ws = nullptr;
+    rc = workspace(s, sizeof(T) * 2 * (size_t)blocks, &ws);
+    if (rc) { return rc; }
+    hipLaunchKernelGGL(k_f_minus<T>, dim3((unsigned)blocks), dim3(256), 0, s, (const T*)f, (T*)r, n, (T*)ws);
+    PM_CHECK_LAUNCH();
+    if (red_out) {
+        hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)blocks, 2, (T*)red_out);
+        PM_CHECK_LAUNCH();
+    }
+    return PM_OK;
+}
+
+template<typename T>
+static int residual2_t(const pm_stencil* st1, const pm_stencil* st2, const void* u, const Box3& ubox,
+                       const void* f, void* r1, void* r2, const Box3& rbox, double scale1, double scale2,
+                       void* red_out, hipStream_t s)
+{
+    ZArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.src = (const T*)u;
+    a.dst = (T*)r1;
+    a.dst2 = (T*)r2;
+    a.f = (const T*)f;
+    a.sbox = ubox;
+    a.dbox = rbox;
+    scaled_coefs<T>(st1, scale1, a.c);
+    scaled_coefs<T>(st2, scale2, a.c2);
+    ZPlan p = zplan(rbox, zmarch_ry<T>());
+    void* ws = nullptr;
+    int rc = workspace(s, sizeof(T) * 4 * (size_t)p.blocks, &ws);
+    if (rc) { return rc; }
+    a.partials = (T*)ws;
+    unsigned nb;
+    rc = (st1->kind == 7) ? launch_zmarch<T, 7, ZM_RESIDUAL2>(a, rbox, s, &nb)
+                          : launch_zmarch<T, 27, ZM_RESIDUAL2>(a, rbox, s, &nb);
+    if (rc) { return rc; }
+    if (red_out) {
+        hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)nb, 4, (T*)red_out);
+        PM_CHECK_LAUNCH();
+    }
+    return PM_OK;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_stencil_create(pm_stencil** out, int dtype, int dim, int ncoef, const void* coefs_host,
+                      const int* offsets_host, const int src_ratio[3], const int dst_ratio[3],
+                      const int dst_shift[3])
+{
+    PM_REQUIRE(out, "pm_stencil_create: null out");
+    *out = nullptr;
+    PM_REQUIRE(dtype == PM_F32 || dtype == PM_F64, "pm_stencil_create: bad dtype %d", dtype);
+    PM_REQUIRE(dim == 2 || dim == 3, "pm_stencil_create: dim must be 2 or 3");
+    PM_REQUIRE(ncoef >= 0 && ncoef <= 343, "pm_stencil_create: ncoef %d outside [0, 343]", ncoef);
+    PM_REQUIRE(ncoef == 0 || (coefs_host && offsets_host), "pm_stencil_create: null coefficient data");
+    pm_stencil* st = new pm_stencil();
+    st->dtype = dtype;
+    st->dim = dim;
+    st->ncoef = ncoef;
+    st->coefs.resize(ncoef);
+    st->offs.assign(offsets_host, offsets_host + 3 * ncoef);
+    for (int j = 0; j < ncoef; j++) {
+        st->coefs[j] = dtype == PM_F64 ? ((const double*)coefs_host)[j] : (double)((const float*)coefs_host)[j];
+    }
+    for (int d = 0; d < 3; d++) {
+        st->sr[d] = src_ratio ? src_ratio[d] : 1;
+        st->dr[d] = dst_ratio ? dst_ratio[d] : 1;
+        st->ds[d] = dst_shift ? dst_shift[d] : 0;
+        if (d >= dim) {
+            st->sr[d] = st->dr[d] = 1;
+            st->ds[d] = 0;
+        }
+        if (st->sr[d] < 1 || st->dr[d] < 1) {
+            delete st;
+            set_error("pm_stencil_create: refinement ratios must be positive");
+            return PM_ERR_INVALID;
+        }
+        st->span_lo[d] = 0;
+        st->span_hi[d] = 0;
+    }
+    // the span is the bounding box of the offsets (m_span &= offset, Proto_StencilImplem.H:267)
+    for (int j = 0; j < ncoef; j++) {
+        for (int d = 0; d < 3; d++) {
+            const int v = st->offs[3 * j + d];
+            if (d >= dim && v != 0) {
+                delete st;
+                set_error("pm_stencil_create: offset %d has a component beyond dim", j);
+                return PM_ERR_INVALID;
+            }
+            st->span_lo[d] = (j == 0) ? v : std::min(v, st->span_lo[d]);
+            st->span_hi[d] = (j == 0) ? v : std::max(v, st->span_hi[d]);
+        }
+    }
+    st->kind = detect_kind(st);
+    st->d_coefs = nullptr;
+    st->d_offs = nullptr;
+    if (ncoef > 0) {
+        const size_t es = dtype_size(dtype);
+        std::vector<char> hc(es * ncoef);
+        for (int j = 0; j < ncoef; j++) {
+            if (dtype == PM_F64) { ((double*)hc.data())[j] = st->coefs[j]; }
+            else { ((float*)hc.data())[j] = (float)st->coefs[j]; }
+        }
+        if (hipMalloc(&st->d_coefs, es * ncoef) != hipSuccess ||
+            hipMalloc((void**)&st->d_offs, sizeof(int) * 3 * ncoef) != hipSuccess) {
+            hipFree(st->d_coefs);
+            delete st;
+            set_error("pm_stencil_create: device allocation failed");
+            return PM_ERR_NOMEM;
+        }
+        if (hipMemcpy(st->d_coefs, hc.data(), es * ncoef, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(st->d_offs, st->offs.data(), sizeof(int) * 3 * ncoef, hipMemcpyHostToDevice) != hipSuccess) {
+            hipFree(st->d_coefs);
+            hipFree(st->d_offs);
+            delete st;
+            set_error("pm_stencil_create: upload failed");
+            return PM_ERR_HIP;
+        }
+    }
+    *out = st;
+    return PM_OK;
+}
+
+int pm_stencil_destroy(pm_stencil* st)
+{
+    if (!st) { return PM_OK; }
+    if (st->d_coefs) { hipFree(st->d_coefs); }
+    if (st->d_offs) { hipFree(st->d_offs); }
+    delete st;
+    return PM_OK;
+}
+
+int pm_stencil_kind(const pm_stencil* st, int* kind)
+{
+    PM_REQUIRE(st && kind, "pm_stencil_kind: null argument");
+    *kind = st->kind;
+    return PM_OK;
+}
+
+int pm_stencil_apply(const pm_stencil* st, const void* src, const pm_box* src_box, void* dst,
+                     const pm_box* dst_box, const pm_box* idx_box, int ncomp, int replace, double scale,
+                     pm_stream_t stream)
+{
+    PM_REQUIRE(st && src_box && dst_box && idx_box, "pm_stencil_apply: null argument");
+    const Box3 sb = box3(src_box), db = box3(dst_box), ib = box3(idx_box);
+    if (st->ncoef == 0 || box_empty(ib)) { return PM_OK; }
+    PM_REQUIRE(src && dst, "pm_stencil_apply: null data pointer");
+    PM_REQUIRE(ncomp >= 1, "pm_stencil_apply: ncomp must be >= 1");
+    PM_REQUIRE(covers(st, sb, db, ib),
+               "pm_stencil_apply: index box [%d,%d,%d]-[%d,%d,%d] reads or writes outside the source/destination boxes",
+               ib.lo[0], ib.lo[1], ib.lo[2], ib.hi[0], ib.hi[1], ib.hi[2]);
+    if (st->dtype == PM_F64) { return apply_t<double>(st, src, sb, dst, db, ib, ncomp, replace, scale, S(stream)); }
+    return apply_t<float>(st, src, sb, dst, db, ib, ncomp, replace, scale, S(stream));
+}
+
+int pm_stencil_residual(const pm_stencil* st, const void* u, const pm_box* u_box, const void* f, void* r,
+                        const pm_box* r_box, double scale, void* red_out, pm_stream_t stream)
+{
+    PM_REQUIRE(st && u_box && r_box && u && f && r, "pm_stencil_residual: null argument");
+    const Box3 ub = box3(u_box), rb = box3(r_box);
+    PM_REQUIRE(!box_empty(rb), "pm_stencil_residual: empty box");
+    PM_REQUIRE(st->ncoef > 0, "pm_stencil_residual: empty stencil");
+    for (int d = 0; d < 3; d++) {
+        PM_REQUIRE(st->sr[d] == 1 && st->dr[d] == 1 && st->ds[d] == 0,
+                   "pm_stencil_residual: stencil must have unit ratios and no shift");
+    }
+    PM_REQUIRE(covers(st, ub, rb, rb), "pm_stencil_residual: u does not cover the stencil's domain of r_box");
+    if (st->dtype == PM_F64) { return residual_t<double>(st, u, ub, f, r, rb, scale, red_out, S(stream)); }
+    return residual_t<float>(st, u, ub, f, r, rb, scale, red_out, S(stream));
+}
+
+int pm_stencil_residual2(const pm_stencil* st1, const pm_stencil* st2, const void* u, const pm_box* u_box,
+                         const void* f, void* r1, void* r2, const pm_box* r_box, double scale1, double scale2,
+                         void* red_out, pm_stream_t stream)
+{
+    PM_REQUIRE(st1 && st2 && u && u_box && f && r1 && r2 && r_box, "pm_stencil_residual2: null argument");
+    PM_REQUIRE(st1->dtype == st2->dtype, "pm_stencil_residual2: dtype mismatch");
+    PM_REQUIRE((st1->kind == 7 || st1->kind == 27) && st2->kind == 27,
+               "pm_stencil_residual2: needs a 7- or 27-point Laplacian shape and a 27-point one");
+    const Box3 ub = box3(u_box), rb = box3(r_box);
+    PM_REQUIRE(!box_empty(rb), "pm_stencil_residual2: empty box");
+    PM_REQUIRE(covers(st1, ub, rb, rb) && covers(st2, ub, rb, rb),
+               "pm_stencil_residual2: u does not cover the stencils' domain of r_box");
+    PM_REQUIRE(scale1 != 0.0 && scale2 != 0.0, "pm_stencil_residual2: zero scale");
+    if (st1->dtype == PM_F64) {
+        return residual2_t<double>(st1, st2, u, ub, f, r1, r2, rb, scale1, scale2, red_out, S(stream));
+    }
+    return residual2_t<float>(st1, st2, u, ub, f, r1, r2, rb, scale1, scale2, red_out, S(stream));
+}
+
+}  // extern "C"

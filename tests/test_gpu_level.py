@@ -1,0 +1,270 @@
+"""GPU parity of the level layer and the multigrid V-cycle through the C ABI:
+periodic exchange against Proto's own LevelBoxData::exchange dumps, GSRB and
+whole solves against Proto's residual histories (bitwise), the FP32-
+preconditioned refinement against the FP64 solve, graph replays against eager
+runs, and config 3 at its full 512^3 size through size-independent checks."""
+import ctypes
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import proto_amd as pa
+import util
+from gpu_util import dev, empty, host, say, vp, zeros
+
+pytestmark = pytest.mark.gpu
+
+G1 = pa.i3([1, 1, 1])
+G0 = pa.i3([0, 0, 0])
+
+
+class Layout:
+    def __init__(self, n, bs, lows=None, dim=3):
+        self.n, self.bs, self.dim = n, bs, dim
+        nb = n // bs
+        if lows is None:
+            lows = [(x * bs, y * bs, z * bs) for z in range(nb) for y in range(nb) for x in range(nb)]
+        self.lows = np.array(lows, dtype=np.int32).reshape(-1, 3)
+        h = ctypes.c_void_p()
+        pa.call("pm_layout_create", ctypes.byref(h), dim, pa.i3([n] * 3), pa.i3([bs] * 3), pa.i3([1] * 3),
+                len(self.lows), self.lows.reshape(-1).ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        self.h = h
+
+    def pack(self, glob, g):
+        """global x-fastest n^3 array -> level storage with g ghosts (ghosts = periodic images)."""
+        n, bs = self.n, self.bs
+        G = glob.reshape(n, n, n)
+        out = []
+        for lo in self.lows:
+            iz = np.arange(lo[2] - g, lo[2] + bs + g) % n
+            iy = np.arange(lo[1] - g, lo[1] + bs + g) % n
+            ix = np.arange(lo[0] - g, lo[0] + bs + g) % n
+            out.append(G[iz][:, iy][:, :, ix].reshape(-1))
+        return np.concatenate(out)
+
+    def unpack(self, data, g):
+        n, bs = self.n, self.bs
+        e = bs + 2 * g
+        G = np.zeros((n, n, n), dtype=data.dtype)
+        for b, lo in enumerate(self.lows):
+            blk = data[b * e ** 3:(b + 1) * e ** 3].reshape(e, e, e)
+            G[lo[2]:lo[2] + bs, lo[1]:lo[1] + bs, lo[0]:lo[0] + bs] = blk[g:g + bs, g:g + bs, g:g + bs]
+        return G.reshape(-1)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            pa.lib().pm_layout_destroy(self.h)
+
+
+def test_exchange_matches_proto_dumps():
+    z = np.load(os.path.join(util.GOLDEN, "exchange.npz"), allow_pickle=False)
+    for key in ("n8_bs4_g1", "n8_bs4_g2", "n12_bs4_g1"):
+        parts = key.split("_")
+        n, bs, g = int(parts[0][1:]), int(parts[1][2:]), int(parts[2][1:])
+        say("exchange", key)
+        lows = z[key + ":lows"]  # Proto's (Morton) layout order
+        L = Layout(n, bs, lows)
+        glob = z[key + ":global"]
+        e = bs + 2 * g
+        data = np.full(len(lows) * e ** 3, -99.0)
+        for b, lo in enumerate(lows):
+            blk = data[b * e ** 3:(b + 1) * e ** 3].reshape(e, e, e)
+            blk[g:g + bs, g:g + bs, g:g + bs] = glob.reshape(n, n, n)[lo[2]:lo[2] + bs, lo[1]:lo[1] + bs,
+                                                                        lo[0]:lo[0] + bs]
+        d = dev(data)
+        pa.call("pm_level_exchange", L.h, pa.PM_F64, vp(d), pa.i3([g] * 3), 1, None)
+        got = host(d).reshape(len(lows), e, e, e)
+        assert got.tobytes() == z[key + ":boxes"].tobytes(), key
+
+
+def test_exchange_with_hole_leaves_unowned_ghosts():
+    # tests/LevelBoxDataTests.cpp:6-21 omits patch (0,0,0): ghosts whose source is the hole keep their value
+    n, bs, g = 8, 4, 1
+    lows = [(x * bs, y * bs, z * bs) for z in range(2) for y in range(2) for x in range(2)][1:]
+    L = Layout(n, bs, lows)
+    glob = util.uniform(3, 3, n ** 3)
+    e = bs + 2
+    data = np.full(len(lows) * e ** 3, -7.0)
+    for b, lo in enumerate(lows):
+        blk = data[b * e ** 3:(b + 1) * e ** 3].reshape(e, e, e)
+        blk[1:-1, 1:-1, 1:-1] = glob.reshape(n, n, n)[lo[2]:lo[2] + bs, lo[1]:lo[1] + bs, lo[0]:lo[0] + bs]
+    d = dev(data)
+    pa.call("pm_level_exchange", L.h, pa.PM_F64, vp(d), G1, 1, None)
+    got = host(d).reshape(len(lows), e, e, e)
+    full = L.pack(glob, 1).reshape(len(lows), e, e, e)
+    for b, lo in enumerate(lows):
+        for idx in np.ndindex(e, e, e):
+            p = np.array([lo[0] + idx[2] - 1, lo[1] + idx[1] - 1, lo[2] + idx[0] - 1]) % n
+            in_hole = np.all(p < bs)
+            expect = -7.0 if in_hole else full[b][idx]
+            assert got[b][idx] == expect
+
+
+def _solve(n, bs, nlev, relax, linear, precision=0, maxiter=20, use_graph=False, tol=1e-10, rhs=None, stream=None):
+    L = Layout(n, bs)
+    opt = pa.MgOptions()
+    pa.lib().pm_mg_default_options(ctypes.byref(opt))
+    opt.relax, opt.pre, opt.post, opt.bottom, opt.interp = relax, 2, 2, 10, linear
+    opt.precision, opt.max_iter, opt.tol, opt.use_graph = precision, maxiter, tol, int(use_graph)
+    mg = ctypes.c_void_p()
+    pa.call("pm_mg_create", ctypes.byref(mg), L.h, nlev, 1.0 / n, ctypes.byref(opt))
+    if rhs is None:
+        rhs = util.gaussian_rhs(n, 6, 2)
+    phi_d = dev(L.pack(np.zeros(n ** 3), 1))
+    rhs_d = dev(L.pack(rhs, 0))
+    hist = np.zeros(maxiter + 1)
+    it = ctypes.c_int()
+    rc = pa.lib().pm_mg_solve(mg, vp(phi_d), vp(rhs_d), hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              ctypes.byref(it), stream)
+    phi = L.unpack(host(phi_d), 1)
+    pa.lib().pm_mg_destroy(mg)
+    return rc, it.value, hist[:it.value + 1], phi
+
+
+@pytest.mark.parametrize("key,relax,linear", [("gsrb_const", 0, 0), ("gsrb_linear", 0, 1), ("jacobi", 1, 0)])
+def test_multigrid_history_matches_proto(key, relax, linear):
+    z = np.load(os.path.join(util.GOLDEN, "multigrid.npz"), allow_pickle=False)
+    meta = json.load(open(os.path.join(util.GOLDEN, "multigrid.json")))
+    say("mg golden", key)
+    rc, it, hist, phi = _solve(32, 16, 4, relax, linear)
+    assert it == meta[key]["iters"]
+    assert hist.tobytes() == z[key + ":hist"].tobytes()
+    assert hashlib.sha256(phi.tobytes()).hexdigest() == meta[key]["phi_sha256"]
+    assert rc == (0 if not linear else -5)  # Proto's Linear does not reach 1e-10 in 20 cycles
+
+
+def test_gsrb_relax_matches_proto():
+    z = np.load(os.path.join(util.GOLDEN, "multigrid.npz"), allow_pickle=False)
+    n, bs = 32, 16
+    L = Layout(n, bs)
+    phi = dev(L.pack(util.uniform(70, 0, n ** 3), 1))
+    rhs = dev(L.pack(util.uniform(70, 1, n ** 3), 0))
+    for it in range(3):
+        for color in (0, 1):
+            pa.call("pm_level_exchange", L.h, pa.PM_F64, vp(phi), G1, 1, None)
+            pa.call("pm_level_gsrb", L.h, pa.PM_F64, vp(phi), G1, vp(rhs), G0, 1.0 / n, color, None)
+    assert L.unpack(host(phi), 1).tobytes() == z["relax:phi"].tobytes()
+
+
+@pytest.mark.parametrize("n,bs,nlev,relax,linear", [(64, 32, 3, 0, 0), (64, 16, 5, 0, 1), (48, 16, 3, 1, 0),
+                                                    (128, 64, 4, 0, 0)])
+def test_multigrid_vs_oracle(n, bs, nlev, relax, linear):
+    say("mg vs oracle", n, bs, nlev, relax, linear)
+    rhs = util.gaussian_rhs(n, 5, 9)
+    maxit = 12
+    rc, it, hist, phi = _solve(n, bs, nlev, relax, linear, maxiter=maxit, rhs=rhs)
+    ophi = np.zeros(n ** 3)
+    ohist = np.zeros(maxit + 2)
+    oit = util.oracle().orc_mg_solve(n, nlev, relax, 2, 2, 10, linear, util.ptr(rhs), util.ptr(ophi), maxit,
+                                     ctypes.c_double(1e-10), util.ptr(ohist))
+    assert it == oit
+    assert hist.tobytes() == ohist[:oit + 1].tobytes()
+    assert phi.tobytes() == ophi.tobytes()
+
+
+def test_mixed_precision_refinement_converges_to_fp64_answer():
+    n, bs, nlev = 64, 32, 3
+    rhs = util.gaussian_rhs(n, 8, 4)
+    say("mg fp64")
+    rc64, it64, h64, phi64 = _solve(n, bs, nlev, 0, 0, precision=0, rhs=rhs)
+    say("mg fp32-ir")
+    rc32, it32, h32, phi32 = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs)
+    assert rc64 == 0 and rc32 == 0
+    assert h32[0] == h64[0]  # same FP64 initial residual, bit for bit
+    assert h32[-1] < 1e-10 * h32[0]
+    assert it32 <= it64 + 2
+    # both solve the same discrete problem: error between them is at the tolerance level
+    err = np.abs(phi32 - phi64).max() / np.abs(phi64).max()
+    assert err < 1e-8, err
+    # an independent FP64 residual of the IR answer (numpy) meets the tolerance
+    P = phi32.reshape(n, n, n)
+    lap = (sum(np.roll(P, s, a) for a in range(3) for s in (1, -1)) - 6 * P) * n * n
+    assert np.abs(lap - rhs.reshape(n, n, n)).max() < 2e-10 * h32[0]
+
+
+def test_graph_replay_equals_eager():
+    n, bs, nlev = 64, 32, 3
+    rhs = util.gaussian_rhs(n, 3, 5)
+    from proto_amd.device import Stream
+    s = Stream()
+    for precision in (0, 1):
+        say("graph precision", precision)
+        a = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, stream=s.h)
+        b = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, use_graph=True, stream=s.h)
+        assert a[0] == b[0] == 0
+        assert a[2].tobytes() == b[2].tobytes()
+        assert a[3].tobytes() == b[3].tobytes()
+
+
+def test_level_copy_and_prolong_restrict_invariants():
+    n = 16
+    F = Layout(n, 8)
+    C = Layout(n // 2, 4)
+    Cs = Layout(n // 2, 8)  # single box
+    # copyTo between layouts of the same domain
+    glob = util.uniform(8, 1, (n // 2) ** 3)
+    a = dev(C.pack(glob, 1))
+    b = dev(Cs.pack(np.zeros((n // 2) ** 3), 0))
+    pa.call("pm_level_copy", C.h, vp(a), G1, Cs.h, vp(b), G0, pa.PM_F64, 1, None)
+    assert Cs.unpack(host(b), 0).tobytes() == glob.tobytes()
+    # prolongation of a constant adds that constant (Multigrid.cpp:226-231)
+    for linear in (0, 1):
+        ones = dev(C.pack(np.ones((n // 2) ** 3), 1))
+        phi = dev(F.pack(np.zeros(n ** 3), 1))
+        pa.call("pm_level_prolong_add", F.h, C.h, pa.PM_F64, vp(ones), G1, vp(phi), G1, linear, None)
+        assert np.all(F.unpack(host(phi), 1) == 1.0)
+    # prolongation vs the oracle for random coarse data
+    for linear in (0, 1):
+        d = util.uniform(12, linear, (n // 2) ** 3)
+        p0 = util.uniform(13, linear, n ** 3)
+        dd = dev(C.pack(d, 1))
+        pd = dev(F.pack(p0, 1))
+        pa.call("pm_level_prolong_add", F.h, C.h, pa.PM_F64, vp(dd), G1, vp(pd), G1, linear, None)
+        ref = p0.copy()
+        util.oracle().orc_fine_interp(n, linear, util.ptr(d), util.ptr(ref))
+        assert F.unpack(host(pd), 1).tobytes() == ref.tobytes()
+    # restricted residual vs the oracle
+    phi = util.uniform(14, 0, n ** 3)
+    rhs = util.uniform(14, 1, n ** 3)
+    pd, rd = dev(F.pack(phi, 1)), dev(F.pack(rhs, 0))
+    rc = dev(Cs.pack(np.zeros((n // 2) ** 3), 0))
+    pa.call("pm_level_restrict_residual", F.h, Cs.h, pa.PM_F64, vp(pd), G1, vp(rd), G0, 1.0 / n, vp(rc), G0, None)
+    ref = np.zeros((n // 2) ** 3)
+    util.oracle().orc_coarse_residual(n, 1.0 / n, util.ptr(rhs), util.ptr(phi), util.ptr(ref))
+    assert Cs.unpack(host(rc), 0).tobytes() == ref.tobytes()
+    # resnorm and the fused residual give the same max norm, equal to the oracle's
+    out = zeros(1)
+    pa.call("pm_level_resnorm", F.h, pa.PM_F64, vp(pd), G1, vp(rd), G0, 1.0 / n, vp(out), 0, None)
+    want = util.oracle().orc_resnorm(n, 1.0 / n, util.ptr(rhs), util.ptr(phi))
+    assert host(out)[0] == want
+    r32 = empty(len(F.lows) * 8 ** 3, np.float32)
+    out2 = zeros(1)
+    pa.call("pm_level_residual", F.h, pa.PM_F64, vp(pd), G1, vp(rd), G0, 1.0 / n, pa.PM_F32, vp(r32), G0, vp(out2),
+            None)
+    assert host(out2)[0] == want
+
+
+def test_config3_full_size_512():
+    """Config 3: periodic 512^3, 64^3 boxes (512), 16-Gaussian RHS minus its
+    mean, GSRB V(2,2) down to 8^3, relative residual 1e-10, plain FP64 and the
+    FP32-preconditioned refinement: both converge, start from the same FP64
+    residual, and agree to the tolerance level."""
+    n, bs, nlev = 512, 64, 6
+    say("config3 rhs")
+    rhs = util.gaussian_rhs(n, 16, 2)
+    say("config3 fp64 solve")
+    rc64, it64, h64, phi64 = _solve(n, bs, nlev, 0, 0, precision=0, rhs=rhs, maxiter=30)
+    say("config3 fp64 iters", it64, h64[-1] / h64[0])
+    assert rc64 == 0 and h64[-1] < 1e-10 * h64[0]
+    assert np.all(np.diff(h64) < 0)
+    say("config3 fp32-ir solve")
+    rc32, it32, h32, phi32 = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs, maxiter=30)
+    say("config3 ir iters", it32, h32[-1] / h32[0])
+    assert rc32 == 0 and h32[-1] < 1e-10 * h32[0]
+    assert h32[0] == h64[0]
+    err = np.abs(phi32 - phi64).max() / np.abs(phi64).max()
+    say("config3 rel err", err)
+    assert err < 1e-8

@@ -1,0 +1,292 @@
+"""GPU parity of Stencil apply, the fused residual and Reduction, through the
+C ABI, against Proto's golden vectors and the oracle.  Integer-exact semantics
+(bitwise) for the stencil arithmetic; Sum within n*eps*sum|x| of the
+sequential order, as the tolerance states below."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import proto_amd as pa
+import util
+from gpu_util import box, config2_inputs, dev, empty, host, say, vp, zeros
+from test_oracle import merge
+
+pytestmark = pytest.mark.gpu
+
+
+def apply_gpu(st, src, slo, shi, dst, dlo, dhi, blo, bhi, replace, scale, ncomp=1):
+    s_d, d_d = dev(src), dev(dst)
+    pa.call("pm_stencil_apply", st.h, vp(s_d), ctypes.byref(box(slo, shi)), vp(d_d),
+            ctypes.byref(box(dlo, dhi)), ctypes.byref(box(blo, bhi)), ncomp, int(replace), float(scale), None)
+    return host(d_d)
+
+
+def test_golden_apply_cases():
+    z = np.load(os.path.join(util.GOLDEN, "stencil_apply.npz"), allow_pickle=False)
+    keys = sorted({k.split(":")[0] for k in z.files})
+    for k in keys:
+        g = {f.split(":")[1]: z[f] for f in z.files if f.startswith(k + ":")}
+        say("golden case", k)
+        dim = int(g["dim"])
+        c, o = merge(g["coefs"], g["offs"])
+        dt = pa.PM_F64 if c.dtype == np.float64 else pa.PM_F32
+        st = pa.Stencil(c, o, dt, dim, g["sr"], g["dr"], g["ds"])
+        b = g["box"]
+        out = apply_gpu(st, g["src"], g["slo"], g["shi"], g["dst0"].copy(), g["dlo"], g["dhi"], b[:3], b[3:],
+                        bool(g["replace"]), g["scale"].item())
+        assert out.tobytes() == g["dst"].tobytes(), k
+
+
+def test_laplacian_kinds():
+    for name, kind in (("Laplacian", 7), ("Laplacian_27", 27)):
+        c, o, _, _, _ = util.lib_stencil(name)
+        assert pa.Stencil(c, o).kind() == kind
+    c, o, _, _, _ = util.lib_stencil("Laplacian_19")
+    assert pa.Stencil(c, o).kind() == 0
+    c, o, _, _, _ = util.lib_stencil("Laplacian")
+    assert pa.Stencil(c[::-1].copy(), o[::-1].copy()).kind() == 0  # other order -> generic
+
+
+@pytest.mark.parametrize("name,dtype", [("Laplacian", pa.PM_F64), ("Laplacian_27", pa.PM_F64),
+                                        ("Laplacian", pa.PM_F32), ("Laplacian_27", pa.PM_F32),
+                                        ("Laplacian_19", pa.PM_F64)])
+def test_random_boxes_vs_oracle(name, dtype):
+    rng = np.random.default_rng(11)
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    c, o, _, _, _ = util.lib_stencil(name, dtype=npdt)
+    st = pa.Stencil(c, o, dtype)
+    for trial in range(8):
+        g = int(rng.integers(1, 3))
+        lo = rng.integers(-20, 20, size=3)
+        n = rng.integers(1, 90, size=3) if trial else np.array([130, 67, 33])
+        hi = lo + n - 1
+        slo, shi = lo - g, hi + g
+        dlo, dhi = lo - rng.integers(0, 2, size=3), hi + rng.integers(0, 2, size=3)
+        replace = trial % 3 != 2
+        scale = [1.0, 4096.0, -0.75][trial % 3]
+        say(name, dtype, "trial", trial, "box", lo.tolist(), hi.tolist())
+        src = util.uniform(300 + trial, 0, util.box_size(slo, shi)).astype(npdt)
+        dst0 = util.uniform(300 + trial, 1, util.box_size(dlo, dhi)).astype(npdt)
+        ref = dst0.copy()
+        util.oracle_apply(c, o, src, slo, shi, ref, dlo, dhi, lo, hi, replace, scale)
+        out = apply_gpu(st, src, slo, shi, dst0.copy(), dlo, dhi, lo, hi, replace, scale)
+        assert out.tobytes() == ref.tobytes(), (name, trial)
+
+
+def test_multicomponent_and_generic_ratios():
+    # 3 components, AvgDown (srcRatio 2) and a destination-refined stencil
+    c, o, sr, dr, ds = util.lib_stencil("AvgDown", 2)
+    st = pa.Stencil(c, o, pa.PM_F64, 3, sr, dr, ds)
+    slo, shi = [0, 0, 0], [15, 13, 9]
+    dlo, dhi = [0, 0, 0], [7, 6, 4]
+    nc = 3
+    src = util.uniform(5, 0, nc * util.box_size(slo, shi))
+    dst = np.zeros(nc * util.box_size(dlo, dhi))
+    ref = dst.copy()
+    n_s, n_d = util.box_size(slo, shi), util.box_size(dlo, dhi)
+    for k in range(nc):
+        r = ref[k * n_d:(k + 1) * n_d]
+        util.oracle_apply(c, o, src[k * n_s:(k + 1) * n_s].copy(), slo, shi, r, dlo, dhi, dlo, dhi, True, 1.0,
+                          sr, dr, ds)
+        ref[k * n_d:(k + 1) * n_d] = r
+    out = apply_gpu(st, src, slo, shi, dst, dlo, dhi, dlo, dhi, True, 1.0, ncomp=nc)
+    assert out.tobytes() == ref.tobytes()
+
+
+def test_out_of_bounds_request_is_rejected():
+    c, o, _, _, _ = util.lib_stencil("Laplacian")
+    st = pa.Stencil(c, o)
+    src = dev(np.zeros(10 ** 3))
+    dst = dev(np.zeros(8 ** 3))
+    rc = pa.lib().pm_stencil_apply(st.h, vp(src), ctypes.byref(box([0] * 3, [9] * 3)), vp(dst),
+                                   ctypes.byref(box([0] * 3, [7] * 3)), ctypes.byref(box([0] * 3, [7] * 3)),
+                                   1, 1, 1.0, None)
+    assert rc == -1 and b"outside" in pa.lib().pm_last_error()
+
+
+def _sum_bound(r):
+    n = r.size
+    eps = np.finfo(r.dtype).eps / 2
+    return 2 * n * eps * float(np.abs(r.astype(np.float64)).sum())
+
+
+def test_residual_config2_small_golden():
+    z = np.load(os.path.join(util.GOLDEN, "config2_small.npz"), allow_pickle=False)
+    n, g = 32, 2
+    u, f = config2_inputs(n, g)
+    # the device generator reproduces the numpy generator bit for bit
+    assert host(u).tobytes() == util.config2_u(n, g).tobytes()
+    assert host(f).tobytes() == util.uniform(1, 1, n ** 3).tobytes()
+    h = 1.0 / n
+    for name in ("Laplacian", "Laplacian_27"):
+        say("residual", name)
+        c, o, _, _, _ = util.lib_stencil(name)
+        st = pa.Stencil(c, o)
+        r = empty(n ** 3)
+        red = zeros(2)
+        pa.call("pm_stencil_residual", st.h, vp(u), ctypes.byref(box([-g] * 3, [n - 1 + g] * 3)), vp(f), vp(r),
+                ctypes.byref(box([0] * 3, [n - 1] * 3)), 1.0 / (h * h), vp(red), None)
+        rr = host(r)
+        assert rr.tobytes() == z[name + ":r"].tobytes()
+        amax, s = host(red)
+        assert amax == z[name + ":absmax"]
+        # Sum tolerance: |S_gpu - S_seq| <= 2 n (eps/2) sum|r|
+        assert abs(s - z[name + ":sum"]) <= _sum_bound(rr)
+
+
+@pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
+def test_residual2_one_pass_equals_two(dtype):
+    n, g = 40, 1
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    u, f = config2_inputs(n, g, npdt)
+    h = 1.0 / n
+    c7, o7, _, _, _ = util.lib_stencil("Laplacian", dtype=npdt)
+    c27, o27, _, _, _ = util.lib_stencil("Laplacian_27", dtype=npdt)
+    s7, s27 = pa.Stencil(c7, o7, dtype), pa.Stencil(c27, o27, dtype)
+    ub, rb = box([-g] * 3, [n - 1 + g] * 3), box([0] * 3, [n - 1] * 3)
+    r1, r2, a1, a2 = (empty(n ** 3, npdt) for _ in range(4))
+    red = zeros(4, npdt)
+    red1 = zeros(2, npdt)
+    red2 = zeros(2, npdt)
+    sc = 1.0 / (h * h)
+    pa.call("pm_stencil_residual2", s7.h, s27.h, vp(u), ctypes.byref(ub), vp(f), vp(r1), vp(r2), ctypes.byref(rb),
+            sc, sc, vp(red), None)
+    pa.call("pm_stencil_residual", s7.h, vp(u), ctypes.byref(ub), vp(f), vp(a1), ctypes.byref(rb), sc, vp(red1), None)
+    pa.call("pm_stencil_residual", s27.h, vp(u), ctypes.byref(ub), vp(f), vp(a2), ctypes.byref(rb), sc, vp(red2), None)
+    assert host(r1).tobytes() == host(a1).tobytes()
+    assert host(r2).tobytes() == host(a2).tobytes()
+    R, R1, R2 = host(red), host(red1), host(red2)
+    assert R[0] == R1[0] and R[2] == R2[0]
+    # oracle check of r (FP32 included: same order, one rounding per op)
+    uh, fh = host(u), host(f)
+    for cc, oo, rr in ((c7, o7, r1), (c27, o27, r2)):
+        ref = np.zeros(n ** 3, dtype=npdt)
+        fn = util.oracle().orc_residual_f64 if dtype == pa.PM_F64 else util.oracle().orc_residual_f32
+        ct = ctypes.c_double if dtype == pa.PM_F64 else ctypes.c_float
+        fn(len(cc), util.ptr(cc), util.ptr(oo.reshape(-1).copy()), util.ptr(uh), util.ptr(util.i3([-g] * 3)),
+           util.ptr(util.i3([n - 1 + g] * 3)), util.ptr(fh), util.ptr(ref), util.ptr(util.i3([0] * 3)),
+           util.ptr(util.i3([n - 1] * 3)), ct(sc))
+        assert host(rr).tobytes() == ref.tobytes()
+
+
+def test_config2_full_size_512():
+    """Config 2 at its real size: 512^3 interior, 2 ghosts, both operators fused
+    with r = f - L(u); every cell bitwise against the oracle, absMax exact,
+    sum within the stated bound, and a second run bit-identical."""
+    n, g = 512, 2
+    say("config2 512^3 inputs")
+    u, f = config2_inputs(n, g)
+    h = 1.0 / n
+    c7, o7, _, _, _ = util.lib_stencil("Laplacian")
+    c27, o27, _, _, _ = util.lib_stencil("Laplacian_27")
+    s7, s27 = pa.Stencil(c7, o7), pa.Stencil(c27, o27)
+    ub, rb = box([-g] * 3, [n - 1 + g] * 3), box([0] * 3, [n - 1] * 3)
+    r1 = empty(n ** 3)
+    r2 = empty(n ** 3)
+    red = zeros(4)
+    sc = 1.0 / (h * h)
+    say("config2 512^3 fused residual2")
+    pa.call("pm_stencil_residual2", s7.h, s27.h, vp(u), ctypes.byref(ub), vp(f), vp(r1), vp(r2), ctypes.byref(rb),
+            sc, sc, vp(red), None)
+    R = host(red).copy()
+    uh, fh = host(u), host(f)
+    for k, (cc, oo, rr) in enumerate(((c7, o7, r1), (c27, o27, r2))):
+        say("config2 512^3 oracle operator", k)
+        ref = np.zeros(n ** 3)
+        util.oracle().orc_residual_f64(len(cc), util.ptr(cc), util.ptr(oo.reshape(-1).copy()), util.ptr(uh),
+                                       util.ptr(util.i3([-g] * 3)), util.ptr(util.i3([n - 1 + g] * 3)),
+                                       util.ptr(fh), util.ptr(ref), util.ptr(util.i3([0] * 3)),
+                                       util.ptr(util.i3([n - 1] * 3)), ctypes.c_double(sc))
+        got = host(rr)
+        assert got.tobytes() == ref.tobytes()
+        assert R[2 * k] == util.oracle().orc_reduce_f64(2, util.ptr(ref), ref.size)
+        seq = util.oracle().orc_reduce_f64(3, util.ptr(ref), ref.size)
+        assert abs(R[2 * k + 1] - seq) <= _sum_bound(ref)
+        del ref, got
+    red2 = zeros(4)
+    pa.call("pm_stencil_residual2", s7.h, s27.h, vp(u), ctypes.byref(ub), vp(f), vp(r1), vp(r2), ctypes.byref(rb),
+            sc, sc, vp(red2), None)
+    assert host(red2).tobytes() == R.tobytes()
+
+
+@pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
+def test_reductions_vs_oracle(dtype):
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    for n in (1, 7, 1000, 65536, 3000017):
+        x = util.uniform(900 + n, 0, n).astype(npdt)
+        xd = dev(x)
+        out = zeros(1, npdt)
+        for op in range(6):
+            say("reduce", dtype, n, op)
+            pa.call("pm_reduce", op, dtype, vp(xd), n, vp(out), 0, None)
+            got = host(out)[0]
+            f = util.oracle().orc_reduce_f64 if dtype == pa.PM_F64 else util.oracle().orc_reduce_f32
+            want = f(op, util.ptr(x), n)
+            if op in (0, 1, 2):
+                assert got == want, (n, op)
+            else:
+                terms = np.abs(x.astype(np.float64)) if op != 5 else x.astype(np.float64) ** 2
+                bound = 2 * n * np.finfo(npdt).eps * float(terms.sum()) + 1e-300
+                assert abs(float(got) - float(want)) <= bound, (n, op, got, want)
+            # deterministic: same bits twice; combine folds into the old value
+            pa.call("pm_reduce", op, dtype, vp(xd), n, vp(out), 0, None)
+            assert host(out)[0] == got
+    # Max starts at numeric_limits<T>::min() (Proto_ReductionImplem.H:68)
+    neg = dev(-np.ones(100, dtype=npdt))
+    out = zeros(1, npdt)
+    pa.call("pm_reduce", pa.RED_MAX, dtype, vp(neg), 100, vp(out), 0, None)
+    assert host(out)[0] == np.finfo(npdt).tiny
+    pa.call("pm_reduce_init", pa.RED_ABS, dtype, vp(out), None)
+    pa.call("pm_reduce", pa.RED_ABS, dtype, vp(neg), 100, vp(out), 1, None)
+    assert host(out)[0] == 1.0
+
+
+def test_reduce_box_region():
+    lo, hi = [-2, -2, -2], [9, 8, 7]
+    nc = 2
+    x = util.uniform(77, 0, nc * util.box_size(lo, hi))
+    xd = dev(x)
+    out = zeros(1)
+    rlo, rhi = [0, 0, 0], [7, 6, 5]
+    pa.call("pm_reduce_box", pa.RED_ABS, pa.PM_F64, vp(xd), ctypes.byref(box(lo, hi)), ctypes.byref(box(rlo, rhi)),
+            1, vp(out), 0, None)
+    a = x[util.box_size(lo, hi):].reshape(util.box_shape(lo, hi))[2:8, 2:9, 2:10]
+    assert host(out)[0] == np.abs(a).max()
+
+
+def test_boxdata_ops():
+    slo, shi = [0, 0, 0], [9, 9, 9]
+    dlo, dhi = [-3, 1, 2], [12, 14, 16]
+    src = util.uniform(1, 0, util.box_size(slo, shi))
+    dst0 = util.uniform(1, 1, util.box_size(dlo, dhi))
+    S = src.reshape(util.box_shape(slo, shi))
+    D = dst0.reshape(util.box_shape(dlo, dhi)).copy()
+    sd, dd = dev(src), dev(dst0)
+    region = box([1, 2, 3], [8, 7, 6])
+    shift = pa.i3([2, 3, 4])
+    pa.call("pm_boxdata_copy", pa.PM_F64, vp(sd), ctypes.byref(box(slo, shi)), vp(dd), ctypes.byref(box(dlo, dhi)),
+            ctypes.byref(region), shift, 1, None)
+    # dst[p + shift] = src[p], p in region; numpy index = p - lo
+    D[3 + 4 - 2:6 + 4 - 2 + 1, 2 + 3 - 1:7 + 3 - 1 + 1, 1 + 2 + 3:8 + 2 + 3 + 1] = S[3:7, 2:8, 1:9]
+    assert host(dd).tobytes() == D.reshape(-1).tobytes()
+    # increment, binop, scalar on a common box
+    a0 = util.uniform(2, 0, 1000)
+    b0 = util.uniform(2, 1, 1000) + 3.0
+    ad, bd = dev(a0), dev(b0)
+    B = box([0, 0, 0], [9, 9, 9])
+    pa.call("pm_boxdata_increment", pa.PM_F64, vp(ad), ctypes.byref(B), vp(bd), ctypes.byref(B), ctypes.byref(B),
+            0.5, 1, None)
+    ref = a0 + b0 * 0.5
+    assert host(ad).tobytes() == ref.tobytes()
+    for op, fn in ((0, np.add), (1, np.subtract), (2, np.multiply), (3, np.divide)):
+        pa.call("pm_boxdata_binop", op, pa.PM_F64, vp(ad), ctypes.byref(B), vp(bd), ctypes.byref(B),
+                ctypes.byref(B), 1, None)
+        ref = fn(ref, b0)
+        assert host(ad).tobytes() == ref.tobytes()
+        pa.call("pm_boxdata_scalar", op, pa.PM_F64, vp(ad), ctypes.byref(B), ctypes.byref(B), 1.25, 1, None)
+        ref = fn(ref, 1.25)
+        assert host(ad).tobytes() == ref.tobytes()
+    pa.call("pm_boxdata_scalar", 4, pa.PM_F64, vp(ad), ctypes.byref(B), ctypes.byref(B), -2.5, 1, None)
+    assert np.all(host(ad) == -2.5)
