@@ -35,10 +35,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--cpu-planes", type=int, default=128, help="z-planes of the CPU baseline sample")
+    ap.add_argument("--cpu-planes", type=int, default=512, help="z-planes of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--only", default="", help="profile helper: run only this kernel family")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tile configurations)")
     return ap.parse_args()
 
 
@@ -88,12 +89,15 @@ def cpu_baseline(n, g, planes):
     x = (np.arange(n) + 0.5) / n
     s = np.sin(2 * np.pi * x)
     nz = planes
-    noise = util.uniform(1, 0, n * n * nz).reshape(nz, n, n)
+    noise = np.zeros(n * n * nz)
+    util.oracle().orc_uniform_f64(1, 0, noise.size, util.ptr(noise))  # same stream as the device inputs
+    noise = noise.reshape(nz, n, n)
     smooth = (s[None, None, :] * s[None, :, None]) * s[:nz, None, None]
     ui = smooth + 1e-3 * noise
-    u = np.pad(ui, ((g, g), (g, g), (g, g)), mode="wrap")  # slab-local ghost fill
+    u = np.pad(ui, ((g, g), (g, g), (g, g)), mode="wrap")  # periodic ghost fill (of the slab when planes < n)
     u = np.ascontiguousarray(u.reshape(-1))
-    f = util.uniform(1, 1, n * n * nz)
+    f = np.zeros(n * n * nz)
+    util.oracle().orc_uniform_f64(1, 1, f.size, util.ptr(f))
     r = np.zeros(n * n * nz)
     ulo, uhi = util.i3([-g, -g, -g]), util.i3([n - 1 + g, n - 1 + g, nz - 1 + g])
     lo, hi = util.i3([0, 0, 0]), util.i3([n - 1, n - 1, nz - 1])
@@ -118,8 +122,8 @@ def cpu_baseline(n, g, planes):
     dt = time.perf_counter() - t0
     nbytes = 8 * ((n + 2 * g) * (n + 2 * g) * (nz + 2 * g) + 3 * n * n * nz)
     return {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
-            "sample": "config-2 step (7- and 27-point r = f - L(u), absMax, sum) on a %dx%dx%d z-slab, "
-                      "%.2f s single-threaded Release build" % (n, n, nz, dt)}
+            "sample": "config-2 step (7- and 27-point r = f - L(u), absMax, sum) on %dx%dx%d cells, "
+                      "%.2f s, Proto's CPU path single-threaded (Release flags)" % (n, n, nz, dt)}
 
 
 def main():
@@ -160,12 +164,31 @@ def main():
     bytes_f = 8 * n ** 3
     step_bytes = bytes_u + bytes_f + 2 * bytes_f
 
+    # autotuner (outside the timed region): tile configuration per kernel and shape
+    tuned = {}
+
+    def tune(name, st, st2, mode):
+        if args.no_tune:
+            return
+        cfg, ms = pa.TileConfig(), ctypes.c_float()
+        pa.call("pm_stencil_autotune", st.h, st2.h if st2 else None, mode, ctypes.byref(ub), ctypes.byref(rb), 3,
+                ctypes.byref(cfg), ctypes.byref(ms), sp)
+        tuned[name] = dict(cfg.as_dict(), ms=round(ms.value, 4))
+
+    tune("residual2", s7, s27, 3)
+
     if args.only:
-        fns = {"residual2": kernel_only, "step": step}
+        # profiling helper: the fused kernel and a plain 1 GiB device copy (the
+        # runtime's blit kernel, a known byte count to calibrate the counters)
+        cpy = DeviceArray(n ** 3)
+        fns = {"residual2": [kernel_only], "step": [step],
+               "copy": [lambda: pa.call("pm_memcpy", cpy.ptr, r1.ptr, bytes_f, 2, sp)],
+               "profile": [kernel_only, lambda: pa.call("pm_memcpy", cpy.ptr, r1.ptr, bytes_f, 2, sp)]}
         for _ in range(args.warmup + args.steps):
-            fns[args.only]()
+            for fn in fns[args.only]:
+                fn()
         stream.sync()
-        print(json.dumps({"only": args.only, "steps": args.steps}))
+        print(json.dumps({"only": args.only, "steps": args.steps, "copy_bytes": 2 * bytes_f}))
         return
 
     for _ in range(args.warmup):
@@ -205,11 +228,16 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
                      "kernel": "k_zmarch<double,7,3,2> (7+27-point fused residual)", "kernel_ms": round(k_ms, 4)},
         "wall_s": round(wall, 3),
+        "tuned": tuned,
     }
 
     if not args.no_extra:
         extra = {}
         dst = DeviceArray(n ** 3)
+        tune("apply_L7_f64", s7, None, 0)
+        tune("apply_L27_f64", s27, None, 0)
+        tune("residual_L7_f64", s7, None, 2)
+        tune("residual_L27_f64", s27, None, 2)
         apply_bytes = bytes_u + bytes_f
         for name, st in (("apply_L7_f64", s7), ("apply_L27_f64", s27)):
             ms = time_launches(lambda: pa.call("pm_stencil_apply", st.h, u.ptr, ctypes.byref(ub), dst.ptr,
@@ -229,6 +257,8 @@ def main():
         u32, f32 = config2_inputs(n, g, np.float32, sp)
         d32 = DeviceArray(n ** 3, np.float32)
         t7, t27 = stencils(pa.PM_F32)
+        tune("apply_L7_f32", t7, None, 0)
+        tune("apply_L27_f32", t27, None, 0)
         for name, st in (("apply_L7_f32", t7), ("apply_L27_f32", t27)):
             ms = time_launches(lambda: pa.call("pm_stencil_apply", st.h, u32.ptr, ctypes.byref(ub), d32.ptr,
                                                ctypes.byref(rb), ctypes.byref(rb), 1, 1, sc, sp), 10, stream)
@@ -239,15 +269,16 @@ def main():
         u32.free()
         f32.free()
         d32.free()
-        # plain device copy of the same size (achievable-bandwidth probe)
+        # plain device copies of the same size (achievable-bandwidth probes): the
+        # runtime's copy kernel and this library's streaming copy (plain / NT stores)
         a = DeviceArray(n ** 3)
-        box_all = pa.Box.of([0] * 3, [n - 1] * 3)
-        ms = time_launches(lambda: pa.call("pm_boxdata_copy", pa.PM_F64, r1.ptr, ctypes.byref(box_all), a.ptr,
-                                           ctypes.byref(box_all), ctypes.byref(box_all), pa.i3([0, 0, 0]), 1, sp),
-                           10, stream)
-        gbs = 2 * bytes_f / (ms * 1e-3) / 1e9
-        extra["copy_probe_f64"] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
-                                   "bytes": 2 * bytes_f}
+        for name, fn in (("copy_runtime", lambda: pa.call("pm_memcpy", a.ptr, r1.ptr, bytes_f, 2, sp)),
+                         ("copy_stream", lambda: pa.call("pm_copy_device", a.ptr, r1.ptr, bytes_f, 0, sp)),
+                         ("copy_stream_nt", lambda: pa.call("pm_copy_device", a.ptr, r1.ptr, bytes_f, 1, sp))):
+            ms = time_launches(fn, 10, stream)
+            gbs = 2 * bytes_f / (ms * 1e-3) / 1e9
+            extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
+                           "bytes": 2 * bytes_f}
         a.free()
         result["extra"] = extra
 

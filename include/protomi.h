@@ -64,6 +64,11 @@ int pm_free(void* ptr);
  * Asynchronous on `stream`; pageable host memory makes it synchronous. */
 int pm_memcpy(void* dst, const void* src, size_t bytes, int kind, pm_stream_t stream);
 int pm_memset_zero(void* ptr, size_t bytes, pm_stream_t stream);
+/* Device-to-device copy by this library's own streaming kernel (16-byte
+ * accesses, grid-stride; nontemporal != 0 streams the stores past the
+ * caches).  Also the achievable-bandwidth probe of the benchmark.  bytes and
+ * both pointers must be multiples of 16. */
+int pm_copy_device(void* dst, const void* src, size_t bytes, int nontemporal, pm_stream_t stream);
 int pm_stream_create(pm_stream_t* stream);
 int pm_stream_destroy(pm_stream_t stream);
 int pm_stream_sync(pm_stream_t stream);
@@ -119,6 +124,28 @@ int pm_stencil_residual2(const pm_stencil* st1, const pm_stencil* st2, const voi
                          const pm_box* u_box, const void* f, void* r1, void* r2,
                          const pm_box* r_box, double scale1, double scale2, void* red_out,
                          pm_stream_t stream);
+
+/* ------------------------------------------------------------- autotuner */
+/* Tile configuration of the z-marching 7/27-point kernels: rows of the
+ * 64-wide tile per thread (1, 2, 4), planes prefetched (1, 2), non-temporal
+ * result stores (0, 1), z planes per workgroup (0 = automatic). */
+typedef struct pm_tile_config { int rows; int prefetch; int nt_store; int zplanes; } pm_tile_config;
+/* Times every tile configuration for this operator (st2: the second operator
+ * of mode 3, else NULL), mode (0 apply |=, 1 apply +=, 2 fused residual,
+ * 3 fused two-operator residual), dtype and index box on scratch data (the
+ * source box src_box), keeps the fastest in the library's table -- later calls
+ * with the same key use it -- and returns it with its time.  No counterpart in
+ * the reference, which has fixed launch shapes (Proto_StencilImplem.H:860-891). */
+int pm_stencil_autotune(const pm_stencil* st, const pm_stencil* st2, int mode, const pm_box* src_box,
+                        const pm_box* idx_box, int reps, pm_tile_config* best, float* best_ms,
+                        pm_stream_t stream);
+int pm_stencil_set_tile_config(const pm_stencil* st, const pm_stencil* st2, int mode, const pm_box* idx_box,
+                               const pm_tile_config* cfg);
+int pm_stencil_get_tile_config(const pm_stencil* st, const pm_stencil* st2, int mode, const pm_box* idx_box,
+                               pm_tile_config* cfg);
+/* The tuned table as text (one entry per line), to keep tuning across runs. */
+int pm_tune_table_save(const char* path);
+int pm_tune_table_load(const char* path);
 
 /* ------------------------------------------------------------- Reduction */
 /* Reduction<T,OP,DEVICE>::reduce (Proto_ReductionImplem.H:157-192) over n
@@ -238,6 +265,18 @@ int pm_level_residual(const pm_layout* layout, int dtype, const void* phi, const
  * refinement with an FP32 correction. */
 int pm_level_add_correction(const pm_layout* layout, int dtype, void* phi, const int phi_ghost[3], int e_dtype,
                             const void* e, const int e_ghost[3], pm_stream_t stream);
+
+/* LevelBoxData::reduce over the valid cells of every box, component comp
+ * (Proto_LevelBoxDataImplem.H:595-617); one T to out_dev (combine folds). */
+int pm_level_reduce(const pm_layout* layout, int dtype, const void* data, const int ghost[3], int ncomp, int comp,
+                    int op, void* out_dev, int combine, pm_stream_t stream);
+/* data(p) += coef * (tx[p.x] * ty[p.y]) * tz[p.z] on the valid cells of every
+ * box, p in global indices, tables of domain-length (host arrays; tz one 1.0
+ * in 2-D).  Builds separable fields and sums of them on a level (the
+ * forallInPlace_p initialisations of Proto's examples); all-ones tables
+ * with coef = -mean subtract a mean. */
+int pm_level_add_separable(const pm_layout* layout, int dtype, void* data, const int ghost[3], double coef,
+                           const double* tx_host, const double* ty_host, const double* tz_host, pm_stream_t stream);
 
 /* ------------------------------------------------------------- multigrid */
 /* Correction-scheme V-cycle solver on a periodic level

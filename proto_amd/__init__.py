@@ -58,6 +58,15 @@ class MgOptions(ctypes.Structure):
 
 _OP = ctypes.POINTER(MgOptions)
 
+
+class TileConfig(ctypes.Structure):
+    """pm_tile_config."""
+    _fields_ = [("rows", ctypes.c_int), ("prefetch", ctypes.c_int), ("nt_store", ctypes.c_int),
+                ("zplanes", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"rows": self.rows, "prefetch": self.prefetch, "nt_store": self.nt_store, "zplanes": self.zplanes}
+
 SIGNATURES = {
     "pm_last_error": (ctypes.c_char_p, []),
     "pm_version": (ctypes.c_char_p, []),
@@ -103,6 +112,14 @@ SIGNATURES = {
     "pm_level_copy": (_i, [_vp, _vp, _ip, _vp, _vp, _ip, _i, _i, _vp]),
     "pm_level_residual": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _d, _i, _vp, _ip, _vp, _vp]),
     "pm_level_add_correction": (_i, [_vp, _i, _vp, _ip, _i, _vp, _ip, _vp]),
+    "pm_copy_device": (_i, [_vp, _vp, _sz, _i, _vp]),
+    "pm_stencil_autotune": (_i, [_vp, _vp, _i, _BP, _BP, _i, ctypes.POINTER(TileConfig), ctypes.POINTER(ctypes.c_float), _vp]),
+    "pm_stencil_set_tile_config": (_i, [_vp, _vp, _i, _BP, ctypes.POINTER(TileConfig)]),
+    "pm_stencil_get_tile_config": (_i, [_vp, _vp, _i, _BP, ctypes.POINTER(TileConfig)]),
+    "pm_tune_table_save": (_i, [ctypes.c_char_p]),
+    "pm_tune_table_load": (_i, [ctypes.c_char_p]),
+    "pm_level_reduce": (_i, [_vp, _i, _vp, _ip, _i, _i, _i, _vp, _i, _vp]),
+    "pm_level_add_separable": (_i, [_vp, _i, _vp, _ip, _d, _dpp, _dpp, _dpp, _vp]),
     "pm_mg_default_options": (None, [_OP]),
     "pm_mg_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _d, _OP]),
     "pm_mg_destroy": (_i, [_vp]),

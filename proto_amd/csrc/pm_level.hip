@@ -887,3 +887,118 @@ int pm_level_add_correction(const pm_layout* L, int dtype, void* phi, const int 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Level reductions and separable initialisation.
+namespace pm {
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_level_reduce1(LayoutDev L, G3 g, const T* __restrict__ data, int ncomp,
+                                                       int comp, int op, T* __restrict__ partials)
+{
+    __shared__ T red[16];
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    const T* d = data + ((long)b * ncomp + comp) * gvol(L, g);
+    T v = red_init<T>(op);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        v = red_update(op, v, d[rel_index(L, g, x, y, z)]);
+    }
+    v = block_reduce(op, v, red);
+    if (threadIdx.x == 0) { partials[(long)b * gridDim.x + blockIdx.x] = v; }
+}
+
+template<typename T>
+__global__ __launch_bounds__(1024) void k_level_reduce2(const T* __restrict__ partials, long n, int op, T* out,
+                                                        int combine)
+{
+    __shared__ T red[16];
+    T v = red_init<T>(op);
+    for (long i = threadIdx.x; i < n; i += blockDim.x) { v = red_combine(op, v, partials[i]); }
+    v = block_reduce(op, v, red);
+    if (threadIdx.x == 0) { *out = combine ? red_combine(op, *out, v) : v; }
+}
+
+template<typename T>
+__global__ __launch_bounds__(256) void k_add_separable(LayoutDev L, G3 g, T* __restrict__ data, double coef,
+                                                       const double* __restrict__ tx, const double* __restrict__ ty,
+                                                       const double* __restrict__ tz)
+{
+    const int b = blockIdx.y;
+    const int* lo = L.lows + 3 * b;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    T* d = data + (long)b * gvol(L, g);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const double v = (tx[lo[0] + x] * ty[lo[1] + y]) * tz[lo[2] + z];
+        const long i = rel_index(L, g, x, y, z);
+        d[i] = (T)((double)d[i] + coef * v);
+    }
+}
+
+}  // namespace pm
+
+extern "C" {
+
+int pm_level_reduce(const pm_layout* L, int dtype, const void* data, const int ghost[3], int ncomp, int comp, int op,
+                    void* out_dev, int combine, pm_stream_t stream)
+{
+    PM_REQUIRE(L && data && out_dev && ncomp >= 1 && comp >= 0 && comp < ncomp, "pm_level_reduce: bad argument");
+    PM_REQUIRE(op >= PM_RED_MAX && op <= PM_RED_SUMSQ, "pm_level_reduce: bad op %d", op);
+    const G3 g = g3(ghost);
+    const LayoutDev D = ldev(L);
+    const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
+    const unsigned bpb = bx_blocks(n, 64);
+    dim3 grid(bpb, L->nboxes);
+    void* ws = nullptr;
+    PM_DISPATCH_T(dtype, {
+        int rc = workspace(S(stream), sizeof(T) * (size_t)bpb * L->nboxes, &ws);
+        if (rc) { return rc; }
+        hipLaunchKernelGGL(k_level_reduce1<T>, grid, dim3(256), 0, S(stream), D, g, (const T*)data, ncomp, comp, op,
+                           (T*)ws);
+        hipLaunchKernelGGL(k_level_reduce2<T>, dim3(1), dim3(1024), 0, S(stream), (const T*)ws,
+                           (long)bpb * L->nboxes, op, (T*)out_dev, combine);
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_add_separable(const pm_layout* L, int dtype, void* data, const int ghost[3], double coef,
+                           const double* tx, const double* ty, const double* tz, pm_stream_t stream)
+{
+    PM_REQUIRE(L && data && tx && ty && tz, "pm_level_add_separable: null argument");
+    const size_t nt = (size_t)L->domain[0] + L->domain[1] + L->domain[2];
+    double* dt = nullptr;
+    PM_CHECK_HIP(hipMalloc(&dt, sizeof(double) * nt));
+    hipMemcpy(dt, tx, sizeof(double) * L->domain[0], hipMemcpyHostToDevice);
+    hipMemcpy(dt + L->domain[0], ty, sizeof(double) * L->domain[1], hipMemcpyHostToDevice);
+    hipMemcpy(dt + L->domain[0] + L->domain[1], tz, sizeof(double) * L->domain[2], hipMemcpyHostToDevice);
+    const G3 g = g3(ghost);
+    const LayoutDev D = ldev(L);
+    const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
+    dim3 grid(bx_blocks(n), L->nboxes);
+    if (dtype == PM_F64) {
+        hipLaunchKernelGGL(k_add_separable<double>, grid, dim3(256), 0, S(stream), D, g, (double*)data, coef, dt,
+                           dt + L->domain[0], dt + L->domain[0] + L->domain[1]);
+    } else {
+        hipLaunchKernelGGL(k_add_separable<float>, grid, dim3(256), 0, S(stream), D, g, (float*)data, coef, dt,
+                           dt + L->domain[0], dt + L->domain[0] + L->domain[1]);
+    }
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(S(stream));
+    hipFree(dt);
+    if (le != hipSuccess || se != hipSuccess) {
+        set_error("pm_level_add_separable: %s", hipGetErrorString(le != hipSuccess ? le : se));
+        return PM_ERR_HIP;
+    }
+    return PM_OK;
+}
+
+}  // extern "C"

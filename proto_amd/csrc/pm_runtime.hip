@@ -109,6 +109,33 @@ __global__ void k_fill_separable(T* __restrict__ out, Box3 box, Box3 region, int
     }
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template<int NT>
+__global__ __launch_bounds__(256) void k_copy16(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // four independent 16-byte loads in flight per thread
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        u32x4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        if (NT) {
+            __builtin_nontemporal_store(a, &dst[i]);
+            __builtin_nontemporal_store(b, &dst[i + stride]);
+            __builtin_nontemporal_store(c, &dst[i + 2 * stride]);
+            __builtin_nontemporal_store(d, &dst[i + 3 * stride]);
+        } else {
+            dst[i] = a;
+            dst[i + stride] = b;
+            dst[i + 2 * stride] = c;
+            dst[i + 3 * stride] = d;
+        }
+    }
+    for (; i < n; i += stride) {
+        if (NT) { __builtin_nontemporal_store(src[i], &dst[i]); } else { dst[i] = src[i]; }
+    }
+}
+
 }  // namespace pm
 
 using namespace pm;
@@ -168,6 +195,24 @@ int pm_memset_zero(void* ptr, size_t bytes, pm_stream_t stream)
     if (bytes == 0) { return PM_OK; }
     PM_REQUIRE(ptr, "pm_memset_zero: null pointer");
     PM_CHECK_HIP(hipMemsetAsync(ptr, 0, bytes, S(stream)));
+    return PM_OK;
+}
+
+int pm_copy_device(void* dst, const void* src, size_t bytes, int nontemporal, pm_stream_t stream)
+{
+    if (bytes == 0) { return PM_OK; }
+    PM_REQUIRE(dst && src, "pm_copy_device: null pointer");
+    PM_REQUIRE(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
+               "pm_copy_device: bytes and pointers must be multiples of 16");
+    const size_t n = bytes / 16;
+    // 8 resident 256-thread workgroups on each of the 256 CUs
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 8);
+    if (nontemporal) {
+        hipLaunchKernelGGL(k_copy16<1>, dim3(blocks), dim3(256), 0, S(stream), (const u32x4*)src, (u32x4*)dst, n);
+    } else {
+        hipLaunchKernelGGL(k_copy16<0>, dim3(blocks), dim3(256), 0, S(stream), (const u32x4*)src, (u32x4*)dst, n);
+    }
+    PM_CHECK_LAUNCH();
     return PM_OK;
 }
 

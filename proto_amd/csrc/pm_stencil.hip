@@ -8,7 +8,10 @@
 // of Stencil::hostApply (Proto_StencilImplem.H:725-828), so results are
 // bit-identical to Proto's CPU path.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "pm_internal.hpp"
@@ -119,8 +122,10 @@ __global__ __launch_bounds__(256) void k_stencil_generic(GenArgs<T> a)
 // x and y) is staged once from HBM into a double-buffered LDS tile; each
 // thread keeps the 3 x 3 neighbourhoods of its RY rows for planes z-1, z,
 // z+1 in registers (register queue along z), so one LDS pass per plane feeds
-// all 27 taps.  Global loads of plane z+1 are issued before the barrier of
-// plane z so their latency overlaps the arithmetic.
+// all 27 taps.  Global loads of the next PF planes are in flight while a
+// plane is computed.  NT = 1 writes results with non-temporal stores (they
+// are streamed out, never re-read by this kernel).  Tile rows (RY), prefetch
+// depth (PF), NT and the z extent per block are the autotuner's knobs.
 enum { ZM_REPLACE = 0, ZM_ACCUM = 1, ZM_RESIDUAL = 2, ZM_RESIDUAL2 = 3 };
 
 template<typename T>
@@ -136,7 +141,13 @@ struct ZArgs {
     T c2[27];       // second operator (ZM_RESIDUAL2, always the 27-point shape)
 };
 
-template<typename T, int SHAPE, int MODE, int RY>
+template<int NT, typename T>
+__device__ inline void put(T* p, T v)
+{
+    if (NT) { __builtin_nontemporal_store(v, p); } else { *p = v; }
+}
+
+template<typename T, int SHAPE, int MODE, int RY, int PF, int NT>
 __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
 {
     constexpr int TX = 64, TY = 4 * RY, LX = TX + 2, LY = TY + 2;
@@ -167,13 +178,12 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
         eok[k] = (e < NE) && gx >= a.sbox.lo[0] && gx <= a.sbox.hi[0] && gy >= a.sbox.lo[1] && gy <= a.sbox.hi[1];
         eoff[k] = (long)(gx - a.sbox.lo[0]) + sx * (long)(gy - a.sbox.lo[1]);
     }
-    T pre[NL];
-    auto load = [&](int z) {
+    auto load = [&](T (&pre)[NL], int z) {
         const T* base = a.src + sxy * (long)(z - a.sbox.lo[2]);
 #pragma unroll
         for (int k = 0; k < NL; k++) { pre[k] = eok[k] ? base[eoff[k]] : (T)0; }
     };
-    auto store = [&](int buf) {
+    auto store = [&](const T (&pre)[NL], int buf) {
 #pragma unroll
         for (int k = 0; k < NL; k++) {
             const int e = tid + 256 * k;
@@ -193,12 +203,8 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
     const int gx = x0 + lane;
     const bool xok = gx <= a.ibox.hi[0];
 
-    int buf = 0;
-    load(zb - 1);
-    for (int zz = zb - 1; zz <= ze; ++zz) {
-        store(buf);
-        if (zz + 1 <= ze) { load(zz + 1); }
-        __syncthreads();
+    // shift the register queue, read plane zz from LDS, emit plane zz-1
+    auto compute = [&](int zz, int buf) {
 #pragma unroll
         for (int r = 0; r < RY + 2; r++)
 #pragma unroll
@@ -207,46 +213,74 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
                 q[1][r][c] = q[2][r][c];
                 q[2][r][c] = sm[buf][w * RY + r][lane + c];
             }
-        if (zz >= zb + 1) {
-            const int z = zz - 1;
+        if (zz < zb + 1) { return; }
+        const int z = zz - 1;
 #pragma unroll
-            for (int o = 0; o < RY; o++) {
-                const int gy = y0 + w * RY + o;
-                if (!(xok && gy <= a.ibox.hi[1])) { continue; }
-                T acc = 0;
-                if (MODE == ZM_ACCUM) { acc = a.dst[box_index(a.dbox, gx, gy, z)]; }
+        for (int o = 0; o < RY; o++) {
+            const int gy = y0 + w * RY + o;
+            if (!(xok && gy <= a.ibox.hi[1])) { continue; }
+            T acc = 0;
+            if (MODE == ZM_ACCUM) { acc = a.dst[box_index(a.dbox, gx, gy, z)]; }
 #pragma unroll
-                for (int j = 0; j < NC; j++) {
-                    const int dx = shape_off<SHAPE>(j, 0), dy = shape_off<SHAPE>(j, 1), dz = shape_off<SHAPE>(j, 2);
-                    acc = acc + a.c[j] * q[1 + dz][o + 1 + dy][1 + dx];
-                }
-                if (MODE == ZM_REPLACE || MODE == ZM_ACCUM) {
-                    a.dst[box_index(a.dbox, gx, gy, z)] = acc;
-                } else {
-                    const long ii = box_index(a.ibox, gx, gy, z);
-                    const T fv = a.f[ii];
-                    const T rv = fv - acc;
-                    a.dst[ii] = rv;
-                    const T av = rv > 0 ? rv : -rv;
-                    amax = amax < av ? av : amax;
-                    sum = sum + rv;
-                    if (MODE == ZM_RESIDUAL2) {
-                        T acc2 = 0;
+            for (int j = 0; j < NC; j++) {
+                const int dx = shape_off<SHAPE>(j, 0), dy = shape_off<SHAPE>(j, 1), dz = shape_off<SHAPE>(j, 2);
+                acc = acc + a.c[j] * q[1 + dz][o + 1 + dy][1 + dx];
+            }
+            if (MODE == ZM_REPLACE || MODE == ZM_ACCUM) {
+                put<NT>(&a.dst[box_index(a.dbox, gx, gy, z)], acc);
+            } else {
+                const long ii = box_index(a.ibox, gx, gy, z);
+                const T fv = a.f[ii];
+                const T rv = fv - acc;
+                put<NT>(&a.dst[ii], rv);
+                const T av = rv > 0 ? rv : -rv;
+                amax = amax < av ? av : amax;
+                sum = sum + rv;
+                if (MODE == ZM_RESIDUAL2) {
+                    T acc2 = 0;
 #pragma unroll
-                        for (int j = 0; j < 27; j++) {
-                            const int dx = lap27_off(j, 0), dy = lap27_off(j, 1), dz = lap27_off(j, 2);
-                            acc2 = acc2 + a.c2[j] * q[1 + dz][o + 1 + dy][1 + dx];
-                        }
-                        const T rv2 = fv - acc2;
-                        a.dst2[ii] = rv2;
-                        const T av2 = rv2 > 0 ? rv2 : -rv2;
-                        amax2 = amax2 < av2 ? av2 : amax2;
-                        sum2 = sum2 + rv2;
+                    for (int j = 0; j < 27; j++) {
+                        const int dx = lap27_off(j, 0), dy = lap27_off(j, 1), dz = lap27_off(j, 2);
+                        acc2 = acc2 + a.c2[j] * q[1 + dz][o + 1 + dy][1 + dx];
                     }
+                    const T rv2 = fv - acc2;
+                    put<NT>(&a.dst2[ii], rv2);
+                    const T av2 = rv2 > 0 ? rv2 : -rv2;
+                    amax2 = amax2 < av2 ? av2 : amax2;
+                    sum2 = sum2 + rv2;
                 }
             }
         }
-        buf ^= 1;
+    };
+
+    T preA[NL], preB[NL];
+    if (PF == 1) {
+        int buf = 0;
+        load(preA, zb - 1);
+        for (int zz = zb - 1; zz <= ze; ++zz) {
+            store(preA, buf);
+            if (zz + 1 <= ze) { load(preA, zz + 1); }
+            __syncthreads();
+            compute(zz, buf);
+            buf ^= 1;
+        }
+    } else {
+        // two planes in flight: plane zz in preA (even steps) / preB (odd steps)
+        load(preA, zb - 1);
+        load(preB, zb);
+        int zz = zb - 1;
+        for (;;) {
+            store(preA, 0);
+            if (zz + 2 <= ze) { load(preA, zz + 2); }
+            __syncthreads();
+            compute(zz, 0);
+            if (++zz > ze) { break; }
+            store(preB, 1);
+            if (zz + 2 <= ze) { load(preB, zz + 2); }
+            __syncthreads();
+            compute(zz, 1);
+            if (++zz > ze) { break; }
+        }
     }
     if (MODE == ZM_RESIDUAL || MODE == ZM_RESIDUAL2) {
         const unsigned pb = blockIdx.x;
@@ -342,39 +376,105 @@ static int launch_generic(const pm_stencil* st, const void* src, const Box3& sbo
     return PM_OK;
 }
 
+// ------------------------------------------------------------ tile configurations
+struct ZCfg {
+    int ry, pf, nt, tz;  // rows per thread, planes prefetched, NT stores, planes per block (0: auto)
+};
+
+static ZCfg default_cfg(int dtype) { return ZCfg{dtype == PM_F64 ? 2 : 4, 1, 0, 0}; }
+
+// Tuned configurations per (operator kind, mode, dtype, index-box shape).
+struct TuneKey {
+    int kind, mode, dtype, nx, ny, nz;
+    bool operator<(const TuneKey& o) const
+    {
+        const int a[6] = {kind, mode, dtype, nx, ny, nz}, b[6] = {o.kind, o.mode, o.dtype, o.nx, o.ny, o.nz};
+        for (int k = 0; k < 6; k++) {
+            if (a[k] != b[k]) { return a[k] < b[k]; }
+        }
+        return false;
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, ZCfg>* g_tune = nullptr;
+
+static TuneKey tune_key(int kind, int mode, int dtype, const Box3& ib)
+{
+    return TuneKey{kind, mode, dtype, (int)box_nx(ib), (int)box_ny(ib), (int)box_nz(ib)};
+}
+
+static ZCfg lookup_cfg(int kind, int mode, int dtype, const Box3& ib)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_tune) {
+        auto it = g_tune->find(tune_key(kind, mode, dtype, ib));
+        if (it != g_tune->end()) { return it->second; }
+    }
+    return default_cfg(dtype);
+}
+
+static void store_cfg(const TuneKey& k, const ZCfg& c)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune) { g_tune = new std::map<TuneKey, ZCfg>(); }
+    (*g_tune)[k] = c;
+}
+
 struct ZPlan {
     int ntx, nty, ntz, tz;
     unsigned blocks;
 };
 
-static ZPlan zplan(const Box3& ibox, int RY)
+static ZPlan zplan(const Box3& ibox, const ZCfg& c)
 {
     ZPlan p;
     const int nx = (int)box_nx(ibox), ny = (int)box_ny(ibox), nz = (int)box_nz(ibox);
     p.ntx = (nx + 63) / 64;
-    p.nty = (ny + 4 * RY - 1) / (4 * RY);
-    // enough blocks to fill 256 CUs several times over, but long z runs
+    p.nty = (ny + 4 * c.ry - 1) / (4 * c.ry);
     int tz = nz;
-    while (tz > 16 && (long)p.ntx * p.nty * ((nz + tz - 1) / tz) < 4096) { tz = (tz + 1) / 2; }
+    if (c.tz > 0) {
+        tz = std::min(c.tz, nz);
+    } else {
+        // enough blocks to fill 256 CUs several times over, but long z runs
+        while (tz > 16 && (long)p.ntx * p.nty * ((nz + tz - 1) / tz) < 4096) { tz = (tz + 1) / 2; }
+    }
     p.tz = tz;
     p.ntz = (nz + tz - 1) / tz;
     p.blocks = (unsigned)((long)p.ntx * p.nty * p.ntz);
     return p;
 }
 
-template<typename T> static int zmarch_ry() { return sizeof(T) == 8 ? 2 : 4; }
+template<typename T, int SHAPE, int MODE>
+using ZKernel = void (*)(ZArgs<T>);
 
 template<typename T, int SHAPE, int MODE>
-static int launch_zmarch(ZArgs<T>& a, const Box3& ibox, hipStream_t s, unsigned* nblocks)
+static ZKernel<T, SHAPE, MODE> zkernel(const ZCfg& c)
 {
-    constexpr int RY = sizeof(T) == 8 ? 2 : 4;
-    ZPlan p = zplan(ibox, RY);
+#define PM_ZK(RY, PF, NT) \
+    if (c.ry == RY && c.pf == PF && c.nt == NT) { return k_zmarch<T, SHAPE, MODE, RY, PF, NT>; }
+    PM_ZK(1, 1, 0) PM_ZK(1, 1, 1) PM_ZK(1, 2, 0) PM_ZK(1, 2, 1)
+    PM_ZK(2, 1, 0) PM_ZK(2, 1, 1) PM_ZK(2, 2, 0) PM_ZK(2, 2, 1)
+    PM_ZK(4, 1, 0) PM_ZK(4, 1, 1) PM_ZK(4, 2, 0) PM_ZK(4, 2, 1)
+#undef PM_ZK
+    return nullptr;
+}
+
+static const int kRY[3] = {1, 2, 4};
+
+static bool valid_cfg(const ZCfg& c) { return zkernel<double, 7, ZM_REPLACE>(c) != nullptr && c.tz >= 0; }
+
+template<typename T, int SHAPE, int MODE>
+static int launch_zmarch(ZArgs<T>& a, const Box3& ibox, const ZCfg& c, hipStream_t s, unsigned* nblocks)
+{
+    ZKernel<T, SHAPE, MODE> k = zkernel<T, SHAPE, MODE>(c);
+    PM_REQUIRE(k != nullptr, "no z-march kernel for rows %d, prefetch %d, nt %d", c.ry, c.pf, c.nt);
+    ZPlan p = zplan(ibox, c);
     a.ntx = p.ntx;
     a.nty = p.nty;
     a.tz = p.tz;
     a.ibox = ibox;
     *nblocks = p.blocks;
-    hipLaunchKernelGGL((k_zmarch<T, SHAPE, MODE, RY>), dim3(p.blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k, dim3(p.blocks), dim3(256), 0, s, a);
     PM_CHECK_LAUNCH();
     return PM_OK;
 }
@@ -411,10 +511,14 @@ static int apply_t(const pm_stencil* st, const void* src, const Box3& sbox, void
             scaled_coefs<T>(st, scale, a.c);
             unsigned nb;
             int rc;
+            const int mode = replace ? ZM_REPLACE : ZM_ACCUM;
+            const ZCfg cfg = lookup_cfg(st->kind, mode, st->dtype, ibox);
             if (st->kind == 7) {
-                rc = replace ? launch_zmarch<T, 7, ZM_REPLACE>(a, ibox, s, &nb) : launch_zmarch<T, 7, ZM_ACCUM>(a, ibox, s, &nb);
+                rc = replace ? launch_zmarch<T, 7, ZM_REPLACE>(a, ibox, cfg, s, &nb)
+                             : launch_zmarch<T, 7, ZM_ACCUM>(a, ibox, cfg, s, &nb);
             } else {
-                rc = replace ? launch_zmarch<T, 27, ZM_REPLACE>(a, ibox, s, &nb) : launch_zmarch<T, 27, ZM_ACCUM>(a, ibox, s, &nb);
+                rc = replace ? launch_zmarch<T, 27, ZM_REPLACE>(a, ibox, cfg, s, &nb)
+                             : launch_zmarch<T, 27, ZM_ACCUM>(a, ibox, cfg, s, &nb);
             }
             if (rc) { return rc; }
         }
@@ -437,14 +541,15 @@ static int residual_t(const pm_stencil* st, const void* u, const Box3& ubox, con
         a.sbox = ubox;
         a.dbox = rbox;
         scaled_coefs<T>(st, scale, a.c);
-        ZPlan p = zplan(rbox, zmarch_ry<T>());
+        const ZCfg cfg = lookup_cfg(st->kind, ZM_RESIDUAL, st->dtype, rbox);
+        ZPlan p = zplan(rbox, cfg);
         void* ws = nullptr;
         int rc = workspace(s, sizeof(T) * 2 * (size_t)p.blocks, &ws);
         if (rc) { return rc; }
         a.partials = (T*)ws;
         unsigned nb;
-        rc = st->kind == 7 ? launch_zmarch<T, 7, ZM_RESIDUAL>(a, rbox, s, &nb)
-                           : launch_zmarch<T, 27, ZM_RESIDUAL>(a, rbox, s, &nb);
+        rc = st->kind == 7 ? launch_zmarch<T, 7, ZM_RESIDUAL>(a, rbox, cfg, s, &nb)
+                           : launch_zmarch<T, 27, ZM_RESIDUAL>(a, rbox, cfg, s, &nb);
         if (rc) { return rc; }
         if (red_out) {
             hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)nb, 2, (T*)red_out);
@@ -485,19 +590,117 @@ static int residual2_t(const pm_stencil* st1, const pm_stencil* st2, const void*
     a.dbox = rbox;
     scaled_coefs<T>(st1, scale1, a.c);
     scaled_coefs<T>(st2, scale2, a.c2);
-    ZPlan p = zplan(rbox, zmarch_ry<T>());
+    const ZCfg cfg = lookup_cfg(st1->kind * 100 + st2->kind, ZM_RESIDUAL2, st1->dtype, rbox);
+    ZPlan p = zplan(rbox, cfg);
     void* ws = nullptr;
     int rc = workspace(s, sizeof(T) * 4 * (size_t)p.blocks, &ws);
     if (rc) { return rc; }
     a.partials = (T*)ws;
     unsigned nb;
-    rc = (st1->kind == 7) ? launch_zmarch<T, 7, ZM_RESIDUAL2>(a, rbox, s, &nb)
-                          : launch_zmarch<T, 27, ZM_RESIDUAL2>(a, rbox, s, &nb);
+    rc = (st1->kind == 7) ? launch_zmarch<T, 7, ZM_RESIDUAL2>(a, rbox, cfg, s, &nb)
+                          : launch_zmarch<T, 27, ZM_RESIDUAL2>(a, rbox, cfg, s, &nb);
     if (rc) { return rc; }
     if (red_out) {
         hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)nb, 4, (T*)red_out);
         PM_CHECK_LAUNCH();
     }
+    return PM_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+template<typename T>
+static int run_cfg(const pm_stencil* st, const pm_stencil* st2, int mode, ZArgs<T>& a, const Box3& ib, const ZCfg& c,
+                   hipStream_t s)
+{
+    unsigned nb;
+    const bool k7 = st->kind == 7;
+    switch (mode) {
+        case ZM_REPLACE: return k7 ? launch_zmarch<T, 7, ZM_REPLACE>(a, ib, c, s, &nb) : launch_zmarch<T, 27, ZM_REPLACE>(a, ib, c, s, &nb);
+        case ZM_ACCUM: return k7 ? launch_zmarch<T, 7, ZM_ACCUM>(a, ib, c, s, &nb) : launch_zmarch<T, 27, ZM_ACCUM>(a, ib, c, s, &nb);
+        case ZM_RESIDUAL: return k7 ? launch_zmarch<T, 7, ZM_RESIDUAL>(a, ib, c, s, &nb) : launch_zmarch<T, 27, ZM_RESIDUAL>(a, ib, c, s, &nb);
+        default: return k7 ? launch_zmarch<T, 7, ZM_RESIDUAL2>(a, ib, c, s, &nb) : launch_zmarch<T, 27, ZM_RESIDUAL2>(a, ib, c, s, &nb);
+    }
+    (void)st2;
+}
+
+static int key_kind(const pm_stencil* st, const pm_stencil* st2, int mode)
+{
+    return mode == ZM_RESIDUAL2 ? st->kind * 100 + st2->kind : st->kind;
+}
+
+template<typename T>
+static int autotune_t(const pm_stencil* st, const pm_stencil* st2, int mode, const Box3& sb, const Box3& ib, int reps,
+                      ZCfg* best, float* best_ms, hipStream_t s)
+{
+    const size_t ns = (size_t)box_size(sb), ni = (size_t)box_size(ib);
+    // the most blocks any candidate launches: one row per thread, 16 planes
+    const size_t maxb = (size_t)((box_nx(ib) + 63) / 64) * (size_t)((box_ny(ib) + 3) / 4) * (size_t)((box_nz(ib) + 15) / 16);
+    T *src = nullptr, *dst = nullptr, *dst2 = nullptr, *f = nullptr, *part = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PM_OK;
+    auto cleanup = [&]() {
+        hipFree(src); hipFree(dst); hipFree(dst2); hipFree(f); hipFree(part);
+        if (e0) { hipEventDestroy(e0); }
+        if (e1) { hipEventDestroy(e1); }
+    };
+    if (hipMalloc(&src, sizeof(T) * ns) != hipSuccess || hipMalloc(&dst, sizeof(T) * ni) != hipSuccess ||
+        hipMalloc(&dst2, sizeof(T) * ni) != hipSuccess || hipMalloc(&f, sizeof(T) * ni) != hipSuccess ||
+        hipMalloc(&part, sizeof(T) * 4 * maxb) != hipSuccess) {
+        cleanup();
+        set_error("pm_stencil_autotune: scratch allocation failed");
+        return PM_ERR_NOMEM;
+    }
+    pm_fill_uniform(dtype_of<T>::value, src, ns, 17, 0, 1.0, 0.0, (pm_stream_t)s);
+    pm_fill_uniform(dtype_of<T>::value, f, ni, 17, 1, 1.0, 0.0, (pm_stream_t)s);
+    pm_fill_uniform(dtype_of<T>::value, dst, ni, 17, 2, 1.0, 0.0, (pm_stream_t)s);
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        set_error("pm_stencil_autotune: event creation failed");
+        return PM_ERR_HIP;
+    }
+    ZArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.src = src;
+    a.dst = dst;
+    a.dst2 = dst2;
+    a.f = f;
+    a.partials = part;
+    a.sbox = sb;
+    a.dbox = ib;
+    scaled_coefs<T>(st, 1.0, a.c);
+    if (st2) { scaled_coefs<T>(st2, 1.0, a.c2); }
+    const int nz = (int)box_nz(ib);
+    const int tzs[6] = {0, 16, 32, 64, 128, 256};
+    float bms = 1e30f;
+    ZCfg bc = default_cfg(dtype_of<T>::value);
+    for (int ry : kRY)
+        for (int pf = 1; pf <= 2; pf++)
+            for (int nt = 0; nt <= 1; nt++)
+                for (int tz : tzs) {
+                    if (tz > nz) { continue; }
+                    ZCfg c{ry, pf, nt, tz};
+                    rc = run_cfg<T>(st, st2, mode, a, ib, c, s);
+                    if (rc) { cleanup(); return rc; }
+                    hipEventRecord(e0, s);
+                    for (int r = 0; r < reps; r++) {
+                        rc = run_cfg<T>(st, st2, mode, a, ib, c, s);
+                        if (rc) { cleanup(); return rc; }
+                    }
+                    hipEventRecord(e1, s);
+                    if (hipEventSynchronize(e1) != hipSuccess) {
+                        cleanup();
+                        set_error("pm_stencil_autotune: timing failed");
+                        return PM_ERR_HIP;
+                    }
+                    float ms = 0;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    ms /= reps;
+                    if (ms < bms) { bms = ms; bc = c; }
+                }
+    cleanup();
+    store_cfg(tune_key(key_kind(st, st2, mode), mode, dtype_of<T>::value, ib), bc);
+    *best = bc;
+    *best_ms = bms;
     return PM_OK;
 }
 
@@ -631,6 +834,93 @@ int pm_stencil_residual(const pm_stencil* st, const void* u, const pm_box* u_box
     PM_REQUIRE(covers(st, ub, rb, rb), "pm_stencil_residual: u does not cover the stencil's domain of r_box");
     if (st->dtype == PM_F64) { return residual_t<double>(st, u, ub, f, r, rb, scale, red_out, S(stream)); }
     return residual_t<float>(st, u, ub, f, r, rb, scale, red_out, S(stream));
+}
+
+int pm_stencil_autotune(const pm_stencil* st, const pm_stencil* st2, int mode, const pm_box* src_box,
+                        const pm_box* idx_box, int reps, pm_tile_config* best, float* best_ms, pm_stream_t stream)
+{
+    PM_REQUIRE(st && src_box && idx_box && best && best_ms, "pm_stencil_autotune: null argument");
+    PM_REQUIRE(mode >= 0 && mode <= 3, "pm_stencil_autotune: bad mode %d", mode);
+    PM_REQUIRE(st->kind == 7 || st->kind == 27, "pm_stencil_autotune: only the 7/27-point z-march kernels are tunable");
+    PM_REQUIRE(mode != ZM_RESIDUAL2 || (st2 && st2->kind == 27 && st2->dtype == st->dtype),
+               "pm_stencil_autotune: mode 3 needs a 27-point second operator of the same dtype");
+    PM_REQUIRE(reps >= 1, "pm_stencil_autotune: reps must be >= 1");
+    const Box3 sb = box3(src_box), ib = box3(idx_box);
+    PM_REQUIRE(!box_empty(ib) && covers(st, sb, ib, ib), "pm_stencil_autotune: source box does not cover the index box");
+    ZCfg c;
+    int rc = st->dtype == PM_F64 ? autotune_t<double>(st, st2, mode, sb, ib, reps, &c, best_ms, S(stream))
+                                 : autotune_t<float>(st, st2, mode, sb, ib, reps, &c, best_ms, S(stream));
+    if (rc) { return rc; }
+    best->rows = c.ry;
+    best->prefetch = c.pf;
+    best->nt_store = c.nt;
+    best->zplanes = c.tz;
+    return PM_OK;
+}
+
+int pm_stencil_set_tile_config(const pm_stencil* st, const pm_stencil* st2, int mode, const pm_box* idx_box,
+                               const pm_tile_config* cfg)
+{
+    PM_REQUIRE(st && idx_box && cfg && mode >= 0 && mode <= 3, "pm_stencil_set_tile_config: bad argument");
+    PM_REQUIRE(mode != ZM_RESIDUAL2 || st2, "pm_stencil_set_tile_config: mode 3 needs st2");
+    ZCfg c{cfg->rows, cfg->prefetch, cfg->nt_store, cfg->zplanes};
+    PM_REQUIRE(valid_cfg(c), "pm_stencil_set_tile_config: no such configuration");
+    store_cfg(tune_key(key_kind(st, st2, mode), mode, st->dtype, box3(idx_box)), c);
+    return PM_OK;
+}
+
+int pm_stencil_get_tile_config(const pm_stencil* st, const pm_stencil* st2, int mode, const pm_box* idx_box,
+                               pm_tile_config* cfg)
+{
+    PM_REQUIRE(st && idx_box && cfg && mode >= 0 && mode <= 3, "pm_stencil_get_tile_config: bad argument");
+    PM_REQUIRE(mode != ZM_RESIDUAL2 || st2, "pm_stencil_get_tile_config: mode 3 needs st2");
+    ZCfg c = lookup_cfg(key_kind(st, st2, mode), mode, st->dtype, box3(idx_box));
+    cfg->rows = c.ry;
+    cfg->prefetch = c.pf;
+    cfg->nt_store = c.nt;
+    cfg->zplanes = c.tz;
+    return PM_OK;
+}
+
+int pm_tune_table_save(const char* path)
+{
+    PM_REQUIRE(path, "pm_tune_table_save: null path");
+    FILE* fp = std::fopen(path, "w");
+    PM_REQUIRE(fp, "pm_tune_table_save: cannot open %s", path);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::fprintf(fp, "# kind mode dtype nx ny nz rows prefetch nt zplanes\n");
+    if (g_tune) {
+        for (auto& kv : *g_tune) {
+            const TuneKey& k = kv.first;
+            const ZCfg& c = kv.second;
+            std::fprintf(fp, "%d %d %d %d %d %d %d %d %d %d\n", k.kind, k.mode, k.dtype, k.nx, k.ny, k.nz, c.ry, c.pf, c.nt,
+                         c.tz);
+        }
+    }
+    std::fclose(fp);
+    return PM_OK;
+}
+
+int pm_tune_table_load(const char* path)
+{
+    PM_REQUIRE(path, "pm_tune_table_load: null path");
+    FILE* fp = std::fopen(path, "r");
+    PM_REQUIRE(fp, "pm_tune_table_load: cannot open %s", path);
+    char line[256];
+    int n = 0;
+    while (std::fgets(line, sizeof(line), fp)) {
+        if (line[0] == '#') { continue; }
+        TuneKey k;
+        ZCfg c;
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &k.kind, &k.mode, &k.dtype, &k.nx, &k.ny, &k.nz, &c.ry,
+                        &c.pf, &c.nt, &c.tz) == 10 &&
+            valid_cfg(c)) {
+            store_cfg(k, c);
+            n++;
+        }
+    }
+    std::fclose(fp);
+    return PM_OK;
 }
 
 int pm_stencil_residual2(const pm_stencil* st1, const pm_stencil* st2, const void* u, const pm_box* u_box,

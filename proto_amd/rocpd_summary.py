@@ -1,0 +1,49 @@
+"""Summarise rocprofv3 rocpd databases (kernel durations; FETCH_SIZE /
+WRITE_SIZE per dispatch, calibrated against a copy kernel of known size) into
+the short text tables kept under profiles/.
+
+    python -m proto_amd.rocpd_summary trace  DB
+    python -m proto_amd.rocpd_summary pmc    FETCH_DB WRITE_DB COPY_BYTES
+"""
+import sqlite3
+import sys
+
+
+def trace(db):
+    c = sqlite3.connect(db)
+    out = ["%-72s %6s %12s %12s" % ("kernel", "calls", "avg_us", "total_us")]
+    for name, calls, total, avg, pct in c.execute("select * from top_kernels"):
+        out.append("%-72s %6d %12.2f %12.1f" % (name[:72], calls, avg, total))
+    return "\n".join(out)
+
+
+def _per_kernel(db):
+    c = sqlite3.connect(db)
+    acc = {}
+    for name, value, grid in c.execute("select kernel_name, value, grid_size from counters_collection"):
+        acc.setdefault((name, grid), []).append(value)
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def pmc(fetch_db, write_db, copy_bytes):
+    f, w = _per_kernel(fetch_db), _per_kernel(write_db)
+    copy_keys = [k for k in f if "copyBuffer" in k[0] and f[k] > 1e4]
+    ck = max(copy_keys, key=lambda k: f[k])
+    half = copy_bytes / 2 / 1024.0  # a copy reads and writes copy_bytes/2 each
+    ff, wf = half / f[ck], half / w[ck]
+    out = ["calibration: plain copy of %d B reads and writes %.0f KiB each; counters read FETCH %.0f KiB, "
+           "WRITE %.0f KiB -> factors fetch x%.4f, write x%.4f" % (copy_bytes // 2, half, f[ck], w[ck], ff, wf),
+           "%-60s %14s %14s %14s" % ("kernel (grid)", "fetch_GB", "write_GB", "traffic_GB")]
+    for k in sorted(f, key=lambda k: -f[k]):
+        if f[k] < 1e3 or k not in w:
+            continue
+        fg, wg = f[k] * ff * 1024 / 1e9, w[k] * wf * 1024 / 1e9
+        out.append("%-60s %14.4f %14.4f %14.4f" % ((k[0][:48] + " (%d)" % k[1]), fg, wg, fg + wg))
+    return "\n".join(out)
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "trace":
+        print(trace(sys.argv[2]))
+    else:
+        print(pmc(sys.argv[2], sys.argv[3], int(sys.argv[4])))

@@ -1,0 +1,22 @@
+"""Runs the C++ API programs of tests/cpp (built by __graft_entry__.build())
+on the GPU: Proto-style user code over proto_amd/cpp and libprotomi.so."""
+import os
+import subprocess
+
+import pytest
+
+import util
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(util.ROOT, "tests", "cpp", "bin")
+
+
+@pytest.mark.parametrize("name", ["test_api_stencil", "test_api_level"])
+def test_cpp_program(name):
+    exe = os.path.join(BIN, name)
+    assert os.path.exists(exe), "C++ test program not built: run __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-4000:])
+    print(r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout[-2000:]

@@ -290,3 +290,48 @@ def test_boxdata_ops():
         assert host(ad).tobytes() == ref.tobytes()
     pa.call("pm_boxdata_scalar", 4, pa.PM_F64, vp(ad), ctypes.byref(B), ctypes.byref(B), -2.5, 1, None)
     assert np.all(host(ad) == -2.5)
+
+
+@pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
+def test_every_tile_configuration_is_bit_identical(dtype):
+    """The autotuner may pick any tile configuration: each one must give the
+    oracle's bits (ragged box not a multiple of any tile, odd z extent)."""
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    lo, hi = [-3, 2, 1], [70, 44, 37]
+    slo, shi = [x - 1 for x in lo], [x + 1 for x in hi]
+    src = util.uniform(61, 0, util.box_size(slo, shi)).astype(npdt)
+    f = util.uniform(61, 1, util.box_size(lo, hi)).astype(npdt)
+    s_d, f_d = dev(src), dev(f)
+    ib, sb = box(lo, hi), box(slo, shi)
+    refs = {}
+    for name in ("Laplacian", "Laplacian_27"):
+        c, o, _, _, _ = util.lib_stencil(name, dtype=npdt)
+        ref = np.zeros(util.box_size(lo, hi), dtype=npdt)
+        util.oracle_apply(c, o, src, slo, shi, ref, lo, hi, lo, hi, True, 3.0)
+        refs[name] = (pa.Stencil(c, o, dtype), ref)
+    s7, r7 = refs["Laplacian"]
+    s27, r27 = refs["Laplacian_27"]
+    out, out2, red = empty(util.box_size(lo, hi), npdt), empty(util.box_size(lo, hi), npdt), zeros(4, npdt)
+    for rows in (1, 2, 4):
+        for pf in (1, 2):
+            for nt in (0, 1):
+                for zp in (0, 5, 16):
+                    cfg = pa.TileConfig(rows, pf, nt, zp)
+                    say("tile config", dtype, rows, pf, nt, zp)
+                    for st, ref in ((s7, r7), (s27, r27)):
+                        pa.call("pm_stencil_set_tile_config", st.h, None, 0, ctypes.byref(ib), ctypes.byref(cfg))
+                        pa.call("pm_stencil_apply", st.h, vp(s_d), ctypes.byref(sb), vp(out), ctypes.byref(ib),
+                                ctypes.byref(ib), 1, 1, 3.0, None)
+                        assert host(out).tobytes() == ref.tobytes()
+                    pa.call("pm_stencil_set_tile_config", s7.h, s27.h, 3, ctypes.byref(ib), ctypes.byref(cfg))
+                    pa.call("pm_stencil_residual2", s7.h, s27.h, vp(s_d), ctypes.byref(sb), vp(f_d), vp(out),
+                            vp(out2), ctypes.byref(ib), 3.0, 3.0, vp(red), None)
+                    assert host(out).tobytes() == (f - r7).tobytes()
+                    assert host(out2).tobytes() == (f - r27).tobytes()
+    # the autotuner itself runs and records a valid configuration
+    best, ms = pa.TileConfig(), ctypes.c_float()
+    pa.call("pm_stencil_autotune", s7.h, s27.h, 3, ctypes.byref(sb), ctypes.byref(ib), 2, ctypes.byref(best),
+            ctypes.byref(ms), None)
+    got = pa.TileConfig()
+    pa.call("pm_stencil_get_tile_config", s7.h, s27.h, 3, ctypes.byref(ib), ctypes.byref(got))
+    assert got.as_dict() == best.as_dict() and ms.value > 0
