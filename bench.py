@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--only", default="", help="profile helper: run only this kernel family")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tile configurations)")
+    ap.add_argument("--no-mg", action="store_true", help="skip the config-3 multigrid measurement")
+    ap.add_argument("--mg-n", type=int, default=512)
     return ap.parse_args()
 
 
@@ -124,6 +126,65 @@ def cpu_baseline(n, g, planes):
     return {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
             "sample": "config-2 step (7- and 27-point r = f - L(u), absMax, sum) on %dx%dx%d cells, "
                       "%.2f s, Proto's CPU path single-threaded (Release flags)" % (n, n, nz, dt)}
+
+
+def mg_config3(pa, stream, n, bs, nlev, reps=3):
+    """Config 3: periodic n^3 in bs^3 boxes, 16-Gaussian RHS (sigma 0.05, seed 2)
+    minus its mean, GSRB V(2,2) to 8^3, relative residual 1e-10.  Plain FP64
+    V-cycles vs the FP32-preconditioned FP64 refinement; solves/s from the
+    wall time of complete solves (phi = 0 start, hierarchy resident)."""
+    import numpy as np
+    from proto_amd.device import DeviceArray
+    from proto_amd.inputs import gaussian_level_rhs
+    nb = n // bs
+    lows = np.array([(x * bs, y * bs, z * bs) for z in range(nb) for y in range(nb) for x in range(nb)],
+                    dtype=np.int32).reshape(-1)
+    lay = ctypes.c_void_p()
+    pa.call("pm_layout_create", ctypes.byref(lay), 3, pa.i3([n] * 3), pa.i3([bs] * 3), pa.i3([1] * 3), nb ** 3,
+            lows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    rhs = DeviceArray.zeros(nb ** 3 * bs ** 3)
+    gaussian_level_rhs(lay, n, rhs, 16, 2, stream=stream.h)
+    nphi = nb ** 3 * (bs + 2) ** 3
+    out = {"workload": "levelpoisson%d_box%d_gsrb22" % (n, bs), "levels": nlev + 1}
+    sols = {}
+    for name, prec in (("fp64", 0), ("fp32_ir", 1)):
+        opt = pa.MgOptions()
+        pa.lib().pm_mg_default_options(ctypes.byref(opt))
+        opt.relax, opt.pre, opt.post, opt.bottom = 0, 2, 2, 10
+        opt.precision, opt.max_iter, opt.tol, opt.use_graph = prec, 40, 1e-10, 1
+        mg = ctypes.c_void_p()
+        pa.call("pm_mg_create", ctypes.byref(mg), lay, nlev, 1.0 / n, ctypes.byref(opt))
+        phi = DeviceArray.zeros(nphi)
+        hist = np.zeros(41)
+        it = ctypes.c_int()
+        times = []
+        for r in range(reps + 1):
+            pa.call("pm_memset_zero", phi.ptr, phi.nbytes, stream.h)
+            stream.sync()
+            t0 = time.perf_counter()
+            pa.call("pm_mg_solve", mg, phi.ptr, rhs.ptr, hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                    ctypes.byref(it), stream.h)
+            times.append(time.perf_counter() - t0)
+        t = sorted(times[1:])[len(times[1:]) // 2]
+        nbytes = ctypes.c_size_t()
+        pa.call("pm_mg_bytes", mg, ctypes.byref(nbytes))
+        out[name] = {"iters": it.value, "solve_s": round(t, 5), "solves_per_s": round(1.0 / t, 3),
+                     "rel_residual": float(hist[it.value] / hist[0]), "hierarchy_bytes": nbytes.value}
+        sols[name] = phi
+        pa.call("pm_mg_destroy", mg)
+    # error of the mixed-precision answer against the FP64 solve
+    a, b = sols["fp64"], sols["fp32_ir"]
+    flat = pa.Box.of([0, 0, 0], [nphi - 1, 0, 0])
+    pa.call("pm_boxdata_binop", 1, pa.PM_F64, b.ptr, ctypes.byref(flat), a.ptr, ctypes.byref(flat),
+            ctypes.byref(flat), 1, stream.h)
+    m = DeviceArray.zeros(2)
+    pa.call("pm_reduce", pa.RED_ABS, pa.PM_F64, b.ptr, nphi, m.ptr, 0, stream.h)
+    pa.call("pm_reduce", pa.RED_ABS, pa.PM_F64, a.ptr, nphi, ctypes.c_void_p(m.ptr.value + 8), 0, stream.h)
+    e, s = m.numpy(stream.h)
+    out["fp32_ir"]["rel_error_vs_fp64"] = float(e / s)
+    out["speedup_fp32_ir"] = round(out["fp64"]["solve_s"] / out["fp32_ir"]["solve_s"], 3)
+    pa.call("pm_layout_destroy", lay)
+    return out
 
 
 def main():
@@ -281,6 +342,9 @@ def main():
                            "bytes": 2 * bytes_f}
         a.free()
         result["extra"] = extra
+
+    if not args.no_mg:
+        result.setdefault("extra", {})["multigrid_config3"] = mg_config3(pa, stream, args.mg_n, 64, 6 if args.mg_n == 512 else 3)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(n, g, args.cpu_planes)

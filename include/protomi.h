@@ -316,6 +316,47 @@ int pm_mg_vcycle(pm_mg* mg, void* phi, const void* rhs, pm_stream_t stream);
 /* resnorm (LevelMultigrid.cpp:75-97) of (phi, rhs); synchronises. */
 int pm_mg_resnorm(pm_mg* mg, void* phi, const void* rhs, double* out_host, pm_stream_t stream);
 
+/* ---------------------------------------------------------------- batches */
+/* A batch of independent patches of uneven size (AMR boxes).  Box b has
+ * valid extents sizes[3b..3b+2] (box-local indices from 0) and `ghost` ghost
+ * layers; ghosted storage holds the boxes one after another (x fastest over
+ * [-ghost, n+ghost)), valid storage likewise over [0, n).  Each operator is
+ * ONE persistent launch over all boxes (a work queue of tiles, largest first)
+ * instead of Proto's launch per box (e.g. LevelMultigridSolverImplem.H:
+ * 366-378); the queue resets itself, so the launch can be graph-captured. */
+typedef struct pm_batch pm_batch;
+int pm_batch_create(pm_batch** out, int nboxes, const int* sizes_host, int ghost);
+int pm_batch_destroy(pm_batch* batch);
+int pm_batch_elems(const pm_batch* batch, size_t* ghosted, size_t* valid);
+/* nboxes + 1 element offsets of every box in ghosted / valid storage. */
+int pm_batch_offsets(const pm_batch* batch, size_t* ghosted_off, size_t* valid_off);
+/* Tiling knobs (the autotuner's): z planes per apply tile (0 = whole box),
+ * rows per GSRB tile, persistent workgroups per CU. */
+int pm_batch_configure(pm_batch* batch, int apply_zplanes, int gsrb_rows, int blocks_per_cu);
+/* dst = L(src, scale) on every box (src ghosted storage, dst valid storage)
+ * for the 7-point Laplacian() or 27-point Laplacian_27() shape: per box the
+ * bits of pm_stencil_apply. */
+int pm_batch_apply(pm_batch* batch, const pm_stencil* st, const void* src, void* dst, double scale,
+                   pm_stream_t stream);
+/* One red-black Gauss-Seidel sweep of the 7-point Laplacian on every box,
+ * ghosts held fixed (LaplaceOp.H:95-123 without the exchange: the patches are
+ * independent): phi_out (ghosted storage) receives the swept phi_in; the
+ * ghosts of phi_out must equal those of phi_in (they are not written). */
+int pm_batch_gsrb(pm_batch* batch, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
+                  pm_stream_t stream);
+
+/* ----------------------------------------------------------------- graphs */
+/* hipGraph capture of work issued on a (non-null) stream by any entry point
+ * above, and replay: the V-cycle and the batched operators are launch-bound
+ * sequences that replay as one submission.  No counterpart in the reference
+ * (its prototype, experimental/cuda-graph/cuda-graph.H, is not in the
+ * library). */
+typedef void* pm_graph_t;
+int pm_graph_begin(pm_stream_t stream);
+int pm_graph_end(pm_stream_t stream, pm_graph_t* graph);
+int pm_graph_launch(pm_graph_t graph, pm_stream_t stream);
+int pm_graph_destroy(pm_graph_t graph);
+
 /* ----------------------------------------------------------------- inputs */
 /* Seeded inputs shared with the tests: value i of the SplitMix64 stream
  * (seed, stream) mapped to [-1, 1); written to data[i] for i < n, as

@@ -569,6 +569,39 @@ int orc_mg_solve(int n, int nlev, int relax, int pre, int post, int bottom, int 
     return it;
 }
 
+#define PATCH_GSRB_BODY(T, C0, CN, LAM)                                                    \
+    long Gx = nx + 2 * g, Gxy = Gx * (ny + 2 * g);                                          \
+    T c[7] = {C0, CN, CN, CN, CN, CN, CN};                                                  \
+    const long off[7] = {0, 1, -1, Gx, -Gx, Gxy, -Gxy};                                     \
+    for (int color = 0; color < 2; color++)                                                 \
+    for (int z = 0; z < nz; z++)                                                            \
+    for (int y = 0; y < ny; y++)                                                            \
+    for (int x = (y + z + color) & 1; x < nx; x += 2)                                       \
+    {                                                                                       \
+        long p = (x + g) + Gx * (y + g) + Gxy * (z + g);                                    \
+        T t = 0;                                                                            \
+        for (int j = 0; j < 7; j++) { t += c[j] * phi[p + off[j]]; }                        \
+        t += (T)1.0 * rhs[x + (long)nx * (y + (long)ny * z)];                               \
+        phi[p] += (LAM) * t;                                                                \
+    }
+
+void orc_patch_gsrb_f64(int nx, int ny, int nz, int g, double dx, const double* rhs, double* phi)
+{
+    double lam = -(dx * dx) / (4.0 * 3);
+    lam *= 2.0;
+    double m = -1.0 / (dx * dx);
+    PATCH_GSRB_BODY(double, -6.0 * m, 1.0 * m, 1.0 * lam)
+}
+
+void orc_patch_gsrb_f32(int nx, int ny, int nz, int g, double dx, const float* rhs, float* phi)
+{
+    double lam = -(dx * dx) / (4.0 * 3);
+    lam *= 2.0;
+    double m = -1.0 / (dx * dx);
+    float fm = (float)m, flam = (float)(1.0 * lam);
+    PATCH_GSRB_BODY(float, -6.0f * fm, 1.0f * fm, flam)
+}
+
 /* SplitMix64 (Steele, Lea, Flood 2014), counter form: z_i = mix(s0 + (i+1)*G)
  * with s0 = seed + stream * 0x632BE59BD9B4E019. */
 static inline uint64_t mix64(uint64_t z)

@@ -91,6 +91,14 @@ void orc_fine_interp(int n, int linear, const double* delta, double* phi);
 int orc_mg_solve(int n, int nlev, int relax, int pre, int post, int bottom, int linear,
                  const double* rhs, double* phi, int maxiter, double tol, double* hist);
 
+/* One GSRB sweep (red then black) on an isolated patch of nx*ny*nz valid
+ * cells with g ghost layers held fixed (no exchange), colours by box-local
+ * (x+y+z) parity, the arithmetic of LaplaceOp.H:56-73, 95-123.  phi: ghosted
+ * x-fastest storage; rhs: valid storage.  The f32 form uses coefficients
+ * (float)Lap_j * (float)(-1/dx^2) and lambda (float)(2*(-(dx*dx)/12)). */
+void orc_patch_gsrb_f64(int nx, int ny, int nz, int g, double dx, const double* rhs, double* phi);
+void orc_patch_gsrb_f32(int nx, int ny, int nz, int g, double dx, const float* rhs, float* phi);
+
 /* ---- seeded inputs (SplitMix64; shared with the GPU input kernel) ---- */
 /* value i of stream (seed, stream): u = ((z >> 11) * 2^-53) * 2 - 1 */
 void orc_uniform_f64(uint64_t seed, uint64_t stream, long n, double* out);

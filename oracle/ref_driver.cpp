@@ -510,6 +510,52 @@ void ref_config2_step(int which, const double* u, const int* ulo, const int* uhi
     *sum = R.sum();
 }
 
+// One GSRB sweep on a single patch (one BoxData with `g` ghost layers held
+// fixed, no exchange), with the sub-lattice stencils of LaplaceOp.H:56-73
+// applied as its relax() does (95-123) -- colour by (x+y+z) parity of the
+// box-local index, the box's low corner being 0.
+void ref_patch_gsrb(int nx, int ny, int nz, int g, double dx, const double* rhs, double* phi)
+{
+    Box valid(Point::Zeros(), Point(nx - 1, ny - 1, nz - 1));
+    BoxData<double, 1, HOST> P(phi, valid.grow(g));
+    BoxData<double, 1, HOST> Rh(rhs, valid);
+    double lambda = -(dx * dx) / (4.0 * DIM);
+    lambda *= 2.0;
+    Stencil<double> Lap = Stencil<double>::Laplacian();
+    Stencil<double> GS[1 << DIM], GI[1 << DIM];
+    Box K = Box::Cube(2);
+    for (auto iter = K.begin(); iter != K.end(); ++iter)
+    {
+        int index = K.index(*iter);
+        GS[index] = Lap;
+        GS[index] *= (-1.0 / (dx * dx)) * Shift(*iter);
+        GS[index].srcRatio() = Point::Ones(2);
+        GS[index].destRatio() = Point::Ones(2);
+        GS[index].destShift() = *iter;
+        GI[index] = 1.0 * Shift(*iter);
+        GI[index].srcRatio() = Point::Ones(2);
+        GI[index].destRatio() = Point::Ones(2);
+        GI[index].destShift() = *iter;
+    }
+    for (int icolor = 0; icolor < 2; icolor++)
+    {
+        BoxData<double> temp(valid);
+        for (auto kiter = K.begin(); kiter != K.end(); ++kiter)
+        {
+            int color = 0;
+            for (int d = 0; d < DIM; d++) { color += (*kiter)[d]; }
+            color = color % 2;
+            if (color == icolor)
+            {
+                int index = K.index(*kiter);
+                temp |= GS[index](P);
+                temp += GI[index](Rh);
+                P += GI[index](temp, lambda);
+            }
+        }
+    }
+}
+
 // One GSRB relaxation pass (niter iterations) on a single level, no coarse grids.
 void ref_gsrb_relax(int n, int boxSize, int niter, const double* rhs, double* phi)
 {

@@ -120,6 +120,17 @@ SIGNATURES = {
     "pm_tune_table_load": (_i, [ctypes.c_char_p]),
     "pm_level_reduce": (_i, [_vp, _i, _vp, _ip, _i, _i, _i, _vp, _i, _vp]),
     "pm_level_add_separable": (_i, [_vp, _i, _vp, _ip, _d, _dpp, _dpp, _dpp, _vp]),
+    "pm_batch_create": (_i, [ctypes.POINTER(_vp), _i, _ip, _i]),
+    "pm_batch_destroy": (_i, [_vp]),
+    "pm_batch_elems": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "pm_batch_offsets": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "pm_batch_configure": (_i, [_vp, _i, _i, _i]),
+    "pm_batch_apply": (_i, [_vp, _vp, _vp, _vp, _d, _vp]),
+    "pm_batch_gsrb": (_i, [_vp, _i, _vp, _vp, _vp, _d, _vp]),
+    "pm_graph_begin": (_i, [_vp]),
+    "pm_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "pm_graph_launch": (_i, [_vp, _vp]),
+    "pm_graph_destroy": (_i, [_vp]),
     "pm_mg_default_options": (None, [_OP]),
     "pm_mg_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _d, _OP]),
     "pm_mg_destroy": (_i, [_vp]),

@@ -1,0 +1,506 @@
+// Batches of independent patches of uneven size (AMR boxes): one persistent
+// launch per operator for the whole batch.
+//
+// Proto applies an operator to a set of boxes as one launch per box
+// (`for (auto iter : layout) ...`, e.g. Proto_LevelMultigridSolverImplem.H:
+// 366-378).  Here a fixed grid of workgroups pulls tiles of every box from an
+// atomic work queue (largest tiles first); the queue counters reset
+// themselves at the end of each launch, so the launch can be captured in a
+// hipGraph and replayed without a memset node.
+//
+// Storage: box b (valid extents n = sizes[b], g ghost layers) at
+// ghosted_off[b] in the ghosted array (x fastest over [-g, n+g)) and at
+// valid_off[b] in the valid-cells array (x fastest over [0, n)).
+//
+// Operators (bit-identical per box to the single-box paths):
+//   apply: dst = sum_j c'_j src(p + o_j), Proto's coefficient order, for the
+//          7-point Laplacian() and 27-point Laplacian_27() shapes;
+//   gsrb:  one red-black Gauss-Seidel sweep with the ghosts held fixed (no
+//          exchange between independent patches): every red cell ((x+y+z)
+//          even, box-local indices) updated from old black neighbours, then
+//          every black cell from the new red ones -- the arithmetic of
+//          LaplaceOp.H:95-123.  Done in one pass, out of place (phi_in is
+//          only read, phi_out only written): walking z, a workgroup updates
+//          red cells of plane k+1 and then black cells of plane k in LDS,
+//          recomputing the red cells of the rows just outside its tile
+//          (identical values to the neighbouring tile's) so that no tile
+//          waits for another.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pm_internal.hpp"
+
+struct pm_batch {
+    int nboxes, ghost;
+    std::vector<int> sizes;
+    std::vector<size_t> goff, voff;  // nboxes + 1
+    void* d_boxes = nullptr;         // BatchBox[nboxes]
+    void* d_items = nullptr;         // BatchItem[] (apply)
+    int nitems_apply = 0, apply_rows = 0, apply_tz = 0;
+    void* d_gitems = nullptr;        // BatchItem[] (gsrb)
+    int nitems_gsrb = 0, gsrb_rows = 0;
+    unsigned* d_counter = nullptr;   // 2 x (queue head, blocks done) per operator
+    int cfg_apply_rows = 0, cfg_apply_tz = 0, cfg_gsrb_rows = 8, cfg_blocks_per_cu = 4;
+};
+
+namespace pm {
+
+struct BatchBox {
+    int nx, ny, nz, pad;
+    long goff, voff;
+};
+
+struct BatchItem {
+    int box, x0, y0, z0, z1, rows, xw, pad;
+};
+
+template<typename T>
+struct BCoefs {
+    T c[27];
+};
+
+// ------------------------------------------------------------ work queue
+__device__ inline int next_item(unsigned* counter, int* s_item)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) { *s_item = (int)atomicAdd(&counter[0], 1u); }
+    __syncthreads();
+    return *s_item;
+}
+
+__device__ inline void queue_done(unsigned* counter)
+{
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned done = atomicAdd(&counter[1], 1u);
+        if (done == gridDim.x - 1) {
+            // every workgroup has drawn its last item: reset for the next launch
+            atomicExch(&counter[0], 0u);
+            atomicExch(&counter[1], 0u);
+            __threadfence();
+        }
+    }
+}
+
+// ------------------------------------------------------------ batched apply
+constexpr int BA_LDS = 400;  // (rows + 2) x (xw + 2) for any tile shape used
+
+template<typename T, int SHAPE, int NT>
+__global__ __launch_bounds__(256) void k_batch_apply(const BatchBox* __restrict__ boxes,
+                                                     const BatchItem* __restrict__ items, int nitems, int g,
+                                                     const T* __restrict__ src, T* __restrict__ dst, BCoefs<T> c,
+                                                     unsigned* counter)
+{
+    constexpr int NC = SHAPE == 7 ? 7 : 27;
+    __shared__ T sm[2][BA_LDS];
+    __shared__ int s_item;
+    const int tid = threadIdx.x;
+    for (;;) {
+        const int it = next_item(counter, &s_item);
+        if (it >= nitems) { break; }
+        const BatchItem wi = items[it];
+        const BatchBox bx = boxes[wi.box];
+        const int XW = wi.xw, R = wi.rows, LX = XW + 2, NE = (R + 2) * LX;
+        const long Gx = bx.nx + 2 * g, Gxy = Gx * (bx.ny + 2 * g);
+        const T* s = src + bx.goff;
+        T* d = dst + bx.voff;
+        const int col = tid % XW, row = tid / XW;
+        const int x = wi.x0 + col, y = wi.y0 + row;
+        const bool active = row < R && x < bx.nx && y < bx.ny;
+        // up to two plane elements per thread (NE <= 512)
+        long eoff[2];
+        bool eok[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int e = tid + 256 * k;
+            const int er = e / LX, ec = e - er * LX;
+            const int gx = wi.x0 - 1 + ec, gy = wi.y0 - 1 + er;
+            eok[k] = e < NE && gx >= -g && gx < bx.nx + g && gy >= -g && gy < bx.ny + g;
+            eoff[k] = (long)(gx + g) + Gx * (long)(gy + g);
+        }
+        T pre[2];
+        auto load = [&](int z) {
+            const T* base = s + Gxy * (long)(z + g);
+#pragma unroll
+            for (int k = 0; k < 2; k++) { pre[k] = eok[k] ? base[eoff[k]] : (T)0; }
+        };
+        auto store = [&](int buf) {
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int e = tid + 256 * k;
+                if (e < NE) { sm[buf][e] = pre[k]; }
+            }
+        };
+        T q[3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int e = 0; e < 3; e++) q[a][b][e] = (T)0;
+        int buf = 0;
+        load(wi.z0 - 1);
+        for (int zz = wi.z0 - 1; zz <= wi.z1; ++zz) {
+            // one barrier per plane: buffers alternate, and a thread can only
+            // overwrite buffer b after every thread passed the barrier that
+            // follows its last read of b
+            store(buf);
+            if (zz + 1 <= wi.z1) { load(zz + 1); }
+            __syncthreads();
+            if (row < R) {
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+#pragma unroll
+                    for (int cc = 0; cc < 3; cc++) {
+                        q[0][r][cc] = q[1][r][cc];
+                        q[1][r][cc] = q[2][r][cc];
+                        q[2][r][cc] = sm[buf][(row + r) * LX + col + cc];
+                    }
+            }
+            if (zz >= wi.z0 + 1 && active) {
+                T acc = 0;
+#pragma unroll
+                for (int j = 0; j < NC; j++) {
+                    const int dx = shape_off<SHAPE>(j, 0), dy = shape_off<SHAPE>(j, 1), dz = shape_off<SHAPE>(j, 2);
+                    acc = acc + c.c[j] * q[1 + dz][1 + dy][1 + dx];
+                }
+                T* o = d + x + (long)bx.nx * (y + (long)bx.ny * (zz - 1));
+                if (NT) { __builtin_nontemporal_store(acc, o); } else { *o = acc; }
+            }
+            buf ^= 1;
+        }
+    }
+    queue_done(counter);
+}
+
+// ------------------------------------------------------------ batched GSRB
+// LDS: 4 planes (k-1 .. k+2) of (rows + 4) x (nx + 2) cells.
+template<typename T>
+__global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__ boxes,
+                                                    const BatchItem* __restrict__ items, int nitems, int g,
+                                                    const T* __restrict__ phi_in, T* __restrict__ phi_out,
+                                                    const T* __restrict__ rhs, BCoefs<T> c, T lambda,
+                                                    unsigned* counter)
+{
+    extern __shared__ char smraw[];
+    T* sm = reinterpret_cast<T*>(smraw);  // 4 * (rows + 4) * (max width + 2) elements
+    __shared__ int s_item;
+    const int tid = threadIdx.x;
+    for (;;) {
+        const int it = next_item(counter, &s_item);
+        if (it >= nitems) { break; }
+        const BatchItem wi = items[it];
+        const BatchBox bx = boxes[wi.box];
+        const int R = wi.rows, LY = R + 4, LX = bx.nx + 2, NP = LY * LX;  // NP * 4 <= lds_elems
+        const long Gx = bx.nx + 2 * g, Gxy = Gx * (bx.ny + 2 * g);
+        const T* p = phi_in + bx.goff;
+        T* po = phi_out + bx.goff;
+        const T* rh = rhs + bx.voff;
+        const int y0 = wi.y0;
+        // plane element e -> (row y0 - 2 + e / LX, col -1 + e % LX)
+        auto gload = [&](int z, T* pl) {
+            if (z < -1 || z > bx.nz) { return; }
+            const T* base = p + Gxy * (long)(z + g);
+            for (int e = tid; e < NP; e += 256) {
+                const int er = e / LX, ec = e - er * LX;
+                const int gy = y0 - 2 + er, gx = ec - 1;
+                T v = 0;
+                if (gy >= -g && gy < bx.ny + g) { v = base[(gx + g) + Gx * (long)(gy + g)]; }
+                pl[e] = v;
+            }
+        };
+        auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
+        // half sweep on plane z, rows [ya, yb), colour col: p += lambda (sum c_j p(nbr) + rhs)
+        auto update = [&](int z, int ya, int yb, int colour) {
+            if (z < 0 || z >= bx.nz) { return; }
+            ya = max(ya, 0);
+            yb = min(yb, bx.ny);
+            const int hx = (bx.nx + 1) / 2;
+            const int ncell = (yb - ya) * hx;
+            T* zc = slot(z);
+            T* zm = slot(z - 1);
+            T* zp = slot(z + 1);
+            for (int cidx = tid; cidx < ncell; cidx += 256) {
+                const int yy = ya + cidx / hx;
+                const int xx = 2 * (cidx % hx) + ((yy + z + colour) & 1);
+                if (xx >= bx.nx) { continue; }
+                const int li = (yy - (y0 - 2)) * LX + (xx + 1);
+                T s = 0;
+                s = s + c.c[0] * zc[li];
+                s = s + c.c[1] * zc[li + 1];
+                s = s + c.c[2] * zc[li - 1];
+                s = s + c.c[3] * zc[li + LX];
+                s = s + c.c[4] * zc[li - LX];
+                s = s + c.c[5] * zp[li];
+                s = s + c.c[6] * zm[li];
+                s = s + rh[xx + (long)bx.nx * (yy + (long)bx.ny * z)];
+                zc[li] = zc[li] + lambda * s;
+            }
+        };
+        // prologue: planes -1 and 0 (plane 1 arrives in the first step)
+        gload(-1, slot(-1));
+        gload(0, slot(0));
+        for (int k = -1; k < bx.nz; k++) {
+            // plane k+2 into the slot of plane k-2 (no longer read)
+            gload(k + 2, slot(k + 2));
+            __syncthreads();
+            update(k + 1, y0 - 1, y0 + R + 1, 0);  // red: tile rows and one row each side
+            __syncthreads();
+            update(k, y0, y0 + R, 1);              // black: tile rows
+            __syncthreads();
+            if (k >= 0) {
+                const T* pl = slot(k);
+                T* base = po + Gxy * (long)(k + g);
+                const int ny_t = min(R, bx.ny - y0);
+                for (int e = tid; e < ny_t * bx.nx; e += 256) {
+                    const int yy = y0 + e / bx.nx, xx = e % bx.nx;
+                    base[(xx + g) + Gx * (long)(yy + g)] = pl[(yy - (y0 - 2)) * LX + xx + 1];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    queue_done(counter);
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+namespace {
+
+int upload(void** dptr, const void* host, size_t bytes)
+{
+    if (*dptr) { hipFree(*dptr); *dptr = nullptr; }
+    if (bytes == 0) { return PM_OK; }
+    if (hipMalloc(dptr, bytes) != hipSuccess) {
+        set_error("batch: device allocation of %zu bytes failed", bytes);
+        return PM_ERR_NOMEM;
+    }
+    if (hipMemcpy(*dptr, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("batch: upload failed");
+        return PM_ERR_HIP;
+    }
+    return PM_OK;
+}
+
+// Apply tiles: x in steps of 64, rows = min(32, 256 / width), z chunks of tz
+// (0 = whole box); largest first.
+int build_apply_items(pm_batch* B, int tz)
+{
+    std::vector<BatchItem> items;
+    for (int b = 0; b < B->nboxes; b++) {
+        const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1], nz = B->sizes[3 * b + 2];
+        for (int x0 = 0; x0 < nx; x0 += 64) {
+            const int xw = std::min(64, nx - x0);
+            const int rows = std::max(1, std::min(32, 256 / xw));
+            for (int y0 = 0; y0 < ny; y0 += rows) {
+                const int zc = tz > 0 ? tz : nz;
+                for (int z0 = 0; z0 < nz; z0 += zc) {
+                    items.push_back(BatchItem{b, x0, y0, z0, std::min(nz, z0 + zc), rows, xw, 0});
+                }
+            }
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const BatchItem& a, const BatchItem& b) {
+        return (long)a.xw * a.rows * (a.z1 - a.z0) > (long)b.xw * b.rows * (b.z1 - b.z0);
+    });
+    B->nitems_apply = (int)items.size();
+    B->apply_tz = tz;
+    return upload(&B->d_items, items.data(), sizeof(BatchItem) * items.size());
+}
+
+int build_gsrb_items(pm_batch* B, int rows)
+{
+    std::vector<BatchItem> items;
+    for (int b = 0; b < B->nboxes; b++) {
+        const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1], nz = B->sizes[3 * b + 2];
+        for (int y0 = 0; y0 < ny; y0 += rows) {
+            items.push_back(BatchItem{b, 0, y0, 0, nz, rows, nx, 0});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const BatchItem& a, const BatchItem& b) {
+        return (long)a.xw * a.rows * a.z1 > (long)b.xw * b.rows * b.z1;
+    });
+    B->nitems_gsrb = (int)items.size();
+    B->gsrb_rows = rows;
+    return upload(&B->d_gitems, items.data(), sizeof(BatchItem) * items.size());
+}
+
+unsigned persistent_blocks(const pm_batch* B)
+{
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) {
+            ncu = prop.multiProcessorCount;
+        }
+    }
+    return (unsigned)(ncu * B->cfg_blocks_per_cu);
+}
+
+template<typename T>
+void coefs_of(const std::vector<double>& c, double scale, BCoefs<T>& out)
+{
+    const T sc = (T)scale;
+    for (size_t j = 0; j < c.size() && j < 27; j++) {
+        T v = (T)c[j];
+        if (sc != (T)1) { v = v * sc; }
+        out.c[j] = v;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_batch_create(pm_batch** out, int nboxes, const int* sizes_host, int ghost)
+{
+    PM_REQUIRE(out && sizes_host && nboxes >= 1, "pm_batch_create: bad argument");
+    PM_REQUIRE(ghost >= 1 && ghost <= 4, "pm_batch_create: ghost must be 1..4");
+    *out = nullptr;
+    pm_batch* B = new pm_batch();
+    B->nboxes = nboxes;
+    B->ghost = ghost;
+    B->sizes.assign(sizes_host, sizes_host + 3 * nboxes);
+    B->goff.assign(nboxes + 1, 0);
+    B->voff.assign(nboxes + 1, 0);
+    std::vector<BatchBox> boxes(nboxes);
+    for (int b = 0; b < nboxes; b++) {
+        const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1], nz = B->sizes[3 * b + 2];
+        if (nx < 1 || ny < 1 || nz < 1 || nx > 4096 || ny > 4096 || nz > 4096) {
+            delete B;
+            set_error("pm_batch_create: box %d has bad extents", b);
+            return PM_ERR_INVALID;
+        }
+        boxes[b] = BatchBox{nx, ny, nz, 0, (long)B->goff[b], (long)B->voff[b]};
+        B->goff[b + 1] = B->goff[b] + (size_t)(nx + 2 * ghost) * (ny + 2 * ghost) * (nz + 2 * ghost);
+        B->voff[b + 1] = B->voff[b] + (size_t)nx * ny * nz;
+    }
+    int rc = upload(&B->d_boxes, boxes.data(), sizeof(BatchBox) * nboxes);
+    if (rc == PM_OK && hipMalloc((void**)&B->d_counter, 4 * sizeof(unsigned)) != hipSuccess) { rc = PM_ERR_NOMEM; }
+    if (rc == PM_OK && hipMemset(B->d_counter, 0, 4 * sizeof(unsigned)) != hipSuccess) { rc = PM_ERR_HIP; }
+    if (rc == PM_OK) { rc = build_apply_items(B, 0); }
+    if (rc == PM_OK) { rc = build_gsrb_items(B, B->cfg_gsrb_rows); }
+    if (rc != PM_OK) {
+        pm_batch_destroy(B);
+        return rc;
+    }
+    *out = B;
+    return PM_OK;
+}
+
+int pm_batch_destroy(pm_batch* B)
+{
+    if (!B) { return PM_OK; }
+    hipFree(B->d_boxes);
+    hipFree(B->d_items);
+    hipFree(B->d_gitems);
+    hipFree(B->d_counter);
+    delete B;
+    return PM_OK;
+}
+
+int pm_batch_elems(const pm_batch* B, size_t* ghosted, size_t* valid)
+{
+    PM_REQUIRE(B && ghosted && valid, "pm_batch_elems: null argument");
+    *ghosted = B->goff[B->nboxes];
+    *valid = B->voff[B->nboxes];
+    return PM_OK;
+}
+
+int pm_batch_offsets(const pm_batch* B, size_t* ghosted_off, size_t* valid_off)
+{
+    PM_REQUIRE(B && ghosted_off && valid_off, "pm_batch_offsets: null argument");
+    for (int b = 0; b <= B->nboxes; b++) {
+        ghosted_off[b] = B->goff[b];
+        valid_off[b] = B->voff[b];
+    }
+    return PM_OK;
+}
+
+int pm_batch_configure(pm_batch* B, int apply_zplanes, int gsrb_rows, int blocks_per_cu)
+{
+    PM_REQUIRE(B && apply_zplanes >= 0 && gsrb_rows >= 1 && gsrb_rows <= 32 && blocks_per_cu >= 1 && blocks_per_cu <= 16,
+               "pm_batch_configure: bad argument");
+    B->cfg_blocks_per_cu = blocks_per_cu;
+    int rc = PM_OK;
+    if (apply_zplanes != B->apply_tz) { rc = build_apply_items(B, apply_zplanes); }
+    if (rc == PM_OK && gsrb_rows != B->gsrb_rows) { rc = build_gsrb_items(B, gsrb_rows); }
+    return rc;
+}
+
+int pm_batch_apply(pm_batch* B, const pm_stencil* st, const void* src, void* dst, double scale, pm_stream_t stream)
+{
+    PM_REQUIRE(B && st && src && dst, "pm_batch_apply: null argument");
+    int kind = 0, dtype = 0;
+    std::vector<double> cv;
+    int rc = stencil_info(st, &kind, &dtype, &cv);
+    if (rc) { return rc; }
+    PM_REQUIRE(kind == 7 || kind == 27, "pm_batch_apply: needs the 7-point or 27-point Laplacian shape");
+    const unsigned nb = persistent_blocks(B);
+    if (dtype == PM_F64) {
+        BCoefs<double> cc;
+        coefs_of<double>(cv, scale, cc);
+        if (kind == 7) {
+            hipLaunchKernelGGL((k_batch_apply<double, 7, 1>), dim3(nb), dim3(256), 0, S(stream),
+                               (const BatchBox*)B->d_boxes, (const BatchItem*)B->d_items, B->nitems_apply, B->ghost,
+                               (const double*)src, (double*)dst, cc, B->d_counter);
+        } else {
+            hipLaunchKernelGGL((k_batch_apply<double, 27, 1>), dim3(nb), dim3(256), 0, S(stream),
+                               (const BatchBox*)B->d_boxes, (const BatchItem*)B->d_items, B->nitems_apply, B->ghost,
+                               (const double*)src, (double*)dst, cc, B->d_counter);
+        }
+    } else {
+        BCoefs<float> cc;
+        coefs_of<float>(cv, scale, cc);
+        if (kind == 7) {
+            hipLaunchKernelGGL((k_batch_apply<float, 7, 1>), dim3(nb), dim3(256), 0, S(stream),
+                               (const BatchBox*)B->d_boxes, (const BatchItem*)B->d_items, B->nitems_apply, B->ghost,
+                               (const float*)src, (float*)dst, cc, B->d_counter);
+        } else {
+            hipLaunchKernelGGL((k_batch_apply<float, 27, 1>), dim3(nb), dim3(256), 0, S(stream),
+                               (const BatchBox*)B->d_boxes, (const BatchItem*)B->d_items, B->nitems_apply, B->ghost,
+                               (const float*)src, (float*)dst, cc, B->d_counter);
+        }
+    }
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
+                  pm_stream_t stream)
+{
+    PM_REQUIRE(B && phi_in && phi_out && rhs, "pm_batch_gsrb: null argument");
+    PM_REQUIRE(phi_in != phi_out, "pm_batch_gsrb: the sweep is out of place (phi_in != phi_out)");
+    PM_REQUIRE(dtype == PM_F32 || dtype == PM_F64, "pm_batch_gsrb: bad dtype");
+    int maxnx = 0;
+    for (int b = 0; b < B->nboxes; b++) { maxnx = std::max(maxnx, B->sizes[3 * b]); }
+    PM_REQUIRE(maxnx <= 256, "pm_batch_gsrb: boxes wider than 256 cells are not supported");
+    const int lds_elems = 4 * (B->gsrb_rows + 4) * (maxnx + 2);
+    const size_t lds = (size_t)lds_elems * (dtype == PM_F64 ? 8 : 4);
+    PM_REQUIRE(lds <= 64 * 1024, "pm_batch_gsrb: tile too large for LDS (rows %d, width %d)", B->gsrb_rows, maxnx);
+    const unsigned nb = persistent_blocks(B);
+    // LaplaceOp::init: coefficients Laplacian()_j * (-1/(dx*dx)), lambda = 2 * (-(dx*dx)/(4 DIM))
+    double lam = -(dx * dx) / (4.0 * 3);
+    lam *= 2.0;
+    const double m = -1.0 / (dx * dx);
+    if (dtype == PM_F64) {
+        BCoefs<double> cc;
+        for (int j = 0; j < 7; j++) { cc.c[j] = (j == 0 ? -6.0 : 1.0) * m; }
+        hipLaunchKernelGGL((k_batch_gsrb<double>), dim3(nb), dim3(256), lds, S(stream), (const BatchBox*)B->d_boxes,
+                           (const BatchItem*)B->d_gitems, B->nitems_gsrb, B->ghost, (const double*)phi_in,
+                           (double*)phi_out, (const double*)rhs, cc, 1.0 * lam, B->d_counter + 2);
+    } else {
+        BCoefs<float> cc;
+        for (int j = 0; j < 7; j++) { cc.c[j] = (float)(j == 0 ? -6.0 : 1.0) * (float)m; }
+        hipLaunchKernelGGL((k_batch_gsrb<float>), dim3(nb), dim3(256), lds, S(stream), (const BatchBox*)B->d_boxes,
+                           (const BatchItem*)B->d_gitems, B->nitems_gsrb, B->ghost, (const float*)phi_in,
+                           (float*)phi_out, (const float*)rhs, cc, (float)(1.0 * lam), B->d_counter + 2);
+    }
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+}  // extern "C"

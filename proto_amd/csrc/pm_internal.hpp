@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "protomi.h"
 
@@ -25,6 +26,8 @@ int workspace(hipStream_t stream, size_t bytes, void** out);
 int layout_shape(const pm_layout* L, int* dim, int* domain, int* bs, int* periodic, int* nboxes);
 // Box low corners in layout order (3 ints per box).
 int layout_lows(const pm_layout* L, int* lows);
+// Kernel family (0 generic, 7, 27), dtype and coefficients of a stencil (pm_stencil.hip).
+int stencil_info(const pm_stencil* st, int* kind, int* dtype, std::vector<double>* coefs);
 
 #define PM_CHECK_HIP(expr)                                                        \
     do {                                                                          \
@@ -90,6 +93,26 @@ __host__ __device__ inline long box_index(const Box3& b, int x, int y, int z)
 inline bool box_valid(const pm_box* b)
 {
     return b != nullptr;
+}
+
+// Offsets of Stencil<T>::Laplacian() in its coefficient order
+// (Proto_StencilDefs.H:77-87): centre, +x, -x, +y, -y, +z, -z.
+__host__ __device__ constexpr int lap7_off(int j, int d)
+{
+    constexpr int t[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    return t[j][d];
+}
+// Offsets of Stencil<T>::Laplacian_27() (Proto_StencilDefs.H:130-149): the
+// centre, then Box::Cube(3) shifted by -1 in x-fastest order without it.
+__host__ __device__ constexpr int lap27_off(int j, int d)
+{
+    int k = (j == 0) ? 13 : ((j - 1) >= 13 ? j : j - 1);
+    int v = (d == 0) ? (k % 3) : (d == 1 ? (k / 3) % 3 : k / 9);
+    return v - 1;
+}
+template<int SHAPE> __host__ __device__ constexpr int shape_off(int j, int d)
+{
+    return SHAPE == 7 ? lap7_off(j, d) : lap27_off(j, d);
 }
 
 // XCD-aware block remap: MI355X dispatches workgroup i to XCD (i mod 8); this

@@ -282,6 +282,52 @@ int pm_event_elapsed_ms(pm_event_t start, pm_event_t stop, float* ms)
     return PM_OK;
 }
 
+struct GraphBox {
+    hipGraph_t g;
+    hipGraphExec_t e;
+};
+
+int pm_graph_begin(pm_stream_t stream)
+{
+    PM_REQUIRE(stream, "pm_graph_begin: capture needs a non-null stream");
+    PM_CHECK_HIP(hipStreamBeginCapture(S(stream), hipStreamCaptureModeThreadLocal));
+    return PM_OK;
+}
+
+int pm_graph_end(pm_stream_t stream, pm_graph_t* graph)
+{
+    PM_REQUIRE(stream && graph, "pm_graph_end: null argument");
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    PM_CHECK_HIP(hipStreamEndCapture(S(stream), &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        hipGraphDestroy(g);
+        set_error("pm_graph_end: instantiate failed: %s", hipGetErrorString(err));
+        return PM_ERR_HIP;
+    }
+    *graph = (pm_graph_t) new GraphBox{g, e};
+    return PM_OK;
+}
+
+int pm_graph_launch(pm_graph_t graph, pm_stream_t stream)
+{
+    PM_REQUIRE(graph, "pm_graph_launch: null graph");
+    PM_CHECK_HIP(hipGraphLaunch(((GraphBox*)graph)->e, S(stream)));
+    return PM_OK;
+}
+
+int pm_graph_destroy(pm_graph_t graph)
+{
+    if (!graph) { return PM_OK; }
+    GraphBox* b = (GraphBox*)graph;
+    hipGraphExecDestroy(b->e);
+    hipGraphDestroy(b->g);
+    delete b;
+    return PM_OK;
+}
+
 int pm_fill_separable(int dtype, void* data, const pm_box* box, const pm_box* region, const int period[3],
                       const double* tx_host, const double* ty_host, const double* tz_host, double amp,
                       uint64_t seed, uint64_t stream_id, pm_stream_t stream)

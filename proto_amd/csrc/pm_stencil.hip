@@ -29,26 +29,6 @@ struct pm_stencil {
 namespace pm {
 
 // ------------------------------------------------------------ shapes
-// Offsets of Stencil<T>::Laplacian() in its coefficient order
-// (Proto_StencilDefs.H:77-87): centre, +x, -x, +y, -y, +z, -z.
-__host__ __device__ constexpr int lap7_off(int j, int d)
-{
-    constexpr int t[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-    return t[j][d];
-}
-// Offsets of Stencil<T>::Laplacian_27() (Proto_StencilDefs.H:130-149): the
-// centre, then Box::Cube(3) shifted by -1 in x-fastest order without it.
-__host__ __device__ constexpr int lap27_off(int j, int d)
-{
-    int k = (j == 0) ? 13 : ((j - 1) >= 13 ? j : j - 1);
-    int v = (d == 0) ? (k % 3) : (d == 1 ? (k / 3) % 3 : k / 9);
-    return v - 1;
-}
-template<int SHAPE> __host__ __device__ constexpr int shape_off(int j, int d)
-{
-    return SHAPE == 7 ? lap7_off(j, d) : lap27_off(j, d);
-}
-
 static int detect_kind(const pm_stencil* s)
 {
     bool unit = true;
@@ -604,6 +584,15 @@ static int residual2_t(const pm_stencil* st1, const pm_stencil* st2, const void*
         hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)nb, 4, (T*)red_out);
         PM_CHECK_LAUNCH();
     }
+    return PM_OK;
+}
+
+int stencil_info(const pm_stencil* st, int* kind, int* dtype, std::vector<double>* coefs)
+{
+    PM_REQUIRE(st, "null stencil");
+    *kind = st->kind;
+    *dtype = st->dtype;
+    *coefs = st->coefs;
     return PM_OK;
 }
 

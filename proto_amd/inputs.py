@@ -32,3 +32,48 @@ def config2_inputs(n, g, dtype=np.float64, stream=None):
             s.ctypes.data_as(_dp), s.ctypes.data_as(_dp), s.ctypes.data_as(_dp), 1e-3, 1, 0, stream)
     f = uniform(n ** 3, 1, 1, dtype, stream)
     return u, f
+
+
+def _splitmix_uniform(seed, stream_id, n):
+    """The pm_fill_uniform stream on the host (small counts)."""
+    M = (1 << 64) - 1
+    s0 = (seed + stream_id * 0x632BE59BD9B4E019) & M
+    out = np.empty(n)
+    for i in range(n):
+        z = (s0 + (i + 1) * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out[i] = ((z >> 11) * 2.0 ** -53) * 2.0 - 1.0
+    return out
+
+
+def gaussian_params(ngauss, seed):
+    """Centres in [0,1)^3 and amplitudes +-1 of the Gaussian right-hand sides."""
+    u = _splitmix_uniform(seed, 7, 4 * ngauss)
+    cen = (u[:3 * ngauss].reshape(ngauss, 3) + 1.0) * 0.5
+    amp = np.where(u[3 * ngauss:] >= 0, 1.0, -1.0)
+    return cen, amp
+
+
+def gaussian_level_rhs(layout, n, data, ngauss, seed, sigma=0.05, stream=None):
+    """rhs = sum_k a_k exp(-|x - c_k|^2_periodic / (2 sigma^2)) - mean, on the
+    valid cells of level data (ghost 0) of a periodic n^3 layout, built on the
+    device as a sum of separable products (pm_level_add_separable)."""
+    cen, amp = gaussian_params(ngauss, seed)
+    x = (np.arange(n) + 0.5) / n
+    g0 = pa.i3([0, 0, 0])
+    for k in range(ngauss):
+        tabs = []
+        for a in range(3):
+            d = np.minimum(np.abs(x - cen[k, a]), 1.0 - np.abs(x - cen[k, a]))
+            tabs.append(np.ascontiguousarray(np.exp(-(d * d) / (2 * sigma * sigma))))
+        pa.call("pm_level_add_separable", layout, pa.PM_F64, data.ptr, g0, float(amp[k]),
+                tabs[0].ctypes.data_as(_dp), tabs[1].ctypes.data_as(_dp), tabs[2].ctypes.data_as(_dp), stream)
+    tot = DeviceArray.zeros(1)
+    pa.call("pm_level_reduce", layout, pa.PM_F64, data.ptr, g0, 1, 0, pa.RED_SUM, tot.ptr, 0, stream)
+    mean = tot.numpy(stream)[0] / float(n) ** 3
+    ones = np.ascontiguousarray(np.ones(n))
+    pa.call("pm_level_add_separable", layout, pa.PM_F64, data.ptr, g0, -mean, ones.ctypes.data_as(_dp),
+            ones.ctypes.data_as(_dp), ones.ctypes.data_as(_dp), stream)
+    return mean

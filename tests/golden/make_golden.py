@@ -233,6 +233,13 @@ def multigrid():
     rhs2 = uniform(70, 1, n ** 3)
     R3.ref_gsrb_relax(n, 16, 3, ptr(rhs2), ptr(phi))
     out["relax:phi"] = phi
+    # one GSRB sweep on isolated patches with fixed ghosts (odd and even extents)
+    for (nx, ny, nz, g) in ((7, 5, 6, 2), (8, 8, 8, 1), (13, 4, 9, 2)):
+        key = "patch_%d_%d_%d_%d" % (nx, ny, nz, g)
+        ph = uniform(80 + nx, 0, (nx + 2 * g) * (ny + 2 * g) * (nz + 2 * g))
+        rh = uniform(80 + nx, 1, nx * ny * nz)
+        R3.ref_patch_gsrb(nx, ny, nz, g, ctypes.c_double(1.0 / 32), ptr(rh), ptr(ph))
+        out[key] = ph
     np.savez_compressed(os.path.join(HERE, "multigrid.npz"), **out)
     with open(os.path.join(HERE, "multigrid.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)

@@ -383,3 +383,28 @@ def test_uniform_generator():
     z ^= z >> 31
     assert z == 0xE220A8397B1DCDAF
     assert uniform(0, 0, 1)[0] == ((z >> 11) * 2.0 ** -53) * 2 - 1
+
+
+@pytest.mark.parametrize("nx,ny,nz,g", [(7, 5, 6, 2), (8, 8, 8, 1), (13, 4, 9, 2)])
+def test_patch_gsrb_golden_and_invariant(nx, ny, nz, g):
+    z = load("multigrid.npz")
+    ph = uniform(80 + nx, 0, (nx + 2 * g) * (ny + 2 * g) * (nz + 2 * g))
+    rh = uniform(80 + nx, 1, nx * ny * nz)
+    dx = 1.0 / 32
+    util.oracle().orc_patch_gsrb_f64(nx, ny, nz, g, ctypes.c_double(dx), ptr(rh), ptr(ph))
+    assert ph.tobytes() == z["patch_%d_%d_%d_%d" % (nx, ny, nz, g)].tobytes()
+    # black cells (updated last, from final red neighbours and fixed ghosts)
+    # satisfy the discrete equation to rounding; red cells do not
+    P = ph.reshape(nz + 2 * g, ny + 2 * g, nx + 2 * g)
+    c = P[g:-g, g:-g, g:-g]
+    lap = (P[g:-g, g:-g, g + 1:nx + g + 1] + P[g:-g, g:-g, g - 1:nx + g - 1] + P[g:-g, g + 1:ny + g + 1, g:-g]
+           + P[g:-g, g - 1:ny + g - 1, g:-g] + P[g + 1:nz + g + 1, g:-g, g:-g] + P[g - 1:nz + g - 1, g:-g, g:-g]
+           - 6 * c) / (dx * dx)
+    res = lap - rh.reshape(nz, ny, nx)
+    par = (np.arange(nx)[None, None, :] + np.arange(ny)[None, :, None] + np.arange(nz)[:, None, None]) % 2
+    assert np.abs(res[par == 1]).max() < 1e-9 * (np.abs(lap).max() + 1)
+    assert np.abs(res[par == 0]).max() > 1e-3
+    # the f32 form stays within float rounding of the f64 sweep
+    ph32 = uniform(80 + nx, 0, ph.size).astype(np.float32)
+    util.oracle().orc_patch_gsrb_f32(nx, ny, nz, g, ctypes.c_double(dx), ptr(rh.astype(np.float32)), ptr(ph32))
+    assert np.abs(ph32 - ph).max() < 1e-4 * (np.abs(ph).max() + 1)
