@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tile configurations)")
     ap.add_argument("--no-mg", action="store_true", help="skip the config-3 multigrid measurement")
     ap.add_argument("--mg-n", type=int, default=512)
+    ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched-patch measurement")
     return ap.parse_args()
 
 
@@ -185,6 +186,73 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3):
     out["speedup_fp32_ir"] = round(out["fp64"]["solve_s"] / out["fp32_ir"]["solve_s"], 3)
     pa.call("pm_layout_destroy", lay)
     return out
+
+
+def batch_config4(pa, stream, dtype, reps=10, nboxes=2000):
+    """Config 4: nboxes patches, per-axis extents iid from {16,24,32,48,64}
+    (seed 3), 2 ghost layers, inputs U[-1,1) over the ghosted boxes (seed 3,
+    stream 0; rhs stream 2).  Operators: Laplacian_27 apply (scale 1) and one
+    GSRB sweep, each ONE persistent launch over every box, tiles chosen by the
+    library's autotuner, then both captured in one graph and replayed.
+    Algorithmic bytes per box: the stencil footprint (n+2)^3 read once plus
+    n^3 written (apply), plus n^3 of rhs read (sweep)."""
+    import numpy as np
+    from proto_amd.device import DeviceArray
+    from proto_amd.inputs import batch_sizes, uniform
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    esz = np.dtype(npdt).itemsize
+    sp = stream.h
+    sizes = batch_sizes(nboxes, (16, 24, 32, 48, 64), 3)
+    B = pa.Batch(sizes, 2)
+    src = uniform(B.ghosted_elems, 3, 0, npdt, sp)
+    rhs = uniform(B.valid_elems, 3, 2, npdt, sp)
+    dst = DeviceArray(B.valid_elems, npdt)
+    phi = DeviceArray(B.ghosted_elems, npdt)
+    pa.call("pm_memcpy", phi.ptr, src.ptr, phi.nbytes, 2, sp)
+    st27 = stencils(dtype)[1]
+    dx = 1.0 / 64
+    valid = int(np.prod(sizes, axis=1).sum())
+    foot = int(np.prod(sizes + 2, axis=1).sum())
+    apply_bytes = esz * (foot + valid)
+    gsrb_bytes = esz * (foot + 2 * valid)
+    cfg, tms = B.autotune(st27, src.ptr, dst.ptr, phi.ptr, rhs.ptr, dx, 5, sp)
+    ms_a = time_launches(lambda: B.apply(st27, src.ptr, dst.ptr, 1.0, sp), reps, stream)
+    ms_g = time_launches(lambda: B.gsrb(dtype, src.ptr, phi.ptr, rhs.ptr, dx, sp), reps, stream)
+    g = ctypes.c_void_p()
+    pa.call("pm_graph_begin", sp)
+    B.apply(st27, src.ptr, dst.ptr, 1.0, sp)
+    B.gsrb(dtype, src.ptr, phi.ptr, rhs.ptr, dx, sp)
+    pa.call("pm_graph_end", sp, ctypes.byref(g))
+    ms_graph = time_launches(lambda: pa.call("pm_graph_launch", g, sp), reps, stream)
+    pa.call("pm_graph_destroy", g)
+    # Proto's pattern for comparison: one single-box apply launch per patch
+    per_box = []
+    for b in range(nboxes):
+        nx, ny, nz = (int(v) for v in sizes[b])
+        sb = pa.Box.of([-2] * 3, [nx + 1, ny + 1, nz + 1])
+        db = pa.Box.of([0] * 3, [nx - 1, ny - 1, nz - 1])
+        per_box.append((ctypes.c_void_p(src.ptr.value + esz * B.goff[b]), sb,
+                        ctypes.c_void_p(dst.ptr.value + esz * B.voff[b]), db))
+
+    def launch_per_box():
+        for sptr, sb, dptr, db in per_box:
+            pa.call("pm_stencil_apply", st27.h, sptr, ctypes.byref(sb), dptr, ctypes.byref(db), ctypes.byref(db), 1,
+                    1, 1.0, sp)
+    ms_pb = time_launches(launch_per_box, 2, stream)
+
+    def gbs(b, ms):
+        return round(b / (ms * 1e-3) / 1e9, 1)
+    out = {"tuned": cfg.as_dict(), "autotune_ms": [round(t, 4) for t in tms],
+           "apply_L27": {"ms": round(ms_a, 4), "GB/s": gbs(apply_bytes, ms_a),
+                         "frac": round(apply_bytes / (ms_a * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "bytes": apply_bytes},
+           "gsrb": {"ms": round(ms_g, 4), "GB/s": gbs(gsrb_bytes, ms_g),
+                    "frac": round(gsrb_bytes / (ms_g * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "bytes": gsrb_bytes},
+           "graph_apply_plus_gsrb": {"ms": round(ms_graph, 4), "GB/s": gbs(apply_bytes + gsrb_bytes, ms_graph)},
+           "apply_L27_launch_per_box": {"ms": round(ms_pb, 4), "GB/s": gbs(apply_bytes, ms_pb),
+                                        "launches": nboxes}}
+    for d in (src, rhs, dst, phi):
+        d.free()
+    return {"boxes": nboxes, "valid_cells": valid, "footprint_cells": foot, "result": out}
 
 
 def main():
@@ -342,6 +410,11 @@ def main():
                            "bytes": 2 * bytes_f}
         a.free()
         result["extra"] = extra
+
+    if not args.no_batch:
+        ex = result.setdefault("extra", {})
+        for name, dt in (("batched_config4_f64", pa.PM_F64), ("batched_config4_f32", pa.PM_F32)):
+            ex[name] = batch_config4(pa, stream, dt)
 
     if not args.no_mg:
         result.setdefault("extra", {})["multigrid_config3"] = mg_config3(pa, stream, args.mg_n, 64, 6 if args.mg_n == 512 else 3)

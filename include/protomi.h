@@ -330,9 +330,23 @@ int pm_batch_destroy(pm_batch* batch);
 int pm_batch_elems(const pm_batch* batch, size_t* ghosted, size_t* valid);
 /* nboxes + 1 element offsets of every box in ghosted / valid storage. */
 int pm_batch_offsets(const pm_batch* batch, size_t* ghosted_off, size_t* valid_off);
-/* Tiling knobs (the autotuner's): z planes per apply tile (0 = whole box),
- * rows per GSRB tile, persistent workgroups per CU. */
-int pm_batch_configure(pm_batch* batch, int apply_zplanes, int gsrb_rows, int blocks_per_cu);
+/* Tiling knobs of the batched operators (what pm_batch_autotune searches):
+ * z planes per apply tile (0 = whole box), persistent workgroups per CU for
+ * the apply, rows per GSRB tile, workgroups per CU for the sweep. */
+typedef struct pm_batch_config {
+    int apply_zplanes, apply_blocks_per_cu, gsrb_rows, gsrb_blocks_per_cu;
+} pm_batch_config;
+int pm_batch_configure(pm_batch* batch, const pm_batch_config* cfg);
+int pm_batch_get_config(const pm_batch* batch, pm_batch_config* cfg);
+/* Times every candidate configuration (reps launches each, on `stream`) of
+ * pm_batch_apply with `st` (src -> dst) and of pm_batch_gsrb (phi_in = src,
+ * phi_out, rhs, dx; dtype from `st`), keeps the fastest of each operator and
+ * returns it in *best; best_ms[0..1] = its mean ms per launch (apply, gsrb).
+ * dst and phi_out are overwritten.  Results are identical under every
+ * configuration (only the schedule changes). */
+int pm_batch_autotune(pm_batch* batch, const pm_stencil* st, const void* src, void* dst, void* phi_out,
+                      const void* rhs, double dx, int reps, pm_batch_config* best, float* best_ms,
+                      pm_stream_t stream);
 /* dst = L(src, scale) on every box (src ghosted storage, dst valid storage)
  * for the 7-point Laplacian() or 27-point Laplacian_27() shape: per box the
  * bits of pm_stencil_apply. */

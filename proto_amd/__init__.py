@@ -59,6 +59,14 @@ class MgOptions(ctypes.Structure):
 _OP = ctypes.POINTER(MgOptions)
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("apply_zplanes", ctypes.c_int), ("apply_blocks_per_cu", ctypes.c_int),
+                ("gsrb_rows", ctypes.c_int), ("gsrb_blocks_per_cu", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class TileConfig(ctypes.Structure):
     """pm_tile_config."""
     _fields_ = [("rows", ctypes.c_int), ("prefetch", ctypes.c_int), ("nt_store", ctypes.c_int),
@@ -124,7 +132,9 @@ SIGNATURES = {
     "pm_batch_destroy": (_i, [_vp]),
     "pm_batch_elems": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
     "pm_batch_offsets": (_i, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
-    "pm_batch_configure": (_i, [_vp, _i, _i, _i]),
+    "pm_batch_configure": (_i, [_vp, _vp]),
+    "pm_batch_get_config": (_i, [_vp, _vp]),
+    "pm_batch_autotune": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
     "pm_batch_apply": (_i, [_vp, _vp, _vp, _vp, _d, _vp]),
     "pm_batch_gsrb": (_i, [_vp, _i, _vp, _vp, _vp, _d, _vp]),
     "pm_graph_begin": (_i, [_vp]),
@@ -207,4 +217,66 @@ class Stencil:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.pm_stencil_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """A batch of independent uneven patches (pm_batch): valid extents
+    sizes[b] = (nx, ny, nz), `ghost` ghost layers, ghosted and valid storage
+    laid out box after box."""
+
+    def __init__(self, sizes, ghost):
+        import numpy as np
+        sz = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32).reshape(-1, 3))
+        self.sizes = sz
+        self.ghost = int(ghost)
+        h = _vp()
+        call("pm_batch_create", ctypes.byref(h), len(sz), sz.ctypes.data_as(_ip), self.ghost)
+        self.h = h
+        ng, nv = ctypes.c_size_t(), ctypes.c_size_t()
+        call("pm_batch_elems", self.h, ctypes.byref(ng), ctypes.byref(nv))
+        self.ghosted_elems, self.valid_elems = ng.value, nv.value
+        go = (ctypes.c_size_t * (len(sz) + 1))()
+        vo = (ctypes.c_size_t * (len(sz) + 1))()
+        call("pm_batch_offsets", self.h, go, vo)
+        self.goff = list(go)
+        self.voff = list(vo)
+
+    def configure(self, apply_zplanes=0, gsrb_rows=8, blocks_per_cu=4, gsrb_blocks_per_cu=None):
+        c = BatchConfig(apply_zplanes, blocks_per_cu, gsrb_rows,
+                        blocks_per_cu if gsrb_blocks_per_cu is None else gsrb_blocks_per_cu)
+        call("pm_batch_configure", self.h, ctypes.byref(c))
+
+    def config(self):
+        c = BatchConfig()
+        call("pm_batch_get_config", self.h, ctypes.byref(c))
+        return c
+
+    def autotune(self, stencil, src, dst, phi_out, rhs, dx, reps=5, stream=None):
+        """Fastest configuration of each operator (kept); returns (config, ms[2])."""
+        c = BatchConfig()
+        ms = (ctypes.c_float * 2)()
+        call("pm_batch_autotune", self.h, stencil.h, src, dst, phi_out, rhs, ctypes.c_double(dx), reps,
+             ctypes.byref(c), ms, stream)
+        return c, (ms[0], ms[1])
+
+    def apply(self, stencil, src, dst, scale=1.0, stream=None):
+        call("pm_batch_apply", self.h, stencil.h, src, dst, ctypes.c_double(scale), stream)
+
+    def gsrb(self, dtype, phi_in, phi_out, rhs, dx, stream=None):
+        call("pm_batch_gsrb", self.h, dtype, phi_in, phi_out, rhs, ctypes.c_double(dx), stream)
+
+    def ghosted_view(self, flat, b):
+        """Box b of a flat ghosted array, shaped (z, y, x)."""
+        g = self.ghost
+        nx, ny, nz = (int(v) for v in self.sizes[b])
+        return flat[self.goff[b]:self.goff[b + 1]].reshape(nz + 2 * g, ny + 2 * g, nx + 2 * g)
+
+    def valid_view(self, flat, b):
+        nx, ny, nz = (int(v) for v in self.sizes[b])
+        return flat[self.voff[b]:self.voff[b + 1]].reshape(nz, ny, nx)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pm_batch_destroy(self.h)
             self.h = None

@@ -41,7 +41,7 @@ struct pm_batch {
     void* d_gitems = nullptr;        // BatchItem[] (gsrb)
     int nitems_gsrb = 0, gsrb_rows = 0;
     unsigned* d_counter = nullptr;   // 2 x (queue head, blocks done) per operator
-    int cfg_apply_rows = 0, cfg_apply_tz = 0, cfg_gsrb_rows = 8, cfg_blocks_per_cu = 4;
+    pm_batch_config cfg{0, 4, 8, 4};
 };
 
 namespace pm {
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
                                                     const T* __restrict__ rhs, BCoefs<T> c, T lambda,
                                                     unsigned* counter)
 {
-    extern __shared__ char smraw[];
+    extern __shared__ __align__(16) unsigned char smraw[];
     T* sm = reinterpret_cast<T*>(smraw);  // 4 * (rows + 4) * (max width + 2) elements
     __shared__ int s_item;
     const int tid = threadIdx.x;
@@ -328,7 +328,7 @@ int build_gsrb_items(pm_batch* B, int rows)
     return upload(&B->d_gitems, items.data(), sizeof(BatchItem) * items.size());
 }
 
-unsigned persistent_blocks(const pm_batch* B)
+unsigned persistent_blocks(int blocks_per_cu)
 {
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
@@ -337,7 +337,7 @@ unsigned persistent_blocks(const pm_batch* B)
             ncu = prop.multiProcessorCount;
         }
     }
-    return (unsigned)(ncu * B->cfg_blocks_per_cu);
+    return (unsigned)(ncu * blocks_per_cu);
 }
 
 template<typename T>
@@ -382,7 +382,7 @@ int pm_batch_create(pm_batch** out, int nboxes, const int* sizes_host, int ghost
     if (rc == PM_OK && hipMalloc((void**)&B->d_counter, 4 * sizeof(unsigned)) != hipSuccess) { rc = PM_ERR_NOMEM; }
     if (rc == PM_OK && hipMemset(B->d_counter, 0, 4 * sizeof(unsigned)) != hipSuccess) { rc = PM_ERR_HIP; }
     if (rc == PM_OK) { rc = build_apply_items(B, 0); }
-    if (rc == PM_OK) { rc = build_gsrb_items(B, B->cfg_gsrb_rows); }
+    if (rc == PM_OK) { rc = build_gsrb_items(B, B->cfg.gsrb_rows); }
     if (rc != PM_OK) {
         pm_batch_destroy(B);
         return rc;
@@ -420,15 +420,24 @@ int pm_batch_offsets(const pm_batch* B, size_t* ghosted_off, size_t* valid_off)
     return PM_OK;
 }
 
-int pm_batch_configure(pm_batch* B, int apply_zplanes, int gsrb_rows, int blocks_per_cu)
+int pm_batch_configure(pm_batch* B, const pm_batch_config* c)
 {
-    PM_REQUIRE(B && apply_zplanes >= 0 && gsrb_rows >= 1 && gsrb_rows <= 32 && blocks_per_cu >= 1 && blocks_per_cu <= 16,
-               "pm_batch_configure: bad argument");
-    B->cfg_blocks_per_cu = blocks_per_cu;
+    PM_REQUIRE(B && c, "pm_batch_configure: null argument");
+    PM_REQUIRE(c->apply_zplanes >= 0 && c->gsrb_rows >= 1 && c->gsrb_rows <= 32 && c->apply_blocks_per_cu >= 1 &&
+                   c->apply_blocks_per_cu <= 16 && c->gsrb_blocks_per_cu >= 1 && c->gsrb_blocks_per_cu <= 16,
+               "pm_batch_configure: bad configuration");
     int rc = PM_OK;
-    if (apply_zplanes != B->apply_tz) { rc = build_apply_items(B, apply_zplanes); }
-    if (rc == PM_OK && gsrb_rows != B->gsrb_rows) { rc = build_gsrb_items(B, gsrb_rows); }
+    if (c->apply_zplanes != B->apply_tz) { rc = build_apply_items(B, c->apply_zplanes); }
+    if (rc == PM_OK && c->gsrb_rows != B->gsrb_rows) { rc = build_gsrb_items(B, c->gsrb_rows); }
+    if (rc == PM_OK) { B->cfg = *c; }
     return rc;
+}
+
+int pm_batch_get_config(const pm_batch* B, pm_batch_config* c)
+{
+    PM_REQUIRE(B && c, "pm_batch_get_config: null argument");
+    *c = B->cfg;
+    return PM_OK;
 }
 
 int pm_batch_apply(pm_batch* B, const pm_stencil* st, const void* src, void* dst, double scale, pm_stream_t stream)
@@ -439,7 +448,7 @@ int pm_batch_apply(pm_batch* B, const pm_stencil* st, const void* src, void* dst
     int rc = stencil_info(st, &kind, &dtype, &cv);
     if (rc) { return rc; }
     PM_REQUIRE(kind == 7 || kind == 27, "pm_batch_apply: needs the 7-point or 27-point Laplacian shape");
-    const unsigned nb = persistent_blocks(B);
+    const unsigned nb = persistent_blocks(B->cfg.apply_blocks_per_cu);
     if (dtype == PM_F64) {
         BCoefs<double> cc;
         coefs_of<double>(cv, scale, cc);
@@ -480,8 +489,8 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
     PM_REQUIRE(maxnx <= 256, "pm_batch_gsrb: boxes wider than 256 cells are not supported");
     const int lds_elems = 4 * (B->gsrb_rows + 4) * (maxnx + 2);
     const size_t lds = (size_t)lds_elems * (dtype == PM_F64 ? 8 : 4);
-    PM_REQUIRE(lds <= 64 * 1024, "pm_batch_gsrb: tile too large for LDS (rows %d, width %d)", B->gsrb_rows, maxnx);
-    const unsigned nb = persistent_blocks(B);
+    PM_REQUIRE(lds + 64 <= 64 * 1024, "pm_batch_gsrb: tile too large for LDS (rows %d, width %d)", B->gsrb_rows, maxnx);
+    const unsigned nb = persistent_blocks(B->cfg.gsrb_blocks_per_cu);
     // LaplaceOp::init: coefficients Laplacian()_j * (-1/(dx*dx)), lambda = 2 * (-(dx*dx)/(4 DIM))
     double lam = -(dx * dx) / (4.0 * 3);
     lam *= 2.0;
@@ -501,6 +510,85 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
     }
     PM_CHECK_LAUNCH();
     return PM_OK;
+}
+
+int pm_batch_autotune(pm_batch* B, const pm_stencil* st, const void* src, void* dst, void* phi_out, const void* rhs,
+                      double dx, int reps, pm_batch_config* best, float* best_ms, pm_stream_t stream)
+{
+    PM_REQUIRE(B && st && src && dst && phi_out && rhs && best && best_ms && reps >= 1,
+               "pm_batch_autotune: bad argument");
+    PM_REQUIRE(src != phi_out, "pm_batch_autotune: phi_out must differ from src");
+    int kind = 0, dtype = 0;
+    std::vector<double> cv;
+    int rc = stencil_info(st, &kind, &dtype, &cv);
+    if (rc) { return rc; }
+    int maxnx = 0;
+    for (int b = 0; b < B->nboxes; b++) { maxnx = std::max(maxnx, B->sizes[3 * b]); }
+    const pm_batch_config keep = B->cfg;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_error("pm_batch_autotune: event creation failed");
+        return PM_ERR_HIP;
+    }
+    auto time = [&](auto&& launch, float* ms) -> int {
+        int r = launch();  // warm-up (also builds the work list)
+        if (r) { return r; }
+        hipEventRecord(e0, S(stream));
+        for (int i = 0; i < reps && r == PM_OK; i++) { r = launch(); }
+        hipEventRecord(e1, S(stream));
+        if (r) { return r; }
+        if (hipEventSynchronize(e1) != hipSuccess) { set_error("pm_batch_autotune: sync failed"); return PM_ERR_HIP; }
+        hipEventElapsedTime(ms, e0, e1);
+        *ms /= (float)reps;
+        return PM_OK;
+    };
+    const int zs[] = {0, 8, 16, 32};
+    const int bpcs[] = {1, 2, 4, 8};
+    const int rows[] = {2, 4, 8, 16};
+    pm_batch_config win = keep;
+    float best_a = 1e30f, best_g = 1e30f;
+    for (int z : zs) {
+        for (int bpc : bpcs) {
+            pm_batch_config c = keep;
+            c.apply_zplanes = z;
+            c.apply_blocks_per_cu = bpc;
+            rc = pm_batch_configure(B, &c);
+            float ms = 0;
+            if (rc == PM_OK) { rc = time([&]() { return pm_batch_apply(B, st, src, dst, 1.0, stream); }, &ms); }
+            if (rc) { break; }
+            if (ms < best_a) { best_a = ms; win.apply_zplanes = z; win.apply_blocks_per_cu = bpc; }
+        }
+        if (rc) { break; }
+    }
+    const size_t esz = dtype == PM_F64 ? 8 : 4;
+    for (int r : rows) {
+        if (rc) { break; }
+        if ((size_t)4 * (r + 4) * (maxnx + 2) * esz + 64 > 64 * 1024) { continue; }
+        for (int bpc : bpcs) {
+            pm_batch_config c = keep;
+            c.apply_zplanes = win.apply_zplanes;
+            c.gsrb_rows = r;
+            c.gsrb_blocks_per_cu = bpc;
+            rc = pm_batch_configure(B, &c);
+            float ms = 0;
+            if (rc == PM_OK) {
+                rc = time([&]() { return pm_batch_gsrb(B, dtype, src, phi_out, rhs, dx, stream); }, &ms);
+            }
+            if (rc) { break; }
+            if (ms < best_g) { best_g = ms; win.gsrb_rows = r; win.gsrb_blocks_per_cu = bpc; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) {
+        pm_batch_configure(B, &keep);
+        return rc;
+    }
+    rc = pm_batch_configure(B, &win);
+    *best = win;
+    best_ms[0] = best_a;
+    best_ms[1] = best_g;
+    return rc;
 }
 
 }  // extern "C"

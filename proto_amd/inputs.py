@@ -77,3 +77,12 @@ def gaussian_level_rhs(layout, n, data, ngauss, seed, sigma=0.05, stream=None):
     pa.call("pm_level_add_separable", layout, pa.PM_F64, data.ptr, g0, -mean, ones.ctypes.data_as(_dp),
             ones.ctypes.data_as(_dp), ones.ctypes.data_as(_dp), stream)
     return mean
+
+
+def batch_sizes(nboxes, edges, seed, stream_id=1):
+    """Per-axis box extents drawn iid from `edges` (x, y, z of box 0, then box
+    1, ...): index floor((u + 1) / 2 * len(edges)) of the pm_fill_uniform
+    stream (seed, stream_id)."""
+    u = _splitmix_uniform(seed, stream_id, 3 * nboxes)
+    idx = np.minimum(((u + 1.0) * 0.5 * len(edges)).astype(np.int64), len(edges) - 1)
+    return np.asarray(edges, dtype=np.int32)[idx].reshape(nboxes, 3)

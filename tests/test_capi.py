@@ -67,6 +67,20 @@ def test_reduce_and_region_validation():
     assert L.pm_boxdata_binop(7, pa.PM_F64, p, ctypes.byref(b), p, ctypes.byref(b), ctypes.byref(b), 1, None) == -1
 
 
+def test_batch_validation():
+    L = pa.lib()
+    h = ctypes.c_void_p()
+    bad = (ctypes.c_int * 6)(8, 8, 8, 8, 0, 8)
+    assert L.pm_batch_create(ctypes.byref(h), 2, bad, 2) == -1
+    assert b"box 1" in L.pm_last_error()
+    good = (ctypes.c_int * 3)(8, 8, 8)
+    assert L.pm_batch_create(ctypes.byref(h), 1, good, 0) == -1
+    assert L.pm_batch_create(ctypes.byref(h), 1, good, 5) == -1
+    assert L.pm_batch_create(ctypes.byref(h), 0, good, 1) == -1
+    assert L.pm_batch_apply(None, None, None, None, 1.0, None) == -1
+    assert L.pm_batch_configure(None, None) == -1
+
+
 def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(pa, "_lib", None)
     monkeypatch.setattr(pa, "LIB_PATH", "/nonexistent/libprotomi.so")

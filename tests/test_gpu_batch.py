@@ -1,0 +1,200 @@
+"""GPU parity of the batched-patch operators (pm_batch_apply, pm_batch_gsrb)
+against the oracle, box by box, bitwise: the 7- and 27-point Laplacian apply
+of Proto's Stencil::apply (Proto_StencilImplem.H hostApply) and one GSRB sweep
+with fixed ghosts (LaplaceOp.H:95-123), FP64 and FP32.  Also: the work queue
+resets itself (repeated launches and every tiling knob give the same bits),
+and a captured graph replays correctly with eager work between replays."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import proto_amd as pa
+import util
+from gpu_util import dev, host, say, vp, zeros
+
+pytestmark = pytest.mark.gpu
+
+# uneven patches: odd and even extents, 1-cell boxes, several 64-wide x tiles
+SIZES = [(16, 16, 16), (24, 24, 24), (33, 7, 5), (1, 1, 1), (2, 3, 4), (64, 64, 8), (65, 9, 11),
+         (130, 5, 6), (5, 130, 3), (3, 4, 130), (48, 32, 16), (7, 7, 7), (100, 1, 2), (1, 50, 1),
+         (17, 19, 23), (32, 32, 32)]
+GHOST = 2
+NPDT = {pa.PM_F64: np.float64, pa.PM_F32: np.float32}
+
+
+def make_batch(sizes=SIZES, ghost=GHOST):
+    return pa.Batch(sizes, ghost)
+
+
+def inputs(B, dt, seed):
+    t = NPDT[dt]
+    src = util.uniform(seed, 0, B.ghosted_elems).astype(t)
+    rhs = util.uniform(seed, 1, B.valid_elems).astype(t)
+    return src, rhs
+
+
+def oracle_apply_batch(B, coefs, offs, src, scale):
+    out = np.zeros(B.valid_elems, dtype=src.dtype)
+    g = B.ghost
+    for b in range(len(B.sizes)):
+        n = [int(v) for v in B.sizes[b]]
+        s = np.ascontiguousarray(src[B.goff[b]:B.goff[b + 1]])
+        d = np.zeros(B.voff[b + 1] - B.voff[b], dtype=src.dtype)
+        hi = [v - 1 for v in n]
+        util.oracle_apply(coefs, offs, s, [-g] * 3, [v + g - 1 for v in n], d, [0, 0, 0], hi, [0, 0, 0], hi,
+                          True, scale)
+        out[B.voff[b]:B.voff[b + 1]] = d
+    return out
+
+
+def oracle_gsrb_batch(B, phi, rhs, dx):
+    out = phi.copy()
+    f = util.oracle().orc_patch_gsrb_f64 if phi.dtype == np.float64 else util.oracle().orc_patch_gsrb_f32
+    for b in range(len(B.sizes)):
+        nx, ny, nz = (int(v) for v in B.sizes[b])
+        p = np.ascontiguousarray(out[B.goff[b]:B.goff[b + 1]])
+        r = np.ascontiguousarray(rhs[B.voff[b]:B.voff[b + 1]])
+        f(nx, ny, nz, B.ghost, dx, util.ptr(r), util.ptr(p))
+        out[B.goff[b]:B.goff[b + 1]] = p
+    return out
+
+
+def first_diff(B, got, want, valid):
+    for b in range(len(B.sizes)):
+        lo, hi = (B.voff[b], B.voff[b + 1]) if valid else (B.goff[b], B.goff[b + 1])
+        if got[lo:hi].tobytes() != want[lo:hi].tobytes():
+            return "box %d %s" % (b, tuple(int(v) for v in B.sizes[b]))
+    return None
+
+
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+@pytest.mark.parametrize("name", ["Laplacian", "Laplacian_27"])
+def test_batch_apply_vs_oracle(dt, name):
+    B = make_batch()
+    t = NPDT[dt]
+    coefs, offs, *_ = util.lib_stencil(name, 0, 3, t)
+    st = pa.Stencil(coefs, offs, dt)
+    src, _ = inputs(B, dt, 11)
+    s_d, d_d = dev(src), zeros(B.valid_elems, t)
+    for scale in (1.0, 0.37):
+        want = oracle_apply_batch(B, coefs, offs, src, scale)
+        for cfg in ((0, 8, 4), (4, 8, 1), (1, 8, 16), (0, 8, 2)):
+            B.configure(*cfg)
+            for rep in range(2):
+                say("batch apply", name, dt, "scale", scale, "cfg", cfg, "rep", rep)
+                B.apply(st, vp(s_d), vp(d_d), scale)
+                got = host(d_d)
+                assert first_diff(B, got, want, True) is None, (cfg, rep, first_diff(B, got, want, True))
+
+
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+def test_batch_gsrb_vs_oracle(dt):
+    B = make_batch()
+    t = NPDT[dt]
+    phi, rhs = inputs(B, dt, 12)
+    dx = 1.0 / 64
+    want1 = oracle_gsrb_batch(B, phi, rhs, dx)
+    want2 = oracle_gsrb_batch(B, want1, rhs, dx)
+    p_in, r_d = dev(phi), dev(rhs)
+    for rows in (1, 3, 8):
+        B.configure(0, rows, 4)
+        for rep in range(2):
+            say("batch gsrb", dt, "rows", rows, "rep", rep)
+            p_out = dev(phi)  # ghosts of phi_out = ghosts of phi_in
+            B.gsrb(dt, vp(p_in), vp(p_out), vp(r_d), dx)
+            got = host(p_out)
+            assert first_diff(B, got, want1, False) is None, (rows, rep, first_diff(B, got, want1, False))
+            # ping-pong: a second sweep back into a buffer holding phi's ghosts
+            p_back = dev(phi)
+            B.gsrb(dt, vp(p_out), vp(p_back), vp(r_d), dx)
+            got2 = host(p_back)
+            assert first_diff(B, got2, want2, False) is None, (rows, rep, first_diff(B, got2, want2, False))
+
+
+def test_batch_gsrb_limits():
+    B = make_batch()
+    phi, rhs = inputs(B, pa.PM_F64, 13)
+    p_in, r_d = dev(phi), dev(rhs)
+    with pytest.raises(pa.NativeError, match="out of place"):
+        B.gsrb(pa.PM_F64, vp(p_in), vp(p_in), vp(r_d), 0.1)
+    B.configure(0, 32, 4)  # 4 x 36 x 132 doubles does not fit a workgroup's LDS
+    with pytest.raises(pa.NativeError, match="LDS"):
+        B.gsrb(pa.PM_F64, vp(p_in), vp(dev(phi)), vp(r_d), 0.1)
+    # the batch stays usable after a rejected call
+    B.configure(0, 8, 4)
+    p_out = dev(phi)
+    B.gsrb(pa.PM_F64, vp(p_in), vp(p_out), vp(r_d), 0.1)
+    assert first_diff(B, host(p_out), oracle_gsrb_batch(B, phi, rhs, 0.1), False) is None
+
+
+def test_batch_many_small_boxes_queue():
+    """Far more tiles than persistent workgroups, one workgroup per CU."""
+    rng = np.random.default_rng(5)
+    sizes = rng.choice([1, 2, 4, 5, 8], size=(3000, 3))
+    B = pa.Batch(sizes, 1)
+    B.configure(0, 2, 1)
+    coefs, offs, *_ = util.lib_stencil("Laplacian", 0, 3, np.float64)
+    st = pa.Stencil(coefs, offs, pa.PM_F64)
+    src, rhs = inputs(B, pa.PM_F64, 14)
+    want = oracle_apply_batch(B, coefs, offs, src, 1.0)
+    wantg = oracle_gsrb_batch(B, src, rhs, 0.25)
+    s_d, d_d, r_d = dev(src), zeros(B.valid_elems), dev(rhs)
+    for rep in range(3):
+        say("small boxes rep", rep)
+        B.apply(st, vp(s_d), vp(d_d))
+        assert first_diff(B, host(d_d), want, True) is None
+        p_out = dev(src)
+        B.gsrb(pa.PM_F64, vp(s_d), vp(p_out), vp(r_d), 0.25)
+        assert first_diff(B, host(p_out), wantg, False) is None
+
+
+def test_batch_graph_replay():
+    """apply + GSRB captured once, replayed three times with eager refills of
+    the inputs between replays; every replay matches the oracle."""
+    from proto_amd.device import Stream
+    B = make_batch()
+    coefs, offs, *_ = util.lib_stencil("Laplacian_27", 0, 3, np.float64)
+    st = pa.Stencil(coefs, offs, pa.PM_F64)
+    s = Stream()
+    src_d, dst_d = zeros(B.ghosted_elems), zeros(B.valid_elems)
+    rhs_d, out_d = zeros(B.valid_elems), zeros(B.ghosted_elems)
+    dx = 1.0 / 32
+    g = ctypes.c_void_p()
+    pa.call("pm_graph_begin", s.h)
+    B.apply(st, vp(src_d), vp(dst_d), 1.0, s.h)
+    B.gsrb(pa.PM_F64, vp(src_d), vp(out_d), vp(rhs_d), dx, s.h)
+    pa.call("pm_graph_end", s.h, ctypes.byref(g))
+    try:
+        for rep in range(3):
+            say("graph replay", rep)
+            pa.call("pm_fill_uniform", pa.PM_F64, vp(src_d), B.ghosted_elems, 100 + rep, 0,
+                    ctypes.c_double(1.0), ctypes.c_double(0.0), s.h)
+            pa.call("pm_fill_uniform", pa.PM_F64, vp(rhs_d), B.valid_elems, 100 + rep, 1,
+                    ctypes.c_double(1.0), ctypes.c_double(0.0), s.h)
+            pa.call("pm_memcpy", vp(out_d), vp(src_d), B.ghosted_elems * 8, 2, s.h)
+            s.sync()
+            src, rhs = host(src_d), host(rhs_d)
+            pa.call("pm_graph_launch", g, s.h)
+            s.sync()
+            want = oracle_apply_batch(B, coefs, offs, src, 1.0)
+            assert first_diff(B, host(dst_d), want, True) is None, rep
+            assert first_diff(B, host(out_d), oracle_gsrb_batch(B, src, rhs, dx), False) is None, rep
+    finally:
+        pa.call("pm_graph_destroy", g)
+
+
+def test_batch_autotune_keeps_bits():
+    B = make_batch()
+    coefs, offs, *_ = util.lib_stencil("Laplacian_27", 0, 3, np.float32)
+    st = pa.Stencil(coefs, offs, pa.PM_F32)
+    src, rhs = inputs(B, pa.PM_F32, 15)
+    s_d, d_d, r_d, p_d = dev(src), zeros(B.valid_elems, np.float32), dev(rhs), dev(src)
+    cfg, ms = B.autotune(st, vp(s_d), vp(d_d), vp(p_d), vp(r_d), 0.5, 2)
+    say("autotuned", cfg.as_dict(), ms)
+    assert B.config().as_dict() == cfg.as_dict() and ms[0] > 0 and ms[1] > 0
+    B.apply(st, vp(s_d), vp(d_d))
+    assert first_diff(B, host(d_d), oracle_apply_batch(B, coefs, offs, src, 1.0), True) is None
+    p_d = dev(src)
+    B.gsrb(pa.PM_F32, vp(s_d), vp(p_d), vp(r_d), 0.5)
+    assert first_diff(B, host(p_d), oracle_gsrb_batch(B, src, rhs, 0.5), False) is None

@@ -51,6 +51,9 @@ def oracle():
         lib.orc_jacobi_relax.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]
         lib.orc_coarse_residual.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
         lib.orc_fine_interp.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+        _ci = ctypes.c_int
+        lib.orc_patch_gsrb_f64.argtypes = [_ci, _ci, _ci, _ci, ctypes.c_double, _dp, _dp]
+        lib.orc_patch_gsrb_f32.argtypes = [_ci, _ci, _ci, _ci, ctypes.c_double, _fp, _fp]
         lib.orc_uniform_f64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_long, _dp]
         _oracle = lib
     return _oracle
