@@ -27,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the fused residual kernel (rocprofv3 FETCH_SIZE x2.0 +
+# WRITE_SIZE x1.0, calibrated on a 1 GiB copy; range over the tuned z chunks)
+TRAFFIC_RESIDUAL2 = 4.36e9
 
 
 def parse():
@@ -42,6 +45,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tile configurations)")
     ap.add_argument("--no-mg", action="store_true", help="skip the config-3 multigrid measurement")
     ap.add_argument("--mg-n", type=int, default=512)
+    ap.add_argument("--mg-graph", type=int, default=1, help="replay each multigrid cycle as a captured graph")
+    ap.add_argument("--serve-n", type=int, default=300, help="requests in the config-5 stream (0 = skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched-patch measurement")
     return ap.parse_args()
 
@@ -129,7 +134,7 @@ def cpu_baseline(n, g, planes):
                       "%.2f s, Proto's CPU path single-threaded (Release flags)" % (n, n, nz, dt)}
 
 
-def mg_config3(pa, stream, n, bs, nlev, reps=3):
+def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
     """Config 3: periodic n^3 in bs^3 boxes, 16-Gaussian RHS (sigma 0.05, seed 2)
     minus its mean, GSRB V(2,2) to 8^3, relative residual 1e-10.  Plain FP64
     V-cycles vs the FP32-preconditioned FP64 refinement; solves/s from the
@@ -152,7 +157,7 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3):
         opt = pa.MgOptions()
         pa.lib().pm_mg_default_options(ctypes.byref(opt))
         opt.relax, opt.pre, opt.post, opt.bottom = 0, 2, 2, 10
-        opt.precision, opt.max_iter, opt.tol, opt.use_graph = prec, 40, 1e-10, 1
+        opt.precision, opt.max_iter, opt.tol, opt.use_graph = prec, 40, 1e-10, use_graph
         mg = ctypes.c_void_p()
         pa.call("pm_mg_create", ctypes.byref(mg), lay, nlev, 1.0 / n, ctypes.byref(opt))
         phi = DeviceArray.zeros(nphi)
@@ -188,7 +193,7 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3):
     return out
 
 
-def batch_config4(pa, stream, dtype, reps=10, nboxes=2000):
+def batch_config4(pa, stream, dtype, reps=10, nboxes=2000, per_box=True):
     """Config 4: nboxes patches, per-axis extents iid from {16,24,32,48,64}
     (seed 3), 2 ghost layers, inputs U[-1,1) over the ghosted boxes (seed 3,
     stream 0; rhs stream 2).  Operators: Laplacian_27 apply (scale 1) and one
@@ -207,8 +212,7 @@ def batch_config4(pa, stream, dtype, reps=10, nboxes=2000):
     src = uniform(B.ghosted_elems, 3, 0, npdt, sp)
     rhs = uniform(B.valid_elems, 3, 2, npdt, sp)
     dst = DeviceArray(B.valid_elems, npdt)
-    phi = DeviceArray(B.ghosted_elems, npdt)
-    pa.call("pm_memcpy", phi.ptr, src.ptr, phi.nbytes, 2, sp)
+    phi = uniform(B.ghosted_elems, 3, 0, npdt, sp)  # the sweep's output buffer holds src's ghosts
     st27 = stencils(dtype)[1]
     dx = 1.0 / 64
     valid = int(np.prod(sizes, axis=1).sum())
@@ -238,7 +242,7 @@ def batch_config4(pa, stream, dtype, reps=10, nboxes=2000):
         for sptr, sb, dptr, db in per_box:
             pa.call("pm_stencil_apply", st27.h, sptr, ctypes.byref(sb), dptr, ctypes.byref(db), ctypes.byref(db), 1,
                     1, 1.0, sp)
-    ms_pb = time_launches(launch_per_box, 2, stream)
+    ms_pb = time_launches(launch_per_box, 2, stream) if per_box else float("nan")
 
     def gbs(b, ms):
         return round(b / (ms * 1e-3) / 1e9, 1)
@@ -253,6 +257,61 @@ def batch_config4(pa, stream, dtype, reps=10, nboxes=2000):
     for d in (src, rhs, dst, phi):
         d.free()
     return {"boxes": nboxes, "valid_cells": valid, "footprint_cells": foot, "result": out}
+
+
+def serve_trace(nreq, lam=200.0, seed=4, sizes=(32, 48, 64, 96, 128, 192, 256), tol=1e-8):
+    """Config 5 request stream: Poisson arrivals at lam req/s (exponential gaps
+    from the SplitMix64 stream (seed, 1)), edge uniform over the index of
+    `sizes` (stream 2), K ~ U{4..32} Gaussians (stream 3), request seeds 1000+i."""
+    import numpy as np
+    from proto_amd.inputs import _splitmix_uniform
+    u1 = (_splitmix_uniform(seed, 1, nreq) + 1.0) * 0.5
+    u2 = (_splitmix_uniform(seed, 2, nreq) + 1.0) * 0.5
+    u3 = (_splitmix_uniform(seed, 3, nreq) + 1.0) * 0.5
+    t = np.cumsum(-np.log1p(-u1) / lam)
+    reqs = []
+    for i in range(nreq):
+        n = sizes[min(int(u2[i] * len(sizes)), len(sizes) - 1)]
+        k = 4 + min(int(u3[i] * 29), 28)
+        reqs.append({"arrival_s": float(t[i]), "n": n, "ngauss": k, "seed": 1000 + i, "tol": tol, "keep_result": 0})
+    return reqs
+
+
+def serve_config5(pa, stream, nreq):
+    """Config 5: the served Poisson stream through the batched FP32-IR server
+    and through the plain FP64 one-at-a-time solver; plus both at saturation
+    (every request present at t = 0) for the capacity in solves/s."""
+    import numpy as np
+    reqs = serve_trace(nreq)
+
+    def summary(st, wall):
+        ok = [s for s in st if s["status"] == pa.SERVE_SOLVED]
+        lat = np.array([s["finish_s"] - s["arrival_s"] for s in ok])
+        span = max(s["finish_s"] for s in st) - min(s["arrival_s"] for s in st)
+        return {"solved": len(ok), "rejected": sum(s["status"] == pa.SERVE_REJECTED for s in st),
+                "not_converged": sum(s["status"] == pa.SERVE_NOT_CONVERGED for s in st),
+                "solves_per_s": round(len(ok) / span, 2), "p50_latency_ms": round(1e3 * float(np.percentile(lat, 50)), 2),
+                "p99_latency_ms": round(1e3 * float(np.percentile(lat, 99)), 2),
+                "mean_iters": round(float(np.mean([s["iters"] for s in ok])), 2),
+                "max_rel_residual": float(max(s["rel_residual"] for s in ok)), "wall_s": round(wall, 3)}
+    out = {"workload": "poisson_stream_lambda200", "requests": nreq,
+           "arrival_window_s": round(reqs[-1]["arrival_s"], 3), "tol": 1e-8}
+    for name, mode in (("batched_fp32_ir", 0), ("plain_fp64", 1)):
+        srv = pa.Server(mode=mode, hbm_budget=200 << 30, max_active=64)
+        srv.run(serve_trace(8), stream.h)  # warm-up: hierarchies and code paths
+        t0 = time.perf_counter()
+        st = srv.run(reqs, stream.h)
+        out[name] = summary(st, time.perf_counter() - t0)
+        sat = [dict(r, arrival_s=0.0) for r in reqs]
+        t0 = time.perf_counter()
+        st = srv.run(sat, stream.h)
+        out[name]["saturated"] = summary(st, time.perf_counter() - t0)
+        out[name]["resident_bytes"] = srv.resident_bytes()
+        del srv
+    b, p = out["batched_fp32_ir"], out["plain_fp64"]
+    out["solves_per_s_ratio"] = round(b["solves_per_s"] / p["solves_per_s"], 3)
+    out["saturated_solves_per_s_ratio"] = round(b["saturated"]["solves_per_s"] / p["saturated"]["solves_per_s"], 3)
+    return out
 
 
 def main():
@@ -306,6 +365,17 @@ def main():
 
     tune("residual2", s7, s27, 3)
 
+    if args.only == "batch":
+        # profiling helper: the config-4 batched operators (both precisions)
+        # and a plain 1 GiB device copy to calibrate the counters
+        for dt in (pa.PM_F64, pa.PM_F32):
+            batch_config4(pa, stream, dt, reps=args.steps, per_box=False)
+        cpy = DeviceArray(n ** 3)
+        pa.call("pm_memcpy", cpy.ptr, r1.ptr, bytes_f, 2, sp)
+        stream.sync()
+        print(json.dumps({"only": args.only, "steps": args.steps, "copy_bytes": 2 * bytes_f}))
+        return
+
     if args.only:
         # profiling helper: the fused kernel and a plain 1 GiB device copy (the
         # runtime's blit kernel, a known byte count to calibrate the counters)
@@ -335,6 +405,10 @@ def main():
     value = step_bytes / (ms_step * 1e-3) / 1e9
 
     k_ms = time_launches(kernel_only, max(5, args.steps), stream)
+    cur = pa.TileConfig()
+    pa.call("pm_stencil_get_tile_config", s7.h, s27.h, 3, ctypes.byref(rb), ctypes.byref(cur))
+    kernel_name = "k_zmarch<double, 7, 3, %d, %d, %d> (7+27-point fused residual, z chunk %d)" % (
+        cur.rows, cur.prefetch, cur.nt_store, cur.zplanes)
     achieved = step_bytes / (k_ms * 1e-3) / 1e9
     result = {
         "metric": "stencil apply GB/s (% HBM roofline); MG solves/s + p99 latency vs plain FP64",
@@ -354,8 +428,9 @@ def main():
                    "fused": "r = f - L(u), absMax(r), sum(r) for both operators, one pass",
                    "step_bytes": step_bytes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "k_zmarch<double,7,3,2> (7+27-point fused residual)", "kernel_ms": round(k_ms, 4)},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": TRAFFIC_RESIDUAL2,
+                     "traffic_source": "profiles/zmarch_residual2_laplacian512_f64.txt (FETCH x2.0 + WRITE x1.0)",
+                     "kernel": kernel_name, "kernel_ms": round(k_ms, 4)},
         "wall_s": round(wall, 3),
         "tuned": tuned,
     }
@@ -416,8 +491,12 @@ def main():
         for name, dt in (("batched_config4_f64", pa.PM_F64), ("batched_config4_f32", pa.PM_F32)):
             ex[name] = batch_config4(pa, stream, dt)
 
+    if args.serve_n > 0:
+        result.setdefault("extra", {})["served_config5"] = serve_config5(pa, stream, args.serve_n)
+
     if not args.no_mg:
-        result.setdefault("extra", {})["multigrid_config3"] = mg_config3(pa, stream, args.mg_n, 64, 6 if args.mg_n == 512 else 3)
+        result.setdefault("extra", {})["multigrid_config3"] = mg_config3(pa, stream, args.mg_n, 64, 6 if args.mg_n == 512 else 3,
+                                                                  use_graph=args.mg_graph)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(n, g, args.cpu_planes)

@@ -312,6 +312,10 @@ int pm_mg_level_layout(const pm_mg* mg, int level, const pm_layout** layout);
  * last iterate).  Synchronises `stream` once per cycle (the norm test). */
 int pm_mg_solve(pm_mg* mg, void* phi, const void* rhs, double* hist_host, int* iters, pm_stream_t stream);
 /* One plain V-cycle on (phi, rhs) of the finest level, asynchronous. */
+/* Stopping rule of later solves: max-norm residual < tol * initial, at most
+ * max_iter cycles (LevelMultigridSolver::solve's a_tolerance / a_maxIter,
+ * LevelMultigridSolverImplem.H:408-433). */
+int pm_mg_set_tolerance(pm_mg* mg, double tol, int max_iter);
 int pm_mg_vcycle(pm_mg* mg, void* phi, const void* rhs, pm_stream_t stream);
 /* resnorm (LevelMultigrid.cpp:75-97) of (phi, rhs); synchronises. */
 int pm_mg_resnorm(pm_mg* mg, void* phi, const void* rhs, double* out_host, pm_stream_t stream);
@@ -358,6 +362,69 @@ int pm_batch_apply(pm_batch* batch, const pm_stencil* st, const void* src, void*
  * ghosts of phi_out must equal those of phi_in (they are not written). */
 int pm_batch_gsrb(pm_batch* batch, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
                   pm_stream_t stream);
+
+/* ---------------------------------------------------------------- serving */
+/* A continuous-batching Poisson solve server.  No counterpart in the reference
+ * (Proto solves one problem per LevelMultigridSolver::solve call,
+ * LevelMultigridSolverImplem.H:408-433).  Each request is a periodic cube of
+ * n^3 cells (unit domain, h = 1/n) with rhs = sum of K Gaussians
+ * (sigma, centres in [0,1)^3 and amplitudes +-1 from the request seed, periodic
+ * minimum-image distance) minus its mean; it is solved for Laplacian(phi) =
+ * rhs to max-norm residual < tol * initial.
+ *
+ * Mode 0 (batched FP32-IR): requests are admitted FIFO while their hierarchy
+ * fits the HBM budget (hierarchies stay resident in a pool keyed by n) and a
+ * slot is free; every loop iteration runs ONE FP32 GSRB V-cycle over all
+ * active requests (each level operator is one launch covering every request
+ * that has that level), the FP64 correction, the FP64 residual and one norm
+ * read-back; converged requests retire and waiting ones join at once.  Per
+ * request the arithmetic is that of pm_mg precision 1 on a one-box layout
+ * (same FP32 coefficients, same order), so a result does not depend on which
+ * other requests were in flight.
+ * Mode 1 (plain FP64): one request at a time through pm_mg precision 0
+ * (Proto's GSRB V-cycle), the reference point for the batched mode.
+ * Levels: n is halved while it is even and larger than 8 (32 -> 8, 48 -> 6). */
+typedef struct pm_server pm_server;
+typedef struct pm_server_options {
+    size_t hbm_budget;  /* bytes the resident hierarchies may hold */
+    int max_active;     /* requests in flight together (mode 0) */
+    int mode;           /* 0 batched FP32-IR, 1 plain FP64 one at a time */
+    int pre, post, bottom; /* GSRB sweeps (even counts) */
+    int max_iter;       /* outer iterations before PM_SERVE_NOT_CONVERGED */
+    double sigma;       /* Gaussian width */
+} pm_server_options;
+typedef struct pm_request {
+    double arrival_s;   /* seconds after the start of pm_server_run */
+    int n;              /* cells per side */
+    int ngauss;         /* K */
+    uint64_t seed;      /* Gaussian centres / amplitudes */
+    double tol;         /* relative max-norm residual */
+    int keep_result;    /* keep phi for pm_server_result */
+} pm_request;
+enum { PM_SERVE_SOLVED = 0, PM_SERVE_NOT_CONVERGED = 1, PM_SERVE_REJECTED = 2 };
+typedef struct pm_request_stats {
+    double arrival_s, admit_s, finish_s; /* seconds after the start of the run */
+    int iters;          /* outer iterations (V-cycles) */
+    int status;         /* PM_SERVE_* */
+    double rel_residual;
+    size_t bytes;       /* resident hierarchy */
+} pm_request_stats;
+void pm_server_default_options(pm_server_options* opt);
+int pm_server_create(pm_server** out, const pm_server_options* opt);
+int pm_server_destroy(pm_server* srv);
+/* Device bytes of one request's hierarchy for n (mode of the server). */
+int pm_server_request_bytes(const pm_server* srv, int n, size_t* bytes);
+/* Serves the requests in real time (arrivals at their arrival_s) on `stream`
+ * and returns when every request is done; stats[i] for reqs[i]. */
+int pm_server_run(pm_server* srv, int nreq, const pm_request* reqs, pm_request_stats* stats, pm_stream_t stream);
+/* phi (n^3 values, x fastest) of request `index` of the last run, when it was
+ * kept (keep_result) and solved. */
+int pm_server_result(pm_server* srv, int index, double* phi_host);
+/* The right-hand side a request (n, ngauss, seed) is served with, written to
+ * rhs_dev (n^3 FP64, x fastest) -- to solve the same problem elsewhere. */
+int pm_server_rhs(int n, int ngauss, uint64_t seed, double sigma, double* rhs_dev, pm_stream_t stream);
+/* Device bytes resident in the server's pools. */
+int pm_server_resident_bytes(const pm_server* srv, size_t* bytes);
 
 /* ----------------------------------------------------------------- graphs */
 /* hipGraph capture of work issued on a (non-null) stream by any entry point

@@ -67,6 +67,26 @@ class BatchConfig(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("mode", ctypes.c_int),
+                ("pre", ctypes.c_int), ("post", ctypes.c_int), ("bottom", ctypes.c_int), ("max_iter", ctypes.c_int),
+                ("sigma", ctypes.c_double)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("n", ctypes.c_int), ("ngauss", ctypes.c_int),
+                ("seed", ctypes.c_uint64), ("tol", ctypes.c_double), ("keep_result", ctypes.c_int)]
+
+
+class RequestStats(ctypes.Structure):
+    _fields_ = [("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("finish_s", ctypes.c_double),
+                ("iters", ctypes.c_int), ("status", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                ("bytes", ctypes.c_size_t)]
+
+
+SERVE_SOLVED, SERVE_NOT_CONVERGED, SERVE_REJECTED = 0, 1, 2
+
+
 class TileConfig(ctypes.Structure):
     """pm_tile_config."""
     _fields_ = [("rows", ctypes.c_int), ("prefetch", ctypes.c_int), ("nt_store", ctypes.c_int),
@@ -137,6 +157,15 @@ SIGNATURES = {
     "pm_batch_autotune": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
     "pm_batch_apply": (_i, [_vp, _vp, _vp, _vp, _d, _vp]),
     "pm_batch_gsrb": (_i, [_vp, _i, _vp, _vp, _vp, _d, _vp]),
+    "pm_mg_set_tolerance": (_i, [_vp, _d, _i]),
+    "pm_server_default_options": (None, [ctypes.POINTER(ServerOptions)]),
+    "pm_server_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(ServerOptions)]),
+    "pm_server_destroy": (_i, [_vp]),
+    "pm_server_request_bytes": (_i, [_vp, _i, ctypes.POINTER(_sz)]),
+    "pm_server_resident_bytes": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "pm_server_run": (_i, [_vp, _i, ctypes.POINTER(Request), ctypes.POINTER(RequestStats), _vp]),
+    "pm_server_result": (_i, [_vp, _i, _vp]),
+    "pm_server_rhs": (_i, [_i, _i, _u64, _d, _vp, _vp]),
     "pm_graph_begin": (_i, [_vp]),
     "pm_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
     "pm_graph_launch": (_i, [_vp, _vp]),
@@ -279,4 +308,51 @@ class Batch:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.pm_batch_destroy(self.h)
+            self.h = None
+
+
+class Server:
+    """Continuous-batching Poisson solve server (pm_server).  mode 0: batched
+    FP32-preconditioned FP64 refinement; mode 1: plain FP64, one at a time."""
+
+    def __init__(self, mode=0, hbm_budget=200 << 30, max_active=64, max_iter=30, sigma=0.05, pre=2, post=2,
+                 bottom=10):
+        o = ServerOptions()
+        lib().pm_server_default_options(ctypes.byref(o))
+        o.mode, o.hbm_budget, o.max_active, o.max_iter, o.sigma = mode, hbm_budget, max_active, max_iter, sigma
+        o.pre, o.post, o.bottom = pre, post, bottom
+        h = _vp()
+        call("pm_server_create", ctypes.byref(h), ctypes.byref(o))
+        self.h = h
+
+    def request_bytes(self, n):
+        b = ctypes.c_size_t()
+        call("pm_server_request_bytes", self.h, n, ctypes.byref(b))
+        return b.value
+
+    def resident_bytes(self):
+        b = ctypes.c_size_t()
+        call("pm_server_resident_bytes", self.h, ctypes.byref(b))
+        return b.value
+
+    def run(self, requests, stream=None):
+        """requests: list of dicts (arrival_s, n, ngauss, seed, tol, keep_result);
+        returns a list of stats dicts in the same order."""
+        arr = (Request * len(requests))()
+        for i, r in enumerate(requests):
+            arr[i] = Request(float(r.get("arrival_s", 0.0)), int(r["n"]), int(r["ngauss"]), int(r["seed"]),
+                             float(r.get("tol", 1e-8)), int(r.get("keep_result", 0)))
+        st = (RequestStats * len(requests))()
+        call("pm_server_run", self.h, len(requests), arr, st, stream)
+        return [{k: getattr(x, k) for k, _ in RequestStats._fields_} for x in st]
+
+    def result(self, index, n):
+        import numpy as np
+        out = np.empty(n ** 3)
+        call("pm_server_result", self.h, index, out.ctypes.data_as(_vp))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pm_server_destroy(self.h)
             self.h = None

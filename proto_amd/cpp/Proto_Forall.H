@@ -115,7 +115,21 @@ void launch(const Box& b, F f, Args&&... args)
     if (hipGetLastError() != hipSuccess) { throw ProtoError(PM_ERR_HIP, "forall: kernel launch failed"); }
 }
 
+// flop accounting of the forallOp family (Proto feeds it to its timers)
+inline unsigned long long& flopCounter()
+{
+    static unsigned long long n = 0;
+    return n;
+}
+inline void countFlops(unsigned long long perPoint, const Box& b)
+{
+    flopCounter() += perPoint * (unsigned long long)b.size();
+}
+
 }  // namespace forall_detail
+
+// total flops recorded by forallOp / forallInPlaceOp calls so far
+inline unsigned long long forallFlops() { return forall_detail::flopCounter(); }
 
 // forallInPlace(F, args...) (Proto_ForallImplem.H:257-268)
 template<typename F, typename... Args>
@@ -157,6 +171,73 @@ BoxData<T, C, MEM, D, E> forall_p(const F& f, Args&&... args)
     BoxData<T, C, MEM, D, E> out(b);
     forall_detail::launch<true>(b, f, out, std::forward<Args>(args)...);
     return out;
+}
+
+template<typename T, unsigned int C = 1, MemType MEM = MEMTYPE_DEFAULT, unsigned int D = 1, unsigned int E = 1,
+         typename F, typename... Args>
+BoxData<T, C, MEM, D, E> forall(const F& f, Box b, Args&&... args)
+{
+    BoxData<T, C, MEM, D, E> out(b);
+    forall_detail::launch<false>(b, f, out, std::forward<Args>(args)...);
+    return out;
+}
+template<typename T, unsigned int C = 1, MemType MEM = MEMTYPE_DEFAULT, unsigned int D = 1, unsigned int E = 1,
+         typename F, typename... Args>
+BoxData<T, C, MEM, D, E> forall_p(const F& f, Box b, Args&&... args)
+{
+    BoxData<T, C, MEM, D, E> out(b);
+    forall_detail::launch<true>(b, f, out, std::forward<Args>(args)...);
+    return out;
+}
+
+// forallOp / forallOp_p / forallInPlaceOp / forallInPlaceOp_p
+// (Proto_ForallImplem.H:178-400): forall plus a flop count per point and a
+// timer name (the count is accumulated; the name is not used).
+template<typename T, unsigned int C = 1, MemType MEM = MEMTYPE_DEFAULT, unsigned int D = 1, unsigned int E = 1,
+         typename F, typename... Args>
+BoxData<T, C, MEM, D, E> forallOp(unsigned long long flops, const char*, const F& f, Args&&... args)
+{
+    Box b = forall_detail::intersect(args...);
+    forall_detail::countFlops(flops, b);
+    BoxData<T, C, MEM, D, E> out(b);
+    forall_detail::launch<false>(b, f, out, std::forward<Args>(args)...);
+    return out;
+}
+template<typename T, unsigned int C = 1, MemType MEM = MEMTYPE_DEFAULT, unsigned int D = 1, unsigned int E = 1,
+         typename F, typename... Args>
+BoxData<T, C, MEM, D, E> forallOp_p(unsigned long long flops, const char*, const F& f, Args&&... args)
+{
+    Box b = forall_detail::intersect(args...);
+    forall_detail::countFlops(flops, b);
+    BoxData<T, C, MEM, D, E> out(b);
+    forall_detail::launch<true>(b, f, out, std::forward<Args>(args)...);
+    return out;
+}
+template<typename F, typename... Args>
+void forallInPlaceOp(unsigned long long flops, const char*, const F& f, Args&&... args)
+{
+    Box b = forall_detail::intersect(args...);
+    forall_detail::countFlops(flops, b);
+    forall_detail::launch<false>(b, f, std::forward<Args>(args)...);
+}
+template<typename F, typename... Args>
+void forallInPlaceOp(unsigned long long flops, const char*, const F& f, Box b, Args&&... args)
+{
+    forall_detail::countFlops(flops, b);
+    forall_detail::launch<false>(b, f, std::forward<Args>(args)...);
+}
+template<typename F, typename... Args>
+void forallInPlaceOp_p(unsigned long long flops, const char*, const F& f, Args&&... args)
+{
+    Box b = forall_detail::intersect(args...);
+    forall_detail::countFlops(flops, b);
+    forall_detail::launch<true>(b, f, std::forward<Args>(args)...);
+}
+template<typename F, typename... Args>
+void forallInPlaceOp_p(unsigned long long flops, const char*, const F& f, Box b, Args&&... args)
+{
+    forall_detail::countFlops(flops, b);
+    forall_detail::launch<true>(b, f, std::forward<Args>(args)...);
 }
 
 }  // namespace Proto

@@ -331,6 +331,14 @@ int pm_mg_level_layout(const pm_mg* mg, int level, const pm_layout** layout)
     return PM_OK;
 }
 
+int pm_mg_set_tolerance(pm_mg* mg, double tol, int max_iter)
+{
+    PM_REQUIRE(mg && tol >= 0 && max_iter >= 0, "pm_mg_set_tolerance: bad argument");
+    mg->opt.tol = tol;
+    mg->opt.max_iter = max_iter;
+    return PM_OK;
+}
+
 int pm_mg_vcycle(pm_mg* mg, void* phi, const void* rhs, pm_stream_t stream)
 {
     PM_REQUIRE(mg && phi && rhs, "pm_mg_vcycle: null argument");

@@ -1,0 +1,978 @@
+// Continuous-batching Poisson solve server.
+//
+// Mode 0 keeps every active request's hierarchy resident and advances all of
+// them together: one loop iteration is one FP32 GSRB V-cycle (the
+// preconditioner), the FP64 correction phi += e, the FP64 residual
+// r = rhs - L(phi) (rounded to FP32 as the next V-cycle's right-hand side) and
+// one read-back of every request's max-norm.  Each level operator of the
+// V-cycle is ONE launch whose work list holds the tiles of every request that
+// has that level, so small requests ride along with large ones instead of
+// paying a launch sequence of their own.
+//
+// Arithmetic per request is that of the multigrid driver with precision 1 on a
+// single periodic box (pm_mg.hip, pm_level.hip): the GSRB update of
+// LaplaceOp.H:95-123, the residual-restriction of LevelMultigrid.cpp:51-73,
+// constant prolongation (InterpStencil::Constant, staged), the FP32
+// coefficients Lap_j * (float)(-1/h^2) and lambda = (float)(2 * (-h^2 / 12)).
+// The sweep is done in one out-of-place pass (red cells of plane k+1, then
+// black cells of plane k, in an LDS ring of planes, red cells recomputed in
+// the one-cell halo from periodic images), which gives the bits of the
+// exchange / red / exchange / black sequence.  So a served result equals the
+// driver's answer for the same problem alone, and never depends on the mix.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "pm_internal.hpp"
+
+namespace pm {
+
+constexpr int SR = 8;    // rows per sweep tile
+constexpr int SZC = 32;  // planes per sweep tile
+constexpr int SCELLS = 2048;  // cells per item of the pointwise kernels
+
+struct SLevel {
+    float* phi;
+    float* tmp;
+    float* rhs;
+    int n;
+    float c0, c1, lam;  // centre / neighbour coefficients, GSRB weight
+};
+
+struct SReq {
+    double* phi;   // FP64 solution, n^3
+    double* rhs;   // FP64 right-hand side, n^3
+    int lev0;      // index of its finest level in the SLevel table
+    int slot;      // norm slot
+    int n;
+    int pad;
+    double d0, d1; // FP64 residual coefficients
+};
+
+struct SItem {
+    int lev;  // SLevel index (sweep, restrict: fine level, prolong: fine level) or SReq index
+    int a, b, c;
+};
+
+__device__ inline int wrapi(int v, int n)
+{
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// One GSRB sweep (both colours) of the tile rows [y0, y0+SR) x planes [z0, z1)
+// of level L, reading `in` and writing `out` (dir 0: phi -> tmp, 1: tmp -> phi).
+__global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv, const SItem* __restrict__ items, int dir)
+{
+    extern __shared__ __align__(16) unsigned char smraw[];
+    float* sm = reinterpret_cast<float*>(smraw);
+    const SItem it = items[blockIdx.x];
+    const SLevel L = lv[it.lev];
+    const int n = L.n, W = n + 4;
+    const int y0 = it.a, y1 = min(y0 + SR, n), z0 = it.b, z1 = it.c;
+    const int NP = (y1 - y0 + 4) * W;  // rows [y0-2, y1+2), columns [-2, n+2)
+    const float* in = dir ? L.tmp : L.phi;
+    float* out = dir ? L.phi : L.tmp;
+    const long nn = (long)n * n;
+    auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
+    auto load = [&](int z) {
+        const float* base = in + nn * wrapi(z, n);
+        float* pl = slot(z);
+        for (int e = threadIdx.x; e < NP; e += 256) {
+            const int er = e / W, ec = e - er * W;
+            pl[e] = base[wrapi(ec - 2, n) + (long)n * wrapi(y0 - 2 + er, n)];
+        }
+    };
+    // colour cells ((x+y+z+colour) even) of plane z, rows [ya, yb), columns [xa, xb)
+    auto update = [&](int z, int ya, int yb, int xa, int xb, int colour) {
+        float* zc = slot(z);
+        const float* zm = slot(z - 1);
+        const float* zp = slot(z + 1);
+        const float* rz = L.rhs + nn * wrapi(z, n);
+        const int hw = (xb - xa + 1) / 2;
+        const int ncell = (yb - ya) * hw;
+        for (int c = threadIdx.x; c < ncell; c += 256) {
+            const int yy = ya + c / hw;
+            const int xx = xa + 2 * (c % hw) + ((xa + yy + z + colour) & 1);
+            if (xx >= xb) { continue; }
+            const int li = (yy - (y0 - 2)) * W + (xx + 2);
+            float s = 0;
+            s = s + L.c0 * zc[li];
+            s = s + L.c1 * zc[li + 1];
+            s = s + L.c1 * zc[li - 1];
+            s = s + L.c1 * zc[li + W];
+            s = s + L.c1 * zc[li - W];
+            s = s + L.c1 * zp[li];
+            s = s + L.c1 * zm[li];
+            s = s + rz[wrapi(xx, n) + (long)n * wrapi(yy, n)];
+            zc[li] = zc[li] + L.lam * s;
+        }
+    };
+    load(z0 - 2);
+    load(z0 - 1);
+    for (int k = z0 - 2; k < z1; k++) {
+        load(k + 2);  // into the slot of plane k-2, last read before the previous barrier
+        __syncthreads();
+        update(k + 1, y0 - 1, y1 + 1, -1, n + 1, 0);  // red, one-cell halo included
+        __syncthreads();
+        if (k >= z0) {
+            update(k, y0, y1, 0, n, 1);  // black, tile cells
+            __syncthreads();
+            const float* pl = slot(k);
+            float* ob = out + nn * k;
+            const int cnt = (y1 - y0) * n;
+            for (int e = threadIdx.x; e < cnt; e += 256) {
+                const int r = e / n, xx = e - r * n;
+                ob[xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Coarse right-hand side = AvgDown(rhs + sum_j c_j phi(p + o_j)) of the fine
+// level (LevelMultigrid.cpp:51-73 order), and coarse phi = 0.
+__global__ __launch_bounds__(256) void k_srv_restrict(const SLevel* __restrict__ lv, const SItem* __restrict__ items)
+{
+    const SItem it = items[blockIdx.x];
+    const SLevel F = lv[it.lev], C = lv[it.lev + 1];
+    const int n = F.n, nc = C.n;
+    const long nn = (long)n * n;
+    const float avg = 0.125f;
+    for (int t = it.a + threadIdx.x; t < it.a + it.b; t += 256) {
+        const int X = t % nc, Y = (t / nc) % nc, Z = t / (nc * nc);
+        float acc = 0;
+#pragma unroll
+        for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+            for (int ky = 0; ky < 2; ky++)
+#pragma unroll
+                for (int kx = 0; kx < 2; kx++) {
+                    const int x = 2 * X + kx, y = 2 * Y + ky, z = 2 * Z + kz;
+                    const float* pz = F.phi + nn * z;
+                    const long row = (long)n * y;
+                    float s = 0;
+                    s = s + F.rhs[x + row + nn * z];
+                    s = s + F.c0 * pz[x + row];
+                    s = s + F.c1 * pz[wrapi(x + 1, n) + row];
+                    s = s + F.c1 * pz[wrapi(x - 1, n) + row];
+                    s = s + F.c1 * pz[x + (long)n * wrapi(y + 1, n)];
+                    s = s + F.c1 * pz[x + (long)n * wrapi(y - 1, n)];
+                    s = s + F.c1 * F.phi[x + row + nn * wrapi(z + 1, n)];
+                    s = s + F.c1 * F.phi[x + row + nn * wrapi(z - 1, n)];
+                    acc = acc + avg * s;
+                }
+        C.rhs[t] = acc;
+        C.phi[t] = 0.0f;
+    }
+}
+
+// Fine phi += constant prolongation of the coarse phi (three staged passes of
+// InterpStencil::Constant, each 0 + 1 * v).
+__global__ __launch_bounds__(256) void k_srv_prolong(const SLevel* __restrict__ lv, const SItem* __restrict__ items)
+{
+    const SItem it = items[blockIdx.x];
+    const SLevel F = lv[it.lev], C = lv[it.lev + 1];
+    const int n = F.n, nc = C.n;
+    const float one = 1.0f;
+    for (int t = it.a + threadIdx.x; t < it.a + it.b; t += 256) {
+        const int x = t % n, y = (t / n) % n, z = t / (n * n);
+        float v = 0;
+        v = v + one * C.phi[(x >> 1) + (long)nc * ((y >> 1) + (long)nc * (z >> 1))];
+        float v1 = 0;
+        v1 = v1 + one * v;
+        float v2 = 0;
+        v2 = v2 + one * v1;
+        F.phi[t] = F.phi[t] + v2;
+    }
+}
+
+// FP64 residual r = rhs + sum_j d_j phi(p + o_j) of the finest level: FP32 copy
+// into the level-0 rhs, level-0 phi (the correction) = 0, max|r| into the
+// request's norm slot (non-negative doubles order like their bit patterns).
+__global__ __launch_bounds__(256) void k_srv_residual(const SReq* __restrict__ rq, const SLevel* __restrict__ lv,
+                                                      const SItem* __restrict__ items,
+                                                      unsigned long long* __restrict__ norms)
+{
+    __shared__ double red[16];
+    const SItem it = items[blockIdx.x];
+    const SReq R = rq[it.lev];
+    const SLevel L = lv[R.lev0];
+    const int n = R.n;
+    const long nn = (long)n * n;
+    double m = 0;
+    for (int t = it.a + threadIdx.x; t < it.a + it.b; t += 256) {
+        const int x = t % n, y = (t / n) % n, z = t / (n * n);
+        const double* pz = R.phi + nn * z;
+        const long row = (long)n * y;
+        double s = 0;
+        s = s + R.rhs[t];
+        s = s + R.d0 * pz[x + row];
+        s = s + R.d1 * pz[wrapi(x + 1, n) + row];
+        s = s + R.d1 * pz[wrapi(x - 1, n) + row];
+        s = s + R.d1 * pz[x + (long)n * wrapi(y + 1, n)];
+        s = s + R.d1 * pz[x + (long)n * wrapi(y - 1, n)];
+        s = s + R.d1 * R.phi[x + row + nn * wrapi(z + 1, n)];
+        s = s + R.d1 * R.phi[x + row + nn * wrapi(z - 1, n)];
+        L.rhs[t] = (float)s;
+        L.phi[t] = 0.0f;
+        const double a = s > 0 ? s : -s;
+        m = m < a ? a : m;
+    }
+    m = block_reduce(PM_RED_MAX, m, red);
+    if (threadIdx.x == 0) { atomicMax(norms + R.slot, (unsigned long long)__double_as_longlong(m)); }
+}
+
+// phi (FP64) += e (the level-0 FP32 correction).
+__global__ __launch_bounds__(256) void k_srv_correct(const SReq* __restrict__ rq, const SLevel* __restrict__ lv,
+                                                     const SItem* __restrict__ items)
+{
+    const SItem it = items[blockIdx.x];
+    const SReq R = rq[it.lev];
+    const float* e = lv[R.lev0].phi;
+    for (int t = it.a + threadIdx.x; t < it.a + it.b; t += 256) { R.phi[t] = R.phi[t] + (double)e[t]; }
+}
+
+__global__ __launch_bounds__(256) void k_srv_zero_u64(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { p[i] = 0ull; }
+}
+
+__global__ __launch_bounds__(256) void k_srv_zero_f64(double* p, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) { p[i] = 0.0; }
+}
+
+// rhs(x,y,z) = sum_k ((a_k * tx_k[x]) * ty_k[y]) * tz_k[z]; tables k-major, n each.
+__global__ __launch_bounds__(256) void k_srv_gauss(double* __restrict__ rhs, int n, int K, const double* __restrict__ tab)
+{
+    const long total = (long)n * n * n;
+    const double* amp = tab;
+    const double* tx = tab + K;
+    const double* ty = tx + (long)K * n;
+    const double* tz = ty + (long)K * n;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % n), y = (int)((t / n) % n), z = (int)(t / ((long)n * n));
+        double v = 0;
+        for (int k = 0; k < K; k++) {
+            v = v + ((amp[k] * tx[(long)k * n + x]) * ty[(long)k * n + y]) * tz[(long)k * n + z];
+        }
+        rhs[t] = v;
+    }
+}
+
+// Fixed-order mean removal: partial sums over chunks of 4096 cells (block
+// tree), their sum in chunk order by one block, then rhs -= sum / n^3.
+constexpr int MEAN_CHUNK = 4096;
+__global__ __launch_bounds__(256) void k_srv_chunk_sums(const double* __restrict__ rhs, long total,
+                                                        double* __restrict__ part)
+{
+    __shared__ double red[16];
+    const long c0 = (long)blockIdx.x * MEAN_CHUNK;
+    double s = 0;
+    for (long i = c0 + threadIdx.x; i < c0 + MEAN_CHUNK && i < total; i += 256) { s = s + rhs[i]; }
+    s = block_reduce(PM_RED_SUM, s, red);
+    if (threadIdx.x == 0) { part[blockIdx.x] = s; }
+}
+
+__global__ __launch_bounds__(256) void k_srv_sum_parts(double* __restrict__ part, int nparts)
+{
+    __shared__ double red[16];
+    double s = 0;
+    for (int i = threadIdx.x; i < nparts; i += 256) { s = s + part[i]; }
+    s = block_reduce(PM_RED_SUM, s, red);
+    if (threadIdx.x == 0) { part[nparts] = s; }
+}
+
+__global__ __launch_bounds__(256) void k_srv_sub_mean(double* __restrict__ rhs, long total,
+                                                      const double* __restrict__ sum)
+{
+    const double mean = *sum / (double)total;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        rhs[t] = rhs[t] - mean;
+    }
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+namespace {
+
+double now_s(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+std::vector<int> level_sizes(int n)
+{
+    std::vector<int> ns{n};
+    while (ns.back() % 2 == 0 && ns.back() > 8) { ns.push_back(ns.back() / 2); }
+    return ns;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// SplitMix64 stream of pm_fill_uniform: value i of (seed, stream) in [-1, 1).
+double splitmix_u(uint64_t seed, uint64_t stream, uint64_t i)
+{
+    uint64_t z = seed + stream * 0x632BE59BD9B4E019ull + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((double)(z >> 11) * 0x1.0p-53) * 2.0 - 1.0;
+}
+
+// One resident hierarchy of a mode-0 request (one allocation).
+struct Hier {
+    int n = 0;
+    void* base = nullptr;
+    size_t bytes = 0;
+    double* phi64 = nullptr;
+    double* rhs64 = nullptr;
+    double* tab = nullptr;  // Gaussian tables and mean partials
+    std::vector<float*> phi, tmp, rhs;
+};
+
+size_t tab_elems(int n) { return 32 + 3 * 32 * (size_t)n + ((size_t)n * n * n + MEAN_CHUNK - 1) / MEAN_CHUNK + 1; }
+
+size_t hier_bytes(int n)
+{
+    const size_t n3 = (size_t)n * n * n;
+    size_t b = 2 * align256(8 * n3) + align256(8 * tab_elems(n));
+    for (int m : level_sizes(n)) { b += 3 * align256(4 * (size_t)m * m * m); }
+    return b;
+}
+
+struct Active {
+    int idx;
+    Hier* h;
+    std::vector<int> ns;
+    double res0 = 0, r = 0;
+    int iters = 0;
+    bool fresh = true;
+};
+
+struct Launch {
+    int kind;  // 0 sweep, 1 restrict, 2 prolong, 3 correct, 4 residual
+    int off, count, dir, nmax;
+};
+
+}  // namespace
+
+struct pm_server {
+    pm_server_options opt;
+    std::map<int, std::vector<Hier*>> pool;  // free hierarchies by n
+    size_t resident = 0;
+    // device tables
+    SLevel* d_lv = nullptr;
+    size_t cap_lv = 0;
+    SReq* d_rq = nullptr;
+    size_t cap_rq = 0;
+    SItem* d_items = nullptr;
+    size_t cap_items = 0;
+    unsigned long long* d_norms = nullptr;
+    unsigned long long* h_norms = nullptr;
+    std::vector<Launch> plan;
+    // kept results of the last run
+    std::map<int, std::vector<double>> results;
+    // mode 1: one pm_mg (FP64) per size, with its phi / rhs
+    struct Plain {
+        pm_layout* layout = nullptr;
+        pm_mg* mg = nullptr;
+        double* phi = nullptr;  // ghost 1
+        double* rhs = nullptr;
+        double* tab = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<int, Plain> plain;
+};
+
+namespace {
+
+int grow(void** p, size_t* cap, size_t need)
+{
+    if (need <= *cap) { return PM_OK; }
+    if (*p) { hipFree(*p); *p = nullptr; }
+    size_t c = std::max(need, *cap * 2);
+    if (hipMalloc(p, c) != hipSuccess) {
+        *cap = 0;
+        set_error("server: device allocation of %zu bytes failed", c);
+        return PM_ERR_NOMEM;
+    }
+    *cap = c;
+    return PM_OK;
+}
+
+void free_hier(Hier* h)
+{
+    hipFree(h->base);
+    delete h;
+}
+
+int alloc_hier(int n, Hier** out)
+{
+    Hier* h = new Hier();
+    h->n = n;
+    h->bytes = hier_bytes(n);
+    if (hipMalloc(&h->base, h->bytes) != hipSuccess) {
+        delete h;
+        set_error("server: hierarchy allocation of %zu bytes failed", hier_bytes(n));
+        return PM_ERR_NOMEM;
+    }
+    char* p = (char*)h->base;
+    const size_t n3 = (size_t)n * n * n;
+    h->phi64 = (double*)p;
+    p += align256(8 * n3);
+    h->rhs64 = (double*)p;
+    p += align256(8 * n3);
+    h->tab = (double*)p;
+    p += align256(8 * tab_elems(n));
+    for (int m : level_sizes(n)) {
+        const size_t b = align256(4 * (size_t)m * m * m);
+        h->phi.push_back((float*)p);
+        p += b;
+        h->tmp.push_back((float*)p);
+        p += b;
+        h->rhs.push_back((float*)p);
+        p += b;
+    }
+    *out = h;
+    return PM_OK;
+}
+
+// Gaussian tables on the host: amplitudes, then x, y, z factors exp(-d^2 / (2 sigma^2)),
+// d the periodic minimum-image distance from the cell centre (i + 1/2) / n.
+void gauss_tables(int n, int K, uint64_t seed, double sigma, std::vector<double>& tab)
+{
+    tab.assign((size_t)K + 3 * (size_t)K * n, 0.0);
+    for (int k = 0; k < K; k++) {
+        const double u = splitmix_u(seed, 7, 3 * (uint64_t)K + k);
+        tab[k] = u >= 0 ? 1.0 : -1.0;
+        for (int d = 0; d < 3; d++) {
+            const double c = (splitmix_u(seed, 7, 3 * (uint64_t)k + d) + 1.0) * 0.5;
+            double* t = tab.data() + K + ((size_t)d * K + k) * n;
+            for (int i = 0; i < n; i++) {
+                double dx = (i + 0.5) / n - c;
+                dx -= std::nearbyint(dx);
+                t[i] = std::exp(-(dx * dx) / (2.0 * sigma * sigma));
+            }
+        }
+    }
+}
+
+// rhs (n^3 FP64, device) = Gaussian sum minus its mean; tab = device scratch.
+int make_rhs(double* rhs, double* tab_dev, int n, const pm_request& q, double sigma, hipStream_t s)
+{
+    std::vector<double> tab;
+    gauss_tables(n, q.ngauss, q.seed, sigma, tab);
+    PM_CHECK_HIP(hipMemcpyAsync(tab_dev, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s));
+    PM_CHECK_HIP(hipStreamSynchronize(s));  // the host table goes out of scope
+    const long total = (long)n * n * n;
+    const int nparts = (int)((total + MEAN_CHUNK - 1) / MEAN_CHUNK);
+    double* part = tab_dev + tab.size();
+    const unsigned g = (unsigned)std::min<long>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_srv_gauss, dim3(g), dim3(256), 0, s, rhs, n, q.ngauss, (const double*)tab_dev);
+    hipLaunchKernelGGL(k_srv_chunk_sums, dim3(nparts), dim3(256), 0, s, (const double*)rhs, total, part);
+    hipLaunchKernelGGL(k_srv_sum_parts, dim3(1), dim3(256), 0, s, part, nparts);
+    hipLaunchKernelGGL(k_srv_sub_mean, dim3(g), dim3(256), 0, s, rhs, total, (const double*)(part + nparts));
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+void level_coefs(int m, float* c0, float* c1, float* lam)
+{
+    const double dx = 1.0 / m;
+    const double mm = -1.0 / (dx * dx);
+    *c0 = (float)(-6.0) * (float)mm;
+    *c1 = (float)(1.0) * (float)mm;
+    double l = -(dx * dx) / (4.0 * 3);
+    l *= 2.0;
+    *lam = (float)(1.0 * l);
+}
+
+void push_cells(std::vector<SItem>& items, int lev, long cells)
+{
+    for (long c = 0; c < cells; c += SCELLS) { items.push_back(SItem{lev, (int)c, (int)std::min<long>(SCELLS, cells - c), 0}); }
+}
+
+// Device tables and the launch plan of one loop iteration for the active set.
+int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
+{
+    std::vector<SLevel> lv;
+    std::vector<SReq> rq;
+    std::vector<int> lev0(act.size());
+    int maxd = 0;
+    for (size_t r = 0; r < act.size(); r++) {
+        Active& a = act[r];
+        lev0[r] = (int)lv.size();
+        for (size_t l = 0; l < a.ns.size(); l++) {
+            SLevel L;
+            L.phi = a.h->phi[l];
+            L.tmp = a.h->tmp[l];
+            L.rhs = a.h->rhs[l];
+            L.n = a.ns[l];
+            level_coefs(a.ns[l], &L.c0, &L.c1, &L.lam);
+            lv.push_back(L);
+        }
+        const double hsqinv = 1. / ((1.0 / a.ns[0]) * (1.0 / a.ns[0]));
+        SReq q;
+        q.phi = a.h->phi64;
+        q.rhs = a.h->rhs64;
+        q.lev0 = lev0[r];
+        q.slot = (int)r;
+        q.n = a.ns[0];
+        q.pad = 0;
+        q.d0 = (-6.0) * (-hsqinv);
+        q.d1 = (1.0) * (-hsqinv);
+        rq.push_back(q);
+        maxd = std::max(maxd, (int)a.ns.size() - 1);
+    }
+    std::vector<SItem> items;
+    S->plan.clear();
+    auto begin = [&](int kind, int dir) {
+        S->plan.push_back(Launch{kind, (int)items.size(), 0, dir, 0});
+    };
+    auto end = [&]() {
+        Launch& L = S->plan.back();
+        L.count = (int)items.size() - L.off;
+        if (L.count == 0) { S->plan.pop_back(); }
+    };
+    auto sweep_items = [&](int levidx, int m) {
+        for (int z0 = 0; z0 < m; z0 += SZC)
+            for (int y0 = 0; y0 < m; y0 += SR) { items.push_back(SItem{levidx, y0, z0, std::min(m, z0 + SZC)}); }
+        S->plan.back().nmax = std::max(S->plan.back().nmax, m);
+    };
+    const int pre = S->opt.pre, post = S->opt.post, bot = S->opt.bottom;
+    // V-cycle of every non-fresh request (fresh ones only get their first residual)
+    for (int d = 0; d <= maxd; d++) {
+        const int nsw = std::max(pre, bot);
+        for (int sw = 0; sw < nsw; sw++) {
+            begin(0, sw & 1);
+            for (size_t r = 0; r < act.size(); r++) {
+                const Active& a = act[r];
+                const int bd = (int)a.ns.size() - 1;
+                if (a.fresh || d > bd) { continue; }
+                if ((d < bd && sw < pre) || (d == bd && sw < bot)) { sweep_items(lev0[r] + d, a.ns[d]); }
+            }
+            end();
+        }
+        begin(1, 0);
+        for (size_t r = 0; r < act.size(); r++) {
+            const Active& a = act[r];
+            if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
+            const long c = (long)a.ns[d + 1] * a.ns[d + 1] * a.ns[d + 1];
+            push_cells(items, lev0[r] + d, c);
+        }
+        end();
+    }
+    for (int d = maxd - 1; d >= 0; d--) {
+        begin(2, 0);
+        for (size_t r = 0; r < act.size(); r++) {
+            const Active& a = act[r];
+            if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
+            push_cells(items, lev0[r] + d, (long)a.ns[d] * a.ns[d] * a.ns[d]);
+        }
+        end();
+        for (int sw = 0; sw < post; sw++) {
+            begin(0, sw & 1);
+            for (size_t r = 0; r < act.size(); r++) {
+                const Active& a = act[r];
+                if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
+                sweep_items(lev0[r] + d, a.ns[d]);
+            }
+            end();
+        }
+    }
+    begin(3, 0);
+    for (size_t r = 0; r < act.size(); r++) {
+        if (!act[r].fresh) { push_cells(items, (int)r, (long)act[r].ns[0] * act[r].ns[0] * act[r].ns[0]); }
+    }
+    end();
+    begin(4, 0);
+    for (size_t r = 0; r < act.size(); r++) { push_cells(items, (int)r, (long)act[r].ns[0] * act[r].ns[0] * act[r].ns[0]); }
+    end();
+    int rc = grow((void**)&S->d_lv, &S->cap_lv, sizeof(SLevel) * lv.size());
+    if (rc == PM_OK) { rc = grow((void**)&S->d_rq, &S->cap_rq, sizeof(SReq) * rq.size()); }
+    if (rc == PM_OK) { rc = grow((void**)&S->d_items, &S->cap_items, sizeof(SItem) * std::max<size_t>(1, items.size())); }
+    if (rc) { return rc; }
+    // the stream is idle here (the last norm read-back synchronised it)
+    PM_CHECK_HIP(hipMemcpy(S->d_lv, lv.data(), sizeof(SLevel) * lv.size(), hipMemcpyHostToDevice));
+    PM_CHECK_HIP(hipMemcpy(S->d_rq, rq.data(), sizeof(SReq) * rq.size(), hipMemcpyHostToDevice));
+    if (!items.empty()) {
+        PM_CHECK_HIP(hipMemcpy(S->d_items, items.data(), sizeof(SItem) * items.size(), hipMemcpyHostToDevice));
+    }
+    (void)s;
+    return PM_OK;
+}
+
+int run_plan(pm_server* S, int nact, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_srv_zero_u64, dim3((nact + 255) / 256), dim3(256), 0, s, S->d_norms, nact);
+    for (const Launch& L : S->plan) {
+        const SItem* it = S->d_items + L.off;
+        switch (L.kind) {
+        case 0: {
+            const size_t lds = sizeof(float) * 4 * (SR + 4) * (size_t)(L.nmax + 4);
+            hipLaunchKernelGGL(k_srv_gsrb, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+            break;
+        }
+        case 1:
+            hipLaunchKernelGGL(k_srv_restrict, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_srv_prolong, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
+            break;
+        case 3:
+            hipLaunchKernelGGL(k_srv_correct, dim3(L.count), dim3(256), 0, s, (const SReq*)S->d_rq,
+                               (const SLevel*)S->d_lv, it);
+            break;
+        default:
+            hipLaunchKernelGGL(k_srv_residual, dim3(L.count), dim3(256), 0, s, (const SReq*)S->d_rq,
+                               (const SLevel*)S->d_lv, it, S->d_norms);
+        }
+    }
+    PM_CHECK_LAUNCH();
+    PM_CHECK_HIP(hipMemcpyAsync(S->h_norms, S->d_norms, sizeof(unsigned long long) * nact, hipMemcpyDeviceToHost, s));
+    PM_CHECK_HIP(hipStreamSynchronize(s));
+    return PM_OK;
+}
+
+// Take a hierarchy for n from the pool or allocate one within the budget
+// (evicting free hierarchies of other sizes when needed).  *h = null: wait.
+int take_hier(pm_server* S, int n, Hier** h)
+{
+    *h = nullptr;
+    auto& fl = S->pool[n];
+    if (!fl.empty()) {
+        *h = fl.back();
+        fl.pop_back();
+        return PM_OK;
+    }
+    const size_t need = hier_bytes(n);
+    for (auto& kv : S->pool) {
+        while (S->resident + need > S->opt.hbm_budget && !kv.second.empty()) {
+            Hier* v = kv.second.back();
+            kv.second.pop_back();
+            S->resident -= v->bytes;
+            free_hier(v);
+        }
+    }
+    if (S->resident + need > S->opt.hbm_budget) { return PM_OK; }
+    int rc = alloc_hier(n, h);
+    if (rc == PM_OK) { S->resident += need; }
+    return rc;
+}
+
+int run_batched(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* st, hipStream_t s)
+{
+    std::vector<int> order(nreq);
+    for (int i = 0; i < nreq; i++) { order[i] = i; }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    size_t next = 0;
+    std::vector<int> waiting;
+    std::vector<Active> act;
+    int done = 0;
+    bool dirty = true;
+    const auto t0 = std::chrono::steady_clock::now();
+    while (done < nreq) {
+        const double t = now_s(t0);
+        while (next < order.size() && reqs[order[next]].arrival_s <= t) {
+            const int i = order[next++];
+            if (hier_bytes(reqs[i].n) > S->opt.hbm_budget) {
+                st[i].status = PM_SERVE_REJECTED;
+                st[i].admit_s = st[i].finish_s = t;
+                done++;
+            } else {
+                waiting.push_back(i);
+            }
+        }
+        // admission, first come first served
+        while (!waiting.empty() && (int)act.size() < S->opt.max_active) {
+            const int i = waiting.front();
+            Hier* h = nullptr;
+            int rc = take_hier(S, reqs[i].n, &h);
+            if (rc) { return rc; }
+            if (!h) { break; }
+            waiting.erase(waiting.begin());
+            Active a;
+            a.idx = i;
+            a.h = h;
+            a.ns = level_sizes(reqs[i].n);
+            const long total = (long)reqs[i].n * reqs[i].n * reqs[i].n;
+            rc = make_rhs(h->rhs64, h->tab, reqs[i].n, reqs[i], S->opt.sigma, s);
+            if (rc) { return rc; }
+            const unsigned g = (unsigned)std::min<long>((total + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_srv_zero_f64, dim3(g), dim3(256), 0, s, h->phi64, total);
+            PM_CHECK_LAUNCH();
+            st[i].admit_s = now_s(t0);
+            st[i].bytes = h->bytes;
+            act.push_back(a);
+            dirty = true;
+        }
+        if (act.empty()) {
+            if (next < order.size()) {
+                const double wait = reqs[order[next]].arrival_s - now_s(t0);
+                if (wait > 0) { std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001))); }
+            }
+            continue;
+        }
+        if (dirty) {
+            int rc = build_plan(S, act, s);
+            if (rc) { return rc; }
+            dirty = false;
+        }
+        int rc = run_plan(S, (int)act.size(), s);
+        if (rc) { return rc; }
+        const double tf = now_s(t0);
+        std::vector<Active> keep;
+        for (size_t r = 0; r < act.size(); r++) {
+            Active& a = act[r];
+            double v;
+            std::memcpy(&v, &S->h_norms[r], sizeof(double));
+            const pm_request& q = reqs[a.idx];
+            bool fin = false;
+            if (a.fresh) {
+                a.fresh = false;
+                a.res0 = a.r = v;
+                fin = v == 0.0;
+                dirty = true;  // its V-cycles start next iteration
+            } else {
+                a.iters++;
+                a.r = v;
+                fin = v < q.tol * a.res0 || a.iters >= S->opt.max_iter;
+            }
+            if (!fin) {
+                keep.push_back(a);
+                continue;
+            }
+            pm_request_stats& o = st[a.idx];
+            o.iters = a.iters;
+            o.rel_residual = a.res0 > 0 ? a.r / a.res0 : 0.0;
+            o.status = (a.r < q.tol * a.res0 || a.res0 == 0.0) ? PM_SERVE_SOLVED : PM_SERVE_NOT_CONVERGED;
+            o.finish_s = tf;
+            if (q.keep_result && o.status == PM_SERVE_SOLVED) {
+                std::vector<double>& out = S->results[a.idx];
+                out.resize((size_t)q.n * q.n * q.n);
+                PM_CHECK_HIP(hipMemcpy(out.data(), a.h->phi64, sizeof(double) * out.size(), hipMemcpyDeviceToHost));
+            }
+            S->pool[a.h->n].push_back(a.h);
+            done++;
+            dirty = true;
+        }
+        act.swap(keep);
+    }
+    return PM_OK;
+}
+
+// Mode 1: one request at a time, FP64 GSRB V-cycles of the multigrid driver.
+int run_plain(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* st, hipStream_t s)
+{
+    std::vector<int> order(nreq);
+    for (int i = 0; i < nreq; i++) { order[i] = i; }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i : order) {
+        const pm_request& q = reqs[i];
+        while (now_s(t0) < q.arrival_s) {
+            std::this_thread::sleep_for(std::chrono::duration<double>(std::min(q.arrival_s - now_s(t0), 0.001)));
+        }
+        const int n = q.n;
+        auto it = S->plain.find(n);
+        if (it == S->plain.end()) {
+            pm_server::Plain P;
+            const int nl = (int)level_sizes(n).size() - 1;
+            int dom[3] = {n, n, n}, per[3] = {1, 1, 1}, lows[3] = {0, 0, 0};
+            int rc = pm_layout_create(&P.layout, 3, dom, dom, per, 1, lows);
+            if (rc) { return rc; }
+            pm_mg_options o;
+            pm_mg_default_options(&o);
+            o.relax = 0;
+            o.pre = S->opt.pre;
+            o.post = S->opt.post;
+            o.bottom = S->opt.bottom;
+            o.precision = 0;
+            o.max_iter = S->opt.max_iter;
+            o.tol = q.tol;
+            rc = pm_mg_create(&P.mg, P.layout, nl, 1.0 / n, &o);
+            if (rc) { return rc; }
+            const size_t n3 = (size_t)n * n * n, ng = (size_t)(n + 2) * (n + 2) * (n + 2);
+            PM_CHECK_HIP(hipMalloc(&P.phi, 8 * ng));
+            PM_CHECK_HIP(hipMalloc(&P.rhs, 8 * n3));
+            PM_CHECK_HIP(hipMalloc(&P.tab, 8 * tab_elems(n)));
+            size_t mb = 0;
+            pm_mg_bytes(P.mg, &mb);
+            P.bytes = mb + 8 * (ng + n3 + tab_elems(n));
+            S->resident += P.bytes;
+            it = S->plain.emplace(n, P).first;
+        }
+        pm_server::Plain& P = it->second;
+        st[i].admit_s = now_s(t0);
+        st[i].bytes = P.bytes;
+        int rc = make_rhs(P.rhs, P.tab, n, q, S->opt.sigma, s);
+        if (rc) { return rc; }
+        const size_t ng = (size_t)(n + 2) * (n + 2) * (n + 2);
+        hipLaunchKernelGGL(k_srv_zero_f64, dim3(4096), dim3(256), 0, s, P.phi, (long)ng);
+        PM_CHECK_LAUNCH();
+        std::vector<double> hist(S->opt.max_iter + 1, 0.0);
+        int iters = 0;
+        rc = pm_mg_set_tolerance(P.mg, q.tol, S->opt.max_iter);  // the request's own tolerance
+        if (rc) { return rc; }
+        rc = pm_mg_solve(P.mg, P.phi, P.rhs, hist.data(), &iters, s);
+        if (rc != PM_OK && rc != PM_ERR_NOT_CONVERGED) { return rc; }
+        st[i].finish_s = now_s(t0);
+        st[i].iters = iters;
+        st[i].rel_residual = hist[0] > 0 ? hist[iters] / hist[0] : 0.0;
+        st[i].status = rc == PM_OK ? PM_SERVE_SOLVED : PM_SERVE_NOT_CONVERGED;
+        if (q.keep_result && rc == PM_OK) {
+            std::vector<double> g(ng);
+            PM_CHECK_HIP(hipMemcpy(g.data(), P.phi, 8 * ng, hipMemcpyDeviceToHost));
+            std::vector<double>& out = S->results[i];
+            out.resize((size_t)n * n * n);
+            const size_t X = n + 2;
+            for (int z = 0; z < n; z++)
+                for (int y = 0; y < n; y++)
+                    for (int x = 0; x < n; x++) {
+                        out[x + (size_t)n * (y + (size_t)n * z)] = g[(x + 1) + X * ((y + 1) + X * (z + 1))];
+                    }
+        }
+    }
+    return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pm_server_default_options(pm_server_options* o)
+{
+    if (!o) { return; }
+    std::memset(o, 0, sizeof(*o));
+    o->hbm_budget = (size_t)200 << 30;
+    o->max_active = 64;
+    o->mode = 0;
+    o->pre = 2;
+    o->post = 2;
+    o->bottom = 10;
+    o->max_iter = 30;
+    o->sigma = 0.05;
+}
+
+int pm_server_create(pm_server** out, const pm_server_options* o)
+{
+    PM_REQUIRE(out && o, "pm_server_create: null argument");
+    *out = nullptr;
+    PM_REQUIRE(o->mode == 0 || o->mode == 1, "pm_server_create: mode must be 0 or 1");
+    PM_REQUIRE(o->max_active >= 1 && o->max_active <= 4096, "pm_server_create: max_active must be 1..4096");
+    PM_REQUIRE(o->pre >= 0 && o->post >= 0 && o->bottom >= 0 && o->pre % 2 == 0 && o->post % 2 == 0 &&
+                   o->bottom % 2 == 0,
+               "pm_server_create: sweep counts must be even");
+    PM_REQUIRE(o->max_iter >= 1 && o->sigma > 0, "pm_server_create: bad max_iter / sigma");
+    pm_server* S = new pm_server();
+    S->opt = *o;
+    if (hipMalloc((void**)&S->d_norms, sizeof(unsigned long long) * o->max_active) != hipSuccess ||
+        hipHostMalloc((void**)&S->h_norms, sizeof(unsigned long long) * o->max_active, hipHostMallocDefault) !=
+            hipSuccess) {
+        pm_server_destroy(S);
+        set_error("pm_server_create: allocation failed");
+        return PM_ERR_NOMEM;
+    }
+    *out = S;
+    return PM_OK;
+}
+
+int pm_server_destroy(pm_server* S)
+{
+    if (!S) { return PM_OK; }
+    for (auto& kv : S->pool)
+        for (Hier* h : kv.second) { free_hier(h); }
+    for (auto& kv : S->plain) {
+        pm_mg_destroy(kv.second.mg);
+        pm_layout_destroy(kv.second.layout);
+        hipFree(kv.second.phi);
+        hipFree(kv.second.rhs);
+        hipFree(kv.second.tab);
+    }
+    hipFree(S->d_lv);
+    hipFree(S->d_rq);
+    hipFree(S->d_items);
+    hipFree(S->d_norms);
+    if (S->h_norms) { hipHostFree(S->h_norms); }
+    delete S;
+    return PM_OK;
+}
+
+int pm_server_request_bytes(const pm_server* S, int n, size_t* bytes)
+{
+    PM_REQUIRE(S && bytes && n >= 4 && n % 2 == 0, "pm_server_request_bytes: bad argument");
+    *bytes = hier_bytes(n);
+    return PM_OK;
+}
+
+int pm_server_resident_bytes(const pm_server* S, size_t* bytes)
+{
+    PM_REQUIRE(S && bytes, "pm_server_resident_bytes: null argument");
+    *bytes = S->resident;
+    return PM_OK;
+}
+
+int pm_server_run(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* st, pm_stream_t stream)
+{
+    PM_REQUIRE(S && reqs && st && nreq >= 1, "pm_server_run: bad argument");
+    for (int i = 0; i < nreq; i++) {
+        const pm_request& q = reqs[i];
+        PM_REQUIRE(q.n >= 8 && q.n <= 1024 && q.n % 2 == 0, "pm_server_run: request %d: n must be even, 8..1024", i);
+        const std::vector<int> ns = level_sizes(q.n);
+        PM_REQUIRE(ns.back() >= 4, "pm_server_run: request %d: coarsest level below 4 cells", i);
+        PM_REQUIRE(q.ngauss >= 1 && q.ngauss <= 32, "pm_server_run: request %d: ngauss must be 1..32", i);
+        PM_REQUIRE(q.tol > 0 && q.arrival_s >= 0, "pm_server_run: request %d: bad tol / arrival", i);
+        PM_REQUIRE(S->opt.mode == 1 || (size_t)q.n * q.n * q.n < ((size_t)1 << 31),
+                   "pm_server_run: request %d too large", i);
+    }
+    S->results.clear();
+    for (int i = 0; i < nreq; i++) {
+        pm_request_stats& o = st[i];
+        std::memset(&o, 0, sizeof(o));
+        o.arrival_s = reqs[i].arrival_s;
+        o.status = -1;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    return S->opt.mode == 0 ? run_batched(S, nreq, reqs, st, s) : run_plain(S, nreq, reqs, st, s);
+}
+
+int pm_server_rhs(int n, int ngauss, uint64_t seed, double sigma, double* rhs_dev, pm_stream_t stream)
+{
+    PM_REQUIRE(rhs_dev && n >= 4 && n <= 1024 && ngauss >= 1 && ngauss <= 32 && sigma > 0,
+               "pm_server_rhs: bad argument");
+    double* tab = nullptr;
+    PM_CHECK_HIP(hipMalloc((void**)&tab, sizeof(double) * tab_elems(n)));
+    pm_request q;
+    std::memset(&q, 0, sizeof(q));
+    q.n = n;
+    q.ngauss = ngauss;
+    q.seed = seed;
+    const hipStream_t s = (hipStream_t)stream;
+    int rc = make_rhs(rhs_dev, tab, n, q, sigma, s);
+    if (rc == PM_OK && hipStreamSynchronize(s) != hipSuccess) {
+        set_error("pm_server_rhs: synchronisation failed");
+        rc = PM_ERR_HIP;
+    }
+    hipFree(tab);
+    return rc;
+}
+
+int pm_server_result(pm_server* S, int index, double* phi_host)
+{
+    PM_REQUIRE(S && phi_host, "pm_server_result: null argument");
+    auto it = S->results.find(index);
+    PM_REQUIRE(it != S->results.end(), "pm_server_result: request %d was not kept or not solved", index);
+    std::memcpy(phi_host, it->second.data(), sizeof(double) * it->second.size());
+    return PM_OK;
+}
+
+}  // extern "C"

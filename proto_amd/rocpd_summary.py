@@ -2,18 +2,34 @@
 WRITE_SIZE per dispatch, calibrated against a copy kernel of known size) into
 the short text tables kept under profiles/.
 
-    python -m proto_amd.rocpd_summary trace  DB
+    python -m proto_amd.rocpd_summary trace  DB [NAME_SUBSTRING]
     python -m proto_amd.rocpd_summary pmc    FETCH_DB WRITE_DB COPY_BYTES
 """
 import sqlite3
 import sys
 
 
-def trace(db):
+def trace(db, pattern=""):
+    """Per kernel: dispatches, mean / median / min duration, the mean of the
+    last 5 dispatches (a benchmark's timed launches come last), registers and
+    LDS.  Durations in microseconds."""
     c = sqlite3.connect(db)
-    out = ["%-72s %6s %12s %12s" % ("kernel", "calls", "avg_us", "total_us")]
-    for name, calls, total, avg, pct in c.execute("select * from top_kernels"):
-        out.append("%-72s %6d %12.2f %12.1f" % (name[:72], calls, avg, total))
+    rows = {}
+    for name, dur, vgpr, agpr, lds in c.execute(
+            "select name, duration, vgpr_count, accum_vgpr_count, lds_size from kernels order by dispatch_id"):
+        if pattern and pattern not in name:
+            continue
+        r = rows.setdefault(name, {"d": [], "vgpr": vgpr, "agpr": agpr, "lds": lds})
+        r["d"].append(dur / 1000.0)
+    out = ["%-84s %6s %10s %10s %10s %10s %5s %6s" % ("kernel", "calls", "avg_us", "median_us", "min_us",
+                                                     "last5_us", "vgpr", "lds")]
+    for name, r in sorted(rows.items(), key=lambda kv: -sum(kv[1]["d"])):
+        d = r["d"]
+        sd = sorted(d)
+        last = d[-5:]
+        out.append("%-84s %6d %10.2f %10.2f %10.2f %10.2f %5d %6d" % (
+            name[:84], len(d), sum(d) / len(d), sd[len(sd) // 2], sd[0], sum(last) / len(last),
+            r["vgpr"] + (r["agpr"] or 0), r["lds"]))
     return "\n".join(out)
 
 
@@ -44,6 +60,6 @@ def pmc(fetch_db, write_db, copy_bytes):
 
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
-        print(trace(sys.argv[2]))
+        print(trace(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else ""))
     else:
         print(pmc(sys.argv[2], sys.argv[3], int(sys.argv[4])))

@@ -60,6 +60,33 @@ TEST(ForallPointAndScalar)
     for (auto it = B.begin(); it != B.end(); ++it) { CHECK_EQ(h(*it), 0.5 * (*it).sum()); }
 }
 
+TEST(ForallOpAndBoxArgument)
+{
+    // forallOp_p counts flops per point; forall with a Box argument runs on that box only
+    Box B = Box(Point::Zeros(), Point::Ones(7));
+    Box S = Box(Point::Ones(2), Point::Ones(4));
+    BoxData<double> d(B);
+    d.setVal(-1.0);
+    const unsigned long long f0 = forallFlops();
+    BoxData<double> a = forallOp_p<double>(3, "idx", [] PROTO_LAMBDA(const Point& p, Var<double>& v, Var<double>& src) {
+        v(0) = src(0) + p[0];
+    }, d);
+    CHECK_EQ(forallFlops() - f0, 3ULL * B.size());
+    CHECK(a.box() == B);
+    BoxData<double> b = forall<double>([] PROTO_LAMBDA(Var<double>& v, Var<double>& src) { v(0) = 2.0 * src(0); }, S, a);
+    CHECK(b.box() == S);
+    forallInPlaceOp(1, "scale", [] PROTO_LAMBDA(Var<double>& v) { v(0) = v(0) + 1.0; }, S, d);
+    BoxData<double, 1, HOST> ha(B), hb(S), hd(B);
+    a.copyTo(ha);
+    b.copyTo(hb);
+    d.copyTo(hd);
+    for (auto it = B.begin(); it != B.end(); ++it) {
+        CHECK_EQ(ha(*it), -1.0 + (*it)[0]);
+        CHECK_EQ(hd(*it), S.contains(*it) ? 0.0 : -1.0);
+    }
+    for (auto it = S.begin(); it != S.end(); ++it) { CHECK_EQ(hb(*it), 2.0 * (-1.0 + (*it)[0])); }
+}
+
 TEST(ExchangePeriodicPointId)
 {
     // tests/LevelBoxDataTests.cpp:345-429: after exchange every ghost cell holds
