@@ -153,10 +153,10 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
     nphi = nb ** 3 * (bs + 2) ** 3
     out = {"workload": "levelpoisson%d_box%d_gsrb22" % (n, bs), "levels": nlev + 1}
     sols = {}
-    for name, prec in (("fp64", 0), ("fp32_ir", 1)):
+    for name, prec, fused in (("fp64_two_colour", 0, 0), ("fp64", 0, 1), ("fp32_ir", 1, 1)):
         opt = pa.MgOptions()
         pa.lib().pm_mg_default_options(ctypes.byref(opt))
-        opt.relax, opt.pre, opt.post, opt.bottom = 0, 2, 2, 10
+        opt.relax, opt.pre, opt.post, opt.bottom, opt.fused = 0, 2, 2, 10, fused
         opt.precision, opt.max_iter, opt.tol, opt.use_graph = prec, 40, 1e-10, use_graph
         mg = ctypes.c_void_p()
         pa.call("pm_mg_create", ctypes.byref(mg), lay, nlev, 1.0 / n, ctypes.byref(opt))
@@ -176,8 +176,32 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
         pa.call("pm_mg_bytes", mg, ctypes.byref(nbytes))
         out[name] = {"iters": it.value, "solve_s": round(t, 5), "solves_per_s": round(1.0 / t, 3),
                      "rel_residual": float(hist[it.value] / hist[0]), "hierarchy_bytes": nbytes.value}
-        sols[name] = phi
+        if name in ("fp64", "fp32_ir"):
+            sols[name] = phi
         pa.call("pm_mg_destroy", mg)
+    # one GSRB sweep of the finest level: exchange / red / exchange / black vs one pass
+    g1, g0 = pa.i3([1, 1, 1]), pa.i3([0, 0, 0])
+    sweeps = {}
+    for tname, dt, t in (("f64", pa.PM_F64, np.float64), ("f32", pa.PM_F32, np.float32)):
+        ph = DeviceArray.zeros(nphi, t)
+        tm = DeviceArray.zeros(nb ** 3 * bs ** 3, t)
+        rh = DeviceArray.zeros(nb ** 3 * bs ** 3, t)
+
+        def two_colour():
+            for color in (0, 1):
+                pa.call("pm_level_exchange", lay, dt, ph.ptr, g1, 1, stream.h)
+                pa.call("pm_level_gsrb", lay, dt, ph.ptr, g1, rh.ptr, g0, 1.0 / n, color, stream.h)
+
+        def one_pass():
+            pa.call("pm_level_gsrb_sweep", lay, dt, ph.ptr, g1, tm.ptr, g0, rh.ptr, g0, 1.0 / n, stream.h)
+        ms2 = time_launches(two_colour, 5, stream)
+        ms1 = time_launches(one_pass, 5, stream)
+        alg = np.dtype(t).itemsize * 3 * n ** 3  # phi read, rhs read, phi written
+        sweeps[tname] = {"two_colour_ms": round(ms2, 4), "one_pass_ms": round(ms1, 4),
+                         "one_pass_GB/s": round(alg / (ms1 * 1e-3) / 1e9, 1)}
+        for d in (ph, tm, rh):
+            d.free()
+    out["fine_sweep"] = sweeps
     # error of the mixed-precision answer against the FP64 solve
     a, b = sols["fp64"], sols["fp32_ir"]
     flat = pa.Box.of([0, 0, 0], [nphi - 1, 0, 0])
@@ -189,6 +213,7 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
     e, s = m.numpy(stream.h)
     out["fp32_ir"]["rel_error_vs_fp64"] = float(e / s)
     out["speedup_fp32_ir"] = round(out["fp64"]["solve_s"] / out["fp32_ir"]["solve_s"], 3)
+    out["speedup_fp32_ir_vs_two_colour_fp64"] = round(out["fp64_two_colour"]["solve_s"] / out["fp32_ir"]["solve_s"], 3)
     pa.call("pm_layout_destroy", lay)
     return out
 

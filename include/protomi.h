@@ -213,6 +213,14 @@ int pm_level_gsrb(const pm_layout* layout, int dtype, void* phi, const int phi_g
 /* Damped point Jacobi of examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:99-123:
  * tmp = L(phi, 1/(4 DIM)) + (-(dx*dx)/(4 DIM)) rhs on all valid cells, then
  * phi += tmp.  tmp: level data without ghosts. */
+/* One full GSRB sweep (red then black, LaplaceOp.H:95-123) in one pass and
+ * out of place, with no exchange: phi_out (valid cells) = swept phi_in; cells
+ * of other boxes are read from their owners.  Bitwise the exchange / red /
+ * exchange / black sequence.  3-D, periodic layouts covering their domain;
+ * PM_ERR_UNSUPPORTED otherwise (use pm_level_gsrb). */
+int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
+                        const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx,
+                        pm_stream_t stream);
 int pm_level_jacobi(const pm_layout* layout, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
                     const int rhs_ghost[3], void* tmp, double dx, pm_stream_t stream);
 
@@ -296,6 +304,8 @@ typedef struct pm_mg_options {
     int max_iter;
     double tol;     /* stop when resnorm < tol * resnorm_0 (LevelMultigridSolverImplem.H:129) */
     int use_graph;  /* capture the V-cycle into a hipGraph and replay it */
+    int fused;      /* GSRB sweeps as one out-of-place pass each (pm_level_gsrb_sweep, same
+                       bits; 3-D periodic levels) instead of exchange / red / exchange / black */
 } pm_mg_options;
 void pm_mg_default_options(pm_mg_options* opt);
 /* nlevels = number of coarsenings (the numLevels argument of LevelMultigrid). */

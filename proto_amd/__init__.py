@@ -53,7 +53,7 @@ class MgOptions(ctypes.Structure):
     """pm_mg_options."""
     _fields_ = [("relax", ctypes.c_int), ("pre", ctypes.c_int), ("post", ctypes.c_int), ("bottom", ctypes.c_int),
                 ("interp", ctypes.c_int), ("precision", ctypes.c_int), ("max_iter", ctypes.c_int),
-                ("tol", ctypes.c_double), ("use_graph", ctypes.c_int)]
+                ("tol", ctypes.c_double), ("use_graph", ctypes.c_int), ("fused", ctypes.c_int)]
 
 
 _OP = ctypes.POINTER(MgOptions)
@@ -166,6 +166,7 @@ SIGNATURES = {
     "pm_server_run": (_i, [_vp, _i, ctypes.POINTER(Request), ctypes.POINTER(RequestStats), _vp]),
     "pm_server_result": (_i, [_vp, _i, _vp]),
     "pm_server_rhs": (_i, [_i, _i, _u64, _d, _vp, _vp]),
+    "pm_level_gsrb_sweep": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _vp, _ip, _d, _vp]),
     "pm_graph_begin": (_i, [_vp]),
     "pm_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
     "pm_graph_launch": (_i, [_vp, _vp]),

@@ -188,6 +188,117 @@ __global__ __launch_bounds__(256) void k_gsrb(LayoutDev L, G3 gp, G3 gr, T* __re
     }
 }
 
+
+// ------------------------------------------------------------ fused GSRB sweep
+// One full red-black sweep in one pass, out of place (pin is only read, pout
+// only written), with no exchange: a workgroup owns rows [y0, y0+R) x planes
+// [z0, z1) of one box and walks z through an LDS ring of 4 planes holding
+// rows [y0-2, y1+2) x columns [-2, bs+2).  Cells outside the box come from
+// their owner box's valid cells (periodic images folded in).  At step k the
+// red cells of plane k+1 are updated over the tile plus a one-cell halo (the
+// halo values equal what the neighbouring tiles compute for those cells:
+// same inputs, same order), then the black cells of plane k over the tile,
+// which is written out.  Same arithmetic and order as k_gsrb's two colours.
+template<typename T, int MAXE>
+__global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout, G3 gr, const T* __restrict__ pin,
+                                                    T* __restrict__ pout, const T* __restrict__ rhs, Lap<T> c,
+                                                    T lambda, int rows, int zc)
+{
+    extern __shared__ __align__(16) unsigned char smraw[];
+    T* sm = reinterpret_cast<T*>(smraw);
+    __shared__ int s_nb[27];  // box owning each of the 27 neighbouring box positions (13 = this box)
+    const int b = blockIdx.y;
+    const int* lo = L.lows + 3 * b;
+    const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
+    const int nyt = (ny + rows - 1) / rows;
+    const int y0 = (blockIdx.x % nyt) * rows, y1 = min(y0 + rows, ny);
+    const int z0 = (blockIdx.x / nyt) * zc, z1 = min(z0 + zc, nz);
+    const int W = nx + 4, NP = (y1 - y0 + 4) * W;
+    const int losum = lo[0] + lo[1] + lo[2];
+    const long vin = gvol(L, gin), vr = gvol(L, gr);
+    if (threadIdx.x < 27) {
+        const int t = threadIdx.x;
+        int p[3] = {lo[0] + (t % 3 - 1) * nx, lo[1] + ((t / 3) % 3 - 1) * ny, lo[2] + (t / 9 - 1) * nz};
+        s_nb[t] = owner(L, p);
+    }
+    __syncthreads();
+    // element of (box-local, possibly outside the box) cell (x, y, z) in data with ghost g
+    auto at = [&](const G3& g, long vol, int x, int y, int z) {
+        const int ox = x < 0 ? -1 : (x >= nx ? 1 : 0);
+        const int oy = y < 0 ? -1 : (y >= ny ? 1 : 0);
+        const int oz = z < 0 ? -1 : (z >= nz ? 1 : 0);
+        const int o = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        return (long)o * vol + rel_index(L, g, x - ox * nx, y - oy * ny, z - oz * nz);
+    };
+    T* ob = pout + (long)b * gvol(L, gout);
+    auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
+    T pre[MAXE];
+    auto issue = [&](int z) {
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NP) {
+                const int er = e / W, ec = e - er * W;
+                pre[i] = pin[at(gin, vin, ec - 2, y0 - 2 + er, z)];
+            }
+        }
+    };
+    auto commit = [&](int z) {
+        T* pl = slot(z);
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NP) { pl[e] = pre[i]; }
+        }
+    };
+    auto update = [&](int z, int ya, int yb, int xa, int xb, int colour) {
+        T* zc_ = slot(z);
+        const T* zm = slot(z - 1);
+        const T* zp = slot(z + 1);
+        const int hw = (xb - xa + 1) / 2;
+        const int ncell = (yb - ya) * hw;
+        for (int q = threadIdx.x; q < ncell; q += 256) {
+            const int yy = ya + q / hw;
+            const int xx = xa + 2 * (q % hw) + ((xa + yy + z + colour + losum) & 1);
+            if (xx >= xb) { continue; }
+            const int li = (yy - (y0 - 2)) * W + (xx + 2);
+            T s = 0;
+            s = s + c.c[0] * zc_[li];
+            s = s + c.c[1] * zc_[li + 1];
+            s = s + c.c[2] * zc_[li - 1];
+            s = s + c.c[3] * zc_[li + W];
+            s = s + c.c[4] * zc_[li - W];
+            s = s + c.c[5] * zp[li];
+            s = s + c.c[6] * zm[li];
+            s = s + rhs[at(gr, vr, xx, yy, z)];
+            zc_[li] = zc_[li] + lambda * s;
+        }
+    };
+    issue(z0 - 2);
+    commit(z0 - 2);
+    issue(z0 - 1);
+    commit(z0 - 1);
+    issue(z0);
+    for (int k = z0 - 2; k < z1; k++) {
+        commit(k + 2);  // into the slot of plane k-2, last read before the previous barrier
+        if (k + 3 <= z1 + 1) { issue(k + 3); }  // in flight during this step's updates
+        __syncthreads();
+        update(k + 1, y0 - 1, y1 + 1, -1, nx + 1, 0);
+        __syncthreads();
+        if (k >= z0) {
+            update(k, y0, y1, 0, nx, 1);
+            __syncthreads();
+            const T* pl = slot(k);
+            const int cnt = (y1 - y0) * nx;
+            for (int e = threadIdx.x; e < cnt; e += 256) {
+                const int r = e / nx, xx = e - r * nx;
+                ob[rel_index(L, gout, xx, y0 + r, k)] = pl[(r + 2) * W + xx + 2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ Jacobi
 template<typename T, int DIMN>
 __global__ __launch_bounds__(256) void k_jacobi_tmp(LayoutDev L, G3 gp, G3 gr, const T* __restrict__ phi,
@@ -598,6 +709,59 @@ int pm_level_gsrb(const pm_layout* L, int dtype, void* phi, const int phi_ghost[
         } else {
             hipLaunchKernelGGL((k_gsrb<T, 2>), grid, dim3(256), 0, S(stream), D, gp, gr, (T*)phi, (const T*)rhs, c,
                                (T)(1.0 * lam), color);
+        }
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
+                        const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, pm_stream_t stream)
+{
+    PM_REQUIRE(L && phi_in && phi_out && rhs, "pm_level_gsrb_sweep: bad argument");
+    PM_REQUIRE(phi_in != phi_out, "pm_level_gsrb_sweep: the sweep is out of place (phi_in != phi_out)");
+    if (L->dim != 3) {
+        set_error("pm_level_gsrb_sweep: 3-D layouts only");
+        return PM_ERR_UNSUPPORTED;
+    }
+    const int full = L->nb[0] * L->nb[1] * L->nb[2];
+    if (L->nboxes != full || !L->periodic[0] || !L->periodic[1] || !L->periodic[2]) {
+        set_error("pm_level_gsrb_sweep: needs a periodic layout covering its domain");
+        return PM_ERR_UNSUPPORTED;
+    }
+    const size_t esz = dtype == PM_F64 ? 8 : 4;
+    int rows = 8;
+    while (rows > 1 && 4 * (size_t)(rows + 4) * (L->bs[0] + 4) * esz > 64 * 1024 - 64) { rows /= 2; }
+    const size_t lds = 4 * (size_t)(rows + 4) * (L->bs[0] + 4) * esz;
+    if (lds > 64 * 1024 - 64) {
+        set_error("pm_level_gsrb_sweep: boxes %d cells wide do not fit the LDS ring", L->bs[0]);
+        return PM_ERR_UNSUPPORTED;
+    }
+    for (int k = 0; k < 3; k++) { PM_REQUIRE(L->bs[k] >= 2, "pm_level_gsrb_sweep: boxes need 2+ cells per side"); }
+    const int nyt = (L->bs[1] + rows - 1) / rows;
+    int zc = L->bs[2];
+    while (zc > 8 && (long)L->nboxes * nyt * ((L->bs[2] + zc - 1) / zc) < 2048) { zc = (zc + 1) / 2; }
+    const G3 gi = g3(in_ghost), go = g3(out_ghost), gr = g3(rhs_ghost);
+    const LayoutDev D = ldev(L);
+    dim3 grid((unsigned)(nyt * ((L->bs[2] + zc - 1) / zc)), L->nboxes);
+    double lam = -(dx * dx) / (4.0 * L->dim);
+    lam *= 2.0;
+    const double m = -1.0 / (dx * dx);
+    PM_DISPATCH_T(dtype, {
+        Lap<T> c = lap_coefs<T>(L->dim, m);
+        if (dtype == PM_F64) {
+            for (int j = 0; j < 7; j++) { c.c[j] = (T)((j == 0 ? -2.0 * L->dim : 1.0) * m); }
+        }
+        const long np = (long)(rows + 4) * (L->bs[0] + 4);
+        if (np <= 1024) {
+            hipLaunchKernelGGL((k_gsrb_fused<T, 4>), grid, dim3(256), lds, S(stream), D, gi, go, gr,
+                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
+        } else if (np <= 2048) {
+            hipLaunchKernelGGL((k_gsrb_fused<T, 8>), grid, dim3(256), lds, S(stream), D, gi, go, gr,
+                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
+        } else {
+            hipLaunchKernelGGL((k_gsrb_fused<T, 16>), grid, dim3(256), lds, S(stream), D, gi, go, gr,
+                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
         }
     });
     PM_CHECK_LAUNCH();

@@ -96,7 +96,18 @@ static void mg_free(pm_mg* mg)
 static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* tmp, int n, hipStream_t s)
 {
     const auto& L = mg->lv[l];
-    for (int it = 0; it < n; it++) {
+    int it = 0;
+    if (mg->opt.relax == 0 && mg->opt.fused && mg->dim == 3 && tmp) {
+        // one-pass sweeps alternating phi -> tmp -> phi; an odd count ends with
+        // the two-colour form below
+        for (; it + 1 < n; it += 2) {
+            int rc = pm_level_gsrb_sweep(L.layout, dtype, phi, mg->g1, tmp, mg->g0, rhs, mg->g0, L.dx, s);
+            if (rc == PM_ERR_UNSUPPORTED) { break; }
+            if (rc != PM_OK) { return rc; }
+            MG_TRY(pm_level_gsrb_sweep(L.layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0, L.dx, s));
+        }
+    }
+    for (; it < n; it++) {
         if (mg->opt.relax == 0) {
             for (int color = 0; color < 2; color++) {
                 MG_TRY(pm_level_exchange(L.layout, dtype, phi, mg->g1, 1, s));
@@ -230,6 +241,7 @@ void pm_mg_default_options(pm_mg_options* o)
     o->max_iter = 30;
     o->tol = 1e-10;
     o->use_graph = 0;
+    o->fused = 1;
 }
 
 int pm_mg_create(pm_mg** out, const pm_layout* fine, int nlevels, double dx, const pm_mg_options* opt)
@@ -292,11 +304,12 @@ int pm_mg_create(pm_mg** out, const pm_layout* fine, int nlevels, double dx, con
         const size_t e1 = level_elems(L.layout, mg->g1) * nbx, e0 = level_elems(L.layout, mg->g0) * nbx;
         if (l > 0 && rc == PM_OK) { rc = dalloc(mg, &L.phi, 8 * e1); }
         if (l > 0 && rc == PM_OK) { rc = dalloc(mg, &L.rhs, 8 * e0); }
-        if (opt->relax == 1 && rc == PM_OK) { rc = dalloc(mg, &L.tmp, 8 * e0); }
+        const bool need_tmp = opt->relax == 1 || (opt->fused && dim == 3);
+        if (need_tmp && rc == PM_OK) { rc = dalloc(mg, &L.tmp, 8 * e0); }
         if (opt->precision == 1) {
             if (rc == PM_OK) { rc = dalloc(mg, &L.phi32, 4 * e1); }
             if (rc == PM_OK) { rc = dalloc(mg, &L.rhs32, 4 * e0); }
-            if (opt->relax == 1 && rc == PM_OK) { rc = dalloc(mg, &L.tmp32, 4 * e0); }
+            if (need_tmp && rc == PM_OK) { rc = dalloc(mg, &L.tmp32, 4 * e0); }
         }
     }
     if (rc == PM_OK) { rc = dalloc(mg, &mg->d_norm, 64); }

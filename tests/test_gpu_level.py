@@ -103,10 +103,13 @@ def test_exchange_with_hole_leaves_unowned_ghosts():
             assert got[b][idx] == expect
 
 
-def _solve(n, bs, nlev, relax, linear, precision=0, maxiter=20, use_graph=False, tol=1e-10, rhs=None, stream=None):
+def _solve(n, bs, nlev, relax, linear, precision=0, maxiter=20, use_graph=False, tol=1e-10, rhs=None, stream=None,
+           fused=None):
     L = Layout(n, bs)
     opt = pa.MgOptions()
     pa.lib().pm_mg_default_options(ctypes.byref(opt))
+    if fused is not None:
+        opt.fused = int(fused)
     opt.relax, opt.pre, opt.post, opt.bottom, opt.interp = relax, 2, 2, 10, linear
     opt.precision, opt.max_iter, opt.tol, opt.use_graph = precision, maxiter, tol, int(use_graph)
     mg = ctypes.c_void_p()
@@ -122,6 +125,57 @@ def _solve(n, bs, nlev, relax, linear, precision=0, maxiter=20, use_graph=False,
     phi = L.unpack(host(phi_d), 1)
     pa.lib().pm_mg_destroy(mg)
     return rc, it.value, hist[:it.value + 1], phi
+
+
+def test_default_options_use_the_fused_sweep():
+    opt = pa.MgOptions()
+    pa.lib().pm_mg_default_options(ctypes.byref(opt))
+    assert opt.fused == 1  # so every solve test below runs the one-pass sweep
+
+
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+@pytest.mark.parametrize("n,bs", [(32, 16), (16, 16), (24, 8), (12, 6), (64, 64)])
+def test_gsrb_sweep_equals_two_colours(dt, n, bs):
+    """pm_level_gsrb_sweep (one pass, out of place, no exchange) gives the bits
+    of exchange / red / exchange / black, also for boxes with odd low-corner
+    sums (bs 6) and for boxes narrower than a tile."""
+    t = np.float64 if dt == pa.PM_F64 else np.float32
+    L = Layout(n, bs)
+    g = util.uniform(80 + n, 0, n ** 3).astype(t)
+    r = util.uniform(80 + n, 1, n ** 3).astype(t)
+    dx = 1.0 / n
+    a = dev(L.pack(g, 1))
+    rh = dev(L.pack(r, 0))
+    out = zeros(len(L.lows) * bs ** 3, t)
+    say("sweep", dt, n, bs)
+    pa.call("pm_level_gsrb_sweep", L.h, dt, vp(a), G1, vp(out), G0, vp(rh), G0, dx, None)
+    for _ in range(2):  # and a second sweep back, as the multigrid driver alternates
+        back = dev(L.pack(np.zeros(n ** 3, t), 1))
+        pa.call("pm_level_gsrb_sweep", L.h, dt, vp(out), G0, vp(back), G1, vp(rh), G0, dx, None)
+    ref = dev(L.pack(g, 1))
+    for _ in range(2):
+        for color in (0, 1):
+            pa.call("pm_level_exchange", L.h, dt, vp(ref), G1, 1, None)
+            pa.call("pm_level_gsrb", L.h, dt, vp(ref), G1, vp(rh), G0, dx, color, None)
+    assert L.unpack(host(back), 1).tobytes() == L.unpack(host(ref), 1).tobytes()
+
+
+def test_gsrb_sweep_unsupported_layouts():
+    hole = Layout(32, 16, lows=[(0, 0, 0), (16, 0, 0)])
+    buf = zeros(2 * 18 ** 3)
+    rc = pa.lib().pm_level_gsrb_sweep(hole.h, pa.PM_F64, vp(buf), G1, vp(zeros(2 * 16 ** 3)), G0, vp(buf), G1,
+                                      0.1, None)
+    assert rc == -4 and b"covering" in pa.lib().pm_last_error()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_and_two_colour_solves_agree(precision):
+    n, bs, nlev = 64, 32, 3
+    rhs = util.gaussian_rhs(n, 4, 6)
+    a = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, fused=0)
+    b = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, fused=1)
+    assert a[0] == b[0] == 0 and a[1] == b[1]
+    assert a[2].tobytes() == b[2].tobytes() and a[3].tobytes() == b[3].tobytes()
 
 
 @pytest.mark.parametrize("key,relax,linear", [("gsrb_const", 0, 0), ("gsrb_linear", 0, 1), ("jacobi", 1, 0)])
