@@ -346,7 +346,9 @@ int pm_batch_elems(const pm_batch* batch, size_t* ghosted, size_t* valid);
 int pm_batch_offsets(const pm_batch* batch, size_t* ghosted_off, size_t* valid_off);
 /* Tiling knobs of the batched operators (what pm_batch_autotune searches):
  * z planes per apply tile (0 = whole box), persistent workgroups per CU for
- * the apply, rows per GSRB tile, workgroups per CU for the sweep. */
+ * the apply, rows per GSRB tile (fewer when an FP64 tile as wide as the widest
+ * box would not fit the 64 KiB LDS of a workgroup), workgroups per CU for
+ * the sweep. */
 typedef struct pm_batch_config {
     int apply_zplanes, apply_blocks_per_cu, gsrb_rows, gsrb_blocks_per_cu;
 } pm_batch_config;

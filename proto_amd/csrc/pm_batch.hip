@@ -175,16 +175,19 @@ __global__ __launch_bounds__(256) void k_batch_apply(const BatchBox* __restrict_
 }
 
 // ------------------------------------------------------------ batched GSRB
-// LDS: 4 planes (k-1 .. k+2) of (rows + 4) x (nx + 2) cells.
-template<typename T>
+// LDS: a ring of 4 phi planes (k-1 .. k+2) of (rows + 4) x (nx + 2) cells and
+// a ring of 2 rhs planes of (rows + 2) x nx cells, both filled from registers
+// loaded one step ahead of their use.
+template<typename T, int MAXE>
 __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__ boxes,
                                                     const BatchItem* __restrict__ items, int nitems, int g,
                                                     const T* __restrict__ phi_in, T* __restrict__ phi_out,
                                                     const T* __restrict__ rhs, BCoefs<T> c, T lambda,
-                                                    unsigned* counter)
+                                                    unsigned* counter, int rhs_lds_off)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
-    T* sm = reinterpret_cast<T*>(smraw);  // 4 * (rows + 4) * (max width + 2) elements
+    T* sm = reinterpret_cast<T*>(smraw);
+    T* rsm = sm + rhs_lds_off;
     __shared__ int s_item;
     const int tid = threadIdx.x;
     for (;;) {
@@ -192,25 +195,61 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
         if (it >= nitems) { break; }
         const BatchItem wi = items[it];
         const BatchBox bx = boxes[wi.box];
-        const int R = wi.rows, LY = R + 4, LX = bx.nx + 2, NP = LY * LX;  // NP * 4 <= lds_elems
+        const int R = wi.rows, LX = bx.nx + 2, NP = (R + 4) * LX;
+        const int NR = (R + 2) * bx.nx;
         const long Gx = bx.nx + 2 * g, Gxy = Gx * (bx.ny + 2 * g);
         const T* p = phi_in + bx.goff;
         T* po = phi_out + bx.goff;
         const T* rh = rhs + bx.voff;
         const int y0 = wi.y0;
-        // plane element e -> (row y0 - 2 + e / LX, col -1 + e % LX)
-        auto gload = [&](int z, T* pl) {
-            if (z < -1 || z > bx.nz) { return; }
-            const T* base = p + Gxy * (long)(z + g);
-            for (int e = tid; e < NP; e += 256) {
+        // in-plane offsets of this thread's elements: phi element e -> row
+        // y0 - 2 + e / LX, column -1 + e % LX of the ghosted plane (-1: outside
+        // the ghosts, reads as 0); rhs element e -> row y0 - 1 + e / nx
+        int offp[MAXE], offr[MAXE];
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = tid + 256 * i;
+            offp[i] = offr[i] = -1;
+            if (e < NP) {
                 const int er = e / LX, ec = e - er * LX;
                 const int gy = y0 - 2 + er, gx = ec - 1;
-                T v = 0;
-                if (gy >= -g && gy < bx.ny + g) { v = base[(gx + g) + Gx * (long)(gy + g)]; }
-                pl[e] = v;
+                if (gy >= -g && gy < bx.ny + g) { offp[i] = (gx + g) + (int)Gx * (gy + g); }
             }
+            if (e < NR) {
+                const int er = e / bx.nx, ec = e - er * bx.nx;
+                const int gy = y0 - 1 + er;
+                if (gy >= 0 && gy < bx.ny) { offr[i] = ec + bx.nx * gy; }
+            }
+        }
+        T pre[MAXE], prr[MAXE];
+        auto issue = [&](int z) {
+            const T* base = p + Gxy * (long)(z + g);
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) { pre[i] = offp[i] >= 0 ? base[offp[i]] : (T)0; }
         };
         auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
+        auto commit = [&](int z) {
+            T* pl = slot(z);
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                const int e = tid + 256 * i;
+                if (e < NP) { pl[e] = pre[i]; }
+            }
+        };
+        auto issue_r = [&](int z) {
+            const T* base = rh + (long)bx.nx * bx.ny * z;
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) { prr[i] = offr[i] >= 0 ? base[offr[i]] : (T)0; }
+        };
+        auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
+        auto commit_r = [&](int z) {
+            T* pl = rslot(z);
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                const int e = tid + 256 * i;
+                if (e < NR) { pl[e] = prr[i]; }
+            }
+        };
         // half sweep on plane z, rows [ya, yb), colour col: p += lambda (sum c_j p(nbr) + rhs)
         auto update = [&](int z, int ya, int yb, int colour) {
             if (z < 0 || z >= bx.nz) { return; }
@@ -219,8 +258,9 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
             const int hx = (bx.nx + 1) / 2;
             const int ncell = (yb - ya) * hx;
             T* zc = slot(z);
-            T* zm = slot(z - 1);
-            T* zp = slot(z + 1);
+            const T* zm = slot(z - 1);
+            const T* zp = slot(z + 1);
+            const T* rz = rslot(z);
             for (int cidx = tid; cidx < ncell; cidx += 256) {
                 const int yy = ya + cidx / hx;
                 const int xx = 2 * (cidx % hx) + ((yy + z + colour) & 1);
@@ -234,16 +274,22 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
                 s = s + c.c[4] * zc[li - LX];
                 s = s + c.c[5] * zp[li];
                 s = s + c.c[6] * zm[li];
-                s = s + rh[xx + (long)bx.nx * (yy + (long)bx.ny * z)];
+                s = s + rz[(yy - (y0 - 1)) * bx.nx + xx];
                 zc[li] = zc[li] + lambda * s;
             }
         };
-        // prologue: planes -1 and 0 (plane 1 arrives in the first step)
-        gload(-1, slot(-1));
-        gload(0, slot(0));
+        // planes -1 .. nz (ghost planes included) and rhs planes 0 .. nz-1
+        issue(-1);
+        commit(-1);
+        issue(0);
+        commit(0);
+        issue(1);
+        issue_r(0);
         for (int k = -1; k < bx.nz; k++) {
-            // plane k+2 into the slot of plane k-2 (no longer read)
-            gload(k + 2, slot(k + 2));
+            if (k + 2 <= bx.nz) { commit(k + 2); }  // into the slot of plane k-2, no longer read
+            if (k + 1 < bx.nz) { commit_r(k + 1); }  // into the slot of rhs plane k-1, likewise
+            if (k + 3 <= bx.nz) { issue(k + 3); }
+            if (k + 2 < bx.nz) { issue_r(k + 2); }
             __syncthreads();
             update(k + 1, y0 - 1, y0 + R + 1, 0);  // red: tile rows and one row each side
             __syncthreads();
@@ -311,8 +357,19 @@ int build_apply_items(pm_batch* B, int tz)
     return upload(&B->d_items, items.data(), sizeof(BatchItem) * items.size());
 }
 
+// LDS of a sweep tile: 4 phi planes of (rows + 4) x (nx + 2), 2 rhs planes of (rows + 2) x nx
+size_t gsrb_lds(int rows, int maxnx, size_t esz)
+{
+    return (4 * (size_t)(rows + 4) * (maxnx + 2) + 2 * (size_t)(rows + 2) * maxnx) * esz;
+}
+
+// Sweep tiles: full box width, `rows` rows (fewer when the FP64 tile would not
+// fit the LDS), whole z extent; largest first.
 int build_gsrb_items(pm_batch* B, int rows)
 {
+    int maxnx = 0;
+    for (int b = 0; b < B->nboxes; b++) { maxnx = std::max(maxnx, B->sizes[3 * b]); }
+    while (rows > 1 && gsrb_lds(rows, maxnx, 8) + 64 > 64 * 1024) { rows--; }
     std::vector<BatchItem> items;
     for (int b = 0; b < B->nboxes; b++) {
         const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1], nz = B->sizes[3 * b + 2];
@@ -428,7 +485,7 @@ int pm_batch_configure(pm_batch* B, const pm_batch_config* c)
                "pm_batch_configure: bad configuration");
     int rc = PM_OK;
     if (c->apply_zplanes != B->apply_tz) { rc = build_apply_items(B, c->apply_zplanes); }
-    if (rc == PM_OK && c->gsrb_rows != B->gsrb_rows) { rc = build_gsrb_items(B, c->gsrb_rows); }
+    if (rc == PM_OK && c->gsrb_rows != B->cfg.gsrb_rows) { rc = build_gsrb_items(B, c->gsrb_rows); }
     if (rc == PM_OK) { B->cfg = *c; }
     return rc;
 }
@@ -487,26 +544,35 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
     int maxnx = 0;
     for (int b = 0; b < B->nboxes; b++) { maxnx = std::max(maxnx, B->sizes[3 * b]); }
     PM_REQUIRE(maxnx <= 256, "pm_batch_gsrb: boxes wider than 256 cells are not supported");
-    const int lds_elems = 4 * (B->gsrb_rows + 4) * (maxnx + 2);
-    const size_t lds = (size_t)lds_elems * (dtype == PM_F64 ? 8 : 4);
+    const int phi_elems = 4 * (B->gsrb_rows + 4) * (maxnx + 2);
+    const size_t esz = dtype == PM_F64 ? 8 : 4;
+    const size_t lds = gsrb_lds(B->gsrb_rows, maxnx, esz);
     PM_REQUIRE(lds + 64 <= 64 * 1024, "pm_batch_gsrb: tile too large for LDS (rows %d, width %d)", B->gsrb_rows, maxnx);
     const unsigned nb = persistent_blocks(B->cfg.gsrb_blocks_per_cu);
+    const long np = (long)(B->gsrb_rows + 4) * (maxnx + 2);
     // LaplaceOp::init: coefficients Laplacian()_j * (-1/(dx*dx)), lambda = 2 * (-(dx*dx)/(4 DIM))
     double lam = -(dx * dx) / (4.0 * 3);
     lam *= 2.0;
     const double m = -1.0 / (dx * dx);
+    auto launch = [&](auto tag, auto cc, auto lamT) {
+        using T = decltype(tag);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds, S(stream), (const BatchBox*)B->d_boxes,
+                               (const BatchItem*)B->d_gitems, B->nitems_gsrb, B->ghost, (const T*)phi_in,
+                               (T*)phi_out, (const T*)rhs, cc, lamT, B->d_counter + 2, phi_elems);
+        };
+        if (np <= 1024) { go(k_batch_gsrb<T, 4>); }
+        else if (np <= 2048) { go(k_batch_gsrb<T, 8>); }
+        else { go(k_batch_gsrb<T, 16>); }
+    };
     if (dtype == PM_F64) {
         BCoefs<double> cc;
         for (int j = 0; j < 7; j++) { cc.c[j] = (j == 0 ? -6.0 : 1.0) * m; }
-        hipLaunchKernelGGL((k_batch_gsrb<double>), dim3(nb), dim3(256), lds, S(stream), (const BatchBox*)B->d_boxes,
-                           (const BatchItem*)B->d_gitems, B->nitems_gsrb, B->ghost, (const double*)phi_in,
-                           (double*)phi_out, (const double*)rhs, cc, 1.0 * lam, B->d_counter + 2);
+        launch(0.0, cc, 1.0 * lam);
     } else {
         BCoefs<float> cc;
         for (int j = 0; j < 7; j++) { cc.c[j] = (float)(j == 0 ? -6.0 : 1.0) * (float)m; }
-        hipLaunchKernelGGL((k_batch_gsrb<float>), dim3(nb), dim3(256), lds, S(stream), (const BatchBox*)B->d_boxes,
-                           (const BatchItem*)B->d_gitems, B->nitems_gsrb, B->ghost, (const float*)phi_in,
-                           (float*)phi_out, (const float*)rhs, cc, (float)(1.0 * lam), B->d_counter + 2);
+        launch(0.0f, cc, (float)(1.0 * lam));
     }
     PM_CHECK_LAUNCH();
     return PM_OK;
@@ -563,7 +629,7 @@ int pm_batch_autotune(pm_batch* B, const pm_stencil* st, const void* src, void* 
     const size_t esz = dtype == PM_F64 ? 8 : 4;
     for (int r : rows) {
         if (rc) { break; }
-        if ((size_t)4 * (r + 4) * (maxnx + 2) * esz + 64 > 64 * 1024) { continue; }
+        if (gsrb_lds(r, maxnx, esz) + 64 > 64 * 1024) { continue; }
         for (int bpc : bpcs) {
             pm_batch_config c = keep;
             c.apply_zplanes = win.apply_zplanes;

@@ -205,7 +205,6 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
                                                     T lambda, int rows, int zc)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
-    T* sm = reinterpret_cast<T*>(smraw);
     __shared__ int s_nb[27];  // box owning each of the 27 neighbouring box positions (13 = this box)
     const int b = blockIdx.y;
     const int* lo = L.lows + 3 * b;
@@ -213,7 +212,10 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     const int nyt = (ny + rows - 1) / rows;
     const int y0 = (blockIdx.x % nyt) * rows, y1 = min(y0 + rows, ny);
     const int z0 = (blockIdx.x / nyt) * zc, z1 = min(z0 + zc, nz);
-    const int W = nx + 4, NP = (y1 - y0 + 4) * W;
+    const int W = nx + 4, NP = (y1 - y0 + 4) * W;  // phi plane: rows [y0-2, y1+2) x columns [-2, nx+2)
+    const int WR = nx + 2, NR = (y1 - y0 + 2) * WR; // rhs plane: rows [y0-1, y1+1) x columns [-1, nx+1)
+    T* sm = reinterpret_cast<T*>(smraw);
+    T* rsm = sm + 4 * (long)(rows + 4) * W;
     const int losum = lo[0] + lo[1] + lo[2];
     const long vin = gvol(L, gin), vr = gvol(L, gr);
     if (threadIdx.x < 27) {
@@ -232,14 +234,41 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     };
     T* ob = pout + (long)b * gvol(L, gout);
     auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
-    T pre[MAXE];
-    auto issue = [&](int z) {
+    auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
+    // in-plane offsets of this thread's elements for planes inside the box
+    // (z = 0; plane z adds z * plane size); planes beyond the box go through at()
+    long offp[MAXE], offr[MAXE];
+    const long XYp = (long)(nx + 2 * gin.g[0]) * (ny + 2 * gin.g[1]);
+    const long XYr = (long)(nx + 2 * gr.g[0]) * (ny + 2 * gr.g[1]);
 #pragma unroll
-        for (int i = 0; i < MAXE; i++) {
-            const int e = threadIdx.x + 256 * i;
-            if (e < NP) {
-                const int er = e / W, ec = e - er * W;
-                pre[i] = pin[at(gin, vin, ec - 2, y0 - 2 + er, z)];
+    for (int i = 0; i < MAXE; i++) {
+        const int e = threadIdx.x + 256 * i;
+        offp[i] = offr[i] = 0;
+        if (e < NP) {
+            const int er = e / W, ec = e - er * W;
+            offp[i] = at(gin, vin, ec - 2, y0 - 2 + er, 0);
+        }
+        if (e < NR) {
+            const int er = e / WR, ec = e - er * WR;
+            offr[i] = at(gr, vr, ec - 1, y0 - 1 + er, 0);
+        }
+    }
+    T pre[MAXE], prr[MAXE];
+    auto issue = [&](int z) {
+        if (z >= 0 && z < nz) {
+            const T* base = pin + XYp * z;
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                if (threadIdx.x + 256 * i < NP) { pre[i] = base[offp[i]]; }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                const int e = threadIdx.x + 256 * i;
+                if (e < NP) {
+                    const int er = e / W, ec = e - er * W;
+                    pre[i] = pin[at(gin, vin, ec - 2, y0 - 2 + er, z)];
+                }
             }
         }
     };
@@ -251,10 +280,37 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
             if (e < NP) { pl[e] = pre[i]; }
         }
     };
+    auto issue_r = [&](int z) {
+        if (z >= 0 && z < nz) {
+            const T* base = rhs + XYr * z;
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                if (threadIdx.x + 256 * i < NR) { prr[i] = base[offr[i]]; }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                const int e = threadIdx.x + 256 * i;
+                if (e < NR) {
+                    const int er = e / WR, ec = e - er * WR;
+                    prr[i] = rhs[at(gr, vr, ec - 1, y0 - 1 + er, z)];
+                }
+            }
+        }
+    };
+    auto commit_r = [&](int z) {
+        T* pl = rslot(z);
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NR) { pl[e] = prr[i]; }
+        }
+    };
     auto update = [&](int z, int ya, int yb, int xa, int xb, int colour) {
         T* zc_ = slot(z);
         const T* zm = slot(z - 1);
         const T* zp = slot(z + 1);
+        const T* rz = rslot(z);
         const int hw = (xb - xa + 1) / 2;
         const int ncell = (yb - ya) * hw;
         for (int q = threadIdx.x; q < ncell; q += 256) {
@@ -270,18 +326,23 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
             s = s + c.c[4] * zc_[li - W];
             s = s + c.c[5] * zp[li];
             s = s + c.c[6] * zm[li];
-            s = s + rhs[at(gr, vr, xx, yy, z)];
+            s = s + rz[(yy - (y0 - 1)) * WR + (xx + 1)];
             zc_[li] = zc_[li] + lambda * s;
         }
     };
+    // phi planes z0-2 .. z1+1 and rhs planes z0-1 .. z1 pass through registers
+    // one step ahead of their use
     issue(z0 - 2);
     commit(z0 - 2);
     issue(z0 - 1);
     commit(z0 - 1);
     issue(z0);
+    issue_r(z0 - 1);
     for (int k = z0 - 2; k < z1; k++) {
-        commit(k + 2);  // into the slot of plane k-2, last read before the previous barrier
-        if (k + 3 <= z1 + 1) { issue(k + 3); }  // in flight during this step's updates
+        commit(k + 2);    // into the slot of plane k-2, last read before the previous barrier
+        commit_r(k + 1);  // into the slot of rhs plane k-1, likewise
+        if (k + 3 <= z1 + 1) { issue(k + 3); }
+        if (k + 2 <= z1) { issue_r(k + 2); }
         __syncthreads();
         update(k + 1, y0 - 1, y1 + 1, -1, nx + 1, 0);
         __syncthreads();
@@ -730,10 +791,13 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
         return PM_ERR_UNSUPPORTED;
     }
     const size_t esz = dtype == PM_F64 ? 8 : 4;
-    int rows = 8;
-    while (rows > 1 && 4 * (size_t)(rows + 4) * (L->bs[0] + 4) * esz > 64 * 1024 - 64) { rows /= 2; }
-    const size_t lds = 4 * (size_t)(rows + 4) * (L->bs[0] + 4) * esz;
-    if (lds > 64 * 1024 - 64) {
+    auto lds_of = [&](int r) {
+        return (4 * (size_t)(r + 4) * (L->bs[0] + 4) + 2 * (size_t)(r + 2) * (L->bs[0] + 2)) * esz;
+    };
+    int rows = esz == 4 ? 16 : 8;
+    while (rows > 1 && lds_of(rows) > 64 * 1024 - 256) { rows /= 2; }
+    const size_t lds = lds_of(rows);
+    if (lds > 64 * 1024 - 256) {
         set_error("pm_level_gsrb_sweep: boxes %d cells wide do not fit the LDS ring", L->bs[0]);
         return PM_ERR_UNSUPPORTED;
     }

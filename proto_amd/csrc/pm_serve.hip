@@ -40,8 +40,17 @@ struct SLevel {
     float* tmp;
     float* rhs;
     int n;
+    int rows;           // rows per sweep tile
     float c0, c1, lam;  // centre / neighbour coefficients, GSRB weight
 };
+
+// rows per sweep tile of a level n cells wide, and the LDS of its tile
+__host__ __device__ inline int sweep_rows(int n) { return n > 128 ? 4 : SR; }
+inline size_t sweep_lds(int n)
+{
+    const int r = sweep_rows(n);
+    return sizeof(float) * (4 * (size_t)(r + 4) * (n + 4) + 2 * (size_t)(r + 2) * (n + 2));
+}
 
 struct SReq {
     double* phi;   // FP64 solution, n^3
@@ -64,27 +73,71 @@ __device__ inline int wrapi(int v, int n)
     return v < 0 ? v + n : v;
 }
 
-// One GSRB sweep (both colours) of the tile rows [y0, y0+SR) x planes [z0, z1)
+// One GSRB sweep (both colours) of the tile rows [y0, y0+rows) x planes [z0, z1)
 // of level L, reading `in` and writing `out` (dir 0: phi -> tmp, 1: tmp -> phi).
+// phi planes (rows [y0-2, y1+2) x columns [-2, n+2)) pass through a ring of 4
+// LDS planes and rhs planes (rows [y0-1, y1+1) x columns [-1, n+1)) through a
+// ring of 2, both loaded into registers one step ahead of their use.
+template<int MAXE>
 __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv, const SItem* __restrict__ items, int dir)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
-    float* sm = reinterpret_cast<float*>(smraw);
     const SItem it = items[blockIdx.x];
     const SLevel L = lv[it.lev];
-    const int n = L.n, W = n + 4;
-    const int y0 = it.a, y1 = min(y0 + SR, n), z0 = it.b, z1 = it.c;
-    const int NP = (y1 - y0 + 4) * W;  // rows [y0-2, y1+2), columns [-2, n+2)
+    const int n = L.n, W = n + 4, WR = n + 2;
+    const int y0 = it.a, y1 = min(y0 + L.rows, n), z0 = it.b, z1 = it.c;
+    const int NP = (y1 - y0 + 4) * W, NR = (y1 - y0 + 2) * WR;
+    float* sm = reinterpret_cast<float*>(smraw);
+    float* rsm = sm + 4 * (L.rows + 4) * W;
     const float* in = dir ? L.tmp : L.phi;
     float* out = dir ? L.phi : L.tmp;
     const long nn = (long)n * n;
     auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
-    auto load = [&](int z) {
-        const float* base = in + nn * wrapi(z, n);
-        float* pl = slot(z);
-        for (int e = threadIdx.x; e < NP; e += 256) {
+    auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
+    // in-plane offsets (periodic images folded in) of this thread's elements
+    int offp[MAXE], offr[MAXE];
+#pragma unroll
+    for (int i = 0; i < MAXE; i++) {
+        const int e = threadIdx.x + 256 * i;
+        offp[i] = offr[i] = 0;
+        if (e < NP) {
             const int er = e / W, ec = e - er * W;
-            pl[e] = base[wrapi(ec - 2, n) + (long)n * wrapi(y0 - 2 + er, n)];
+            offp[i] = wrapi(ec - 2, n) + n * wrapi(y0 - 2 + er, n);
+        }
+        if (e < NR) {
+            const int er = e / WR, ec = e - er * WR;
+            offr[i] = wrapi(ec - 1, n) + n * wrapi(y0 - 1 + er, n);
+        }
+    }
+    float pre[MAXE], prr[MAXE];
+    auto issue = [&](int z) {
+        const float* base = in + nn * wrapi(z, n);
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            if (threadIdx.x + 256 * i < NP) { pre[i] = base[offp[i]]; }
+        }
+    };
+    auto commit = [&](int z) {
+        float* pl = slot(z);
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NP) { pl[e] = pre[i]; }
+        }
+    };
+    auto issue_r = [&](int z) {
+        const float* base = L.rhs + nn * wrapi(z, n);
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            if (threadIdx.x + 256 * i < NR) { prr[i] = base[offr[i]]; }
+        }
+    };
+    auto commit_r = [&](int z) {
+        float* pl = rslot(z);
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NR) { pl[e] = prr[i]; }
         }
     };
     // colour cells ((x+y+z+colour) even) of plane z, rows [ya, yb), columns [xa, xb)
@@ -92,7 +145,7 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
         float* zc = slot(z);
         const float* zm = slot(z - 1);
         const float* zp = slot(z + 1);
-        const float* rz = L.rhs + nn * wrapi(z, n);
+        const float* rz = rslot(z);
         const int hw = (xb - xa + 1) / 2;
         const int ncell = (yb - ya) * hw;
         for (int c = threadIdx.x; c < ncell; c += 256) {
@@ -108,14 +161,21 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
             s = s + L.c1 * zc[li - W];
             s = s + L.c1 * zp[li];
             s = s + L.c1 * zm[li];
-            s = s + rz[wrapi(xx, n) + (long)n * wrapi(yy, n)];
+            s = s + rz[(yy - (y0 - 1)) * WR + (xx + 1)];
             zc[li] = zc[li] + L.lam * s;
         }
     };
-    load(z0 - 2);
-    load(z0 - 1);
+    issue(z0 - 2);
+    commit(z0 - 2);
+    issue(z0 - 1);
+    commit(z0 - 1);
+    issue(z0);
+    issue_r(z0 - 1);
     for (int k = z0 - 2; k < z1; k++) {
-        load(k + 2);  // into the slot of plane k-2, last read before the previous barrier
+        commit(k + 2);    // into the slot of plane k-2, last read before the previous barrier
+        commit_r(k + 1);  // into the slot of rhs plane k-1, likewise
+        if (k + 3 <= z1 + 1) { issue(k + 3); }
+        if (k + 2 <= z1) { issue_r(k + 2); }
         __syncthreads();
         update(k + 1, y0 - 1, y1 + 1, -1, n + 1, 0);  // red, one-cell halo included
         __syncthreads();
@@ -360,7 +420,9 @@ struct Active {
 
 struct Launch {
     int kind;  // 0 sweep, 1 restrict, 2 prolong, 3 correct, 4 residual
-    int off, count, dir, nmax;
+    int off, count, dir;
+    size_t lds;  // sweeps: dynamic LDS of the largest tile
+    long np;     // sweeps: largest phi plane of a tile (elements)
 };
 
 }  // namespace
@@ -517,6 +579,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             L.tmp = a.h->tmp[l];
             L.rhs = a.h->rhs[l];
             L.n = a.ns[l];
+            L.rows = sweep_rows(a.ns[l]);
             level_coefs(a.ns[l], &L.c0, &L.c1, &L.lam);
             lv.push_back(L);
         }
@@ -536,7 +599,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
     std::vector<SItem> items;
     S->plan.clear();
     auto begin = [&](int kind, int dir) {
-        S->plan.push_back(Launch{kind, (int)items.size(), 0, dir, 0});
+        S->plan.push_back(Launch{kind, (int)items.size(), 0, dir, 0, 0});
     };
     auto end = [&]() {
         Launch& L = S->plan.back();
@@ -544,9 +607,12 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
         if (L.count == 0) { S->plan.pop_back(); }
     };
     auto sweep_items = [&](int levidx, int m) {
+        const int r = sweep_rows(m);
         for (int z0 = 0; z0 < m; z0 += SZC)
-            for (int y0 = 0; y0 < m; y0 += SR) { items.push_back(SItem{levidx, y0, z0, std::min(m, z0 + SZC)}); }
-        S->plan.back().nmax = std::max(S->plan.back().nmax, m);
+            for (int y0 = 0; y0 < m; y0 += r) { items.push_back(SItem{levidx, y0, z0, std::min(m, z0 + SZC)}); }
+        Launch& L = S->plan.back();
+        L.lds = std::max(L.lds, sweep_lds(m));
+        L.np = std::max(L.np, (long)(r + 4) * (m + 4));
     };
     const int pre = S->opt.pre, post = S->opt.post, bot = S->opt.bottom;
     // V-cycle of every non-fresh request (fresh ones only get their first residual)
@@ -618,8 +684,15 @@ int run_plan(pm_server* S, int nact, hipStream_t s)
         const SItem* it = S->d_items + L.off;
         switch (L.kind) {
         case 0: {
-            const size_t lds = sizeof(float) * 4 * (SR + 4) * (size_t)(L.nmax + 4);
-            hipLaunchKernelGGL(k_srv_gsrb, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+            const size_t lds = L.lds;
+            const long np = L.np;
+            if (np <= 1024) {
+                hipLaunchKernelGGL(k_srv_gsrb<4>, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+            } else if (np <= 2048) {
+                hipLaunchKernelGGL(k_srv_gsrb<8>, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+            } else {
+                hipLaunchKernelGGL(k_srv_gsrb<16>, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+            }
             break;
         }
         case 1:

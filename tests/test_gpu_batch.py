@@ -118,14 +118,16 @@ def test_batch_gsrb_limits():
     p_in, r_d = dev(phi), dev(rhs)
     with pytest.raises(pa.NativeError, match="out of place"):
         B.gsrb(pa.PM_F64, vp(p_in), vp(p_in), vp(r_d), 0.1)
-    B.configure(0, 32, 4)  # 4 x 36 x 132 doubles does not fit a workgroup's LDS
-    with pytest.raises(pa.NativeError, match="LDS"):
-        B.gsrb(pa.PM_F64, vp(p_in), vp(dev(phi)), vp(r_d), 0.1)
-    # the batch stays usable after a rejected call
-    B.configure(0, 8, 4)
+    # 32 rows of 132-wide FP64 planes do not fit a workgroup's LDS: the tiles
+    # are cut to fewer rows and the bits do not change
+    B.configure(0, 32, 4)
     p_out = dev(phi)
     B.gsrb(pa.PM_F64, vp(p_in), vp(p_out), vp(r_d), 0.1)
     assert first_diff(B, host(p_out), oracle_gsrb_batch(B, phi, rhs, 0.1), False) is None
+    wide = pa.Batch([(300, 4, 4)], 1)
+    src = dev(np.zeros(wide.ghosted_elems))
+    with pytest.raises(pa.NativeError, match="wider than 256"):
+        wide.gsrb(pa.PM_F64, vp(src), vp(dev(np.zeros(wide.ghosted_elems))), vp(dev(np.zeros(wide.valid_elems))), 0.1)
 
 
 def test_batch_many_small_boxes_queue():
