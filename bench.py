@@ -390,6 +390,28 @@ def main():
 
     tune("residual2", s7, s27, 3)
 
+    if args.only == "sweep":
+        # profiling helper: one-pass level GSRB sweeps of the 512^3 level in 64^3 boxes
+        nb, bs = n // 64, 64
+        lows = np.array([(x * bs, y * bs, z * bs) for z in range(nb) for y in range(nb) for x in range(nb)],
+                        dtype=np.int32).reshape(-1)
+        lay = ctypes.c_void_p()
+        pa.call("pm_layout_create", ctypes.byref(lay), 3, pa.i3([n] * 3), pa.i3([bs] * 3), pa.i3([1] * 3), nb ** 3,
+                lows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        g1, g0 = pa.i3([1, 1, 1]), pa.i3([0, 0, 0])
+        for dt, t in ((pa.PM_F64, np.float64), (pa.PM_F32, np.float32)):
+            ph = DeviceArray.zeros(nb ** 3 * (bs + 2) ** 3, t)
+            tm = DeviceArray.zeros(nb ** 3 * bs ** 3, t)
+            rh = DeviceArray.zeros(nb ** 3 * bs ** 3, t)
+            for _ in range(args.steps):
+                pa.call("pm_level_gsrb_sweep", lay, dt, ph.ptr, g1, tm.ptr, g0, rh.ptr, g0, 1.0 / n, sp)
+            stream.sync()
+            for d in (ph, tm, rh):
+                d.free()
+        pa.call("pm_layout_destroy", lay)
+        print(json.dumps({"only": args.only, "steps": args.steps}))
+        return
+
     if args.only == "batch":
         # profiling helper: the config-4 batched operators (both precisions)
         # and a plain 1 GiB device copy to calibrate the counters
