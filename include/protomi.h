@@ -375,6 +375,29 @@ int pm_batch_apply(pm_batch* batch, const pm_stencil* st, const void* src, void*
 int pm_batch_gsrb(pm_batch* batch, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
                   pm_stream_t stream);
 
+/* ------------------------------------------------------ library multigrid */
+/* Proto's LevelMultigridSolver<BoxOp_Laplace> (include/utils/
+ * Proto_LevelMultigridSolver.H, implem 1-433; examples/_common/BoxOp_Laplace.H):
+ * FAS V-cycles with the 4th-order flux-form Laplacian (DiffCellToFace fluxes,
+ * flux scale -1), damped Jacobi relaxation (2 DIM / 2 DIM / 10 sweeps, lambda =
+ * 0.5 / spectral radius), AvgDown restriction of state and residual and
+ * constant prolongation of the correction, on a periodic 3-D layout covering
+ * its domain.  num_levels counts the finest level (Proto's a_numLevels).
+ * state: level storage with 2 ghost layers; force: no ghosts.  Each
+ * operator is one launch per level; results are Proto's bits. */
+typedef struct pm_fas pm_fas;
+int pm_fas_create(pm_fas** out, const pm_layout* fine, int num_levels, double dx);
+int pm_fas_destroy(pm_fas* fas);
+int pm_fas_bytes(const pm_fas* fas, size_t* bytes);
+/* MGLevel::vCycle (408-433) */
+int pm_fas_vcycle(pm_fas* fas, void* state, const void* force, pm_stream_t stream);
+/* LevelMultigridSolver::resnorm (134-160): max |L(state) + force| */
+int pm_fas_resnorm(pm_fas* fas, void* state, const void* force, double* out_host, pm_stream_t stream);
+/* LevelMultigridSolver::solve (85-132): V-cycles until resnorm < tol *
+ * initial or max_iter; hist[0..*iters] the norms (hist may be NULL). */
+int pm_fas_solve(pm_fas* fas, void* state, const void* force, int max_iter, double tol, double* hist, int* iters,
+                 pm_stream_t stream);
+
 /* ---------------------------------------------------------------- serving */
 /* A continuous-batching Poisson solve server.  No counterpart in the reference
  * (Proto solves one problem per LevelMultigridSolver::solve call,

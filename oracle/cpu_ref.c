@@ -585,6 +585,170 @@ int orc_mg_solve(int n, int nlev, int relax, int pre, int post, int bottom, int 
         phi[p] += (LAM) * t;                                                                \
     }
 
+/* ---- library FAS multigrid with BoxOp_Laplace ------------------------------
+ * LevelMultigridSolver<BoxOp_Laplace> on a periodic cube, as one global array
+ * per level (the per-box work with exchanged ghosts gives the same values):
+ *   L(phi) = fluxScale * sum_d DIV_d(F_d) with fluxScale = -1
+ *     (Proto_BoxOpImplem.H:129-176; setFluxScale(-1) at
+ *     Proto_LevelMultigridSolverImplem.H:206),
+ *   F_d = DiffCellToFace(d, Lo) applied with scale 1/dx: coefficients
+ *     {1, -15, 15, -1} * (1/12) at offsets -2, -1, 0, +1, each then times 1/dx
+ *     (BoxOp_Laplace.H:24-31, Proto_StencilDefs.H:232-268),
+ *   DIV_d = 1.0 Shift(+e_d) - 1.0 Shift(0) with scale 1/dx, accumulated over
+ *     d = x, y, z into a zeroed divFlux; output = (0 + divFlux) * (-1);
+ *   residual = L(phi) + force; Jacobi relax phi += (-lambda) * residual with
+ *     lambda = 0.5 / (2 DIM / dx^2) (LevelMultigridSolverImplem.H:207, 358-379);
+ *   V-cycle (408-433): 2 DIM pre / post sweeps, 10 at the bottom;
+ *     crseState = AvgDown(phi); crse0 = crseState;
+ *     crseForce = AvgDown(L(phi) + force) - L_c(crseState) (coarseForce, 307-336);
+ *     fine += Constant interp of (crse0 - crseState) * -1 (fineCorrect, 381-406);
+ *   solve (85-132): res0 = max|L(phi) + force|, V-cycles until res < tol res0. */
+static void fas_apply(int n, double dx, const double* u, double* out)
+{
+    const double k[4] = {1, -15, 15, -1};
+    const double twelfth = 1.0 / 12.0;
+    const double s = 1.0 / dx;
+    double a[4];
+    for (int j = 0; j < 4; j++) { a[j] = (k[j] * twelfth) * s; }
+    const double dp = 1.0 * s, dm = -1.0 * s;
+    for (int z = 0; z < n; z++)
+    for (int y = 0; y < n; y++)
+    for (int x = 0; x < n; x++)
+    {
+        const int p[3] = {x, y, z};
+        double acc = 0;
+        for (int d = 0; d < 3; d++)
+        {
+            double v[6];  /* cells p-2e .. p+2e */
+            for (int o = -2; o <= 2; o++)
+            {
+                int q[3] = {p[0], p[1], p[2]};
+                q[d] = (q[d] + o + n) % n;
+                v[o + 2] = u[gidx(n, q[0], q[1], q[2])];
+            }
+            double fr = 0;
+            fr = fr + a[0] * v[1];
+            fr = fr + a[1] * v[2];
+            fr = fr + a[2] * v[3];
+            fr = fr + a[3] * v[4];
+            double fl = 0;
+            fl = fl + a[0] * v[0];
+            fl = fl + a[1] * v[1];
+            fl = fl + a[2] * v[2];
+            fl = fl + a[3] * v[3];
+            acc = acc + dp * fr;
+            acc = acc + dm * fl;
+        }
+        double o = 0;
+        o = o + acc;
+        o = o * -1.0;
+        out[gidx(n, x, y, z)] = o;
+    }
+}
+
+static double fas_lambda(double dx) { return 0.5 / ((2.0 * 3) / pow(dx, 2)); }
+
+void orc_fas_relax(int n, double dx, int niter, const double* force, double* phi)
+{
+    long nc = (long)n * n * n;
+    double* res = (double*)malloc(sizeof(double) * nc);
+    const double ml = -fas_lambda(dx);
+    for (int it = 0; it < niter; it++)
+    {
+        fas_apply(n, dx, phi, res);
+        for (long i = 0; i < nc; i++)
+        {
+            double r = res[i] + force[i];
+            phi[i] = phi[i] + ml * r;
+        }
+    }
+    free(res);
+}
+
+double orc_fas_resnorm(int n, double dx, const double* force, const double* phi)
+{
+    long nc = (long)n * n * n;
+    double* res = (double*)malloc(sizeof(double) * nc);
+    fas_apply(n, dx, phi, res);
+    double m = 0;
+    for (long i = 0; i < nc; i++)
+    {
+        double r = res[i] + force[i];
+        double a = r < 0 ? -r : r;
+        if (a > m) { m = a; }
+    }
+    free(res);
+    return m;
+}
+
+static void fas_avgdown(int n, const double* f, double* c)
+{
+    int nc = n / 2;
+    for (int Z = 0; Z < nc; Z++)
+    for (int Y = 0; Y < nc; Y++)
+    for (int X = 0; X < nc; X++)
+    {
+        double acc = 0;
+        for (int kz = 0; kz < 2; kz++)
+        for (int ky = 0; ky < 2; ky++)
+        for (int kx = 0; kx < 2; kx++) { acc = acc + 0.125 * f[gidx(n, 2 * X + kx, 2 * Y + ky, 2 * Z + kz)]; }
+        c[gidx(nc, X, Y, Z)] = acc;
+    }
+}
+
+static void fas_vcycle(int n, int level, double dx, const double* force, double* phi)
+{
+    if (level > 0)
+    {
+        long nf = (long)n * n * n;
+        int nc = n / 2;
+        long ncc = (long)nc * nc * nc;
+        orc_fas_relax(n, dx, 6, force, phi);
+        double* cs = (double*)malloc(sizeof(double) * ncc);
+        double* c0 = (double*)malloc(sizeof(double) * ncc);
+        double* cf = (double*)malloc(sizeof(double) * ncc);
+        double* lc = (double*)malloc(sizeof(double) * ncc);
+        double* res = (double*)malloc(sizeof(double) * nf);
+        fas_avgdown(n, phi, cs);
+        memcpy(c0, cs, sizeof(double) * ncc);
+        fas_apply(n, dx, phi, res);
+        for (long i = 0; i < nf; i++) { res[i] = res[i] + force[i]; }
+        fas_avgdown(n, res, cf);
+        fas_apply(nc, 2 * dx, cs, lc);
+        for (long i = 0; i < ncc; i++) { cf[i] = cf[i] - lc[i]; }
+        fas_vcycle(nc, level - 1, 2 * dx, cf, cs);
+        for (long i = 0; i < ncc; i++)
+        {
+            double d = c0[i] - cs[i];
+            c0[i] = d * -1.0;
+        }
+        orc_fine_interp(n, 0, c0, phi);
+        orc_fas_relax(n, dx, 6, force, phi);
+        free(cs); free(c0); free(cf); free(lc); free(res);
+    }
+    else
+    {
+        orc_fas_relax(n, dx, 10, force, phi);
+    }
+}
+
+int orc_fas_solve(int n, int num_levels, const double* force, double* phi, int max_iter, double tol,
+                  double* hist)
+{
+    double dx = 1.0 / n;
+    double res0 = orc_fas_resnorm(n, dx, force, phi);
+    hist[0] = res0;
+    int it = 0;
+    for (; it < max_iter; it++)
+    {
+        fas_vcycle(n, num_levels - 1, dx, force, phi);
+        double r = orc_fas_resnorm(n, dx, force, phi);
+        hist[it + 1] = r;
+        if (r < tol * res0) { it++; break; }
+    }
+    return it;
+}
+
 void orc_patch_gsrb_f64(int nx, int ny, int nz, int g, double dx, const double* rhs, double* phi)
 {
     double lam = -(dx * dx) / (4.0 * 3);

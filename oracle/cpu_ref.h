@@ -96,6 +96,14 @@ int orc_mg_solve(int n, int nlev, int relax, int pre, int post, int bottom, int 
  * (x+y+z) parity, the arithmetic of LaplaceOp.H:56-73, 95-123.  phi: ghosted
  * x-fastest storage; rhs: valid storage.  The f32 form uses coefficients
  * (float)Lap_j * (float)(-1/dx^2) and lambda (float)(2*(-(dx*dx)/12)). */
+/* Library FAS multigrid, LevelMultigridSolver<BoxOp_Laplace> (Jacobi, 2 DIM / 2 DIM / 10
+ * sweeps, constant prolongation) on a periodic n^3 cube with num_levels levels;
+ * hist[0..iters]; returns the V-cycle count.  Pinned by Proto's golden history
+ * and solution hash (multigrid "fas"). */
+void orc_fas_relax(int n, double dx, int niter, const double* force, double* phi);
+double orc_fas_resnorm(int n, double dx, const double* force, const double* phi);
+int orc_fas_solve(int n, int num_levels, const double* force, double* phi, int max_iter, double tol,
+                  double* hist);
 void orc_patch_gsrb_f64(int nx, int ny, int nz, int g, double dx, const double* rhs, double* phi);
 void orc_patch_gsrb_f32(int nx, int ny, int nz, int g, double dx, const float* rhs, float* phi);
 

@@ -167,6 +167,12 @@ SIGNATURES = {
     "pm_server_result": (_i, [_vp, _i, _vp]),
     "pm_server_rhs": (_i, [_i, _i, _u64, _d, _vp, _vp]),
     "pm_level_gsrb_sweep": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _vp, _ip, _d, _vp]),
+    "pm_fas_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _d]),
+    "pm_fas_destroy": (_i, [_vp]),
+    "pm_fas_bytes": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "pm_fas_vcycle": (_i, [_vp, _vp, _vp, _vp]),
+    "pm_fas_resnorm": (_i, [_vp, _vp, _vp, ctypes.POINTER(_d), _vp]),
+    "pm_fas_solve": (_i, [_vp, _vp, _vp, _i, _d, _vp, ctypes.POINTER(_i), _vp]),
     "pm_graph_begin": (_i, [_vp]),
     "pm_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
     "pm_graph_launch": (_i, [_vp, _vp]),

@@ -1,0 +1,442 @@
+// Proto's library multigrid: LevelMultigridSolver<BoxOp_Laplace> (FAS,
+// Jacobi relaxation, constant prolongation) on a periodic level, as launches
+// over whole levels.
+//
+// Operator (Proto_BoxOpImplem.H:129-176 with BoxOp_Laplace.H:24-31 and
+// setFluxScale(-1) at Proto_LevelMultigridSolverImplem.H:206):
+//   F_d(face p) = sum_j a_j u(p + (j - 2) e_d), a_j = ({1,-15,15,-1}_j * (1/12)) * (1/dx)
+//   divFlux     = sum over d = x, y, z of (1/dx) F_d(p + e_d) + (-1/dx) F_d(p)
+//   L(u)        = (0 + divFlux) * (-1)
+// accumulated in that order with one rounding per operation, so every level
+// operator gives Proto's bits.  Relaxation (358-379): u += (-lambda) (L(u) + f),
+// lambda = 0.5 / (2 DIM / dx^2), Jacobi (out of place, ping-pong with a
+// scratch level).  V-cycle (408-433): 2 DIM pre / post sweeps, 10 at the
+// bottom; crseState = AvgDown(u); crse0 = crseState; crseForce =
+// AvgDown(L(u) + f) - L_c(crseState) (coarseForce 307-336); u += Constant
+// interpolation of (crse0 - crseState) * -1 (fineCorrect 381-406).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pm_layout.hpp"
+
+struct pm_fas {
+    int nlev = 0;  // levels below the finest
+    struct Level {
+        pm_layout* layout = nullptr;
+        double dx = 0;
+        void* phi = nullptr;   // state, ghost 2 (level 0: the caller's)
+        void* tmp = nullptr;   // Jacobi scratch, ghost 2
+        void* f = nullptr;     // force, ghost 0 (level 0: the caller's)
+        void* phi0 = nullptr;  // crseState_0, then the correction, ghost 0
+    };
+    std::vector<Level> lv;
+    double* d_part = nullptr;  // per-block maxima of the residual norm
+    double* d_norm = nullptr;
+    size_t cap_part = 0;
+    size_t bytes = 0;
+};
+
+namespace pm {
+
+struct FasC {
+    double a[4];   // face-flux coefficients
+    double dp, dm; // flux-divergence coefficients
+    double mlam;   // -lambda
+};
+
+static FasC fas_coefs(double dx)
+{
+    FasC c;
+    const double k[4] = {1, -15, 15, -1};
+    const double twelfth = 1.0 / 12.0;
+    const double s = 1.0 / dx;
+    for (int j = 0; j < 4; j++) { c.a[j] = (k[j] * twelfth) * s; }
+    c.dp = 1.0 * s;
+    c.dm = -1.0 * s;
+    c.mlam = -(0.5 / ((2.0 * 3) / std::pow(dx, 2)));
+    return c;
+}
+
+// L(u) at storage index i of a box with ghost-2 strides X, XY
+__device__ inline double fas_L(const double* __restrict__ u, long i, long X, long XY, const FasC& c)
+{
+    const long st[3] = {1, X, XY};
+    double acc = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const long s = st[d];
+        double fr = 0;
+        fr = fr + c.a[0] * u[i - s];
+        fr = fr + c.a[1] * u[i];
+        fr = fr + c.a[2] * u[i + s];
+        fr = fr + c.a[3] * u[i + 2 * s];
+        double fl = 0;
+        fl = fl + c.a[0] * u[i - 2 * s];
+        fl = fl + c.a[1] * u[i - s];
+        fl = fl + c.a[2] * u[i];
+        fl = fl + c.a[3] * u[i + s];
+        acc = acc + c.dp * fr;
+        acc = acc + c.dm * fl;
+    }
+    double o = 0;
+    o = o + acc;
+    return o * -1.0;
+}
+
+// one Jacobi sweep: out = u + (-lambda) (L(u) + f) on the valid cells
+__global__ __launch_bounds__(256) void k_fas_relax(LayoutDev L, G3 gu, G3 gf, const double* __restrict__ u,
+                                                   const double* __restrict__ f, double* __restrict__ out, FasC c)
+{
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    const double* ub = u + (long)b * gvol(L, gu);
+    const double* fb = f + (long)b * gvol(L, gf);
+    double* ob = out + (long)b * gvol(L, gu);
+    const long X = L.bs[0] + 2 * gu.g[0], XY = X * (L.bs[1] + 2 * gu.g[1]);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        const long i = rel_index(L, gu, x, y, z);
+        double res = fas_L(ub, i, X, XY, c);
+        res = res + fb[rel_index(L, gf, x, y, z)];
+        ob[i] = ub[i] + c.mlam * res;
+    }
+}
+
+// per-block max |L(u) + f|
+__global__ __launch_bounds__(256) void k_fas_resnorm(LayoutDev L, G3 gu, G3 gf, const double* __restrict__ u,
+                                                     const double* __restrict__ f, FasC c, double* __restrict__ part)
+{
+    __shared__ double red[16];
+    const int b = blockIdx.y;
+    const long n = (long)L.bs[0] * L.bs[1] * L.bs[2];
+    const double* ub = u + (long)b * gvol(L, gu);
+    const double* fb = f + (long)b * gvol(L, gf);
+    const long X = L.bs[0] + 2 * gu.g[0], XY = X * (L.bs[1] + 2 * gu.g[1]);
+    double m = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % L.bs[0]);
+        const long r = t / L.bs[0];
+        const int y = (int)(r % L.bs[1]);
+        const int z = (int)(r / L.bs[1]);
+        double res = fas_L(ub, rel_index(L, gu, x, y, z), X, XY, c);
+        res = res + fb[rel_index(L, gf, x, y, z)];
+        const double a = res > 0 ? res : -res;
+        m = m < a ? a : m;
+    }
+    m = block_reduce(PM_RED_MAX, m, red);
+    if (threadIdx.x == 0) { part[(long)b * gridDim.x + blockIdx.x] = m; }
+}
+
+__global__ __launch_bounds__(1024) void k_fas_max(const double* __restrict__ part, long n, double* __restrict__ out)
+{
+    __shared__ double red[16];
+    double m = 0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) { m = m < part[i] ? part[i] : m; }
+    m = block_reduce(PM_RED_MAX, m, red);
+    if (threadIdx.x == 0) { *out = m; }
+}
+
+// crseState = AvgDown(u) (Box(2) order, x fastest) and crse0 = crseState
+__global__ __launch_bounds__(256) void k_fas_avgdown(LayoutDev F, LayoutDev C, G3 gu, G3 gc, G3 g0,
+                                                     const double* __restrict__ u, double* __restrict__ uc,
+                                                     double* __restrict__ uc0)
+{
+    const int b = blockIdx.y;
+    const int* clo = C.lows + 3 * b;
+    const long n = (long)C.bs[0] * C.bs[1] * C.bs[2];
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % C.bs[0]);
+        const long r = t / C.bs[0];
+        const int y = (int)(r % C.bs[1]);
+        const int z = (int)(r / C.bs[1]);
+        int p[3] = {2 * (clo[0] + x), 2 * (clo[1] + y), 2 * (clo[2] + z)};
+        const int fb = owner(F, p);
+        const int* flo = F.lows + 3 * fb;
+        const double* ub = u + (long)fb * gvol(F, gu);
+        const int fx = p[0] - flo[0], fy = p[1] - flo[1], fz = p[2] - flo[2];
+        double acc = 0;
+#pragma unroll
+        for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+            for (int ky = 0; ky < 2; ky++)
+#pragma unroll
+                for (int kx = 0; kx < 2; kx++) { acc = acc + 0.125 * ub[rel_index(F, gu, fx + kx, fy + ky, fz + kz)]; }
+        uc[(long)b * gvol(C, gc) + rel_index(C, gc, x, y, z)] = acc;
+        uc0[(long)b * gvol(C, g0) + rel_index(C, g0, x, y, z)] = acc;
+    }
+}
+
+// crseForce = AvgDown(L_f(u) + f) - L_c(crseState); u and crseState exchanged
+__global__ __launch_bounds__(256) void k_fas_coarse_force(LayoutDev F, LayoutDev C, G3 gu, G3 gf, G3 gc, G3 g0,
+                                                          const double* __restrict__ u, const double* __restrict__ f,
+                                                          const double* __restrict__ uc, double* __restrict__ fc,
+                                                          FasC cf, FasC cc)
+{
+    const int b = blockIdx.y;
+    const int* clo = C.lows + 3 * b;
+    const long n = (long)C.bs[0] * C.bs[1] * C.bs[2];
+    const long FX = F.bs[0] + 2 * gu.g[0], FXY = FX * (F.bs[1] + 2 * gu.g[1]);
+    const long CX = C.bs[0] + 2 * gc.g[0], CXY = CX * (C.bs[1] + 2 * gc.g[1]);
+    const double* ucb = uc + (long)b * gvol(C, gc);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % C.bs[0]);
+        const long r = t / C.bs[0];
+        const int y = (int)(r % C.bs[1]);
+        const int z = (int)(r / C.bs[1]);
+        int p[3] = {2 * (clo[0] + x), 2 * (clo[1] + y), 2 * (clo[2] + z)};
+        const int fbx = owner(F, p);
+        const int* flo = F.lows + 3 * fbx;
+        const double* ub = u + (long)fbx * gvol(F, gu);
+        const double* fb = f + (long)fbx * gvol(F, gf);
+        const int fx = p[0] - flo[0], fy = p[1] - flo[1], fz = p[2] - flo[2];
+        double acc = 0;
+#pragma unroll
+        for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+            for (int ky = 0; ky < 2; ky++)
+#pragma unroll
+                for (int kx = 0; kx < 2; kx++) {
+                    double res = fas_L(ub, rel_index(F, gu, fx + kx, fy + ky, fz + kz), FX, FXY, cf);
+                    res = res + fb[rel_index(F, gf, fx + kx, fy + ky, fz + kz)];
+                    acc = acc + 0.125 * res;
+                }
+        const double lc = fas_L(ucb, rel_index(C, gc, x, y, z), CX, CXY, cc);
+        fc[(long)b * gvol(C, g0) + rel_index(C, g0, x, y, z)] = acc - lc;
+    }
+}
+
+// correction = (crse0 - crseState) * -1, written over crse0
+__global__ __launch_bounds__(256) void k_fas_delta(LayoutDev C, G3 gc, G3 g0, const double* __restrict__ uc,
+                                                   double* __restrict__ uc0)
+{
+    const int b = blockIdx.y;
+    const long n = (long)C.bs[0] * C.bs[1] * C.bs[2];
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % C.bs[0]);
+        const long r = t / C.bs[0];
+        const int y = (int)(r % C.bs[1]);
+        const int z = (int)(r / C.bs[1]);
+        const long i0 = (long)b * gvol(C, g0) + rel_index(C, g0, x, y, z);
+        const double d = uc0[i0] - uc[(long)b * gvol(C, gc) + rel_index(C, gc, x, y, z)];
+        uc0[i0] = d * -1.0;
+    }
+}
+
+static unsigned fas_blocks(long n) { return (unsigned)std::min<long>((n + 255) / 256, 1024); }
+
+#define FAS_TRY(x)                    \
+    do {                              \
+        int rc_ = (x);                \
+        if (rc_ != PM_OK) return rc_; \
+    } while (0)
+
+static const int G2[3] = {2, 2, 2}, G0[3] = {0, 0, 0};
+
+static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_t s)
+{
+    auto& Lv = M->lv[l];
+    const LayoutDev D = ldev(Lv.layout);
+    const FasC c = fas_coefs(Lv.dx);
+    const long cells = (long)D.bs[0] * D.bs[1] * D.bs[2];
+    void* buf[2] = {u, Lv.tmp};
+    for (int it = 0; it < n; it++) {
+        FAS_TRY(pm_level_exchange(Lv.layout, PM_F64, buf[it & 1], G2, 1, s));
+        hipLaunchKernelGGL(k_fas_relax, dim3(fas_blocks(cells), D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
+                           (const double*)buf[it & 1], (const double*)f, (double*)buf[(it + 1) & 1], c);
+        PM_CHECK_LAUNCH();
+    }
+    if (n & 1) {
+        FAS_TRY(pm_level_copy(Lv.layout, Lv.tmp, G2, Lv.layout, u, G2, PM_F64, 1, s));
+    }
+    return PM_OK;
+}
+
+static int fas_vcycle(pm_fas* M, int l, void* u, const void* f, hipStream_t s)
+{
+    if (l == M->nlev) { return fas_relax(M, l, u, f, 10, s); }
+    auto& F = M->lv[l];
+    auto& C = M->lv[l + 1];
+    FAS_TRY(fas_relax(M, l, u, f, 6, s));
+    FAS_TRY(pm_level_exchange(F.layout, PM_F64, u, G2, 1, s));
+    const LayoutDev DF = ldev(F.layout), DC = ldev(C.layout);
+    const long cc = (long)DC.bs[0] * DC.bs[1] * DC.bs[2];
+    const dim3 grid(fas_blocks(cc), DC.nboxes);
+    hipLaunchKernelGGL(k_fas_avgdown, grid, dim3(256), 0, s, DF, DC, g3(G2), g3(G2), g3(G0), (const double*)u,
+                       (double*)C.phi, (double*)C.phi0);
+    PM_CHECK_LAUNCH();
+    FAS_TRY(pm_level_exchange(C.layout, PM_F64, C.phi, G2, 1, s));
+    hipLaunchKernelGGL(k_fas_coarse_force, grid, dim3(256), 0, s, DF, DC, g3(G2), g3(G0), g3(G2), g3(G0),
+                       (const double*)u, (const double*)f, (const double*)C.phi, (double*)C.f, fas_coefs(F.dx),
+                       fas_coefs(C.dx));
+    PM_CHECK_LAUNCH();
+    FAS_TRY(fas_vcycle(M, l + 1, C.phi, C.f, s));
+    hipLaunchKernelGGL(k_fas_delta, grid, dim3(256), 0, s, DC, g3(G2), g3(G0), (const double*)C.phi, (double*)C.phi0);
+    PM_CHECK_LAUNCH();
+    FAS_TRY(pm_level_prolong_add(F.layout, C.layout, PM_F64, C.phi0, G0, u, G2, 0, s));
+    return fas_relax(M, l, u, f, 6, s);
+}
+
+static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream_t s)
+{
+    auto& Lv = M->lv[0];
+    const LayoutDev D = ldev(Lv.layout);
+    FAS_TRY(pm_level_exchange(Lv.layout, PM_F64, u, G2, 1, s));
+    const long cells = (long)D.bs[0] * D.bs[1] * D.bs[2];
+    const unsigned bpb = fas_blocks(cells);
+    hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0), (const double*)u,
+                       (const double*)f, fas_coefs(Lv.dx), M->d_part);
+    hipLaunchKernelGGL(k_fas_max, dim3(1), dim3(1024), 0, s, (const double*)M->d_part, (long)bpb * D.nboxes,
+                       M->d_norm);
+    PM_CHECK_LAUNCH();
+    PM_CHECK_HIP(hipMemcpyAsync(out, M->d_norm, sizeof(double), hipMemcpyDeviceToHost, s));
+    PM_CHECK_HIP(hipStreamSynchronize(s));
+    return PM_OK;
+}
+
+static int fas_alloc(pm_fas* M, void** p, size_t bytes)
+{
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        set_error("pm_fas_create: allocation of %zu bytes failed", bytes);
+        return PM_ERR_NOMEM;
+    }
+    M->bytes += bytes;
+    return PM_OK;
+}
+
+}  // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_fas_destroy(pm_fas* M)
+{
+    if (!M) { return PM_OK; }
+    for (size_t l = 0; l < M->lv.size(); l++) {
+        auto& L = M->lv[l];
+        if (l > 0) {
+            hipFree(L.phi);
+            hipFree(L.f);
+            hipFree(L.phi0);
+        }
+        hipFree(L.tmp);
+        pm_layout_destroy(L.layout);
+    }
+    hipFree(M->d_part);
+    hipFree(M->d_norm);
+    delete M;
+    return PM_OK;
+}
+
+int pm_fas_create(pm_fas** out, const pm_layout* fine, int num_levels, double dx)
+{
+    PM_REQUIRE(out && fine && dx > 0, "pm_fas_create: bad argument");
+    *out = nullptr;
+    PM_REQUIRE(num_levels >= 1 && num_levels <= 24, "pm_fas_create: num_levels must be 1..24");
+    PM_REQUIRE(fine->dim == 3, "pm_fas_create: 3-D layouts only");
+    for (int k = 0; k < 3; k++) {
+        PM_REQUIRE(fine->periodic[k], "pm_fas_create: the solver needs a periodic domain");
+        PM_REQUIRE(fine->domain[k] % (1 << (num_levels - 1)) == 0, "pm_fas_create: domain not coarsenable");
+        PM_REQUIRE(fine->bs[k] >= 2, "pm_fas_create: boxes need 2+ cells per side");
+    }
+    PM_REQUIRE(fine->nboxes == fine->nb[0] * fine->nb[1] * fine->nb[2],
+               "pm_fas_create: the layout must cover its domain");
+    pm_fas* M = new pm_fas();
+    M->nlev = num_levels - 1;
+    M->lv.resize(num_levels);
+    int rc = pm_layout_create(&M->lv[0].layout, 3, fine->domain, fine->bs, fine->periodic, fine->nboxes,
+                              fine->lows.data());
+    M->lv[0].dx = dx;
+    // coarse layouts (MGLevel::define, 213-239): domain halves, box size kept
+    // when it divides the coarse domain, else one box
+    int dom[3] = {fine->domain[0], fine->domain[1], fine->domain[2]};
+    int box[3] = {fine->bs[0], fine->bs[1], fine->bs[2]};
+    for (int l = 1; l < num_levels && rc == PM_OK; l++) {
+        bool divisible = true;
+        for (int k = 0; k < 3; k++) {
+            dom[k] /= 2;
+            if (dom[k] % box[k] != 0) { divisible = false; }
+        }
+        if (!divisible) {
+            for (int k = 0; k < 3; k++) { box[k] = dom[k]; }
+        }
+        std::vector<int> lows;
+        for (int z = 0; z < dom[2] / box[2]; z++)
+            for (int y = 0; y < dom[1] / box[1]; y++)
+                for (int x = 0; x < dom[0] / box[0]; x++) {
+                    lows.push_back(x * box[0]);
+                    lows.push_back(y * box[1]);
+                    lows.push_back(z * box[2]);
+                }
+        rc = pm_layout_create(&M->lv[l].layout, 3, dom, box, fine->periodic, (int)(lows.size() / 3), lows.data());
+        M->lv[l].dx = M->lv[l - 1].dx * 2;
+    }
+    size_t maxpart = 0;
+    for (int l = 0; l < num_levels && rc == PM_OK; l++) {
+        auto& L = M->lv[l];
+        const LayoutDev D = ldev(L.layout);
+        const size_t e2 = (size_t)gvol(D, g3(G2)) * D.nboxes, e0 = (size_t)gvol(D, g3(G0)) * D.nboxes;
+        rc = fas_alloc(M, &L.tmp, 8 * e2);
+        if (l > 0 && rc == PM_OK) { rc = fas_alloc(M, &L.phi, 8 * e2); }
+        if (l > 0 && rc == PM_OK) { rc = fas_alloc(M, &L.f, 8 * e0); }
+        if (l > 0 && rc == PM_OK) { rc = fas_alloc(M, &L.phi0, 8 * e0); }
+        if (l == 0) { maxpart = (size_t)fas_blocks((long)e0 / D.nboxes) * D.nboxes; }
+    }
+    if (rc == PM_OK) { rc = fas_alloc(M, (void**)&M->d_part, 8 * std::max<size_t>(maxpart, 1)); }
+    if (rc == PM_OK) { rc = fas_alloc(M, (void**)&M->d_norm, 64); }
+    if (rc != PM_OK) {
+        pm_fas_destroy(M);
+        return rc;
+    }
+    *out = M;
+    return PM_OK;
+}
+
+int pm_fas_vcycle(pm_fas* M, void* state, const void* force, pm_stream_t stream)
+{
+    PM_REQUIRE(M && state && force, "pm_fas_vcycle: null argument");
+    return fas_vcycle(M, 0, state, force, (hipStream_t)stream);
+}
+
+int pm_fas_resnorm(pm_fas* M, void* state, const void* force, double* out_host, pm_stream_t stream)
+{
+    PM_REQUIRE(M && state && force && out_host, "pm_fas_resnorm: null argument");
+    return fas_resnorm(M, state, force, out_host, (hipStream_t)stream);
+}
+
+int pm_fas_solve(pm_fas* M, void* state, const void* force, int max_iter, double tol, double* hist, int* iters,
+                 pm_stream_t stream)
+{
+    PM_REQUIRE(M && state && force && max_iter >= 0, "pm_fas_solve: bad argument");
+    const hipStream_t s = (hipStream_t)stream;
+    double res0 = 0, r = 0;
+    FAS_TRY(fas_resnorm(M, state, force, &res0, s));
+    if (hist) { hist[0] = res0; }
+    r = res0;
+    int it = 0;
+    for (; it < max_iter; it++) {
+        FAS_TRY(fas_vcycle(M, 0, state, force, s));
+        FAS_TRY(fas_resnorm(M, state, force, &r, s));
+        if (hist) { hist[it + 1] = r; }
+        if (r < tol * res0) {
+            it++;
+            break;
+        }
+    }
+    if (iters) { *iters = it; }
+    return PM_OK;
+}
+
+int pm_fas_bytes(const pm_fas* M, size_t* bytes)
+{
+    PM_REQUIRE(M && bytes, "pm_fas_bytes: null argument");
+    *bytes = M->bytes;
+    return PM_OK;
+}
+
+}  // extern "C"

@@ -323,3 +323,47 @@ def test_config3_full_size_512():
     err = np.abs(phi32 - phi64).max() / np.abs(phi64).max()
     say("config3 rel err", err)
     assert err < 1e-8
+
+
+G2 = pa.i3([2, 2, 2])
+
+
+def _fas(n, bs, num_levels, force, maxiter=20, tol=1e-10):
+    L = Layout(n, bs)
+    fas = ctypes.c_void_p()
+    pa.call("pm_fas_create", ctypes.byref(fas), L.h, num_levels, 1.0 / n)
+    phi_d = dev(L.pack(np.zeros(n ** 3), 2))
+    f_d = dev(L.pack(force, 0))
+    hist = np.zeros(maxiter + 1)
+    it = ctypes.c_int()
+    pa.call("pm_fas_solve", fas, vp(phi_d), vp(f_d), maxiter, tol, hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            ctypes.byref(it), None)
+    phi = L.unpack(host(phi_d), 2)
+    pa.call("pm_fas_destroy", fas)
+    return it.value, hist[:it.value + 1], phi
+
+
+def test_fas_matches_proto():
+    """LevelMultigridSolver<BoxOp_Laplace> on the device: Proto's residual
+    history and solution bits (32^3 in 16^3 boxes, 4 levels)."""
+    z = np.load(os.path.join(util.GOLDEN, "multigrid.npz"), allow_pickle=False)
+    meta = json.load(open(os.path.join(util.GOLDEN, "multigrid.json")))
+    say("fas golden")
+    it, hist, phi = _fas(32, 16, 4, util.gaussian_rhs(32, 6, 2))
+    assert it == meta["fas"]["iters"]
+    assert hist.tobytes() == z["fas:hist"].tobytes()
+    assert hashlib.sha256(phi.tobytes()).hexdigest() == meta["fas"]["phi_sha256"]
+
+
+@pytest.mark.parametrize("n,bs,levels", [(64, 32, 3), (48, 16, 3), (32, 8, 4)])
+def test_fas_vs_oracle(n, bs, levels):
+    say("fas vs oracle", n, bs, levels)
+    force = util.gaussian_rhs(n, 5, 17)
+    maxit = 8
+    it, hist, phi = _fas(n, bs, levels, force, maxiter=maxit)
+    ophi = np.zeros(n ** 3)
+    ohist = np.zeros(maxit + 2)
+    oit = util.oracle().orc_fas_solve(n, levels, util.ptr(force), util.ptr(ophi), maxit, 1e-10, util.ptr(ohist))
+    assert it == oit
+    assert hist.tobytes() == ohist[:oit + 1].tobytes()
+    assert phi.tobytes() == ophi.tobytes()

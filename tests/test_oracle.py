@@ -408,3 +408,20 @@ def test_patch_gsrb_golden_and_invariant(nx, ny, nz, g):
     ph32 = uniform(80 + nx, 0, ph.size).astype(np.float32)
     util.oracle().orc_patch_gsrb_f32(nx, ny, nz, g, ctypes.c_double(dx), ptr(rh.astype(np.float32)), ptr(ph32))
     assert np.abs(ph32 - ph).max() < 1e-4 * (np.abs(ph).max() + 1)
+
+
+def test_fas_matches_proto_history():
+    """Library FAS (LevelMultigridSolver<BoxOp_Laplace>) restatement: Proto's
+    residual history and solution bits at 32^3, 4 levels."""
+    z = load("multigrid.npz")
+    meta = json.load(open(os.path.join(util.GOLDEN, "multigrid.json")))
+    n = 32
+    rhs = util.gaussian_rhs(n, 6, 2)
+    phi = np.zeros(n ** 3)
+    hist = np.zeros(32)
+    it = util.oracle().orc_fas_solve(n, 4, ptr(rhs), ptr(phi), 20, 1e-10, ptr(hist))
+    assert it == meta["fas"]["iters"]
+    assert hist[:it + 1].tobytes() == z["fas:hist"].tobytes()
+    assert hashlib.sha256(phi.tobytes()).hexdigest() == meta["fas"]["phi_sha256"]
+    # the last residual meets the tolerance and the history falls monotonically
+    assert hist[it] < 1e-10 * hist[0] and np.all(np.diff(hist[:it + 1]) < 0)

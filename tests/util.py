@@ -54,6 +54,10 @@ def oracle():
         _ci = ctypes.c_int
         lib.orc_patch_gsrb_f64.argtypes = [_ci, _ci, _ci, _ci, ctypes.c_double, _dp, _dp]
         lib.orc_patch_gsrb_f32.argtypes = [_ci, _ci, _ci, _ci, ctypes.c_double, _fp, _fp]
+        lib.orc_fas_solve.restype = ctypes.c_int
+        lib.orc_fas_solve.argtypes = [_ci, _ci, _dp, _dp, _ci, ctypes.c_double, _dp]
+        lib.orc_fas_resnorm.restype = ctypes.c_double
+        lib.orc_fas_resnorm.argtypes = [_ci, ctypes.c_double, _dp, _dp]
         lib.orc_uniform_f64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_long, _dp]
         _oracle = lib
     return _oracle
