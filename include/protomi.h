@@ -393,6 +393,9 @@ int pm_fas_bytes(const pm_fas* fas, size_t* bytes);
 int pm_fas_vcycle(pm_fas* fas, void* state, const void* force, pm_stream_t stream);
 /* LevelMultigridSolver::resnorm (134-160): max |L(state) + force| */
 int pm_fas_resnorm(pm_fas* fas, void* state, const void* force, double* out_host, pm_stream_t stream);
+/* the same, also writing the residual L(state) + force (valid cells, no ghosts) */
+int pm_fas_residual(pm_fas* fas, void* state, const void* force, void* residual, double* out_host,
+                    pm_stream_t stream);
 /* LevelMultigridSolver::solve (85-132): V-cycles until resnorm < tol *
  * initial or max_iter; hist[0..*iters] the norms (hist may be NULL). */
 int pm_fas_solve(pm_fas* fas, void* state, const void* force, int max_iter, double tol, double* hist, int* iters,

@@ -172,6 +172,7 @@ SIGNATURES = {
     "pm_fas_bytes": (_i, [_vp, ctypes.POINTER(_sz)]),
     "pm_fas_vcycle": (_i, [_vp, _vp, _vp, _vp]),
     "pm_fas_resnorm": (_i, [_vp, _vp, _vp, ctypes.POINTER(_d), _vp]),
+    "pm_fas_residual": (_i, [_vp, _vp, _vp, _vp, ctypes.POINTER(_d), _vp]),
     "pm_fas_solve": (_i, [_vp, _vp, _vp, _i, _d, _vp, ctypes.POINTER(_i), _vp]),
     "pm_graph_begin": (_i, [_vp]),
     "pm_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),

@@ -1,6 +1,6 @@
 // Proto's C++ API for the MI355X hot path: include this instead of Proto's
 // include/Proto.H.  Point/Box, BoxData, Stencil, Reduction, InterpStencil,
-// LevelBoxData, averageDown and LevelMultigrid compile with any C++17
+// LevelBoxData, averageDown, LevelMultigrid and LevelMultigridSolver compile with any C++17
 // compiler and link proto_amd/libprotomi.so; forall needs hipcc (device
 // lambdas) and is included when compiling with it.
 #pragma once
@@ -11,6 +11,7 @@
 #include "Proto_InterpStencil.H"
 #include "Proto_Level.H"
 #include "Proto_LevelMultigrid.H"
+#include "Proto_LevelMultigridSolver.H"
 #ifdef __HIPCC__
 #include "Proto_Forall.H"
 #endif

@@ -62,14 +62,9 @@ public:
               std::vector<double>* hist = nullptr)
     {
         check(phi, rhs);
-        if (m_opt.max_iter != maxIter || m_opt.tol != tol) {
-            // the handle keeps the options it was created with: rebuild it for new ones
-            m_opt.max_iter = maxIter;
-            m_opt.tol = tol;
-            pm_mg* nmg = nullptr;
-            pm_check(pm_mg_create(&nmg, m_layout.handle(), m_level, m_dx, &m_opt), "LevelMultigrid::solve");
-            m_mg.reset(nmg, [](pm_mg* p) { pm_mg_destroy(p); });
-        }
+        m_opt.max_iter = maxIter;
+        m_opt.tol = tol;
+        pm_check(pm_mg_set_tolerance(m_mg.get(), tol, maxIter), "LevelMultigrid::solve");
         std::vector<double> h(maxIter + 1, 0.0);
         int iters = 0;
         const int rc = pm_mg_solve(m_mg.get(), phi.data(), rhs.data(), h.data(), &iters, protoCurrentStream());

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(256) void k_fas_relax(LayoutDev L, G3 gu, G3 gf, co
 
 // per-block max |L(u) + f|
 __global__ __launch_bounds__(256) void k_fas_resnorm(LayoutDev L, G3 gu, G3 gf, const double* __restrict__ u,
-                                                     const double* __restrict__ f, FasC c, double* __restrict__ part)
+                                                     const double* __restrict__ f, FasC c, double* __restrict__ part,
+                                                     double* __restrict__ res_out)
 {
     __shared__ double red[16];
     const int b = blockIdx.y;
@@ -124,6 +125,7 @@ __global__ __launch_bounds__(256) void k_fas_resnorm(LayoutDev L, G3 gu, G3 gf, 
         const int z = (int)(r / L.bs[1]);
         double res = fas_L(ub, rel_index(L, gu, x, y, z), X, XY, c);
         res = res + fb[rel_index(L, gf, x, y, z)];
+        if (res_out) { res_out[b * n + t] = res; }
         const double a = res > 0 ? res : -res;
         m = m < a ? a : m;
     }
@@ -280,7 +282,7 @@ static int fas_vcycle(pm_fas* M, int l, void* u, const void* f, hipStream_t s)
     return fas_relax(M, l, u, f, 6, s);
 }
 
-static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream_t s)
+static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream_t s, double* res_out = nullptr)
 {
     auto& Lv = M->lv[0];
     const LayoutDev D = ldev(Lv.layout);
@@ -288,7 +290,7 @@ static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream
     const long cells = (long)D.bs[0] * D.bs[1] * D.bs[2];
     const unsigned bpb = fas_blocks(cells);
     hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0), (const double*)u,
-                       (const double*)f, fas_coefs(Lv.dx), M->d_part);
+                       (const double*)f, fas_coefs(Lv.dx), M->d_part, res_out);
     hipLaunchKernelGGL(k_fas_max, dim3(1), dim3(1024), 0, s, (const double*)M->d_part, (long)bpb * D.nboxes,
                        M->d_norm);
     PM_CHECK_LAUNCH();
@@ -407,6 +409,12 @@ int pm_fas_resnorm(pm_fas* M, void* state, const void* force, double* out_host, 
 {
     PM_REQUIRE(M && state && force && out_host, "pm_fas_resnorm: null argument");
     return fas_resnorm(M, state, force, out_host, (hipStream_t)stream);
+}
+
+int pm_fas_residual(pm_fas* M, void* state, const void* force, void* residual, double* out_host, pm_stream_t stream)
+{
+    PM_REQUIRE(M && state && force && residual && out_host, "pm_fas_residual: null argument");
+    return fas_resnorm(M, state, force, out_host, (hipStream_t)stream, (double*)residual);
 }
 
 int pm_fas_solve(pm_fas* M, void* state, const void* force, int max_iter, double tol, double* hist, int* iters,
