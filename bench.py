@@ -179,6 +179,29 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
         if name in ("fp64", "fp32_ir"):
             sols[name] = phi
         pa.call("pm_mg_destroy", mg)
+    # Proto's library solver on the same problem: LevelMultigridSolver<BoxOp_Laplace>
+    # (FAS, 4th-order flux-form operator, Jacobi), state with 2 ghost layers
+    fas = ctypes.c_void_p()
+    pa.call("pm_fas_create", ctypes.byref(fas), lay, nlev + 1, 1.0 / n)
+    st = DeviceArray.zeros(nb ** 3 * (bs + 4) ** 3)
+    hist = np.zeros(41)
+    it = ctypes.c_int()
+    times = []
+    for r in range(2):
+        pa.call("pm_memset_zero", st.ptr, st.nbytes, stream.h)
+        stream.sync()
+        t0 = time.perf_counter()
+        pa.call("pm_fas_solve", fas, st.ptr, rhs.ptr, 40, 1e-10, hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                ctypes.byref(it), stream.h)
+        times.append(time.perf_counter() - t0)
+    fb = ctypes.c_size_t()
+    pa.call("pm_fas_bytes", fas, ctypes.byref(fb))
+    out["library_fas_fp64"] = {"iters": it.value, "solve_s": round(times[-1], 5),
+                               "solves_per_s": round(1.0 / times[-1], 3),
+                               "rel_residual": float(hist[it.value] / hist[0]), "hierarchy_bytes": fb.value,
+                               "operator": "BoxOp_Laplace (4th-order flux form), FAS, Jacobi 6/6/10"}
+    pa.call("pm_fas_destroy", fas)
+    st.free()
     # one GSRB sweep of the finest level: exchange / red / exchange / black vs one pass
     g1, g0 = pa.i3([1, 1, 1]), pa.i3([0, 0, 0])
     sweeps = {}
