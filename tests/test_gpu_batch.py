@@ -200,3 +200,22 @@ def test_batch_autotune_keeps_bits():
     p_d = dev(src)
     B.gsrb(pa.PM_F32, vp(s_d), vp(p_d), vp(r_d), 0.5)
     assert first_diff(B, host(p_d), oracle_gsrb_batch(B, src, rhs, 0.5), False) is None
+
+
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+@pytest.mark.parametrize("ghost", [1, 2])
+def test_batch_gsrb_even_widths_vs_oracle(dt, ghost):
+    """Batches whose boxes all have even widths take the colour-by-colour sweep."""
+    sizes = [(16, 16, 16), (24, 8, 6), (2, 2, 2), (64, 10, 4), (4, 30, 7), (8, 2, 20), (48, 16, 16), (6, 6, 6)]
+    B = pa.Batch(sizes, ghost)
+    t = NPDT[dt]
+    phi, rhs = inputs(B, dt, 31 + ghost)
+    dx = 1.0 / 48
+    want = oracle_gsrb_batch(B, phi, rhs, dx)
+    p_in, r_d = dev(phi), dev(rhs)
+    for rows in (1, 3, 8, 16):
+        B.configure(0, rows, 4)
+        say("even batch gsrb", dt, ghost, rows)
+        p_out = dev(phi)
+        B.gsrb(dt, vp(p_in), vp(p_out), vp(r_d), dx)
+        assert first_diff(B, host(p_out), want, False) is None, (rows, first_diff(B, host(p_out), want, False))

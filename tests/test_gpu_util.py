@@ -1,0 +1,139 @@
+"""GPU tests of the utility entry points: device copy and zero fill, the
+seeded separable initialiser, level reductions and separable updates (against
+numpy on the same bits), layout queries of the multigrid hierarchy, the FAS
+residual norm, and the tune-table round trip."""
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import proto_amd as pa
+import util
+from gpu_util import dev, host, say, vp, zeros
+from test_gpu_level import G0, G1, Layout
+
+pytestmark = pytest.mark.gpu
+
+_dpp = ctypes.POINTER(ctypes.c_double)
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dpp)
+
+
+def test_copy_device_and_zero_fill():
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 2 ** 32, size=(1 << 18) + 12, dtype=np.uint32)  # bytes a multiple of 16
+    src = dev(a)
+    for nt in (0, 1):
+        dst = zeros(a.size, np.uint32)
+        pa.call("pm_copy_device", vp(dst), vp(src), a.nbytes, nt, None)
+        assert host(dst).tobytes() == a.tobytes()
+    assert pa.lib().pm_copy_device(vp(dst), vp(src), a.nbytes - 4, 0, None) == -1
+    pa.call("pm_memset_zero", vp(dst), a.nbytes, None)
+    assert not host(dst).any()
+
+
+def test_fill_separable_matches_numpy():
+    n, g = 12, 2
+    x = (np.arange(n) + 0.5) / n
+    tx, ty, tz = np.sin(2 * np.pi * x), np.cos(2 * np.pi * x), np.sin(4 * np.pi * x) + 2
+    b = pa.Box.of([-g] * 3, [n - 1 + g] * 3)
+    d = zeros(b.size())
+    pa.call("pm_fill_separable", pa.PM_F64, vp(d), ctypes.byref(b), ctypes.byref(b), pa.i3([n] * 3), dptr(tx),
+            dptr(ty), dptr(tz), 1e-3, 5, 2, None)
+    idx = np.arange(-g, n + g) % n
+    w = idx[:, None, None], idx[None, :, None], idx[None, None, :]  # z, y, x
+    u = util.uniform(5, 2, n ** 3).reshape(n, n, n)
+    want = (tx[w[2]] * ty[w[1]]) * tz[w[0]] + 1e-3 * u[w[0], w[1], w[2]]
+    assert host(d).tobytes() == np.ascontiguousarray(want).reshape(-1).tobytes()
+
+
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+def test_level_reduce_and_add_separable(dt):
+    t = np.float64 if dt == pa.PM_F64 else np.float32
+    n, bs = 16, 8
+    L = Layout(n, bs)
+    glob = util.uniform(91, 0, n ** 3).astype(t)
+    data = dev(L.pack(glob, 1))
+    out = zeros(1, t)
+    say("level reduce", dt)
+    for op, f in ((pa.RED_MAX, np.max), (pa.RED_MIN, np.min), (pa.RED_ABS, lambda v: np.abs(v).max())):
+        pa.call("pm_level_reduce", L.h, dt, vp(data), G1, 1, 0, op, vp(out), 0, None)
+        assert host(out)[0] == f(glob)
+    pa.call("pm_level_reduce", L.h, dt, vp(data), G1, 1, 0, pa.RED_SUM, vp(out), 0, None)
+    tol = n ** 3 * np.finfo(t).eps * np.abs(glob).sum()
+    assert abs(float(host(out)[0]) - float(glob.astype(np.float64).sum())) <= tol
+    # data += coef (tx[x] ty[y]) tz[z] on valid cells, ghosts untouched
+    tx, ty, tz = (util.uniform(92, k, n) for k in range(3))
+    before = host(data)
+    pa.call("pm_level_add_separable", L.h, dt, vp(data), G1, 0.25, dptr(tx), dptr(ty), dptr(tz), None)
+    prod = (tx[None, None, :] * ty[None, :, None]) * tz[:, None, None]
+    want = (glob.reshape(n, n, n).astype(np.float64) + 0.25 * prod).astype(t)  # one rounding to T
+    got = L.unpack(host(data), 1).reshape(n, n, n)
+    assert got.tobytes() == want.tobytes()
+    gb = before.reshape(len(L.lows), bs + 2, bs + 2, bs + 2)
+    ga = host(data).reshape(len(L.lows), bs + 2, bs + 2, bs + 2)
+    assert ga[:, 0].tobytes() == gb[:, 0].tobytes()  # a ghost plane
+
+
+def test_multigrid_level_layouts():
+    n, bs, nlev = 64, 16, 3
+    L = Layout(n, bs)
+    opt = pa.MgOptions()
+    pa.lib().pm_mg_default_options(ctypes.byref(opt))
+    mg = ctypes.c_void_p()
+    pa.call("pm_mg_create", ctypes.byref(mg), L.h, nlev, 1.0 / n, ctypes.byref(opt))
+    sizes = []
+    for l in range(nlev + 1):
+        lay = ctypes.c_void_p()
+        pa.call("pm_mg_level_layout", mg, l, ctypes.byref(lay))
+        e = ctypes.c_size_t()
+        pa.call("pm_layout_box_elems", lay, G1, ctypes.byref(e))
+        sizes.append(round(e.value ** (1 / 3)) - 2)
+    # LevelMultigrid::define: box size kept while it divides the coarse domain, then one box
+    assert sizes == [16, 16, 16, 8]
+    assert pa.lib().pm_mg_level_layout(mg, nlev + 1, ctypes.byref(ctypes.c_void_p())) == -1
+    pa.call("pm_mg_destroy", mg)
+
+
+def test_fas_resnorm_equals_residual_norm():
+    n, bs = 32, 16
+    L = Layout(n, bs)
+    fas = ctypes.c_void_p()
+    pa.call("pm_fas_create", ctypes.byref(fas), L.h, 3, 1.0 / n)
+    phi = dev(L.pack(util.uniform(93, 0, n ** 3), 2))
+    f = dev(L.pack(util.uniform(93, 1, n ** 3), 0))
+    r1, r2 = ctypes.c_double(), ctypes.c_double()
+    pa.call("pm_fas_resnorm", fas, vp(phi), vp(f), ctypes.byref(r1), None)
+    res = zeros(len(L.lows) * bs ** 3)
+    pa.call("pm_fas_residual", fas, vp(phi), vp(f), vp(res), ctypes.byref(r2), None)
+    assert r1.value == r2.value == np.abs(host(res)).max()
+    # the residual of a constant state is the force
+    pa.call("pm_level_setval", L.h, pa.PM_F64, vp(phi), pa.i3([2, 2, 2]), 1, 3.0, None)
+    pa.call("pm_fas_residual", fas, vp(phi), vp(f), vp(res), ctypes.byref(r2), None)
+    assert np.abs(host(res) - host(f)).max() < 1e-9
+    pa.call("pm_fas_destroy", fas)
+
+
+def test_tune_table_round_trip():
+    coefs, offs, *_ = util.lib_stencil("Laplacian", 0, 3, np.float64)
+    st = pa.Stencil(coefs, offs, pa.PM_F64)
+    n = 64
+    ub = pa.Box.of([-1] * 3, [n] * 3)
+    rb = pa.Box.of([0] * 3, [n - 1] * 3)
+    cfg, ms = pa.TileConfig(), ctypes.c_float()
+    pa.call("pm_stencil_autotune", st.h, None, 0, ctypes.byref(ub), ctypes.byref(rb), 1, ctypes.byref(cfg),
+            ctypes.byref(ms), None)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "tune.txt").encode()
+        pa.call("pm_tune_table_save", path)
+        other = pa.TileConfig(1, 2, 1 - cfg.nt_store, 16)
+        pa.call("pm_stencil_set_tile_config", st.h, None, 0, ctypes.byref(rb), ctypes.byref(other))
+        pa.call("pm_tune_table_load", path)
+        got = pa.TileConfig()
+        pa.call("pm_stencil_get_tile_config", st.h, None, 0, ctypes.byref(rb), ctypes.byref(got))
+        assert got.as_dict() == cfg.as_dict()
+        assert len(open(path).read().strip().splitlines()) >= 1
