@@ -286,6 +286,18 @@ int pm_level_reduce(const pm_layout* layout, int dtype, const void* data, const 
 int pm_level_add_separable(const pm_layout* layout, int dtype, void* data, const int ghost[3], double coef,
                            const double* tx_host, const double* ty_host, const double* tz_host, pm_stream_t stream);
 
+/* LevelBoxData::linearSize / linearOut / linearIn (Proto_LevelBoxDataImplem.H:
+ * 412-507 over BoxData::linearOut, Proto_BoxDataImplem.H:1619-1648): the
+ * buffer holds box after box in layout order, component after component,
+ * each over the ghosted box x fastest -- this library's level storage -- so
+ * both are one copy.  to_host / from_host: the buffer is host memory
+ * (otherwise device memory, as Proto's DEVICE buffers are). */
+int pm_level_linear_size(const pm_layout* layout, int dtype, const int ghost[3], int ncomp, size_t* bytes);
+int pm_level_linear_out(const pm_layout* layout, int dtype, const void* data, const int ghost[3], int ncomp,
+                        void* buffer, int to_host, pm_stream_t stream);
+int pm_level_linear_in(const pm_layout* layout, int dtype, void* data, const int ghost[3], int ncomp,
+                       const void* buffer, int from_host, pm_stream_t stream);
+
 /* ------------------------------------------------------------- multigrid */
 /* Correction-scheme V-cycle solver on a periodic level
  * (examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:13-160 with the

@@ -428,6 +428,34 @@ int ref_level_exchange(int n, int boxSize, int ghost, const double* g, double fi
     return ib;
 }
 
+// LevelBoxData<double, 2>::linearOut (Proto_LevelBoxDataImplem.H:412-507 over
+// BoxData::linearOut, Proto_BoxDataImplem.H:1619-1648) of a level whose every
+// cell -- ghosts included -- holds value(p, c) = p.x + 1000 p.y + 1e6 p.z +
+// 0.5 c (p the global, unwrapped point).  Returns the byte count written.
+long ref_level_linear_out(int n, int boxSize, int ghost, double* buffer, int* lows)
+{
+    DisjointBoxLayout layout = periodic_layout(n, boxSize);
+    LevelBoxData<double, 2> L(layout, Point::Ones(ghost));
+    int ib = 0;
+    for (auto dit : layout)
+    {
+        Box v = layout[dit];
+        for (int d = 0; d < 3; d++) { lows[3 * ib + d] = (d < DIM) ? v.low()[d] : 0; }
+        ib++;
+        BoxData<double, 2>& b = L[dit];
+        for (auto it = b.box().begin(); it != b.box().end(); ++it)
+        {
+            Point p = *it;
+            for (int c = 0; c < 2; c++)
+            {
+                b(p, c) = p[0] + 1000.0 * p[1] + (DIM > 2 ? 1e6 * p[2] : 0.0) + 0.5 * c;
+            }
+        }
+    }
+    L.linearOut(buffer);
+    return (long)L.linearSize();
+}
+
 // Archived correction-scheme multigrid with damped point Jacobi
 // (examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp, compiled from its
 // own source).  phi (in/out) and rhs are global x-fastest arrays; hist gets

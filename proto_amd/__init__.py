@@ -174,6 +174,9 @@ SIGNATURES = {
     "pm_fas_resnorm": (_i, [_vp, _vp, _vp, ctypes.POINTER(_d), _vp]),
     "pm_fas_residual": (_i, [_vp, _vp, _vp, _vp, ctypes.POINTER(_d), _vp]),
     "pm_fas_solve": (_i, [_vp, _vp, _vp, _i, _d, _vp, ctypes.POINTER(_i), _vp]),
+    "pm_level_linear_size": (_i, [_vp, _i, _ip, _i, ctypes.POINTER(_sz)]),
+    "pm_level_linear_out": (_i, [_vp, _i, _vp, _ip, _i, _vp, _i, _vp]),
+    "pm_level_linear_in": (_i, [_vp, _i, _vp, _ip, _i, _vp, _i, _vp]),
     "pm_graph_begin": (_i, [_vp]),
     "pm_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
     "pm_graph_launch": (_i, [_vp, _vp]),

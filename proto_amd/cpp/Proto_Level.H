@@ -204,6 +204,33 @@ public:
     T max(unsigned int comp = 0) const { return reduce<Max>(comp); }
     T min(unsigned int comp = 0) const { return reduce<Min>(comp); }
     T integrate(T dx, unsigned int comp = 0) const { return reduce<Sum>(comp) * std::pow(dx, DIM); }
+    // linearSize / linearOut / linearIn (Proto_LevelBoxDataImplem.H:412-507): the
+    // buffer is device memory (Proto's MemType rule), box after box in layout
+    // order, component after component over each ghosted box
+    size_t linearSize() const
+    {
+        size_t b = 0;
+        int g[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; d++) { g[d] = m_ghost[d]; }
+        pm_check(pm_level_linear_size(m_layout.handle(), pm_type<T>::value, g, C, &b), "LevelBoxData::linearSize");
+        return b;
+    }
+    void linearOut(void* buffer) const
+    {
+        int g[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; d++) { g[d] = m_ghost[d]; }
+        pm_check(pm_level_linear_out(m_layout.handle(), pm_type<T>::value, data(), g, C, buffer, 0,
+                                     protoCurrentStream()),
+                 "LevelBoxData::linearOut");
+    }
+    void linearIn(const void* buffer)
+    {
+        int g[3] = {0, 0, 0};
+        for (int d = 0; d < DIM; d++) { g[d] = m_ghost[d]; }
+        pm_check(pm_level_linear_in(m_layout.handle(), pm_type<T>::value, data(), g, C, buffer, 0,
+                                    protoCurrentStream()),
+                 "LevelBoxData::linearIn");
+    }
     void increment(const LevelBoxData& o, T scale = 1)
     {
         for (auto it : m_layout) { m_boxes[it].increment(o[it], scale); }

@@ -852,6 +852,39 @@ int pm_layout_box_elems(const pm_layout* L, const int ghost[3], size_t* elems)
         else { PM_REQUIRE(false, "bad dtype %d", (int)(dtype)); }          \
     } while (0)
 
+int pm_level_linear_size(const pm_layout* L, int dtype, const int ghost[3], int ncomp, size_t* bytes)
+{
+    PM_REQUIRE(L && bytes && ncomp >= 1 && (dtype == PM_F32 || dtype == PM_F64), "pm_level_linear_size: bad argument");
+    *bytes = (size_t)gvol(ldev(L), g3(ghost)) * ncomp * L->nboxes * (dtype == PM_F64 ? 8 : 4);
+    return PM_OK;
+}
+
+int pm_level_linear_out(const pm_layout* L, int dtype, const void* data, const int ghost[3], int ncomp, void* buffer,
+                        int to_host, pm_stream_t stream)
+{
+    PM_REQUIRE(data && buffer, "pm_level_linear_out: null argument");
+    size_t bytes = 0;
+    int rc = pm_level_linear_size(L, dtype, ghost, ncomp, &bytes);
+    if (rc) { return rc; }
+    PM_CHECK_HIP(hipMemcpyAsync(buffer, data, bytes, to_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                                S(stream)));
+    if (to_host) { PM_CHECK_HIP(hipStreamSynchronize(S(stream))); }
+    return PM_OK;
+}
+
+int pm_level_linear_in(const pm_layout* L, int dtype, void* data, const int ghost[3], int ncomp, const void* buffer,
+                       int from_host, pm_stream_t stream)
+{
+    PM_REQUIRE(data && buffer, "pm_level_linear_in: null argument");
+    size_t bytes = 0;
+    int rc = pm_level_linear_size(L, dtype, ghost, ncomp, &bytes);
+    if (rc) { return rc; }
+    PM_CHECK_HIP(hipMemcpyAsync(data, buffer, bytes, from_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                                S(stream)));
+    if (from_host) { PM_CHECK_HIP(hipStreamSynchronize(S(stream))); }
+    return PM_OK;
+}
+
 int pm_level_exchange(const pm_layout* L, int dtype, void* data, const int ghost[3], int ncomp, pm_stream_t stream)
 {
     PM_REQUIRE(L && data && ncomp >= 1, "pm_level_exchange: bad argument");

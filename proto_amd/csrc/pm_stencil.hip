@@ -182,6 +182,17 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
     T amax = 0, sum = 0, amax2 = 0, sum2 = 0;
     const int gx = x0 + lane;
     const bool xok = gx <= a.ibox.hi[0];
+    // residual modes: f of the next output plane, loaded one plane ahead
+    const long ixy = (long)box_nx(a.ibox) * box_ny(a.ibox);
+    T fpre[RY];
+#pragma unroll
+    for (int o = 0; o < RY; o++) {
+        const int gy = y0 + w * RY + o;
+        fpre[o] = (T)0;
+        if ((MODE == ZM_RESIDUAL || MODE == ZM_RESIDUAL2) && xok && gy <= a.ibox.hi[1] && zb < ze) {
+            fpre[o] = a.f[box_index(a.ibox, gx, gy, zb)];
+        }
+    }
 
     // shift the register queue, read plane zz from LDS, emit plane zz-1
     auto compute = [&](int zz, int buf) {
@@ -210,7 +221,8 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
                 put<NT>(&a.dst[box_index(a.dbox, gx, gy, z)], acc);
             } else {
                 const long ii = box_index(a.ibox, gx, gy, z);
-                const T fv = a.f[ii];
+                const T fv = fpre[o];
+                if (z + 1 < ze) { fpre[o] = a.f[ii + ixy]; }
                 const T rv = fv - acc;
                 put<NT>(&a.dst[ii], rv);
                 const T av = rv > 0 ? rv : -rv;

@@ -292,6 +292,24 @@ def config2_small():
     np.savez_compressed(os.path.join(HERE, "config2_small.npz"), **out)
 
 
+def linear():
+    """LevelBoxData::linearOut buffers written by Proto (2 components, ghosts
+    included) with the box lows in Proto's layout order."""
+    R3 = util.ref(3)
+    R3.ref_level_linear_out.restype = ctypes.c_long
+    out = {}
+    for (n, bs, g) in ((16, 8, 1), (8, 4, 2)):
+        nb = (n // bs) ** 3
+        buf = np.zeros(nb * 2 * (bs + 2 * g) ** 3)
+        lows = np.zeros(3 * nb, dtype=np.int32)
+        nbytes = R3.ref_level_linear_out(n, bs, g, ptr(buf), ptr(lows))
+        assert nbytes == buf.nbytes
+        key = "n%d_bs%d_g%d" % (n, bs, g)
+        out[key + ":buffer"] = buf
+        out[key + ":lows"] = lows.reshape(nb, 3)
+    np.savez_compressed(os.path.join(HERE, "linear.npz"), **out)
+
+
 if __name__ == "__main__":
     if util.ref(3) is None or util.ref(2) is None:
         sys.exit("build the reference checker first: make -C oracle ref")
@@ -301,6 +319,7 @@ if __name__ == "__main__":
     interp()
     exchange()
     multigrid()
+    linear()
     config1()
     config2_small()
     print("golden fixtures written to", HERE)

@@ -137,3 +137,30 @@ def test_tune_table_round_trip():
         pa.call("pm_stencil_get_tile_config", st.h, None, 0, ctypes.byref(rb), ctypes.byref(got))
         assert got.as_dict() == cfg.as_dict()
         assert len(open(path).read().strip().splitlines()) >= 1
+
+
+@pytest.mark.parametrize("n,bs,g", [(16, 8, 1), (8, 4, 2)])
+def test_level_linear_out_in(n, bs, g):
+    """LevelBoxData::linearOut / linearIn in Proto's buffer format (golden
+    buffers written by Proto), to host and device buffers."""
+    z = np.load(os.path.join(util.GOLDEN, "linear.npz"), allow_pickle=False)
+    key = "n%d_bs%d_g%d" % (n, bs, g)
+    gold, lows = z[key + ":buffer"], np.ascontiguousarray(z[key + ":lows"].astype(np.int32))
+    lay = ctypes.c_void_p()
+    pa.call("pm_layout_create", ctypes.byref(lay), 3, pa.i3([n] * 3), pa.i3([bs] * 3), pa.i3([1] * 3), len(lows),
+            lows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    gg = pa.i3([g] * 3)
+    size = ctypes.c_size_t()
+    pa.call("pm_level_linear_size", lay, pa.PM_F64, gg, 2, ctypes.byref(size))
+    assert size.value == gold.nbytes
+    data = zeros(gold.size)
+    pa.call("pm_level_linear_in", lay, pa.PM_F64, vp(data), gg, 2, gold.ctypes.data_as(ctypes.c_void_p), 1, None)
+    out = np.zeros_like(gold)
+    pa.call("pm_level_linear_out", lay, pa.PM_F64, vp(data), gg, 2, out.ctypes.data_as(ctypes.c_void_p), 1, None)
+    assert out.tobytes() == gold.tobytes()
+    dbuf = zeros(gold.size)
+    pa.call("pm_level_linear_out", lay, pa.PM_F64, vp(data), gg, 2, vp(dbuf), 0, None)
+    back = zeros(gold.size)
+    pa.call("pm_level_linear_in", lay, pa.PM_F64, vp(back), gg, 2, vp(dbuf), 0, None)
+    assert host(back).tobytes() == gold.tobytes()
+    pa.call("pm_layout_destroy", lay)

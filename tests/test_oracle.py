@@ -425,3 +425,19 @@ def test_fas_matches_proto_history():
     assert hashlib.sha256(phi.tobytes()).hexdigest() == meta["fas"]["phi_sha256"]
     # the last residual meets the tolerance and the history falls monotonically
     assert hist[it] < 1e-10 * hist[0] and np.all(np.diff(hist[:it + 1]) < 0)
+
+
+@pytest.mark.parametrize("n,bs,g", [(16, 8, 1), (8, 4, 2)])
+def test_linear_out_format_is_the_level_storage(n, bs, g):
+    """Proto's LevelBoxData::linearOut buffer (written by Proto) is box after
+    box in layout order, component after component, each over the ghosted box
+    x fastest -- the one-allocation level storage of this library."""
+    z = load("linear.npz")
+    key = "n%d_bs%d_g%d" % (n, bs, g)
+    parts = []
+    for lo in z[key + ":lows"]:
+        r = [np.arange(lo[d] - g, lo[d] + bs + g) for d in range(3)]
+        pz, py, px = np.meshgrid(r[2], r[1], r[0], indexing="ij")
+        for c in range(2):
+            parts.append((px + 1000.0 * py + 1e6 * pz + 0.5 * c).reshape(-1))
+    assert np.concatenate(parts).tobytes() == z[key + ":buffer"].tobytes()
