@@ -403,6 +403,18 @@ void ref_interp_apply(int linear, int ratio, const double* crse, const int* clo,
     if (replace) { f |= I(c); } else { f += I(c); }
 }
 
+// The same with Proto's other InterpStencils (Proto_InterpStencilDefs.H):
+// kind 2 Quadratic(ratio), kind 3 FiniteVolume(ratio, order).
+void ref_interp_apply_kind(int kind, int order, int ratio, const double* crse, const int* clo, const int* chi,
+                           double* fine, const int* flo, const int* fhi, int replace)
+{
+    InterpStencil<double> I = kind == 2 ? InterpStencil<double>::Quadratic(Point::Ones(ratio))
+                                        : InterpStencil<double>::FiniteVolume(Point::Ones(ratio), order);
+    BoxData<double, 1, HOST> c(crse, mkbox(clo, chi));
+    BoxData<double, 1, HOST> f(fine, mkbox(flo, fhi));
+    if (replace) { f |= I(c); } else { f += I(c); }
+}
+
 // LevelBoxData::exchange on a periodic n^DIM domain tiled by boxSize^DIM boxes
 // with `ghost` ghost layers (Proto_LevelBoxDataImplem.H:313-341).  Valid cells
 // come from the global array g; ghost cells start at `fill`.  Output: for each

@@ -6,6 +6,8 @@
 // output (|= or +=) scale*stage.  Every stage runs through pm_stencil_apply.
 #pragma once
 
+#include <array>
+
 #include "Proto_Stencil.H"
 
 namespace Proto {
@@ -87,6 +89,74 @@ public:
                 auto S = ((T)q) * Shift::Zeros() + ((T)p) * Shift::Basis(d, 1);
                 r.m_interp[d].set(S, s);
             }
+        return r;
+    }
+    // Quadratic (Proto_InterpStencilDefs.H:37-75): per direction and shift n,
+    // coefficients qcoef(n, r, side) at offsets -1, 0, +1
+    static double qcoef(int ni, int ri, int side)
+    {
+        const double n = ni, r = ri;
+        double c = 0;
+        if (side == -1) { c = 1.0 + 3.0 * n + 3.0 * n * n - 3.0 * r - 6.0 * n * r + 2.0 * r * r; }
+        else if (side == 0) { c = -2.0 - 6.0 * n - 6.0 * n * n + 3.0 * r + 6.0 * n * r + 5.0 * r * r; }
+        else if (side == 1) { c = -(-1.0 - 3.0 * n - 3.0 * n * n + r * r); }
+        c /= (6.0 * r * r);
+        return c;
+    }
+    static InterpStencil Quadratic(Point ratio)
+    {
+        InterpStencil r(ratio);
+        for (int d = 0; d < DIM; d++)
+            for (int s = 0; s < ratio[d]; s++) {
+                Stencil<T> S;
+                for (int ii = -1; ii <= 1; ii++) {
+                    const T coef = (T)qcoef(s, ratio[d], ii);
+                    S += coef * Shift::Basis(d, ii);
+                }
+                r.m_interp[d].set(S, s);
+            }
+        return r;
+    }
+    // FiniteVolume (Proto_InterpStencilDefs.H:77-214): 5-point tables at
+    // offsets -2 .. +2 for ratio 2 (orders 2-5) and ratio 4 (orders 2-5)
+    static InterpStencil FiniteVolume(Point ratio, int order)
+    {
+        InterpStencil r(ratio);
+        for (int d = 0; d < DIM; d++) {
+            const int rr = ratio[d];
+            std::vector<std::array<T, 5>> c;
+            if (rr == 2 && (order == 2 || order == 3)) {
+                c = {{0, (T)(1.0 / 8.0), (T)1.0, (T)(-1.0 / 8.0), 0}, {0, (T)(-1.0 / 8.0), (T)1.0, (T)(1.0 / 8.0), 0}};
+            } else if (rr == 2 && (order == 4 || order == 5)) {
+                c = {{(T)(-3.0 / 128), (T)(22.0 / 128), (T)(128.0 / 128), (T)(-22.0 / 128), (T)(3.0 / 128)},
+                     {(T)(3.0 / 128), (T)(-22.0 / 128), (T)(128.0 / 128), (T)(22.0 / 128), (T)(-3.0 / 128)}};
+            } else if (rr == 4 && order == 2) {
+                c = {{0, (T)(3.0 / 16.0), (T)1.0, (T)(-3.0 / 16.0), 0}, {0, (T)(1.0 / 16.0), (T)1.0, (T)(-1.0 / 16.0), 0},
+                     {0, (T)(-1.0 / 16.0), (T)1.0, (T)(1.0 / 16.0), 0}, {0, (T)(-3.0 / 16.0), (T)1.0, (T)(3.0 / 16.0), 0}};
+            } else if (rr == 4 && order == 3) {
+                c = {{0, (T)(7.0 / 32.0), (T)(30.0 / 32.0), (T)(-5.0 / 32.0), 0},
+                     {0, (T)(1.0 / 32.0), (T)(34.0 / 32.0), (T)(-3.0 / 32.0), 0},
+                     {0, (T)(-3.0 / 32.0), (T)(34.0 / 32.0), (T)(1.0 / 32.0), 0},
+                     {0, (T)(-5.0 / 32.0), (T)(30.0 / 32.0), (T)(7.0 / 32.0), 0}};
+            } else if (rr == 4 && order == 4) {
+                c = {{(T)(-35.0 / 1024), (T)(294.0 / 1024), (T)(960.0 / 1024), (T)(-230.0 / 1024), (T)(35.0 / 1024)},
+                     {(T)(-13.0 / 1024), (T)(58.0 / 1024), (T)(1088.0 / 1024), (T)(-122.0 / 1024), (T)(13.0 / 1024)},
+                     {(T)(13.0 / 1024), (T)(-122.0 / 1024), (T)(1088.0 / 1024), (T)(58.0 / 1024), (T)(-13.0 / 1024)},
+                     {(T)(35.0 / 1024), (T)(-230.0 / 1024), (T)(960.0 / 1024), (T)(294.0 / 1024), (T)(-35.0 / 1024)}};
+            } else if (rr == 4 && order == 5) {
+                c = {{(T)(-77.0 / 2048), (T)(616.0 / 2048), (T)(1878.0 / 2048), (T)(-432.0 / 2048), (T)(63.0 / 2048)},
+                     {(T)(-19.0 / 2048), (T)(88.0 / 2048), (T)(2218.0 / 2048), (T)(-272.0 / 2048), (T)(33.0 / 2048)},
+                     {(T)(33.0 / 2048), (T)(-272.0 / 2048), (T)(2218.0 / 2048), (T)(88.0 / 2048), (T)(-19.0 / 2048)},
+                     {(T)(63.0 / 2048), (T)(-432.0 / 2048), (T)(1878.0 / 2048), (T)(616.0 / 2048), (T)(-77.0 / 2048)}};
+            } else {
+                throw ProtoError(PM_ERR_UNSUPPORTED, "InterpStencil::FiniteVolume: ratio 2 or 4, order 2..5");
+            }
+            for (int s = 0; s < rr; s++) {
+                Stencil<T> S;
+                for (int ii = 0; ii < 5; ii++) { S += c[s][ii] * Shift::Basis(d, ii - 2); }
+                r.m_interp[d].set(S, s);
+            }
+        }
         return r;
     }
     Box range(const Box& domain) const

@@ -188,6 +188,29 @@ def interp():
     np.savez_compressed(os.path.join(HERE, "interp.npz"), **out)
 
 
+def interp_hi():
+    """Quadratic and FiniteVolume InterpStencils applied by Proto, as a binary
+    fixture the C++ API test reads: per case int32 [kind, order, ratio,
+    replace, clo[3], chi[3], flo[3], fhi[3]], then float64 crse, fine0, fine."""
+    R3 = util.ref(3)
+    cases = [(2, 0, 2), (2, 0, 4), (3, 2, 2), (3, 4, 2), (3, 5, 2), (3, 3, 4), (3, 4, 4)]
+    clo, chi = (-2, -2, -2), (5, 4, 6)
+    buf = bytearray()
+    for k, (kind, order, ratio) in enumerate(cases):
+        flo = (0, 0, 0)
+        fhi = (4 * ratio - 1, 3 * ratio - 1, 5 * ratio - 1)
+        crse = uniform(60 + k, 0, util.box_size(clo, chi))
+        fine0 = uniform(60 + k, 1, util.box_size(flo, fhi))
+        for replace in (0, 1):
+            fine = fine0.copy()
+            R3.ref_interp_apply_kind(kind, order, ratio, ptr(crse), ptr(i3(clo)), ptr(i3(chi)), ptr(fine),
+                                     ptr(i3(flo)), ptr(i3(fhi)), replace)
+            head = np.array([kind, order, ratio, replace, *clo, *chi, *flo, *fhi], dtype=np.int32)
+            buf += head.tobytes() + crse.tobytes() + fine0.tobytes() + fine.tobytes()
+    with open(os.path.join(HERE, "interp_hi.bin"), "wb") as f:
+        f.write(bytes(buf))
+
+
 def exchange():
     R3 = util.ref(3)
     out = {}
@@ -320,6 +343,7 @@ if __name__ == "__main__":
     exchange()
     multigrid()
     linear()
+    interp_hi()
     config1()
     config2_small()
     print("golden fixtures written to", HERE)

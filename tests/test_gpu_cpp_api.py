@@ -12,11 +12,14 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(util.ROOT, "tests", "cpp", "bin")
 
 
-@pytest.mark.parametrize("name", ["test_api_stencil", "test_api_level"])
+ARGS = {"test_api_interp": [os.path.join(util.GOLDEN, "interp_hi.bin")]}
+
+
+@pytest.mark.parametrize("name", ["test_api_stencil", "test_api_level", "test_api_interp"])
 def test_cpp_program(name):
     exe = os.path.join(BIN, name)
     assert os.path.exists(exe), "C++ test program not built: run __graft_entry__.build()"
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([exe] + ARGS.get(name, []), capture_output=True, text=True, timeout=300)
     print(r.stdout[-4000:])
     print(r.stderr[-2000:])
     assert r.returncode == 0, r.stdout[-2000:]
