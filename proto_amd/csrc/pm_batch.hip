@@ -310,168 +310,6 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
     queue_done(counter);
 }
 
-// The sweep with each plane row held colour by colour (arrays indexed by
-// k = (x + 2) >> 1): a colour's cells and their neighbours are read at
-// stride 1, and every per-thread index is computed once per tile.  Boxes of
-// even width; ghosts held fixed (no halo columns updated).
-template<typename T, int MAXE>
-__global__ __launch_bounds__(256) void k_batch_gsrb_cc(const BatchBox* __restrict__ boxes,
-                                                       const BatchItem* __restrict__ items, int nitems, int g,
-                                                       const T* __restrict__ phi_in, T* __restrict__ phi_out,
-                                                       const T* __restrict__ rhs, BCoefs<T> c, T lambda,
-                                                       unsigned* counter, int plm, int rplm)
-{
-    extern __shared__ __align__(16) unsigned char smraw[];
-    T* sm = reinterpret_cast<T*>(smraw);
-    T* rsm = sm + 8 * plm;
-    __shared__ int s_item;
-    const int tid = threadIdx.x;
-    for (;;) {
-        const int it = next_item(counter, &s_item);
-        if (it >= nitems) { break; }
-        const BatchItem wi = items[it];
-        const BatchBox bx = boxes[wi.box];
-        const int nx = bx.nx, ny = bx.ny, nz = bx.nz;
-        const int y0 = wi.y0, y1 = min(y0 + wi.rows, ny);
-        const int WC = (nx + 4) / 2, W = nx + 4;
-        const int NP = (y1 - y0 + 4) * W, NR = (y1 - y0 + 2) * nx;
-        const long Gx = nx + 2 * g, Gxy = Gx * (ny + 2 * g);
-        const T* p = phi_in + bx.goff;
-        T* po = phi_out + bx.goff;
-        const T* rh = rhs + bx.voff;
-        // loads (-1: outside the ghosts, reads 0), LDS slot | colour seed << 20
-        int offp[MAXE], lsp[MAXE], offr[MAXE], lsr[MAXE];
-#pragma unroll
-        for (int i = 0; i < MAXE; i++) {
-            const int e = tid + 256 * i;
-            offp[i] = offr[i] = -1;
-            lsp[i] = lsr[i] = -1;
-            if (e < NP) {
-                const int er = e / W, ec = e - er * W;
-                const int x = ec - 2, y = y0 - 2 + er;
-                if (x >= -g && x < nx + g && y >= -g && y < ny + g) { offp[i] = (x + g) + (int)Gx * (y + g); }
-                lsp[i] = (er * WC + (ec >> 1)) | (((ec + er + y0) & 1) << 20);
-            }
-            if (e < NR) {
-                const int er = e / nx, ec = e - er * nx;
-                const int y = y0 - 1 + er;
-                if (y >= 0 && y < ny) { offr[i] = ec + nx * y; }
-                lsr[i] = (er * WC + ((ec + 2) >> 1)) | (((ec + y0 - 1 + er) & 1) << 20);
-            }
-        }
-        // cells: red rows max(y0-1, 0) .. min(y1, ny-1), black rows y0 .. y1-1,
-        // x in [0, nx) (nx/2 per row and colour): slot (x parity 0) | seed << 20
-        int cr[MAXE], cb[MAXE], wl[MAXE], wo[MAXE];
-        const int hb = nx / 2;
-        const int ra = max(y0 - 1, 0), rb = min(y1 + 1, ny);
-#pragma unroll
-        for (int m = 0; m < MAXE; m++) {
-            const int q = tid + 256 * m;
-            cr[m] = cb[m] = wl[m] = -1;
-            wo[m] = 0;
-            if (q < (rb - ra) * hb) {
-                const int r = q / hb, j = q - r * hb;
-                const int y = ra + r;
-                cr[m] = ((y - (y0 - 2)) * WC + j + 1) | ((y & 1) << 20);
-            }
-            if (q < (y1 - y0) * hb) {
-                const int r = q / hb, j = q - r * hb;
-                cb[m] = ((r + 2) * WC + j + 1) | (((1 + y0 + r) & 1) << 20);
-            }
-            if (q < (y1 - y0) * nx) {
-                const int r = q / nx, x = q - r * nx;
-                wl[m] = ((r + 2) * WC + ((x + 2) >> 1)) | (((x + y0 + r) & 1) << 20);
-                wo[m] = (x + g) + (int)Gx * (y0 + r + g);
-            }
-        }
-        auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * 2 * plm; };
-        auto rslot = [&](int z) { return rsm + (long)(z & 1) * 2 * rplm; };
-        T pre[MAXE], prr[MAXE];
-        auto issue = [&](int z) {
-            const T* base = p + Gxy * (long)(z + g);
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) { pre[i] = offp[i] >= 0 ? base[offp[i]] : (T)0; }
-        };
-        auto issue_r = [&](int z) {
-            const T* base = rh + (long)nx * ny * z;
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) { prr[i] = offr[i] >= 0 ? base[offr[i]] : (T)0; }
-        };
-        auto commit = [&](int z) {
-            T* pl = slot(z);
-            const int zp = z & 1;
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                const int w = lsp[i];
-                if (w >= 0) { pl[(((w >> 20) ^ zp) & 1) * plm + (w & 0xFFFFF)] = pre[i]; }
-            }
-        };
-        auto commit_r = [&](int z) {
-            T* pl = rslot(z);
-            const int zp = z & 1;
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                const int w = lsr[i];
-                if (w >= 0) { pl[(((w >> 20) ^ zp) & 1) * rplm + (w & 0xFFFFF)] = prr[i]; }
-            }
-        };
-        auto update = [&](int z, int colour, const int* cells) {
-            T* A = slot(z) + colour * plm;
-            const T* O = slot(z) + (1 - colour) * plm;
-            const T* Zp = slot(z + 1) + (1 - colour) * plm;
-            const T* Zm = slot(z - 1) + (1 - colour) * plm;
-            const T* R = rslot(z) + colour * rplm - WC;
-#pragma unroll
-            for (int m = 0; m < MAXE; m++) {
-                const int w = cells[m];
-                if (w < 0) { continue; }
-                const int px = ((w >> 20) ^ z) & 1;
-                const int li = w & 0xFFFFF;
-                T s = 0;
-                s = s + c.c[0] * A[li];
-                s = s + c.c[1] * O[li + px];
-                s = s + c.c[2] * O[li - 1 + px];
-                s = s + c.c[3] * O[li + WC];
-                s = s + c.c[4] * O[li - WC];
-                s = s + c.c[5] * Zp[li];
-                s = s + c.c[6] * Zm[li];
-                s = s + R[li];
-                A[li] = A[li] + lambda * s;
-            }
-        };
-        // planes -1 .. nz (ghost planes included), rhs planes 0 .. nz-1
-        issue(-1);
-        commit(-1);
-        issue(0);
-        commit(0);
-        issue(1);
-        issue_r(0);
-        for (int k = -1; k < nz; k++) {
-            if (k + 2 <= nz) { commit(k + 2); }   // slot of plane k-2: last read before a barrier
-            if (k + 1 < nz) { commit_r(k + 1); }  // slot of rhs plane k-1, likewise
-            if (k + 3 <= nz) { issue(k + 3); }
-            if (k + 2 < nz) { issue_r(k + 2); }
-            __syncthreads();
-            if (k + 1 < nz) { update(k + 1, 0, cr); }
-            __syncthreads();
-            if (k >= 0) {
-                update(k, 1, cb);
-                __syncthreads();
-                const T* pl = slot(k);
-                T* op = po + Gxy * (long)(k + g);
-                const int zp = k & 1;
-#pragma unroll
-                for (int m = 0; m < MAXE; m++) {
-                    const int w = wl[m];
-                    if (w >= 0) { op[wo[m]] = pl[(((w >> 20) ^ zp) & 1) * plm + (w & 0xFFFFF)]; }
-                }
-            }
-        }
-        __syncthreads();  // the next item's loads reuse every slot
-    }
-    queue_done(counter);
-}
-
 }  // namespace pm
 
 using namespace pm;
@@ -712,15 +550,6 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
     PM_REQUIRE(lds + 64 <= 64 * 1024, "pm_batch_gsrb: tile too large for LDS (rows %d, width %d)", B->gsrb_rows, maxnx);
     const unsigned nb = persistent_blocks(B->cfg.gsrb_blocks_per_cu);
     const long np = (long)(B->gsrb_rows + 4) * (maxnx + 2);
-    // the colour-by-colour kernel when every box has an even width and its
-    // (slightly larger) LDS fits
-    bool even = true;
-    for (int b = 0; b < B->nboxes; b++) { even = even && B->sizes[3 * b] % 2 == 0; }
-    const int wcm = (maxnx + 4) / 2;
-    const int plm = (B->gsrb_rows + 4) * wcm, rplm = (B->gsrb_rows + 2) * wcm;
-    const size_t lds_cc = (size_t)(8 * plm + 4 * rplm) * esz;
-    const bool use_cc = even && lds_cc + 64 <= 64 * 1024;
-    const long np_cc = (long)(B->gsrb_rows + 4) * (maxnx + 4);
     // LaplaceOp::init: coefficients Laplacian()_j * (-1/(dx*dx)), lambda = 2 * (-(dx*dx)/(4 DIM))
     double lam = -(dx * dx) / (4.0 * 3);
     lam *= 2.0;
@@ -732,16 +561,7 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
                                (const BatchItem*)B->d_gitems, B->nitems_gsrb, B->ghost, (const T*)phi_in,
                                (T*)phi_out, (const T*)rhs, coef, lamT, B->d_counter + 2, phi_elems);
         };
-        auto go_cc = [&](auto kern) {
-            hipLaunchKernelGGL(kern, dim3(nb), dim3(256), lds_cc, S(stream), (const BatchBox*)B->d_boxes,
-                               (const BatchItem*)B->d_gitems, B->nitems_gsrb, B->ghost, (const T*)phi_in,
-                               (T*)phi_out, (const T*)rhs, coef, lamT, B->d_counter + 2, plm, rplm);
-        };
-        if (use_cc) {
-            if (np_cc <= 1024) { go_cc(k_batch_gsrb_cc<T, 4>); }
-            else if (np_cc <= 2048) { go_cc(k_batch_gsrb_cc<T, 8>); }
-            else { go_cc(k_batch_gsrb_cc<T, 16>); }
-        } else if (np <= 1024) { go(k_batch_gsrb<T, 4>); }
+        if (np <= 1024) { go(k_batch_gsrb<T, 4>); }
         else if (np <= 2048) { go(k_batch_gsrb<T, 8>); }
         else { go(k_batch_gsrb<T, 16>); }
     };

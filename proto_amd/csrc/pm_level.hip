@@ -281,209 +281,6 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     }
 }
 
-// The same sweep with the planes held colour by colour: row y of plane z
-// keeps its two colours in separate arrays indexed by k = (x + 2) >> 1, so
-// the cells of one colour in a row (all of one x parity) and each of their
-// neighbours are read at stride 1 -- no bank conflicts and no per-cell
-// parity tests.  Every per-thread index (loads, stores, cells) is fixed for
-// the whole z walk and computed once.  Even box widths; 32-bit offsets.
-template<typename T, int MAXE>
-__global__ __launch_bounds__(256) void k_gsrb_cc(LayoutDev L, G3 gin, G3 gout, G3 gr, const T* __restrict__ pin,
-                                                 T* __restrict__ pout, const T* __restrict__ rhs, Lap<T> c, T lambda,
-                                                 int rows, int zc)
-{
-    extern __shared__ __align__(16) unsigned char smraw[];
-    __shared__ int s_nb[27];
-    const int b = blockIdx.y;
-    const int* lo = L.lows + 3 * b;
-    const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
-    const int nyt = (ny + rows - 1) / rows;
-    const int y0 = (blockIdx.x % nyt) * rows, y1 = min(y0 + rows, ny);
-    const int z0 = (blockIdx.x / nyt) * zc, z1 = min(z0 + zc, nz);
-    const int WC = (nx + 4) / 2;          // slots per colour row
-    const int PLM = (rows + 4) * WC;      // one colour of one phi plane (rows y0-2 .. y1+1)
-    const int RPLM = (rows + 2) * WC;     // one colour of one rhs plane (rows y0-1 .. y1)
-    T* sm = reinterpret_cast<T*>(smraw);
-    T* rsm = sm + 8 * PLM;
-    const int losum = lo[0] + lo[1] + lo[2];
-    const int W = nx + 4, NP = (y1 - y0 + 4) * W;
-    const int WR = nx + 2, NR = (y1 - y0 + 2) * WR;
-    const long vin = gvol(L, gin), vr = gvol(L, gr);
-    if (threadIdx.x < 27) {
-        const int t = threadIdx.x;
-        int p[3] = {lo[0] + (t % 3 - 1) * nx, lo[1] + ((t / 3) % 3 - 1) * ny, lo[2] + (t / 9 - 1) * nz};
-        s_nb[t] = owner(L, p);
-    }
-    __syncthreads();
-    auto at = [&](const G3& g, long vol, int x, int y, int z) {
-        const int ox = x < 0 ? -1 : (x >= nx ? 1 : 0);
-        const int oy = y < 0 ? -1 : (y >= ny ? 1 : 0);
-        const int oz = z < 0 ? -1 : (z >= nz ? 1 : 0);
-        const int o = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
-        return (long)o * vol + rel_index(L, g, x - ox * nx, y - oy * ny, z - oz * nz);
-    };
-    // loads: global offset for planes inside the box (plane z adds z * XY),
-    // LDS slot (bits 0-19) and colour at z = 0 (bit 20), -1 = none
-    int offp[MAXE], lsp[MAXE], offr[MAXE], lsr[MAXE];
-    const long XYp = (long)(nx + 2 * gin.g[0]) * (ny + 2 * gin.g[1]);
-    const long XYr = (long)(nx + 2 * gr.g[0]) * (ny + 2 * gr.g[1]);
-#pragma unroll
-    for (int i = 0; i < MAXE; i++) {
-        const int e = threadIdx.x + 256 * i;
-        offp[i] = offr[i] = 0;
-        lsp[i] = lsr[i] = -1;
-        if (e < NP) {
-            const int er = e / W, ec = e - er * W;
-            offp[i] = (int)at(gin, vin, ec - 2, y0 - 2 + er, 0);
-            lsp[i] = (er * WC + (ec >> 1)) | (((ec + er + y0 + losum) & 1) << 20);
-        }
-        if (e < NR) {
-            const int er = e / WR, ec = e - er * WR;
-            offr[i] = (int)at(gr, vr, ec - 1, y0 - 1 + er, 0);
-            lsr[i] = (er * WC + ((ec + 1) >> 1)) | (((ec + er + y0 + losum) & 1) << 20);
-        }
-    }
-    // cells: red (rows y0-1 .. y1, x in [-1, nx]: nx/2 + 1 per row) and black
-    // (rows y0 .. y1-1, x in [0, nx): nx/2 per row): LDS index of the slot for
-    // x parity 0 (bits 0-19; red subtracts the parity) and the row's parity
-    // seed (bit 20), -1 = none; write-out: LDS slot and colour seed, output offset
-    int cr[MAXE], cb[MAXE], wl[MAXE], wo[MAXE];
-    const int hr = nx / 2 + 1, hb = nx / 2;
-#pragma unroll
-    for (int m = 0; m < MAXE; m++) {
-        const int q = threadIdx.x + 256 * m;
-        cr[m] = cb[m] = wl[m] = -1;
-        wo[m] = 0;
-        if (q < (y1 - y0 + 2) * hr) {
-            const int r = q / hr, j = q - r * hr;
-            cr[m] = ((r + 1) * WC + j + 1) | (((y0 - 1 + r + losum) & 1) << 20);
-        }
-        if (q < (y1 - y0) * hb) {
-            const int r = q / hb, j = q - r * hb;
-            cb[m] = ((r + 2) * WC + j + 1) | (((1 + y0 + r + losum) & 1) << 20);
-        }
-        if (q < (y1 - y0) * nx) {
-            const int r = q / nx, x = q - r * nx;
-            wl[m] = ((r + 2) * WC + ((x + 2) >> 1)) | (((x + y0 + r + losum) & 1) << 20);
-            wo[m] = (int)rel_index(L, gout, x, y0 + r, 0);
-        }
-    }
-    const long XYo = (long)(nx + 2 * gout.g[0]) * (ny + 2 * gout.g[1]);
-    T* ob = pout + (long)b * gvol(L, gout);
-    auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * 2 * PLM; };
-    auto rslot = [&](int z) { return rsm + (long)(z & 1) * 2 * RPLM; };
-    T pre[MAXE], prr[MAXE];
-    auto issue = [&](int z) {
-        if (z >= 0 && z < nz) {
-            const T* base = pin + XYp * z;
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                if (lsp[i] >= 0) { pre[i] = base[offp[i]]; }
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                const int e = threadIdx.x + 256 * i;
-                if (e < NP) {
-                    const int er = e / W, ec = e - er * W;
-                    pre[i] = pin[at(gin, vin, ec - 2, y0 - 2 + er, z)];
-                }
-            }
-        }
-    };
-    auto issue_r = [&](int z) {
-        if (z >= 0 && z < nz) {
-            const T* base = rhs + XYr * z;
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                if (lsr[i] >= 0) { prr[i] = base[offr[i]]; }
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                const int e = threadIdx.x + 256 * i;
-                if (e < NR) {
-                    const int er = e / WR, ec = e - er * WR;
-                    prr[i] = rhs[at(gr, vr, ec - 1, y0 - 1 + er, z)];
-                }
-            }
-        }
-    };
-    auto commit = [&](int z) {
-        T* pl = slot(z);
-        const int zp = z & 1;
-#pragma unroll
-        for (int i = 0; i < MAXE; i++) {
-            const int w = lsp[i];
-            if (w >= 0) { pl[(((w >> 20) ^ zp) & 1) * PLM + (w & 0xFFFFF)] = pre[i]; }
-        }
-    };
-    auto commit_r = [&](int z) {
-        T* pl = rslot(z);
-        const int zp = z & 1;
-#pragma unroll
-        for (int i = 0; i < MAXE; i++) {
-            const int w = lsr[i];
-            if (w >= 0) { pl[(((w >> 20) ^ zp) & 1) * RPLM + (w & 0xFFFFF)] = prr[i]; }
-        }
-    };
-    // colour cells of plane z; the x parity of a row's colour cells is (seed ^ z) & 1
-    auto update = [&](int z, int colour, const int* cells) {
-        T* A = slot(z) + colour * PLM;
-        const T* O = slot(z) + (1 - colour) * PLM;
-        const T* Zp = slot(z + 1) + (1 - colour) * PLM;
-        const T* Zm = slot(z - 1) + (1 - colour) * PLM;
-        const T* R = rslot(z) + colour * RPLM - WC;
-#pragma unroll
-        for (int m = 0; m < MAXE; m++) {
-            const int w = cells[m];
-            if (w < 0) { continue; }
-            const int px = ((w >> 20) ^ z) & 1;
-            const int li = (w & 0xFFFFF) - (colour == 0 ? px : 0);
-            T s = 0;
-            s = s + c.c[0] * A[li];
-            s = s + c.c[1] * O[li + px];
-            s = s + c.c[2] * O[li - 1 + px];
-            s = s + c.c[3] * O[li + WC];
-            s = s + c.c[4] * O[li - WC];
-            s = s + c.c[5] * Zp[li];
-            s = s + c.c[6] * Zm[li];
-            s = s + R[li];
-            A[li] = A[li] + lambda * s;
-        }
-    };
-    issue(z0 - 2);
-    commit(z0 - 2);
-    issue(z0 - 1);
-    commit(z0 - 1);
-    issue(z0);
-    issue_r(z0 - 1);
-    for (int k = z0 - 2; k < z1; k++) {
-        // slot(k+2) last held plane k-2 (last read by black(k-1), before a
-        // barrier) and rslot(k+1) rhs plane k-1 (likewise); the write-out of
-        // plane k-1 reads slot(k-1) only
-        commit(k + 2);
-        commit_r(k + 1);
-        if (k + 3 <= z1 + 1) { issue(k + 3); }
-        if (k + 2 <= z1) { issue_r(k + 2); }
-        __syncthreads();
-        update(k + 1, 0, cr);
-        __syncthreads();
-        if (k >= z0) {
-            update(k, 1, cb);
-            __syncthreads();
-            const T* pl = slot(k);
-            T* op = ob + XYo * k;
-            const int zp = k & 1;
-#pragma unroll
-            for (int m = 0; m < MAXE; m++) {
-                const int w = wl[m];
-                if (w >= 0) { op[wo[m]] = pl[(((w >> 20) ^ zp) & 1) * PLM + (w & 0xFFFFF)]; }
-            }
-        }
-    }
-}
-
 // ------------------------------------------------------------ Jacobi
 template<typename T, int DIMN>
 __global__ __launch_bounds__(256) void k_jacobi_tmp(LayoutDev L, G3 gp, G3 gr, const T* __restrict__ phi,
@@ -949,7 +746,7 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
     }
     const size_t esz = dtype == PM_F64 ? 8 : 4;
     auto lds_of = [&](int r) {
-        return (4 * (size_t)(r + 4) * (L->bs[0] + 4) + 2 * (size_t)(r + 2) * (L->bs[0] + 4)) * esz;
+        return (4 * (size_t)(r + 4) * (L->bs[0] + 4) + 2 * (size_t)(r + 2) * (L->bs[0] + 2)) * esz;
     };
     int rows = esz == 4 ? 16 : 8;
     while (rows > 1 && lds_of(rows) > 64 * 1024 - 256) { rows /= 2; }
@@ -974,21 +771,7 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
             for (int j = 0; j < 7; j++) { c.c[j] = (T)((j == 0 ? -2.0 * L->dim : 1.0) * m); }
         }
         const long np = (long)(rows + 4) * (L->bs[0] + 4);
-        const long e_in = gvol(D, gi) * L->nboxes;
-        const long e_out = gvol(D, go) * L->nboxes;
-        const long e_r = gvol(D, gr) * L->nboxes;
-        const bool cc = L->bs[0] % 2 == 0 && e_in < (1L << 31) && e_out < (1L << 31) && e_r < (1L << 31);
-        const size_t lds_cc = (size_t)(8 * (rows + 4) + 4 * (rows + 2)) * ((L->bs[0] + 4) / 2) * sizeof(T);
-        if (cc && np <= 1024) {
-            hipLaunchKernelGGL((k_gsrb_cc<T, 4>), grid, dim3(256), lds_cc, S(stream), D, gi, go, gr,
-                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
-        } else if (cc && np <= 2048) {
-            hipLaunchKernelGGL((k_gsrb_cc<T, 8>), grid, dim3(256), lds_cc, S(stream), D, gi, go, gr,
-                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
-        } else if (cc) {
-            hipLaunchKernelGGL((k_gsrb_cc<T, 16>), grid, dim3(256), lds_cc, S(stream), D, gi, go, gr,
-                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
-        } else if (np <= 1024) {
+        if (np <= 1024) {
             hipLaunchKernelGGL((k_gsrb_fused<T, 4>), grid, dim3(256), lds, S(stream), D, gi, go, gr,
                                (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
         } else if (np <= 2048) {

@@ -49,7 +49,7 @@ __host__ __device__ inline int sweep_rows(int n) { return n > 128 ? 4 : SR; }
 inline size_t sweep_lds(int n)
 {
     const int r = sweep_rows(n);
-    return sizeof(float) * (8 * (size_t)(r + 4) + 4 * (size_t)(r + 2)) * ((n + 4) / 2);
+    return sizeof(float) * (4 * (size_t)(r + 4) * (n + 4) + 2 * (size_t)(r + 2) * (n + 2));
 }
 
 struct SReq {
@@ -77,125 +77,92 @@ __device__ inline int wrapi(int v, int n)
 // of level L, reading `in` and writing `out` (dir 0: phi -> tmp, 1: tmp -> phi).
 // phi planes (rows [y0-2, y1+2) x columns [-2, n+2)) pass through a ring of 4
 // LDS planes and rhs planes (rows [y0-1, y1+1) x columns [-1, n+1)) through a
-// ring of 2, both loaded into registers one step ahead of their use.  Each
-// plane row keeps its two colours in separate arrays indexed by
-// k = (x + 2) >> 1, so a colour's cells and their neighbours are read at
-// stride 1; every per-thread index is computed once per tile.
+// ring of 2, both loaded into registers one step ahead of their use.
 template<int MAXE>
 __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv, const SItem* __restrict__ items, int dir)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
     const SItem it = items[blockIdx.x];
     const SLevel L = lv[it.lev];
-    const int n = L.n, rows = L.rows;
-    const int y0 = it.a, y1 = min(y0 + rows, n), z0 = it.b, z1 = it.c;
-    const int WC = (n + 4) / 2, PLM = (rows + 4) * WC, RPLM = (rows + 2) * WC;
-    const int W = n + 4, NP = (y1 - y0 + 4) * W, WR = n + 2, NR = (y1 - y0 + 2) * WR;
+    const int n = L.n, W = n + 4, WR = n + 2;
+    const int y0 = it.a, y1 = min(y0 + L.rows, n), z0 = it.b, z1 = it.c;
+    const int NP = (y1 - y0 + 4) * W, NR = (y1 - y0 + 2) * WR;
     float* sm = reinterpret_cast<float*>(smraw);
-    float* rsm = sm + 8 * PLM;
+    float* rsm = sm + 4 * (L.rows + 4) * W;
     const float* in = dir ? L.tmp : L.phi;
     float* out = dir ? L.phi : L.tmp;
     const long nn = (long)n * n;
-    // loads: in-plane offset (periodic images folded in), LDS slot (bits 0-19)
-    // and colour at z = 0 (bit 20); -1 = none
-    int offp[MAXE], lsp[MAXE], offr[MAXE], lsr[MAXE];
+    auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
+    auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
+    // in-plane offsets (periodic images folded in) of this thread's elements
+    int offp[MAXE], offr[MAXE];
 #pragma unroll
     for (int i = 0; i < MAXE; i++) {
         const int e = threadIdx.x + 256 * i;
         offp[i] = offr[i] = 0;
-        lsp[i] = lsr[i] = -1;
         if (e < NP) {
             const int er = e / W, ec = e - er * W;
             offp[i] = wrapi(ec - 2, n) + n * wrapi(y0 - 2 + er, n);
-            lsp[i] = (er * WC + (ec >> 1)) | (((ec + er + y0) & 1) << 20);
         }
         if (e < NR) {
             const int er = e / WR, ec = e - er * WR;
             offr[i] = wrapi(ec - 1, n) + n * wrapi(y0 - 1 + er, n);
-            lsr[i] = (er * WC + ((ec + 1) >> 1)) | (((ec + er + y0) & 1) << 20);
         }
     }
-    // cells: red (rows y0-1 .. y1, x in [-1, n]) and black (rows y0 .. y1-1,
-    // x in [0, n)): slot for x parity 0 (red subtracts the parity) and the
-    // row's parity seed; write-out: slot, colour seed and in-plane offset
-    int cr[MAXE], cb[MAXE], wl[MAXE], wo[MAXE];
-    const int hr = n / 2 + 1, hb = n / 2;
-#pragma unroll
-    for (int m = 0; m < MAXE; m++) {
-        const int q = threadIdx.x + 256 * m;
-        cr[m] = cb[m] = wl[m] = -1;
-        wo[m] = 0;
-        if (q < (y1 - y0 + 2) * hr) {
-            const int r = q / hr, j = q - r * hr;
-            cr[m] = ((r + 1) * WC + j + 1) | (((y0 - 1 + r) & 1) << 20);
-        }
-        if (q < (y1 - y0) * hb) {
-            const int r = q / hb, j = q - r * hb;
-            cb[m] = ((r + 2) * WC + j + 1) | (((1 + y0 + r) & 1) << 20);
-        }
-        if (q < (y1 - y0) * n) {
-            const int r = q / n, x = q - r * n;
-            wl[m] = ((r + 2) * WC + ((x + 2) >> 1)) | (((x + y0 + r) & 1) << 20);
-            wo[m] = x + n * (y0 + r);
-        }
-    }
-    auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * 2 * PLM; };
-    auto rslot = [&](int z) { return rsm + (long)(z & 1) * 2 * RPLM; };
     float pre[MAXE], prr[MAXE];
     auto issue = [&](int z) {
         const float* base = in + nn * wrapi(z, n);
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
-            if (lsp[i] >= 0) { pre[i] = base[offp[i]]; }
+            if (threadIdx.x + 256 * i < NP) { pre[i] = base[offp[i]]; }
+        }
+    };
+    auto commit = [&](int z) {
+        float* pl = slot(z);
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < NP) { pl[e] = pre[i]; }
         }
     };
     auto issue_r = [&](int z) {
         const float* base = L.rhs + nn * wrapi(z, n);
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
-            if (lsr[i] >= 0) { prr[i] = base[offr[i]]; }
-        }
-    };
-    auto commit = [&](int z) {
-        float* pl = slot(z);
-        const int zp = z & 1;
-#pragma unroll
-        for (int i = 0; i < MAXE; i++) {
-            const int w = lsp[i];
-            if (w >= 0) { pl[(((w >> 20) ^ zp) & 1) * PLM + (w & 0xFFFFF)] = pre[i]; }
+            if (threadIdx.x + 256 * i < NR) { prr[i] = base[offr[i]]; }
         }
     };
     auto commit_r = [&](int z) {
         float* pl = rslot(z);
-        const int zp = z & 1;
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
-            const int w = lsr[i];
-            if (w >= 0) { pl[(((w >> 20) ^ zp) & 1) * RPLM + (w & 0xFFFFF)] = prr[i]; }
+            const int e = threadIdx.x + 256 * i;
+            if (e < NR) { pl[e] = prr[i]; }
         }
     };
-    auto update = [&](int z, int colour, const int* cells) {
-        float* A = slot(z) + colour * PLM;
-        const float* O = slot(z) + (1 - colour) * PLM;
-        const float* Zp = slot(z + 1) + (1 - colour) * PLM;
-        const float* Zm = slot(z - 1) + (1 - colour) * PLM;
-        const float* R = rslot(z) + colour * RPLM - WC;
-#pragma unroll
-        for (int m = 0; m < MAXE; m++) {
-            const int w = cells[m];
-            if (w < 0) { continue; }
-            const int px = ((w >> 20) ^ z) & 1;
-            const int li = (w & 0xFFFFF) - (colour == 0 ? px : 0);
+    // colour cells ((x+y+z+colour) even) of plane z, rows [ya, yb), columns [xa, xb)
+    auto update = [&](int z, int ya, int yb, int xa, int xb, int colour) {
+        float* zc = slot(z);
+        const float* zm = slot(z - 1);
+        const float* zp = slot(z + 1);
+        const float* rz = rslot(z);
+        const int hw = (xb - xa + 1) / 2;
+        const int ncell = (yb - ya) * hw;
+        for (int c = threadIdx.x; c < ncell; c += 256) {
+            const int yy = ya + c / hw;
+            const int xx = xa + 2 * (c % hw) + ((xa + yy + z + colour) & 1);
+            if (xx >= xb) { continue; }
+            const int li = (yy - (y0 - 2)) * W + (xx + 2);
             float s = 0;
-            s = s + L.c0 * A[li];
-            s = s + L.c1 * O[li + px];
-            s = s + L.c1 * O[li - 1 + px];
-            s = s + L.c1 * O[li + WC];
-            s = s + L.c1 * O[li - WC];
-            s = s + L.c1 * Zp[li];
-            s = s + L.c1 * Zm[li];
-            s = s + R[li];
-            A[li] = A[li] + L.lam * s;
+            s = s + L.c0 * zc[li];
+            s = s + L.c1 * zc[li + 1];
+            s = s + L.c1 * zc[li - 1];
+            s = s + L.c1 * zc[li + W];
+            s = s + L.c1 * zc[li - W];
+            s = s + L.c1 * zp[li];
+            s = s + L.c1 * zm[li];
+            s = s + rz[(yy - (y0 - 1)) * WR + (xx + 1)];
+            zc[li] = zc[li] + L.lam * s;
         }
     };
     issue(z0 - 2);
@@ -205,28 +172,25 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     issue(z0);
     issue_r(z0 - 1);
     for (int k = z0 - 2; k < z1; k++) {
-        // slot(k+2) last held plane k-2 (last read by black(k-1), before a
-        // barrier) and rslot(k+1) rhs plane k-1 (likewise); the write-out of
-        // plane k-1 reads slot(k-1) only
-        commit(k + 2);
-        commit_r(k + 1);
+        commit(k + 2);    // into the slot of plane k-2, last read before the previous barrier
+        commit_r(k + 1);  // into the slot of rhs plane k-1, likewise
         if (k + 3 <= z1 + 1) { issue(k + 3); }
         if (k + 2 <= z1) { issue_r(k + 2); }
         __syncthreads();
-        update(k + 1, 0, cr);
+        update(k + 1, y0 - 1, y1 + 1, -1, n + 1, 0);  // red, one-cell halo included
         __syncthreads();
         if (k >= z0) {
-            update(k, 1, cb);
+            update(k, y0, y1, 0, n, 1);  // black, tile cells
             __syncthreads();
             const float* pl = slot(k);
-            float* op = out + nn * k;
-            const int zp = k & 1;
-#pragma unroll
-            for (int m = 0; m < MAXE; m++) {
-                const int w = wl[m];
-                if (w >= 0) { op[wo[m]] = pl[(((w >> 20) ^ zp) & 1) * PLM + (w & 0xFFFFF)]; }
+            float* ob = out + nn * k;
+            const int cnt = (y1 - y0) * n;
+            for (int e = threadIdx.x; e < cnt; e += 256) {
+                const int r = e / n, xx = e - r * n;
+                ob[xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
             }
         }
+        __syncthreads();
     }
 }
 
