@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "pm_internal.hpp"
@@ -97,15 +98,6 @@ __global__ __launch_bounds__(256) void k_stencil_generic(GenArgs<T> a)
 }
 
 // ------------------------------------------------------------ z-march kernels
-// Block = 4 wave64s over a 64 x (4 RY) tile of one (x, y) column range; the
-// block walks its z range plane by plane.  Each plane (with a 1-cell halo in
-// x and y) is staged once from HBM into a double-buffered LDS tile; each
-// thread keeps the 3 x 3 neighbourhoods of its RY rows for planes z-1, z,
-// z+1 in registers (register queue along z), so one LDS pass per plane feeds
-// all 27 taps.  Global loads of the next PF planes are in flight while a
-// plane is computed.  NT = 1 writes results with non-temporal stores (they
-// are streamed out, never re-read by this kernel).  Tile rows (RY), prefetch
-// depth (PF), NT and the z extent per block are the autotuner's knobs.
 enum { ZM_REPLACE = 0, ZM_ACCUM = 1, ZM_RESIDUAL = 2, ZM_RESIDUAL2 = 3 };
 
 template<typename T>
@@ -127,27 +119,146 @@ __device__ inline void put(T* p, T v)
     if (NT) { __builtin_nontemporal_store(v, p); } else { *p = v; }
 }
 
+template<int RY>
+struct ZGeom {
+    static constexpr int TX = 64, TY = 4 * RY, LX = TX + 2, LY = TY + 2;
+    static constexpr int NE = LX * LY, NL = (NE + 255) / 256, NP = NL * 256;  // NP: plane padded to whole passes
+};
+
+// Interior tiles (tile, halo and outputs all inside their boxes).  The loop
+// holds no divergent branch, so the compiler can count outstanding memory
+// operations exactly and each wait covers only the load it needs: loads of
+// plane zz + PF stay in flight while plane zz - 1 is computed.  Three LDS
+// planes, three register planes and three staging sets rotate with the
+// 3-way unrolled loop, so no register is copied from plane to plane.  Loads
+// past the end of the z range are clamped to plane ze (in the source box)
+// and land in sets that are never stored.
 template<typename T, int SHAPE, int MODE, int RY, int PF, int NT>
-__global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
+__device__ __forceinline__ void zm_interior(const ZArgs<T>& a, T (*sm)[ZGeom<RY>::NP], int x0, int y0, int zb, int ze,
+                                            T& amax, T& sum, T& amax2, T& sum2)
 {
-    constexpr int TX = 64, TY = 4 * RY, LX = TX + 2, LY = TY + 2;
-    constexpr int NE = LX * LY, NL = (NE + 255) / 256;
+    using G = ZGeom<RY>;
+    constexpr int NL = G::NL, LX = G::LX;
     constexpr int NC = (SHAPE == 7) ? 7 : 27;
-    __shared__ T sm[2][LY][LX];
-    __shared__ T red[16];
-
+    constexpr bool RES = (MODE == ZM_RESIDUAL || MODE == ZM_RESIDUAL2);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    unsigned b = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx = b % a.ntx;
-    b /= a.ntx;
-    const int ty = b % a.nty;
-    const int tzi = b / a.nty;
-    const int x0 = a.ibox.lo[0] + tx * TX, y0 = a.ibox.lo[1] + ty * TY;
-    const int zb = a.ibox.lo[2] + tzi * a.tz;
-    const int ze = min(zb + a.tz, a.ibox.hi[2] + 1);  // outputs z in [zb, ze)
-
     const long sx = box_nx(a.sbox), sxy = sx * box_ny(a.sbox);
-    // per-thread plane element offsets (constant along z)
+    long eoff[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = min(tid + 256 * k, G::NE - 1);
+        const int row = e / LX, col = e - (e / LX) * LX;
+        eoff[k] = (long)(x0 - 1 + col - a.sbox.lo[0]) + sx * (long)(y0 - 1 + row - a.sbox.lo[1]);
+    }
+    T P[3][NL];
+    auto gload = [&](T (&p)[NL], int z) {
+        const T* base = a.src + sxy * (long)(z - a.sbox.lo[2]);
+#pragma unroll
+        for (int k = 0; k < NL; k++) { p[k] = base[eoff[k]]; }
+    };
+    auto lstore = [&](const T (&p)[NL], T* s) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) { s[tid + 256 * k] = p[k]; }
+    };
+    T q[3][RY + 2][3];
+    auto lread = [&](T (&qq)[RY + 2][3], const T* s) {
+#pragma unroll
+        for (int r = 0; r < RY + 2; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) { qq[r][c] = s[(w * RY + r) * LX + lane + c]; }
+    };
+    // outputs (and f, or the old dst when accumulating) of rows y0 + w RY + o
+    const Box3& ob = RES ? a.ibox : a.dbox;
+    const long onx = box_nx(ob), oxy = onx * box_ny(ob);
+    const long o0 = box_index(ob, x0 + lane, y0 + w * RY, zb);
+    const T* pv = (MODE == ZM_ACCUM) ? a.dst : a.f;
+    T F[3][RY];
+    auto fload = [&](T (&f)[RY], int z) {
+        if (MODE == ZM_REPLACE) { return; }
+#pragma unroll
+        for (int o = 0; o < RY; o++) { f[o] = pv[o0 + o * onx + oxy * (long)(z - zb)]; }
+    };
+
+    // one z step: stage plane zz (slot S), then emit plane zz - 1 from the
+    // register planes in slots S+1 (zz - 2), S+2 (zz - 1) and S (zz)
+    auto step = [&](auto sc, int zz) {
+        constexpr int S = decltype(sc)::value, SB = (S + 1) % 3, SC = (S + 2) % 3;
+        lstore(P[S], sm[S]);
+        const int z = zz - 1;
+        fload(F[(S + 1) % 3], min(z + 1, ze - 1));
+        gload(P[(S + PF) % 3], min(zz + PF, ze));
+        __syncthreads();
+        lread(q[S], sm[S]);
+#pragma unroll
+        for (int o = 0; o < RY; o++) {
+            auto tap = [&](int dx, int dy, int dz) -> T {
+                return dz < 0 ? q[SB][o + 1 + dy][1 + dx] : (dz == 0 ? q[SC][o + 1 + dy][1 + dx] : q[S][o + 1 + dy][1 + dx]);
+            };
+            const long oi = o0 + o * onx + oxy * (long)(z - zb);
+            T acc = (MODE == ZM_ACCUM) ? F[S][o] : (T)0;
+#pragma unroll
+            for (int j = 0; j < NC; j++) {
+                acc = acc + a.c[j] * tap(shape_off<SHAPE>(j, 0), shape_off<SHAPE>(j, 1), shape_off<SHAPE>(j, 2));
+            }
+            if (!RES) {
+                put<NT>(&a.dst[oi], acc);
+            } else {
+                const T fv = F[S][o];
+                const T rv = fv - acc;
+                put<NT>(&a.dst[oi], rv);
+                const T av = rv > 0 ? rv : -rv;
+                amax = amax < av ? av : amax;
+                sum = sum + rv;
+                if (MODE == ZM_RESIDUAL2) {
+                    T acc2 = 0;
+#pragma unroll
+                    for (int j = 0; j < 27; j++) {
+                        acc2 = acc2 + a.c2[j] * tap(lap27_off(j, 0), lap27_off(j, 1), lap27_off(j, 2));
+                    }
+                    const T rv2 = fv - acc2;
+                    put<NT>(&a.dst2[oi], rv2);
+                    const T av2 = rv2 > 0 ? rv2 : -rv2;
+                    amax2 = amax2 < av2 ? av2 : amax2;
+                    sum2 = sum2 + rv2;
+                }
+            }
+        }
+    };
+
+    gload(P[0], zb - 1);
+    if (PF == 2) { gload(P[1], zb); }
+    fload(F[2], zb);
+    // planes zb - 1 and zb: staged, nothing to emit yet
+    lstore(P[0], sm[0]);
+    gload(P[PF % 3], zb - 1 + PF);
+    __syncthreads();
+    lread(q[0], sm[0]);
+    lstore(P[1], sm[1]);
+    gload(P[(1 + PF) % 3], min(zb + PF, ze));
+    __syncthreads();
+    lread(q[1], sm[1]);
+    int zz = zb + 1;
+    for (;;) {
+        step(std::integral_constant<int, 2>{}, zz);
+        if (++zz > ze) { break; }
+        step(std::integral_constant<int, 0>{}, zz);
+        if (++zz > ze) { break; }
+        step(std::integral_constant<int, 1>{}, zz);
+        if (++zz > ze) { break; }
+    }
+}
+
+// Tiles that touch a box edge: every load and store is bounds-checked (zero
+// outside the source box), two LDS planes, register planes shifted per step.
+template<typename T, int SHAPE, int MODE, int RY, int PF, int NT>
+__device__ __forceinline__ void zm_edge(const ZArgs<T>& a, T (*sm)[ZGeom<RY>::NP], int x0, int y0, int zb, int ze,
+                                        T& amax, T& sum, T& amax2, T& sum2)
+{
+    using G = ZGeom<RY>;
+    constexpr int NL = G::NL, LX = G::LX, NE = G::NE;
+    constexpr int NC = (SHAPE == 7) ? 7 : 27;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long sx = box_nx(a.sbox), sxy = sx * box_ny(a.sbox);
     long eoff[NL];
     bool eok[NL];
 #pragma unroll
@@ -165,10 +276,7 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
     };
     auto store = [&](const T (&pre)[NL], int buf) {
 #pragma unroll
-        for (int k = 0; k < NL; k++) {
-            const int e = tid + 256 * k;
-            if (e < NE) { (&sm[buf][0][0])[e] = pre[k]; }
-        }
+        for (int k = 0; k < NL; k++) { sm[buf][tid + 256 * k] = pre[k]; }
     };
 
     T q[3][RY + 2][3];
@@ -179,7 +287,6 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
 #pragma unroll
             for (int c = 0; c < 3; c++) q[p][r][c] = (T)0;
 
-    T amax = 0, sum = 0, amax2 = 0, sum2 = 0;
     const int gx = x0 + lane;
     const bool xok = gx <= a.ibox.hi[0];
     // residual modes: f of the next output plane, loaded one plane ahead
@@ -202,7 +309,7 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
             for (int c = 0; c < 3; c++) {
                 q[0][r][c] = q[1][r][c];
                 q[1][r][c] = q[2][r][c];
-                q[2][r][c] = sm[buf][w * RY + r][lane + c];
+                q[2][r][c] = sm[buf][(w * RY + r) * LX + lane + c];
             }
         if (zz < zb + 1) { return; }
         const int z = zz - 1;
@@ -273,6 +380,43 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
             compute(zz, 1);
             if (++zz > ze) { break; }
         }
+    }
+}
+
+// Block = 4 wave64s over a 64 x (4 RY) tile of one (x, y) column range; the
+// block walks its z range plane by plane.  Each plane (with a 1-cell halo in
+// x and y) is staged once from HBM into LDS; each thread keeps the 3 x 3
+// neighbourhoods of its RY rows for planes z-1, z, z+1 in registers, so one
+// LDS pass per plane feeds all 27 taps.  Global loads of the next PF planes
+// are in flight while a plane is computed.  NT = 1 writes results with
+// non-temporal stores (they are streamed out, never re-read by this kernel).
+// Tile rows (RY), prefetch depth (PF), NT and the z extent per block are the
+// autotuner's knobs.
+template<typename T, int SHAPE, int MODE, int RY, int PF, int NT>
+__global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
+{
+    using G = ZGeom<RY>;
+    __shared__ T sm[3][G::NP];
+    __shared__ T red[16];
+
+    const int tid = threadIdx.x;
+    unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = b % a.ntx;
+    b /= a.ntx;
+    const int ty = b % a.nty;
+    const int tzi = b / a.nty;
+    const int x0 = a.ibox.lo[0] + tx * G::TX, y0 = a.ibox.lo[1] + ty * G::TY;
+    const int zb = a.ibox.lo[2] + tzi * a.tz;
+    const int ze = min(zb + a.tz, a.ibox.hi[2] + 1);  // outputs z in [zb, ze)
+    const bool interior = x0 - 1 >= a.sbox.lo[0] && x0 + G::TX <= a.sbox.hi[0] && y0 - 1 >= a.sbox.lo[1] &&
+                          y0 + G::TY <= a.sbox.hi[1] && x0 + G::TX - 1 <= a.ibox.hi[0] &&
+                          y0 + G::TY - 1 <= a.ibox.hi[1];
+
+    T amax = 0, sum = 0, amax2 = 0, sum2 = 0;
+    if (interior) {
+        zm_interior<T, SHAPE, MODE, RY, PF, NT>(a, sm, x0, y0, zb, ze, amax, sum, amax2, sum2);
+    } else {
+        zm_edge<T, SHAPE, MODE, RY, PF, NT>(a, sm, x0, y0, zb, ze, amax, sum, amax2, sum2);
     }
     if (MODE == ZM_RESIDUAL || MODE == ZM_RESIDUAL2) {
         const unsigned pb = blockIdx.x;
