@@ -203,29 +203,32 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
         const T* rh = rhs + bx.voff;
         const int y0 = wi.y0;
         // in-plane offsets of this thread's elements: phi element e -> row
-        // y0 - 2 + e / LX, column -1 + e % LX of the ghosted plane (-1: outside
-        // the ghosts, reads as 0); rhs element e -> row y0 - 1 + e / nx
+        // y0 - 2 + e / LX, column -1 + e % LX of the ghosted plane (bit i of
+        // pout: outside the ghosts, reads as 0); rhs element e -> row
+        // y0 - 1 + e / nx (bit i of rout: outside the box).  Elements past the
+        // plane repeat the last one; every load is issued unconditionally
+        // (outside elements load element 0 and are zeroed at their commit), so
+        // the waits only cover loads a step old.
         int offp[MAXE], offr[MAXE];
+        unsigned pout = 0, rout = 0;
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
-            const int e = tid + 256 * i;
-            offp[i] = offr[i] = -1;
-            if (e < NP) {
-                const int er = e / LX, ec = e - er * LX;
-                const int gy = y0 - 2 + er, gx = ec - 1;
-                if (gy >= -g && gy < bx.ny + g) { offp[i] = (gx + g) + (int)Gx * (gy + g); }
-            }
-            if (e < NR) {
-                const int er = e / bx.nx, ec = e - er * bx.nx;
-                const int gy = y0 - 1 + er;
-                if (gy >= 0 && gy < bx.ny) { offr[i] = ec + bx.nx * gy; }
-            }
+            const int e = min(tid + 256 * i, NP - 1), er = e / LX, ec = e - er * LX;
+            const int gy = y0 - 2 + er, gx = ec - 1;
+            const bool pin_ = gy >= -g && gy < bx.ny + g;
+            offp[i] = pin_ ? (gx + g) + (int)Gx * (gy + g) : 0;
+            pout |= pin_ ? 0u : (1u << i);
+            const int f = min(tid + 256 * i, NR - 1), fr = f / bx.nx, fc = f - fr * bx.nx;
+            const int fy = y0 - 1 + fr;
+            const bool rin = fy >= 0 && fy < bx.ny;
+            offr[i] = rin ? fc + bx.nx * fy : 0;
+            rout |= rin ? 0u : (1u << i);
         }
         T pre[MAXE], prr[MAXE];
         auto issue = [&](int z) {
             const T* base = p + Gxy * (long)(z + g);
 #pragma unroll
-            for (int i = 0; i < MAXE; i++) { pre[i] = offp[i] >= 0 ? base[offp[i]] : (T)0; }
+            for (int i = 0; i < MAXE; i++) { pre[i] = base[offp[i]]; }
         };
         auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
         auto commit = [&](int z) {
@@ -233,13 +236,13 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
 #pragma unroll
             for (int i = 0; i < MAXE; i++) {
                 const int e = tid + 256 * i;
-                if (e < NP) { pl[e] = pre[i]; }
+                if (e < NP) { pl[e] = (pout >> i) & 1u ? (T)0 : pre[i]; }
             }
         };
         auto issue_r = [&](int z) {
             const T* base = rh + (long)bx.nx * bx.ny * z;
 #pragma unroll
-            for (int i = 0; i < MAXE; i++) { prr[i] = offr[i] >= 0 ? base[offr[i]] : (T)0; }
+            for (int i = 0; i < MAXE; i++) { prr[i] = base[offr[i]]; }
         };
         auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
         auto commit_r = [&](int z) {
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
 #pragma unroll
             for (int i = 0; i < MAXE; i++) {
                 const int e = tid + 256 * i;
-                if (e < NR) { pl[e] = prr[i]; }
+                if (e < NR) { pl[e] = (rout >> i) & 1u ? (T)0 : prr[i]; }
             }
         };
         // half sweep on plane z, rows [ya, yb), colour col: p += lambda (sum c_j p(nbr) + rhs)
@@ -285,27 +288,31 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
         commit(0);
         issue(1);
         issue_r(0);
+        auto write_out = [&](int z) {
+            const T* pl = slot(z);
+            T* base = po + Gxy * (long)(z + g);
+            const int ny_t = min(R, bx.ny - y0);
+            for (int e = tid; e < ny_t * bx.nx; e += 256) {
+                const int yy = y0 + e / bx.nx, xx = e % bx.nx;
+                base[(xx + g) + Gx * (long)(yy + g)] = pl[(yy - (y0 - 2)) * LX + xx + 1];
+            }
+        };
+        // every memory operation of a step is issued at its start (loads of
+        // the planes ahead, then the write-out of finished plane k-1), so the
+        // wait at the next step's start covers only operations a whole step old
         for (int k = -1; k < bx.nz; k++) {
             if (k + 2 <= bx.nz) { commit(k + 2); }  // into the slot of plane k-2, no longer read
             if (k + 1 < bx.nz) { commit_r(k + 1); }  // into the slot of rhs plane k-1, likewise
             if (k + 3 <= bx.nz) { issue(k + 3); }
             if (k + 2 < bx.nz) { issue_r(k + 2); }
+            if (k >= 1) { write_out(k - 1); }        // slot of plane k-1: only read during this step
             __syncthreads();
             update(k + 1, y0 - 1, y0 + R + 1, 0);  // red: tile rows and one row each side
             __syncthreads();
             update(k, y0, y0 + R, 1);              // black: tile rows
             __syncthreads();
-            if (k >= 0) {
-                const T* pl = slot(k);
-                T* base = po + Gxy * (long)(k + g);
-                const int ny_t = min(R, bx.ny - y0);
-                for (int e = tid; e < ny_t * bx.nx; e += 256) {
-                    const int yy = y0 + e / bx.nx, xx = e % bx.nx;
-                    base[(xx + g) + Gx * (long)(yy + g)] = pl[(yy - (y0 - 2)) * LX + xx + 1];
-                }
-            }
-            __syncthreads();
         }
+        write_out(bx.nz - 1);
     }
     queue_done(counter);
 }

@@ -157,40 +157,31 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
     auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
     // in-plane offsets of this thread's elements for planes inside the box
-    // (z = 0; plane z adds z * plane size); planes beyond the box go through at()
+    // (z = 0; plane z adds z * plane size); planes beyond the box go through
+    // at().  Elements past the plane repeat the last one, so every load is
+    // issued unconditionally, straight into the register that holds it until
+    // its commit: the compiler's waits then only ever cover loads a step old.
     long offp[MAXE], offr[MAXE];
     const long XYp = (long)(nx + 2 * gin.g[0]) * (ny + 2 * gin.g[1]);
     const long XYr = (long)(nx + 2 * gr.g[0]) * (ny + 2 * gr.g[1]);
 #pragma unroll
     for (int i = 0; i < MAXE; i++) {
-        const int e = threadIdx.x + 256 * i;
-        offp[i] = offr[i] = 0;
-        if (e < NP) {
-            const int er = e / W, ec = e - er * W;
-            offp[i] = at(gin, vin, ec - 2, y0 - 2 + er, 0);
-        }
-        if (e < NR) {
-            const int er = e / WR, ec = e - er * WR;
-            offr[i] = at(gr, vr, ec - 1, y0 - 1 + er, 0);
-        }
+        const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
+        offp[i] = at(gin, vin, ec - 2, y0 - 2 + er, 0);
+        const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
+        offr[i] = at(gr, vr, fc - 1, y0 - 1 + fr, 0);
     }
     T pre[MAXE], prr[MAXE];
     auto issue = [&](int z) {
-        if (z >= 0 && z < nz) {
-            const T* base = pin + XYp * z;
+        const bool inb = z >= 0 && z < nz;
 #pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                if (threadIdx.x + 256 * i < NP) { pre[i] = base[offp[i]]; }
+        for (int i = 0; i < MAXE; i++) {
+            const T* ptr = pin + XYp * z + offp[i];
+            if (!inb) {
+                const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
+                ptr = pin + at(gin, vin, ec - 2, y0 - 2 + er, z);
             }
-        } else {
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                const int e = threadIdx.x + 256 * i;
-                if (e < NP) {
-                    const int er = e / W, ec = e - er * W;
-                    pre[i] = pin[at(gin, vin, ec - 2, y0 - 2 + er, z)];
-                }
-            }
+            pre[i] = *ptr;
         }
     };
     auto commit = [&](int z) {
@@ -202,21 +193,15 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         }
     };
     auto issue_r = [&](int z) {
-        if (z >= 0 && z < nz) {
-            const T* base = rhs + XYr * z;
+        const bool inb = z >= 0 && z < nz;
 #pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                if (threadIdx.x + 256 * i < NR) { prr[i] = base[offr[i]]; }
+        for (int i = 0; i < MAXE; i++) {
+            const T* ptr = rhs + XYr * z + offr[i];
+            if (!inb) {
+                const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
+                ptr = rhs + at(gr, vr, fc - 1, y0 - 1 + fr, z);
             }
-        } else {
-#pragma unroll
-            for (int i = 0; i < MAXE; i++) {
-                const int e = threadIdx.x + 256 * i;
-                if (e < NR) {
-                    const int er = e / WR, ec = e - er * WR;
-                    prr[i] = rhs[at(gr, vr, ec - 1, y0 - 1 + er, z)];
-                }
-            }
+            prr[i] = *ptr;
         }
     };
     auto commit_r = [&](int z) {
@@ -259,26 +244,31 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     commit(z0 - 1);
     issue(z0);
     issue_r(z0 - 1);
+    auto write_out = [&](int z) {
+        const T* pl = slot(z);
+        const int cnt = (y1 - y0) * nx;
+        for (int e = threadIdx.x; e < cnt; e += 256) {
+            const int r = e / nx, xx = e - r * nx;
+            ob[rel_index(L, gout, xx, y0 + r, z)] = pl[(r + 2) * W + xx + 2];
+        }
+    };
+    // Memory operations of a step are all issued at its start (loads of the
+    // planes ahead, then the write-out of finished plane k-1), so the wait for
+    // the staged planes at the next step's start covers only operations a
+    // whole step old: neither load nor store latency is exposed.
     for (int k = z0 - 2; k < z1; k++) {
         commit(k + 2);    // into the slot of plane k-2, last read before the previous barrier
         commit_r(k + 1);  // into the slot of rhs plane k-1, likewise
         if (k + 3 <= z1 + 1) { issue(k + 3); }
         if (k + 2 <= z1) { issue_r(k + 2); }
+        if (k - 1 >= z0) { write_out(k - 1); }  // slot of plane k-1: only read during this step
         __syncthreads();
         update(k + 1, y0 - 1, y1 + 1, -1, nx + 1, 0);
         __syncthreads();
-        if (k >= z0) {
-            update(k, y0, y1, 0, nx, 1);
-            __syncthreads();
-            const T* pl = slot(k);
-            const int cnt = (y1 - y0) * nx;
-            for (int e = threadIdx.x; e < cnt; e += 256) {
-                const int r = e / nx, xx = e - r * nx;
-                ob[rel_index(L, gout, xx, y0 + r, k)] = pl[(r + 2) * W + xx + 2];
-            }
-        }
+        if (k >= z0) { update(k, y0, y1, 0, nx, 1); }
         __syncthreads();
     }
+    write_out(z1 - 1);
 }
 
 // ------------------------------------------------------------ Jacobi

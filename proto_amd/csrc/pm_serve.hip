@@ -94,28 +94,22 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     const long nn = (long)n * n;
     auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
     auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
-    // in-plane offsets (periodic images folded in) of this thread's elements
+    // in-plane offsets (periodic images folded in) of this thread's elements;
+    // elements past the plane repeat the last one, so every load is issued
+    // unconditionally and the waits only cover loads a step old
     int offp[MAXE], offr[MAXE];
 #pragma unroll
     for (int i = 0; i < MAXE; i++) {
-        const int e = threadIdx.x + 256 * i;
-        offp[i] = offr[i] = 0;
-        if (e < NP) {
-            const int er = e / W, ec = e - er * W;
-            offp[i] = wrapi(ec - 2, n) + n * wrapi(y0 - 2 + er, n);
-        }
-        if (e < NR) {
-            const int er = e / WR, ec = e - er * WR;
-            offr[i] = wrapi(ec - 1, n) + n * wrapi(y0 - 1 + er, n);
-        }
+        const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
+        offp[i] = wrapi(ec - 2, n) + n * wrapi(y0 - 2 + er, n);
+        const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
+        offr[i] = wrapi(fc - 1, n) + n * wrapi(y0 - 1 + fr, n);
     }
     float pre[MAXE], prr[MAXE];
     auto issue = [&](int z) {
         const float* base = in + nn * wrapi(z, n);
 #pragma unroll
-        for (int i = 0; i < MAXE; i++) {
-            if (threadIdx.x + 256 * i < NP) { pre[i] = base[offp[i]]; }
-        }
+        for (int i = 0; i < MAXE; i++) { pre[i] = base[offp[i]]; }
     };
     auto commit = [&](int z) {
         float* pl = slot(z);
@@ -128,9 +122,7 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     auto issue_r = [&](int z) {
         const float* base = L.rhs + nn * wrapi(z, n);
 #pragma unroll
-        for (int i = 0; i < MAXE; i++) {
-            if (threadIdx.x + 256 * i < NR) { prr[i] = base[offr[i]]; }
-        }
+        for (int i = 0; i < MAXE; i++) { prr[i] = base[offr[i]]; }
     };
     auto commit_r = [&](int z) {
         float* pl = rslot(z);
@@ -171,27 +163,31 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     commit(z0 - 1);
     issue(z0);
     issue_r(z0 - 1);
+    auto write_out = [&](int z) {
+        const float* pl = slot(z);
+        float* ob = out + nn * z;
+        const int cnt = (y1 - y0) * n;
+        for (int e = threadIdx.x; e < cnt; e += 256) {
+            const int r = e / n, xx = e - r * n;
+            ob[xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
+        }
+    };
+    // all memory operations of a step are issued at its start (loads of the
+    // planes ahead, then the write-out of finished plane k-1), so the wait at
+    // the next step's start covers only operations a whole step old
     for (int k = z0 - 2; k < z1; k++) {
         commit(k + 2);    // into the slot of plane k-2, last read before the previous barrier
         commit_r(k + 1);  // into the slot of rhs plane k-1, likewise
         if (k + 3 <= z1 + 1) { issue(k + 3); }
         if (k + 2 <= z1) { issue_r(k + 2); }
+        if (k - 1 >= z0) { write_out(k - 1); }
         __syncthreads();
         update(k + 1, y0 - 1, y1 + 1, -1, n + 1, 0);  // red, one-cell halo included
         __syncthreads();
-        if (k >= z0) {
-            update(k, y0, y1, 0, n, 1);  // black, tile cells
-            __syncthreads();
-            const float* pl = slot(k);
-            float* ob = out + nn * k;
-            const int cnt = (y1 - y0) * n;
-            for (int e = threadIdx.x; e < cnt; e += 256) {
-                const int r = e / n, xx = e - r * n;
-                ob[xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
-            }
-        }
+        if (k >= z0) { update(k, y0, y1, 0, n, 1); }  // black, tile cells
         __syncthreads();
     }
+    write_out(z1 - 1);
 }
 
 // Coarse right-hand side = AvgDown(rhs + sum_j c_j phi(p + o_j)) of the fine
