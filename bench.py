@@ -28,8 +28,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the fused residual kernel (rocprofv3 FETCH_SIZE x2.0 +
-# WRITE_SIZE x1.0, calibrated on a 1 GiB copy; range over the tuned z chunks)
-TRAFFIC_RESIDUAL2 = 4.36e9
+# WRITE_SIZE x1.0, calibrated on a 1 GiB copy; 4.50-4.53 GB over the z chunks
+# 16..256 of the 4-row, 2-plane-prefetch, non-temporal configuration)
+TRAFFIC_RESIDUAL2 = 4.52e9
 
 
 def parse():
@@ -432,7 +433,10 @@ def main():
             for d in (ph, tm, rh):
                 d.free()
         pa.call("pm_layout_destroy", lay)
-        print(json.dumps({"only": args.only, "steps": args.steps}))
+        cpy = DeviceArray(n ** 3)  # plain 1 GiB device copy to calibrate the counters
+        pa.call("pm_memcpy", cpy.ptr, r1.ptr, bytes_f, 2, sp)
+        stream.sync()
+        print(json.dumps({"only": args.only, "steps": args.steps, "copy_bytes": 2 * bytes_f}))
         return
 
     if args.only == "batch":
