@@ -565,6 +565,18 @@ def main():
             gbs = 2 * bytes_f / (ms * 1e-3) / 1e9
             extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
                            "bytes": 2 * bytes_f}
+        # streaming ceilings over 4 GiB (16 workgroups per CU): read-only,
+        # write-only (non-temporal), copy
+        big = DeviceArray(4 * n ** 3)
+        big2 = DeviceArray(4 * n ** 3)
+        nb4 = 4 * bytes_f
+        for name, kind, moved in (("stream_read", 0, nb4), ("stream_write", 1, nb4), ("stream_copy", 2, 2 * nb4)):
+            ms = time_launches(lambda: pa.call("pm_stream_probe", kind, big2.ptr, big.ptr, nb4, 16, sp), 10, stream)
+            gbs = moved / (ms * 1e-3) / 1e9
+            extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
+                           "bytes": moved}
+        big.free()
+        big2.free()
         a.free()
         result["extra"] = extra
 

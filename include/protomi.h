@@ -69,6 +69,13 @@ int pm_memset_zero(void* ptr, size_t bytes, pm_stream_t stream);
  * caches).  Also the achievable-bandwidth probe of the benchmark.  bytes and
  * both pointers must be multiples of 16. */
 int pm_copy_device(void* dst, const void* src, size_t bytes, int nontemporal, pm_stream_t stream);
+/* Streaming-bandwidth probe (no counterpart in Proto; measures the ceilings the
+ * kernels' roofline fractions are read against).  kind 0 reads `src` (dst may be
+ * null or any 16-byte-aligned device word), 1 writes `dst` with non-temporal
+ * stores, 2 copies src -> dst; 16-byte accesses over 256 * blocks_per_cu
+ * 256-thread workgroups (blocks_per_cu in [1, 64]).  bytes and the pointers
+ * must be multiples of 16. */
+int pm_stream_probe(int kind, void* dst, const void* src, size_t bytes, int blocks_per_cu, pm_stream_t stream);
 int pm_stream_create(pm_stream_t* stream);
 int pm_stream_destroy(pm_stream_t stream);
 int pm_stream_sync(pm_stream_t stream);

@@ -141,6 +141,7 @@ SIGNATURES = {
     "pm_level_residual": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _d, _i, _vp, _ip, _vp, _vp]),
     "pm_level_add_correction": (_i, [_vp, _i, _vp, _ip, _i, _vp, _ip, _vp]),
     "pm_copy_device": (_i, [_vp, _vp, _sz, _i, _vp]),
+    "pm_stream_probe": (_i, [_i, _vp, _vp, _sz, _i, _vp]),
     "pm_stencil_autotune": (_i, [_vp, _vp, _i, _BP, _BP, _i, ctypes.POINTER(TileConfig), ctypes.POINTER(ctypes.c_float), _vp]),
     "pm_stencil_set_tile_config": (_i, [_vp, _vp, _i, _BP, ctypes.POINTER(TileConfig)]),
     "pm_stencil_get_tile_config": (_i, [_vp, _vp, _i, _BP, ctypes.POINTER(TileConfig)]),

@@ -27,6 +27,7 @@
 //          waits for another.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pm_internal.hpp"
@@ -84,7 +85,19 @@ __device__ inline void queue_done(unsigned* counter)
 }
 
 // ------------------------------------------------------------ batched apply
-constexpr int BA_LDS = 400;  // (rows + 2) x (xw + 2) for any tile shape used
+// A tile is XW (<= 64) columns x TR thread-rows x BA_RY rows per thread of one
+// box over a z range: thread t owns column t % XW and rows
+// y0 + (t / XW) BA_RY + [0, BA_RY).  Each plane (tile + 1-cell halo, at most
+// BA_NP elements) is staged once in LDS; each thread keeps the 3 x 3
+// neighbourhoods of its rows for planes z-1, z, z+1 in registers, so one LDS
+// pass per plane feeds all taps of BA_RY outputs.  Three LDS planes, three
+// register planes and three staging sets rotate with a 3-way unrolled z loop.
+// Every load is issued unconditionally (offsets clamped into the ghosted box;
+// clamped elements only feed outputs outside the box, which are not stored),
+// and a step's results are stored at the start of the next step, after its
+// loads: the only waits are at the LDS commits and cover operations a whole
+// step old.
+constexpr int BA_RY = 4, BA_NL = 5, BA_NP = 256 * BA_NL;
 
 template<typename T, int SHAPE, int NT>
 __global__ __launch_bounds__(256) void k_batch_apply(const BatchBox* __restrict__ boxes,
@@ -92,8 +105,8 @@ __global__ __launch_bounds__(256) void k_batch_apply(const BatchBox* __restrict_
                                                      const T* __restrict__ src, T* __restrict__ dst, BCoefs<T> c,
                                                      unsigned* counter)
 {
-    constexpr int NC = SHAPE == 7 ? 7 : 27;
-    __shared__ T sm[2][BA_LDS];
+    constexpr int NC = SHAPE == 7 ? 7 : 27, RY = BA_RY, NL = BA_NL;
+    __shared__ T sm[3][BA_NP];
     __shared__ int s_item;
     const int tid = threadIdx.x;
     for (;;) {
@@ -101,75 +114,95 @@ __global__ __launch_bounds__(256) void k_batch_apply(const BatchBox* __restrict_
         if (it >= nitems) { break; }
         const BatchItem wi = items[it];
         const BatchBox bx = boxes[wi.box];
-        const int XW = wi.xw, R = wi.rows, LX = XW + 2, NE = (R + 2) * LX;
+        const int XW = wi.xw, TR = wi.rows, LX = XW + 2, NE = (TR * RY + 2) * LX;
         const long Gx = bx.nx + 2 * g, Gxy = Gx * (bx.ny + 2 * g);
         const T* s = src + bx.goff;
         T* d = dst + bx.voff;
-        const int col = tid % XW, row = tid / XW;
-        const int x = wi.x0 + col, y = wi.y0 + row;
-        const bool active = row < R && x < bx.nx && y < bx.ny;
-        // up to two plane elements per thread (NE <= 512)
-        long eoff[2];
-        bool eok[2];
+        const int col = tid % XW, trow = tid / XW;
+        const bool tact = trow < TR;
+        const int x = wi.x0 + col, yb = wi.y0 + trow * RY;
+        int eoff[NL];
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const int e = tid + 256 * k;
+        for (int k = 0; k < NL; k++) {
+            const int e = min(tid + 256 * k, NE - 1);
             const int er = e / LX, ec = e - er * LX;
-            const int gx = wi.x0 - 1 + ec, gy = wi.y0 - 1 + er;
-            eok[k] = e < NE && gx >= -g && gx < bx.nx + g && gy >= -g && gy < bx.ny + g;
-            eoff[k] = (long)(gx + g) + Gx * (long)(gy + g);
+            const int gx = wi.x0 - 1 + ec, gy = min(wi.y0 - 1 + er, bx.ny + g - 1);
+            eoff[k] = (gx + g) + (int)Gx * (gy + g);
         }
-        T pre[2];
-        auto load = [&](int z) {
+        T P[3][NL];
+        auto gload = [&](T (&p)[NL], int z) {
             const T* base = s + Gxy * (long)(z + g);
 #pragma unroll
-            for (int k = 0; k < 2; k++) { pre[k] = eok[k] ? base[eoff[k]] : (T)0; }
+            for (int k = 0; k < NL; k++) { p[k] = base[eoff[k]]; }
         };
-        auto store = [&](int buf) {
+        auto lstore = [&](const T (&p)[NL], T* pl) {
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int e = tid + 256 * k;
-                if (e < NE) { sm[buf][e] = pre[k]; }
+            for (int k = 0; k < NL; k++) { pl[tid + 256 * k] = p[k]; }
+        };
+        T q[3][RY + 2][3];
+        const int lr0 = tact ? trow * RY : 0;  // idle threads read row 0 (never stored)
+        auto lread = [&](T (&qq)[RY + 2][3], const T* pl) {
+#pragma unroll
+            for (int r = 0; r < RY + 2; r++)
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) { qq[r][cc] = pl[(lr0 + r) * LX + col + cc]; }
+        };
+        T res[RY];
+        int zres = -1;  // plane of the results held in res (-1: none)
+        auto put_res = [&]() {
+            if (zres < 0 || !tact) { return; }
+            T* o = d + x + (long)bx.nx * ((long)yb + (long)bx.ny * zres);
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                if (yb + r < bx.ny) {
+                    if (NT) { __builtin_nontemporal_store(res[r], o + (long)r * bx.nx); } else { o[(long)r * bx.nx] = res[r]; }
+                }
             }
         };
-        T q[3][3][3];
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-#pragma unroll
-            for (int b = 0; b < 3; b++)
-#pragma unroll
-                for (int e = 0; e < 3; e++) q[a][b][e] = (T)0;
-        int buf = 0;
-        load(wi.z0 - 1);
-        for (int zz = wi.z0 - 1; zz <= wi.z1; ++zz) {
-            // one barrier per plane: buffers alternate, and a thread can only
-            // overwrite buffer b after every thread passed the barrier that
-            // follows its last read of b
-            store(buf);
-            if (zz + 1 <= wi.z1) { load(zz + 1); }
+        // one z step: stage plane zz (slot S), store the previous step's
+        // results, emit plane zz - 1 from slots S+1 (zz-2), S+2 (zz-1), S (zz)
+        auto step = [&](auto sc, int zz) {
+            constexpr int S = decltype(sc)::value, SB = (S + 1) % 3, SC = (S + 2) % 3;
+            lstore(P[S], sm[S]);
+            gload(P[(S + 1) % 3], min(zz + 1, wi.z1));
+            put_res();
             __syncthreads();
-            if (row < R) {
+            lread(q[S], sm[S]);
 #pragma unroll
-                for (int r = 0; r < 3; r++)
-#pragma unroll
-                    for (int cc = 0; cc < 3; cc++) {
-                        q[0][r][cc] = q[1][r][cc];
-                        q[1][r][cc] = q[2][r][cc];
-                        q[2][r][cc] = sm[buf][(row + r) * LX + col + cc];
-                    }
-            }
-            if (zz >= wi.z0 + 1 && active) {
+            for (int r = 0; r < RY; r++) {
+                auto tap = [&](int dx, int dy, int dz) -> T {
+                    return dz < 0 ? q[SB][r + 1 + dy][1 + dx]
+                                  : (dz == 0 ? q[SC][r + 1 + dy][1 + dx] : q[S][r + 1 + dy][1 + dx]);
+                };
                 T acc = 0;
 #pragma unroll
                 for (int j = 0; j < NC; j++) {
-                    const int dx = shape_off<SHAPE>(j, 0), dy = shape_off<SHAPE>(j, 1), dz = shape_off<SHAPE>(j, 2);
-                    acc = acc + c.c[j] * q[1 + dz][1 + dy][1 + dx];
+                    acc = acc + c.c[j] * tap(shape_off<SHAPE>(j, 0), shape_off<SHAPE>(j, 1), shape_off<SHAPE>(j, 2));
                 }
-                T* o = d + x + (long)bx.nx * (y + (long)bx.ny * (zz - 1));
-                if (NT) { __builtin_nontemporal_store(acc, o); } else { *o = acc; }
+                res[r] = acc;
             }
-            buf ^= 1;
+            zres = zz - 1;
+        };
+        // planes z0 - 1 and z0: staged, nothing to emit yet
+        gload(P[0], wi.z0 - 1);
+        lstore(P[0], sm[0]);
+        gload(P[1], wi.z0);
+        __syncthreads();
+        lread(q[0], sm[0]);
+        lstore(P[1], sm[1]);
+        gload(P[2], min(wi.z0 + 1, wi.z1));
+        __syncthreads();
+        lread(q[1], sm[1]);
+        int zz = wi.z0 + 1;
+        for (;;) {
+            step(std::integral_constant<int, 2>{}, zz);
+            if (++zz > wi.z1) { break; }
+            step(std::integral_constant<int, 0>{}, zz);
+            if (++zz > wi.z1) { break; }
+            step(std::integral_constant<int, 1>{}, zz);
+            if (++zz > wi.z1) { break; }
         }
+        put_res();
     }
     queue_done(counter);
 }
@@ -338,8 +371,10 @@ int upload(void** dptr, const void* host, size_t bytes)
     return PM_OK;
 }
 
-// Apply tiles: x in steps of 64, rows = min(32, 256 / width), z chunks of tz
-// (0 = whole box); largest first.
+// Apply tiles: x in steps of 64; TR thread-rows of BA_RY rows each, TR the
+// largest with TR * XW <= 256 threads, TR * BA_RY no more than the rows left
+// (rounded up) and the staged plane (TR BA_RY + 2) (XW + 2) <= BA_NP; z chunks
+// of tz (0 = whole box); largest first.
 int build_apply_items(pm_batch* B, int tz)
 {
     std::vector<BatchItem> items;
@@ -347,18 +382,19 @@ int build_apply_items(pm_batch* B, int tz)
         const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1], nz = B->sizes[3 * b + 2];
         for (int x0 = 0; x0 < nx; x0 += 64) {
             const int xw = std::min(64, nx - x0);
-            const int rows = std::max(1, std::min(32, 256 / xw));
-            for (int y0 = 0; y0 < ny; y0 += rows) {
+            const int trmax = std::max(1, std::min(256 / xw, (BA_NP / (xw + 2) - 2) / BA_RY));
+            for (int y0 = 0; y0 < ny; y0 += trmax * BA_RY) {
+                const int tr = std::min(trmax, (ny - y0 + BA_RY - 1) / BA_RY);
                 const int zc = tz > 0 ? tz : nz;
                 for (int z0 = 0; z0 < nz; z0 += zc) {
-                    items.push_back(BatchItem{b, x0, y0, z0, std::min(nz, z0 + zc), rows, xw, 0});
+                    items.push_back(BatchItem{b, x0, y0, z0, std::min(nz, z0 + zc), tr, xw, 0});
                 }
             }
         }
     }
     std::stable_sort(items.begin(), items.end(), [](const BatchItem& a, const BatchItem& b) {
         return (long)a.xw * a.rows * (a.z1 - a.z0) > (long)b.xw * b.rows * (b.z1 - b.z0);
-    });
+    });  // rows = thread-rows: proportional to the tile's cells
     B->nitems_apply = (int)items.size();
     B->apply_tz = tz;
     return upload(&B->d_items, items.data(), sizeof(BatchItem) * items.size());

@@ -216,6 +216,59 @@ int pm_copy_device(void* dst, const void* src, size_t bytes, int nontemporal, pm
     return PM_OK;
 }
 
+// Streaming-bandwidth probes: read-only (eight 16-byte loads in flight per
+// thread, folded into one word stored only when it matches a value the data
+// never produces, so the loads stay live) and write-only.
+__global__ __launch_bounds__(256) void k_probe_read(const u32x4* __restrict__ src, size_t n, unsigned* __restrict__ sink)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned acc = 0;
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        u32x4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) { v[u] = __builtin_nontemporal_load(&src[i + u * stride]); }
+#pragma unroll
+        for (int u = 0; u < 8; u++) { acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w; }
+    }
+    for (; i < n; i += stride) {
+        const u32x4 v = src[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9e3779b9u && threadIdx.x == 0x3ff) { sink[0] = acc; }  // never true: threadIdx.x < 256
+}
+
+__global__ __launch_bounds__(256) void k_probe_write(u32x4* __restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const unsigned w = (unsigned)i;
+        __builtin_nontemporal_store(u32x4{w, w + 1u, w + 2u, w + 3u}, &dst[i]);
+    }
+}
+
+int pm_stream_probe(int kind, void* dst, const void* src, size_t bytes, int blocks_per_cu, pm_stream_t stream)
+{
+    PM_REQUIRE(kind >= 0 && kind <= 2, "pm_stream_probe: kind must be 0 (read), 1 (write) or 2 (copy)");
+    PM_REQUIRE(blocks_per_cu >= 1 && blocks_per_cu <= 64, "pm_stream_probe: blocks_per_cu outside [1, 64]");
+    PM_REQUIRE(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
+               "pm_stream_probe: bytes and pointers must be multiples of 16");
+    PM_REQUIRE(kind == 1 || src, "pm_stream_probe: null source");
+    PM_REQUIRE(kind == 0 || dst, "pm_stream_probe: null destination");
+    if (bytes == 0) { return PM_OK; }
+    const size_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)256 * blocks_per_cu);
+    if (kind == 0) {
+        hipLaunchKernelGGL(k_probe_read, dim3(blocks), dim3(256), 0, S(stream), (const u32x4*)src, n, (unsigned*)dst);
+    } else if (kind == 1) {
+        hipLaunchKernelGGL(k_probe_write, dim3(blocks), dim3(256), 0, S(stream), (u32x4*)dst, n);
+    } else {
+        hipLaunchKernelGGL(k_copy16<1>, dim3(blocks), dim3(256), 0, S(stream), (const u32x4*)src, (u32x4*)dst, n);
+    }
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
 int pm_stream_create(pm_stream_t* stream)
 {
     PM_REQUIRE(stream, "pm_stream_create: null out");

@@ -164,3 +164,24 @@ def test_level_linear_out_in(n, bs, g):
     pa.call("pm_level_linear_in", lay, pa.PM_F64, vp(back), gg, 2, vp(dbuf), 0, None)
     assert host(back).tobytes() == gold.tobytes()
     pa.call("pm_layout_destroy", lay)
+
+
+def test_stream_probe_kinds():
+    """The bandwidth probe: copy reproduces the source, write fills the
+    word pattern (w, w+1, w+2, w+3) of 16-byte vector w, read leaves memory
+    alone; bad arguments are refused."""
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 2 ** 32, size=4 * 4099, dtype=np.uint32)
+    src = dev(a)
+    dst = zeros(a.size, np.uint32)
+    pa.call("pm_stream_probe", 2, vp(dst), vp(src), a.nbytes, 4, None)
+    assert host(dst).tobytes() == a.tobytes()
+    pa.call("pm_stream_probe", 1, vp(dst), None, a.nbytes, 4, None)
+    w = np.arange(a.size // 4, dtype=np.uint32)[:, None] + np.arange(4, dtype=np.uint32)[None, :]
+    assert host(dst).tobytes() == w.reshape(-1).tobytes()
+    sink = zeros(4, np.uint32)
+    pa.call("pm_stream_probe", 0, vp(sink), vp(src), a.nbytes, 4, None)
+    assert not host(sink).any() and host(src).tobytes() == a.tobytes()
+    assert pa.lib().pm_stream_probe(3, vp(dst), vp(src), a.nbytes, 4, None) == -1
+    assert pa.lib().pm_stream_probe(2, vp(dst), vp(src), a.nbytes - 4, 4, None) == -1
+    assert pa.lib().pm_stream_probe(2, vp(dst), vp(src), a.nbytes, 65, None) == -1
