@@ -14,7 +14,8 @@ Inputs are resident in HBM before the timed region.
 roofline: the fused kernel's own average duration (HIP events on its stream,
 launches without the tiny final-reduction kernel), against 8 TB/s.
 cpu_baseline: Proto's own CPU path (oracle/_ref, Release flags, one thread)
-running the same program on a bounded z-slab of the same field.
+running the same program on a bounded z-slab of the same field; its
+"all_cores" entry runs 16 such slabs in 16 host processes at once.
 """
 import argparse
 import ctypes
@@ -87,9 +88,12 @@ def time_launches(fn, reps, stream):
     return e0.elapsed_ms(e1) / reps
 
 
-def cpu_baseline(n, g, planes):
-    """Proto's CPU path on a z-slab of `planes` planes of the same config-2
-    field (both operators, forall residual, absMax, sum)."""
+def _cpu_slab(n, g, nz, t_go=0.0):
+    """Proto's CPU path (oracle/_ref, or the C restatement where the reference
+    is not built) on a z-slab of nz planes of the config-2 field: both
+    operators, forall residual, absMax, sum.  Returns (wall start, wall end,
+    algorithmic bytes, kind); inputs are prepared outside the timed region,
+    which starts no earlier than the wall-clock time t_go."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import util
@@ -97,13 +101,12 @@ def cpu_baseline(n, g, planes):
     kind = "reference"
     x = (np.arange(n) + 0.5) / n
     s = np.sin(2 * np.pi * x)
-    nz = planes
     noise = np.zeros(n * n * nz)
     util.oracle().orc_uniform_f64(1, 0, noise.size, util.ptr(noise))  # same stream as the device inputs
     noise = noise.reshape(nz, n, n)
     smooth = (s[None, None, :] * s[None, :, None]) * s[:nz, None, None]
     ui = smooth + 1e-3 * noise
-    u = np.pad(ui, ((g, g), (g, g), (g, g)), mode="wrap")  # periodic ghost fill (of the slab when planes < n)
+    u = np.pad(ui, ((g, g), (g, g), (g, g)), mode="wrap")  # periodic ghost fill (of the slab when nz < n)
     u = np.ascontiguousarray(u.reshape(-1))
     f = np.zeros(n * n * nz)
     util.oracle().orc_uniform_f64(1, 1, f.size, util.ptr(f))
@@ -112,7 +115,8 @@ def cpu_baseline(n, g, planes):
     lo, hi = util.i3([0, 0, 0]), util.i3([n - 1, n - 1, nz - 1])
     h = 1.0 / n
     am, sm = ctypes.c_double(), ctypes.c_double()
-    t0 = time.perf_counter()
+    time.sleep(max(0.0, t_go - time.time()))
+    t0 = time.time()
     if lib is not None:
         for which in (0, 1):
             lib.ref_config2_step(which, util.ptr(u), util.ptr(ulo), util.ptr(uhi), util.ptr(f), util.ptr(r),
@@ -128,11 +132,42 @@ def cpu_baseline(n, g, planes):
                                ctypes.c_double(1.0 / (h * h)))
             o.orc_reduce_f64(2, util.ptr(r), r.size)
             o.orc_reduce_f64(3, util.ptr(r), r.size)
-    dt = time.perf_counter() - t0
+    t1 = time.time()
     nbytes = 8 * ((n + 2 * g) * (n + 2 * g) * (nz + 2 * g) + 3 * n * n * nz)
-    return {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
-            "sample": "config-2 step (7- and 27-point r = f - L(u), absMax, sum) on %dx%dx%d cells, "
-                      "%.2f s, Proto's CPU path single-threaded (Release flags)" % (n, n, nz, dt)}
+    return t0, t1, nbytes, kind
+
+
+def _cpu_slab_job(job):
+    return _cpu_slab(*job)
+
+
+def cpu_baseline(n, g, planes, workers=16):
+    """Proto's CPU path on a z-slab of `planes` planes of the same config-2
+    field, one thread; plus the whole field split into `workers` z-slabs run by
+    as many host processes at once (Proto's CPU path is single-threaded, so
+    cores are used one process per slab)."""
+    import concurrent.futures
+    import multiprocessing
+    t0, t1, nbytes, kind = _cpu_slab(n, g, planes)
+    dt = t1 - t0
+    out = {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
+           "sample": "config-2 step (7- and 27-point r = f - L(u), absMax, sum) on %dx%dx%d cells, "
+                     "%.2f s, Proto's CPU path single-threaded (Release flags)" % (n, n, planes, dt)}
+    workers = max(1, min(workers, os.cpu_count() or 1))
+    if workers > 1:
+        nz = max(1, planes // 4)
+        ctx = multiprocessing.get_context("spawn")  # fresh interpreters: no GPU state in the children
+        t_go = time.time() + 12.0  # every process has its inputs ready by then
+        with concurrent.futures.ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+            res = list(ex.map(_cpu_slab_job, [(n, g, nz, t_go)] * workers))
+        span = max(r[1] for r in res) - min(min(r[0] for r in res), t_go)
+        tot = sum(r[2] for r in res)
+        out["all_cores"] = {"value": round(tot / span / 1e9, 4), "unit": "GB/s", "cores": workers,
+                            "kind": res[0][3],
+                            "sample": "%d host processes, each the same step on its own %dx%dx%d z-slab "
+                                      "(%d planes in all), started together, %.2f s to the last end"
+                                      % (workers, n, n, nz, workers * nz, span)}
+    return out
 
 
 def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
