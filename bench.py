@@ -412,6 +412,9 @@ def main():
         print("bench.py: this build measures one GPU only (the path is a single-GPU kernel); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    # CPU baseline first, before this process touches the GPU (its all-cores
+    # leg starts host processes)
+    cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, 2, args.cpu_planes)
     import numpy as np
     import proto_amd as pa
     from proto_amd.device import DeviceArray, Event, Stream, gpu_available
@@ -627,8 +630,8 @@ def main():
         result.setdefault("extra", {})["multigrid_config3"] = mg_config3(pa, stream, args.mg_n, 64, 6 if args.mg_n == 512 else 3,
                                                                   use_graph=args.mg_graph)
 
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(n, g, args.cpu_planes)
+    if cpu is not None:
+        result["cpu_baseline"] = cpu
     print(json.dumps(result))
 
 
