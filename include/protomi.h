@@ -266,6 +266,15 @@ int pm_level_setval(const pm_layout* layout, int dtype, void* data, const int gh
 int pm_level_copy(const pm_layout* src_layout, const void* src, const int src_ghost[3],
                   const pm_layout* dst_layout, void* dst, const int dst_ghost[3], int dtype, int ncomp,
                   pm_stream_t stream);
+/* LevelBoxData::copyTo with Proto's regions (Proto_LevelBoxDataImplem.H:346-386):
+ * for compatible layouts (same domain, box size and patches) the ghosted box
+ * data is copied box by box over the common ghosted region and the destination
+ * then exchanged (copyToSimple); otherwise (LevelCopier, 795-870) every cell of
+ * every ghosted destination box that lies in a source patch, or whose periodic
+ * image does, takes the source's valid value, and all other cells are kept. */
+int pm_level_copy_to(const pm_layout* src_layout, const void* src, const int src_ghost[3],
+                     const pm_layout* dst_layout, void* dst, const int dst_ghost[3], int dtype, int ncomp,
+                     pm_stream_t stream);
 
 /* Residual of the correction scheme, r = (0 + rhs) + sum_j (c_j * (-1/dx^2)) phi(p+o_j)
  * (the order of coarseResidual, LevelMultigrid.cpp:64-66) on every valid cell,

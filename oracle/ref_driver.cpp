@@ -415,6 +415,48 @@ void ref_interp_apply_kind(int kind, int order, int ratio, const double* crse, c
     if (replace) { f |= I(c); } else { f += I(c); }
 }
 
+// interpBoundaries (Proto_LevelBoxDataImplem.H:953-990): a periodic coarse
+// level of 16^3 cells in 8^3 boxes (1 ghost) and a ratio-2 fine level on the
+// 32^3 domain covering the 27 patches {1,2,3}^3 of its 8^3-box grid (2
+// ghosts).  Coarse valid cells come from crse_g (16^3, x fastest), fine valid
+// cells from fine_g (32^3), every ghost cell starts at `fill`.  kind 0
+// Constant, 1 Linear, 2 Quadratic (ratio 2).  Output: for each fine box in
+// layout order its low corner (3 ints) and its ghosted data, x fastest.
+int ref_interp_boundaries(int kind, const double* crse_g, const double* fine_g, double fill, int* lows, double* out)
+{
+    Array<bool, DIM> per;
+    per.fill(true);
+    ProblemDomain cpd(Box::Cube(16), per), fpd(Box::Cube(32), per);
+    DisjointBoxLayout clayout(cpd, Point::Ones(8));
+    std::vector<Point> patches;
+    for (auto it = Box(Point::Ones(1), Point::Ones(3)).begin(); it != Box(Point::Ones(1), Point::Ones(3)).end(); ++it)
+    {
+        patches.push_back(*it);
+    }
+    DisjointBoxLayout flayout(fpd, patches, Point::Ones(8));
+    LevelBoxData<double> crse(clayout, Point::Ones(1)), fine(flayout, Point::Ones(2));
+    for (auto dit : clayout) { crse[dit].setVal(fill); }
+    for (auto dit : flayout) { fine[dit].setVal(fill); }
+    level_from_global(crse, crse_g, 16);
+    level_from_global(fine, fine_g, 32);
+    InterpStencil<double> I = kind == 0 ? InterpStencil<double>::Constant(Point::Ones(2))
+                            : (kind == 1 ? InterpStencil<double>::Linear(Point::Ones(2))
+                                         : InterpStencil<double>::Quadratic(Point::Ones(2)));
+    interpBoundaries(crse, fine, I);
+    int ib = 0;
+    long off = 0;
+    for (auto dit : flayout)
+    {
+        Box v = flayout[dit];
+        for (int d = 0; d < 3; d++) { lows[3 * ib + d] = (d < DIM) ? v.low()[d] : 0; }
+        const BoxData<double>& b = fine[dit];
+        std::memcpy(out + off, b.data(), b.box().size() * sizeof(double));
+        off += b.box().size();
+        ib++;
+    }
+    return ib;
+}
+
 // LevelBoxData::exchange on a periodic n^DIM domain tiled by boxSize^DIM boxes
 // with `ghost` ghost layers (Proto_LevelBoxDataImplem.H:313-341).  Valid cells
 // come from the global array g; ghost cells start at `fill`.  Output: for each

@@ -138,6 +138,7 @@ SIGNATURES = {
     "pm_level_prolong_add": (_i, [_vp, _vp, _i, _vp, _ip, _vp, _ip, _i, _vp]),
     "pm_level_setval": (_i, [_vp, _i, _vp, _ip, _i, _d, _vp]),
     "pm_level_copy": (_i, [_vp, _vp, _ip, _vp, _vp, _ip, _i, _i, _vp]),
+    "pm_level_copy_to": (_i, [_vp, _vp, _ip, _vp, _vp, _ip, _i, _i, _vp]),
     "pm_level_residual": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _d, _i, _vp, _ip, _vp, _vp]),
     "pm_level_add_correction": (_i, [_vp, _i, _vp, _ip, _i, _vp, _ip, _vp]),
     "pm_copy_device": (_i, [_vp, _vp, _sz, _i, _vp]),

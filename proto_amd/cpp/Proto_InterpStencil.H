@@ -7,6 +7,7 @@
 #pragma once
 
 #include <array>
+#include <cstdlib>
 
 #include "Proto_Stencil.H"
 
@@ -23,6 +24,24 @@ public:
     }
     int ratio() const { return m_r; }
     const Stencil<T>& get(int shift) const { return m_s[shift]; }
+    // union of the stencils' spans (Proto_InterpStencil1DImplem.H:36-46)
+    Box span() const
+    {
+        Box b;
+        for (auto& S : m_s) {
+            b = b & S.span().low();
+            b = b & S.span().high();
+        }
+        return b;
+    }
+    // per direction the largest |offset| of the span (48-57)
+    Point ghost() const
+    {
+        const Box b = span();
+        Point v;
+        for (int d = 0; d < DIM; d++) { v[d] = std::max(std::abs(b.low()[d]), std::abs(b.high()[d])); }
+        return v;
+    }
     void set(Stencil<T> S, int shift)
     {
         Point r = Point::Ones();
@@ -171,6 +190,17 @@ public:
         return Box(L, H);
     }
     Point ratio() const { return m_ratio; }
+    // the largest per-direction ghost of the 1D stencils
+    // (Proto_InterpStencilImplem.H:56-68)
+    Point ghost() const
+    {
+        Point g = m_interp[0].ghost();
+        for (int j = 1; j < DIM; j++) {
+            const Point h = m_interp[j].ghost();
+            for (int d = 0; d < DIM; d++) { g[d] = std::max(h[d], g[d]); }
+        }
+        return g;
+    }
 
     template<unsigned int C, MemType MEM, unsigned int D, unsigned int E>
     void apply(BoxData<T, C, MEM, D, E>& out, const BoxData<T, C, MEM, D, E>& in, Box range, bool overwrite,

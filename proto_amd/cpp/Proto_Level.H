@@ -89,6 +89,37 @@ public:
         c.m_impl->patches = m_impl->patches;
         return c;
     }
+    // layout index of a patch point or of its periodic image, -1 if none
+    // (DisjointBoxLayout::find, Proto_DisjointBoxLayoutImplem.H:262-275 with
+    // ProblemDomain::image of the patch domain)
+    int find(const Point& pt) const
+    {
+        const Box pb = m_impl->domain.box().coarsen(m_impl->boxSize);
+        Point q = pt;
+        for (int d = 0; d < DIM; d++) {
+            const int n = pb.size(d);
+            if (q[d] < pb.low()[d] || q[d] > pb.high()[d]) {
+                if (!m_impl->domain.isPeriodic(d)) { return -1; }
+                q[d] = pb.low()[d] + (((q[d] - pb.low()[d]) % n) + n) % n;
+            }
+        }
+        for (int i = 0; i < size(); i++) {
+            if (m_impl->patches[i] == q) { return i; }
+        }
+        return -1;
+    }
+    bool contains(const Point& pt) const { return find(pt) >= 0; }
+    // a patch with a missing neighbour patch among its 3^DIM - 1 neighbours
+    // (onLevelBoundary, Proto_DisjointBoxLayoutImplem.H:316-330)
+    bool onLevelBoundary(const Point& pt) const
+    {
+        const Box K = Box::Kernel(1).shift(pt);
+        for (auto it = K.begin(); it != K.end(); ++it) {
+            if (*it == pt) { continue; }
+            if (find(*it) < 0) { return true; }
+        }
+        return false;
+    }
     bool operator==(const DisjointBoxLayout& o) const { return m_impl == o.m_impl; }
     bool compatible(const DisjointBoxLayout& o) const
     {
@@ -183,13 +214,16 @@ public:
                  "LevelBoxData::setVal");
     }
     void setToZero() { setVal(T(0)); }
-    // copyTo another layout of the same domain: valid cells (343-386)
+    // copyTo another layout of the same domain (343-386): compatible layouts
+    // copy box by box over the common ghosted region and exchange the
+    // destination (copyToSimple); others fill every ghosted destination cell
+    // that a source patch (or its periodic image) covers (LevelCopier)
     void copyTo(LevelBoxData& dst) const
     {
         int gs[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
         for (int d = 0; d < DIM; d++) { gs[d] = m_ghost[d]; gd[d] = dst.m_ghost[d]; }
-        pm_check(pm_level_copy(m_layout.handle(), data(), gs, dst.m_layout.handle(), dst.data(), gd,
-                               pm_type<T>::value, C, protoCurrentStream()),
+        pm_check(pm_level_copy_to(m_layout.handle(), data(), gs, dst.m_layout.handle(), dst.data(), gd,
+                                  pm_type<T>::value, C, protoCurrentStream()),
                  "LevelBoxData::copyTo");
     }
     // LevelBoxData::reduce / absMax / sum / integrate over valid cells (595-678)
@@ -258,6 +292,37 @@ template<typename T, unsigned int C, MemType MEM>
 void averageDown(LevelBoxData<T, C, MEM>& crse, LevelBoxData<T, C, MEM>& fine, int ratio)
 {
     averageDown(crse, fine, Point::Ones(ratio));
+}
+
+// interpBoundaries (Proto_LevelBoxDataImplem.H:953-990): coarse data copied
+// onto the coarsened fine layout (with the ghosts the interpolation needs),
+// then on every fine patch at the level's boundary the InterpStencil output
+// overwrites the ghosted box and the valid cells are put back; finally the
+// fine level is exchanged, so ghosts facing other fine patches hold fine data.
+template<typename T, unsigned int C, MemType MEM>
+void interpBoundaries(LevelBoxData<T, C, MEM>& crse, LevelBoxData<T, C, MEM>& fine,
+                      LevelBoxData<T, C, MEM>& crseFine, InterpStencil<T>& interp)
+{
+    crse.exchange();
+    crse.copyTo(crseFine);
+    const DisjointBoxLayout& fl = fine.layout();
+    for (auto it : fl) {
+        if (!fl.onLevelBoundary(fl.point(it))) { continue; }
+        BoxData<T, C, MEM>& fine_i = fine[it];
+        BoxData<T, C, MEM>& crse_i = crseFine[it];
+        BoxData<T, C, MEM> fine_0(fl[it]);
+        fine_i.copyTo(fine_0);  // the valid cells
+        fine_i |= interp(crse_i);
+        fine_0.copyTo(fine_i);
+    }
+    fine.exchange();
+}
+template<typename T, unsigned int C, MemType MEM>
+void interpBoundaries(LevelBoxData<T, C, MEM>& crse, LevelBoxData<T, C, MEM>& fine, InterpStencil<T>& interp)
+{
+    const Point r = interp.ratio();
+    LevelBoxData<T, C, MEM> temp(fine.layout().coarsen(r), fine.ghost() / r + Point::Ones() + interp.ghost());
+    interpBoundaries(crse, fine, temp, interp);
 }
 
 }  // namespace Proto

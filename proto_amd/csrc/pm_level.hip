@@ -481,6 +481,49 @@ __global__ __launch_bounds__(256) void k_level_copy(LayoutDev S, G3 gs, const T*
     }
 }
 
+// LevelBoxData::copyTo with Proto's regions.  mode 0 (LevelCopier between
+// different layouts, Proto_LevelBoxDataImplem.H:795-870): every cell of each
+// destination box grown by its ghosts whose point (or its periodic image) lies
+// in a source patch takes that patch's valid value; other cells are left alone.
+// mode 1 (copyToSimple between compatible layouts, 346-356: BoxData::copyTo box
+// by box): every cell of the destination's ghosted box that also lies in the
+// same box's ghosted source box takes the source value at that cell; the
+// caller then exchanges the destination.
+template<typename T>
+__global__ __launch_bounds__(256) void k_level_copy_to(LayoutDev S, G3 gs, const T* __restrict__ src, LayoutDev D,
+                                                       G3 gd, T* __restrict__ dst, int ncomp, int mode)
+{
+    const int b = blockIdx.y;
+    const int* lo = D.lows + 3 * b;
+    const int X = D.bs[0] + 2 * gd.g[0], Y = D.bs[1] + 2 * gd.g[1], Z = D.bs[2] + 2 * gd.g[2];
+    const long n = (long)X * Y * Z;
+    const long dv = gvol(D, gd), sv = gvol(S, gs);
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int x = (int)(t % X) - gd.g[0];
+        const long r = t / X;
+        const int y = (int)(r % Y) - gd.g[1];
+        const int z = (int)(r / Y) - gd.g[2];
+        int o;
+        long si;
+        if (mode == 1) {
+            if (x < -gs.g[0] || x >= S.bs[0] + gs.g[0] || y < -gs.g[1] || y >= S.bs[1] + gs.g[1] ||
+                z < -gs.g[2] || z >= S.bs[2] + gs.g[2]) { continue; }
+            o = b;
+            si = rel_index(S, gs, x, y, z);
+        } else {
+            int p[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
+            o = owner(S, p);
+            if (o < 0) { continue; }
+            const int* slo = S.lows + 3 * o;
+            si = rel_index(S, gs, p[0] - slo[0], p[1] - slo[1], p[2] - slo[2]);
+        }
+        const long di = rel_index(D, gd, x, y, z);
+        for (int c = 0; c < ncomp; c++) {
+            dst[((long)b * ncomp + c) * dv + di] = src[((long)o * ncomp + c) * sv + si];
+        }
+    }
+}
+
 template<typename T>
 __global__ __launch_bounds__(256) void k_fill(T* __restrict__ data, long n, T v)
 {
@@ -929,6 +972,30 @@ int pm_level_copy(const pm_layout* SL, const void* src, const int src_ghost[3], 
     PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_level_copy<T>, grid, dim3(256), 0, S(stream), DS, g3(src_ghost),
                                             (const T*)src, DD, g3(dst_ghost), (T*)dst, ncomp));
     PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int pm_level_copy_to(const pm_layout* SL, const void* src, const int src_ghost[3], const pm_layout* DL, void* dst,
+                     const int dst_ghost[3], int dtype, int ncomp, pm_stream_t stream)
+{
+    PM_REQUIRE(SL && DL && src && dst && ncomp >= 1 && src_ghost && dst_ghost, "pm_level_copy_to: bad argument");
+    PM_REQUIRE(same_layout_shape(SL, DL), "pm_level_copy_to: layouts of different domains");
+    PM_REQUIRE(src != dst, "pm_level_copy_to: source and destination are the same array");
+    // Proto's DisjointBoxLayout::compatible plus equal box sizes: same domain,
+    // same patches in the same order
+    bool compatible = SL->nboxes == DL->nboxes && SL->lows == DL->lows;
+    for (int k = 0; k < 3; k++) {
+        compatible = compatible && SL->bs[k] == DL->bs[k] && SL->periodic[k] == DL->periodic[k];
+    }
+    const LayoutDev DS = ldev(SL), DD = ldev(DL);
+    const G3 gs = g3(src_ghost), gd = g3(dst_ghost);
+    const long n = gvol(DD, gd);
+    dim3 grid(bx_blocks(n), DL->nboxes);
+    const int mode = compatible ? 1 : 0;
+    PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_level_copy_to<T>, grid, dim3(256), 0, S(stream), DS, gs,
+                                            (const T*)src, DD, gd, (T*)dst, ncomp, mode));
+    PM_CHECK_LAUNCH();
+    if (compatible) { return pm_level_exchange(DL, dtype, dst, dst_ghost, ncomp, stream); }
     return PM_OK;
 }
 

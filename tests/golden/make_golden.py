@@ -333,6 +333,27 @@ def linear():
     np.savez_compressed(os.path.join(HERE, "linear.npz"), **out)
 
 
+def interp_bnd():
+    """interpBoundaries run by Proto (coarse periodic 16^3 in 8^3 boxes, ghost
+    1; ratio-2 fine level over patches {1,2,3}^3 of the 32^3 domain, ghost 2;
+    ghosts start at -7) with Constant, Linear and Quadratic InterpStencils, as
+    a binary fixture the C++ API test reads: int32 ncase, float64 crse (16^3)
+    and fine (32^3) valid inputs, then per case int32 [kind, nboxes], int32
+    lows[nboxes][3], float64 ghosted data of every fine box in layout order."""
+    R3 = util.ref(3)
+    crse = uniform(70, 0, 16 ** 3)
+    fine = uniform(70, 1, 32 ** 3)
+    buf = bytearray(np.array([3], dtype=np.int32).tobytes() + crse.tobytes() + fine.tobytes())
+    for kind in (0, 1, 2):
+        lows = np.zeros(3 * 27, dtype=np.int32)
+        out = np.zeros(27 * 12 ** 3)
+        nb = R3.ref_interp_boundaries(kind, ptr(crse), ptr(fine), ctypes.c_double(-7.0), ptr(lows), ptr(out))
+        assert nb == 27
+        buf += np.array([kind, nb], dtype=np.int32).tobytes() + lows.tobytes() + out.tobytes()
+    with open(os.path.join(HERE, "interp_bnd.bin"), "wb") as f:
+        f.write(bytes(buf))
+
+
 if __name__ == "__main__":
     if util.ref(3) is None or util.ref(2) is None:
         sys.exit("build the reference checker first: make -C oracle ref")
@@ -344,6 +365,7 @@ if __name__ == "__main__":
     multigrid()
     linear()
     interp_hi()
+    interp_bnd()
     config1()
     config2_small()
     print("golden fixtures written to", HERE)

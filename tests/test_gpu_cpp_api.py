@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(util.ROOT, "tests", "cpp", "bin")
 
 
-ARGS = {"test_api_interp": [os.path.join(util.GOLDEN, "interp_hi.bin")]}
+ARGS = {"test_api_interp": [os.path.join(util.GOLDEN, "interp_hi.bin"), os.path.join(util.GOLDEN, "interp_bnd.bin")]}
 
 
 @pytest.mark.parametrize("name", ["test_api_stencil", "test_api_level", "test_api_interp"])

@@ -335,3 +335,37 @@ def test_every_tile_configuration_is_bit_identical(dtype):
     got = pa.TileConfig()
     pa.call("pm_stencil_get_tile_config", s7.h, s27.h, 3, ctypes.byref(ib), ctypes.byref(got))
     assert got.as_dict() == best.as_dict() and ms.value > 0
+
+
+@pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
+def test_residual_reductions_every_call(dtype):
+    """Back-to-back residual calls (the reduction output cleared in between)
+    each rewrite absMax and sum, agree with a host reduction of the residual
+    arrays they wrote, and give the same bits every time."""
+    n, g = 48, 2
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    u, f = config2_inputs(n, g, npdt)
+    h = 1.0 / n
+    c7, o7, _, _, _ = util.lib_stencil("Laplacian", dtype=npdt)
+    c27, o27, _, _, _ = util.lib_stencil("Laplacian_27", dtype=npdt)
+    s7, s27 = pa.Stencil(c7, o7, dtype), pa.Stencil(c27, o27, dtype)
+    ub, rb = box([-g] * 3, [n - 1 + g] * 3), box([0] * 3, [n - 1] * 3)
+    r1, r2 = empty(n ** 3, npdt), empty(n ** 3, npdt)
+    red, red1 = zeros(4, npdt), zeros(2, npdt)
+    sc = 1.0 / (h * h)
+    first = None
+    for it in range(3):
+        pa.call("pm_memset_zero", vp(red), 4 * np.dtype(npdt).itemsize, None)
+        pa.call("pm_memset_zero", vp(red1), 2 * np.dtype(npdt).itemsize, None)
+        pa.call("pm_stencil_residual2", s7.h, s27.h, vp(u), ctypes.byref(ub), vp(f), vp(r1), vp(r2),
+                ctypes.byref(rb), sc, sc, vp(red), None)
+        pa.call("pm_stencil_residual", s27.h, vp(u), ctypes.byref(ub), vp(f), vp(r2), ctypes.byref(rb), sc,
+                vp(red1), None)
+        R, R1 = host(red), host(red1)
+        a, b = host(r1).astype(np.float64), host(r2).astype(np.float64)
+        assert R[0] == np.abs(a).max() and R[2] == np.abs(b).max() and R1[0] == R[2]
+        tol = n ** 3 * np.finfo(npdt).eps
+        assert abs(R[1] - a.sum()) <= tol * np.abs(a).sum() and abs(R[3] - b.sum()) <= tol * np.abs(b).sum()
+        if first is None:
+            first = R.tobytes()
+        assert R.tobytes() == first  # deterministic

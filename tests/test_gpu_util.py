@@ -185,3 +185,42 @@ def test_stream_probe_kinds():
     assert pa.lib().pm_stream_probe(3, vp(dst), vp(src), a.nbytes, 4, None) == -1
     assert pa.lib().pm_stream_probe(2, vp(dst), vp(src), a.nbytes - 4, 4, None) == -1
     assert pa.lib().pm_stream_probe(2, vp(dst), vp(src), a.nbytes, 65, None) == -1
+
+
+def test_level_copy_to_regions():
+    """LevelBoxData::copyTo regions (Proto_LevelBoxDataImplem.H:346-386,
+    795-870) against a numpy model.  Different layouts: every ghosted
+    destination cell covered by a source patch (periodic images included)
+    takes the source's valid value, uncovered cells keep theirs.  Compatible
+    layouts: box-by-box copy of the common ghosted region, then exchange."""
+    n = 16
+    glob = util.uniform(94, 0, n ** 3)
+    G = glob.reshape(n, n, n)
+    # source: 8^3 boxes, only the patches with x-patch index 0 (half the domain)
+    S = Layout(n, 8, lows=[(0, y, z) for z in (0, 8) for y in (0, 8)])
+    src = dev(S.pack(glob, 1))
+    D = Layout(n, 4)
+    gd = 2
+    e = 4 + 2 * gd
+    old = np.full(len(D.lows) * e ** 3, -5.0)
+    dst = dev(old)
+    pa.call("pm_level_copy_to", S.h, vp(src), G1, D.h, vp(dst), pa.i3([gd] * 3), pa.PM_F64, 1, None)
+    got = host(dst).reshape(len(D.lows), e, e, e)
+    want = old.reshape(len(D.lows), e, e, e).copy()
+    for b, lo in enumerate(D.lows):
+        for k in range(e):
+            for j in range(e):
+                for i in range(e):
+                    x, y, z = ((lo[0] - gd + i) % n, (lo[1] - gd + j) % n, (lo[2] - gd + k) % n)
+                    if x < 8:
+                        want[b, k, j, i] = G[z, y, x]
+    assert got.tobytes() == want.tobytes()
+    # compatible layouts: ghost 1 source (ghosts hold a marker), ghost 2 destination
+    A = Layout(n, 8)
+    srcA = A.pack(glob, 1).reshape(len(A.lows), 10, 10, 10)
+    srcA[:, 0] = 3.0  # marker on one ghost face; exchange must overwrite it in the destination
+    srcA_d = dev(srcA.reshape(-1))
+    dstA = dev(np.full(len(A.lows) * 12 ** 3, -5.0))
+    pa.call("pm_level_copy_to", A.h, vp(srcA_d), G1, A.h, vp(dstA), pa.i3([2] * 3), pa.PM_F64, 1, None)
+    assert host(dstA).tobytes() == A.pack(glob, 2).tobytes()
+    assert pa.lib().pm_level_copy_to(A.h, vp(srcA_d), G1, A.h, vp(srcA_d), G1, pa.PM_F64, 1, None) == -1
