@@ -106,6 +106,153 @@ __global__ __launch_bounds__(256) void k_fas_relax(LayoutDev L, G3 gu, G3 gf, co
     }
 }
 
+// The same operator on boxes 32..64 cells wide, marching z: a workgroup owns
+// a 64 x 8 tile (4 wave64s, 2 rows each) of one box over a z range.  Each
+// plane of u -- the tile with 2 halo cells on every side, all inside the box's
+// own 2-ghost storage -- is staged once in a 6-slot LDS ring, so every one of
+// the 13 taps is an LDS read (one barrier per plane: a slot is rewritten only
+// after the step that last read it has passed the next barrier).  Loads of
+// the next plane are issued before each plane is computed; results are stored
+// at the start of the next step, after those loads.  The arithmetic is
+// fas_L's, term by term.  MODE 0: Jacobi relaxation out = u + (-lambda)(L(u)+f);
+// MODE 1: per-block max |L(u) + f| (and the residual itself if res_out).
+constexpr int FZ_TY = 8, FZ_NE = (FZ_TY + 4) * 68;
+
+template<int MODE>
+__global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __restrict__ u,
+                                                  const double* __restrict__ f, double* __restrict__ out, FasC c,
+                                                  double* __restrict__ part, double* __restrict__ res_out, int zc)
+{
+    constexpr int RY = 2, NL = (FZ_NE + 255) / 256;
+    __shared__ double sm[6][FZ_NE];
+    __shared__ double red[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int b = blockIdx.y;
+    const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
+    const int nyt = (ny + FZ_TY - 1) / FZ_TY;
+    const int y0 = (int)(blockIdx.x % nyt) * FZ_TY;
+    const int z0 = (int)(blockIdx.x / nyt) * zc, z1 = min(z0 + zc, nz);
+    const int LX = nx + 4, NE = (FZ_TY + 4) * LX;
+    const long X = nx + 4, XY = X * (ny + 4), vol = XY * (nz + 4);
+    const double* ub = u + (long)b * vol;
+    const long nv = (long)nx * ny * nz;
+    const double* fb = f + (long)b * nv;
+    // staged element e: storage row y0 + e / LX (clamped to the last ghost
+    // row), storage column e % LX; elements past the plane repeat the last one
+    long goff[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = min(tid + 256 * k, NE - 1);
+        const int er = e / LX, ec = e - er * LX;
+        goff[k] = X * (long)min(y0 + er, ny + 3) + ec;
+    }
+    double P[NL];
+    auto gload = [&](int z) {
+        const double* base = ub + XY * (long)(z + 2);
+#pragma unroll
+        for (int k = 0; k < NL; k++) { P[k] = base[goff[k]]; }
+    };
+    auto lstore = [&](double* pl) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const int e = tid + 256 * k;
+            if (e < NE) { pl[e] = P[k]; }
+        }
+    };
+    const int x = lane;
+    const bool xok = x < nx;
+    const int xc = xok ? x : 0;  // idle lanes read column 0 (results never stored)
+    double fq[RY], res[RY];
+    int zres = -1;
+    auto fload = [&](int zo) {
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int y = min(y0 + w * RY + r, ny - 1);
+            fq[r] = fb[xc + (long)nx * (y + (long)ny * zo)];
+        }
+    };
+    double m = 0;
+    auto put_res = [&]() {
+        if (zres < 0 || !xok) { return; }
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int y = y0 + w * RY + r;
+            if (y >= ny) { continue; }
+            if (MODE == 0) {
+                out[(long)b * vol + XY * (zres + 2) + X * (y + 2) + (x + 2)] = res[r];
+            } else if (res_out) {
+                res_out[(long)b * nv + x + (long)nx * (y + (long)ny * zres)] = res[r];
+            }
+        }
+    };
+    const int kb = z0 - 2;  // first staged plane
+    gload(kb);
+    fload(z0);
+    for (int zz = kb; zz <= z1 + 1; zz++) {
+        lstore(sm[(zz - kb) % 6]);
+        if (zz + 1 <= z1 + 1) { gload(zz + 1); }
+        put_res();
+        zres = -1;
+        __syncthreads();
+        if (zz < z0 + 2) { continue; }
+        const int zo = zz - 2;
+        const double* s0 = sm[(zo - kb) % 6];
+        const double* sm2 = sm[(zo - 2 - kb) % 6];
+        const double* sm1 = sm[(zo - 1 - kb) % 6];
+        const double* sp1 = sm[(zo + 1 - kb) % 6];
+        const double* sp2 = sm[(zo + 2 - kb) % 6];
+        double fcur[RY];
+#pragma unroll
+        for (int r = 0; r < RY; r++) { fcur[r] = fq[r]; }
+        if (zo + 1 < z1) { fload(zo + 1); }
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int li = (w * RY + r + 2) * LX + (xc + 2);
+            double acc = 0;
+            double fr, fl;
+            // x
+            fr = 0; fr = fr + c.a[0] * s0[li - 1]; fr = fr + c.a[1] * s0[li]; fr = fr + c.a[2] * s0[li + 1];
+            fr = fr + c.a[3] * s0[li + 2];
+            fl = 0; fl = fl + c.a[0] * s0[li - 2]; fl = fl + c.a[1] * s0[li - 1]; fl = fl + c.a[2] * s0[li];
+            fl = fl + c.a[3] * s0[li + 1];
+            acc = acc + c.dp * fr;
+            acc = acc + c.dm * fl;
+            // y
+            fr = 0; fr = fr + c.a[0] * s0[li - LX]; fr = fr + c.a[1] * s0[li]; fr = fr + c.a[2] * s0[li + LX];
+            fr = fr + c.a[3] * s0[li + 2 * LX];
+            fl = 0; fl = fl + c.a[0] * s0[li - 2 * LX]; fl = fl + c.a[1] * s0[li - LX]; fl = fl + c.a[2] * s0[li];
+            fl = fl + c.a[3] * s0[li + LX];
+            acc = acc + c.dp * fr;
+            acc = acc + c.dm * fl;
+            // z
+            fr = 0; fr = fr + c.a[0] * sm1[li]; fr = fr + c.a[1] * s0[li]; fr = fr + c.a[2] * sp1[li];
+            fr = fr + c.a[3] * sp2[li];
+            fl = 0; fl = fl + c.a[0] * sm2[li]; fl = fl + c.a[1] * sm1[li]; fl = fl + c.a[2] * s0[li];
+            fl = fl + c.a[3] * sp1[li];
+            acc = acc + c.dp * fr;
+            acc = acc + c.dm * fl;
+            double o = 0;
+            o = o + acc;
+            double rv = o * -1.0;
+            rv = rv + fcur[r];
+            if (MODE == 0) {
+                res[r] = s0[li] + c.mlam * rv;
+            } else {
+                res[r] = rv;
+                const bool ok = xok && (y0 + w * RY + r) < ny;
+                const double a = rv > 0 ? rv : -rv;
+                m = (ok && m < a) ? a : m;
+            }
+        }
+        zres = zo;
+    }
+    put_res();
+    if (MODE == 1) {
+        m = block_reduce(PM_RED_MAX, m, red);
+        if (tid == 0) { part[(long)b * gridDim.x + blockIdx.x] = m; }
+    }
+}
+
 // per-block max |L(u) + f|
 __global__ __launch_bounds__(256) void k_fas_resnorm(LayoutDev L, G3 gu, G3 gf, const double* __restrict__ u,
                                                      const double* __restrict__ f, FasC c, double* __restrict__ part,
@@ -238,6 +385,20 @@ static unsigned fas_blocks(long n) { return (unsigned)std::min<long>((n + 255) /
 
 static const int G2[3] = {2, 2, 2}, G0[3] = {0, 0, 0};
 
+// boxes the z-marching operator kernel takes: 32..64 cells wide, at least a tile
+// tall and deep
+static bool fz_ok(const LayoutDev& D) { return D.bs[0] >= 32 && D.bs[0] <= 64 && D.bs[1] >= 8 && D.bs[2] >= 8; }
+
+// its grid: 8-row tiles x z chunks (halved while fewer than 2048 workgroups) x boxes
+static dim3 fz_grid(const LayoutDev& D, int* zc)
+{
+    const int nyt = (D.bs[1] + FZ_TY - 1) / FZ_TY;
+    int z = D.bs[2];
+    while (z > 8 && (long)nyt * ((D.bs[2] + z - 1) / z) * D.nboxes < 2048) { z = (z + 1) / 2; }
+    *zc = z;
+    return dim3((unsigned)(nyt * ((D.bs[2] + z - 1) / z)), (unsigned)D.nboxes);
+}
+
 static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_t s)
 {
     auto& Lv = M->lv[l];
@@ -245,10 +406,18 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
     const FasC c = fas_coefs(Lv.dx);
     const long cells = (long)D.bs[0] * D.bs[1] * D.bs[2];
     void* buf[2] = {u, Lv.tmp};
+    int zc = 0;
+    const dim3 zgrid = fz_grid(D, &zc);
     for (int it = 0; it < n; it++) {
         FAS_TRY(pm_level_exchange(Lv.layout, PM_F64, buf[it & 1], G2, 1, s));
-        hipLaunchKernelGGL(k_fas_relax, dim3(fas_blocks(cells), D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
-                           (const double*)buf[it & 1], (const double*)f, (double*)buf[(it + 1) & 1], c);
+        if (fz_ok(D)) {
+            hipLaunchKernelGGL(k_fas_op_z<0>, zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
+                               (const double*)f, (double*)buf[(it + 1) & 1], c, (double*)nullptr, (double*)nullptr,
+                               zc);
+        } else {
+            hipLaunchKernelGGL(k_fas_relax, dim3(fas_blocks(cells), D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
+                               (const double*)buf[it & 1], (const double*)f, (double*)buf[(it + 1) & 1], c);
+        }
         PM_CHECK_LAUNCH();
     }
     if (n & 1) {
@@ -288,9 +457,17 @@ static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream
     const LayoutDev D = ldev(Lv.layout);
     FAS_TRY(pm_level_exchange(Lv.layout, PM_F64, u, G2, 1, s));
     const long cells = (long)D.bs[0] * D.bs[1] * D.bs[2];
-    const unsigned bpb = fas_blocks(cells);
-    hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0), (const double*)u,
-                       (const double*)f, fas_coefs(Lv.dx), M->d_part, res_out);
+    unsigned bpb = fas_blocks(cells);
+    if (fz_ok(D)) {
+        int zc = 0;
+        const dim3 zgrid = fz_grid(D, &zc);
+        bpb = zgrid.x;  // no more partials than the generic kernel's (<= 64 per box)
+        hipLaunchKernelGGL(k_fas_op_z<1>, zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
+                           (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc);
+    } else {
+        hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
+                           (const double*)u, (const double*)f, fas_coefs(Lv.dx), M->d_part, res_out);
+    }
     hipLaunchKernelGGL(k_fas_max, dim3(1), dim3(1024), 0, s, (const double*)M->d_part, (long)bpb * D.nboxes,
                        M->d_norm);
     PM_CHECK_LAUNCH();

@@ -355,7 +355,7 @@ def test_fas_matches_proto():
     assert hashlib.sha256(phi.tobytes()).hexdigest() == meta["fas"]["phi_sha256"]
 
 
-@pytest.mark.parametrize("n,bs,levels", [(64, 32, 3), (48, 16, 3), (32, 8, 4)])
+@pytest.mark.parametrize("n,bs,levels", [(64, 32, 3), (72, 36, 3), (48, 16, 3), (32, 8, 4)])
 def test_fas_vs_oracle(n, bs, levels):
     say("fas vs oracle", n, bs, levels)
     force = util.gaussian_rhs(n, 5, 17)
