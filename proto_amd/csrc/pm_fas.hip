@@ -107,24 +107,27 @@ __global__ __launch_bounds__(256) void k_fas_relax(LayoutDev L, G3 gu, G3 gf, co
 }
 
 // The same operator on boxes 32..64 cells wide, marching z: a workgroup owns
-// a 64 x 8 tile (4 wave64s, 2 rows each) of one box over a z range.  Each
+// a 64 x 16 tile (4 wave64s, 4 rows each) of one box over a z range.  Each
 // plane of u -- the tile with 2 halo cells on every side, all inside the box's
-// own 2-ghost storage -- is staged once in a 6-slot LDS ring, so every one of
-// the 13 taps is an LDS read (one barrier per plane: a slot is rewritten only
-// after the step that last read it has passed the next barrier).  Loads of
-// the next plane are issued before each plane is computed; results are stored
-// at the start of the next step, after those loads.  The arithmetic is
-// fas_L's, term by term.  MODE 0: Jacobi relaxation out = u + (-lambda)(L(u)+f);
-// MODE 1: per-block max |L(u) + f| (and the residual itself if res_out).
-constexpr int FZ_TY = 8, FZ_NE = (FZ_TY + 4) * 68;
+// own 2-ghost storage -- is staged once in LDS (a ring of 4 planes, one
+// barrier per plane: plane p is last read in step p + 2 and its slot is
+// rewritten in step p + 4, after every thread passed the barrier of step
+// p + 3); each thread copies its own cells of every plane into a
+// 5-plane register window for the z taps and reads the x and y taps of the
+// output plane from LDS.  Loads of the next plane are issued before each plane
+// is computed; results are stored at the start of the next step, after those
+// loads.  The arithmetic is fas_L's, term by term.  MODE 0: Jacobi relaxation
+// out = u + (-lambda)(L(u) + f); MODE 1: per-block max |L(u) + f| (and the
+// residual itself if res_out).
+constexpr int FZ_RY = 4, FZ_TY = 4 * FZ_RY, FZ_NE = (FZ_TY + 4) * 68;
 
 template<int MODE>
 __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __restrict__ u,
                                                   const double* __restrict__ f, double* __restrict__ out, FasC c,
                                                   double* __restrict__ part, double* __restrict__ res_out, int zc)
 {
-    constexpr int RY = 2, NL = (FZ_NE + 255) / 256;
-    __shared__ double sm[6][FZ_NE];
+    constexpr int RY = FZ_RY, NL = (FZ_NE + 255) / 256;
+    __shared__ double sm[4][FZ_NE];
     __shared__ double red[16];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int b = blockIdx.y;
@@ -162,6 +165,7 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
     const int x = lane;
     const bool xok = x < nx;
     const int xc = xok ? x : 0;  // idle lanes read column 0 (results never stored)
+    double zw[5][RY];             // own cells of planes zo-2 .. zo+2
     double fq[RY], res[RY];
     int zres = -1;
     auto fload = [&](int zo) {
@@ -189,18 +193,22 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
     gload(kb);
     fload(z0);
     for (int zz = kb; zz <= z1 + 1; zz++) {
-        lstore(sm[(zz - kb) % 6]);
+        double* sl = sm[(zz - kb) % 4];
+        lstore(sl);
         if (zz + 1 <= z1 + 1) { gload(zz + 1); }
         put_res();
         zres = -1;
         __syncthreads();
+        // shift the register window and take this thread's cells of plane zz
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) { zw[q][r] = zw[q + 1][r]; }
+            zw[4][r] = sl[(w * RY + r + 2) * LX + (xc + 2)];
+        }
         if (zz < z0 + 2) { continue; }
         const int zo = zz - 2;
-        const double* s0 = sm[(zo - kb) % 6];
-        const double* sm2 = sm[(zo - 2 - kb) % 6];
-        const double* sm1 = sm[(zo - 1 - kb) % 6];
-        const double* sp1 = sm[(zo + 1 - kb) % 6];
-        const double* sp2 = sm[(zo + 2 - kb) % 6];
+        const double* s0 = sm[(zo - kb) % 4];
         double fcur[RY];
 #pragma unroll
         for (int r = 0; r < RY; r++) { fcur[r] = fq[r]; }
@@ -208,27 +216,28 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
 #pragma unroll
         for (int r = 0; r < RY; r++) {
             const int li = (w * RY + r + 2) * LX + (xc + 2);
+            const double uc = zw[2][r];
             double acc = 0;
             double fr, fl;
             // x
-            fr = 0; fr = fr + c.a[0] * s0[li - 1]; fr = fr + c.a[1] * s0[li]; fr = fr + c.a[2] * s0[li + 1];
+            fr = 0; fr = fr + c.a[0] * s0[li - 1]; fr = fr + c.a[1] * uc; fr = fr + c.a[2] * s0[li + 1];
             fr = fr + c.a[3] * s0[li + 2];
-            fl = 0; fl = fl + c.a[0] * s0[li - 2]; fl = fl + c.a[1] * s0[li - 1]; fl = fl + c.a[2] * s0[li];
+            fl = 0; fl = fl + c.a[0] * s0[li - 2]; fl = fl + c.a[1] * s0[li - 1]; fl = fl + c.a[2] * uc;
             fl = fl + c.a[3] * s0[li + 1];
             acc = acc + c.dp * fr;
             acc = acc + c.dm * fl;
             // y
-            fr = 0; fr = fr + c.a[0] * s0[li - LX]; fr = fr + c.a[1] * s0[li]; fr = fr + c.a[2] * s0[li + LX];
+            fr = 0; fr = fr + c.a[0] * s0[li - LX]; fr = fr + c.a[1] * uc; fr = fr + c.a[2] * s0[li + LX];
             fr = fr + c.a[3] * s0[li + 2 * LX];
-            fl = 0; fl = fl + c.a[0] * s0[li - 2 * LX]; fl = fl + c.a[1] * s0[li - LX]; fl = fl + c.a[2] * s0[li];
+            fl = 0; fl = fl + c.a[0] * s0[li - 2 * LX]; fl = fl + c.a[1] * s0[li - LX]; fl = fl + c.a[2] * uc;
             fl = fl + c.a[3] * s0[li + LX];
             acc = acc + c.dp * fr;
             acc = acc + c.dm * fl;
             // z
-            fr = 0; fr = fr + c.a[0] * sm1[li]; fr = fr + c.a[1] * s0[li]; fr = fr + c.a[2] * sp1[li];
-            fr = fr + c.a[3] * sp2[li];
-            fl = 0; fl = fl + c.a[0] * sm2[li]; fl = fl + c.a[1] * sm1[li]; fl = fl + c.a[2] * s0[li];
-            fl = fl + c.a[3] * sp1[li];
+            fr = 0; fr = fr + c.a[0] * zw[1][r]; fr = fr + c.a[1] * uc; fr = fr + c.a[2] * zw[3][r];
+            fr = fr + c.a[3] * zw[4][r];
+            fl = 0; fl = fl + c.a[0] * zw[0][r]; fl = fl + c.a[1] * zw[1][r]; fl = fl + c.a[2] * uc;
+            fl = fl + c.a[3] * zw[3][r];
             acc = acc + c.dp * fr;
             acc = acc + c.dm * fl;
             double o = 0;
@@ -236,7 +245,7 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
             double rv = o * -1.0;
             rv = rv + fcur[r];
             if (MODE == 0) {
-                res[r] = s0[li] + c.mlam * rv;
+                res[r] = uc + c.mlam * rv;
             } else {
                 res[r] = rv;
                 const bool ok = xok && (y0 + w * RY + r) < ny;
@@ -387,9 +396,9 @@ static const int G2[3] = {2, 2, 2}, G0[3] = {0, 0, 0};
 
 // boxes the z-marching operator kernel takes: 32..64 cells wide, at least a tile
 // tall and deep
-static bool fz_ok(const LayoutDev& D) { return D.bs[0] >= 32 && D.bs[0] <= 64 && D.bs[1] >= 8 && D.bs[2] >= 8; }
+static bool fz_ok(const LayoutDev& D) { return D.bs[0] >= 32 && D.bs[0] <= 64 && D.bs[1] >= 16 && D.bs[2] >= 8; }
 
-// its grid: 8-row tiles x z chunks (halved while fewer than 2048 workgroups) x boxes
+// its grid: 16-row tiles x z chunks (halved while fewer than 2048 workgroups) x boxes
 static dim3 fz_grid(const LayoutDev& D, int* zc)
 {
     const int nyt = (D.bs[1] + FZ_TY - 1) / FZ_TY;
