@@ -41,15 +41,19 @@ struct SLevel {
     float* rhs;
     int n;
     int rows;           // rows per sweep tile
+    int tw;             // columns per sweep tile
     float c0, c1, lam;  // centre / neighbour coefficients, GSRB weight
 };
 
-// rows per sweep tile of a level n cells wide, and the LDS of its tile
-__host__ __device__ inline int sweep_rows(int n) { return n > 128 ? 4 : SR; }
+// rows and columns per sweep tile of a level n cells wide (levels wider than
+// 128 cells split in two column halves, so that 8-row tiles fit the LDS), and
+// the LDS of its tile
+__host__ __device__ inline int sweep_rows(int) { return SR; }
+__host__ __device__ inline int sweep_cols(int n) { return n > 128 ? (n + 1) / 2 : n; }
 inline size_t sweep_lds(int n)
 {
-    const int r = sweep_rows(n);
-    return sizeof(float) * (4 * (size_t)(r + 4) * (n + 4) + 2 * (size_t)(r + 2) * (n + 2));
+    const int r = sweep_rows(n), w = sweep_cols(n);
+    return sizeof(float) * (4 * (size_t)(r + 4) * (w + 4) + 2 * (size_t)(r + 2) * (w + 2));
 }
 
 struct SReq {
@@ -65,6 +69,7 @@ struct SReq {
 struct SItem {
     int lev;  // SLevel index (sweep, restrict: fine level, prolong: fine level) or SReq index
     int a, b, c;
+    int x0, x1;  // sweep tiles: columns [x0, x1)
 };
 
 __device__ inline int wrapi(int v, int n)
@@ -84,11 +89,11 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     extern __shared__ __align__(16) unsigned char smraw[];
     const SItem it = items[blockIdx.x];
     const SLevel L = lv[it.lev];
-    const int n = L.n, W = n + 4, WR = n + 2;
+    const int n = L.n, x0 = it.x0, x1 = it.x1, W = x1 - x0 + 4, WR = x1 - x0 + 2;
     const int y0 = it.a, y1 = min(y0 + L.rows, n), z0 = it.b, z1 = it.c;
     const int NP = (y1 - y0 + 4) * W, NR = (y1 - y0 + 2) * WR;
     float* sm = reinterpret_cast<float*>(smraw);
-    float* rsm = sm + 4 * (L.rows + 4) * W;
+    float* rsm = sm + 4 * (L.rows + 4) * (L.tw + 4);
     const float* in = dir ? L.tmp : L.phi;
     float* out = dir ? L.phi : L.tmp;
     const long nn = (long)n * n;
@@ -101,9 +106,9 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
 #pragma unroll
     for (int i = 0; i < MAXE; i++) {
         const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
-        offp[i] = wrapi(ec - 2, n) + n * wrapi(y0 - 2 + er, n);
+        offp[i] = wrapi(x0 + ec - 2, n) + n * wrapi(y0 - 2 + er, n);
         const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
-        offr[i] = wrapi(fc - 1, n) + n * wrapi(y0 - 1 + fr, n);
+        offr[i] = wrapi(x0 + fc - 1, n) + n * wrapi(y0 - 1 + fr, n);
     }
     float pre[MAXE], prr[MAXE];
     auto issue = [&](int z) {
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
             const int yy = ya + c / hw;
             const int xx = xa + 2 * (c % hw) + ((xa + yy + z + colour) & 1);
             if (xx >= xb) { continue; }
-            const int li = (yy - (y0 - 2)) * W + (xx + 2);
+            const int li = (yy - (y0 - 2)) * W + (xx - x0 + 2);
             float s = 0;
             s = s + L.c0 * zc[li];
             s = s + L.c1 * zc[li + 1];
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
             s = s + L.c1 * zc[li - W];
             s = s + L.c1 * zp[li];
             s = s + L.c1 * zm[li];
-            s = s + rz[(yy - (y0 - 1)) * WR + (xx + 1)];
+            s = s + rz[(yy - (y0 - 1)) * WR + (xx - x0 + 1)];
             zc[li] = zc[li] + L.lam * s;
         }
     };
@@ -166,10 +171,10 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     auto write_out = [&](int z) {
         const float* pl = slot(z);
         float* ob = out + nn * z;
-        const int cnt = (y1 - y0) * n;
+        const int tw = x1 - x0, cnt = (y1 - y0) * tw;
         for (int e = threadIdx.x; e < cnt; e += 256) {
-            const int r = e / n, xx = e - r * n;
-            ob[xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
+            const int r = e / tw, xx = e - r * tw;
+            ob[x0 + xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
         }
     };
     // all memory operations of a step are issued at its start (loads of the
@@ -182,9 +187,9 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
         if (k + 2 <= z1) { issue_r(k + 2); }
         if (k - 1 >= z0) { write_out(k - 1); }
         __syncthreads();
-        update(k + 1, y0 - 1, y1 + 1, -1, n + 1, 0);  // red, one-cell halo included
+        update(k + 1, y0 - 1, y1 + 1, x0 - 1, x1 + 1, 0);  // red, one-cell halo included
         __syncthreads();
-        if (k >= z0) { update(k, y0, y1, 0, n, 1); }  // black, tile cells
+        if (k >= z0) { update(k, y0, y1, x0, x1, 1); }  // black, tile cells
         __syncthreads();
     }
     write_out(z1 - 1);
@@ -576,6 +581,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             L.rhs = a.h->rhs[l];
             L.n = a.ns[l];
             L.rows = sweep_rows(a.ns[l]);
+            L.tw = sweep_cols(a.ns[l]);
             level_coefs(a.ns[l], &L.c0, &L.c1, &L.lam);
             lv.push_back(L);
         }
@@ -603,12 +609,15 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
         if (L.count == 0) { S->plan.pop_back(); }
     };
     auto sweep_items = [&](int levidx, int m) {
-        const int r = sweep_rows(m);
+        const int r = sweep_rows(m), w = sweep_cols(m);
         for (int z0 = 0; z0 < m; z0 += SZC)
-            for (int y0 = 0; y0 < m; y0 += r) { items.push_back(SItem{levidx, y0, z0, std::min(m, z0 + SZC)}); }
+            for (int y0 = 0; y0 < m; y0 += r)
+                for (int x0 = 0; x0 < m; x0 += w) {
+                    items.push_back(SItem{levidx, y0, z0, std::min(m, z0 + SZC), x0, std::min(m, x0 + w)});
+                }
         Launch& L = S->plan.back();
         L.lds = std::max(L.lds, sweep_lds(m));
-        L.np = std::max(L.np, (long)(r + 4) * (m + 4));
+        L.np = std::max(L.np, (long)(r + 4) * (w + 4));
     };
     const int pre = S->opt.pre, post = S->opt.post, bot = S->opt.bottom;
     // V-cycle of every non-fresh request (fresh ones only get their first residual)

@@ -56,7 +56,7 @@ def req(n, ngauss, seed, t=0.0, tol=1e-8, keep=1):
     return {"arrival_s": t, "n": n, "ngauss": ngauss, "seed": seed, "tol": tol, "keep_result": keep}
 
 
-@pytest.mark.parametrize("n,ngauss,seed", [(32, 5, 11), (48, 9, 12), (64, 16, 13)])
+@pytest.mark.parametrize("n,ngauss,seed", [(32, 5, 11), (48, 9, 12), (64, 16, 13), (192, 6, 15), (256, 12, 14)])
 def test_served_equals_driver_fp32_ir(n, ngauss, seed):
     srv = pa.Server(mode=0)
     st = srv.run([req(n, ngauss, seed)])
