@@ -170,6 +170,67 @@ def cpu_baseline(n, g, planes, workers=16):
     return out
 
 
+def cpu_reference(nbox=200):
+    """Proto's CPU path (oracle/_ref, Release flags, one thread) on bounded
+    samples of the multigrid and batched workloads, reported beside the GPU
+    figures: the plain FP64 GSRB V(2,2) correction-scheme cycle at 256^3 in
+    64^3 boxes and the library FAS cycle at 128^3 in 64^3 boxes (two cycles
+    each; per-solve figures for 512^3 are extrapolated by cell count and cycle
+    count and labelled so), and the Laplacian_27 apply on the first `nbox`
+    boxes of the batched size list."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import util
+    from proto_amd.inputs import batch_sizes
+    R3 = util.ref(3)
+    if R3 is None:
+        return {"note": "oracle/_ref (Proto's CPU build) is not present"}
+    out = {"cores": 1, "kind": "reference"}
+    n = 256
+    rhs = util.gaussian_rhs(n, 16, 2)
+    phi, hist = np.zeros(n ** 3), np.zeros(8)
+    t0 = time.perf_counter()
+    it = R3.ref_mg_gsrb_solve(n, 64, 6, 2, 2, 10, 0, util.ptr(rhs), util.ptr(phi), 2, ctypes.c_double(1e-10),
+                              util.ptr(hist))
+    dt = (time.perf_counter() - t0) / max(it, 1)
+    out["multigrid_gsrb_fp64"] = {
+        "s_per_cycle_256": round(dt, 4), "s_per_solve_512_extrapolated": round(dt * 8 * 11, 2),
+        "sample": "Proto CPU: 256^3 in 64^3 boxes, %d V(2,2) cycles to 8^3 with residual norms; "
+                  "512^3 solve extrapolated as 8x cells x 11 cycles" % it}
+    n = 128
+    rhs = util.gaussian_rhs(n, 16, 2)
+    phi, hist = np.zeros(n ** 3), np.zeros(8)
+    t0 = time.perf_counter()
+    it = R3.ref_mg_fas_solve(n, 64, 5, util.ptr(rhs), util.ptr(phi), 2, ctypes.c_double(1e-10), util.ptr(hist))
+    dt = (time.perf_counter() - t0) / max(it, 1)
+    out["library_fas_fp64"] = {
+        "s_per_cycle_128": round(dt, 4), "s_per_solve_512_extrapolated": round(dt * 64 * 31, 2),
+        "sample": "Proto CPU LevelMultigridSolver<BoxOp_Laplace>: 128^3 in 64^3 boxes, %d FAS cycles; "
+                  "512^3 solve extrapolated as 64x cells x 31 cycles" % it}
+    c, o, _, _, _ = util.lib_stencil("Laplacian_27")
+    offs = np.ascontiguousarray(o.reshape(-1), dtype=np.int32)
+    one = util.i3([1, 1, 1])
+    zero = util.i3([0, 0, 0])
+    ob = np.zeros(6, dtype=np.int32)
+    total, tsum = 0, 0.0
+    for k, (nx, ny, nz) in enumerate(batch_sizes(2000, (16, 24, 32, 48, 64), 3)[:nbox]):
+        src = np.random.default_rng(k).uniform(-1, 1, (nx + 4) * (ny + 4) * (nz + 4))
+        dst = np.zeros(nx * ny * nz)
+        slo, shi = util.i3([-2, -2, -2]), util.i3([nx + 1, ny + 1, nz + 1])
+        lo, hi = util.i3([0, 0, 0]), util.i3([nx - 1, ny - 1, nz - 1])
+        t0 = time.perf_counter()
+        R3.ref_stencil_apply(len(c), util.ptr(c), util.ptr(offs), util.ptr(one), util.ptr(one), util.ptr(zero),
+                             util.ptr(src), util.ptr(slo), util.ptr(shi), util.ptr(dst), util.ptr(lo), util.ptr(hi),
+                             util.ptr(lo), util.ptr(hi), 0, 1, ctypes.c_double(1.0), util.ptr(ob))
+        tsum += time.perf_counter() - t0
+        total += 8 * ((nx + 2) * (ny + 2) * (nz + 2) + nx * ny * nz)
+    out["batched_l27_apply_fp64"] = {
+        "GB/s": round(total / tsum / 1e9, 4),
+        "sample": "Proto CPU Stencil::apply of Laplacian_27, box by box, first %d of the 2000 boxes (%.2f s)"
+                  % (nbox, tsum)}
+    return out
+
+
 def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
     """Config 3: periodic n^3 in bs^3 boxes, 16-Gaussian RHS (sigma 0.05, seed 2)
     minus its mean, GSRB V(2,2) to 8^3, relative residual 1e-10.  Plain FP64
@@ -415,6 +476,7 @@ def main():
     # CPU baseline first, before this process touches the GPU (its all-cores
     # leg starts host processes)
     cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, 2, args.cpu_planes)
+    cpu_ref = None if (args.no_cpu or args.only or args.n != 512) else cpu_reference()
     import numpy as np
     import proto_amd as pa
     from proto_amd.device import DeviceArray, Event, Stream, gpu_available
@@ -632,6 +694,8 @@ def main():
 
     if cpu is not None:
         result["cpu_baseline"] = cpu
+    if cpu_ref is not None:
+        result.setdefault("extra", {})["cpu_reference"] = cpu_ref
     print(json.dumps(result))
 
 
