@@ -141,11 +141,12 @@ def _cpu_slab_job(job):
     return _cpu_slab(*job)
 
 
-def cpu_baseline(n, g, planes, workers=16):
+def cpu_baseline(n, g, planes, workers=16, other_workloads=False):
     """Proto's CPU path on a z-slab of `planes` planes of the same config-2
     field, one thread; plus the whole field split into `workers` z-slabs run by
     as many host processes at once (Proto's CPU path is single-threaded, so
-    cores are used one process per slab)."""
+    cores are used one process per slab); with other_workloads, also Proto's
+    CPU path on bounded samples of the multigrid and batched workloads."""
     import concurrent.futures
     import multiprocessing
     t0, t1, nbytes, kind = _cpu_slab(n, g, planes)
@@ -167,13 +168,15 @@ def cpu_baseline(n, g, planes, workers=16):
                             "sample": "%d host processes, each the same step on its own %dx%dx%d z-slab "
                                       "(%d planes in all), started together, %.2f s to the last end"
                                       % (workers, n, n, nz, workers * nz, span)}
+    if other_workloads:
+        out["other_workloads"] = cpu_reference()
     return out
 
 
 def cpu_reference(nbox=200):
-    """Proto's CPU path (oracle/_ref, Release flags, one thread) on bounded
-    samples of the multigrid and batched workloads, reported beside the GPU
-    figures: the plain FP64 GSRB V(2,2) correction-scheme cycle at 256^3 in
+    """Part of the CPU baseline: Proto's CPU path (oracle/_ref, Release flags,
+    one thread) on bounded samples of the multigrid and batched workloads,
+    reported beside the GPU figures: the plain FP64 GSRB V(2,2) correction-scheme cycle at 256^3 in
     64^3 boxes and the library FAS cycle at 128^3 in 64^3 boxes (two cycles
     each; per-solve figures for 512^3 are extrapolated by cell count and cycle
     count and labelled so), and the Laplacian_27 apply on the first `nbox`
@@ -475,8 +478,8 @@ def main():
         sys.exit(2)
     # CPU baseline first, before this process touches the GPU (its all-cores
     # leg starts host processes)
-    cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, 2, args.cpu_planes)
-    cpu_ref = None if (args.no_cpu or args.only or args.n != 512) else cpu_reference()
+    cpu = None if (args.no_cpu or args.only) else cpu_baseline(args.n, 2, args.cpu_planes,
+                                                               other_workloads=args.n == 512)
     import numpy as np
     import proto_amd as pa
     from proto_amd.device import DeviceArray, Event, Stream, gpu_available
@@ -694,8 +697,6 @@ def main():
 
     if cpu is not None:
         result["cpu_baseline"] = cpu
-    if cpu_ref is not None:
-        result.setdefault("extra", {})["cpu_reference"] = cpu_ref
     print(json.dumps(result))
 
 
