@@ -406,18 +406,25 @@ size_t gsrb_lds(int rows, int maxnx, size_t esz)
     return (4 * (size_t)(rows + 4) * (maxnx + 2) + 2 * (size_t)(rows + 2) * maxnx) * esz;
 }
 
-// Sweep tiles: full box width, `rows` rows (fewer when the FP64 tile would not
-// fit the LDS), whole z extent; largest first.
+// Sweep tiles: full box width, whole z extent, largest first.  The widest box
+// gets `rows` rows (fewer when its FP64 tile would not fit the LDS); narrower
+// boxes get proportionally more rows (rows x widest / width), as far as the
+// widest tile's LDS planes hold them, so every tile has about the same number
+// of cells per colour update and re-reads about the same share of halo rows.
 int build_gsrb_items(pm_batch* B, int rows)
 {
     int maxnx = 0;
     for (int b = 0; b < B->nboxes; b++) { maxnx = std::max(maxnx, B->sizes[3 * b]); }
     while (rows > 1 && gsrb_lds(rows, maxnx, 8) + 64 > 64 * 1024) { rows--; }
+    const long pcap = (long)(rows + 4) * (maxnx + 2), rcap = (long)(rows + 2) * maxnx;
     std::vector<BatchItem> items;
     for (int b = 0; b < B->nboxes; b++) {
         const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1], nz = B->sizes[3 * b + 2];
-        for (int y0 = 0; y0 < ny; y0 += rows) {
-            items.push_back(BatchItem{b, 0, y0, 0, nz, rows, nx, 0});
+        int r = std::max(rows, (int)(((long)rows * maxnx) / nx));
+        while (r > rows && ((long)(r + 4) * (nx + 2) > pcap || (long)(r + 2) * nx > rcap)) { r--; }
+        r = std::min(r, ny);
+        for (int y0 = 0; y0 < ny; y0 += r) {
+            items.push_back(BatchItem{b, 0, y0, 0, nz, r, nx, 0});
         }
     }
     std::stable_sort(items.begin(), items.end(), [](const BatchItem& a, const BatchItem& b) {
