@@ -286,20 +286,30 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
                 if (e < NR) { pl[e] = (rout >> i) & 1u ? (T)0 : prr[i]; }
             }
         };
-        // half sweep on plane z, rows [ya, yb), colour col: p += lambda (sum c_j p(nbr) + rhs)
+        // half sweep on plane z, rows [ya, yb), colour col: p += lambda (sum c_j p(nbr) + rhs).
+        // Cell index c = tid + 256 i <-> (row ya + c / hx, column pair c % hx),
+        // walked without divisions from the per-tile split of tid and 256.
+        const int hx = (bx.nx + 1) / 2;
+        const int q0 = tid / hx, r0 = tid - q0 * hx, qs = 256 / hx, rs = 256 - qs * hx;
         auto update = [&](int z, int ya, int yb, int colour) {
             if (z < 0 || z >= bx.nz) { return; }
             ya = max(ya, 0);
             yb = min(yb, bx.ny);
-            const int hx = (bx.nx + 1) / 2;
             const int ncell = (yb - ya) * hx;
             T* zc = slot(z);
             const T* zm = slot(z - 1);
             const T* zp = slot(z + 1);
             const T* rz = rslot(z);
+            int q = q0, rr = r0;
             for (int cidx = tid; cidx < ncell; cidx += 256) {
-                const int yy = ya + cidx / hx;
-                const int xx = 2 * (cidx % hx) + ((yy + z + colour) & 1);
+                const int yy = ya + q;
+                const int xx = 2 * rr + ((yy + z + colour) & 1);
+                q += qs;
+                rr += rs;
+                if (rr >= hx) {
+                    rr -= hx;
+                    q++;
+                }
                 if (xx >= bx.nx) { continue; }
                 const int li = (yy - (y0 - 2)) * LX + (xx + 1);
                 T s = 0;
