@@ -212,16 +212,33 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
             if (e < NR) { pl[e] = prr[i]; }
         }
     };
-    auto update = [&](int z, int ya, int yb, int xa, int xb, int colour) {
+    // cell index q = tid + 256 i of a colour update <-> (row ya + q / hw, column
+    // pair q % hw), walked without divisions from the split of tid and 256 by hw
+    struct Walk {
+        int hw, q0, r0, qs, rs;
+    };
+    auto mk_walk = [&](int hw) {
+        const int t = (int)threadIdx.x;
+        return Walk{hw, t / hw, t % hw, 256 / hw, 256 % hw};
+    };
+    const Walk wred = mk_walk((nx + 2 + 1) / 2), wblk = mk_walk((nx + 1) / 2);
+    auto update = [&](int z, int ya, int yb, int xa, int xb, int colour, const Walk& wk) {
         T* zc_ = slot(z);
         const T* zm = slot(z - 1);
         const T* zp = slot(z + 1);
         const T* rz = rslot(z);
-        const int hw = (xb - xa + 1) / 2;
+        const int hw = wk.hw;
         const int ncell = (yb - ya) * hw;
+        int qq = wk.q0, rr = wk.r0;
         for (int q = threadIdx.x; q < ncell; q += 256) {
-            const int yy = ya + q / hw;
-            const int xx = xa + 2 * (q % hw) + ((xa + yy + z + colour + losum) & 1);
+            const int yy = ya + qq;
+            const int xx = xa + 2 * rr + ((xa + yy + z + colour + losum) & 1);
+            qq += wk.qs;
+            rr += wk.rs;
+            if (rr >= hw) {
+                rr -= hw;
+                qq++;
+            }
             if (xx >= xb) { continue; }
             const int li = (yy - (y0 - 2)) * W + (xx + 2);
             T s = 0;
@@ -263,9 +280,9 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         if (k + 2 <= z1) { issue_r(k + 2); }
         if (k - 1 >= z0) { write_out(k - 1); }  // slot of plane k-1: only read during this step
         __syncthreads();
-        update(k + 1, y0 - 1, y1 + 1, -1, nx + 1, 0);
+        update(k + 1, y0 - 1, y1 + 1, -1, nx + 1, 0, wred);
         __syncthreads();
-        if (k >= z0) { update(k, y0, y1, 0, nx, 1); }
+        if (k >= z0) { update(k, y0, y1, 0, nx, 1, wblk); }
         __syncthreads();
     }
     write_out(z1 - 1);

@@ -138,16 +138,33 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
         }
     };
     // colour cells ((x+y+z+colour) even) of plane z, rows [ya, yb), columns [xa, xb)
-    auto update = [&](int z, int ya, int yb, int xa, int xb, int colour) {
+    // cell index c = tid + 256 i of a colour update <-> (row ya + c / hw, column
+    // pair c % hw), walked without divisions from the split of tid and 256 by hw
+    struct Walk {
+        int hw, q0, r0, qs, rs;
+    };
+    auto mk_walk = [&](int hw) {
+        const int t = (int)threadIdx.x;
+        return Walk{hw, t / hw, t % hw, 256 / hw, 256 % hw};
+    };
+    const Walk wred = mk_walk((x1 - x0 + 2 + 1) / 2), wblk = mk_walk((x1 - x0 + 1) / 2);
+    auto update = [&](int z, int ya, int yb, int xa, int xb, int colour, const Walk& wk) {
         float* zc = slot(z);
         const float* zm = slot(z - 1);
         const float* zp = slot(z + 1);
         const float* rz = rslot(z);
-        const int hw = (xb - xa + 1) / 2;
+        const int hw = wk.hw;
         const int ncell = (yb - ya) * hw;
+        int qq = wk.q0, rr = wk.r0;
         for (int c = threadIdx.x; c < ncell; c += 256) {
-            const int yy = ya + c / hw;
-            const int xx = xa + 2 * (c % hw) + ((xa + yy + z + colour) & 1);
+            const int yy = ya + qq;
+            const int xx = xa + 2 * rr + ((xa + yy + z + colour) & 1);
+            qq += wk.qs;
+            rr += wk.rs;
+            if (rr >= hw) {
+                rr -= hw;
+                qq++;
+            }
             if (xx >= xb) { continue; }
             const int li = (yy - (y0 - 2)) * W + (xx - x0 + 2);
             float s = 0;
@@ -187,9 +204,9 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
         if (k + 2 <= z1) { issue_r(k + 2); }
         if (k - 1 >= z0) { write_out(k - 1); }
         __syncthreads();
-        update(k + 1, y0 - 1, y1 + 1, x0 - 1, x1 + 1, 0);  // red, one-cell halo included
+        update(k + 1, y0 - 1, y1 + 1, x0 - 1, x1 + 1, 0, wred);  // red, one-cell halo included
         __syncthreads();
-        if (k >= z0) { update(k, y0, y1, x0, x1, 1); }  // black, tile cells
+        if (k >= z0) { update(k, y0, y1, x0, x1, 1, wblk); }  // black, tile cells
         __syncthreads();
     }
     write_out(z1 - 1);
