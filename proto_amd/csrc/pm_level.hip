@@ -408,6 +408,88 @@ __global__ __launch_bounds__(256) void k_restrict_residual(LayoutDev F, LayoutDe
     }
 }
 
+// The same residual restriction when every fine box's coarsened region lies in
+// one coarse box (box sizes of a multigrid hierarchy, see box_aligned): one
+// workgroup grid per fine box, the coarse box found once per workgroup, 32-bit
+// cell indices.  Same arithmetic and order as k_restrict_residual.
+template<typename T>
+__global__ __launch_bounds__(256) void k_restrict_residual_box(LayoutDev F, LayoutDev C, G3 gp, G3 gr, G3 gc,
+                                                               const T* __restrict__ phi, const T* __restrict__ rhs,
+                                                               T* __restrict__ resc, Lap<T> c, T avg)
+{
+    const int b = blockIdx.y;  // fine box
+    const int* flo = F.lows + 3 * b;
+    const int hx = F.bs[0] >> 1, hy = F.bs[1] >> 1, hz = F.bs[2] >> 1;
+    int p[3] = {flo[0] >> 1, flo[1] >> 1, flo[2] >> 1};
+    const int o = owner(C, p);
+    const int* clo = C.lows + 3 * o;
+    const int ox = (flo[0] >> 1) - clo[0], oy = (flo[1] >> 1) - clo[1], oz = (flo[2] >> 1) - clo[2];
+    const T* ph = phi + (long)b * gvol(F, gp);
+    const T* rh = rhs + (long)b * gvol(F, gr);
+    T* rc = resc + (long)o * gvol(C, gc);
+    const long FX = F.bs[0] + 2 * gp.g[0], FXY = FX * (F.bs[1] + 2 * gp.g[1]);
+    const int n = hx * hy * hz;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int x = t % hx, r = t / hx, y = r % hy, z = r / hy;
+        const int fx = 2 * x, fy = 2 * y, fz = 2 * z;
+        T acc = 0;
+#pragma unroll
+        for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+            for (int ky = 0; ky < 2; ky++)
+#pragma unroll
+                for (int kx = 0; kx < 2; kx++) {
+                    const long i = rel_index(F, gp, fx + kx, fy + ky, fz + kz);
+                    T s = 0;
+                    s = s + rh[rel_index(F, gr, fx + kx, fy + ky, fz + kz)];
+#pragma unroll
+                    for (int j = 0; j < 7; j++) {
+                        s = s + c.c[j] * ph[i + lapo(j, 0) + lapo(j, 1) * FX + lapo(j, 2) * FXY];
+                    }
+                    acc = acc + avg * s;
+                }
+        rc[rel_index(C, gc, ox + x, oy + y, oz + z)] = acc;
+    }
+}
+
+// Constant prolongation on the same box-aligned hierarchies: the coarse box is
+// found once per workgroup and each fine row is walked from one row base of
+// the fine and the coarse storage.  Same arithmetic as k_prolong_add<.., 0>.
+template<typename T>
+__global__ __launch_bounds__(256) void k_prolong_const_box(LayoutDev F, LayoutDev C, G3 gd, G3 gp,
+                                                           const T* __restrict__ delta, T* __restrict__ phi)
+{
+    const int b = blockIdx.y;
+    const int* flo = F.lows + 3 * b;
+    const int nx = F.bs[0], ny = F.bs[1], nz = F.bs[2];
+    int p[3] = {flo[0] >> 1, flo[1] >> 1, flo[2] >> 1};
+    const int o = owner(C, p);
+    const int* clo = C.lows + 3 * o;
+    const T* d = delta + (long)o * gvol(C, gd);
+    T* ph = phi + (long)b * gvol(F, gp);
+    const long FX = nx + 2 * gp.g[0], FY = ny + 2 * gp.g[1];
+    const long CX = C.bs[0] + 2 * gd.g[0], CY = C.bs[1] + 2 * gd.g[1];
+    const T one = 1;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int rows = ny * nz;
+    for (int row = blockIdx.x * 4 + ty; row < rows; row += gridDim.x * 4) {
+        const int y = row % ny, z = row / ny;
+        const int J = ((flo[1] + y) >> 1) - clo[1], K = ((flo[2] + z) >> 1) - clo[2];
+        T* prow = ph + FX * ((y + gp.g[1]) + FY * (z + gp.g[2])) + gp.g[0];
+        const T* drow = d + CX * ((J + gd.g[1]) + CY * (K + gd.g[2])) + gd.g[0];
+        for (int x = tx; x < nx; x += 64) {
+            const T dv = drow[((flo[0] + x) >> 1) - clo[0]];
+            T s0 = 0;
+            s0 = s0 + one * dv;
+            T s1 = 0;
+            s1 = s1 + one * s0;
+            T s2 = 0;
+            s2 = s2 + one * s1;
+            prow[x] = prow[x] + s2;
+        }
+    }
+}
+
 // ------------------------------------------------------------ prolongation
 template<typename T, int DIMN, int LINEAR>
 __global__ __launch_bounds__(256) void k_prolong_add(LayoutDev F, LayoutDev C, G3 gd, G3 gp,
@@ -566,6 +648,15 @@ static unsigned bx_blocks(long n, long cap = 1024)
     return (unsigned)std::max(1L, std::min(b, cap));
 }
 
+// Workgroups per box of a grid-stride launch over every box of a level: about
+// 16384 workgroups for the level however many boxes it has (a 512-box level
+// with one 256-cell pass per workgroup would dispatch 524288 workgroups, and
+// dispatch, not memory, would set the time), at most one per 256 items.
+static unsigned lvl_blocks(long n, int nboxes, long cap = 1024)
+{
+    return bx_blocks(n, std::min(cap, std::max(1L, 16384L / std::max(1, nboxes))));
+}
+
 template<typename T>
 static Lap<T> lap_coefs(int dim, double m)
 {
@@ -597,6 +688,21 @@ int layout_lows(const pm_layout* L, int* lows)
     PM_REQUIRE(L && lows, "null argument");
     for (size_t i = 0; i < L->lows.size(); i++) { lows[i] = L->lows[i]; }
     return PM_OK;
+}
+
+// 3D, both levels fully covered, fine boxes of even size and coarse boxes
+// whose size is a multiple of half the fine box size: then every fine box's
+// coarsened region lies in one coarse box, and the fine boxes' regions tile
+// the coarse level
+static bool box_aligned(const pm_layout* F, const pm_layout* C)
+{
+    if (F->dim != 3 || C->nboxes != C->nb[0] * C->nb[1] * C->nb[2] || F->nboxes != F->nb[0] * F->nb[1] * F->nb[2]) {
+        return false;
+    }
+    for (int k = 0; k < 3; k++) {
+        if (F->bs[k] % 2 != 0 || C->bs[k] % (F->bs[k] / 2) != 0) { return false; }
+    }
+    return true;
 }
 
 static bool same_layout_shape(const pm_layout* a, const pm_layout* b)
@@ -744,7 +850,7 @@ int pm_level_exchange(const pm_layout* L, int dtype, void* data, const int ghost
     const LayoutDev D = ldev(L);
     const long X = L->bs[0] + 2 * g.g[0], Y = L->bs[1] + 2 * g.g[1];
     const long nghost = 2 * (X * Y * g.g[2] + X * g.g[1] * L->bs[2] + (long)g.g[0] * L->bs[1] * L->bs[2]);
-    dim3 grid(bx_blocks(nghost, 256), L->nboxes);
+    dim3 grid(lvl_blocks(nghost, L->nboxes, 256), L->nboxes);
     PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_exchange<T>, grid, dim3(256), 0, S(stream), D, g, (T*)data, ncomp));
     PM_CHECK_LAUNCH();
     return PM_OK;
@@ -758,7 +864,7 @@ int pm_level_gsrb(const pm_layout* L, int dtype, void* phi, const int phi_ghost[
     for (int k = 0; k < L->dim; k++) { PM_REQUIRE(gp.g[k] >= 1, "pm_level_gsrb: phi needs a ghost layer"); }
     const LayoutDev D = ldev(L);
     const long n = (long)((L->bs[0] + 1) / 2) * L->bs[1] * L->bs[2];
-    dim3 grid(bx_blocks(n), L->nboxes);
+    dim3 grid(bx_blocks(n), L->nboxes);  // one cell per thread measured fastest for this kernel
     // LaplaceOp::init (LaplaceOp.H:56-73): lambda = 2 * (-(dx*dx)/(4 DIM)); coefficients Lap_j * (-1/(dx*dx))
     double lam = -(dx * dx) / (4.0 * L->dim);
     lam *= 2.0;
@@ -847,7 +953,7 @@ int pm_level_jacobi(const pm_layout* L, int dtype, void* phi, const int phi_ghos
     const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost);
     const LayoutDev D = ldev(L);
     const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
-    dim3 grid(bx_blocks(n), L->nboxes);
+    dim3 grid(lvl_blocks(n, L->nboxes), L->nboxes);
     // pointRelax (LevelMultigrid.cpp:99-123): weight 1/(4 DIM), lambda = dx^2/(4 DIM)
     const double weight = 1.0 / (4 * L->dim);
     const double lambda = dx * dx / (4 * L->dim);
@@ -912,14 +1018,18 @@ int pm_level_restrict_residual(const pm_layout* F, const pm_layout* C, int dtype
     const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost), gc = g3(resc_ghost);
     const LayoutDev DF = ldev(F), DC = ldev(C);
     const long n = (long)C->bs[0] * C->bs[1] * C->bs[2];
-    dim3 grid(bx_blocks(n), C->nboxes);
+    dim3 grid(lvl_blocks(n, C->nboxes), C->nboxes);
     const double hsqinv = 1. / (dx * dx);
     PM_DISPATCH_T(dtype, {
         Lap<T> c;
         for (int j = 0; j < 7; j++) { c.c[j] = 0; }
         for (int j = 0; j < 2 * F->dim + 1; j++) { c.c[j] = (T)(j == 0 ? -2.0 * F->dim : 1.0) * (T)(-hsqinv); }
         const T avg = (T)(1.0 / (F->dim == 3 ? 8 : 4));
-        if (F->dim == 3) {
+        if (box_aligned(F, C)) {
+            const long nh = (long)(F->bs[0] / 2) * (F->bs[1] / 2) * (F->bs[2] / 2);
+            hipLaunchKernelGGL((k_restrict_residual_box<T>), dim3(lvl_blocks(nh, F->nboxes), F->nboxes), dim3(256), 0,
+                               S(stream), DF, DC, gp, gr, gc, (const T*)phi, (const T*)rhs, (T*)resc, c, avg);
+        } else if (F->dim == 3) {
             hipLaunchKernelGGL((k_restrict_residual<T, 3>), grid, dim3(256), 0, S(stream), DF, DC, gp, gr, gc,
                                (const T*)phi, (const T*)rhs, (T*)resc, c, avg);
         } else {
@@ -943,12 +1053,19 @@ int pm_level_prolong_add(const pm_layout* F, const pm_layout* C, int dtype, cons
     const G3 gd = g3(delta_ghost), gp = g3(phi_ghost);
     const LayoutDev DF = ldev(F), DC = ldev(C);
     const long n = (long)F->bs[0] * F->bs[1] * F->bs[2];
-    dim3 grid(bx_blocks(n), F->nboxes);
+    dim3 grid(lvl_blocks(n, F->nboxes), F->nboxes);
     PM_DISPATCH_T(dtype, {
         if (F->dim == 3) {
             if (linear) {
                 hipLaunchKernelGGL((k_prolong_add<T, 3, 1>), grid, dim3(256), 0, S(stream), DF, DC, gd, gp,
                                    (const T*)delta, (T*)phi);
+            } else if (box_aligned(F, C)) {
+                // about 8192 workgroups over the level, each walking many rows: one
+                // 4-row pass carries too little data to amortise a workgroup
+                const long rows = (long)F->bs[1] * F->bs[2];
+                const long per = std::max<long>(1, std::min<long>((rows + 3) / 4, 8192 / F->nboxes));
+                hipLaunchKernelGGL((k_prolong_const_box<T>), dim3((unsigned)per, F->nboxes), dim3(256), 0,
+                                   S(stream), DF, DC, gd, gp, (const T*)delta, (T*)phi);
             } else {
                 hipLaunchKernelGGL((k_prolong_add<T, 3, 0>), grid, dim3(256), 0, S(stream), DF, DC, gd, gp,
                                    (const T*)delta, (T*)phi);
@@ -972,7 +1089,7 @@ int pm_level_setval(const pm_layout* L, int dtype, void* data, const int ghost[3
 {
     PM_REQUIRE(L && data && ncomp >= 1, "pm_level_setval: bad argument");
     const long n = gvol(ldev(L), g3(ghost)) * ncomp * L->nboxes;
-    PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_fill<T>, dim3(bx_blocks(n, 1 << 16)), dim3(256), 0, S(stream), (T*)data,
+    PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_fill<T>, dim3(bx_blocks(n, 1 << 14)), dim3(256), 0, S(stream), (T*)data,
                                             n, (T)value));
     PM_CHECK_LAUNCH();
     return PM_OK;
@@ -985,7 +1102,7 @@ int pm_level_copy(const pm_layout* SL, const void* src, const int src_ghost[3], 
     PM_REQUIRE(same_layout_shape(SL, DL), "pm_level_copy: layouts of different domains");
     const LayoutDev DS = ldev(SL), DD = ldev(DL);
     const long n = (long)DL->bs[0] * DL->bs[1] * DL->bs[2];
-    dim3 grid(bx_blocks(n), DL->nboxes);
+    dim3 grid(lvl_blocks(n, DL->nboxes), DL->nboxes);
     PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_level_copy<T>, grid, dim3(256), 0, S(stream), DS, g3(src_ghost),
                                             (const T*)src, DD, g3(dst_ghost), (T*)dst, ncomp));
     PM_CHECK_LAUNCH();
@@ -1007,7 +1124,7 @@ int pm_level_copy_to(const pm_layout* SL, const void* src, const int src_ghost[3
     const LayoutDev DS = ldev(SL), DD = ldev(DL);
     const G3 gs = g3(src_ghost), gd = g3(dst_ghost);
     const long n = gvol(DD, gd);
-    dim3 grid(bx_blocks(n), DL->nboxes);
+    dim3 grid(lvl_blocks(n, DL->nboxes), DL->nboxes);
     const int mode = compatible ? 1 : 0;
     PM_DISPATCH_T(dtype, hipLaunchKernelGGL(k_level_copy_to<T>, grid, dim3(256), 0, S(stream), DS, gs,
                                             (const T*)src, DD, gd, (T*)dst, ncomp, mode));
@@ -1129,7 +1246,7 @@ int pm_level_add_correction(const pm_layout* L, int dtype, void* phi, const int 
     const G3 gp = g3(phi_ghost), ge = g3(e_ghost);
     const LayoutDev D = ldev(L);
     const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
-    dim3 grid(bx_blocks(n), L->nboxes);
+    dim3 grid(lvl_blocks(n, L->nboxes), L->nboxes);
     if (dtype == PM_F64 && e_dtype == PM_F32) {
         hipLaunchKernelGGL((k_add_correction<double, float>), grid, dim3(256), 0, S(stream), D, gp, ge, (double*)phi,
                            (const float*)e);
@@ -1243,7 +1360,7 @@ int pm_level_add_separable(const pm_layout* L, int dtype, void* data, const int 
     const G3 g = g3(ghost);
     const LayoutDev D = ldev(L);
     const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
-    dim3 grid(bx_blocks(n), L->nboxes);
+    dim3 grid(lvl_blocks(n, L->nboxes), L->nboxes);
     if (dtype == PM_F64) {
         hipLaunchKernelGGL(k_add_separable<double>, grid, dim3(256), 0, S(stream), D, g, (double*)data, coef, dt,
                            dt + L->domain[0], dt + L->domain[0] + L->domain[1]);

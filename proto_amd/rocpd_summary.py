@@ -3,6 +3,7 @@ WRITE_SIZE per dispatch, calibrated against a copy kernel of known size) into
 the short text tables kept under profiles/.
 
     python -m proto_amd.rocpd_summary trace  DB [NAME_SUBSTRING]
+    python -m proto_amd.rocpd_summary calls  DB NAME_SUBSTRING [LAST_N]
     python -m proto_amd.rocpd_summary pmc    FETCH_DB WRITE_DB COPY_BYTES
 """
 import sqlite3
@@ -30,6 +31,18 @@ def trace(db, pattern=""):
         out.append("%-84s %6d %10.2f %10.2f %10.2f %10.2f %5d %6d" % (
             name[:84], len(d), sum(d) / len(d), sd[len(sd) // 2], sd[0], sum(last) / len(last),
             r["vgpr"] + (r["agpr"] or 0), r["lds"]))
+    return "\n".join(out)
+
+
+def calls(db, pattern, last=40):
+    """The last `last` dispatches of the kernels whose name contains `pattern`:
+    duration, grid and workgroup size, in dispatch order."""
+    c = sqlite3.connect(db)
+    rows = [r for r in c.execute("select name, duration, grid_x, grid_y, grid_z, workgroup_x from kernels "
+                                 "order by dispatch_id") if pattern in r[0]]
+    out = ["%-70s %10s %20s %6s" % ("kernel", "dur_us", "grid (threads)", "wg")]
+    for name, dur, gx, gy, gz, wg in rows[-last:]:
+        out.append("%-70s %10.2f %20s %6s" % (name[:70], dur / 1000.0, "%dx%dx%d" % (gx, gy, gz), wg))
     return "\n".join(out)
 
 
@@ -61,5 +74,7 @@ def pmc(fetch_db, write_db, copy_bytes):
 if __name__ == "__main__":
     if sys.argv[1] == "trace":
         print(trace(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else ""))
+    elif sys.argv[1] == "calls":
+        print(calls(sys.argv[2], sys.argv[3], int(sys.argv[4]) if len(sys.argv) > 4 else 40))
     else:
         print(pmc(sys.argv[2], sys.argv[3], int(sys.argv[4])))
