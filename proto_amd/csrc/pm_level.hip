@@ -490,6 +490,145 @@ __global__ __launch_bounds__(256) void k_prolong_const_box(LayoutDev F, LayoutDe
     }
 }
 
+// The same residual on boxes 32..64 cells wide, marching z: a workgroup owns
+// a 64 x 16 tile (4 wave64s, 4 rows each) of one box over a z range.  Each
+// plane of phi (tile + 1-cell halo, inside the box's ghosted storage) is
+// staged once in LDS (a ring of 3 planes: plane p is last read in step p + 1
+// and rewritten in step p + 3, after every thread passed the barrier of step
+// p + 2); each thread keeps its own cells of planes z-1, z, z+1 in registers
+// for the z taps.  Next-plane loads are issued before each plane is computed;
+// results are stored at the start of the next step.  NEG = 0: the arithmetic
+// and order of k_residual (0 + rhs + sum c_j phi); NEG = 1: those of k_resnorm
+// (0 - rhs + sum c_j phi).  out may be null (norm only).
+constexpr int RZ_RY = 4, RZ_TY = 4 * RZ_RY, RZ_NE = (RZ_TY + 2) * 66;
+
+template<typename T, typename TO, int NEG>
+__global__ __launch_bounds__(256) void k_residual_z(LayoutDev L, G3 gp, G3 gr, G3 go, const T* __restrict__ phi,
+                                                    const T* __restrict__ rhs, TO* __restrict__ out, Lap<T> c,
+                                                    T* __restrict__ partials, int zc)
+{
+    constexpr int RY = RZ_RY, NL = (RZ_NE + 255) / 256;
+    __shared__ T sm[3][RZ_NE];
+    __shared__ T red[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int b = blockIdx.y;
+    const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
+    const int nyt = (ny + RZ_TY - 1) / RZ_TY;
+    const int y0 = (int)(blockIdx.x % nyt) * RZ_TY;
+    const int z0 = (int)(blockIdx.x / nyt) * zc, z1 = min(z0 + zc, nz);
+    const int LX = nx + 2, NE = (RZ_TY + 2) * LX;
+    const long X = nx + 2 * gp.g[0], XY = X * (ny + 2 * gp.g[1]);
+    const T* ph = phi + (long)b * gvol(L, gp);
+    const T* rh = rhs + (long)b * gvol(L, gr);
+    // staged element e: box row y0 - 1 + e / LX (clamped to the last storage
+    // row), box column e % LX - 1; elements past the plane repeat the last one
+    long goff[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = min(tid + 256 * k, NE - 1);
+        const int er = e / LX, ec = e - er * LX;
+        const int sr = min(y0 - 1 + er + gp.g[1], ny + 2 * gp.g[1] - 1);
+        goff[k] = X * (long)sr + (ec - 1 + gp.g[0]);
+    }
+    T P[NL];
+    auto gload = [&](int z) {
+        const T* base = ph + XY * (long)(z + gp.g[2]);
+#pragma unroll
+        for (int k = 0; k < NL; k++) { P[k] = base[goff[k]]; }
+    };
+    auto lstore = [&](T* pl) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const int e = tid + 256 * k;
+            if (e < NE) { pl[e] = P[k]; }
+        }
+    };
+    const int x = lane;
+    const bool xok = x < nx;
+    const int xc = xok ? x : 0;
+    T zw[3][RY];
+    T fq[RY], res[RY];
+    int zres = -1;
+    auto fload = [&](int zo) {
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int y = min(y0 + w * RY + r, ny - 1);
+            fq[r] = rh[rel_index(L, gr, xc, y, zo)];
+        }
+    };
+    T m = 0;
+    auto put_res = [&]() {
+        if (zres < 0 || !xok || !out) { return; }
+        TO* ot = out + (long)b * gvol(L, go);
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int y = y0 + w * RY + r;
+            if (y < ny) { ot[rel_index(L, go, x, y, zres)] = (TO)res[r]; }
+        }
+    };
+    const int kb = z0 - 1;
+    gload(kb);
+    fload(z0);
+    for (int zz = kb; zz <= z1; zz++) {
+        T* sl = sm[(zz - kb) % 3];
+        lstore(sl);
+        if (zz + 1 <= z1) { gload(zz + 1); }
+        put_res();
+        zres = -1;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            zw[0][r] = zw[1][r];
+            zw[1][r] = zw[2][r];
+            zw[2][r] = sl[(w * RY + r + 1) * LX + (xc + 1)];
+        }
+        if (zz < z0 + 1) { continue; }
+        const int zo = zz - 1;
+        const T* s0 = sm[(zo - kb) % 3];
+        T fcur[RY];
+#pragma unroll
+        for (int r = 0; r < RY; r++) { fcur[r] = fq[r]; }
+        if (zo + 1 < z1) { fload(zo + 1); }
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int li = (w * RY + r + 1) * LX + (xc + 1);
+            T sv = 0;
+            sv = NEG ? sv - fcur[r] : sv + fcur[r];
+            sv = sv + c.c[0] * zw[1][r];
+            sv = sv + c.c[1] * s0[li + 1];
+            sv = sv + c.c[2] * s0[li - 1];
+            sv = sv + c.c[3] * s0[li + LX];
+            sv = sv + c.c[4] * s0[li - LX];
+            sv = sv + c.c[5] * zw[2][r];
+            sv = sv + c.c[6] * zw[0][r];
+            res[r] = sv;
+            const bool ok = xok && (y0 + w * RY + r) < ny;
+            const T a = sv > 0 ? sv : -sv;
+            m = (ok && m < a) ? a : m;
+        }
+        zres = zo;
+    }
+    put_res();
+    m = block_reduce(PM_RED_ABS, m, red);
+    if (tid == 0) { partials[(long)b * gridDim.x + blockIdx.x] = m; }
+}
+
+// boxes and ghosts the z-marching residual takes, and its grid (16-row tiles x
+// z chunks, halved while fewer than 2048 workgroups)
+static bool rz_ok(const pm_layout* L, const G3& gp)
+{
+    return L->dim == 3 && L->bs[0] >= 32 && L->bs[0] <= 64 && L->bs[1] >= 16 && L->bs[2] >= 4 && gp.g[0] >= 1 &&
+           gp.g[1] >= 1 && gp.g[2] >= 1;
+}
+static dim3 rz_grid(const pm_layout* L, int* zc)
+{
+    const int nyt = (L->bs[1] + RZ_TY - 1) / RZ_TY;
+    int z = L->bs[2];
+    while (z > 8 && (long)nyt * ((L->bs[2] + z - 1) / z) * L->nboxes < 2048) { z = (z + 1) / 2; }
+    *zc = z;
+    return dim3((unsigned)(nyt * ((L->bs[2] + z - 1) / z)), (unsigned)L->nboxes);
+}
+
 // ------------------------------------------------------------ prolongation
 template<typename T, int DIMN, int LINEAR>
 __global__ __launch_bounds__(256) void k_prolong_add(LayoutDev F, LayoutDev C, G3 gd, G3 gp,
@@ -981,8 +1120,14 @@ int pm_level_resnorm(const pm_layout* L, int dtype, const void* phi, const int p
     const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost);
     const LayoutDev D = ldev(L);
     const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
-    const unsigned bpb = bx_blocks(n, 64);
+    unsigned bpb = bx_blocks(n, 64);
     dim3 grid(bpb, L->nboxes);
+    int zc = 0;
+    const bool z = rz_ok(L, gp);
+    if (z) {
+        grid = rz_grid(L, &zc);
+        bpb = grid.x;
+    }
     const double hsqinv = 1. / (dx * dx);
     void* ws = nullptr;
     PM_DISPATCH_T(dtype, {
@@ -991,7 +1136,10 @@ int pm_level_resnorm(const pm_layout* L, int dtype, const void* phi, const int p
         Lap<T> c;
         for (int j = 0; j < 7; j++) { c.c[j] = 0; }
         for (int j = 0; j < 2 * L->dim + 1; j++) { c.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)hsqinv; }
-        if (L->dim == 3) {
+        if (z) {
+            hipLaunchKernelGGL((k_residual_z<T, T, 1>), grid, dim3(256), 0, S(stream), D, gp, gr, gr, (const T*)phi,
+                               (const T*)rhs, (T*)nullptr, c, (T*)ws, zc);
+        } else if (L->dim == 3) {
             hipLaunchKernelGGL((k_resnorm<T, 3>), grid, dim3(256), 0, S(stream), D, gp, gr, (const T*)phi,
                                (const T*)rhs, c, (T*)ws);
         } else {
@@ -1205,8 +1353,14 @@ int pm_level_residual(const pm_layout* L, int dtype, const void* phi, const int 
     const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost), go = g3(out_ghost);
     const LayoutDev D = ldev(L);
     const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
-    const unsigned bpb = bx_blocks(n, 64);
+    unsigned bpb = bx_blocks(n, 64);
     dim3 grid(bpb, L->nboxes);
+    int zc = 0;
+    const bool z = rz_ok(L, gp);
+    if (z) {
+        grid = rz_grid(L, &zc);
+        bpb = grid.x;
+    }
     const double hsqinv = 1. / (dx * dx);
     void* ws = nullptr;
     PM_DISPATCH_T(dtype, {
@@ -1215,7 +1369,13 @@ int pm_level_residual(const pm_layout* L, int dtype, const void* phi, const int 
         Lap<T> c;
         for (int j = 0; j < 7; j++) { c.c[j] = 0; }
         for (int j = 0; j < 2 * L->dim + 1; j++) { c.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)(-hsqinv); }
-        if (out_dtype == PM_F32) {
+        if (z && out_dtype == PM_F32) {
+            hipLaunchKernelGGL((k_residual_z<T, float, 0>), grid, dim3(256), 0, S(stream), D, gp, gr, go,
+                               (const T*)phi, (const T*)rhs, (float*)out, c, (T*)ws, zc);
+        } else if (z) {
+            hipLaunchKernelGGL((k_residual_z<T, double, 0>), grid, dim3(256), 0, S(stream), D, gp, gr, go,
+                               (const T*)phi, (const T*)rhs, (double*)out, c, (T*)ws, zc);
+        } else if (out_dtype == PM_F32) {
             if (L->dim == 3) {
                 hipLaunchKernelGGL((k_residual<T, float, 3>), grid, dim3(256), 0, S(stream), D, gp, gr, go,
                                    (const T*)phi, (const T*)rhs, (float*)out, c, (T*)ws);
