@@ -231,7 +231,78 @@ def cpu_reference(nbox=200):
         "GB/s": round(total / tsum / 1e9, 4),
         "sample": "Proto CPU Stencil::apply of Laplacian_27, box by box, first %d of the 2000 boxes (%.2f s)"
                   % (nbox, tsum)}
+    R2 = util.ref(2)
+    if R2 is not None:
+        # DIM=2 256^2 Laplacian apply (replace, scale 1/h^2) + absMax
+        n = 256
+        c, o, _, _, _ = util.lib_stencil("Laplacian", dim=2)
+        offs = np.ascontiguousarray(o.reshape(-1), dtype=np.int32)
+        src = np.pad(util.uniform(0, 0, n * n).reshape(n, n), 1, mode="wrap").reshape(-1).copy()
+        dst = np.zeros(n * n)
+        slo, shi = util.i3([-1, -1, 0]), util.i3([n, n, 0])
+        lo, hi = util.i3([0, 0, 0]), util.i3([n - 1, n - 1, 0])
+        reps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 1.0:
+            R2.ref_stencil_apply(len(c), util.ptr(c), util.ptr(offs), util.ptr(one), util.ptr(one), util.ptr(zero),
+                                 util.ptr(src), util.ptr(slo), util.ptr(shi), util.ptr(dst), util.ptr(lo),
+                                 util.ptr(hi), util.ptr(lo), util.ptr(hi), 0, 1, ctypes.c_double(float(n * n)),
+                                 util.ptr(ob))
+            R2.ref_reduce(2, util.ptr(dst), n * n)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        out["config1_2d_apply_absmax"] = {
+            "us_per_step": round(1e6 * dt, 2),
+            "sample": "Proto CPU DIM=2: dst |= Laplacian()(u, 1/h^2) on 256^2 (ghost 1) + absMax, %d steps" % reps}
     return out
+
+
+def config1_2d(pa, stream, reps=200):
+    """Config 1: DIM=2, 256^2 interior with one periodic ghost layer,
+    dst |= Laplacian()(u, 1/h^2), then absMax(dst).  The arrays (1 MB) stay in
+    cache, so the figures are times: back-to-back steps on the stream, the same
+    two launches replayed as a graph, and the host-observed latency of one step
+    (launches + synchronize)."""
+    import numpy as np
+    from proto_amd.device import DeviceArray
+    from proto_amd.inputs import uniform
+    n = 256
+    h = 1.0 / n
+    c = np.array([-4.0, 1, 1, 1, 1])
+    o = [[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0]]  # Laplacian() in 2D (Proto_StencilDefs.H:77-87)
+    st = pa.Stencil(c, o, pa.PM_F64, 2)
+    ui = uniform(n * n, 0, 0, stream=stream.h)  # U[-1,1), seed 0, stream 0, x fastest
+    u = DeviceArray.from_numpy(np.pad(ui.numpy().reshape(n, n), 1, mode="wrap").reshape(-1).copy())
+    ui.free()
+    dst = DeviceArray(n * n)
+    out = DeviceArray.zeros(1)
+    sb, db = pa.Box.of([-1, -1], [n, n]), pa.Box.of([0, 0], [n - 1, n - 1])
+    sp = stream.h
+
+    def step():
+        pa.call("pm_stencil_apply", st.h, u.ptr, ctypes.byref(sb), dst.ptr, ctypes.byref(db), ctypes.byref(db), 1, 1,
+                1.0 / (h * h), sp)
+        pa.call("pm_reduce", pa.RED_ABS, pa.PM_F64, dst.ptr, n * n, out.ptr, 0, sp)
+
+    us_stream = 1e3 * time_launches(step, reps, stream)
+    gr = ctypes.c_void_p()
+    pa.call("pm_graph_begin", sp)
+    step()
+    pa.call("pm_graph_end", sp, ctypes.byref(gr))
+    us_graph = 1e3 * time_launches(lambda: pa.call("pm_graph_launch", gr, sp), reps, stream)
+    pa.call("pm_graph_destroy", gr)
+    lat = []
+    for _ in range(50):
+        t0 = time.perf_counter()
+        step()
+        stream.sync()
+        lat.append(time.perf_counter() - t0)
+    amax = float(out.numpy()[0])
+    for d in (u, dst, out):
+        d.free()
+    return {"workload": "laplacian2d_256_absmax", "us_per_step_stream": round(us_stream, 2),
+            "us_per_step_graph": round(us_graph, 2), "us_latency_p50": round(1e6 * float(np.median(lat)), 2),
+            "absmax": amax.hex(), "bytes": 8 * (258 * 258 + 2 * 256 * 256)}
 
 
 def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
@@ -681,6 +752,7 @@ def main():
         big.free()
         big2.free()
         a.free()
+        extra["config1_2d"] = config1_2d(pa, stream)
         result["extra"] = extra
 
     if not args.no_batch:

@@ -26,6 +26,15 @@ int workspace(hipStream_t stream, size_t bytes, void** out);
 int layout_shape(const pm_layout* L, int* dim, int* domain, int* bs, int* periodic, int* nboxes);
 // Box low corners in layout order (3 ints per box).
 int layout_lows(const pm_layout* L, int* lows);
+// pm_level_gsrb_sweep with a choice of input (pm_level.hip): in_mode 0 reads
+// phi_in; 1 sweeps a zero initial guess without reading it (phi_in unused);
+// 2 sweeps phi_in + the constant prolongation of `coarse` (layout C, levels
+// box-aligned), the cells pm_level_prolong_add(interp 0) would have left in
+// phi_in, which is not written.  PM_ERR_UNSUPPORTED where the one-pass sweep
+// does not apply.
+int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
+                        const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
+                        const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream);
 // Kernel family (0 generic, 7, 27), dtype and coefficients of a stencil (pm_stencil.hip).
 int stencil_info(const pm_stencil* st, int* kind, int* dtype, std::vector<double>* coefs);
 

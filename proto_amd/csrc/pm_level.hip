@@ -120,10 +120,15 @@ __global__ __launch_bounds__(256) void k_gsrb(LayoutDev L, G3 gp, G3 gr, T* __re
 // halo values equal what the neighbouring tiles compute for those cells:
 // same inputs, same order), then the black cells of plane k over the tile,
 // which is written out.  Same arithmetic and order as k_gsrb's two colours.
-template<typename T, int MAXE>
+// IN selects what the staged input is: 0 pin; 1 zero (pin is not read: the
+// sweep of a zero initial guess); 2 pin + the constant prolongation of the
+// coarse level crs (layout C, ghost gc), rounded as k_prolong_const_box
+// rounds it, so the sweep sees the cells fineInterp would have written.
+template<typename T, int MAXE, int IN>
 __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout, G3 gr, const T* __restrict__ pin,
                                                     T* __restrict__ pout, const T* __restrict__ rhs, Lap<T> c,
-                                                    T lambda, int rows, int zc)
+                                                    T lambda, int rows, int zc, LayoutDev C, G3 gc,
+                                                    const T* __restrict__ crs)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
     __shared__ int s_nb[27];  // box owning each of the 27 neighbouring box positions (13 = this box)
@@ -171,17 +176,47 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
         offr[i] = at(gr, vr, fc - 1, y0 - 1 + fr, 0);
     }
-    T pre[MAXE], prr[MAXE];
+    // IN == 2: coarse element under each staged fine cell.  Inside the box
+    // every staged cell's owner lies in this box's z layer of boxes, whose
+    // coarse boxes share one z range, so plane z adds (z >> 1) coarse planes
+    // (box lows are even); planes beyond the box go through atc().
+    constexpr int NCO = IN == 2 ? MAXE : 1;
+    int offc[NCO];  // the coarse level holds fewer than 2^31 elements (checked by the host)
+    const long cvol = gvol(C, gc);
+    const long XYc = (long)(C.bs[0] + 2 * gc.g[0]) * (C.bs[1] + 2 * gc.g[1]);
+    auto atc = [&](int x, int y, int z) {
+        int p[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
+        for (int k = 0; k < 3; k++) {
+            p[k] = ((p[k] % L.domain[k]) + L.domain[k]) % L.domain[k];
+            p[k] >>= 1;
+        }
+        const int o = owner(C, p);
+        const int* clo = C.lows + 3 * o;
+        return (long)o * cvol + rel_index(C, gc, p[0] - clo[0], p[1] - clo[1], p[2] - clo[2]);
+    };
+    if constexpr (IN == 2) {
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
+            offc[i] = (int)atc(ec - 2, y0 - 2 + er, 0);
+        }
+    }
+    T pre[MAXE], prr[MAXE], prc[NCO];
     auto issue = [&](int z) {
+        if constexpr (IN == 1) { return; }
         const bool inb = z >= 0 && z < nz;
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
             const T* ptr = pin + XYp * z + offp[i];
+            const T* cp = crs;
+            if constexpr (IN == 2) { cp = crs + XYc * (z >> 1) + offc[i]; }
             if (!inb) {
                 const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
                 ptr = pin + at(gin, vin, ec - 2, y0 - 2 + er, z);
+                if constexpr (IN == 2) { cp = crs + atc(ec - 2, y0 - 2 + er, z); }
             }
             pre[i] = *ptr;
+            if constexpr (IN == 2) { prc[i] = *cp; }
         }
     };
     auto commit = [&](int z) {
@@ -189,7 +224,16 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
             const int e = threadIdx.x + 256 * i;
-            if (e < NP) { pl[e] = pre[i]; }
+            T v;
+            if constexpr (IN == 1) {
+                v = 0;
+            } else if constexpr (IN == 2) {
+                const T zero = 0;
+                v = pre[i] + (zero + prc[i]);
+            } else {
+                v = pre[i];
+            }
+            if (e < NP) { pl[e] = v; }
         }
     };
     auto issue_r = [&](int z) {
@@ -1025,11 +1069,29 @@ int pm_level_gsrb(const pm_layout* L, int dtype, void* phi, const int phi_ghost[
     return PM_OK;
 }
 
-int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
-                        const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, pm_stream_t stream)
+}  // extern "C"
+
+namespace pm {
+
+#define PM_SWEEP_LAUNCH(E, M)                                                                                 \
+    hipLaunchKernelGGL((k_gsrb_fused<T, E, M>), grid, dim3(256), lds, stream, D, gi, go, gr, pi, (T*)phi_out, \
+                       (const T*)rhs, c, lt, rows, zc, DC, gc, cr)
+#define PM_SWEEP_MODES(E)      \
+    if (in_mode == 0) {        \
+        PM_SWEEP_LAUNCH(E, 0); \
+    } else if (in_mode == 1) { \
+        PM_SWEEP_LAUNCH(E, 1); \
+    } else {                   \
+        PM_SWEEP_LAUNCH(E, 2); \
+    }
+
+int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
+                        const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
+                        const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream)
 {
-    PM_REQUIRE(L && phi_in && phi_out && rhs, "pm_level_gsrb_sweep: bad argument");
-    PM_REQUIRE(phi_in != phi_out, "pm_level_gsrb_sweep: the sweep is out of place (phi_in != phi_out)");
+    PM_REQUIRE(L && phi_out && rhs && (phi_in || in_mode == 1), "pm_level_gsrb_sweep: bad argument");
+    PM_REQUIRE(in_mode == 1 || phi_in != phi_out, "pm_level_gsrb_sweep: the sweep is out of place (phi_in != phi_out)");
+    PM_REQUIRE(in_mode >= 0 && in_mode <= 2, "pm_level_gsrb_sweep: bad input mode");
     if (L->dim != 3) {
         set_error("pm_level_gsrb_sweep: 3-D layouts only");
         return PM_ERR_UNSUPPORTED;
@@ -1038,6 +1100,14 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
     if (L->nboxes != full || !L->periodic[0] || !L->periodic[1] || !L->periodic[2]) {
         set_error("pm_level_gsrb_sweep: needs a periodic layout covering its domain");
         return PM_ERR_UNSUPPORTED;
+    }
+    if (in_mode == 2) {
+        PM_REQUIRE(C && coarse, "pm_level_gsrb_sweep: the prolongation input needs the coarse level");
+        const G3 gcc = g3(coarse_ghost);
+        if (!box_aligned(L, C) || (long)C->nboxes * gvol(ldev(C), gcc) >= (1L << 31)) {
+            set_error("pm_level_gsrb_sweep: the prolongation input needs box-aligned levels");
+            return PM_ERR_UNSUPPORTED;
+        }
     }
     const size_t esz = dtype == PM_F64 ? 8 : 4;
     auto lds_of = [&](int r) {
@@ -1058,8 +1128,9 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
     const int nyt = (L->bs[1] + rows - 1) / rows;
     int zc = L->bs[2];
     while (zc > 8 && (long)L->nboxes * nyt * ((L->bs[2] + zc - 1) / zc) < 2048) { zc = (zc + 1) / 2; }
-    const G3 gi = g3(in_ghost), go = g3(out_ghost), gr = g3(rhs_ghost);
+    const G3 gi = g3(in_ghost), go = g3(out_ghost), gr = g3(rhs_ghost), gc = g3(coarse_ghost);
     const LayoutDev D = ldev(L);
+    const LayoutDev DC = in_mode == 2 ? ldev(C) : D;
     dim3 grid((unsigned)(nyt * ((L->bs[2] + zc - 1) / zc)), L->nboxes);
     double lam = -(dx * dx) / (4.0 * L->dim);
     lam *= 2.0;
@@ -1070,19 +1141,34 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
             for (int j = 0; j < 7; j++) { c.c[j] = (T)((j == 0 ? -2.0 * L->dim : 1.0) * m); }
         }
         const long np = (long)(rows + 4) * (L->bs[0] + 4);
+        const T* pi = (const T*)(in_mode == 1 ? phi_out : phi_in);
+        const T* cr = (const T*)coarse;
+        const T lt = (T)(1.0 * lam);
         if (np <= 1024) {
-            hipLaunchKernelGGL((k_gsrb_fused<T, 4>), grid, dim3(256), lds, S(stream), D, gi, go, gr,
-                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
+            PM_SWEEP_MODES(4)
         } else if (np <= 2048) {
-            hipLaunchKernelGGL((k_gsrb_fused<T, 8>), grid, dim3(256), lds, S(stream), D, gi, go, gr,
-                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
+            PM_SWEEP_MODES(8)
         } else {
-            hipLaunchKernelGGL((k_gsrb_fused<T, 16>), grid, dim3(256), lds, S(stream), D, gi, go, gr,
-                               (const T*)phi_in, (T*)phi_out, (const T*)rhs, c, (T)(1.0 * lam), rows, zc);
+            PM_SWEEP_MODES(16)
         }
     });
     PM_CHECK_LAUNCH();
     return PM_OK;
+}
+
+#undef PM_SWEEP_MODES
+#undef PM_SWEEP_LAUNCH
+
+}  // namespace pm
+
+extern "C" {
+
+int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
+                        const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, pm_stream_t stream)
+{
+    PM_REQUIRE(phi_in, "pm_level_gsrb_sweep: bad argument");
+    return level_gsrb_sweep_in(L, dtype, phi_in, in_ghost, phi_out, out_ghost, rhs, rhs_ghost, dx, 0, nullptr, nullptr,
+                               nullptr, S(stream));
 }
 
 int pm_level_jacobi(const pm_layout* L, int dtype, void* phi, const int phi_ghost[3], const void* rhs,

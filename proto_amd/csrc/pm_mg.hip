@@ -93,20 +93,48 @@ static void mg_free(pm_mg* mg)
     if (mg->graph) { hipGraphDestroy(mg->graph); }
 }
 
-static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* tmp, int n, hipStream_t s)
+// n relaxation sweeps of level l.  in_mode 1: phi is a zero initial guess not
+// yet written (m_delta.setToZero()); 2: phi still lacks fineInterp's
+// constant prolongation of cphi (level l + 1).  The first one-pass sweep reads
+// such an input directly; otherwise it is written first, as Proto does.
+static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* tmp, int n, hipStream_t s,
+                 int in_mode = 0, const void* cphi = nullptr)
 {
     const auto& L = mg->lv[l];
+    auto prepare = [&]() -> int {
+        if (in_mode == 1) {
+            MG_TRY(pm_level_setval(L.layout, dtype, phi, mg->g1, 1, 0.0, s));
+        } else if (in_mode == 2) {
+            MG_TRY(pm_level_prolong_add(L.layout, mg->lv[l + 1].layout, dtype, cphi, mg->g1, phi, mg->g1, 0, s));
+        }
+        in_mode = 0;
+        return PM_OK;
+    };
     int it = 0;
     if (mg->opt.relax == 0 && mg->opt.fused && mg->dim == 3 && tmp) {
         // one-pass sweeps alternating phi -> tmp -> phi; an odd count ends with
         // the two-colour form below
         for (; it + 1 < n; it += 2) {
-            int rc = pm_level_gsrb_sweep(L.layout, dtype, phi, mg->g1, tmp, mg->g0, rhs, mg->g0, L.dx, s);
-            if (rc == PM_ERR_UNSUPPORTED) { break; }
-            if (rc != PM_OK) { return rc; }
+            int rc = PM_ERR_UNSUPPORTED;
+            if (in_mode != 0) {
+                rc = level_gsrb_sweep_in(L.layout, dtype, phi, mg->g1, tmp, mg->g0, rhs, mg->g0, L.dx, in_mode,
+                                         in_mode == 2 ? mg->lv[l + 1].layout : nullptr, cphi, mg->g1, s);
+                if (rc != PM_OK && rc != PM_ERR_UNSUPPORTED) { return rc; }
+                if (rc == PM_OK) {
+                    in_mode = 0;
+                } else {
+                    MG_TRY(prepare());
+                }
+            }
+            if (rc != PM_OK) {
+                rc = pm_level_gsrb_sweep(L.layout, dtype, phi, mg->g1, tmp, mg->g0, rhs, mg->g0, L.dx, s);
+                if (rc == PM_ERR_UNSUPPORTED) { break; }
+                if (rc != PM_OK) { return rc; }
+            }
             MG_TRY(pm_level_gsrb_sweep(L.layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0, L.dx, s));
         }
     }
+    MG_TRY(prepare());
     for (; it < n; it++) {
         if (mg->opt.relax == 0) {
             for (int color = 0; color < 2; color++) {
@@ -121,29 +149,34 @@ static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* 
     return PM_OK;
 }
 
-// vCycle (LevelMultigrid.cpp:142-160) on level l with (phi, rhs) of dtype.
-static int vcycle(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, hipStream_t s)
+// vCycle (LevelMultigrid.cpp:142-160) on level l with (phi, rhs) of dtype;
+// zero_in: phi is the zero initial guess of a correction, not yet written.
+static int vcycle(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, hipStream_t s, bool zero_in = false)
 {
     auto& L = mg->lv[l];
     const bool f32 = dtype == PM_F32;
     void* tmp = f32 ? L.tmp32 : L.tmp;
     int pre = mg->opt.pre, post = mg->opt.post, bottom = mg->opt.bottom;
     if (mg->opt.relax == 1) { pre = post = 2 * mg->dim; bottom = 10; }
+    const int zin = zero_in ? 1 : 0;
     if (l < mg->nlev) {
         auto& C = mg->lv[l + 1];
         void* cphi = f32 ? C.phi32 : C.phi;
         void* crhs = f32 ? C.rhs32 : C.rhs;
-        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, pre, s));
+        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, pre, s, zin));
         // coarseResidual (51-73): exchange, residual, AvgDown, copy to the coarse layout
         MG_TRY(pm_level_exchange(L.layout, dtype, phi, mg->g1, 1, s));
         MG_TRY(pm_level_restrict_residual(L.layout, C.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, crhs, mg->g0, s));
-        MG_TRY(pm_level_setval(C.layout, dtype, cphi, mg->g1, 1, 0.0, s));  // m_delta.setToZero()
-        MG_TRY(vcycle(mg, l + 1, dtype, cphi, crhs, s));
-        // fineInterp (125-140): phi += I(delta)
-        MG_TRY(pm_level_prolong_add(L.layout, C.layout, dtype, cphi, mg->g1, phi, mg->g1, mg->opt.interp, s));
-        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, post, s));
+        MG_TRY(vcycle(mg, l + 1, dtype, cphi, crhs, s, true));  // m_delta.setToZero() folded into its first sweep
+        // fineInterp (125-140): phi += I(delta), folded into the first post sweep for Constant
+        if (mg->opt.interp == 0) {
+            MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, post, s, 2, cphi));
+        } else {
+            MG_TRY(pm_level_prolong_add(L.layout, C.layout, dtype, cphi, mg->g1, phi, mg->g1, mg->opt.interp, s));
+            MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, post, s));
+        }
     } else {
-        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, bottom, s));
+        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, bottom, s, zin));
     }
     return PM_OK;
 }
@@ -168,8 +201,7 @@ static int resnorm(pm_mg* mg, void* phi, const void* rhs, double* out, hipStream
 static int ir_correct(pm_mg* mg, void* phi, hipStream_t s)
 {
     auto& L = mg->lv[0];
-    MG_TRY(pm_level_setval(L.layout, PM_F32, L.phi32, mg->g1, 1, 0.0, s));
-    MG_TRY(vcycle(mg, 0, PM_F32, L.phi32, L.rhs32, s));
+    MG_TRY(vcycle(mg, 0, PM_F32, L.phi32, L.rhs32, s, true));  // e = 0 folded into its first sweep
     MG_TRY(pm_level_add_correction(L.layout, PM_F64, phi, mg->g1, PM_F32, L.phi32, mg->g1, s));
     return PM_OK;
 }

@@ -211,6 +211,31 @@ def test_config2_full_size_512():
     assert host(red2).tobytes() == R.tobytes()
 
 
+def test_config1_full_size_256_2d():
+    """Config 1 at its real size: DIM=2, 256^2 interior, one periodic ghost layer,
+    dst |= Laplacian()(u, 1/h^2), then absMax and sum of dst.  Every cell and the
+    absMax bitwise against the values Proto's CPU path wrote (config1.json); the sum
+    within the bound stated in _sum_bound of Proto's sequential sum."""
+    import hashlib
+    import json
+    meta = json.load(open(os.path.join(util.GOLDEN, "config1.json")))
+    n = 256
+    h = 1.0 / n
+    c, o, _, _, _ = util.lib_stencil("Laplacian", dim=2)
+    st = pa.Stencil(c, o, pa.PM_F64, 2)
+    u = np.pad(util.uniform(0, 0, n * n).reshape(n, n), 1, mode="wrap").reshape(-1).copy()
+    say("config1 256^2 apply")
+    dst = apply_gpu(st, u, [-1, -1], [n, n], np.full(n * n, 7.77e77), [0, 0], [n - 1, n - 1], [0, 0],
+                    [n - 1, n - 1], True, 1.0 / (h * h))
+    assert hashlib.sha256(dst.tobytes()).hexdigest() == meta["sha256"]
+    dd = dev(dst)
+    out = zeros(1)
+    pa.call("pm_reduce", pa.RED_ABS, pa.PM_F64, vp(dd), n * n, vp(out), 0, None)
+    assert host(out)[0] == float.fromhex(meta["absmax"])
+    pa.call("pm_reduce", pa.RED_SUM, pa.PM_F64, vp(dd), n * n, vp(out), 0, None)
+    assert abs(host(out)[0] - float.fromhex(meta["sum"])) <= _sum_bound(dst)
+
+
 @pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
 def test_reductions_vs_oracle(dtype):
     npdt = np.float64 if dtype == pa.PM_F64 else np.float32
