@@ -132,6 +132,10 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
 {
     extern __shared__ __align__(16) unsigned char smraw[];
     __shared__ int s_nb[27];  // box owning each of the 27 neighbouring box positions (13 = this box)
+    // offset corrections of planes beyond the box (below: 0..8, above: 9..17),
+    // per xy neighbour position k = (ox + 1) + 3 (oy + 1) of the staged cell:
+    // phi, rhs and (IN == 2) coarse element
+    __shared__ int s_dz[18], s_dr[18], s_dc[18];
     const int b = blockIdx.y;
     const int* lo = L.lows + 3 * b;
     const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
@@ -144,6 +148,10 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     T* rsm = sm + 4 * (long)(rows + 4) * W;
     const int losum = lo[0] + lo[1] + lo[2];
     const long vin = gvol(L, gin), vr = gvol(L, gr);
+    const long XYp = (long)(nx + 2 * gin.g[0]) * (ny + 2 * gin.g[1]);
+    const long XYr = (long)(nx + 2 * gr.g[0]) * (ny + 2 * gr.g[1]);
+    const long cvol = gvol(C, gc);
+    const long XYc = (long)(C.bs[0] + 2 * gc.g[0]) * (C.bs[1] + 2 * gc.g[1]);
     if (threadIdx.x < 27) {
         const int t = threadIdx.x;
         int p[3] = {lo[0] + (t % 3 - 1) * nx, lo[1] + ((t / 3) % 3 - 1) * ny, lo[2] + (t / 9 - 1) * nz};
@@ -158,32 +166,7 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         const int o = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
         return (long)o * vol + rel_index(L, g, x - ox * nx, y - oy * ny, z - oz * nz);
     };
-    T* ob = pout + (long)b * gvol(L, gout);
-    auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
-    auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
-    // in-plane offsets of this thread's elements for planes inside the box
-    // (z = 0; plane z adds z * plane size); planes beyond the box go through
-    // at().  Elements past the plane repeat the last one, so every load is
-    // issued unconditionally, straight into the register that holds it until
-    // its commit: the compiler's waits then only ever cover loads a step old.
-    long offp[MAXE], offr[MAXE];
-    const long XYp = (long)(nx + 2 * gin.g[0]) * (ny + 2 * gin.g[1]);
-    const long XYr = (long)(nx + 2 * gr.g[0]) * (ny + 2 * gr.g[1]);
-#pragma unroll
-    for (int i = 0; i < MAXE; i++) {
-        const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
-        offp[i] = at(gin, vin, ec - 2, y0 - 2 + er, 0);
-        const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
-        offr[i] = at(gr, vr, fc - 1, y0 - 1 + fr, 0);
-    }
-    // IN == 2: coarse element under each staged fine cell.  Inside the box
-    // every staged cell's owner lies in this box's z layer of boxes, whose
-    // coarse boxes share one z range, so plane z adds (z >> 1) coarse planes
-    // (box lows are even); planes beyond the box go through atc().
-    constexpr int NCO = IN == 2 ? MAXE : 1;
-    int offc[NCO];  // the coarse level holds fewer than 2^31 elements (checked by the host)
-    const long cvol = gvol(C, gc);
-    const long XYc = (long)(C.bs[0] + 2 * gc.g[0]) * (C.bs[1] + 2 * gc.g[1]);
+    // coarse element under fine cell (x, y, z) (IN == 2)
     auto atc = [&](int x, int y, int z) {
         int p[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
         for (int k = 0; k < 3; k++) {
@@ -194,6 +177,51 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         const int* clo = C.lows + 3 * o;
         return (long)o * cvol + rel_index(C, gc, p[0] - clo[0], p[1] - clo[1], p[2] - clo[2]);
     };
+    if (threadIdx.x < 18) {
+        // A plane beyond the box is the nearest planes of the box below /
+        // above, so for every staged cell of neighbour position k the element
+        // differs from the in-box formula (plane z * plane size + in-plane
+        // offset) by the same amount for both planes on that side: the
+        // difference at a representative cell of k (for the coarse element,
+        // both fine planes of a side lie in one coarse plane, box lows even).
+        const int t = threadIdx.x, side = t / 9, k = t % 9;
+        const int x = k % 3 == 0 ? -1 : (k % 3 == 1 ? 0 : nx);
+        const int y = k / 3 == 0 ? -1 : (k / 3 == 1 ? 0 : ny);
+        const int z = side ? nz : -1;
+        s_dz[t] = (int)(at(gin, vin, x, y, z) - (XYp * z + at(gin, vin, x, y, 0)));
+        s_dr[t] = (int)(at(gr, vr, x, y, z) - (XYr * z + at(gr, vr, x, y, 0)));
+        s_dc[t] = IN == 2 ? (int)(atc(x, y, z) - (XYc * (z >> 1) + atc(x, y, 0))) : 0;
+    }
+    __syncthreads();
+    T* ob = pout + (long)b * gvol(L, gout);
+    auto slot = [&](int z) { return sm + (long)(((z % 4) + 4) % 4) * NP; };
+    auto rslot = [&](int z) { return rsm + (long)(z & 1) * NR; };
+    // in-plane offsets of this thread's elements for planes inside the box
+    // (z = 0; plane z adds z * plane size; planes beyond the box add s_dz /
+    // s_dr / s_dc of the cell's neighbour position, packed 4 bits per element
+    // in kp / kr).  Elements past the plane repeat the last one, so every
+    // load is issued unconditionally, straight into the register that holds
+    // it until its commit: the compiler's waits then only ever cover loads a
+    // step old.
+    int offp[MAXE], offr[MAXE];  // level arrays hold fewer than 2^31 elements (checked by the host)
+    unsigned long long kp = 0, kr = 0;
+    auto kpos = [&](int x, int y) {
+        return (x < 0 ? 0 : (x >= nx ? 2 : 1)) + 3 * (y < 0 ? 0 : (y >= ny ? 2 : 1));
+    };
+#pragma unroll
+    for (int i = 0; i < MAXE; i++) {
+        const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
+        offp[i] = (int)at(gin, vin, ec - 2, y0 - 2 + er, 0);
+        kp |= (unsigned long long)kpos(ec - 2, y0 - 2 + er) << (4 * i);
+        const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
+        offr[i] = (int)at(gr, vr, fc - 1, y0 - 1 + fr, 0);
+        kr |= (unsigned long long)kpos(fc - 1, y0 - 1 + fr) << (4 * i);
+    }
+    // IN == 2: coarse element under each staged fine cell.  Inside the box
+    // every staged cell's owner lies in this box's z layer of boxes, whose
+    // coarse boxes share one z range, so plane z adds (z >> 1) coarse planes.
+    constexpr int NCO = IN == 2 ? MAXE : 1;
+    int offc[NCO];  // the coarse level holds fewer than 2^31 elements (checked by the host)
     if constexpr (IN == 2) {
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
@@ -205,18 +233,19 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     auto issue = [&](int z) {
         if constexpr (IN == 1) { return; }
         const bool inb = z >= 0 && z < nz;
+        const int sd = z < 0 ? 0 : 9;
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
-            const T* ptr = pin + XYp * z + offp[i];
-            const T* cp = crs;
-            if constexpr (IN == 2) { cp = crs + XYc * (z >> 1) + offc[i]; }
+            long o = XYp * z + offp[i];
+            long oc = 0;
+            if constexpr (IN == 2) { oc = XYc * (z >> 1) + offc[i]; }
             if (!inb) {
-                const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
-                ptr = pin + at(gin, vin, ec - 2, y0 - 2 + er, z);
-                if constexpr (IN == 2) { cp = crs + atc(ec - 2, y0 - 2 + er, z); }
+                const int k = sd + (int)((kp >> (4 * i)) & 15);
+                o += s_dz[k];
+                if constexpr (IN == 2) { oc += s_dc[k]; }
             }
-            pre[i] = *ptr;
-            if constexpr (IN == 2) { prc[i] = *cp; }
+            pre[i] = pin[o];
+            if constexpr (IN == 2) { prc[i] = crs[oc]; }
         }
     };
     auto commit = [&](int z) {
@@ -238,14 +267,12 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     };
     auto issue_r = [&](int z) {
         const bool inb = z >= 0 && z < nz;
+        const int sd = z < 0 ? 0 : 9;
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
-            const T* ptr = rhs + XYr * z + offr[i];
-            if (!inb) {
-                const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
-                ptr = rhs + at(gr, vr, fc - 1, y0 - 1 + fr, z);
-            }
-            prr[i] = *ptr;
+            long o = XYr * z + offr[i];
+            if (!inb) { o += s_dr[sd + (int)((kr >> (4 * i)) & 15)]; }
+            prr[i] = rhs[o];
         }
     };
     auto commit_r = [&](int z) {
@@ -1100,6 +1127,14 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
     if (L->nboxes != full || !L->periodic[0] || !L->periodic[1] || !L->periodic[2]) {
         set_error("pm_level_gsrb_sweep: needs a periodic layout covering its domain");
         return PM_ERR_UNSUPPORTED;
+    }
+    {
+        const LayoutDev D = ldev(L);
+        const long big = std::max(gvol(D, g3(in_ghost)), gvol(D, g3(rhs_ghost))) * L->nboxes;
+        if (big >= (1L << 31)) {
+            set_error("pm_level_gsrb_sweep: levels of 2^31 or more elements are not supported");
+            return PM_ERR_UNSUPPORTED;
+        }
     }
     if (in_mode == 2) {
         PM_REQUIRE(C && coarse, "pm_level_gsrb_sweep: the prolongation input needs the coarse level");

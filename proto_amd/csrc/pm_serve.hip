@@ -82,8 +82,12 @@ __device__ inline int wrapi(int v, int n)
 // of level L, reading `in` and writing `out` (dir 0: phi -> tmp, 1: tmp -> phi).
 // phi planes (rows [y0-2, y1+2) x columns [-2, n+2)) pass through a ring of 4
 // LDS planes and rhs planes (rows [y0-1, y1+1) x columns [-1, n+1)) through a
-// ring of 2, both loaded into registers one step ahead of their use.
-template<int MAXE>
+// ring of 2, both loaded into registers one step ahead of their use.  IN
+// selects the staged input: 0 `in`; 1 zero (`in` is not read: the first sweep
+// of a zero initial guess); 2 `in` + the constant prolongation of the next
+// coarser level's phi, rounded as k_srv_prolong rounds it (the first sweep
+// after fineInterp, which is then not run).
+template<int MAXE, int IN>
 __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv, const SItem* __restrict__ items, int dir)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
@@ -110,18 +114,46 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
         const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
         offr[i] = wrapi(x0 + fc - 1, n) + n * wrapi(y0 - 1 + fr, n);
     }
-    float pre[MAXE], prr[MAXE];
+    // IN == 2: coarse cell under each staged fine cell (in-plane part)
+    constexpr int NCO = IN == 2 ? MAXE : 1;
+    int offc[NCO];
+    const float* cphi = lv[it.lev + (IN == 2 ? 1 : 0)].phi;
+    const int nc = lv[it.lev + (IN == 2 ? 1 : 0)].n;  // n / 2 (IN == 2)
+    if constexpr (IN == 2) {
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
+            offc[i] = (wrapi(x0 + ec - 2, n) >> 1) + nc * (wrapi(y0 - 2 + er, n) >> 1);
+        }
+    }
+    float pre[MAXE], prr[MAXE], prc[NCO];
     auto issue = [&](int z) {
-        const float* base = in + nn * wrapi(z, n);
+        if constexpr (IN == 1) { return; }
+        const int zw = wrapi(z, n);
+        const float* base = in + nn * zw;
 #pragma unroll
         for (int i = 0; i < MAXE; i++) { pre[i] = base[offp[i]]; }
+        if constexpr (IN == 2) {
+            const float* cb = cphi + (long)nc * nc * (zw >> 1);
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) { prc[i] = cb[offc[i]]; }
+        }
     };
     auto commit = [&](int z) {
         float* pl = slot(z);
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
             const int e = threadIdx.x + 256 * i;
-            if (e < NP) { pl[e] = pre[i]; }
+            float v;
+            if constexpr (IN == 1) {
+                v = 0.0f;
+            } else if constexpr (IN == 2) {
+                const float zero = 0.0f;
+                v = pre[i] + (zero + prc[i]);
+            } else {
+                v = pre[i];
+            }
+            if (e < NP) { pl[e] = v; }
         }
     };
     auto issue_r = [&](int z) {
@@ -213,8 +245,10 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
 }
 
 // Coarse right-hand side = AvgDown(rhs + sum_j c_j phi(p + o_j)) of the fine
-// level (LevelMultigrid.cpp:51-73 order), and coarse phi = 0.
-__global__ __launch_bounds__(256) void k_srv_restrict(const SLevel* __restrict__ lv, const SItem* __restrict__ items)
+// level (LevelMultigrid.cpp:51-73 order), and, with zero_phi, coarse phi = 0
+// (otherwise the coarse level's first sweep takes the zero guess as its input).
+__global__ __launch_bounds__(256) void k_srv_restrict(const SLevel* __restrict__ lv, const SItem* __restrict__ items,
+                                                      int zero_phi)
 {
     const SItem it = items[blockIdx.x];
     const SLevel F = lv[it.lev], C = lv[it.lev + 1];
@@ -245,7 +279,7 @@ __global__ __launch_bounds__(256) void k_srv_restrict(const SLevel* __restrict__
                     acc = acc + avg * s;
                 }
         C.rhs[t] = acc;
-        C.phi[t] = 0.0f;
+        if (zero_phi) { C.phi[t] = 0.0f; }
     }
 }
 
@@ -270,11 +304,12 @@ __global__ __launch_bounds__(256) void k_srv_prolong(const SLevel* __restrict__ 
 }
 
 // FP64 residual r = rhs + sum_j d_j phi(p + o_j) of the finest level: FP32 copy
-// into the level-0 rhs, level-0 phi (the correction) = 0, max|r| into the
-// request's norm slot (non-negative doubles order like their bit patterns).
+// into the level-0 rhs, with zero_e level-0 phi (the correction) = 0, max|r|
+// into the request's norm slot (non-negative doubles order like their bit
+// patterns).
 __global__ __launch_bounds__(256) void k_srv_residual(const SReq* __restrict__ rq, const SLevel* __restrict__ lv,
                                                       const SItem* __restrict__ items,
-                                                      unsigned long long* __restrict__ norms)
+                                                      unsigned long long* __restrict__ norms, int zero_e)
 {
     __shared__ double red[16];
     const SItem it = items[blockIdx.x];
@@ -297,7 +332,7 @@ __global__ __launch_bounds__(256) void k_srv_residual(const SReq* __restrict__ r
         s = s + R.d1 * R.phi[x + row + nn * wrapi(z + 1, n)];
         s = s + R.d1 * R.phi[x + row + nn * wrapi(z - 1, n)];
         L.rhs[t] = (float)s;
-        L.phi[t] = 0.0f;
+        if (zero_e) { L.phi[t] = 0.0f; }
         const double a = s > 0 ? s : -s;
         m = m < a ? a : m;
     }
@@ -439,6 +474,7 @@ struct Active {
 struct Launch {
     int kind;  // 0 sweep, 1 restrict, 2 prolong, 3 correct, 4 residual
     int off, count, dir;
+    int in;    // sweeps: staged input (k_srv_gsrb IN); restrict / residual: 1 = write the zero guess
     size_t lds;  // sweeps: dynamic LDS of the largest tile
     long np;     // sweeps: largest phi plane of a tile (elements)
 };
@@ -617,8 +653,8 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
     }
     std::vector<SItem> items;
     S->plan.clear();
-    auto begin = [&](int kind, int dir) {
-        S->plan.push_back(Launch{kind, (int)items.size(), 0, dir, 0, 0});
+    auto begin = [&](int kind, int dir, int in = 0) {
+        S->plan.push_back(Launch{kind, (int)items.size(), 0, dir, in, 0, 0});
     };
     auto end = [&]() {
         Launch& L = S->plan.back();
@@ -637,11 +673,17 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
         L.np = std::max(L.np, (long)(r + 4) * (w + 4));
     };
     const int pre = S->opt.pre, post = S->opt.post, bot = S->opt.bottom;
+    // The zero initial guess of every level's correction is folded into that
+    // level's first sweep when every level sweeps first (pre and bottom
+    // counts > 0); otherwise restriction and residual write it.  The constant
+    // prolongation is folded into the first post sweep when there is one.
+    const bool zero_in = pre > 0 && bot > 0;
+    const bool fold_prolong = post > 0;
     // V-cycle of every non-fresh request (fresh ones only get their first residual)
     for (int d = 0; d <= maxd; d++) {
         const int nsw = std::max(pre, bot);
         for (int sw = 0; sw < nsw; sw++) {
-            begin(0, sw & 1);
+            begin(0, sw & 1, sw == 0 && zero_in ? 1 : 0);
             for (size_t r = 0; r < act.size(); r++) {
                 const Active& a = act[r];
                 const int bd = (int)a.ns.size() - 1;
@@ -650,7 +692,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             }
             end();
         }
-        begin(1, 0);
+        begin(1, 0, zero_in ? 0 : 1);
         for (size_t r = 0; r < act.size(); r++) {
             const Active& a = act[r];
             if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
@@ -660,15 +702,17 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
         end();
     }
     for (int d = maxd - 1; d >= 0; d--) {
-        begin(2, 0);
-        for (size_t r = 0; r < act.size(); r++) {
-            const Active& a = act[r];
-            if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
-            push_cells(items, lev0[r] + d, (long)a.ns[d] * a.ns[d] * a.ns[d]);
+        if (!fold_prolong) {
+            begin(2, 0);
+            for (size_t r = 0; r < act.size(); r++) {
+                const Active& a = act[r];
+                if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
+                push_cells(items, lev0[r] + d, (long)a.ns[d] * a.ns[d] * a.ns[d]);
+            }
+            end();
         }
-        end();
         for (int sw = 0; sw < post; sw++) {
-            begin(0, sw & 1);
+            begin(0, sw & 1, sw == 0 && fold_prolong ? 2 : 0);
             for (size_t r = 0; r < act.size(); r++) {
                 const Active& a = act[r];
                 if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
@@ -682,7 +726,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
         if (!act[r].fresh) { push_cells(items, (int)r, (long)act[r].ns[0] * act[r].ns[0] * act[r].ns[0]); }
     }
     end();
-    begin(4, 0);
+    begin(4, 0, zero_in ? 0 : 1);
     for (size_t r = 0; r < act.size(); r++) { push_cells(items, (int)r, (long)act[r].ns[0] * act[r].ns[0] * act[r].ns[0]); }
     end();
     int rc = grow((void**)&S->d_lv, &S->cap_lv, sizeof(SLevel) * lv.size());
@@ -708,17 +752,26 @@ int run_plan(pm_server* S, int nact, hipStream_t s)
         case 0: {
             const size_t lds = L.lds;
             const long np = L.np;
+#define PM_SRV_SWEEP(E)                                                                                              \
+    if (L.in == 0) {                                                                                                 \
+        hipLaunchKernelGGL((k_srv_gsrb<E, 0>), dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir); \
+    } else if (L.in == 1) {                                                                                          \
+        hipLaunchKernelGGL((k_srv_gsrb<E, 1>), dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir); \
+    } else {                                                                                                         \
+        hipLaunchKernelGGL((k_srv_gsrb<E, 2>), dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir); \
+    }
             if (np <= 1024) {
-                hipLaunchKernelGGL(k_srv_gsrb<4>, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+                PM_SRV_SWEEP(4)
             } else if (np <= 2048) {
-                hipLaunchKernelGGL(k_srv_gsrb<8>, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+                PM_SRV_SWEEP(8)
             } else {
-                hipLaunchKernelGGL(k_srv_gsrb<16>, dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir);
+                PM_SRV_SWEEP(16)
             }
+#undef PM_SRV_SWEEP
             break;
         }
         case 1:
-            hipLaunchKernelGGL(k_srv_restrict, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
+            hipLaunchKernelGGL(k_srv_restrict, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it, L.in);
             break;
         case 2:
             hipLaunchKernelGGL(k_srv_prolong, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
@@ -729,7 +782,7 @@ int run_plan(pm_server* S, int nact, hipStream_t s)
             break;
         default:
             hipLaunchKernelGGL(k_srv_residual, dim3(L.count), dim3(256), 0, s, (const SReq*)S->d_rq,
-                               (const SLevel*)S->d_lv, it, S->d_norms);
+                               (const SLevel*)S->d_lv, it, S->d_norms, L.in);
         }
     }
     PM_CHECK_LAUNCH();
