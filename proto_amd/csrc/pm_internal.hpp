@@ -35,6 +35,20 @@ int layout_lows(const pm_layout* L, int* lows);
 int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
                         const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream);
+// pm_level_resnorm, pm_level_residual and pm_level_restrict_residual that
+// read the halo from the neighbouring boxes' valid cells instead of the ghost
+// layer, so no exchange has to precede them (pm_level.hip).  Same results as
+// the exchange followed by the C-ABI call; PM_ERR_UNSUPPORTED where the level
+// does not qualify (32..64-wide boxes, periodic, fully covered; restriction:
+// box-aligned levels with even box sizes).
+int level_resnorm_nb(const pm_layout* L, int dtype, const void* phi, const int phi_ghost[3], const void* rhs,
+                     const int rhs_ghost[3], double dx, void* out_dev, int combine, hipStream_t stream);
+int level_residual_nb(const pm_layout* L, int dtype, const void* phi, const int phi_ghost[3], const void* rhs,
+                      const int rhs_ghost[3], double dx, int out_dtype, void* out, const int out_ghost[3],
+                      void* norm_out, hipStream_t stream);
+int level_restrict_nb(const pm_layout* F, const pm_layout* C, int dtype, const void* phi, const int phi_ghost[3],
+                      const void* rhs, const int rhs_ghost[3], double dx, void* resc, const int resc_ghost[3],
+                      hipStream_t stream);
 // Kernel family (0 generic, 7, 27), dtype and coefficients of a stencil (pm_stencil.hip).
 int stencil_info(const pm_stencil* st, int* kind, int* dtype, std::vector<double>* coefs);
 

@@ -684,6 +684,201 @@ __global__ __launch_bounds__(256) void k_residual_z(LayoutDev L, G3 gp, G3 gr, G
     if (tid == 0) { partials[(long)b * gridDim.x + blockIdx.x] = m; }
 }
 
+// The z-marching residual of k_residual_z on a periodic level covering its
+// domain, with the halo staged from the neighbouring boxes' valid cells
+// (periodic images folded in) instead of the ghost layer: the values an
+// exchange would have put there, with no exchange.  Planes beyond the box go
+// through per-workgroup offset tables (s_dz, as in k_gsrb_fused).  MODE 0:
+// r = 0 + rhs + sum c_j phi into out (TO) and max|r| partials; 1: the norm of
+// 0 - rhs + sum c_j phi only; 2: residual-restriction, the coarse right-hand
+// side sum over the 2x2x2 children (z, y, x nested, x fastest) of avg * r into
+// resc on the box-aligned coarse level C (ghost gc), in the order of
+// k_restrict_residual_box; x neighbours come from the next lane.
+template<typename T, typename TO, int MODE>
+__global__ __launch_bounds__(256) void k_residual_nb(LayoutDev L, G3 gp, G3 gr, G3 go, const T* __restrict__ phi,
+                                                     const T* __restrict__ rhs, TO* __restrict__ out, Lap<T> c,
+                                                     T* __restrict__ partials, int zc, LayoutDev C, G3 gc,
+                                                     T* __restrict__ resc, T avg)
+{
+    constexpr int RY = RZ_RY, NL = (RZ_NE + 255) / 256;
+    __shared__ T sm[3][RZ_NE];
+    __shared__ T red[16];
+    __shared__ int s_nb[27], s_dz[18];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int b = blockIdx.y;
+    const int* lo = L.lows + 3 * b;
+    const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
+    const int nyt = (ny + RZ_TY - 1) / RZ_TY;
+    const int y0 = (int)(blockIdx.x % nyt) * RZ_TY;
+    const int z0 = (int)(blockIdx.x / nyt) * zc, z1 = min(z0 + zc, nz);
+    const int LX = nx + 2, NE = (RZ_TY + 2) * LX;
+    const long vol = gvol(L, gp);
+    const long XY = (long)(nx + 2 * gp.g[0]) * (ny + 2 * gp.g[1]);
+    const T* rh = rhs + (long)b * gvol(L, gr);
+    if (tid < 27) {
+        int p[3] = {lo[0] + (tid % 3 - 1) * nx, lo[1] + ((tid / 3) % 3 - 1) * ny, lo[2] + (tid / 9 - 1) * nz};
+        s_nb[tid] = owner(L, p);
+    }
+    __syncthreads();
+    auto at = [&](int x, int y, int z) {
+        const int ox = x < 0 ? -1 : (x >= nx ? 1 : 0);
+        const int oy = y < 0 ? -1 : (y >= ny ? 1 : 0);
+        const int oz = z < 0 ? -1 : (z >= nz ? 1 : 0);
+        const int o = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        return (long)o * vol + rel_index(L, gp, x - ox * nx, y - oy * ny, z - oz * nz);
+    };
+    if (tid < 18) {
+        // element of a plane beyond the box minus the in-box formula, per
+        // side and xy neighbour position (the same for every cell of it)
+        const int side = tid / 9, k = tid % 9;
+        const int x = k % 3 == 0 ? -1 : (k % 3 == 1 ? 0 : nx);
+        const int y = k / 3 == 0 ? -1 : (k / 3 == 1 ? 0 : ny);
+        const int z = side ? nz : -1;
+        s_dz[tid] = (int)(at(x, y, z) - (XY * z + at(x, y, 0)));
+    }
+    __syncthreads();
+    // staged element e: row y0 - 1 + e / LX, column e % LX - 1 (cells beyond
+    // the box belong to its neighbours); elements past the plane repeat the
+    // last one.  Level offsets fit 32 bits (checked by the host).
+    int goff[NL];
+    unsigned long long kp = 0;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = min(tid + 256 * k, NE - 1);
+        const int er = e / LX, ec = e - er * LX;
+        const int x = ec - 1, y = y0 - 1 + er;
+        goff[k] = (int)at(x, y, 0);
+        kp |= (unsigned long long)((x < 0 ? 0 : (x >= nx ? 2 : 1)) + 3 * (y < 0 ? 0 : (y >= ny ? 2 : 1))) << (4 * k);
+    }
+    T P[NL];
+    auto gload = [&](int z) {
+        const bool inb = z >= 0 && z < nz;
+        const int sd = z < 0 ? 0 : 9;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            long o = XY * z + goff[k];
+            if (!inb) { o += s_dz[sd + (int)((kp >> (4 * k)) & 15)]; }
+            P[k] = phi[o];
+        }
+    };
+    auto lstore = [&](T* pl) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const int e = tid + 256 * k;
+            if (e < NE) { pl[e] = P[k]; }
+        }
+    };
+    const int x = lane;
+    const bool xok = x < nx;
+    const int xc = xok ? x : 0;
+    T zw[3][RY];
+    T fq[RY], res[RY];
+    int zres = -1;
+    auto fload = [&](int zo) {
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int y = min(y0 + w * RY + r, ny - 1);
+            fq[r] = rh[rel_index(L, gr, xc, y, zo)];
+        }
+    };
+    T m = 0;
+    auto put_res = [&]() {
+        if (MODE != 0 || zres < 0 || !xok) { return; }
+        TO* ot = out + (long)b * gvol(L, go);
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int y = y0 + w * RY + r;
+            if (y < ny) { ot[rel_index(L, go, x, y, zres)] = (TO)res[r]; }
+        }
+    };
+    // MODE 2: coarse box of this fine box and the running sums of row pairs
+    T acc[RY / 2];
+    T* rc = resc;
+    int cox = 0, coy = 0, coz = 0;
+    if constexpr (MODE == 2) {
+        int p[3] = {lo[0] >> 1, lo[1] >> 1, lo[2] >> 1};
+        const int o = owner(C, p);
+        const int* clo = C.lows + 3 * o;
+        cox = (lo[0] >> 1) - clo[0];
+        coy = (lo[1] >> 1) - clo[1];
+        coz = (lo[2] >> 1) - clo[2];
+        rc = resc + (long)o * gvol(C, gc);
+    }
+    const int kb = z0 - 1;
+    gload(kb);
+    fload(z0);
+    for (int zz = kb; zz <= z1; zz++) {
+        T* sl = sm[(zz - kb) % 3];
+        lstore(sl);
+        if (zz + 1 <= z1) { gload(zz + 1); }
+        put_res();
+        zres = -1;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            zw[0][r] = zw[1][r];
+            zw[1][r] = zw[2][r];
+            zw[2][r] = sl[(w * RY + r + 1) * LX + (xc + 1)];
+        }
+        if (zz < z0 + 1) { continue; }
+        const int zo = zz - 1;
+        const T* s0 = sm[(zo - kb) % 3];
+        T fcur[RY];
+#pragma unroll
+        for (int r = 0; r < RY; r++) { fcur[r] = fq[r]; }
+        if (zo + 1 < z1) { fload(zo + 1); }
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            const int li = (w * RY + r + 1) * LX + (xc + 1);
+            T sv = 0;
+            sv = MODE == 1 ? sv - fcur[r] : sv + fcur[r];
+            sv = sv + c.c[0] * zw[1][r];
+            sv = sv + c.c[1] * s0[li + 1];
+            sv = sv + c.c[2] * s0[li - 1];
+            sv = sv + c.c[3] * s0[li + LX];
+            sv = sv + c.c[4] * s0[li - LX];
+            sv = sv + c.c[5] * zw[2][r];
+            sv = sv + c.c[6] * zw[0][r];
+            res[r] = sv;
+            if constexpr (MODE != 2) {
+                const bool ok = xok && (y0 + w * RY + r) < ny;
+                const T a = sv > 0 ? sv : -sv;
+                m = (ok && m < a) ? a : m;
+            }
+        }
+        if constexpr (MODE == 2) {
+            // children (x, y), (x+1, y), (x, y+1), (x+1, y+1) of plane zo, in
+            // that order, onto the sums of plane pair zo & ~1
+            const bool first = (zo & 1) == 0;
+#pragma unroll
+            for (int q = 0; q < RY / 2; q++) {
+                const T a0 = res[2 * q], a1 = __shfl_down(res[2 * q], 1, 64);
+                const T b0 = res[2 * q + 1], b1 = __shfl_down(res[2 * q + 1], 1, 64);
+                T t = first ? (T)0 : acc[q];
+                t = t + avg * a0;
+                t = t + avg * a1;
+                t = t + avg * b0;
+                t = t + avg * b1;
+                acc[q] = t;
+            }
+            if (!first && xok && (x & 1) == 0) {
+#pragma unroll
+                for (int q = 0; q < RY / 2; q++) {
+                    const int y = y0 + w * RY + 2 * q;
+                    if (y < ny) { rc[rel_index(C, gc, cox + (x >> 1), coy + (y >> 1), coz + (zo >> 1))] = acc[q]; }
+                }
+            }
+        } else {
+            zres = zo;
+        }
+    }
+    put_res();
+    if constexpr (MODE != 2) {
+        m = block_reduce(PM_RED_ABS, m, red);
+        if (tid == 0) { partials[(long)b * gridDim.x + blockIdx.x] = m; }
+    }
+}
+
 // boxes and ghosts the z-marching residual takes, and its grid (16-row tiles x
 // z chunks, halved while fewer than 2048 workgroups)
 static bool rz_ok(const pm_layout* L, const G3& gp)
@@ -1519,6 +1714,122 @@ int pm_level_residual(const pm_layout* L, int dtype, const void* phi, const int 
     PM_CHECK_LAUNCH();
     return PM_OK;
 }
+
+}  // extern "C"
+
+namespace pm {
+
+// Levels whose residual kernels can stage halos from the neighbouring boxes
+// (k_residual_nb): 32..64-wide boxes as k_residual_z takes, periodic in every
+// direction, every box present, fewer than 2^31 elements.
+static bool nb_ok(const pm_layout* L, const G3& gp)
+{
+    if (!rz_ok(L, gp) || L->nboxes != L->nb[0] * L->nb[1] * L->nb[2]) { return false; }
+    for (int k = 0; k < 3; k++) {
+        if (!L->periodic[k]) { return false; }
+    }
+    return gvol(ldev(L), gp) * L->nboxes < (1L << 31);
+}
+
+int level_resnorm_nb(const pm_layout* L, int dtype, const void* phi, const int phi_ghost[3], const void* rhs,
+                     const int rhs_ghost[3], double dx, void* out_dev, int combine, hipStream_t stream)
+{
+    PM_REQUIRE(L && phi && rhs && out_dev, "pm_level_resnorm: bad argument");
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost);
+    if (!nb_ok(L, gp)) {
+        set_error("pm_level_resnorm: this level needs its ghost cells exchanged");
+        return PM_ERR_UNSUPPORTED;
+    }
+    const LayoutDev D = ldev(L);
+    int zc = 0;
+    const dim3 grid = rz_grid(L, &zc);
+    const double hsqinv = 1. / (dx * dx);
+    void* ws = nullptr;
+    PM_DISPATCH_T(dtype, {
+        int rc = workspace(stream, sizeof(T) * (size_t)grid.x * L->nboxes, &ws);
+        if (rc) { return rc; }
+        Lap<T> c;
+        for (int j = 0; j < 7; j++) { c.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)hsqinv; }
+        hipLaunchKernelGGL((k_residual_nb<T, T, 1>), grid, dim3(256), 0, stream, D, gp, gr, gr, (const T*)phi,
+                           (const T*)rhs, (T*)nullptr, c, (T*)ws, zc, D, gp, (T*)nullptr, (T)0);
+        hipLaunchKernelGGL(k_max_partials<T>, dim3(1), dim3(1024), 0, stream, (const T*)ws,
+                           (long)grid.x * L->nboxes, (T*)out_dev, combine);
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int level_residual_nb(const pm_layout* L, int dtype, const void* phi, const int phi_ghost[3], const void* rhs,
+                      const int rhs_ghost[3], double dx, int out_dtype, void* out, const int out_ghost[3],
+                      void* norm_out, hipStream_t stream)
+{
+    PM_REQUIRE(L && phi && rhs && out && norm_out, "pm_level_residual: bad argument");
+    PM_REQUIRE(dtype == PM_F64 || (dtype == PM_F32 && out_dtype == PM_F32), "pm_level_residual: unsupported dtypes");
+    PM_REQUIRE(out_dtype == PM_F32 || out_dtype == PM_F64, "pm_level_residual: bad out dtype");
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost), go = g3(out_ghost);
+    if (!nb_ok(L, gp)) {
+        set_error("pm_level_residual: this level needs its ghost cells exchanged");
+        return PM_ERR_UNSUPPORTED;
+    }
+    const LayoutDev D = ldev(L);
+    int zc = 0;
+    const dim3 grid = rz_grid(L, &zc);
+    const double hsqinv = 1. / (dx * dx);
+    void* ws = nullptr;
+    PM_DISPATCH_T(dtype, {
+        int rc = workspace(stream, sizeof(T) * (size_t)grid.x * L->nboxes, &ws);
+        if (rc) { return rc; }
+        Lap<T> c;
+        for (int j = 0; j < 7; j++) { c.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)(-hsqinv); }
+        if (out_dtype == PM_F32) {
+            hipLaunchKernelGGL((k_residual_nb<T, float, 0>), grid, dim3(256), 0, stream, D, gp, gr, go,
+                               (const T*)phi, (const T*)rhs, (float*)out, c, (T*)ws, zc, D, gp, (T*)nullptr, (T)0);
+        } else {
+            hipLaunchKernelGGL((k_residual_nb<T, double, 0>), grid, dim3(256), 0, stream, D, gp, gr, go,
+                               (const T*)phi, (const T*)rhs, (double*)out, c, (T*)ws, zc, D, gp, (T*)nullptr, (T)0);
+        }
+        hipLaunchKernelGGL(k_max_partials<T>, dim3(1), dim3(1024), 0, stream, (const T*)ws,
+                           (long)grid.x * L->nboxes, (T*)norm_out, 0);
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+int level_restrict_nb(const pm_layout* F, const pm_layout* C, int dtype, const void* phi, const int phi_ghost[3],
+                      const void* rhs, const int rhs_ghost[3], double dx, void* resc, const int resc_ghost[3],
+                      hipStream_t stream)
+{
+    PM_REQUIRE(F && C && phi && rhs && resc, "pm_level_restrict_residual: bad argument");
+    const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost), gc = g3(resc_ghost);
+    if (!nb_ok(F, gp) || !box_aligned(F, C) || F->bs[0] % 2 || F->bs[1] % 2 || F->bs[2] % 2) {
+        set_error("pm_level_restrict_residual: this level needs its ghost cells exchanged");
+        return PM_ERR_UNSUPPORTED;
+    }
+    for (int k = 0; k < 3; k++) {
+        PM_REQUIRE(C->domain[k] * 2 == F->domain[k], "pm_level_restrict_residual: coarse domain is not fine/2");
+    }
+    const LayoutDev DF = ldev(F), DC = ldev(C);
+    int zc = 0;
+    const dim3 grid = rz_grid(F, &zc);
+    if (zc % 2) {
+        set_error("pm_level_restrict_residual: odd z chunk");
+        return PM_ERR_UNSUPPORTED;
+    }
+    const double hsqinv = 1. / (dx * dx);
+    PM_DISPATCH_T(dtype, {
+        Lap<T> c;
+        for (int j = 0; j < 7; j++) { c.c[j] = (T)(j == 0 ? -2.0 * F->dim : 1.0) * (T)(-hsqinv); }
+        const T avg = (T)(1.0 / 8);
+        hipLaunchKernelGGL((k_residual_nb<T, T, 2>), grid, dim3(256), 0, stream, DF, gp, gr, gr, (const T*)phi,
+                           (const T*)rhs, (T*)nullptr, c, (T*)nullptr, zc, DC, gc, (T*)resc, avg);
+    });
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
+}  // namespace pm
+
+extern "C" {
 
 int pm_level_add_correction(const pm_layout* L, int dtype, void* phi, const int phi_ghost[3], int e_dtype,
                             const void* e, const int e_ghost[3], pm_stream_t stream)

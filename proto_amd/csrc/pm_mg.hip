@@ -164,9 +164,17 @@ static int vcycle(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, hipSt
         void* cphi = f32 ? C.phi32 : C.phi;
         void* crhs = f32 ? C.rhs32 : C.rhs;
         MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, pre, s, zin));
-        // coarseResidual (51-73): exchange, residual, AvgDown, copy to the coarse layout
-        MG_TRY(pm_level_exchange(L.layout, dtype, phi, mg->g1, 1, s));
-        MG_TRY(pm_level_restrict_residual(L.layout, C.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, crhs, mg->g0, s));
+        // coarseResidual (51-73): exchange, residual, AvgDown, copy to the coarse
+        // layout; where the level qualifies the halo is read from the
+        // neighbouring boxes instead of an exchange
+        const int rc = level_restrict_nb(L.layout, C.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, crhs, mg->g0, s);
+        if (rc == PM_ERR_UNSUPPORTED) {
+            MG_TRY(pm_level_exchange(L.layout, dtype, phi, mg->g1, 1, s));
+            MG_TRY(pm_level_restrict_residual(L.layout, C.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, crhs, mg->g0,
+                                              s));
+        } else if (rc != PM_OK) {
+            return rc;
+        }
         MG_TRY(vcycle(mg, l + 1, dtype, cphi, crhs, s, true));  // m_delta.setToZero() folded into its first sweep
         // fineInterp (125-140): phi += I(delta), folded into the first post sweep for Constant
         if (mg->opt.interp == 0) {
@@ -188,12 +196,36 @@ static int read_norm(pm_mg* mg, double* out, hipStream_t s)
     return PM_OK;
 }
 
+// resnorm (LevelMultigrid.cpp:75-97) of level 0; the exchange is skipped
+// where the norm kernel reads the neighbouring boxes itself (pm_mg_solve
+// exchanges phi once at its end, so callers see the ghosts Proto leaves)
 static int resnorm(pm_mg* mg, void* phi, const void* rhs, double* out, hipStream_t s)
 {
     auto& L = mg->lv[0];
-    MG_TRY(pm_level_exchange(L.layout, PM_F64, phi, mg->g1, 1, s));
-    MG_TRY(pm_level_resnorm(L.layout, PM_F64, phi, mg->g1, rhs, mg->g0, L.dx, mg->d_norm, 0, s));
+    const int rc = level_resnorm_nb(L.layout, PM_F64, phi, mg->g1, rhs, mg->g0, L.dx, mg->d_norm, 0, s);
+    if (rc == PM_ERR_UNSUPPORTED) {
+        MG_TRY(pm_level_exchange(L.layout, PM_F64, phi, mg->g1, 1, s));
+        MG_TRY(pm_level_resnorm(L.layout, PM_F64, phi, mg->g1, rhs, mg->g0, L.dx, mg->d_norm, 0, s));
+    } else if (rc != PM_OK) {
+        return rc;
+    }
     return read_norm(mg, out, s);
+}
+
+// FP64 residual of level 0, rounded into the FP32 right-hand side, and its norm
+static int residual32(pm_mg* mg, void* phi, const void* rhs, hipStream_t s)
+{
+    auto& L = mg->lv[0];
+    const int rc = level_residual_nb(L.layout, PM_F64, phi, mg->g1, rhs, mg->g0, L.dx, PM_F32, L.rhs32, mg->g0,
+                                     mg->d_norm, s);
+    if (rc == PM_ERR_UNSUPPORTED) {
+        MG_TRY(pm_level_exchange(L.layout, PM_F64, phi, mg->g1, 1, s));
+        MG_TRY(pm_level_residual(L.layout, PM_F64, phi, mg->g1, rhs, mg->g0, L.dx, PM_F32, L.rhs32, mg->g0,
+                                 mg->d_norm, s));
+    } else if (rc != PM_OK) {
+        return rc;
+    }
+    return PM_OK;
 }
 
 // One step of the FP32-preconditioned refinement after the residual r32 of
@@ -208,9 +240,7 @@ static int ir_correct(pm_mg* mg, void* phi, hipStream_t s)
 
 static int ir_residual(pm_mg* mg, void* phi, const void* rhs, double* out, hipStream_t s)
 {
-    auto& L = mg->lv[0];
-    MG_TRY(pm_level_exchange(L.layout, PM_F64, phi, mg->g1, 1, s));
-    MG_TRY(pm_level_residual(L.layout, PM_F64, phi, mg->g1, rhs, mg->g0, L.dx, PM_F32, L.rhs32, mg->g0, mg->d_norm, s));
+    MG_TRY(residual32(mg, phi, rhs, s));
     return read_norm(mg, out, s);
 }
 
@@ -218,11 +248,8 @@ static int ir_residual(pm_mg* mg, void* phi, const void* rhs, double* out, hipSt
 static int cycle_body(pm_mg* mg, void* phi, const void* rhs, hipStream_t s)
 {
     if (mg->opt.precision == 0) { return vcycle(mg, 0, PM_F64, phi, rhs, s); }
-    auto& L = mg->lv[0];
     MG_TRY(ir_correct(mg, phi, s));
-    MG_TRY(pm_level_exchange(L.layout, PM_F64, phi, mg->g1, 1, s));
-    MG_TRY(pm_level_residual(L.layout, PM_F64, phi, mg->g1, rhs, mg->g0, L.dx, PM_F32, L.rhs32, mg->g0, mg->d_norm, s));
-    return PM_OK;
+    return residual32(mg, phi, rhs, s);
 }
 
 static int run_cycle(pm_mg* mg, void* phi, const void* rhs, hipStream_t s)
@@ -393,6 +420,7 @@ int pm_mg_vcycle(pm_mg* mg, void* phi, const void* rhs, pm_stream_t stream)
 int pm_mg_resnorm(pm_mg* mg, void* phi, const void* rhs, double* out_host, pm_stream_t stream)
 {
     PM_REQUIRE(mg && phi && rhs && out_host, "pm_mg_resnorm: null argument");
+    MG_TRY(pm_level_exchange(mg->lv[0].layout, PM_F64, phi, mg->g1, 1, S(stream)));  // as Proto's resnorm does
     return resnorm(mg, phi, rhs, out_host, S(stream));
 }
 
@@ -422,6 +450,8 @@ int pm_mg_solve(pm_mg* mg, void* phi, const void* rhs, double* hist, int* iters,
         }
     }
     if (iters) { *iters = it; }
+    // the ghost cells of phi as Proto's last resnorm leaves them (exchanged)
+    MG_TRY(pm_level_exchange(mg->lv[0].layout, PM_F64, phi, mg->g1, 1, s));
     if (!ok) {
         set_error("pm_mg_solve: %d cycles left the residual at %.3e of the initial one (tol %.1e)", it,
                   res0 > 0 ? r / res0 : 0.0, mg->opt.tol);
