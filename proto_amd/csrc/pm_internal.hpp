@@ -30,11 +30,14 @@ int layout_lows(const pm_layout* L, int* lows);
 // phi_in; 1 sweeps a zero initial guess without reading it (phi_in unused);
 // 2 sweeps phi_in + the constant prolongation of `coarse` (layout C, levels
 // box-aligned), the cells pm_level_prolong_add(interp 0) would have left in
-// phi_in, which is not written.  PM_ERR_UNSUPPORTED where the one-pass sweep
-// does not apply.
+// phi_in, which is not written.  acc (FP32 sweeps, in_mode 0): phi_out is not
+// written; the result is added to the FP64 array acc (layout and ghosts of
+// phi_out) as pm_level_add_correction adds it.  PM_ERR_UNSUPPORTED where the
+// one-pass sweep does not apply.
 int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
-                        const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream);
+                        const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream,
+                        double* acc = nullptr);
 // pm_level_resnorm, pm_level_residual and pm_level_restrict_residual that
 // read the halo from the neighbouring boxes' valid cells instead of the ghost
 // layer, so no exchange has to precede them (pm_level.hip).  Same results as

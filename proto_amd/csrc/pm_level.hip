@@ -124,11 +124,14 @@ __global__ __launch_bounds__(256) void k_gsrb(LayoutDev L, G3 gp, G3 gr, T* __re
 // sweep of a zero initial guess); 2 pin + the constant prolongation of the
 // coarse level crs (layout C, ghost gc), rounded as k_prolong_const_box
 // rounds it, so the sweep sees the cells fineInterp would have written.
-template<typename T, int MAXE, int IN>
+// OUT = 1: instead of storing its result, the sweep adds it to the FP64 array
+// pacc (layout and ghosts of pout) as k_add_correction adds a correction, and
+// pout is not written (the last sweep of the FP32 preconditioner).
+template<typename T, int MAXE, int IN, int OUT>
 __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout, G3 gr, const T* __restrict__ pin,
                                                     T* __restrict__ pout, const T* __restrict__ rhs, Lap<T> c,
                                                     T lambda, int rows, int zc, LayoutDev C, G3 gc,
-                                                    const T* __restrict__ crs)
+                                                    const T* __restrict__ crs, double* __restrict__ pacc)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
     __shared__ int s_nb[27];  // box owning each of the 27 neighbouring box positions (13 = this box)
@@ -332,12 +335,41 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     commit(z0 - 1);
     issue(z0);
     issue_r(z0 - 1);
+    // OUT == 1: this thread's output cells (q = tid + 256 i < cnt) as box
+    // offsets at plane 0 and LDS offsets; their pacc values of the plane
+    // finished next are loaded one step ahead, clamped like the staging loads
+    constexpr int NO = OUT == 1 ? MAXE : 1;
+    int oo[NO], ol[NO];
+    double acc_in[NO];
+    const int cnt = (y1 - y0) * nx;
+    const long XYo = (long)(nx + 2 * gout.g[0]) * (ny + 2 * gout.g[1]);
+    double* oacc = pacc + (long)b * gvol(L, gout);
+    if constexpr (OUT == 1) {
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int q = min((int)threadIdx.x + 256 * i, cnt - 1), r = q / nx, xx = q - r * nx;
+            oo[i] = (int)rel_index(L, gout, xx, y0 + r, 0);
+            ol[i] = (r + 2) * W + xx + 2;
+        }
+    }
+    auto issue_o = [&](int z) {
+        if constexpr (OUT == 1) {
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) { acc_in[i] = oacc[XYo * z + oo[i]]; }
+        }
+    };
     auto write_out = [&](int z) {
         const T* pl = slot(z);
-        const int cnt = (y1 - y0) * nx;
-        for (int e = threadIdx.x; e < cnt; e += 256) {
-            const int r = e / nx, xx = e - r * nx;
-            ob[rel_index(L, gout, xx, y0 + r, z)] = pl[(r + 2) * W + xx + 2];
+        if constexpr (OUT == 1) {
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                if ((int)threadIdx.x + 256 * i < cnt) { oacc[XYo * z + oo[i]] = acc_in[i] + (double)pl[ol[i]]; }
+            }
+        } else {
+            for (int e = threadIdx.x; e < cnt; e += 256) {
+                const int r = e / nx, xx = e - r * nx;
+                ob[rel_index(L, gout, xx, y0 + r, z)] = pl[(r + 2) * W + xx + 2];
+            }
         }
     };
     // Memory operations of a step are all issued at its start (loads of the
@@ -350,6 +382,7 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         if (k + 3 <= z1 + 1) { issue(k + 3); }
         if (k + 2 <= z1) { issue_r(k + 2); }
         if (k - 1 >= z0) { write_out(k - 1); }  // slot of plane k-1: only read during this step
+        if (k >= z0) { issue_o(k); }           // OUT == 1: pacc of plane k, written next step
         __syncthreads();
         update(k + 1, y0 - 1, y1 + 1, -1, nx + 1, 0, wred);
         __syncthreads();
@@ -1295,23 +1328,27 @@ int pm_level_gsrb(const pm_layout* L, int dtype, void* phi, const int phi_ghost[
 
 namespace pm {
 
-#define PM_SWEEP_LAUNCH(E, M)                                                                                 \
-    hipLaunchKernelGGL((k_gsrb_fused<T, E, M>), grid, dim3(256), lds, stream, D, gi, go, gr, pi, (T*)phi_out, \
-                       (const T*)rhs, c, lt, rows, zc, DC, gc, cr)
-#define PM_SWEEP_MODES(E)      \
-    if (in_mode == 0) {        \
-        PM_SWEEP_LAUNCH(E, 0); \
-    } else if (in_mode == 1) { \
-        PM_SWEEP_LAUNCH(E, 1); \
-    } else {                   \
-        PM_SWEEP_LAUNCH(E, 2); \
+#define PM_SWEEP_LAUNCH(E, M, O)                                                                                 \
+    hipLaunchKernelGGL((k_gsrb_fused<T, E, M, O>), grid, dim3(256), lds, stream, D, gi, go, gr, pi, (T*)phi_out, \
+                       (const T*)rhs, c, lt, rows, zc, DC, gc, cr, acc)
+#define PM_SWEEP_MODES(E)         \
+    if (acc) {                    \
+        PM_SWEEP_LAUNCH(E, 0, 1); \
+    } else if (in_mode == 0) {    \
+        PM_SWEEP_LAUNCH(E, 0, 0); \
+    } else if (in_mode == 1) {    \
+        PM_SWEEP_LAUNCH(E, 1, 0); \
+    } else {                      \
+        PM_SWEEP_LAUNCH(E, 2, 0); \
     }
 
 int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
-                        const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream)
+                        const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream,
+                        double* acc)
 {
     PM_REQUIRE(L && phi_out && rhs && (phi_in || in_mode == 1), "pm_level_gsrb_sweep: bad argument");
+    PM_REQUIRE(!acc || (dtype == PM_F32 && in_mode == 0), "pm_level_gsrb_sweep: the FP64 accumulation is for FP32 sweeps");
     PM_REQUIRE(in_mode == 1 || phi_in != phi_out, "pm_level_gsrb_sweep: the sweep is out of place (phi_in != phi_out)");
     PM_REQUIRE(in_mode >= 0 && in_mode <= 2, "pm_level_gsrb_sweep: bad input mode");
     if (L->dim != 3) {
@@ -1398,7 +1435,7 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
 {
     PM_REQUIRE(phi_in, "pm_level_gsrb_sweep: bad argument");
     return level_gsrb_sweep_in(L, dtype, phi_in, in_ghost, phi_out, out_ghost, rhs, rhs_ghost, dx, 0, nullptr, nullptr,
-                               nullptr, S(stream));
+                               nullptr, S(stream), nullptr);
 }
 
 int pm_level_jacobi(const pm_layout* L, int dtype, void* phi, const int phi_ghost[3], const void* rhs,

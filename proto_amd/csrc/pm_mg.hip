@@ -97,8 +97,11 @@ static void mg_free(pm_mg* mg)
 // yet written (m_delta.setToZero()); 2: phi still lacks fineInterp's
 // constant prolongation of cphi (level l + 1).  The first one-pass sweep reads
 // such an input directly; otherwise it is written first, as Proto does.
+// acc64 (FP32 level 0 of the refinement): the last sweep adds its result to
+// the FP64 solution instead of storing it (*folded = true), which is then the
+// correction phi += e.
 static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* tmp, int n, hipStream_t s,
-                 int in_mode = 0, const void* cphi = nullptr)
+                 int in_mode = 0, const void* cphi = nullptr, double* acc64 = nullptr, bool* folded = nullptr)
 {
     const auto& L = mg->lv[l];
     auto prepare = [&]() -> int {
@@ -131,7 +134,13 @@ static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* 
                 if (rc == PM_ERR_UNSUPPORTED) { break; }
                 if (rc != PM_OK) { return rc; }
             }
-            MG_TRY(pm_level_gsrb_sweep(L.layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0, L.dx, s));
+            if (acc64 && it + 2 == n) {
+                MG_TRY(level_gsrb_sweep_in(L.layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0, L.dx, 0, nullptr,
+                                           nullptr, nullptr, s, acc64));
+                *folded = true;
+            } else {
+                MG_TRY(pm_level_gsrb_sweep(L.layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0, L.dx, s));
+            }
         }
     }
     MG_TRY(prepare());
@@ -150,8 +159,10 @@ static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* 
 }
 
 // vCycle (LevelMultigrid.cpp:142-160) on level l with (phi, rhs) of dtype;
-// zero_in: phi is the zero initial guess of a correction, not yet written.
-static int vcycle(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, hipStream_t s, bool zero_in = false)
+// zero_in: phi is the zero initial guess of a correction, not yet written;
+// acc64 / folded: see relax (level l's last sweep).
+static int vcycle(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, hipStream_t s, bool zero_in = false,
+                  double* acc64 = nullptr, bool* folded = nullptr)
 {
     auto& L = mg->lv[l];
     const bool f32 = dtype == PM_F32;
@@ -178,13 +189,13 @@ static int vcycle(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, hipSt
         MG_TRY(vcycle(mg, l + 1, dtype, cphi, crhs, s, true));  // m_delta.setToZero() folded into its first sweep
         // fineInterp (125-140): phi += I(delta), folded into the first post sweep for Constant
         if (mg->opt.interp == 0) {
-            MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, post, s, 2, cphi));
+            MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, post, s, 2, cphi, acc64, folded));
         } else {
             MG_TRY(pm_level_prolong_add(L.layout, C.layout, dtype, cphi, mg->g1, phi, mg->g1, mg->opt.interp, s));
-            MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, post, s));
+            MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, post, s, 0, nullptr, acc64, folded));
         }
     } else {
-        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, bottom, s, zin));
+        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, bottom, s, zin, nullptr, acc64, folded));
     }
     return PM_OK;
 }
@@ -233,8 +244,10 @@ static int residual32(pm_mg* mg, void* phi, const void* rhs, hipStream_t s)
 static int ir_correct(pm_mg* mg, void* phi, hipStream_t s)
 {
     auto& L = mg->lv[0];
-    MG_TRY(vcycle(mg, 0, PM_F32, L.phi32, L.rhs32, s, true));  // e = 0 folded into its first sweep
-    MG_TRY(pm_level_add_correction(L.layout, PM_F64, phi, mg->g1, PM_F32, L.phi32, mg->g1, s));
+    // e = 0 folded into the first sweep, phi += e into the last where it can be
+    bool folded = false;
+    MG_TRY(vcycle(mg, 0, PM_F32, L.phi32, L.rhs32, s, true, (double*)phi, &folded));
+    if (!folded) { MG_TRY(pm_level_add_correction(L.layout, PM_F64, phi, mg->g1, PM_F32, L.phi32, mg->g1, s)); }
     return PM_OK;
 }
 
