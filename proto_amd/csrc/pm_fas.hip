@@ -118,10 +118,14 @@ __global__ __launch_bounds__(256) void k_fas_relax(LayoutDev L, G3 gu, G3 gf, co
 // is computed; results are stored at the start of the next step, after those
 // loads.  The arithmetic is fas_L's, term by term.  MODE 0: Jacobi relaxation
 // out = u + (-lambda)(L(u) + f); MODE 1: per-block max |L(u) + f| (and the
-// residual itself if res_out).
+// residual itself if res_out).  NB = 1 (a periodic level covering its domain):
+// the halo is staged from the neighbouring boxes' valid cells instead of the
+// ghost layer -- the values an exchange would put there, with no exchange;
+// planes beyond the box go through per-workgroup offset tables (s_dz: per side
+// and xy neighbour position, the same for both planes of a side).
 constexpr int FZ_RY = 4, FZ_TY = 4 * FZ_RY, FZ_NE = (FZ_TY + 4) * 68;
 
-template<int MODE>
+template<int MODE, int NB>
 __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __restrict__ u,
                                                   const double* __restrict__ f, double* __restrict__ out, FasC c,
                                                   double* __restrict__ part, double* __restrict__ res_out, int zc)
@@ -129,6 +133,7 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
     constexpr int RY = FZ_RY, NL = (FZ_NE + 255) / 256;
     __shared__ double sm[4][FZ_NE];
     __shared__ double red[16];
+    __shared__ int s_nb[NB ? 27 : 1], s_dz[NB ? 18 : 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int b = blockIdx.y;
     const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
@@ -140,20 +145,66 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
     const double* ub = u + (long)b * vol;
     const long nv = (long)nx * ny * nz;
     const double* fb = f + (long)b * nv;
-    // staged element e: storage row y0 + e / LX (clamped to the last ghost
-    // row), storage column e % LX; elements past the plane repeat the last one
+    // NB: element of box-relative cell (x, y, z), possibly in a neighbour
+    const int* lo = L.lows + 3 * b;
+    auto at = [&](int x, int y, int z) {
+        const int ox = x < 0 ? -1 : (x >= nx ? 1 : 0);
+        const int oy = y < 0 ? -1 : (y >= ny ? 1 : 0);
+        const int oz = z < 0 ? -1 : (z >= nz ? 1 : 0);
+        const int o = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        return (long)o * vol + (x - ox * nx + 2) + X * ((y - oy * ny + 2) + (long)(ny + 4) * (z - oz * nz + 2));
+    };
+    if constexpr (NB) {
+        if (tid < 27) {
+            int p[3] = {lo[0] + (tid % 3 - 1) * nx, lo[1] + ((tid / 3) % 3 - 1) * ny, lo[2] + (tid / 9 - 1) * nz};
+            s_nb[tid] = owner(L, p);
+        }
+        __syncthreads();
+        if (tid < 18) {
+            const int side = tid / 9, k = tid % 9;
+            const int x = k % 3 == 0 ? -1 : (k % 3 == 1 ? 0 : nx);
+            const int y = k / 3 == 0 ? -1 : (k / 3 == 1 ? 0 : ny);
+            const int z = side ? nz : -1;
+            s_dz[tid] = (int)(at(x, y, z) - (XY * z + at(x, y, 0)));
+        }
+        __syncthreads();
+    }
+    // staged element e: row y0 - 2 + e / LX, column e % LX - 2 of the box.
+    // Ghost staging: storage row clamped to the last ghost row.  NB: level
+    // offsets (fewer than 2^31 elements, checked by the host) and the xy
+    // neighbour position, 4 bits per element.  Elements past the plane repeat
+    // the last one.
     long goff[NL];
+    unsigned long long kp = 0;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
         const int e = min(tid + 256 * k, NE - 1);
         const int er = e / LX, ec = e - er * LX;
-        goff[k] = X * (long)min(y0 + er, ny + 3) + ec;
+        if constexpr (NB) {
+            const int xx = ec - 2, yy = y0 - 2 + er;
+            goff[k] = (int)at(xx, yy, 0);
+            kp |= (unsigned long long)((xx < 0 ? 0 : (xx >= nx ? 2 : 1)) + 3 * (yy < 0 ? 0 : (yy >= ny ? 2 : 1)))
+                  << (4 * k);
+        } else {
+            goff[k] = X * (long)min(y0 + er, ny + 3) + ec;
+        }
     }
     double P[NL];
     auto gload = [&](int z) {
-        const double* base = ub + XY * (long)(z + 2);
+        if constexpr (NB) {
+            const bool inb = z >= 0 && z < nz;
+            const int sd = z < 0 ? 0 : 9;
 #pragma unroll
-        for (int k = 0; k < NL; k++) { P[k] = base[goff[k]]; }
+            for (int k = 0; k < NL; k++) {
+                long o = XY * z + goff[k];
+                if (!inb) { o += s_dz[sd + (int)((kp >> (4 * k)) & 15)]; }
+                P[k] = u[o];
+            }
+        } else {
+            const double* base = ub + XY * (long)(z + 2);
+#pragma unroll
+            for (int k = 0; k < NL; k++) { P[k] = base[goff[k]]; }
+        }
     };
     auto lstore = [&](double* pl) {
 #pragma unroll
@@ -397,6 +448,17 @@ static const int G2[3] = {2, 2, 2}, G0[3] = {0, 0, 0};
 // boxes the z-marching operator kernel takes: 32..64 cells wide, at least a tile
 // tall and deep
 static bool fz_ok(const LayoutDev& D) { return D.bs[0] >= 32 && D.bs[0] <= 64 && D.bs[1] >= 16 && D.bs[2] >= 8; }
+// levels whose z-marching operator can read the neighbouring boxes instead of
+// the ghost layer: periodic, every box present, fewer than 2^31 elements
+static bool fz_nb(const pm_layout* L)
+{
+    const LayoutDev D = ldev(L);
+    if (!fz_ok(D) || L->nboxes != L->nb[0] * L->nb[1] * L->nb[2]) { return false; }
+    for (int k = 0; k < 3; k++) {
+        if (!L->periodic[k]) { return false; }
+    }
+    return gvol(D, g3(G2)) * L->nboxes < (1L << 31);
+}
 
 // its grid: 16-row tiles x z chunks (halved while fewer than 2048 workgroups) x boxes
 static dim3 fz_grid(const LayoutDev& D, int* zc)
@@ -417,10 +479,20 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
     void* buf[2] = {u, Lv.tmp};
     int zc = 0;
     const dim3 zgrid = fz_grid(D, &zc);
+    const bool nb = fz_nb(Lv.layout);
     for (int it = 0; it < n; it++) {
+        if (nb) {
+            // halo read from the neighbouring boxes of the input buffer (the
+            // relaxation is out of place): no exchange
+            hipLaunchKernelGGL((k_fas_op_z<0, 1>), zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
+                               (const double*)f, (double*)buf[(it + 1) & 1], c, (double*)nullptr, (double*)nullptr,
+                               zc);
+            PM_CHECK_LAUNCH();
+            continue;
+        }
         FAS_TRY(pm_level_exchange(Lv.layout, PM_F64, buf[it & 1], G2, 1, s));
         if (fz_ok(D)) {
-            hipLaunchKernelGGL(k_fas_op_z<0>, zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
+            hipLaunchKernelGGL((k_fas_op_z<0, 0>), zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
                                (const double*)f, (double*)buf[(it + 1) & 1], c, (double*)nullptr, (double*)nullptr,
                                zc);
         } else {
@@ -471,7 +543,7 @@ static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream
         int zc = 0;
         const dim3 zgrid = fz_grid(D, &zc);
         bpb = zgrid.x;  // no more partials than the generic kernel's (<= 64 per box)
-        hipLaunchKernelGGL(k_fas_op_z<1>, zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
+        hipLaunchKernelGGL((k_fas_op_z<1, 0>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
                            (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc);
     } else {
         hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
