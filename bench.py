@@ -257,6 +257,50 @@ def cpu_reference(nbox=200):
     return out
 
 
+def mg_one_solve(pa, stream, which, n=512, bs=64, nlev=6):
+    """One config-3 solve for the profiler: which = mg_fp64 / mg_ir (GSRB
+    V(2,2), fused sweeps, no graph) or mg_fas (LevelMultigridSolver)."""
+    import numpy as np
+    from proto_amd.device import DeviceArray
+    from proto_amd.inputs import gaussian_level_rhs
+    nb = n // bs
+    lows = np.array([(x * bs, y * bs, z * bs) for z in range(nb) for y in range(nb) for x in range(nb)],
+                    dtype=np.int32).reshape(-1)
+    lay = ctypes.c_void_p()
+    pa.call("pm_layout_create", ctypes.byref(lay), 3, pa.i3([n] * 3), pa.i3([bs] * 3), pa.i3([1] * 3), nb ** 3,
+            lows.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    rhs = DeviceArray.zeros(nb ** 3 * bs ** 3)
+    gaussian_level_rhs(lay, n, rhs, 16, 2, stream=stream.h)
+    hist = np.zeros(41)
+    it = ctypes.c_int()
+    if which == "mg_fas":
+        h = ctypes.c_void_p()
+        pa.call("pm_fas_create", ctypes.byref(h), lay, nlev + 1, 1.0 / n)
+        st = DeviceArray.zeros(nb ** 3 * (bs + 4) ** 3)
+        stream.sync()
+        t0 = time.perf_counter()
+        pa.call("pm_fas_solve", h, st.ptr, rhs.ptr, 40, 1e-10, hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                ctypes.byref(it), stream.h)
+        dt = time.perf_counter() - t0
+        pa.call("pm_fas_destroy", h)
+    else:
+        opt = pa.MgOptions()
+        pa.lib().pm_mg_default_options(ctypes.byref(opt))
+        opt.relax, opt.pre, opt.post, opt.bottom, opt.fused = 0, 2, 2, 10, 1
+        opt.precision, opt.max_iter, opt.tol, opt.use_graph = (1 if which == "mg_ir" else 0), 40, 1e-10, 0
+        h = ctypes.c_void_p()
+        pa.call("pm_mg_create", ctypes.byref(h), lay, nlev, 1.0 / n, ctypes.byref(opt))
+        phi = DeviceArray.zeros(nb ** 3 * (bs + 2) ** 3)
+        stream.sync()
+        t0 = time.perf_counter()
+        pa.call("pm_mg_solve", h, phi.ptr, rhs.ptr, hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                ctypes.byref(it), stream.h)
+        dt = time.perf_counter() - t0
+        pa.call("pm_mg_destroy", h)
+    pa.call("pm_layout_destroy", lay)
+    return {"only": which, "iters": it.value, "solve_s": round(dt, 5), "rel_residual": float(hist[it.value] / hist[0])}
+
+
 def config1_2d(pa, stream, reps=200):
     """Config 1: DIM=2, 256^2 interior with one periodic ghost layer,
     dst |= Laplacian()(u, 1/h^2), then absMax(dst).  The arrays (1 MB) stay in
@@ -619,6 +663,13 @@ def main():
         pa.call("pm_memcpy", cpy.ptr, r1.ptr, bytes_f, 2, sp)
         stream.sync()
         print(json.dumps({"only": args.only, "steps": args.steps, "copy_bytes": 2 * bytes_f}))
+        return
+
+    if args.only in ("mg_fp64", "mg_ir", "mg_fas"):
+        # profiling helper: ONE config-3 solve (512^3 in 64^3 boxes, no graph
+        # replay): the fused FP64 V-cycles, the FP32-preconditioned refinement
+        # or Proto's library FAS solver
+        print(json.dumps(mg_one_solve(pa, stream, args.only)))
         return
 
     if args.only == "batch":

@@ -4,6 +4,7 @@ the short text tables kept under profiles/.
 
     python -m proto_amd.rocpd_summary trace  DB [NAME_SUBSTRING]
     python -m proto_amd.rocpd_summary calls  DB NAME_SUBSTRING [LAST_N]
+    python -m proto_amd.rocpd_summary groups DB [TOP_N]
     python -m proto_amd.rocpd_summary pmc    FETCH_DB WRITE_DB COPY_BYTES
 """
 import sqlite3
@@ -46,6 +47,25 @@ def calls(db, pattern, last=40):
     return "\n".join(out)
 
 
+def groups(db, top=30):
+    """Total time per (kernel, grid) over the whole database -- one solve's
+    breakdown by kernel and level size when the profiled program runs one --
+    with calls and the mean per call, largest first."""
+    c = sqlite3.connect(db)
+    acc = {}
+    for name, dur, gx, gy, gz in c.execute("select name, duration, grid_x, grid_y, grid_z from kernels"):
+        k = (name.split("(")[0][:64], "%dx%dx%d" % (gx, gy, gz))
+        a = acc.setdefault(k, [0, 0.0])
+        a[0] += 1
+        a[1] += dur / 1e6
+    tot = sum(v[1] for v in acc.values())
+    out = ["total %.3f ms over %d dispatches" % (tot, sum(v[0] for v in acc.values())),
+           "%-64s %-18s %6s %10s %10s" % ("kernel", "grid (threads)", "calls", "total_ms", "us/call")]
+    for (name, grid), (n, t) in sorted(acc.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append("%-64s %-18s %6d %10.3f %10.1f" % (name, grid, n, t, 1e3 * t / n))
+    return "\n".join(out)
+
+
 def _per_kernel(db):
     c = sqlite3.connect(db)
     acc = {}
@@ -76,5 +96,7 @@ if __name__ == "__main__":
         print(trace(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else ""))
     elif sys.argv[1] == "calls":
         print(calls(sys.argv[2], sys.argv[3], int(sys.argv[4]) if len(sys.argv) > 4 else 40))
+    elif sys.argv[1] == "groups":
+        print(groups(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else 30))
     else:
         print(pmc(sys.argv[2], sys.argv[3], int(sys.argv[4])))
