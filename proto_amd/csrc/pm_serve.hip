@@ -39,6 +39,7 @@ struct SLevel {
     float* phi;
     float* tmp;
     float* rhs;
+    double* acc;        // level 0: the request's FP64 solution (the last sweep adds e into it)
     int n;
     int rows;           // rows per sweep tile
     int tw;             // columns per sweep tile
@@ -86,8 +87,10 @@ __device__ inline int wrapi(int v, int n)
 // selects the staged input: 0 `in`; 1 zero (`in` is not read: the first sweep
 // of a zero initial guess); 2 `in` + the constant prolongation of the next
 // coarser level's phi, rounded as k_srv_prolong rounds it (the first sweep
-// after fineInterp, which is then not run).
-template<int MAXE, int IN>
+// after fineInterp, which is then not run).  OUT = 1 (the last sweep of level
+// 0): the result is added to the request's FP64 solution L.acc as
+// k_srv_correct adds the correction, and `out` is not written.
+template<int MAXE, int IN, int OUT>
 __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv, const SItem* __restrict__ items, int dir)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
@@ -217,13 +220,40 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     commit(z0 - 1);
     issue(z0);
     issue_r(z0 - 1);
+    // OUT == 1: this thread's output cells (q = tid + 256 i < cnt, clamped like
+    // the staging loads) as in-plane offsets and LDS offsets; their FP64
+    // values of the plane finished next are loaded one step ahead
+    const int tw = x1 - x0, cnt = (y1 - y0) * tw;
+    constexpr int NO = OUT == 1 ? MAXE : 1;
+    int oo[NO], ol[NO];
+    double acc_in[NO];
+    if constexpr (OUT == 1) {
+#pragma unroll
+        for (int i = 0; i < MAXE; i++) {
+            const int q = min((int)threadIdx.x + 256 * i, cnt - 1), r = q / tw, xx = q - r * tw;
+            oo[i] = x0 + xx + n * (y0 + r);
+            ol[i] = (r + 2) * W + xx + 2;
+        }
+    }
+    auto issue_o = [&](int z) {
+        if constexpr (OUT == 1) {
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) { acc_in[i] = L.acc[nn * z + oo[i]]; }
+        }
+    };
     auto write_out = [&](int z) {
         const float* pl = slot(z);
-        float* ob = out + nn * z;
-        const int tw = x1 - x0, cnt = (y1 - y0) * tw;
-        for (int e = threadIdx.x; e < cnt; e += 256) {
-            const int r = e / tw, xx = e - r * tw;
-            ob[x0 + xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
+        if constexpr (OUT == 1) {
+#pragma unroll
+            for (int i = 0; i < MAXE; i++) {
+                if ((int)threadIdx.x + 256 * i < cnt) { L.acc[nn * z + oo[i]] = acc_in[i] + (double)pl[ol[i]]; }
+            }
+        } else {
+            float* ob = out + nn * z;
+            for (int e = threadIdx.x; e < cnt; e += 256) {
+                const int r = e / tw, xx = e - r * tw;
+                ob[x0 + xx + (long)n * (y0 + r)] = pl[(r + 2) * W + xx + 2];
+            }
         }
     };
     // all memory operations of a step are issued at its start (loads of the
@@ -235,6 +265,7 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
         if (k + 3 <= z1 + 1) { issue(k + 3); }
         if (k + 2 <= z1) { issue_r(k + 2); }
         if (k - 1 >= z0) { write_out(k - 1); }
+        if (k >= z0) { issue_o(k); }  // OUT == 1: FP64 values of plane k, written next step
         __syncthreads();
         update(k + 1, y0 - 1, y1 + 1, x0 - 1, x1 + 1, 0, wred);  // red, one-cell halo included
         __syncthreads();
@@ -474,7 +505,7 @@ struct Active {
 struct Launch {
     int kind;  // 0 sweep, 1 restrict, 2 prolong, 3 correct, 4 residual
     int off, count, dir;
-    int in;    // sweeps: staged input (k_srv_gsrb IN); restrict / residual: 1 = write the zero guess
+    int in;    // sweeps: staged input (k_srv_gsrb IN), 3 = IN 0 with OUT 1; restrict / residual: 1 = write the zero guess
     size_t lds;  // sweeps: dynamic LDS of the largest tile
     long np;     // sweeps: largest phi plane of a tile (elements)
 };
@@ -635,6 +666,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             L.n = a.ns[l];
             L.rows = sweep_rows(a.ns[l]);
             L.tw = sweep_cols(a.ns[l]);
+            L.acc = l == 0 ? a.h->phi64 : nullptr;
             level_coefs(a.ns[l], &L.c0, &L.c1, &L.lam);
             lv.push_back(L);
         }
@@ -712,7 +744,9 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             end();
         }
         for (int sw = 0; sw < post; sw++) {
-            begin(0, sw & 1, sw == 0 && fold_prolong ? 2 : 0);
+            // the last sweep of level 0 adds e into the FP64 solution
+            const int in = sw == 0 && fold_prolong ? 2 : 0;
+            begin(0, sw & 1, d == 0 && sw == post - 1 && in == 0 ? 3 : in);
             for (size_t r = 0; r < act.size(); r++) {
                 const Active& a = act[r];
                 if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
@@ -721,9 +755,16 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             end();
         }
     }
+    // phi += e where no post sweep of level 0 added it (single-level requests,
+    // or no post sweeps, or a post count of 1 whose only sweep takes the
+    // prolongation input)
+    const bool folded_e = post >= 2;
     begin(3, 0);
     for (size_t r = 0; r < act.size(); r++) {
-        if (!act[r].fresh) { push_cells(items, (int)r, (long)act[r].ns[0] * act[r].ns[0] * act[r].ns[0]); }
+        const bool single = act[r].ns.size() == 1;
+        if (!act[r].fresh && (single || !folded_e)) {
+            push_cells(items, (int)r, (long)act[r].ns[0] * act[r].ns[0] * act[r].ns[0]);
+        }
     }
     end();
     begin(4, 0, zero_in ? 0 : 1);
@@ -752,13 +793,17 @@ int run_plan(pm_server* S, int nact, hipStream_t s)
         case 0: {
             const size_t lds = L.lds;
             const long np = L.np;
-#define PM_SRV_SWEEP(E)                                                                                              \
-    if (L.in == 0) {                                                                                                 \
-        hipLaunchKernelGGL((k_srv_gsrb<E, 0>), dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir); \
-    } else if (L.in == 1) {                                                                                          \
-        hipLaunchKernelGGL((k_srv_gsrb<E, 1>), dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir); \
-    } else {                                                                                                         \
-        hipLaunchKernelGGL((k_srv_gsrb<E, 2>), dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir); \
+#define PM_SRV_LAUNCH(E, I, O) \
+    hipLaunchKernelGGL((k_srv_gsrb<E, I, O>), dim3(L.count), dim3(256), lds, s, (const SLevel*)S->d_lv, it, L.dir)
+#define PM_SRV_SWEEP(E)            \
+    if (L.in == 0) {               \
+        PM_SRV_LAUNCH(E, 0, 0);    \
+    } else if (L.in == 1) {        \
+        PM_SRV_LAUNCH(E, 1, 0);    \
+    } else if (L.in == 2) {        \
+        PM_SRV_LAUNCH(E, 2, 0);    \
+    } else {                       \
+        PM_SRV_LAUNCH(E, 0, 1);    \
     }
             if (np <= 1024) {
                 PM_SRV_SWEEP(4)
@@ -768,6 +813,7 @@ int run_plan(pm_server* S, int nact, hipStream_t s)
                 PM_SRV_SWEEP(16)
             }
 #undef PM_SRV_SWEEP
+#undef PM_SRV_LAUNCH
             break;
         }
         case 1:
