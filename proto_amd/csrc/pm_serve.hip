@@ -46,11 +46,16 @@ struct SLevel {
     float c0, c1, lam;  // centre / neighbour coefficients, GSRB weight
 };
 
-// rows and columns per sweep tile of a level n cells wide (levels wider than
-// 128 cells split in two column halves, so that 8-row tiles fit the LDS), and
-// the LDS of its tile
+// rows and columns per sweep tile of a level n cells wide, and the LDS of its
+// tile.  Levels wider than 64 cells are split into column tiles of at most 64:
+// the tile's LDS (about 18 KB in FP32) then admits 8 workgroups per CU, where
+// 128-wide tiles (36 KB) admitted 4; measured 665 vs 605 saturated solves/s.
 __host__ __device__ inline int sweep_rows(int) { return SR; }
-__host__ __device__ inline int sweep_cols(int n) { return n > 128 ? (n + 1) / 2 : n; }
+__host__ __device__ inline int sweep_cols(int n)
+{
+    const int k = (n + 63) / 64;
+    return (n + k - 1) / k;
+}
 inline size_t sweep_lds(int n)
 {
     const int r = sweep_rows(n), w = sweep_cols(n);
