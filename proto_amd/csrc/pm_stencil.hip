@@ -434,21 +434,21 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
     }
 }
 
-// Deterministic second stage over per-block {amax, sum} pairs: one block, a
-// fixed strided split and a fixed tree.
-template<typename T>
-__global__ __launch_bounds__(1024) void k_finish_pairs(const T* __restrict__ partials, int nblocks, int per,
+// Deterministic second stage over per-block {amax, sum} groups of PER values
+// (one pair per operator): one workgroup per value q (blockIdx.x), a fixed
+// strided split and a fixed tree, so the value's bits do not depend on the
+// launch.
+template<typename T, int PER>
+__global__ __launch_bounds__(1024) void k_finish_pairs(const T* __restrict__ partials, int nblocks,
                                                        T* __restrict__ out)
 {
     __shared__ T red[16];
-    for (int q = 0; q < per; q++) {
-        const int op = (q % 2 == 0) ? PM_RED_ABS : PM_RED_SUM;
-        T v = 0;
-        for (int i = threadIdx.x; i < nblocks; i += blockDim.x) { v = red_combine(op, v, partials[per * i + q]); }
-        v = block_reduce(op, v, red);
-        if (threadIdx.x == 0) { out[q] = v; }
-        __syncthreads();
-    }
+    const int q = blockIdx.x;
+    const int op = (q % 2 == 0) ? PM_RED_ABS : PM_RED_SUM;
+    T v = 0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) { v = red_combine(op, v, partials[PER * i + q]); }
+    v = block_reduce(op, v, red);
+    if (threadIdx.x == 0) { out[q] = v; }
 }
 
 // r = f - r followed by {absMax, sum} partials (generic residual path).
@@ -688,7 +688,7 @@ static int residual_t(const pm_stencil* st, const void* u, const Box3& ubox, con
                            : launch_zmarch<T, 27, ZM_RESIDUAL>(a, rbox, cfg, s, &nb);
         if (rc) { return rc; }
         if (red_out) {
-            hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)nb, 2, (T*)red_out);
+            hipLaunchKernelGGL((k_finish_pairs<T, 2>), dim3(2), dim3(1024), 0, s, (const T*)ws, (int)nb, (T*)red_out);
             PM_CHECK_LAUNCH();
         }
         return PM_OK;
@@ -705,7 +705,7 @@ static int residual_t(const pm_stencil* st, const void* u, const Box3& ubox, con
     hipLaunchKernelGGL(k_f_minus<T>, dim3((unsigned)blocks), dim3(256), 0, s, (const T*)f, (T*)r, n, (T*)ws);
     PM_CHECK_LAUNCH();
     if (red_out) {
-        hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)blocks, 2, (T*)red_out);
+        hipLaunchKernelGGL((k_finish_pairs<T, 2>), dim3(2), dim3(1024), 0, s, (const T*)ws, (int)blocks, (T*)red_out);
         PM_CHECK_LAUNCH();
     }
     return PM_OK;
@@ -737,7 +737,7 @@ static int residual2_t(const pm_stencil* st1, const pm_stencil* st2, const void*
                           : launch_zmarch<T, 27, ZM_RESIDUAL2>(a, rbox, cfg, s, &nb);
     if (rc) { return rc; }
     if (red_out) {
-        hipLaunchKernelGGL(k_finish_pairs<T>, dim3(1), dim3(1024), 0, s, (const T*)ws, (int)nb, 4, (T*)red_out);
+        hipLaunchKernelGGL((k_finish_pairs<T, 4>), dim3(4), dim3(1024), 0, s, (const T*)ws, (int)nb, (T*)red_out);
         PM_CHECK_LAUNCH();
     }
     return PM_OK;
