@@ -532,19 +532,29 @@ static int fas_vcycle(pm_fas* M, int l, void* u, const void* f, hipStream_t s)
     return fas_relax(M, l, u, f, 6, s);
 }
 
-static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream_t s, double* res_out = nullptr)
+// residual norm of level 0 (Proto's MG resnorm exchanges first).  in_solve:
+// where the level qualifies, the norm kernel reads the neighbouring boxes and
+// the exchange is skipped (pm_fas_solve exchanges once at its end).
+static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream_t s, double* res_out = nullptr,
+                       bool in_solve = false)
 {
     auto& Lv = M->lv[0];
     const LayoutDev D = ldev(Lv.layout);
-    FAS_TRY(pm_level_exchange(Lv.layout, PM_F64, u, G2, 1, s));
+    const bool nb = in_solve && fz_nb(Lv.layout);
+    if (!nb) { FAS_TRY(pm_level_exchange(Lv.layout, PM_F64, u, G2, 1, s)); }
     const long cells = (long)D.bs[0] * D.bs[1] * D.bs[2];
     unsigned bpb = fas_blocks(cells);
     if (fz_ok(D)) {
         int zc = 0;
         const dim3 zgrid = fz_grid(D, &zc);
         bpb = zgrid.x;  // no more partials than the generic kernel's (<= 64 per box)
-        hipLaunchKernelGGL((k_fas_op_z<1, 0>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
-                           (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc);
+        if (nb) {
+            hipLaunchKernelGGL((k_fas_op_z<1, 1>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
+                               (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc);
+        } else {
+            hipLaunchKernelGGL((k_fas_op_z<1, 0>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
+                               (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc);
+        }
     } else {
         hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
                            (const double*)u, (const double*)f, fas_coefs(Lv.dx), M->d_part, res_out);
@@ -681,13 +691,13 @@ int pm_fas_solve(pm_fas* M, void* state, const void* force, int max_iter, double
     PM_REQUIRE(M && state && force && max_iter >= 0, "pm_fas_solve: bad argument");
     const hipStream_t s = (hipStream_t)stream;
     double res0 = 0, r = 0;
-    FAS_TRY(fas_resnorm(M, state, force, &res0, s));
+    FAS_TRY(fas_resnorm(M, state, force, &res0, s, nullptr, true));
     if (hist) { hist[0] = res0; }
     r = res0;
     int it = 0;
     for (; it < max_iter; it++) {
         FAS_TRY(fas_vcycle(M, 0, state, force, s));
-        FAS_TRY(fas_resnorm(M, state, force, &r, s));
+        FAS_TRY(fas_resnorm(M, state, force, &r, s, nullptr, true));
         if (hist) { hist[it + 1] = r; }
         if (r < tol * res0) {
             it++;
@@ -695,6 +705,8 @@ int pm_fas_solve(pm_fas* M, void* state, const void* force, int max_iter, double
         }
     }
     if (iters) { *iters = it; }
+    // the ghost cells of the state as Proto's last resnorm leaves them
+    FAS_TRY(pm_level_exchange(M->lv[0].layout, PM_F64, state, G2, 1, s));
     return PM_OK;
 }
 
