@@ -347,7 +347,9 @@ int pm_mg_level_layout(const pm_mg* mg, int level, const pm_layout** layout);
  * hist_host (max_iter + 1 doubles, may be NULL) gets the max-norm residual
  * before the first cycle and after each; *iters the cycles done.  Returns
  * PM_ERR_NOT_CONVERGED if max_iter cycles did not reach tol (phi holds the
- * last iterate).  Synchronises `stream` once per cycle (the norm test). */
+ * last iterate).  Synchronises `stream` once per cycle (the norm test).  On
+ * return the ghost cells of phi are exchanged, as Proto's last resnorm leaves
+ * them; the cycles themselves read halos from the neighbouring boxes. */
 int pm_mg_solve(pm_mg* mg, void* phi, const void* rhs, double* hist_host, int* iters, pm_stream_t stream);
 /* One plain V-cycle on (phi, rhs) of the finest level, asynchronous. */
 /* Stopping rule of later solves: max-norm residual < tol * initial, at most
@@ -425,7 +427,9 @@ int pm_fas_resnorm(pm_fas* fas, void* state, const void* force, double* out_host
 int pm_fas_residual(pm_fas* fas, void* state, const void* force, void* residual, double* out_host,
                     pm_stream_t stream);
 /* LevelMultigridSolver::solve (85-132): V-cycles until resnorm < tol *
- * initial or max_iter; hist[0..*iters] the norms (hist may be NULL). */
+ * initial or max_iter; hist[0..*iters] the norms (hist may be NULL).  On
+ * return the ghost cells of state are exchanged, as Proto's last resnorm
+ * leaves them. */
 int pm_fas_solve(pm_fas* fas, void* state, const void* force, int max_iter, double tol, double* hist, int* iters,
                  pm_stream_t stream);
 
