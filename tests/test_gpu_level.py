@@ -104,13 +104,13 @@ def test_exchange_with_hole_leaves_unowned_ghosts():
 
 
 def _solve(n, bs, nlev, relax, linear, precision=0, maxiter=20, use_graph=False, tol=1e-10, rhs=None, stream=None,
-           fused=None):
+           fused=None, sweeps=(2, 2, 10)):
     L = Layout(n, bs)
     opt = pa.MgOptions()
     pa.lib().pm_mg_default_options(ctypes.byref(opt))
     if fused is not None:
         opt.fused = int(fused)
-    opt.relax, opt.pre, opt.post, opt.bottom, opt.interp = relax, 2, 2, 10, linear
+    opt.relax, opt.pre, opt.post, opt.bottom, opt.interp = relax, sweeps[0], sweeps[1], sweeps[2], linear
     opt.precision, opt.max_iter, opt.tol, opt.use_graph = precision, maxiter, tol, int(use_graph)
     mg = ctypes.c_void_p()
     pa.call("pm_mg_create", ctypes.byref(mg), L.h, nlev, 1.0 / n, ctypes.byref(opt))
@@ -175,6 +175,22 @@ def test_fused_and_two_colour_solves_agree(precision):
     a = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, fused=0)
     b = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, fused=1)
     assert a[0] == b[0] == 0 and a[1] == b[1]
+    assert a[2].tobytes() == b[2].tobytes() and a[3].tobytes() == b[3].tobytes()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("sweeps", [(1, 3, 5), (3, 1, 2), (0, 2, 1), (2, 0, 4)])
+def test_odd_and_zero_sweep_counts_agree(precision, sweeps):
+    """Pre / post / bottom counts that leave a two-colour sweep at the end of a
+    relaxation, or none at all: the one-pass solve (with its zero-guess,
+    prolongation and correction folds where they apply) equals the two-colour
+    solve bit for bit."""
+    n, bs, nlev = 64, 32, 3
+    rhs = util.gaussian_rhs(n, 4, 8)
+    say("sweep counts", precision, sweeps)
+    a = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, fused=0, sweeps=sweeps, maxiter=6)
+    b = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs, fused=1, sweeps=sweeps, maxiter=6)
+    assert a[0] == b[0] and a[1] == b[1]
     assert a[2].tobytes() == b[2].tobytes() and a[3].tobytes() == b[3].tobytes()
 
 
