@@ -125,15 +125,20 @@ __global__ __launch_bounds__(256) void k_fas_relax(LayoutDev L, G3 gu, G3 gf, co
 // and xy neighbour position, the same for both planes of a side).
 constexpr int FZ_RY = 4, FZ_TY = 4 * FZ_RY, FZ_NE = (FZ_TY + 4) * 68;
 
-template<int MODE, int NB>
+// IN = 2 (with NB): the staged input is u + the constant prolongation of the
+// coarse correction crs (layout C, ghost gc), rounded as fineCorrect's
+// InterpStencil::Constant rounds it (0 + delta), so the first relaxation after
+// fineCorrect reads the cells the prolongation would have written into u.
+template<int MODE, int NB, int IN = 0>
 __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __restrict__ u,
                                                   const double* __restrict__ f, double* __restrict__ out, FasC c,
-                                                  double* __restrict__ part, double* __restrict__ res_out, int zc)
+                                                  double* __restrict__ part, double* __restrict__ res_out, int zc,
+                                                  LayoutDev C, G3 gc, const double* __restrict__ crs)
 {
     constexpr int RY = FZ_RY, NL = (FZ_NE + 255) / 256;
     __shared__ double sm[4][FZ_NE];
     __shared__ double red[16];
-    __shared__ int s_nb[NB ? 27 : 1], s_dz[NB ? 18 : 1];
+    __shared__ int s_nb[NB ? 27 : 1], s_dz[NB ? 18 : 1], s_dc[IN == 2 ? 18 : 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int b = blockIdx.y;
     const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
@@ -154,6 +159,19 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
         const int o = s_nb[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
         return (long)o * vol + (x - ox * nx + 2) + X * ((y - oy * ny + 2) + (long)(ny + 4) * (z - oz * nz + 2));
     };
+    // IN == 2: coarse element under fine cell (x, y, z) of this box
+    const long cvol = gvol(C, gc);
+    const long XYc = (long)(C.bs[0] + 2 * gc.g[0]) * (C.bs[1] + 2 * gc.g[1]);
+    auto atc = [&](int x, int y, int z) {
+        int p[3] = {lo[0] + x, lo[1] + y, lo[2] + z};
+        for (int k = 0; k < 3; k++) {
+            p[k] = ((p[k] % L.domain[k]) + L.domain[k]) % L.domain[k];
+            p[k] >>= 1;
+        }
+        const int o = owner(C, p);
+        const int* clo = C.lows + 3 * o;
+        return (long)o * cvol + rel_index(C, gc, p[0] - clo[0], p[1] - clo[1], p[2] - clo[2]);
+    };
     if constexpr (NB) {
         if (tid < 27) {
             int p[3] = {lo[0] + (tid % 3 - 1) * nx, lo[1] + ((tid / 3) % 3 - 1) * ny, lo[2] + (tid / 9 - 1) * nz};
@@ -166,6 +184,7 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
             const int y = k / 3 == 0 ? -1 : (k / 3 == 1 ? 0 : ny);
             const int z = side ? nz : -1;
             s_dz[tid] = (int)(at(x, y, z) - (XY * z + at(x, y, 0)));
+            if constexpr (IN == 2) { s_dc[tid] = (int)(atc(x, y, z) - (XYc * (z >> 1) + atc(x, y, 0))); }
         }
         __syncthreads();
     }
@@ -189,7 +208,19 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
             goff[k] = X * (long)min(y0 + er, ny + 3) + ec;
         }
     }
-    double P[NL];
+    // IN == 2: coarse elements of the staged cells at plane 0; plane z adds
+    // z >> 1 coarse planes inside the box (box lows even, levels box-aligned)
+    constexpr int NCO = IN == 2 ? NL : 1;
+    int offc[NCO];
+    if constexpr (IN == 2) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const int e = min(tid + 256 * k, NE - 1);
+            const int er = e / LX, ec = e - er * LX;
+            offc[k] = (int)atc(ec - 2, y0 - 2 + er, 0);
+        }
+    }
+    double P[NL], PC[NCO];
     auto gload = [&](int z) {
         if constexpr (NB) {
             const bool inb = z >= 0 && z < nz;
@@ -197,8 +228,15 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
 #pragma unroll
             for (int k = 0; k < NL; k++) {
                 long o = XY * z + goff[k];
-                if (!inb) { o += s_dz[sd + (int)((kp >> (4 * k)) & 15)]; }
+                long oc = 0;
+                if constexpr (IN == 2) { oc = XYc * (z >> 1) + offc[k]; }
+                if (!inb) {
+                    const int ks = sd + (int)((kp >> (4 * k)) & 15);
+                    o += s_dz[ks];
+                    if constexpr (IN == 2) { oc += s_dc[ks]; }
+                }
                 P[k] = u[o];
+                if constexpr (IN == 2) { PC[k] = crs[oc]; }
             }
         } else {
             const double* base = ub + XY * (long)(z + 2);
@@ -210,7 +248,12 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
 #pragma unroll
         for (int k = 0; k < NL; k++) {
             const int e = tid + 256 * k;
-            if (e < NE) { pl[e] = P[k]; }
+            double v = P[k];
+            if constexpr (IN == 2) {
+                const double zero = 0;
+                v = v + (zero + PC[k]);
+            }
+            if (e < NE) { pl[e] = v; }
         }
     };
     const int x = lane;
@@ -470,7 +513,10 @@ static dim3 fz_grid(const LayoutDev& D, int* zc)
     return dim3((unsigned)(nyt * ((D.bs[2] + z - 1) / z)), (unsigned)D.nboxes);
 }
 
-static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_t s)
+// n Jacobi relaxations of level l.  delta (the coarse correction of level
+// l + 1, ghost 0): fineCorrect's u += I(delta) is still to be applied; the first
+// relaxation stages it where the levels allow, else it is applied first.
+static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_t s, const void* delta = nullptr)
 {
     auto& Lv = M->lv[l];
     const LayoutDev D = ldev(Lv.layout);
@@ -480,13 +526,27 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
     int zc = 0;
     const dim3 zgrid = fz_grid(D, &zc);
     const bool nb = fz_nb(Lv.layout);
+    bool fold = false;
+    if (delta) {
+        const pm_layout* CL = M->lv[l + 1].layout;
+        fold = nb && n > 0 && levels_box_aligned(Lv.layout, CL) &&
+               gvol(ldev(CL), g3(G0)) * CL->nboxes < (1L << 31);
+        if (!fold) { FAS_TRY(pm_level_prolong_add(Lv.layout, CL, PM_F64, delta, G0, u, G2, 0, s)); }
+    }
     for (int it = 0; it < n; it++) {
+        if (nb && fold && it == 0) {
+            hipLaunchKernelGGL((k_fas_op_z<0, 1, 2>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
+                               (double*)buf[1], c, (double*)nullptr, (double*)nullptr, zc,
+                               ldev(M->lv[l + 1].layout), g3(G0), (const double*)delta);
+            PM_CHECK_LAUNCH();
+            continue;
+        }
         if (nb) {
             // halo read from the neighbouring boxes of the input buffer (the
             // relaxation is out of place): no exchange
             hipLaunchKernelGGL((k_fas_op_z<0, 1>), zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
                                (const double*)f, (double*)buf[(it + 1) & 1], c, (double*)nullptr, (double*)nullptr,
-                               zc);
+                               zc, D, g3(G0), (const double*)nullptr);
             PM_CHECK_LAUNCH();
             continue;
         }
@@ -494,7 +554,7 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
         if (fz_ok(D)) {
             hipLaunchKernelGGL((k_fas_op_z<0, 0>), zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
                                (const double*)f, (double*)buf[(it + 1) & 1], c, (double*)nullptr, (double*)nullptr,
-                               zc);
+                               zc, D, g3(G0), (const double*)nullptr);
         } else {
             hipLaunchKernelGGL(k_fas_relax, dim3(fas_blocks(cells), D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
                                (const double*)buf[it & 1], (const double*)f, (double*)buf[(it + 1) & 1], c);
@@ -528,8 +588,8 @@ static int fas_vcycle(pm_fas* M, int l, void* u, const void* f, hipStream_t s)
     FAS_TRY(fas_vcycle(M, l + 1, C.phi, C.f, s));
     hipLaunchKernelGGL(k_fas_delta, grid, dim3(256), 0, s, DC, g3(G2), g3(G0), (const double*)C.phi, (double*)C.phi0);
     PM_CHECK_LAUNCH();
-    FAS_TRY(pm_level_prolong_add(F.layout, C.layout, PM_F64, C.phi0, G0, u, G2, 0, s));
-    return fas_relax(M, l, u, f, 6, s);
+    // fineCorrect (u += I(delta)) folded into the first post relaxation
+    return fas_relax(M, l, u, f, 6, s, C.phi0);
 }
 
 // residual norm of level 0 (Proto's MG resnorm exchanges first).  in_solve:
@@ -550,10 +610,12 @@ static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream
         bpb = zgrid.x;  // no more partials than the generic kernel's (<= 64 per box)
         if (nb) {
             hipLaunchKernelGGL((k_fas_op_z<1, 1>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
-                               (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc);
+                               (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc, D, g3(G0),
+                               (const double*)nullptr);
         } else {
             hipLaunchKernelGGL((k_fas_op_z<1, 0>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
-                               (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc);
+                               (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc, D, g3(G0),
+                               (const double*)nullptr);
         }
     } else {
         hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),

@@ -38,6 +38,10 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
                         const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream,
                         double* acc = nullptr);
+// Every fine box's coarsened region lies in one coarse box (both layouts
+// cover their domains, even fine box sizes, coarse box size a multiple of half
+// the fine one) (pm_level.hip).
+bool levels_box_aligned(const pm_layout* F, const pm_layout* C);
 // pm_level_resnorm, pm_level_residual and pm_level_restrict_residual that
 // read the halo from the neighbouring boxes' valid cells instead of the ghost
 // layer, so no exchange has to precede them (pm_level.hip).  Same results as

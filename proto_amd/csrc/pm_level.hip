@@ -1426,6 +1426,8 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
 #undef PM_SWEEP_MODES
 #undef PM_SWEEP_LAUNCH
 
+bool levels_box_aligned(const pm_layout* F, const pm_layout* C) { return box_aligned(F, C); }
+
 }  // namespace pm
 
 extern "C" {
