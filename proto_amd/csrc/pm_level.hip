@@ -112,9 +112,10 @@ __global__ __launch_bounds__(256) void k_gsrb(LayoutDev L, G3 gp, G3 gr, T* __re
 
 // ------------------------------------------------------------ fused GSRB sweep
 // One full red-black sweep in one pass, out of place (pin is only read, pout
-// only written), with no exchange: a workgroup owns rows [y0, y0+R) x planes
-// [z0, z1) of one box and walks z through an LDS ring of 4 planes holding
-// rows [y0-2, y1+2) x columns [-2, bs+2).  Cells outside the box come from
+// only written), with no exchange: a workgroup owns columns [x0, x1) x rows
+// [y0, y0+R) x planes [z0, z1) of one box (column tiles of tw cells, at most
+// 64 wide) and walks z through an LDS ring of 4 planes holding rows
+// [y0-2, y1+2) x columns [x0-2, x1+2).  Cells outside the box come from
 // their owner box's valid cells (periodic images folded in).  At step k the
 // red cells of plane k+1 are updated over the tile plus a one-cell halo (the
 // halo values equal what the neighbouring tiles compute for those cells:
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256) void k_gsrb(LayoutDev L, G3 gp, G3 gr, T* __re
 template<typename T, int MAXE, int IN, int OUT>
 __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout, G3 gr, const T* __restrict__ pin,
                                                     T* __restrict__ pout, const T* __restrict__ rhs, Lap<T> c,
-                                                    T lambda, int rows, int zc, LayoutDev C, G3 gc,
+                                                    T lambda, int rows, int zc, int tw, LayoutDev C, G3 gc,
                                                     const T* __restrict__ crs, double* __restrict__ pacc)
 {
     extern __shared__ __align__(16) unsigned char smraw[];
@@ -142,13 +143,16 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     const int b = blockIdx.y;
     const int* lo = L.lows + 3 * b;
     const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2];
-    const int nyt = (ny + rows - 1) / rows;
-    const int y0 = (blockIdx.x % nyt) * rows, y1 = min(y0 + rows, ny);
-    const int z0 = (blockIdx.x / nyt) * zc, z1 = min(z0 + zc, nz);
-    const int W = nx + 4, NP = (y1 - y0 + 4) * W;  // phi plane: rows [y0-2, y1+2) x columns [-2, nx+2)
-    const int WR = nx + 2, NR = (y1 - y0 + 2) * WR; // rhs plane: rows [y0-1, y1+1) x columns [-1, nx+1)
+    const int nyt = (ny + rows - 1) / rows, nxt = (nx + tw - 1) / tw;
+    const int bt = blockIdx.x / nxt;
+    const int x0 = (blockIdx.x % nxt) * tw, x1 = min(x0 + tw, nx);
+    const int y0 = (bt % nyt) * rows, y1 = min(y0 + rows, ny);
+    const int z0 = (bt / nyt) * zc, z1 = min(z0 + zc, nz);
+    const int TWc = x1 - x0;                         // columns of this tile
+    const int W = TWc + 4, NP = (y1 - y0 + 4) * W;   // phi plane: rows [y0-2, y1+2) x columns [x0-2, x1+2)
+    const int WR = TWc + 2, NR = (y1 - y0 + 2) * WR; // rhs plane: rows [y0-1, y1+1) x columns [x0-1, x1+1)
     T* sm = reinterpret_cast<T*>(smraw);
-    T* rsm = sm + 4 * (long)(rows + 4) * W;
+    T* rsm = sm + 4 * (long)(rows + 4) * (tw + 4);
     const int losum = lo[0] + lo[1] + lo[2];
     const long vin = gvol(L, gin), vr = gvol(L, gr);
     const long XYp = (long)(nx + 2 * gin.g[0]) * (ny + 2 * gin.g[1]);
@@ -214,11 +218,11 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
 #pragma unroll
     for (int i = 0; i < MAXE; i++) {
         const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
-        offp[i] = (int)at(gin, vin, ec - 2, y0 - 2 + er, 0);
-        kp |= (unsigned long long)kpos(ec - 2, y0 - 2 + er) << (4 * i);
+        offp[i] = (int)at(gin, vin, x0 + ec - 2, y0 - 2 + er, 0);
+        kp |= (unsigned long long)kpos(x0 + ec - 2, y0 - 2 + er) << (4 * i);
         const int f = min((int)threadIdx.x + 256 * i, NR - 1), fr = f / WR, fc = f - fr * WR;
-        offr[i] = (int)at(gr, vr, fc - 1, y0 - 1 + fr, 0);
-        kr |= (unsigned long long)kpos(fc - 1, y0 - 1 + fr) << (4 * i);
+        offr[i] = (int)at(gr, vr, x0 + fc - 1, y0 - 1 + fr, 0);
+        kr |= (unsigned long long)kpos(x0 + fc - 1, y0 - 1 + fr) << (4 * i);
     }
     // IN == 2: coarse element under each staged fine cell.  Inside the box
     // every staged cell's owner lies in this box's z layer of boxes, whose
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
             const int e = min((int)threadIdx.x + 256 * i, NP - 1), er = e / W, ec = e - er * W;
-            offc[i] = (int)atc(ec - 2, y0 - 2 + er, 0);
+            offc[i] = (int)atc(x0 + ec - 2, y0 - 2 + er, 0);
         }
     }
     T pre[MAXE], prr[MAXE], prc[NCO];
@@ -295,7 +299,7 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         const int t = (int)threadIdx.x;
         return Walk{hw, t / hw, t % hw, 256 / hw, 256 % hw};
     };
-    const Walk wred = mk_walk((nx + 2 + 1) / 2), wblk = mk_walk((nx + 1) / 2);
+    const Walk wred = mk_walk((TWc + 2 + 1) / 2), wblk = mk_walk((TWc + 1) / 2);
     auto update = [&](int z, int ya, int yb, int xa, int xb, int colour, const Walk& wk) {
         T* zc_ = slot(z);
         const T* zm = slot(z - 1);
@@ -314,7 +318,7 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
                 qq++;
             }
             if (xx >= xb) { continue; }
-            const int li = (yy - (y0 - 2)) * W + (xx + 2);
+            const int li = (yy - (y0 - 2)) * W + (xx - x0 + 2);
             T s = 0;
             s = s + c.c[0] * zc_[li];
             s = s + c.c[1] * zc_[li + 1];
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
             s = s + c.c[4] * zc_[li - W];
             s = s + c.c[5] * zp[li];
             s = s + c.c[6] * zm[li];
-            s = s + rz[(yy - (y0 - 1)) * WR + (xx + 1)];
+            s = s + rz[(yy - (y0 - 1)) * WR + (xx - x0 + 1)];
             zc_[li] = zc_[li] + lambda * s;
         }
     };
@@ -341,14 +345,14 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     constexpr int NO = OUT == 1 ? MAXE : 1;
     int oo[NO], ol[NO];
     double acc_in[NO];
-    const int cnt = (y1 - y0) * nx;
+    const int cnt = (y1 - y0) * TWc;
     const long XYo = (long)(nx + 2 * gout.g[0]) * (ny + 2 * gout.g[1]);
     double* oacc = pacc + (long)b * gvol(L, gout);
     if constexpr (OUT == 1) {
 #pragma unroll
         for (int i = 0; i < MAXE; i++) {
-            const int q = min((int)threadIdx.x + 256 * i, cnt - 1), r = q / nx, xx = q - r * nx;
-            oo[i] = (int)rel_index(L, gout, xx, y0 + r, 0);
+            const int q = min((int)threadIdx.x + 256 * i, cnt - 1), r = q / TWc, xx = q - r * TWc;
+            oo[i] = (int)rel_index(L, gout, x0 + xx, y0 + r, 0);
             ol[i] = (r + 2) * W + xx + 2;
         }
     }
@@ -367,8 +371,8 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
             }
         } else {
             for (int e = threadIdx.x; e < cnt; e += 256) {
-                const int r = e / nx, xx = e - r * nx;
-                ob[rel_index(L, gout, xx, y0 + r, z)] = pl[(r + 2) * W + xx + 2];
+                const int r = e / TWc, xx = e - r * TWc;
+                ob[rel_index(L, gout, x0 + xx, y0 + r, z)] = pl[(r + 2) * W + xx + 2];
             }
         }
     };
@@ -384,9 +388,9 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         if (k - 1 >= z0) { write_out(k - 1); }  // slot of plane k-1: only read during this step
         if (k >= z0) { issue_o(k); }           // OUT == 1: pacc of plane k, written next step
         __syncthreads();
-        update(k + 1, y0 - 1, y1 + 1, -1, nx + 1, 0, wred);
+        update(k + 1, y0 - 1, y1 + 1, x0 - 1, x1 + 1, 0, wred);
         __syncthreads();
-        if (k >= z0) { update(k, y0, y1, 0, nx, 1, wblk); }
+        if (k >= z0) { update(k, y0, y1, x0, x1, 1, wblk); }
         __syncthreads();
     }
     write_out(z1 - 1);
@@ -1095,6 +1099,15 @@ static unsigned lvl_blocks(long n, int nboxes, long cap = 1024)
     return bx_blocks(n, std::min(cap, std::max(1L, 16384L / std::max(1, nboxes))));
 }
 
+// Workgroups per box of the generic residual / norm kernels (one max-norm
+// partial each): 64 per box, and on levels of few large boxes enough for about
+// 16384 over the level (a single 256^3 box with 64 workgroups took 289 us for
+// its norm, 16 times its streaming time).
+static unsigned res_blocks(long n, int nboxes)
+{
+    return bx_blocks(n, std::max(64L, 16384L / std::max(1, nboxes)));
+}
+
 template<typename T>
 static Lap<T> lap_coefs(int dim, double m)
 {
@@ -1330,7 +1343,7 @@ namespace pm {
 
 #define PM_SWEEP_LAUNCH(E, M, O)                                                                                 \
     hipLaunchKernelGGL((k_gsrb_fused<T, E, M, O>), grid, dim3(256), lds, stream, D, gi, go, gr, pi, (T*)phi_out, \
-                       (const T*)rhs, c, lt, rows, zc, DC, gc, cr, acc)
+                       (const T*)rhs, c, lt, rows, zc, tw, DC, gc, cr, acc)
 #define PM_SWEEP_MODES(E)         \
     if (acc) {                    \
         PM_SWEEP_LAUNCH(E, 0, 1); \
@@ -1377,8 +1390,11 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
         }
     }
     const size_t esz = dtype == PM_F64 ? 8 : 4;
+    // column tiles of at most 64 cells (boxes wider than 64 split evenly), so
+    // that 8-row tiles fit the LDS however wide the box is
+    const int nxt = (L->bs[0] + 63) / 64, tw = (L->bs[0] + nxt - 1) / nxt;
     auto lds_of = [&](int r) {
-        return (4 * (size_t)(r + 4) * (L->bs[0] + 4) + 2 * (size_t)(r + 2) * (L->bs[0] + 2)) * esz;
+        return (4 * (size_t)(r + 4) * (tw + 4) + 2 * (size_t)(r + 2) * (tw + 2)) * esz;
     };
     // 8-row tiles: measured fastest for both precisions on 64^3 boxes (FP32
     // 0.52 ms vs 0.61 ms at 16 rows, FP64 0.73 ms vs 0.92 ms at 16 rows, for
@@ -1388,17 +1404,17 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
     while (rows > 1 && lds_of(rows) > 64 * 1024 - 256) { rows /= 2; }
     const size_t lds = lds_of(rows);
     if (lds > 64 * 1024 - 256) {
-        set_error("pm_level_gsrb_sweep: boxes %d cells wide do not fit the LDS ring", L->bs[0]);
+        set_error("pm_level_gsrb_sweep: tiles %d cells wide do not fit the LDS ring", tw);
         return PM_ERR_UNSUPPORTED;
     }
     for (int k = 0; k < 3; k++) { PM_REQUIRE(L->bs[k] >= 2, "pm_level_gsrb_sweep: boxes need 2+ cells per side"); }
     const int nyt = (L->bs[1] + rows - 1) / rows;
     int zc = L->bs[2];
-    while (zc > 8 && (long)L->nboxes * nyt * ((L->bs[2] + zc - 1) / zc) < 2048) { zc = (zc + 1) / 2; }
+    while (zc > 8 && (long)L->nboxes * nxt * nyt * ((L->bs[2] + zc - 1) / zc) < 2048) { zc = (zc + 1) / 2; }
     const G3 gi = g3(in_ghost), go = g3(out_ghost), gr = g3(rhs_ghost), gc = g3(coarse_ghost);
     const LayoutDev D = ldev(L);
     const LayoutDev DC = in_mode == 2 ? ldev(C) : D;
-    dim3 grid((unsigned)(nyt * ((L->bs[2] + zc - 1) / zc)), L->nboxes);
+    dim3 grid((unsigned)(nxt * nyt * ((L->bs[2] + zc - 1) / zc)), L->nboxes);
     double lam = -(dx * dx) / (4.0 * L->dim);
     lam *= 2.0;
     const double m = -1.0 / (dx * dx);
@@ -1407,7 +1423,7 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
         if (dtype == PM_F64) {
             for (int j = 0; j < 7; j++) { c.c[j] = (T)((j == 0 ? -2.0 * L->dim : 1.0) * m); }
         }
-        const long np = (long)(rows + 4) * (L->bs[0] + 4);
+        const long np = (long)(rows + 4) * (tw + 4);
         const T* pi = (const T*)(in_mode == 1 ? phi_out : phi_in);
         const T* cr = (const T*)coarse;
         const T lt = (T)(1.0 * lam);
@@ -1475,7 +1491,7 @@ int pm_level_resnorm(const pm_layout* L, int dtype, const void* phi, const int p
     const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost);
     const LayoutDev D = ldev(L);
     const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
-    unsigned bpb = bx_blocks(n, 64);
+    unsigned bpb = res_blocks(n, L->nboxes);
     dim3 grid(bpb, L->nboxes);
     int zc = 0;
     const bool z = rz_ok(L, gp);
@@ -1708,7 +1724,7 @@ int pm_level_residual(const pm_layout* L, int dtype, const void* phi, const int 
     const G3 gp = g3(phi_ghost), gr = g3(rhs_ghost), go = g3(out_ghost);
     const LayoutDev D = ldev(L);
     const long n = (long)L->bs[0] * L->bs[1] * L->bs[2];
-    unsigned bpb = bx_blocks(n, 64);
+    unsigned bpb = res_blocks(n, L->nboxes);
     dim3 grid(bpb, L->nboxes);
     int zc = 0;
     const bool z = rz_ok(L, gp);

@@ -134,7 +134,7 @@ def test_default_options_use_the_fused_sweep():
 
 
 @pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
-@pytest.mark.parametrize("n,bs", [(32, 16), (16, 16), (24, 8), (12, 6), (64, 64)])
+@pytest.mark.parametrize("n,bs", [(32, 16), (16, 16), (24, 8), (12, 6), (64, 64), (128, 128), (96, 96), (160, 80)])
 def test_gsrb_sweep_equals_two_colours(dt, n, bs):
     """pm_level_gsrb_sweep (one pass, out of place, no exchange) gives the bits
     of exchange / red / exchange / black, also for boxes with odd low-corner
