@@ -84,6 +84,10 @@ __device__ inline int wrapi(int v, int n)
     return v < 0 ? v + n : v;
 }
 
+// the same for an index at most one period outside [0, n): a compare and
+// select instead of an integer remainder
+__device__ inline int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
 // One GSRB sweep (both colours) of the tile rows [y0, y0+rows) x planes [z0, z1)
 // of level L, reading `in` and writing `out` (dir 0: phi -> tmp, 1: tmp -> phi).
 // phi planes (rows [y0-2, y1+2) x columns [-2, n+2)) pass through a ring of 4
@@ -306,12 +310,12 @@ __global__ __launch_bounds__(256) void k_srv_restrict(const SLevel* __restrict__
                     float s = 0;
                     s = s + F.rhs[x + row + nn * z];
                     s = s + F.c0 * pz[x + row];
-                    s = s + F.c1 * pz[wrapi(x + 1, n) + row];
-                    s = s + F.c1 * pz[wrapi(x - 1, n) + row];
-                    s = s + F.c1 * pz[x + (long)n * wrapi(y + 1, n)];
-                    s = s + F.c1 * pz[x + (long)n * wrapi(y - 1, n)];
-                    s = s + F.c1 * F.phi[x + row + nn * wrapi(z + 1, n)];
-                    s = s + F.c1 * F.phi[x + row + nn * wrapi(z - 1, n)];
+                    s = s + F.c1 * pz[wrap1(x + 1, n) + row];
+                    s = s + F.c1 * pz[wrap1(x - 1, n) + row];
+                    s = s + F.c1 * pz[x + (long)n * wrap1(y + 1, n)];
+                    s = s + F.c1 * pz[x + (long)n * wrap1(y - 1, n)];
+                    s = s + F.c1 * F.phi[x + row + nn * wrap1(z + 1, n)];
+                    s = s + F.c1 * F.phi[x + row + nn * wrap1(z - 1, n)];
                     acc = acc + avg * s;
                 }
         C.rhs[t] = acc;
@@ -361,12 +365,12 @@ __global__ __launch_bounds__(256) void k_srv_residual(const SReq* __restrict__ r
         double s = 0;
         s = s + R.rhs[t];
         s = s + R.d0 * pz[x + row];
-        s = s + R.d1 * pz[wrapi(x + 1, n) + row];
-        s = s + R.d1 * pz[wrapi(x - 1, n) + row];
-        s = s + R.d1 * pz[x + (long)n * wrapi(y + 1, n)];
-        s = s + R.d1 * pz[x + (long)n * wrapi(y - 1, n)];
-        s = s + R.d1 * R.phi[x + row + nn * wrapi(z + 1, n)];
-        s = s + R.d1 * R.phi[x + row + nn * wrapi(z - 1, n)];
+        s = s + R.d1 * pz[wrap1(x + 1, n) + row];
+        s = s + R.d1 * pz[wrap1(x - 1, n) + row];
+        s = s + R.d1 * pz[x + (long)n * wrap1(y + 1, n)];
+        s = s + R.d1 * pz[x + (long)n * wrap1(y - 1, n)];
+        s = s + R.d1 * R.phi[x + row + nn * wrap1(z + 1, n)];
+        s = s + R.d1 * R.phi[x + row + nn * wrap1(z - 1, n)];
         L.rhs[t] = (float)s;
         if (zero_e) { L.phi[t] = 0.0f; }
         const double a = s > 0 ? s : -s;
