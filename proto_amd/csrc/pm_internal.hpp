@@ -38,6 +38,14 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
                         const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream,
                         double* acc = nullptr);
+// nsweeps GSRB sweeps (pm_level_gsrb's colouring and arithmetic, bit for bit)
+// of a one-box periodic 3-D level of at most 4096 cells in one launch, in
+// place; in_mode as level_gsrb_sweep_in (1 zero guess, 2 + constant
+// prolongation of the one-box level C).  PM_ERR_UNSUPPORTED otherwise
+// (pm_level.hip).
+int level_gsrb_small(const pm_layout* L, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
+                     const int rhs_ghost[3], double dx, int nsweeps, int in_mode, const pm_layout* C,
+                     const void* coarse, const int coarse_ghost[3], hipStream_t stream);
 // Every fine box's coarsened region lies in one coarse box (both layouts
 // cover their domains, even fine box sizes, coarse box size a multiple of half
 // the fine one) (pm_level.hip).

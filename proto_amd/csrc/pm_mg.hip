@@ -113,6 +113,13 @@ static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* 
         in_mode = 0;
         return PM_OK;
     };
+    // tiny one-box levels: every sweep in one launch, phi held in LDS
+    if (mg->opt.relax == 0 && mg->opt.fused && mg->dim == 3 && !acc64 && n > 0) {
+        const int rc = level_gsrb_small(L.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, n, in_mode,
+                                        in_mode == 2 ? mg->lv[l + 1].layout : nullptr, cphi, mg->g1, s);
+        if (rc == PM_OK) { return PM_OK; }
+        if (rc != PM_ERR_UNSUPPORTED) { return rc; }
+    }
     int it = 0;
     if (mg->opt.relax == 0 && mg->opt.fused && mg->dim == 3 && tmp) {
         // one-pass sweeps alternating phi -> tmp -> phi; an odd count ends with
