@@ -284,6 +284,66 @@ __global__ __launch_bounds__(256) void k_srv_gsrb(const SLevel* __restrict__ lv,
     write_out(z1 - 1);
 }
 
+// All it.a sweeps of one small level (n^3 <= SMALL_CELLS) of one request in one
+// workgroup: phi lives in LDS for the whole relaxation (periodic neighbours by
+// index), the right-hand side of each thread's cells in registers, one pass
+// per colour.  Same update, order and colouring as k_srv_gsrb, so the same
+// bits; IN as k_srv_gsrb (1 zero guess, 2 + prolongation of the next coarser
+// level).  Small levels are latency-bound: one launch instead of one per sweep.
+constexpr int SMALL_CELLS = 4096, SMALL_PER = SMALL_CELLS / 256;
+
+template<int IN>
+__global__ __launch_bounds__(256) void k_srv_small(const SLevel* __restrict__ lv, const SItem* __restrict__ items)
+{
+    __shared__ float u[SMALL_CELLS];
+    const SItem it = items[blockIdx.x];
+    const SLevel L = lv[it.lev];
+    const int n = L.n, N = n * n * n, nsw = it.a;
+    const float* cphi = lv[it.lev + (IN == 2 ? 1 : 0)].phi;
+    const int nc = n >> 1;
+    float f[SMALL_PER];
+#pragma unroll
+    for (int j = 0; j < SMALL_PER; j++) {
+        const int i = threadIdx.x + 256 * j;
+        if (i < N) {
+            const int x = i % n, y = (i / n) % n, z = i / (n * n);
+            f[j] = L.rhs[i];
+            float v = 0.0f;
+            if constexpr (IN != 1) { v = L.phi[i]; }
+            if constexpr (IN == 2) {
+                const float zero = 0.0f;
+                v = v + (zero + cphi[(x >> 1) + nc * ((y >> 1) + nc * (z >> 1))]);
+            }
+            u[i] = v;
+        }
+    }
+    for (int sw = 0; sw < nsw; sw++) {
+        for (int colour = 0; colour < 2; colour++) {
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < SMALL_PER; j++) {
+                const int i = threadIdx.x + 256 * j;
+                if (i >= N) { continue; }
+                const int x = i % n, y = (i / n) % n, z = i / (n * n);
+                if (((x + y + z + colour) & 1) != 0) { continue; }
+                const int row = n * (y + n * z);
+                float s = 0;
+                s = s + L.c0 * u[i];
+                s = s + L.c1 * u[wrap1(x + 1, n) + row];
+                s = s + L.c1 * u[wrap1(x - 1, n) + row];
+                s = s + L.c1 * u[x + n * (wrap1(y + 1, n) + n * z)];
+                s = s + L.c1 * u[x + n * (wrap1(y - 1, n) + n * z)];
+                s = s + L.c1 * u[x + n * (y + n * wrap1(z + 1, n))];
+                s = s + L.c1 * u[x + n * (y + n * wrap1(z - 1, n))];
+                s = s + f[j];
+                u[i] = u[i] + L.lam * s;
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += 256) { L.phi[i] = u[i]; }
+}
+
 // Coarse right-hand side = AvgDown(rhs + sum_j c_j phi(p + o_j)) of the fine
 // level (LevelMultigrid.cpp:51-73 order), and, with zero_phi, coarse phi = 0
 // (otherwise the coarse level's first sweep takes the zero guess as its input).
@@ -512,7 +572,7 @@ struct Active {
 };
 
 struct Launch {
-    int kind;  // 0 sweep, 1 restrict, 2 prolong, 3 correct, 4 residual
+    int kind;  // 0 sweep, 1 restrict, 2 prolong, 3 correct, 4 residual, 5 small-level relaxation
     int off, count, dir;
     int in;    // sweeps: staged input (k_srv_gsrb IN), 3 = IN 0 with OUT 1; restrict / residual: 1 = write the zero guess
     size_t lds;  // sweeps: dynamic LDS of the largest tile
@@ -720,15 +780,27 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
     // prolongation is folded into the first post sweep when there is one.
     const bool zero_in = pre > 0 && bot > 0;
     const bool fold_prolong = post > 0;
+    // levels below the finest of at most SMALL_CELLS cells relax in one launch
+    // each (k_srv_small), all requests' such levels of one depth together
+    auto small = [&](const Active& a, int d) { return d > 0 && (long)a.ns[d] * a.ns[d] * a.ns[d] <= SMALL_CELLS; };
     // V-cycle of every non-fresh request (fresh ones only get their first residual)
     for (int d = 0; d <= maxd; d++) {
+        begin(5, 0, zero_in ? 1 : 0);
+        for (size_t r = 0; r < act.size(); r++) {
+            const Active& a = act[r];
+            const int bd = (int)a.ns.size() - 1;
+            if (a.fresh || d > bd || !small(a, d)) { continue; }
+            const int cnt = d < bd ? pre : bot;
+            if (cnt > 0) { items.push_back(SItem{lev0[r] + d, cnt, 0, 0, 0, 0}); }
+        }
+        end();
         const int nsw = std::max(pre, bot);
         for (int sw = 0; sw < nsw; sw++) {
             begin(0, sw & 1, sw == 0 && zero_in ? 1 : 0);
             for (size_t r = 0; r < act.size(); r++) {
                 const Active& a = act[r];
                 const int bd = (int)a.ns.size() - 1;
-                if (a.fresh || d > bd) { continue; }
+                if (a.fresh || d > bd || small(a, d)) { continue; }
                 if ((d < bd && sw < pre) || (d == bd && sw < bot)) { sweep_items(lev0[r] + d, a.ns[d]); }
             }
             end();
@@ -752,13 +824,22 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             }
             end();
         }
+        if (post > 0) {
+            begin(5, 0, fold_prolong ? 2 : 0);
+            for (size_t r = 0; r < act.size(); r++) {
+                const Active& a = act[r];
+                if (a.fresh || d >= (int)a.ns.size() - 1 || !small(a, d)) { continue; }
+                items.push_back(SItem{lev0[r] + d, post, 0, 0, 0, 0});
+            }
+            end();
+        }
         for (int sw = 0; sw < post; sw++) {
             // the last sweep of level 0 adds e into the FP64 solution
             const int in = sw == 0 && fold_prolong ? 2 : 0;
             begin(0, sw & 1, d == 0 && sw == post - 1 && in == 0 ? 3 : in);
             for (size_t r = 0; r < act.size(); r++) {
                 const Active& a = act[r];
-                if (a.fresh || d >= (int)a.ns.size() - 1) { continue; }
+                if (a.fresh || d >= (int)a.ns.size() - 1 || small(a, d)) { continue; }
                 sweep_items(lev0[r] + d, a.ns[d]);
             }
             end();
@@ -827,6 +908,15 @@ int run_plan(pm_server* S, int nact, hipStream_t s)
         }
         case 1:
             hipLaunchKernelGGL(k_srv_restrict, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it, L.in);
+            break;
+        case 5:
+            if (L.in == 0) {
+                hipLaunchKernelGGL(k_srv_small<0>, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
+            } else if (L.in == 1) {
+                hipLaunchKernelGGL(k_srv_small<1>, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
+            } else {
+                hipLaunchKernelGGL(k_srv_small<2>, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
+            }
             break;
         case 2:
             hipLaunchKernelGGL(k_srv_prolong, dim3(L.count), dim3(256), 0, s, (const SLevel*)S->d_lv, it);
