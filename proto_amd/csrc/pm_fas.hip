@@ -132,11 +132,18 @@ constexpr int FZ_RY = 2, FZ_TY = 4 * FZ_RY, FZ_NE = (FZ_TY + 4) * 68;
 // coarse correction crs (layout C, ghost gc), rounded as fineCorrect's
 // InterpStencil::Constant rounds it (0 + delta), so the first relaxation after
 // fineCorrect reads the cells the prolongation would have written into u.
+// MODE 2 (with NB, even box sizes, box-aligned levels): crseForce, the coarse
+// right-hand side sum over the 2x2x2 children (z, y, x nested, x fastest) of
+// 0.125 (L(u) + f) minus L_c(crs) at the coarse cell (crs the coarse state on
+// C with ghost gc, coefficients cc), into fco (ghost 0) -- k_fas_coarse_force's
+// arithmetic and order; x neighbours come from the next lane, y pairs are a
+// thread's two rows, z pairs consecutive planes.
 template<int MODE, int NB, int IN = 0>
 __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __restrict__ u,
                                                   const double* __restrict__ f, double* __restrict__ out, FasC c,
                                                   double* __restrict__ part, double* __restrict__ res_out, int zc,
-                                                  LayoutDev C, G3 gc, const double* __restrict__ crs)
+                                                  LayoutDev C, G3 gc, const double* __restrict__ crs,
+                                                  double* __restrict__ fco, FasC cc)
 {
     constexpr int RY = FZ_RY, NL = (FZ_NE + 255) / 256;
     __shared__ double sm[4][FZ_NE];
@@ -286,6 +293,20 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
             }
         }
     };
+    // MODE 2: coarse box of this fine box (box-aligned levels) and the running
+    // sums of the row pairs
+    double cacc[RY / 2 > 0 ? RY / 2 : 1];
+    int cbox = 0, cox = 0, coy = 0, coz = 0;
+    const long CXc = C.bs[0] + 2 * gc.g[0], CXYc = CXc * (C.bs[1] + 2 * gc.g[1]);
+    const long cv0 = (long)C.bs[0] * C.bs[1] * C.bs[2];
+    if constexpr (MODE == 2) {
+        int p[3] = {lo[0] >> 1, lo[1] >> 1, lo[2] >> 1};
+        cbox = owner(C, p);
+        const int* clo = C.lows + 3 * cbox;
+        cox = (lo[0] >> 1) - clo[0];
+        coy = (lo[1] >> 1) - clo[1];
+        coz = (lo[2] >> 1) - clo[2];
+    }
     const int kb = z0 - 2;  // first staged plane
     gload(kb);
     fload(z0);
@@ -343,14 +364,39 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
             rv = rv + fcur[r];
             if (MODE == 0) {
                 res[r] = uc + c.mlam * rv;
-            } else {
+            } else if (MODE == 1) {
                 res[r] = rv;
                 const bool ok = xok && (y0 + w * RY + r) < ny;
                 const double a = rv > 0 ? rv : -rv;
                 m = (ok && m < a) ? a : m;
+            } else {
+                res[r] = rv;
             }
         }
-        zres = zo;
+        if constexpr (MODE == 2) {
+            static_assert(RY % 2 == 0, "row pairs");
+            const bool first = (zo & 1) == 0;
+#pragma unroll
+            for (int q = 0; q < RY / 2; q++) {
+                const double a0 = res[2 * q], a1 = __shfl_down(res[2 * q], 1, 64);
+                const double b0 = res[2 * q + 1], b1 = __shfl_down(res[2 * q + 1], 1, 64);
+                double t = first ? 0.0 : cacc[q];
+                t = t + 0.125 * a0;
+                t = t + 0.125 * a1;
+                t = t + 0.125 * b0;
+                t = t + 0.125 * b1;
+                cacc[q] = t;
+                const int y = y0 + w * RY + 2 * q;
+                if (!first && xok && (x & 1) == 0 && y < ny) {
+                    const long ci = rel_index(C, gc, cox + (x >> 1), coy + (y >> 1), coz + (zo >> 1));
+                    const double lc = fas_L(crs + (long)cbox * cvol, ci, CXc, CXYc, cc);
+                    fco[(long)cbox * cv0 + rel_index(C, G3{{0, 0, 0}}, cox + (x >> 1), coy + (y >> 1),
+                                                     coz + (zo >> 1))] = t - lc;
+                }
+            }
+        } else {
+            zres = zo;
+        }
     }
     put_res();
     if (MODE == 1) {
@@ -540,7 +586,7 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
         if (nb && fold && it == 0) {
             hipLaunchKernelGGL((k_fas_op_z<0, 1, 2>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
                                (double*)buf[1], c, (double*)nullptr, (double*)nullptr, zc,
-                               ldev(M->lv[l + 1].layout), g3(G0), (const double*)delta);
+                               ldev(M->lv[l + 1].layout), g3(G0), (const double*)delta, (double*)nullptr, c);
             PM_CHECK_LAUNCH();
             continue;
         }
@@ -549,7 +595,7 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
             // relaxation is out of place): no exchange
             hipLaunchKernelGGL((k_fas_op_z<0, 1>), zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
                                (const double*)f, (double*)buf[(it + 1) & 1], c, (double*)nullptr, (double*)nullptr,
-                               zc, D, g3(G0), (const double*)nullptr);
+                               zc, D, g3(G0), (const double*)nullptr, (double*)nullptr, c);
             PM_CHECK_LAUNCH();
             continue;
         }
@@ -557,7 +603,7 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
         if (fz_ok(D)) {
             hipLaunchKernelGGL((k_fas_op_z<0, 0>), zgrid, dim3(256), 0, s, D, (const double*)buf[it & 1],
                                (const double*)f, (double*)buf[(it + 1) & 1], c, (double*)nullptr, (double*)nullptr,
-                               zc, D, g3(G0), (const double*)nullptr);
+                               zc, D, g3(G0), (const double*)nullptr, (double*)nullptr, c);
         } else {
             hipLaunchKernelGGL(k_fas_relax, dim3(fas_blocks(cells), D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
                                (const double*)buf[it & 1], (const double*)f, (double*)buf[(it + 1) & 1], c);
@@ -576,17 +622,30 @@ static int fas_vcycle(pm_fas* M, int l, void* u, const void* f, hipStream_t s)
     auto& F = M->lv[l];
     auto& C = M->lv[l + 1];
     FAS_TRY(fas_relax(M, l, u, f, 6, s));
-    FAS_TRY(pm_level_exchange(F.layout, PM_F64, u, G2, 1, s));
     const LayoutDev DF = ldev(F.layout), DC = ldev(C.layout);
+    // crseForce from the z-marching operator kernel where the fine level reads
+    // its neighbours (then u needs no exchange: averageDown reads valid cells)
+    int zc = 0;
+    const dim3 zgrid = fz_grid(DF, &zc);
+    const bool zforce = fz_nb(F.layout) && levels_box_aligned(F.layout, C.layout) && zc % 2 == 0 &&
+                        DF.bs[0] % 2 == 0 && DF.bs[1] % 2 == 0 && DF.bs[2] % 2 == 0 &&
+                        gvol(DC, g3(G2)) * DC.nboxes < (1L << 31);
+    if (!zforce) { FAS_TRY(pm_level_exchange(F.layout, PM_F64, u, G2, 1, s)); }
     const long cc = (long)DC.bs[0] * DC.bs[1] * DC.bs[2];
     const dim3 grid(fas_blocks(cc), DC.nboxes);
     hipLaunchKernelGGL(k_fas_avgdown, grid, dim3(256), 0, s, DF, DC, g3(G2), g3(G2), g3(G0), (const double*)u,
                        (double*)C.phi, (double*)C.phi0);
     PM_CHECK_LAUNCH();
     FAS_TRY(pm_level_exchange(C.layout, PM_F64, C.phi, G2, 1, s));
-    hipLaunchKernelGGL(k_fas_coarse_force, grid, dim3(256), 0, s, DF, DC, g3(G2), g3(G0), g3(G2), g3(G0),
-                       (const double*)u, (const double*)f, (const double*)C.phi, (double*)C.f, fas_coefs(F.dx),
-                       fas_coefs(C.dx));
+    if (zforce) {
+        hipLaunchKernelGGL((k_fas_op_z<2, 1>), zgrid, dim3(256), 0, s, DF, (const double*)u, (const double*)f,
+                           (double*)nullptr, fas_coefs(F.dx), (double*)nullptr, (double*)nullptr, zc, DC, g3(G2),
+                           (const double*)C.phi, (double*)C.f, fas_coefs(C.dx));
+    } else {
+        hipLaunchKernelGGL(k_fas_coarse_force, grid, dim3(256), 0, s, DF, DC, g3(G2), g3(G0), g3(G2), g3(G0),
+                           (const double*)u, (const double*)f, (const double*)C.phi, (double*)C.f, fas_coefs(F.dx),
+                           fas_coefs(C.dx));
+    }
     PM_CHECK_LAUNCH();
     FAS_TRY(fas_vcycle(M, l + 1, C.phi, C.f, s));
     hipLaunchKernelGGL(k_fas_delta, grid, dim3(256), 0, s, DC, g3(G2), g3(G0), (const double*)C.phi, (double*)C.phi0);
@@ -614,11 +673,11 @@ static int fas_resnorm(pm_fas* M, void* u, const void* f, double* out, hipStream
         if (nb) {
             hipLaunchKernelGGL((k_fas_op_z<1, 1>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
                                (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc, D, g3(G0),
-                               (const double*)nullptr);
+                               (const double*)nullptr, (double*)nullptr, fas_coefs(Lv.dx));
         } else {
             hipLaunchKernelGGL((k_fas_op_z<1, 0>), zgrid, dim3(256), 0, s, D, (const double*)u, (const double*)f,
                                (double*)nullptr, fas_coefs(Lv.dx), M->d_part, res_out, zc, D, g3(G0),
-                               (const double*)nullptr);
+                               (const double*)nullptr, (double*)nullptr, fas_coefs(Lv.dx));
         }
     } else {
         hipLaunchKernelGGL(k_fas_resnorm, dim3(bpb, D.nboxes), dim3(256), 0, s, D, g3(G2), g3(G0),
