@@ -333,7 +333,9 @@ typedef struct pm_mg_options {
     double tol;     /* stop when resnorm < tol * resnorm_0 (LevelMultigridSolverImplem.H:129) */
     int use_graph;  /* capture the V-cycle into a hipGraph and replay it */
     int fused;      /* GSRB sweeps as one out-of-place pass each (pm_level_gsrb_sweep, same
-                       bits; 3-D periodic levels) instead of exchange / red / exchange / black */
+                       bits; 3-D periodic levels) instead of exchange / red / exchange / black;
+                       one-box levels of up to 4096 cells run all sweeps of a relaxation in
+                       one launch (same bits) */
 } pm_mg_options;
 void pm_mg_default_options(pm_mg_options* opt);
 /* nlevels = number of coarsenings (the numLevels argument of LevelMultigrid). */
