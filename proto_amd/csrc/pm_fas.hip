@@ -405,6 +405,82 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
     }
 }
 
+// All n Jacobi relaxations of a one-box periodic level of at most FS_MAX cells
+// in one workgroup: u lives in LDS (periodic neighbours by index), each thread
+// keeps f of its own cells in registers and computes its new values before a
+// barrier and the write-back (out of place, as the ping-pong).  fas_L's terms
+// in its order, then (L(u) + f) and u + (-lambda) res as k_fas_relax rounds
+// them, so the same bits.  u has 2 ghost layers, f none.  Small levels are
+// latency-bound: one launch instead of an exchange and a relaxation per sweep.
+constexpr int FS_MAX = 4096, FS_PER = FS_MAX / 256;
+
+__global__ __launch_bounds__(256) void k_fas_relax_small(LayoutDev L, double* __restrict__ u,
+                                                         const double* __restrict__ f, FasC c, int nsw)
+{
+    __shared__ double us[FS_MAX];
+    const int nx = L.bs[0], ny = L.bs[1], nz = L.bs[2], N = nx * ny * nz;
+    const long X = nx + 4, XY = X * (ny + 4);
+    double fv[FS_PER], nv[FS_PER];
+#pragma unroll
+    for (int j = 0; j < FS_PER; j++) {
+        const int i = threadIdx.x + 256 * j;
+        if (i < N) {
+            const int x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+            fv[j] = f[i];
+            us[i] = u[(x + 2) + X * (y + 2) + XY * (z + 2)];
+        }
+    }
+    auto at = [&](int x, int y, int z) {
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+        return us[x + nx * (y + ny * z)];
+    };
+    for (int sw = 0; sw < nsw; sw++) {
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < FS_PER; j++) {
+            const int i = threadIdx.x + 256 * j;
+            if (i >= N) { continue; }
+            const int x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+            const double uc = us[i];
+            double acc = 0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int ex = d == 0, ey = d == 1, ez = d == 2;
+                double fr = 0;
+                fr = fr + c.a[0] * at(x - ex, y - ey, z - ez);
+                fr = fr + c.a[1] * uc;
+                fr = fr + c.a[2] * at(x + ex, y + ey, z + ez);
+                fr = fr + c.a[3] * at(x + 2 * ex, y + 2 * ey, z + 2 * ez);
+                double fl = 0;
+                fl = fl + c.a[0] * at(x - 2 * ex, y - 2 * ey, z - 2 * ez);
+                fl = fl + c.a[1] * at(x - ex, y - ey, z - ez);
+                fl = fl + c.a[2] * uc;
+                fl = fl + c.a[3] * at(x + ex, y + ey, z + ez);
+                acc = acc + c.dp * fr;
+                acc = acc + c.dm * fl;
+            }
+            double o = 0;
+            o = o + acc;
+            double res = o * -1.0;
+            res = res + fv[j];
+            nv[j] = uc + c.mlam * res;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < FS_PER; j++) {
+            const int i = threadIdx.x + 256 * j;
+            if (i < N) { us[i] = nv[j]; }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += 256) {
+        const int x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+        u[(x + 2) + X * (y + 2) + XY * (z + 2)] = us[i];
+    }
+}
+
 // per-block max |L(u) + f|
 __global__ __launch_bounds__(256) void k_fas_resnorm(LayoutDev L, G3 gu, G3 gf, const double* __restrict__ u,
                                                      const double* __restrict__ f, FasC c, double* __restrict__ part,
@@ -581,6 +657,13 @@ static int fas_relax(pm_fas* M, int l, void* u, const void* f, int n, hipStream_
         fold = nb && n > 0 && levels_box_aligned(Lv.layout, CL) &&
                gvol(ldev(CL), g3(G0)) * CL->nboxes < (1L << 31);
         if (!fold) { FAS_TRY(pm_level_prolong_add(Lv.layout, CL, PM_F64, delta, G0, u, G2, 0, s)); }
+    }
+    // a small one-box periodic level: every relaxation in one launch
+    if (!nb && n > 0 && Lv.layout->nboxes == 1 && cells <= FS_MAX && Lv.layout->periodic[0] &&
+        Lv.layout->periodic[1] && Lv.layout->periodic[2]) {
+        hipLaunchKernelGGL(k_fas_relax_small, dim3(1), dim3(256), 0, s, D, (double*)u, (const double*)f, c, n);
+        PM_CHECK_LAUNCH();
+        return PM_OK;
     }
     for (int it = 0; it < n; it++) {
         if (nb && fold && it == 0) {
