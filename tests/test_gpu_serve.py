@@ -6,8 +6,7 @@
   whose FP64 form is bitwise Proto's.
 - A request's answer is the same alone and in a mix of other requests.
 - The plain FP64 mode is the multigrid driver's FP64 solve (pm_mg precision 0),
-  and the two modes agree to
-  the solve tolerance.
+  and the two modes agree to the solve tolerance.
 - Admission: over-budget requests are rejected; a budget for one hierarchy
   serialises requests; hierarchies stay resident and within the budget.
 - A request that runs out of iterations reports NOT_CONVERGED."""
