@@ -546,9 +546,10 @@ def serve_trace(nreq, lam=200.0, seed=4, sizes=(32, 48, 64, 96, 128, 192, 256), 
 
 def serve_config5(pa, stream, nreq):
     """Config 5: the served Poisson stream through the batched FP32-IR server
-    and through the plain FP64 one-at-a-time solver; plus both at saturation
-    (every request present at t = 0) for the capacity in solves/s, at the
-    stream's tolerance 1e-8 and at relative residual 1e-10."""
+    and through the plain FP64 one-at-a-time solver; the same requests arriving
+    at 400 and 800 req/s; plus both at saturation (every request present at
+    t = 0) for the capacity in solves/s, at the stream's tolerance 1e-8 and at
+    relative residual 1e-10."""
     import numpy as np
     reqs = serve_trace(nreq)
 
@@ -570,6 +571,12 @@ def serve_config5(pa, stream, nreq):
         t0 = time.perf_counter()
         st = srv.run(reqs, stream.h)
         out[name] = summary(st, time.perf_counter() - t0)
+        # the same sizes and seeds arriving faster (Poisson, 400 and 800 req/s):
+        # latency under load between the 200/s stream and saturation
+        for lam in (400, 800):
+            t0 = time.perf_counter()
+            st = srv.run(serve_trace(nreq, lam=float(lam)), stream.h)
+            out[name]["lambda_%d" % lam] = summary(st, time.perf_counter() - t0)
         sat = [dict(r, arrival_s=0.0) for r in reqs]
         t0 = time.perf_counter()
         st = srv.run(sat, stream.h)
