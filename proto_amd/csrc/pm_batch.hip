@@ -335,9 +335,19 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
             const T* pl = slot(z);
             T* base = po + Gxy * (long)(z + g);
             const int ny_t = min(R, bx.ny - y0);
+            // element e = tid + 256 j <-> (row y0 + e / nx, column e % nx),
+            // walked without divisions; non-temporal stores (FP64 sweep 0.765
+            // vs 0.827 ms on 2000 boxes, FP32 0.509 vs 0.521 ms)
+            int yy = y0 + tid / bx.nx, xx = tid % bx.nx;
+            const int ys = 256 / bx.nx, xs = 256 % bx.nx;
             for (int e = tid; e < ny_t * bx.nx; e += 256) {
-                const int yy = y0 + e / bx.nx, xx = e % bx.nx;
-                base[(xx + g) + Gx * (long)(yy + g)] = pl[(yy - (y0 - 2)) * LX + xx + 1];
+                __builtin_nontemporal_store(pl[(yy - (y0 - 2)) * LX + xx + 1], &base[(xx + g) + Gx * (long)(yy + g)]);
+                yy += ys;
+                xx += xs;
+                if (xx >= bx.nx) {
+                    xx -= bx.nx;
+                    yy++;
+                }
             }
         };
         // every memory operation of a step is issued at its start (loads of

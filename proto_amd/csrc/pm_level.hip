@@ -372,7 +372,14 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
         } else {
             for (int e = threadIdx.x; e < cnt; e += 256) {
                 const int r = e / TWc, xx = e - r * TWc;
-                ob[rel_index(L, gout, x0 + xx, y0 + r, z)] = pl[(r + 2) * W + xx + 2];
+                // non-temporal in FP64: 0.736 vs 0.756 ms for the 512^3 sweep
+                // (FP32 measured 1 % slower with it)
+                T* o = &ob[rel_index(L, gout, x0 + xx, y0 + r, z)];
+                if constexpr (sizeof(T) == 8) {
+                    __builtin_nontemporal_store(pl[(r + 2) * W + xx + 2], o);
+                } else {
+                    *o = pl[(r + 2) * W + xx + 2];
+                }
             }
         }
     };
