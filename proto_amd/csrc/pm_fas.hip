@@ -287,7 +287,8 @@ __global__ __launch_bounds__(256) void k_fas_op_z(LayoutDev L, const double* __r
             const int y = y0 + w * RY + r;
             if (y >= ny) { continue; }
             if (MODE == 0) {
-                out[(long)b * vol + XY * (zres + 2) + X * (y + 2) + (x + 2)] = res[r];
+                // non-temporal: the 31-cycle 512^3 solve 0.406 vs 0.434 s
+                __builtin_nontemporal_store(res[r], &out[(long)b * vol + XY * (zres + 2) + X * (y + 2) + (x + 2)]);
             } else if (res_out) {
                 res_out[(long)b * nv + x + (long)nx * (y + (long)ny * zres)] = res[r];
             }
