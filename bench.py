@@ -527,21 +527,9 @@ def batch_config4(pa, stream, dtype, reps=10, nboxes=2000, per_box=True):
 
 
 def serve_trace(nreq, lam=200.0, seed=4, sizes=(32, 48, 64, 96, 128, 192, 256), tol=1e-8):
-    """Config 5 request stream: Poisson arrivals at lam req/s (exponential gaps
-    from the SplitMix64 stream (seed, 1)), edge uniform over the index of
-    `sizes` (stream 2), K ~ U{4..32} Gaussians (stream 3), request seeds 1000+i."""
-    import numpy as np
-    from proto_amd.inputs import _splitmix_uniform
-    u1 = (_splitmix_uniform(seed, 1, nreq) + 1.0) * 0.5
-    u2 = (_splitmix_uniform(seed, 2, nreq) + 1.0) * 0.5
-    u3 = (_splitmix_uniform(seed, 3, nreq) + 1.0) * 0.5
-    t = np.cumsum(-np.log1p(-u1) / lam)
-    reqs = []
-    for i in range(nreq):
-        n = sizes[min(int(u2[i] * len(sizes)), len(sizes) - 1)]
-        k = 4 + min(int(u3[i] * 29), 28)
-        reqs.append({"arrival_s": float(t[i]), "n": n, "ngauss": k, "seed": 1000 + i, "tol": tol, "keep_result": 0})
-    return reqs
+    """Config 5 request stream (proto_amd.inputs.serve_trace)."""
+    from proto_amd.inputs import serve_trace as trace
+    return trace(nreq, lam, seed, sizes, tol)
 
 
 def serve_config5(pa, stream, nreq):

@@ -78,6 +78,11 @@ int pm_copy_device(void* dst, const void* src, size_t bytes, int nontemporal, pm
 int pm_stream_probe(int kind, void* dst, const void* src, size_t bytes, int blocks_per_cu, pm_stream_t stream);
 int pm_stream_create(pm_stream_t* stream);
 int pm_stream_destroy(pm_stream_t stream);
+/* Per-stream reduction scratch (no counterpart in Proto, which allocates per
+ * call, Proto_ReductionImplem.H:161-179): its current size and the number of
+ * superseded buffers kept alive because a graph capture may reference them
+ * (see the graphs section). */
+int pm_stream_scratch_info(pm_stream_t stream, size_t* bytes, int* retired);
 int pm_stream_sync(pm_stream_t stream);
 int pm_device_sync(void);
 /* HIP events for timing work on a stream (hipEventRecord / hipEventElapsedTime). */
@@ -503,7 +508,14 @@ int pm_server_resident_bytes(const pm_server* srv, size_t* bytes);
  * above, and replay: the V-cycle and the batched operators are launch-bound
  * sequences that replay as one submission.  No counterpart in the reference
  * (its prototype, experimental/cuda-graph/cuda-graph.H, is not in the
- * library). */
+ * library).
+ * Entry points that reduce keep per-block partials in a scratch buffer owned
+ * by the stream.  A capture records that buffer's address, so a buffer seen
+ * by a capture is never freed when a later, larger call grows the scratch:
+ * it is kept until pm_stream_destroy, and graphs captured on a stream must
+ * not be launched after that stream is destroyed.  A call that would need
+ * more scratch than the stream holds while it is capturing fails with
+ * PM_ERR_UNSUPPORTED (run it once on the stream before capturing). */
 typedef void* pm_graph_t;
 int pm_graph_begin(pm_stream_t stream);
 int pm_graph_end(pm_stream_t stream, pm_graph_t* graph);

@@ -18,6 +18,11 @@
 // file is compiled inside this translation unit, from where it lies.
 #include "LevelMultigrid.cpp"
 #include "BoxOp_Laplace.H"
+// the test operators of Proto's own BoxOp / LevelOp tests (tests/BoxOp_Test*.H)
+#include "BoxOp_TestLaplace.H"
+#include "BoxOp_TestFlux.H"
+#include "BoxOp_TestSource.H"
+#include "BoxOp_TestDiag.H"
 
 #include <cstring>
 #include <vector>
@@ -99,6 +104,29 @@ int lib_stencil_face(const char* name, int arg, double* coefs, int* offs, int* s
     if (nm == "DiffCellToFaceLo") { S = Stencil<double>::DiffCellToFace(arg, Side::Lo); }
     else if (nm == "DiffCellToFaceHi") { S = Stencil<double>::DiffCellToFace(arg, Side::Hi); }
     else { return lib_stencil<double>(name, arg, coefs, offs, sr, dr, ds, maxn); }
+    return dump_stencil(S, coefs, offs, sr, dr, ds, maxn);
+}
+
+// The rest of the stencil library (Proto_StencilDefs.H:152-229, 273-300,
+// 356-382, 392-527), FP64, with up to three integer arguments.
+int lib_stencil_ext(const char* name, int a, int b, int c, double* coefs, int* offs, int* sr, int* dr, int* ds,
+                    int maxn)
+{
+    std::string nm(name);
+    Stencil<double> S;
+    const Side::LoHiSide side = b ? Side::Hi : Side::Lo;
+    if (nm == "Identity") { S = Stencil<double>::Identity(); }
+    else if (nm == "LaplacianFace") { S = Stencil<double>::LaplacianFace(a, b); }
+    else if (nm == "CellToFace") { S = Stencil<double>::CellToFace(a, side, c); }
+    else if (nm == "CellToFaceL") { S = Stencil<double>::CellToFaceL(a, b); }
+    else if (nm == "CellToFaceH") { S = Stencil<double>::CellToFaceH(a, b); }
+    else if (nm == "DiffCellToFace") { S = Stencil<double>::DiffCellToFace(a, side, c); }
+    else if (nm == "Derivative") { S = Stencil<double>::Derivative(a, b, c); }
+    else if (nm == "AvgDownFace") { S = Stencil<double>::AvgDownFace(a, side, c); }
+    else if (nm == "faceToCell") { S = Stencil<double>::faceToCell(a, b); }
+    else if (nm == "cornersToFaces") { S = Stencil<double>::cornersToFaces(a, b); }
+    else if (nm == "CornersToCells") { S = Stencil<double>::CornersToCells(a); }
+    else { return 0; }
     return dump_stencil(S, coefs, offs, sr, dr, ds, maxn);
 }
 
@@ -389,6 +417,11 @@ void ref_stencil_apply_f(int n, const float* coefs, const int* offs, const int* 
 
 // Reduction<T,OP,HOST> (Proto_ReductionImplem.H:64-102,157-192); op = Max,Min,Abs,Sum,SumAbs,SumSquare.
 double ref_reduce(int op, const double* data, long n) { return reduce_dispatch<double>(op, data, n); }
+int ref_stencil_lib_ext(const char* name, int a, int b, int c, double* coefs, int* offs, int* sr, int* dr,
+                        int* ds, int maxn)
+{
+    return lib_stencil_ext(name, a, b, c, coefs, offs, sr, dr, ds, maxn);
+}
 float ref_reduce_f(int op, const float* data, long n) { return reduce_dispatch<float>(op, data, n); }
 
 // InterpStencil<double>::{Constant,Linear}(ratio) applied as fine (+= or |=) I(crse)
@@ -680,6 +713,67 @@ int ref_mg_fas_solve(int n, int boxSize, int numLevels, const double* rhs, doubl
     }
     level_to_global(state, phi, n);
     return it;
+}
+
+// LevelOp<BoxOp_TestLaplace> applied to a periodic level
+// (tests/LevelOpTests.cpp:22-60 with a seeded state): state from the global
+// n^3 array (2 ghost layers, exchanged), out = L(state) with Proto's default
+// scalings (flux scale 1, no diagonal), written back as a global array.
+void ref_levelop_apply(int n, int boxSize, const double* phi, double* out)
+{
+    DisjointBoxLayout layout = periodic_layout(n, boxSize);
+    Array<double, DIM> dx;
+    dx.fill(1.0 / n);
+    LevelOp<BoxOp_TestLaplace, double, PeriodicBC, HOST> op(layout, dx);
+    LevelBoxData<double, 1, HOST> state(layout, Point::Ones(2));
+    LevelBoxData<double, 1, HOST> dst(layout, Point::Zeros());
+    state.setToZero();
+    level_from_global(state, phi, n);
+    state.exchange();
+    dst.setToZero();
+    op(dst, state);
+    level_to_global(dst, out, n);
+}
+
+// BoxOp_TestFlux / _TestSource / _TestDiag (which 0 / 1 / 2) on the single box
+// [0, n)^3 of a non-periodic domain, as tests/BoxOpTests.cpp:9-210 sets them
+// up (flux scale 31 for flux and source, diagonal scale 31 for diag, operator
+// scale 17), with seeded src and aux on the box grown by one.  dst gets L on
+// the box; flux (which 0 only) the DIM face fluxes one after another.
+void ref_boxop_apply(int which, int n, const double* src, const double* aux, double* dst, double* flux)
+{
+    Box B0 = Box::Cube(n);
+    Array<bool, DIM> per;
+    per.fill(false);
+    ProblemDomain domain(B0, per);
+    DisjointBoxLayout layout(domain, B0.sizes());
+    const double dx = 1.0 / n;
+    BoxData<double, 1, HOST> hsrc(B0.grow(1)), haux(B0.grow(1)), hdst(B0);
+    std::memcpy(hsrc.data(), src, sizeof(double) * hsrc.box().size());
+    std::memcpy(haux.data(), aux, sizeof(double) * haux.box().size());
+    Array<BoxData<double, 1, HOST>, DIM> fl;
+    for (int d = 0; d < DIM; d++) { fl[d].define(B0.grow((Centering)d)); }
+    if (which == 0) {
+        BoxOp_TestFlux<double, HOST> op(layout, *layout.begin(), dx);
+        op.setFluxScale(31.0);
+        op(hdst, fl, hsrc, haux, 17.0);
+    } else if (which == 1) {
+        BoxOp_TestSource<double, HOST> op(layout, *layout.begin(), dx);
+        op.setFluxScale(31.0);
+        op(hdst, hsrc, haux, 17.0);
+    } else {
+        BoxOp_TestDiag<double, HOST> op(layout, *layout.begin(), dx);
+        op.setDiagScale(31.0);
+        op(hdst, hsrc, haux, 17.0);
+    }
+    std::memcpy(dst, hdst.data(), sizeof(double) * B0.size());
+    if (which == 0 && flux) {
+        long off = 0;
+        for (int d = 0; d < DIM; d++) {
+            std::memcpy(flux + off, fl[d].data(), sizeof(double) * fl[d].box().size());
+            off += fl[d].box().size();
+        }
+    }
 }
 
 } // extern "C"

@@ -105,6 +105,7 @@ SIGNATURES = {
     "pm_memset_zero": (_i, [_vp, _sz, _vp]),
     "pm_stream_create": (_i, [ctypes.POINTER(_vp)]),
     "pm_stream_destroy": (_i, [_vp]),
+    "pm_stream_scratch_info": (_i, [_vp, ctypes.POINTER(_sz), _ip]),
     "pm_stream_sync": (_i, [_vp]),
     "pm_device_sync": (_i, []),
     "pm_stencil_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _ip, _ip, _ip, _ip]),

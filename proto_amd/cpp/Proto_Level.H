@@ -129,12 +129,14 @@ public:
 
     struct iterator {
         int i;
+        int n;
         DataIndex operator*() const { return i; }
         iterator& operator++() { ++i; return *this; }
         bool operator!=(const iterator& o) const { return i != o.i; }
+        bool ok() const { return i < n; }
     };
-    iterator begin() const { return {0}; }
-    iterator end() const { return {size()}; }
+    iterator begin() const { return {0, size()}; }
+    iterator end() const { return {size(), size()}; }
 
     // the device layout handle (created on first use)
     pm_layout* handle() const
@@ -166,9 +168,10 @@ private:
     std::shared_ptr<Impl> m_impl;
 };
 
-template<typename T, unsigned int C = 1, MemType MEM = MEMTYPE_DEFAULT>
+template<typename T, unsigned int C = 1, MemType MEM = MEMTYPE_DEFAULT, Centering CTR = PR_CELL>
 class LevelBoxData {
     static_assert(MEM == DEVICE, "LevelBoxData on this library lives on the device");
+    static_assert(CTR == PR_CELL, "LevelBoxData on this library holds cell-centred data");
 
 public:
     LevelBoxData() {}
@@ -268,6 +271,14 @@ public:
     void increment(const LevelBoxData& o, T scale = 1)
     {
         for (auto it : m_layout) { m_boxes[it].increment(o[it], scale); }
+    }
+    // initialize (Proto_LevelBoxDataImplem.H:176-187): a pointwise functor run
+    // with forallInPlace_p on every ghosted box (needs hipcc); the extra
+    // arguments are passed to every box as they are
+    template<typename Func, typename... Srcs>
+    void initialize(const Func& f, Srcs&&... srcs)
+    {
+        for (auto it : m_layout) { forallInPlace_p(f, m_boxes[it], std::forward<Srcs>(srcs)...); }
     }
 
 private:

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 #include <initializer_list>
 #include <iostream>
 #include <vector>
@@ -92,6 +93,90 @@ inline std::ostream& operator<<(std::ostream& os, const Point& p)
     return os;
 }
 
+// Side of a cell face (Proto_Face.H: class Side with enum LoHiSide).
+class Side {
+public:
+    enum LoHiSide { Invalid = -1, Lo = 0, Hi, NUMSIDES };
+};
+inline Side::LoHiSide flip(const Side::LoHiSide& s) { return s == Side::Lo ? Side::Hi : Side::Lo; }
+inline int sign(const Side::LoHiSide& s) { return s == Side::Lo ? -1 : 1; }
+
+// Data centering (Proto_Centering.H): cells, the faces normal to direction d, nodes.
+enum Centering {
+    PR_CELL = -1, PR_FACE_0 = 0, PR_FACE_1, PR_FACE_2, PR_FACE_3, PR_FACE_4, PR_FACE_5, PR_FACE_6,
+    PR_NODE = 7, PR_FACE, PR_EDGE
+};
+
+// A face of a box: its normal direction and side (Proto_Face.H).
+struct Face {
+    int dir = 0;
+    Side::LoHiSide side = Side::Lo;
+    Face() {}
+    Face(int d, Side::LoHiSide s) : dir(d), side(s) {}
+    bool operator==(const Face& o) const { return dir == o.dir && side == o.side; }
+};
+
+// Fixed-size array with element-wise arithmetic (Proto_Array.H): grid
+// spacings, per-direction fluxes.  Elements are value-initialised.
+template<typename T, size_t N>
+class Array {
+public:
+    Array() : m_data{} {}
+    Array(T v) { fill(v); }
+    // the first N values of the list (Proto_ArrayImplem.H:31-40)
+    Array(std::initializer_list<T> list) : m_data{}
+    {
+        size_t i = 0;
+        for (const T& x : list) {
+            if (i >= N) { break; }
+            m_data[i++] = x;
+        }
+    }
+    Array(const Point& p) : m_data{}
+    {
+        for (int d = 0; d < (int)N && d < DIM; d++) { m_data[d] = (T)p[d]; }
+    }
+    Array(const std::array<T, N>& a) : m_data(a) {}
+    static Array Zeros() { return Array(T(0)); }
+    static Array Ones(T scale = 1) { return Array(scale); }
+
+    void fill(T v) { for (auto& x : m_data) { x = v; } }
+    T* data() { return m_data.data(); }
+    const T* data() const { return m_data.data(); }
+    T& operator[](size_t i) { return m_data[i]; }
+    const T& operator[](size_t i) const { return m_data[i]; }
+    static constexpr size_t size() { return N; }
+    bool operator==(const Array& o) const { return m_data == o.m_data; }
+    bool operator!=(const Array& o) const { return !(*this == o); }
+
+#define PR_ARRAY_OP(op)                                                                                \
+    Array operator op(const Array& o) const { Array r; for (size_t i = 0; i < N; i++) { r[i] = m_data[i] op o[i]; } return r; } \
+    Array operator op(T v) const { Array r; for (size_t i = 0; i < N; i++) { r[i] = m_data[i] op v; } return r; }             \
+    void operator op##=(const Array& o) { for (size_t i = 0; i < N; i++) { m_data[i] = m_data[i] op o[i]; } }                  \
+    void operator op##=(T v) { for (size_t i = 0; i < N; i++) { m_data[i] = m_data[i] op v; } }
+    PR_ARRAY_OP(+)
+    PR_ARRAY_OP(-)
+    PR_ARRAY_OP(*)
+    PR_ARRAY_OP(/)
+#undef PR_ARRAY_OP
+    Array operator-() const { Array r; for (size_t i = 0; i < N; i++) { r[i] = -m_data[i]; } return r; }
+
+    T max() const { T r = m_data[0]; for (size_t i = 1; i < N; i++) { r = std::max(r, m_data[i]); } return r; }
+    T min() const { T r = m_data[0]; for (size_t i = 1; i < N; i++) { r = std::min(r, m_data[i]); } return r; }
+    T absMax() const { T r = 0; for (size_t i = 0; i < N; i++) { r = std::max(r, (T)std::abs(m_data[i])); } return r; }
+    T sum() const { T r = 0; for (size_t i = 0; i < N; i++) { r += m_data[i]; } return r; }
+    T product() const { T r = 1; for (size_t i = 0; i < N; i++) { r *= m_data[i]; } return r; }
+    T dot(const Array& o) const { T r = 0; for (size_t i = 0; i < N; i++) { r += m_data[i] * o[i]; } return r; }
+    T norm() const { return std::sqrt(dot(*this)); }
+    T* begin() { return m_data.data(); }
+    T* end() { return m_data.data() + N; }
+    const T* begin() const { return m_data.data(); }
+    const T* end() const { return m_data.data() + N; }
+
+private:
+    std::array<T, N> m_data;
+};
+
 class Box;
 
 class BoxIterator {
@@ -145,6 +230,27 @@ public:
     Box grow(int g) const { return grow(Point::Ones(g)); }
     Box grow(const Point& g) const { return Box(m_low - g, m_high + g); }
     Box grow(int dir, int g) const { return grow(Point::Basis(dir, g)); }
+    // one side only (Proto_BoxImplem.H:409-413).  The direction is
+    // Point::Basis(dir, side) as Proto forms it, so Side::Lo (0) gives the zero
+    // vector and the box comes back unchanged; Side::Hi grows the high end.
+    Box grow(int dir, Side::LoHiSide side, int g) const { return extrude(Point::Basis(dir, (int)side), g); }
+    // face / node boxes of a cell box (Proto_BoxImplem.H:415-433)
+    Box grow(Centering ctr) const
+    {
+        if (ctr == PR_CELL) { return *this; }
+        if (ctr == PR_NODE) { return extrude(Point::Ones(), 1); }
+        if ((int)ctr >= 0 && (int)ctr < DIM) { return extrude(Point::Basis((int)ctr, 1)); }
+        throw std::invalid_argument("Box::grow(Centering): undefined for FACE / EDGE centering");
+    }
+    // every direction of dir that is positive grows the high end, negative the low end
+    Box extrude(const Point& dir, int dist = 1) const
+    {
+        Point lo = m_low, hi = m_high;
+        for (int d = 0; d < DIM; d++) {
+            if (dir[d] > 0) { hi[d] += dist; } else if (dir[d] < 0) { lo[d] -= dist; }
+        }
+        return Box(lo, hi);
+    }
 
     // intersection
     Box operator&(const Box& b) const
@@ -210,6 +316,57 @@ public:
         Point lo = m_low, hi = m_high;
         if (d > 0) { hi[dir] += d; } else { lo[dir] += d; }
         return Box(lo, hi);
+    }
+    // the region of thickness dist just outside the box in the directions of
+    // dir (Proto_BoxImplem.H:685-722); dist < 0: inside, as edge
+    Box adjacent(const Point& dir, int dist) const
+    {
+        if (dist == 0) { return Box(); }
+        if (dist < 0) { return edge(dir, -dist); }
+        Point lo = m_low, hi = m_high;
+        for (int k = 0; k < DIM; k++) {
+            if (dir[k] > 0) { lo[k] = m_high[k] + 1; hi[k] = m_high[k] + dist; }
+            else if (dir[k] < 0) { hi[k] = m_low[k] - 1; lo[k] = m_low[k] - dist; }
+        }
+        return Box(lo, hi);
+    }
+    Box adjacent(const Point& dir) const
+    {
+        Point lo = m_low, hi = m_high;
+        for (int k = 0; k < DIM; k++) {
+            if (dir[k] > 0) { lo[k] = m_high[k] + 1; hi[k] = m_high[k] + dir[k]; }
+            else if (dir[k] < 0) { hi[k] = m_low[k] - 1; lo[k] = m_low[k] + dir[k]; }
+        }
+        return Box(lo, hi);
+    }
+    // the layer of cells inside the box along the directions of dir, dir[k]
+    // cells thick (Proto_BoxImplem.H:608-640)
+    Box edge(const Point& dir) const
+    {
+        Point lo = m_low, hi = m_high;
+        for (int k = 0; k < DIM; k++) {
+            if (dir[k] > 0) { lo[k] = m_high[k] - dir[k] + 1; }
+            if (dir[k] < 0) { hi[k] = m_low[k] - dir[k] - 1; }
+        }
+        return Box(lo, hi);
+    }
+    Box edge(const Point& dir, int dist) const
+    {
+        if (dist == 0) { return Box(); }
+        if (dist < 0) { return adjacent(dir, -dist); }
+        Point d;
+        for (int k = 0; k < DIM; k++) { d[k] = dir[k] > 0 ? dist : (dir[k] < 0 ? -dist : 0); }
+        return edge(d);
+    }
+    // Point::Basis(dir, side) as Proto forms it (Side::Lo gives the zero vector, 636-640)
+    Box edge(int dir, Side::LoHiSide side, int dist) const { return edge(Point::Basis(dir, (int)side), dist); }
+    // the cells of the low or high face in direction dir, at most dist thick (642-658)
+    Box face(int dir, Side::LoHiSide side, int dist = 1) const
+    {
+        const int d = std::min(dist, size(dir));
+        if (side == Side::Lo) { return edge(Point::Basis(dir, -1), d); }
+        if (side == Side::Hi) { return edge(Point::Basis(dir, 1), d); }
+        return Box();
     }
 
     // Box::sindex / index and its inverse, x fastest (Proto_BoxImplem.H:76-86, 208-222)

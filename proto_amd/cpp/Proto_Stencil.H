@@ -14,10 +14,6 @@
 
 namespace Proto {
 
-namespace Side {
-enum LoHiSide { Lo = 0, Hi = 1 };
-}
-
 class Shift {
 public:
     Shift() {}
@@ -331,6 +327,136 @@ public:
         for (auto it = K.begin(); it != K.end(); ++it) { r += ((T)1.0) * Shift(*it); }
         r.m_srcRefratio = ratio;
         return r;
+    }
+    static Stencil Identity() { return ((T)1.0) * Shift::Zeros(); }
+    // second-order Laplacian in the directions other than dir (Proto_StencilDefs.H:152-182)
+    static Stencil LaplacianFace(int dir, int order = 2)
+    {
+        if (order != 2) { throw ProtoError(PM_ERR_UNSUPPORTED, "Stencil::LaplacianFace: order 2 only"); }
+        Stencil r = ((T)(-2 * (DIM - 1))) * Shift(Point::Zeros());
+        for (int d = 0; d < DIM; d++) {
+            if (d == dir) { continue; }
+            r += ((T)1) * Shift(Point::Basis(d));
+            r += ((T)1) * Shift(Point::Basis(d) * (-1));
+        }
+        return r;
+    }
+    // cell averages to the low face of direction dir, orders 2, 4, 5; Side::Hi
+    // mirrors it (Proto_StencilDefs.H:184-229); terms in Proto's order
+    static Stencil CellToFace(int dir, Side::LoHiSide side = Side::Lo, int order = 4)
+    {
+        Stencil r;
+        if (order == 2) {
+            r = ((T).5) * Shift::Basis(dir, -1) + ((T).5) * Shift::Zeros();
+        } else if (order == 4) {
+            T c[4] = {-1, 7, 7, -1};
+            r = c[0] * Shift::Basis(dir, -2) + c[3] * Shift::Basis(dir, 1) + c[1] * Shift::Basis(dir, -1) +
+                c[2] * Shift::Zeros();
+            r *= (T)(1.0 / 12.0);
+        } else if (order == 5) {
+            T c[5] = {2, -13, 47, 27, -3};
+            r = c[0] * Shift::Basis(dir, -3) + c[4] * Shift::Basis(dir, 1) + c[1] * Shift::Basis(dir, -2) +
+                c[3] * Shift::Zeros() + c[2] * Shift::Basis(dir, -1);
+            r *= (T)(1 / 60.0);
+        } else {
+            throw ProtoError(PM_ERR_UNSUPPORTED, "Stencil::CellToFace: order must be 2, 4 or 5");
+        }
+        if (side == Side::Hi) { r.invert(dir); }
+        return r;
+    }
+    // Proto_StencilDefs.H:270-300
+    static Stencil CellToFaceL(int dir, int order = 4) { return CellToFace(dir, Side::Lo, order); }
+    static Stencil CellToFaceH(int dir, int order = 4)
+    {
+        if (order % 2 == 0) { return CellToFace(dir, Side::Lo, order); }
+        Stencil S = CellToFace(dir, Side::Hi, order);
+        return S * (((T)1.0) * Shift(Point::Basis(dir, -1)));
+    }
+    // AvgDown over the cells of one face of the refinement kernel (356-382)
+    static Stencil AvgDownFace(int dir, Side::LoHiSide side, int ratio)
+    {
+        return AvgDownFace(dir, side, Point::Ones(ratio));
+    }
+    static Stencil AvgDownFace(int dir, Side::LoHiSide side, Point ratio)
+    {
+        Stencil r;
+        Box K(ratio);
+        K = K.face(dir, side);
+        T c = 1.0 / K.size();
+        for (auto it = K.begin(); it != K.end(); ++it) { r += c * Shift(*it); }
+        r.m_srcRefratio = ratio;
+        return r;
+    }
+    // n-th derivative in direction dir to the given (even) order, from powers
+    // of the second difference (Proto_StencilDefs.H:392-457)
+    static Stencil Derivative(int n, int dir, int order = 2)
+    {
+        constexpr int MAXORDH = 6;
+        if (dir < 0 || dir >= DIM || n < 1 || n > 2 * MAXORDH || order < 1 || order > 2 * MAXORDH ||
+            2 * MAXORDH + 2 - n - order < 0) {
+            throw ProtoError(PM_ERR_INVALID, "Stencil::Derivative: derivative, direction or order out of range");
+        }
+        if (order % 2 != 0) { order++; }
+        const Point e = Point::Basis(dir);
+        Stencil factor = ((T)(1.0)) * Shift(e) + ((T)(-2.0)) * Shift(Point::Zeros()) + ((T)(1.0)) * Shift(e * (-1));
+        Stencil deriv;
+        Stencil temps[MAXORDH];
+        if (n % 2 == 0) {
+            // coefficients of the even series in powers of the second difference
+            static const T ev[MAXORDH][MAXORDH] = {{1., -1. / 12, 1. / 90, -1. / 560, 1. / 3150, -1. / 16632},
+                                                   {0, 1., -1. / 6, 7. / 240, -41. / 7560, 479. / 453600},
+                                                   {0, 0, 1., -1. / 4, 13. / 240, -139. / 12096},
+                                                   {0, 0, 0, 1., -1. / 3, 31. / 360},
+                                                   {0, 0, 0, 0, 1., -5. / 12},
+                                                   {0, 0, 0, 0, 0, 1.}};
+            const int p = (n - 2) / 2;
+            temps[0] = factor;
+            for (int k = 1; k < MAXORDH; k++) { temps[k] = temps[k - 1] * factor; }
+            for (int k = 0; k < (n + order - 2) / 2; k++) { deriv += ev[p][k] * temps[k]; }
+        } else {
+            static const T od[MAXORDH][MAXORDH] = {{1., -1. / 6, 1. / 30, -1. / 140, 1. / 630, -1. / 2772},
+                                                   {0, 1., -1. / 4, 7. / 120, -41. / 3024, 479. / 151200},
+                                                   {0, 0, 1., -1. / 3, 13. / 144, -139. / 6048},
+                                                   {0, 0, 0, 1., -5. / 12, 31. / 240},
+                                                   {0, 0, 0, 0, 1., -1. / 2},
+                                                   {0, 0, 0, 0, 0, 1.}};
+            const int p = (n - 1) / 2;
+            temps[0] = ((T)(0.5)) * Shift(e) + ((T)(-0.5)) * Shift(e * (-1));
+            for (int k = 1; k < MAXORDH; k++) { temps[k] = temps[k - 1] * factor; }
+            for (int k = 0; k < (n + 1 + order - 2) / 2; k++) { deriv += od[p][k] * temps[k]; }
+        }
+        return deriv;
+    }
+    // face averages to cell averages in direction dir, orders 2 and 4 (459-477)
+    static Stencil faceToCell(int dir, int order = 4)
+    {
+        Stencil av2 = ((T).5) * Shift::Zeros() + ((T).5) * Shift::Basis(dir, 1);
+        if (order == 2) { return av2; }
+        if (order != 4) { throw ProtoError(PM_ERR_UNSUPPORTED, "Stencil::faceToCell: order must be 2 or 4"); }
+        return (((T)1.0) * Shift::Zeros() + ((T)(-1. / 8.)) * (((T)(-2.0)) * Shift::Zeros() +
+                                                               ((T)1.0) * Shift::Basis(dir, 1) +
+                                                               ((T)1.0) * Shift::Basis(dir, -1))) *
+               av2;
+    }
+    // corner values to face averages normal to dir, orders 2 and 4 (478-502)
+    static Stencil cornersToFaces(int dir, int order = 4)
+    {
+        Stencil av2 = ((T)1.0) * Shift::Zeros();
+        for (int d = 0; d < DIM; d++) {
+            if (d != dir) { av2 = av2 * (((T).5) * Shift::Zeros() + ((T).5) * Shift::Basis(d, 1)); }
+        }
+        if (order == 2) { return av2; }
+        if (order != 4) { throw ProtoError(PM_ERR_UNSUPPORTED, "Stencil::cornersToFaces: order must be 2 or 4"); }
+        return (((T)1.0) * Shift::Zeros() + ((T)(-1. / 12.)) * LaplacianFace(dir, 2)) * av2;
+    }
+    // corner values to cell averages, orders 2 and 4 (503-527)
+    static Stencil CornersToCells(int order = 4)
+    {
+        Stencil av2 = ((T)1.0) * Shift::Zeros();
+        for (int d = 0; d < DIM; d++) { av2 = av2 * (((T).5) * Shift::Zeros() + ((T).5) * Shift::Basis(d, 1)); }
+        if (order == 2) { return av2; }
+        if (order != 4) { throw ProtoError(PM_ERR_UNSUPPORTED, "Stencil::CornersToCells: order must be 2 or 4"); }
+        return (((T)1.0) * Shift::Zeros() + ((T)(-1. / 12.)) * Laplacian()) * av2;
     }
     static Stencil FluxDivergence(int dir)
     {

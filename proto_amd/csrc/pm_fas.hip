@@ -431,12 +431,12 @@ __global__ __launch_bounds__(256) void k_fas_relax_small(LayoutDev L, double* __
             us[i] = u[(x + 2) + X * (y + 2) + XY * (z + 2)];
         }
     }
-    auto at = [&](int x, int y, int z) {
-        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
-        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
-        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-        return us[x + nx * (y + ny * z)];
+    // periodic fold of offsets up to +-2: a level 1 cell wide wraps more than once
+    auto fold = [](int x, int n) {
+        x %= n;
+        return x < 0 ? x + n : x;
     };
+    auto at = [&](int x, int y, int z) { return us[fold(x, nx) + nx * (fold(y, ny) + ny * fold(z, nz))]; };
     for (int sw = 0; sw < nsw; sw++) {
         __syncthreads();
 #pragma unroll

@@ -18,8 +18,9 @@ void set_error(const char* fmt, ...);
 
 // Grow-only device scratch private to one stream (per-block partials of the
 // reductions).  Kernels on different streams never share it; growing it
-// synchronises that stream first.  Not for use inside hipGraph capture unless
-// already large enough.
+// synchronises that stream first.  A buffer handed out during a graph capture
+// is kept (not freed) when the scratch grows later, until the stream is
+// destroyed.  Under capture it cannot grow: PM_ERR_UNSUPPORTED.
 int workspace(hipStream_t stream, size_t bytes, void** out);
 
 // Shape of a layout (pm_level.hip): dim, domain, box size, periodic flags, box count.

@@ -4,6 +4,7 @@
 #include <map>
 #include <mutex>
 #include <algorithm>
+#include <vector>
 
 #include "pm_internal.hpp"
 
@@ -19,24 +20,55 @@ void set_error(const char* fmt, ...)
     va_end(ap);
 }
 
+// Per-stream scratch.  A buffer handed out while the stream was capturing a
+// graph may be referenced by that graph for as long as it lives, which this
+// library cannot see (pm_graph_*, the multigrid driver's cycle graph, or a
+// caller's own capture).  Such a buffer is never freed on growth: it moves to
+// the stream's retired list and is released only by pm_stream_destroy.
+// Buffers never seen by a capture are freed on growth after the stream
+// drains, as before.
 struct WsEntry {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool captured = false;       // handed out while the stream was capturing
+    std::vector<void*> retired;  // superseded buffers a graph may still use
 };
 static std::mutex g_ws_mu;
 static std::map<hipStream_t, WsEntry>* g_ws = nullptr;
+
+static bool stream_capturing(hipStream_t stream)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;  // unknown: treat the buffer as captured
+    }
+    return st != hipStreamCaptureStatusNone;
+}
 
 int workspace(hipStream_t stream, size_t bytes, void** out)
 {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     if (!g_ws) { g_ws = new std::map<hipStream_t, WsEntry>(); }
     WsEntry& w = (*g_ws)[stream];
+    const bool capturing = stream_capturing(stream);
     if (bytes > w.bytes) {
+        if (capturing) {
+            // neither a stream drain nor an allocation may be recorded into a graph
+            set_error("workspace: %zu bytes of per-stream scratch needed under graph capture, %zu available; "
+                      "run the call once on the stream before capturing it", bytes, w.bytes);
+            return PM_ERR_UNSUPPORTED;
+        }
         if (w.ptr) {
-            PM_CHECK_HIP(hipStreamSynchronize(stream));
-            PM_CHECK_HIP(hipFree(w.ptr));
+            if (w.captured) {
+                w.retired.push_back(w.ptr);
+            } else {
+                PM_CHECK_HIP(hipStreamSynchronize(stream));
+                PM_CHECK_HIP(hipFree(w.ptr));
+            }
             w.ptr = nullptr;
             w.bytes = 0;
+            w.captured = false;
         }
         size_t nb = bytes < ((size_t)1 << 16) ? ((size_t)1 << 16) : bytes;
         hipError_t e = hipMalloc(&w.ptr, nb);
@@ -47,21 +79,36 @@ int workspace(hipStream_t stream, size_t bytes, void** out)
         }
         w.bytes = nb;
     }
+    if (capturing) { w.captured = true; }
     *out = w.ptr;
     return PM_OK;
 }
 
-// Drops the scratch of a stream that is about to be destroyed.
+// Scratch of a stream: current size and number of retired buffers kept for graphs.
+static int workspace_info(hipStream_t stream, size_t* bytes, int* retired)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    *bytes = 0;
+    *retired = 0;
+    if (!g_ws) { return PM_OK; }
+    auto it = g_ws->find(stream);
+    if (it == g_ws->end()) { return PM_OK; }
+    *bytes = it->second.bytes;
+    *retired = (int)it->second.retired.size();
+    return PM_OK;
+}
+
+// Drops the scratch of a stream that is about to be destroyed (current and
+// retired buffers: graphs that used them must not be launched afterwards).
 static void workspace_release(hipStream_t stream)
 {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     if (!g_ws) { return; }
     auto it = g_ws->find(stream);
     if (it == g_ws->end()) { return; }
-    if (it->second.ptr) {
-        (void)hipStreamSynchronize(stream);
-        (void)hipFree(it->second.ptr);
-    }
+    (void)hipStreamSynchronize(stream);
+    if (it->second.ptr) { (void)hipFree(it->second.ptr); }
+    for (void* p : it->second.retired) { (void)hipFree(p); }
     g_ws->erase(it);
 }
 
@@ -284,6 +331,12 @@ int pm_stream_destroy(pm_stream_t stream)
     workspace_release(S(stream));
     PM_CHECK_HIP(hipStreamDestroy(S(stream)));
     return PM_OK;
+}
+
+int pm_stream_scratch_info(pm_stream_t stream, size_t* bytes, int* retired)
+{
+    PM_REQUIRE(bytes && retired, "pm_stream_scratch_info: null argument");
+    return workspace_info(S(stream), bytes, retired);
 }
 
 int pm_stream_sync(pm_stream_t stream)

@@ -86,3 +86,22 @@ def batch_sizes(nboxes, edges, seed, stream_id=1):
     u = _splitmix_uniform(seed, stream_id, 3 * nboxes)
     idx = np.minimum(((u + 1.0) * 0.5 * len(edges)).astype(np.int64), len(edges) - 1)
     return np.asarray(edges, dtype=np.int32)[idx].reshape(nboxes, 3)
+
+
+def serve_trace(nreq, lam=200.0, seed=4, sizes=(32, 48, 64, 96, 128, 192, 256), tol=1e-8, keep=()):
+    """The served Poisson stream: arrivals at lam requests/s (exponential gaps
+    from the SplitMix64 stream (seed, 1)), edge uniform over the index of
+    `sizes` (stream 2), K ~ U{4..32} Gaussians (stream 3), request seeds
+    1000 + i, tolerance tol; requests whose index is in `keep` keep their
+    answer for reading back."""
+    u1 = (_splitmix_uniform(seed, 1, nreq) + 1.0) * 0.5
+    u2 = (_splitmix_uniform(seed, 2, nreq) + 1.0) * 0.5
+    u3 = (_splitmix_uniform(seed, 3, nreq) + 1.0) * 0.5
+    t = np.cumsum(-np.log1p(-u1) / lam)
+    reqs = []
+    for i in range(nreq):
+        n = sizes[min(int(u2[i] * len(sizes)), len(sizes) - 1)]
+        k = 4 + min(int(u3[i] * 29), 28)
+        reqs.append({"arrival_s": float(t[i]), "n": n, "ngauss": k, "seed": 1000 + i, "tol": tol,
+                     "keep_result": 1 if i in keep else 0})
+    return reqs

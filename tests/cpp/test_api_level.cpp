@@ -175,37 +175,4 @@ TEST(LevelMultigridSolve)
     }
 }
 
-TEST(LevelMultigridSolverFAS)
-{
-    // Proto's library solver (LevelMultigridSolver<BoxOp_Laplace>), as a Proto
-    // user calls it (examples/LevelPoisson/exec/main.cpp:91-127)
-    const int n = 32;
-    double dx = 1.0 / n;
-    ProblemDomain pd(Box::Cube(n), true);
-    DisjointBoxLayout layout(pd, Point::Ones(16));
-    LevelBoxData<double> state(layout, BoxOp_Laplace<double>::ghost()), force(layout, Point::Zeros()),
-        res(layout, Point::Zeros());
-    state.setToZero();
-    force.setToZero();
-    auto sine = [] PROTO_LAMBDA(const Point& p, Var<double>& v, double h) {
-        double x = (p[0] + 0.5) * h, y = (p[1] + 0.5) * h;
-        v(0) = sin(2 * M_PI * x) * cos(4 * M_PI * y);
-    };
-    for (auto dit : layout) { forallInPlace_p(sine, force[dit], dx); }
-    LevelMultigridSolver<BoxOp_Laplace, double> mg(layout, Point::Ones(2), 4, dx);
-    double r0 = mg.resnorm(res, state, force);
-    CHECK(r0 > 0);
-    CHECK_EQ(r0, res.absMax());
-    double r = mg.solve(state, force, 20, 1e-10);
-    const auto& h = mg.history();
-    CHECK(h.size() >= 2 && h.front() == r0 && h.back() == r);
-    for (size_t i = 1; i < h.size(); i++) { CHECK(h[i] < h[i - 1]); }
-    CHECK(r < 1e-10 * r0);
-    CHECK_EQ(mg.resnorm(res, state, force), r);
-    bool threw = false;
-    LevelBoxData<double> bad(layout, Point::Ones(1));
-    try { mg.solve(bad, force, 1, 0.0); } catch (const ProtoError&) { threw = true; }
-    CHECK(threw);
-}
-
 int main() { return run_all(); }

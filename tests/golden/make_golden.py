@@ -56,6 +56,49 @@ def stencil_table():
     return tab
 
 
+def ext_cases(dim):
+    """(name, a, b, c) argument sets of the rest of Proto's stencil library."""
+    cases = [("Identity", 0, 0, 0)]
+    for d in range(dim):
+        cases.append(("LaplacianFace", d, 2, 0))
+        for side in (0, 1):
+            cases += [("CellToFace", d, side, o) for o in (2, 4, 5)]
+            cases += [("DiffCellToFace", d, side, o) for o in (2, 4)]
+            cases += [("AvgDownFace", d, side, r) for r in (2, 4)]
+        cases += [("CellToFaceL", d, o, 0) for o in (2, 4, 5)]
+        cases += [("CellToFaceH", d, o, 0) for o in (2, 4, 5)]
+        cases += [("faceToCell", d, o, 0) for o in (2, 4)]
+        cases += [("cornersToFaces", d, o, 0) for o in (2, 4)]
+        for n_, o in ((1, 1), (1, 2), (1, 4), (1, 6), (1, 12), (2, 2), (2, 4), (2, 6), (2, 12), (3, 2), (3, 4),
+                      (4, 2), (4, 4), (5, 2), (6, 4), (7, 2), (12, 2)):
+            cases.append(("Derivative", n_, d, o))
+    cases += [("CornersToCells", o, 0, 0) for o in (2, 4)]
+    return cases
+
+
+def stencil_table_ext():
+    """Coefficient lists of LaplacianFace, CellToFace(L/H), DiffCellToFace
+    (orders 2 and 4), Derivative, AvgDownFace, faceToCell, cornersToFaces,
+    CornersToCells and Identity, FP64, as Proto builds them."""
+    tab = {}
+    for dim in (2, 3):
+        R = util.ref(dim)
+        for name, a, b, c in ext_cases(dim):
+            coefs = np.zeros(400)
+            offs = np.zeros(1200, dtype=np.int32)
+            sr, dr, ds = i3([0, 0, 0]), i3([0, 0, 0]), i3([0, 0, 0])
+            n = R.ref_stencil_lib_ext(name.encode(), a, b, c, ptr(coefs), ptr(offs), ptr(sr), ptr(dr), ptr(ds), 400)
+            assert n > 0, (dim, name, a, b, c, n)
+            tab["%dD/%s/%d_%d_%d" % (dim, name, a, b, c)] = {
+                "coefs": [float(x).hex() for x in coefs[:n]],
+                "offsets": offs[:3 * n].tolist(),
+                "sr": sr.tolist(), "dr": dr.tolist(), "ds": ds.tolist(),
+            }
+    with open(os.path.join(HERE, "stencil_lib_ext.json"), "w") as f:
+        json.dump(tab, f, indent=0, sort_keys=True)
+    return tab
+
+
 def ref_apply(dim, coefs, offs, sr, dr, ds, src, slo, shi, dst, dlo, dhi, infer, replace, scale,
               blo=(0, 0, 0), bhi=(-1, -1, -1)):
     R = util.ref(dim)
@@ -354,10 +397,57 @@ def interp_bnd():
         f.write(bytes(buf))
 
 
+def _records(path, recs):
+    """Binary fixture read by the C++ API tests: int32 count, then per record
+    int32 name length, the name, int64 value count, float64 values."""
+    buf = bytearray(np.array([len(recs)], dtype=np.int32).tobytes())
+    for name, a in recs:
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        nb = name.encode()
+        buf += np.array([len(nb)], dtype=np.int32).tobytes() + nb
+        buf += np.array([a.size], dtype=np.int64).tobytes() + a.tobytes()
+    with open(path, "wb") as f:
+        f.write(bytes(buf))
+
+
+def boxop():
+    """Proto's BoxOp / LevelOp / LevelMultigridSolver on seeded inputs, for the
+    C++ API test that compiles Proto's own operator headers against this
+    library: the FAS solve of LevelMultigridSolver<BoxOp_Laplace> (32^3 in
+    16^3 boxes, 4 levels: right-hand side, residual history, solution),
+    LevelOp<BoxOp_TestLaplace> on a 16^3 periodic level in 8^3 boxes, and
+    BoxOp_TestFlux / _TestSource / _TestDiag on one 16^3 box."""
+    R3 = util.ref(3)
+    recs = []
+    n = 32
+    rhs = util.gaussian_rhs(n, 6, 2)
+    phi = np.zeros(n ** 3)
+    hist = np.zeros(32)
+    it = R3.ref_mg_fas_solve(n, 16, 4, ptr(rhs), ptr(phi), 20, ctypes.c_double(1e-10), ptr(hist))
+    recs += [("fas:rhs", rhs), ("fas:hist", hist[:it + 1]), ("fas:phi", phi)]
+    m = 16
+    lp = uniform(90, 0, m ** 3)
+    lo = np.zeros(m ** 3)
+    R3.ref_levelop_apply(m, 8, ptr(lp), ptr(lo))
+    recs += [("levelop:phi", lp), ("levelop:out", lo)]
+    src = uniform(91, 0, (m + 2) ** 3)
+    aux = uniform(91, 1, (m + 2) ** 3)
+    recs += [("boxop:src", src), ("boxop:aux", aux)]
+    for which in (0, 1, 2):
+        dst = np.zeros(m ** 3)
+        flux = np.zeros(3 * (m + 1) * m * m)
+        R3.ref_boxop_apply(which, m, ptr(src), ptr(aux), ptr(dst), ptr(flux))
+        recs.append(("boxop%d:dst" % which, dst))
+        if which == 0:
+            recs.append(("boxop0:flux", flux))
+    _records(os.path.join(HERE, "boxop.bin"), recs)
+
+
 if __name__ == "__main__":
     if util.ref(3) is None or util.ref(2) is None:
         sys.exit("build the reference checker first: make -C oracle ref")
     tab = stencil_table()
+    stencil_table_ext()
     save_cases(apply_cases(tab))
     reductions()
     interp()
@@ -368,4 +458,5 @@ if __name__ == "__main__":
     interp_bnd()
     config1()
     config2_small()
+    boxop()
     print("golden fixtures written to", HERE)

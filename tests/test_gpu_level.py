@@ -371,7 +371,10 @@ def test_fas_matches_proto():
     assert hashlib.sha256(phi.tobytes()).hexdigest() == meta["fas"]["phi_sha256"]
 
 
-@pytest.mark.parametrize("n,bs,levels", [(64, 32, 3), (72, 36, 3), (48, 16, 3), (32, 8, 4)])
+@pytest.mark.parametrize("n,bs,levels", [(64, 32, 3), (72, 36, 3), (48, 16, 3), (32, 8, 4),
+                                         # coarsest levels of 2^3 and 1^3 cells (Proto's default level count
+                                         # log2(N) + 1 reaches 1^3): periodic folds of +-2 on a 1-cell level
+                                         (16, 8, 4), (16, 8, 5), (32, 16, 6)])
 def test_fas_vs_oracle(n, bs, levels):
     say("fas vs oracle", n, bs, levels)
     force = util.gaussian_rhs(n, 5, 17)

@@ -224,3 +224,72 @@ def test_level_copy_to_regions():
     pa.call("pm_level_copy_to", A.h, vp(srcA_d), G1, A.h, vp(dstA), pa.i3([2] * 3), pa.PM_F64, 1, None)
     assert host(dstA).tobytes() == A.pack(glob, 2).tobytes()
     assert pa.lib().pm_level_copy_to(A.h, vp(srcA_d), G1, A.h, vp(srcA_d), G1, pa.PM_F64, 1, None) == -1
+
+
+def test_graph_scratch_survives_growth():
+    """A graph captured around a reduction keeps the stream scratch it
+    recorded: a later, larger eager call on the same stream grows the scratch
+    without freeing the captured buffer (it is retired until the stream is
+    destroyed), so the replay writes its partials into live memory and gives
+    the eager answer.  A buffer allocated after the growth is not touched by
+    the replay."""
+    from proto_amd.device import DeviceArray, Stream
+    s = Stream()
+    a = util.uniform(21, 0, 5000)
+    ad, out, ref = dev(a), zeros(1), zeros(1)
+    pa.call("pm_reduce", pa.RED_ABS, pa.PM_F64, vp(ad), a.size, vp(ref), 0, s.h)  # sizes the scratch eagerly
+    s.sync()
+    g = ctypes.c_void_p()
+    pa.call("pm_graph_begin", s.h)
+    pa.call("pm_reduce", pa.RED_ABS, pa.PM_F64, vp(ad), a.size, vp(out), 0, s.h)
+    pa.call("pm_graph_end", s.h, ctypes.byref(g))
+    nb, nret = ctypes.c_size_t(), ctypes.c_int()
+    pa.call("pm_stream_scratch_info", s.h, ctypes.byref(nb), ctypes.byref(nret))
+    say("scratch before growth", nb.value, nret.value)
+    assert nret.value == 0
+    small = nb.value
+    # 2048 x 2048 x 1 residual: 32 x 256 workgroups x 2 FP64 partials = 128 KiB of scratch
+    n = 2048
+    c7, o7, _, _, _ = util.lib_stencil("Laplacian")
+    st = pa.Stencil(c7, o7)
+    ub, rb = pa.Box.of([-1, -1, -1], [n, n, 1]), pa.Box.of([0, 0, 0], [n - 1, n - 1, 0])
+    u = zeros((n + 2) * (n + 2) * 3)
+    f, r, red = zeros(n * n), zeros(n * n), zeros(2)
+    pa.call("pm_stencil_residual", st.h, vp(u), ctypes.byref(ub), vp(f), vp(r), ctypes.byref(rb), 1.0, vp(red), s.h)
+    s.sync()
+    pa.call("pm_stream_scratch_info", s.h, ctypes.byref(nb), ctypes.byref(nret))
+    say("scratch after growth", nb.value, nret.value)
+    assert nb.value > small and nret.value == 1
+    filler = DeviceArray(small // 8)
+    pa.call("pm_fill_uniform", pa.PM_F64, vp(filler), filler.n, 4, 0, 0.0, 7.0, s.h)
+    try:
+        for rep in range(3):
+            pa.call("pm_memset_zero", vp(out), 8, s.h)
+            pa.call("pm_graph_launch", g, s.h)
+            s.sync()
+            assert host(out)[0] == host(ref)[0] == np.abs(a).max(), rep
+        assert (host(filler) == 7.0).all()
+    finally:
+        pa.call("pm_graph_destroy", g)
+
+
+def test_scratch_growth_under_capture_is_refused():
+    """A call that needs more stream scratch than exists while the stream is
+    capturing fails with PM_ERR_UNSUPPORTED before recording anything (no
+    drain or allocation inside a capture)."""
+    from proto_amd.device import Stream
+    s = Stream()
+    n = 2048
+    c7, o7, _, _, _ = util.lib_stencil("Laplacian")
+    st = pa.Stencil(c7, o7)
+    ub, rb = pa.Box.of([-1, -1, -1], [n, n, 1]), pa.Box.of([0, 0, 0], [n - 1, n - 1, 0])
+    u = zeros((n + 2) * (n + 2) * 3)
+    f, r, red = zeros(n * n), zeros(n * n), zeros(2)
+    g = ctypes.c_void_p()
+    pa.call("pm_graph_begin", s.h)
+    rc = pa.lib().pm_stencil_residual(st.h, vp(u), ctypes.byref(ub), vp(f), vp(r), ctypes.byref(rb), 1.0, vp(red),
+                                      s.h)
+    pa.call("pm_graph_end", s.h, ctypes.byref(g))
+    pa.call("pm_graph_destroy", g)
+    assert rc == -4
+    assert b"under graph capture" in pa.lib().pm_last_error()
