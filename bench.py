@@ -53,6 +53,16 @@ def parse():
     return ap.parse_args()
 
 
+def _proto_stencil(name):
+    """Coefficients and offsets of one of Proto's library stencils (3-D, FP64),
+    from the table Proto itself wrote (tests/golden/stencil_lib.json)."""
+    import numpy as np
+    with open(os.path.join(ROOT, "tests", "golden", "stencil_lib.json")) as fp:
+        e = json.load(fp)["3D/%s/0/f64" % name]
+    return (np.array([float.fromhex(c) for c in e["coefs"]]),
+            np.array(e["offsets"], dtype=np.int32).reshape(-1, 3))
+
+
 def stencils(dtype):
     import numpy as np
     import proto_amd as pa
@@ -141,6 +151,24 @@ def _cpu_slab_job(job):
     return _cpu_slab(*job)
 
 
+def _host_info():
+    """CPU model and core counts of this host (the box running the baseline)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fp:
+            for line in fp:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": usable, "cpu_count": os.cpu_count() or 1}
+
+
 def cpu_baseline(n, g, planes, workers=16, other_workloads=False):
     """Proto's CPU path on a z-slab of `planes` planes of the same config-2
     field, one thread; plus the whole field split into `workers` z-slabs run by
@@ -154,7 +182,10 @@ def cpu_baseline(n, g, planes, workers=16, other_workloads=False):
     out = {"value": round(nbytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
            "sample": "config-2 step (7- and 27-point r = f - L(u), absMax, sum) on %dx%dx%d cells, "
                      "%.2f s, Proto's CPU path single-threaded (Release flags)" % (n, n, planes, dt)}
-    workers = max(1, min(workers, os.cpu_count() or 1))
+    host = _host_info()
+    out["host"] = dict(host, all_cores_workers=min(workers, host["nproc"]),
+                       why="one GPU's share of the box's host cores is 16; nproc counts the whole host")
+    workers = max(1, min(workers, host["nproc"]))
     if workers > 1:
         nz = max(1, planes // 4)
         ctx = multiprocessing.get_context("spawn")  # fresh interpreters: no GPU state in the children
@@ -757,6 +788,21 @@ def main():
             gbs = apply_bytes / (ms * 1e-3) / 1e9
             extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
                            "bytes": apply_bytes}
+        # stencils outside the two fixed shapes: the generic z-march (taps from an
+        # LDS plane ring, any order) against the per-cell kernel it replaced, at
+        # the same 512^3 box: Laplacian_19, and the 7-point Laplacian served by
+        # each family
+        gen = {}
+        for sname, kinds in (("Laplacian_19", (1, 0)), ("Laplacian", (7, 1, 0))):
+            c_, o_ = _proto_stencil(sname)
+            for kind in kinds:
+                stg = pa.Stencil(c_, o_).set_kind(kind)
+                ms = time_launches(lambda: pa.call("pm_stencil_apply", stg.h, u.ptr, ctypes.byref(ub), dst.ptr,
+                                                   ctypes.byref(rb), ctypes.byref(rb), 1, 1, sc, sp), 5, stream)
+                gbs = apply_bytes / (ms * 1e-3) / 1e9
+                gen["%s_kind%d" % (sname, kind)] = {"ms": round(ms, 4), "GB/s": round(gbs, 1),
+                                                    "frac": round(gbs / PEAK_HBM_GBS, 4)}
+        extra["generic_apply_f64"] = gen
         for name, st in (("residual_L7_f64", s7), ("residual_L27_f64", s27)):
             ms = time_launches(lambda: pa.call("pm_stencil_residual", st.h, u.ptr, ctypes.byref(ub), f.ptr,
                                                dst.ptr, ctypes.byref(rb), sc, None, sp), 10, stream)
@@ -801,6 +847,20 @@ def main():
             gbs = moved / (ms * 1e-3) / 1e9
             extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
                            "bytes": moved}
+        # the 1:1 read:write mix of the stencil (2 GiB read, 2 GiB written) by three
+        # copy forms and four grid sizes; the best is the achievable ceiling the
+        # 7-point apply is read against
+        sweep = {}
+        for kind, kname in ((2, "gridstride_nt"), (3, "chunk64k_nt"), (4, "chunk64k")):
+            for bpc in (2, 4, 8, 16):
+                ms = time_launches(lambda: pa.call("pm_stream_probe", kind, big2.ptr, big.ptr, nb4 // 2, bpc, sp),
+                                   10, stream)
+                sweep["%s_bpc%d" % (kname, bpc)] = round(nb4 / (ms * 1e-3) / 1e9, 1)
+        best = max(sweep, key=sweep.get)
+        l7 = extra["apply_L7_f64"]["GB/s"]
+        extra["copy_sweep"] = {"GB/s": sweep, "best": best, "best_GB/s": sweep[best],
+                               "best_frac": round(sweep[best] / PEAK_HBM_GBS, 4),
+                               "apply_L7_f64_vs_best_copy": round(l7 / sweep[best], 4)}
         big.free()
         big2.free()
         a.free()

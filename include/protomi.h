@@ -72,9 +72,10 @@ int pm_copy_device(void* dst, const void* src, size_t bytes, int nontemporal, pm
 /* Streaming-bandwidth probe (no counterpart in Proto; measures the ceilings the
  * kernels' roofline fractions are read against).  kind 0 reads `src` (dst may be
  * null or any 16-byte-aligned device word), 1 writes `dst` with non-temporal
- * stores, 2 copies src -> dst; 16-byte accesses over 256 * blocks_per_cu
- * 256-thread workgroups (blocks_per_cu in [1, 64]).  bytes and the pointers
- * must be multiples of 16. */
+ * stores, 2 copies src -> dst (grid-stride, non-temporal stores), 3 / 4 copy
+ * src -> dst in 64 KB chunks per workgroup (non-temporal loads and stores /
+ * plain); 16-byte accesses over 256 * blocks_per_cu 256-thread workgroups
+ * (blocks_per_cu in [1, 64]).  bytes and the pointers must be multiples of 16. */
 int pm_stream_probe(int kind, void* dst, const void* src, size_t bytes, int blocks_per_cu, pm_stream_t stream);
 int pm_stream_create(pm_stream_t* stream);
 int pm_stream_destroy(pm_stream_t stream);
@@ -104,9 +105,17 @@ int pm_stencil_create(pm_stencil** out, int dtype, int dim, int ncoef, const voi
                       const int* offsets_host, const int src_ratio[3], const int dst_ratio[3],
                       const int dst_shift[3]);
 int pm_stencil_destroy(pm_stencil* st);
-/* Which kernel family serves this stencil: 0 generic, 7 the 7-point z-march,
- * 27 the 27-point z-march, 5 the 2-D 5-point tile (for tests and tools). */
+/* Which kernel family serves this stencil (for tests and tools): 7 / 27 the
+ * 7- / 27-point z-march (Proto's Laplacian / Laplacian_27 in Proto's order,
+ * 3-D), 1 the generic z-march (any other unit-ratio stencil whose offsets lie
+ * in [-1,1]^DIM, 2-D included: taps read from an LDS plane ring in the
+ * stencil's order), 0 the per-cell kernel (refinement ratios, shifts, wider
+ * offsets). */
 int pm_stencil_kind(const pm_stencil* st, int* kind);
+/* Serve this stencil with another kernel family that can (tests and
+ * benchmarks compare the families): 0 (any stencil), 1 (a 7/27-point one too)
+ * or its own kind.  Every family gives the same bits. */
+int pm_stencil_set_kind(pm_stencil* st, int kind);
 
 /* Stencil::apply -> protoApply<DEVICE> -> sIndexer3::gpu
  * (Proto_StencilImplem.H:412-431, 844-895, 565-614 and hostApply 725-828):
@@ -222,9 +231,6 @@ int pm_level_exchange(const pm_layout* layout, int dtype, void* data, const int 
 int pm_level_gsrb(const pm_layout* layout, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
                   const int rhs_ghost[3], double dx, int color, pm_stream_t stream);
 
-/* Damped point Jacobi of examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:99-123:
- * tmp = L(phi, 1/(4 DIM)) + (-(dx*dx)/(4 DIM)) rhs on all valid cells, then
- * phi += tmp.  tmp: level data without ghosts. */
 /* One full GSRB sweep (red then black, LaplaceOp.H:95-123) in one pass and
  * out of place, with no exchange: phi_out (valid cells) = swept phi_in; cells
  * of other boxes are read from their owners.  Bitwise the exchange / red /
@@ -233,6 +239,9 @@ int pm_level_gsrb(const pm_layout* layout, int dtype, void* phi, const int phi_g
 int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx,
                         pm_stream_t stream);
+/* Damped point Jacobi of examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:99-123:
+ * tmp = L(phi, 1/(4 DIM)) + (-(dx*dx)/(4 DIM)) rhs on all valid cells, then
+ * phi += tmp.  tmp: level data without ghosts. */
 int pm_level_jacobi(const pm_layout* layout, int dtype, void* phi, const int phi_ghost[3], const void* rhs,
                     const int rhs_ghost[3], void* tmp, double dx, pm_stream_t stream);
 
@@ -459,7 +468,9 @@ int pm_fas_solve(pm_fas* fas, void* state, const void* force, int max_iter, doub
  * (same FP32 coefficients, same order), so a result does not depend on which
  * other requests were in flight.
  * Mode 1 (plain FP64): one request at a time through pm_mg precision 0
- * (Proto's GSRB V-cycle), the reference point for the batched mode.
+ * (Proto's GSRB V-cycle), the reference point for the batched mode; one
+ * solver per size stays resident within the HBM budget (solvers of other
+ * sizes are evicted first; a size that cannot fit alone is rejected).
  * Levels: n is halved while it is even and larger than 8 (32 -> 8, 48 -> 6). */
 typedef struct pm_server pm_server;
 typedef struct pm_server_options {
@@ -502,6 +513,31 @@ int pm_server_result(pm_server* srv, int index, double* phi_host);
 int pm_server_rhs(int n, int ngauss, uint64_t seed, double sigma, double* rhs_dev, pm_stream_t stream);
 /* Device bytes resident in the server's pools. */
 int pm_server_resident_bytes(const pm_server* srv, size_t* bytes);
+
+/* Checkpoint / resume of a served run (mode 0).  No counterpart in the
+ * reference; the solution blocks use LevelBoxData::linearOut's layout
+ * (Proto_LevelBoxDataImplem.H:412-507): each in-flight request's FP64 solution
+ * as the linearOut of its one-box periodic level (n^3 values, x fastest),
+ * readable by LevelBoxData<double,1>::linearIn on that layout.
+ *
+ * pm_server_set_stop: pm_server_run stops after `iterations` loop iterations
+ * (0 = never) and returns PM_ERR_CANCELLED, holding the run: in-flight
+ * hierarchies, the waiting queue, the statistics and kept results so far.
+ * The next pm_server_run with the same request list continues it (results
+ * and iteration counts as if it had not stopped); pm_server_discard drops it.
+ * pm_server_checkpoint writes a held run into a host buffer of
+ * pm_server_checkpoint_size bytes (it synchronises `stream`);
+ * pm_server_restore loads one into a server with the same sweep counts,
+ * max_iter and sigma (another process, another GPU), allocating the in-flight
+ * hierarchies within its HBM budget and the resident pool the run had, after
+ * which pm_server_run with the same request list resumes it.  A resumed
+ * request recomputes the FP64 residual of its saved solution, then continues
+ * its V-cycles: same bits and the same cycle count as an uninterrupted run. */
+int pm_server_set_stop(pm_server* srv, int iterations);
+int pm_server_discard(pm_server* srv);
+int pm_server_checkpoint_size(const pm_server* srv, size_t* bytes);
+int pm_server_checkpoint(pm_server* srv, void* buffer_host, size_t bytes, pm_stream_t stream);
+int pm_server_restore(pm_server* srv, const void* buffer_host, size_t bytes, pm_stream_t stream);
 
 /* ----------------------------------------------------------------- graphs */
 /* hipGraph capture of work issued on a (non-null) stream by any entry point

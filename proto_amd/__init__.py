@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libprotomi.so")
 
 PM_F32, PM_F64 = 0, 1
+PM_ERR_CANCELLED = -6
 RED_MAX, RED_MIN, RED_ABS, RED_SUM, RED_SUMABS, RED_SUMSQ = range(6)
 
 # Every symbol declared in include/protomi.h with its ctypes signature.
@@ -111,6 +112,7 @@ SIGNATURES = {
     "pm_stencil_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _ip, _ip, _ip, _ip]),
     "pm_stencil_destroy": (_i, [_vp]),
     "pm_stencil_kind": (_i, [_vp, _ip]),
+    "pm_stencil_set_kind": (_i, [_vp, _i]),
     "pm_stencil_apply": (_i, [_vp, _vp, _BP, _vp, _BP, _BP, _i, _i, _d, _vp]),
     "pm_stencil_residual": (_i, [_vp, _vp, _BP, _vp, _vp, _BP, _d, _vp, _vp]),
     "pm_stencil_residual2": (_i, [_vp, _vp, _vp, _BP, _vp, _vp, _vp, _BP, _d, _d, _vp, _vp]),
@@ -169,6 +171,11 @@ SIGNATURES = {
     "pm_server_run": (_i, [_vp, _i, ctypes.POINTER(Request), ctypes.POINTER(RequestStats), _vp]),
     "pm_server_result": (_i, [_vp, _i, _vp]),
     "pm_server_rhs": (_i, [_i, _i, _u64, _d, _vp, _vp]),
+    "pm_server_set_stop": (_i, [_vp, _i]),
+    "pm_server_discard": (_i, [_vp]),
+    "pm_server_checkpoint_size": (_i, [_vp, ctypes.POINTER(_sz)]),
+    "pm_server_checkpoint": (_i, [_vp, _vp, _sz, _vp]),
+    "pm_server_restore": (_i, [_vp, _vp, _sz, _vp]),
     "pm_level_gsrb_sweep": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _vp, _ip, _d, _vp]),
     "pm_fas_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _d]),
     "pm_fas_destroy": (_i, [_vp]),
@@ -257,6 +264,10 @@ class Stencil:
         call("pm_stencil_kind", self.h, ctypes.byref(k))
         return k.value
 
+    def set_kind(self, kind):
+        call("pm_stencil_set_kind", self.h, int(kind))
+        return self
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.pm_stencil_destroy(self.h)
@@ -325,6 +336,15 @@ class Batch:
             self.h = None
 
 
+class RunStopped(Exception):
+    """pm_server_run stopped at its set iteration count (PM_ERR_CANCELLED);
+    .stats holds the statistics so far and the server holds the run."""
+
+    def __init__(self, stats):
+        super().__init__("server run stopped and held")
+        self.stats = stats
+
+
 class Server:
     """Continuous-batching Poisson solve server (pm_server).  mode 0: batched
     FP32-preconditioned FP64 refinement; mode 1: plain FP64, one at a time."""
@@ -357,8 +377,32 @@ class Server:
             arr[i] = Request(float(r.get("arrival_s", 0.0)), int(r["n"]), int(r["ngauss"]), int(r["seed"]),
                              float(r.get("tol", 1e-8)), int(r.get("keep_result", 0)))
         st = (RequestStats * len(requests))()
-        call("pm_server_run", self.h, len(requests), arr, st, stream)
-        return [{k: getattr(x, k) for k, _ in RequestStats._fields_} for x in st]
+        rc = lib().pm_server_run(self.h, len(requests), arr, st, stream)
+        if rc != PM_ERR_CANCELLED:
+            check(rc, "pm_server_run")
+        out = [{k: getattr(x, k) for k, _ in RequestStats._fields_} for x in st]
+        if rc == PM_ERR_CANCELLED:
+            raise RunStopped(out)
+        return out
+
+    def set_stop(self, iterations):
+        """Stop the next run after `iterations` loop iterations (0: never)."""
+        call("pm_server_set_stop", self.h, int(iterations))
+
+    def discard(self):
+        call("pm_server_discard", self.h)
+
+    def checkpoint(self, stream=None):
+        """The held run as bytes (pm_server_checkpoint)."""
+        b = ctypes.c_size_t()
+        call("pm_server_checkpoint_size", self.h, ctypes.byref(b))
+        buf = ctypes.create_string_buffer(b.value)
+        call("pm_server_checkpoint", self.h, buf, b.value, stream)
+        return buf.raw
+
+    def restore(self, data, stream=None):
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        call("pm_server_restore", self.h, buf, len(data), stream)
 
     def result(self, index, n):
         import numpy as np

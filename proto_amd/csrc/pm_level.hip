@@ -1372,7 +1372,10 @@ int pm_level_exchange(const pm_layout* L, int dtype, void* data, const int ghost
     const G3 g = g3(ghost);
     if (g.g[0] == 0 && g.g[1] == 0 && g.g[2] == 0) { return PM_OK; }
     for (int k = 0; k < 3; k++) {
-        PM_REQUIRE(g.g[k] >= 0 && g.g[k] <= L->bs[k], "pm_level_exchange: ghost wider than a box");
+        // a ghost layer wider than a box takes images of boxes further out (the
+        // owner lookup folds any periodic distance), as Proto's NeighborIterator
+        // over bx0.grow(ghost) does (Proto_LevelBoxDataImplem.H:905-925)
+        PM_REQUIRE(g.g[k] >= 0, "pm_level_exchange: negative ghost");
         PM_REQUIRE(k < L->dim || g.g[k] == 0, "pm_level_exchange: ghost beyond dim");
     }
     const LayoutDev D = ldev(L);

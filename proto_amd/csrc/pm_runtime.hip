@@ -183,6 +183,34 @@ __global__ __launch_bounds__(256) void k_copy16(const u32x4* __restrict__ src, u
     }
 }
 
+// Block-contiguous copy: workgroup b copies 64 KB chunks b, b + grid, ...;
+// each thread has 16 sixteen-byte loads in flight, then 16 stores.  NT: the
+// loads and stores are non-temporal.  A probe of the 1:1 read:write mix with
+// DRAM-page-friendly streams, beside the grid-stride copy.
+template<int NT>
+__global__ __launch_bounds__(256) void k_copy_chunk(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n)
+{
+    constexpr int CH = 4096, U = CH / 256;
+    const size_t nch = n / CH;
+    for (size_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const size_t base = c * CH + threadIdx.x;
+        u32x4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) { v[u] = NT ? __builtin_nontemporal_load(&src[base + 256 * u]) : src[base + 256 * u]; }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (NT) {
+                __builtin_nontemporal_store(v[u], &dst[base + 256 * u]);
+            } else {
+                dst[base + 256 * u] = v[u];
+            }
+        }
+    }
+    for (size_t i = nch * CH + (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        dst[i] = src[i];
+    }
+}
+
 }  // namespace pm
 
 using namespace pm;
@@ -296,7 +324,8 @@ __global__ __launch_bounds__(256) void k_probe_write(u32x4* __restrict__ dst, si
 
 int pm_stream_probe(int kind, void* dst, const void* src, size_t bytes, int blocks_per_cu, pm_stream_t stream)
 {
-    PM_REQUIRE(kind >= 0 && kind <= 2, "pm_stream_probe: kind must be 0 (read), 1 (write) or 2 (copy)");
+    PM_REQUIRE(kind >= 0 && kind <= 4,
+               "pm_stream_probe: kind must be 0 (read), 1 (write), 2 (copy), 3 / 4 (chunked copy, NT / plain)");
     PM_REQUIRE(blocks_per_cu >= 1 && blocks_per_cu <= 64, "pm_stream_probe: blocks_per_cu outside [1, 64]");
     PM_REQUIRE(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
                "pm_stream_probe: bytes and pointers must be multiples of 16");
@@ -309,8 +338,16 @@ int pm_stream_probe(int kind, void* dst, const void* src, size_t bytes, int bloc
         hipLaunchKernelGGL(k_probe_read, dim3(blocks), dim3(256), 0, S(stream), (const u32x4*)src, n, (unsigned*)dst);
     } else if (kind == 1) {
         hipLaunchKernelGGL(k_probe_write, dim3(blocks), dim3(256), 0, S(stream), (u32x4*)dst, n);
-    } else {
+    } else if (kind == 2) {
         hipLaunchKernelGGL(k_copy16<1>, dim3(blocks), dim3(256), 0, S(stream), (const u32x4*)src, (u32x4*)dst, n);
+    } else {
+        // chunked: one 64 KB chunk per workgroup per pass
+        const unsigned cb = (unsigned)std::max<size_t>(1, std::min<size_t>(n / 4096, (size_t)256 * blocks_per_cu));
+        if (kind == 3) {
+            hipLaunchKernelGGL(k_copy_chunk<1>, dim3(cb), dim3(256), 0, S(stream), (const u32x4*)src, (u32x4*)dst, n);
+        } else {
+            hipLaunchKernelGGL(k_copy_chunk<0>, dim3(cb), dim3(256), 0, S(stream), (const u32x4*)src, (u32x4*)dst, n);
+        }
     }
     PM_CHECK_LAUNCH();
     return PM_OK;

@@ -568,7 +568,24 @@ struct Active {
     std::vector<int> ns;
     double res0 = 0, r = 0;
     int iters = 0;
-    bool fresh = true;
+    bool fresh = true;     // its first iteration only computes the residual
+    bool restored = false; // resumed from a checkpoint: that residual is the one it had, not res0
+};
+
+// A mode-0 run between two calls of pm_server_run: stopped after the set
+// number of loop iterations, or restored from a checkpoint.  The next
+// pm_server_run with the same requests continues it.
+struct HeldRun {
+    bool held = false;
+    bool restored = false;     // from a checkpoint: right-hand sides still to be made
+    int nreq = 0;
+    uint64_t fingerprint = 0;  // of the request list
+    double elapsed = 0;        // seconds of the run before the stop
+    size_t next = 0;           // position in arrival order of the next arrival
+    int done = 0;
+    std::vector<int> waiting;
+    std::vector<Active> act;
+    std::vector<pm_request_stats> stats;
 };
 
 struct Launch {
@@ -597,6 +614,9 @@ struct pm_server {
     std::vector<Launch> plan;
     // kept results of the last run
     std::map<int, std::vector<double>> results;
+    // mode 0: stop pm_server_run after this many loop iterations (0: never)
+    int stop_after = 0;
+    HeldRun run;
     // mode 1: one pm_mg (FP64) per size, with its phi / rhs
     struct Plain {
         pm_layout* layout = nullptr;
@@ -962,78 +982,149 @@ int take_hier(pm_server* S, int n, Hier** h)
     return rc;
 }
 
+// FNV-1a over the request fields that decide a run; a held run continues only
+// with the list it was started with.
+uint64_t fingerprint(int nreq, const pm_request* reqs)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t b) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < b; i++) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    };
+    mix(&nreq, sizeof(nreq));
+    for (int i = 0; i < nreq; i++) {
+        const pm_request& q = reqs[i];
+        mix(&q.arrival_s, sizeof(q.arrival_s));
+        mix(&q.n, sizeof(q.n));
+        mix(&q.ngauss, sizeof(q.ngauss));
+        mix(&q.seed, sizeof(q.seed));
+        mix(&q.tol, sizeof(q.tol));
+        mix(&q.keep_result, sizeof(q.keep_result));
+    }
+    return h;
+}
+
+// Hierarchies of in-flight requests back to the pool (a failed run, a
+// discarded hold): pm_server_destroy then frees them.
+void release_active(pm_server* S, std::vector<Active>& act)
+{
+    for (Active& a : act) {
+        if (a.h) { S->pool[a.h->n].push_back(a.h); }
+        a.h = nullptr;
+    }
+    act.clear();
+}
+
 int run_batched(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* st, hipStream_t s)
 {
     std::vector<int> order(nreq);
     for (int i = 0; i < nreq; i++) { order[i] = i; }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
-    size_t next = 0;
-    std::vector<int> waiting;
-    std::vector<Active> act;
-    int done = 0;
+    const uint64_t fp = fingerprint(nreq, reqs);
+    HeldRun R;
+    if (S->run.held) {
+        PM_REQUIRE(S->run.nreq == nreq && S->run.fingerprint == fp,
+                   "pm_server_run: the held run (stopped or restored) belongs to another request list; pass the same "
+                   "list or drop it with pm_server_discard");
+        R = std::move(S->run);
+        S->run = HeldRun();
+        std::copy(R.stats.begin(), R.stats.end(), st);
+    }
+    // on error the in-flight hierarchies go back to the pool
+    auto fail = [&](int rc) {
+        release_active(S, R.act);
+        return rc;
+    };
+    // a restored run: the right-hand sides are regenerated from the requests
+    for (Active& a : R.act) {
+        if (!R.restored) { break; }
+        if (reqs[a.idx].n != a.ns[0]) {
+            set_error("pm_server_run: request %d does not match its checkpoint", a.idx);
+            return fail(PM_ERR_INVALID);
+        }
+        const int rc = make_rhs(a.h->rhs64, a.h->tab, reqs[a.idx].n, reqs[a.idx], S->opt.sigma, s);
+        if (rc) { return fail(rc); }
+    }
+    R.restored = false;
     bool dirty = true;
-    const auto t0 = std::chrono::steady_clock::now();
-    while (done < nreq) {
+    int iterations = 0;
+    const auto t0 = std::chrono::steady_clock::now() -
+                    std::chrono::duration_cast<std::chrono::steady_clock::duration>(std::chrono::duration<double>(R.elapsed));
+    while (R.done < nreq) {
         const double t = now_s(t0);
-        while (next < order.size() && reqs[order[next]].arrival_s <= t) {
-            const int i = order[next++];
+        while (R.next < order.size() && reqs[order[R.next]].arrival_s <= t) {
+            const int i = order[R.next++];
             if (hier_bytes(reqs[i].n) > S->opt.hbm_budget) {
                 st[i].status = PM_SERVE_REJECTED;
                 st[i].admit_s = st[i].finish_s = t;
-                done++;
+                R.done++;
             } else {
-                waiting.push_back(i);
+                R.waiting.push_back(i);
             }
         }
         // admission, first come first served
-        while (!waiting.empty() && (int)act.size() < S->opt.max_active) {
-            const int i = waiting.front();
+        while (!R.waiting.empty() && (int)R.act.size() < S->opt.max_active) {
+            const int i = R.waiting.front();
             Hier* h = nullptr;
             int rc = take_hier(S, reqs[i].n, &h);
-            if (rc) { return rc; }
+            if (rc) { return fail(rc); }
             if (!h) { break; }
-            waiting.erase(waiting.begin());
+            R.waiting.erase(R.waiting.begin());
             Active a;
             a.idx = i;
             a.h = h;
             a.ns = level_sizes(reqs[i].n);
+            R.act.push_back(a);
             const long total = (long)reqs[i].n * reqs[i].n * reqs[i].n;
             rc = make_rhs(h->rhs64, h->tab, reqs[i].n, reqs[i], S->opt.sigma, s);
-            if (rc) { return rc; }
+            if (rc) { return fail(rc); }
             const unsigned g = (unsigned)std::min<long>((total + 255) / 256, 4096);
             hipLaunchKernelGGL(k_srv_zero_f64, dim3(g), dim3(256), 0, s, h->phi64, total);
-            PM_CHECK_LAUNCH();
+            if (hipGetLastError() != hipSuccess) {
+                set_error("server: zero fill launch failed");
+                return fail(PM_ERR_HIP);
+            }
             st[i].admit_s = now_s(t0);
             st[i].bytes = h->bytes;
-            act.push_back(a);
             dirty = true;
         }
-        if (act.empty()) {
-            if (next < order.size()) {
-                const double wait = reqs[order[next]].arrival_s - now_s(t0);
+        if (R.act.empty()) {
+            if (R.next < order.size()) {
+                const double wait = reqs[order[R.next]].arrival_s - now_s(t0);
                 if (wait > 0) { std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.001))); }
             }
             continue;
         }
         if (dirty) {
-            int rc = build_plan(S, act, s);
-            if (rc) { return rc; }
+            int rc = build_plan(S, R.act, s);
+            if (rc) { return fail(rc); }
             dirty = false;
         }
-        int rc = run_plan(S, (int)act.size(), s);
-        if (rc) { return rc; }
+        int rc = run_plan(S, (int)R.act.size(), s);
+        if (rc) { return fail(rc); }
         const double tf = now_s(t0);
         std::vector<Active> keep;
-        for (size_t r = 0; r < act.size(); r++) {
-            Active& a = act[r];
+        for (size_t r = 0; r < R.act.size(); r++) {
+            Active& a = R.act[r];
             double v;
             std::memcpy(&v, &S->h_norms[r], sizeof(double));
             const pm_request& q = reqs[a.idx];
             bool fin = false;
             if (a.fresh) {
                 a.fresh = false;
-                a.res0 = a.r = v;
-                fin = v == 0.0;
+                if (a.restored) {
+                    // the residual of the checkpointed solution: its norm is the
+                    // one recorded at the checkpoint, and no cycle is counted
+                    a.restored = false;
+                    a.r = v;
+                    fin = v < q.tol * a.res0 || a.iters >= S->opt.max_iter;
+                } else {
+                    a.res0 = a.r = v;
+                    fin = v == 0.0;
+                }
                 dirty = true;  // its V-cycles start next iteration
             } else {
                 a.iters++;
@@ -1052,18 +1143,47 @@ int run_batched(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats
             if (q.keep_result && o.status == PM_SERVE_SOLVED) {
                 std::vector<double>& out = S->results[a.idx];
                 out.resize((size_t)q.n * q.n * q.n);
-                PM_CHECK_HIP(hipMemcpy(out.data(), a.h->phi64, sizeof(double) * out.size(), hipMemcpyDeviceToHost));
+                if (hipMemcpy(out.data(), a.h->phi64, sizeof(double) * out.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+                    set_error("server: result read-back failed");
+                    for (size_t k = r; k < R.act.size(); k++) { keep.push_back(R.act[k]); }
+                    R.act.swap(keep);
+                    return fail(PM_ERR_HIP);
+                }
             }
             S->pool[a.h->n].push_back(a.h);
-            done++;
+            R.done++;
             dirty = true;
         }
-        act.swap(keep);
+        R.act.swap(keep);
+        if (S->stop_after > 0 && ++iterations >= S->stop_after && R.done < nreq) {
+            R.held = true;
+            R.nreq = nreq;
+            R.fingerprint = fp;
+            R.elapsed = now_s(t0);
+            R.stats.assign(st, st + nreq);
+            S->run = std::move(R);
+            set_error("pm_server_run: stopped after %d loop iterations with %d of %d requests done; the run is held "
+                      "for pm_server_checkpoint and the next pm_server_run",
+                      iterations, S->run.done, nreq);
+            return PM_ERR_CANCELLED;
+        }
     }
     return PM_OK;
 }
 
-// Mode 1: one request at a time, FP64 GSRB V-cycles of the multigrid driver.
+void free_plain(pm_server::Plain& P)
+{
+    pm_mg_destroy(P.mg);
+    pm_layout_destroy(P.layout);
+    hipFree(P.phi);
+    hipFree(P.rhs);
+    hipFree(P.tab);
+    P = pm_server::Plain();
+}
+
+// Mode 1: one request at a time, FP64 GSRB V-cycles of the multigrid driver,
+// within the HBM budget (solvers of other sizes are evicted; a size that does
+// not fit alone is rejected).
 int run_plain(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* st, hipStream_t s)
 {
     std::vector<int> order(nreq);
@@ -1093,14 +1213,32 @@ int run_plain(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* 
             o.max_iter = S->opt.max_iter;
             o.tol = q.tol;
             rc = pm_mg_create(&P.mg, P.layout, nl, 1.0 / n, &o);
-            if (rc) { return rc; }
+            if (rc) {
+                free_plain(P);
+                return rc;
+            }
             const size_t n3 = (size_t)n * n * n, ng = (size_t)(n + 2) * (n + 2) * (n + 2);
-            PM_CHECK_HIP(hipMalloc(&P.phi, 8 * ng));
-            PM_CHECK_HIP(hipMalloc(&P.rhs, 8 * n3));
-            PM_CHECK_HIP(hipMalloc(&P.tab, 8 * tab_elems(n)));
+            if (hipMalloc(&P.phi, 8 * ng) != hipSuccess || hipMalloc(&P.rhs, 8 * n3) != hipSuccess ||
+                hipMalloc(&P.tab, 8 * tab_elems(n)) != hipSuccess) {
+                free_plain(P);
+                set_error("server: plain-mode allocation for n = %d failed", n);
+                return PM_ERR_NOMEM;
+            }
             size_t mb = 0;
             pm_mg_bytes(P.mg, &mb);
             P.bytes = mb + 8 * (ng + n3 + tab_elems(n));
+            // the HBM budget: evict the solvers of other sizes, else reject
+            for (auto e = S->plain.begin(); e != S->plain.end() && S->resident + P.bytes > S->opt.hbm_budget;) {
+                S->resident -= e->second.bytes;
+                free_plain(e->second);
+                e = S->plain.erase(e);
+            }
+            if (S->resident + P.bytes > S->opt.hbm_budget) {
+                free_plain(P);
+                st[i].status = PM_SERVE_REJECTED;
+                st[i].admit_s = st[i].finish_s = now_s(t0);
+                continue;
+            }
             S->resident += P.bytes;
             it = S->plain.emplace(n, P).first;
         }
@@ -1135,6 +1273,248 @@ int run_plain(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* 
                     }
         }
     }
+    return PM_OK;
+}
+
+// ---------------------------------------------------------------- checkpoint
+// Buffer of a held mode-0 run (host memory, native byte order):
+//   CkHead | pm_request_stats[nreq] | int32 waiting[] | CkActive[] | CkResult[]
+//   | CkPool[] | (8-byte aligned) solution blocks of the active requests, then
+//   the kept results.
+// Each solution block is LevelBoxData<double, 1>::linearOut of the request's
+// one-box periodic level without ghosts (Proto_LevelBoxDataImplem.H:465-477:
+// every patch in layout order, C * |box| values x fastest): n^3 values, which
+// a Proto program reads back with linearIn into a LevelBoxData on the same
+// layout.  The FP32 hierarchy is not saved: each V-cycle starts it from zero,
+// and the right-hand side is regenerated from the request.
+constexpr char kCkMagic[8] = {'P', 'M', 'S', 'R', 'V', 'C', 'K', '1'};
+constexpr uint32_t kCkVersion = 1;
+
+struct CkHead {
+    char magic[8];
+    uint32_t version, stats_bytes;
+    int32_t nreq, done, nwaiting, nactive, nresults, npool;
+    uint64_t fingerprint, next;
+    double elapsed;
+    int32_t pre, post, bottom, max_iter;
+    double sigma;
+};
+struct CkActive {
+    int32_t idx, n, iters, fresh;
+    double res0, r;
+};
+struct CkResult {
+    int32_t idx, n;
+};
+struct CkPool {
+    int32_t n, count;
+};
+
+struct CkLayout {
+    size_t stats, waiting, active, results, pool, data, total;
+    std::vector<CkPool> pools;
+};
+
+size_t align8(size_t b) { return (b + 7) & ~(size_t)7; }
+
+CkLayout ck_layout(const pm_server* S)
+{
+    const HeldRun& R = S->run;
+    CkLayout L;
+    for (const auto& kv : S->pool) {
+        if (!kv.second.empty()) { L.pools.push_back(CkPool{kv.first, (int32_t)kv.second.size()}); }
+    }
+    L.stats = sizeof(CkHead);
+    L.waiting = L.stats + sizeof(pm_request_stats) * (size_t)R.nreq;
+    L.active = L.waiting + sizeof(int32_t) * R.waiting.size();
+    L.results = L.active + sizeof(CkActive) * R.act.size();
+    L.pool = L.results + sizeof(CkResult) * S->results.size();
+    L.data = align8(L.pool + sizeof(CkPool) * L.pools.size());
+    size_t b = L.data;
+    for (const Active& a : R.act) { b += 8 * (size_t)a.ns[0] * a.ns[0] * a.ns[0]; }
+    for (const auto& kv : S->results) { b += 8 * kv.second.size(); }
+    L.total = b;
+    return L;
+}
+
+// Proto's linearOut / linearIn of a one-box level of n^3 cells without ghosts
+int level_linear(int n, double* dev, char* host, bool out, hipStream_t s)
+{
+    pm_layout* lay = nullptr;
+    int dom[3] = {n, n, n}, per[3] = {1, 1, 1}, lows[3] = {0, 0, 0}, g0[3] = {0, 0, 0};
+    int rc = pm_layout_create(&lay, 3, dom, dom, per, 1, lows);
+    if (rc) { return rc; }
+    rc = out ? pm_level_linear_out(lay, PM_F64, dev, g0, 1, host, 1, (pm_stream_t)s)
+             : pm_level_linear_in(lay, PM_F64, dev, g0, 1, host, 1, (pm_stream_t)s);
+    pm_layout_destroy(lay);
+    return rc;
+}
+
+int ck_write(pm_server* S, char* buf, const CkLayout& L, hipStream_t s)
+{
+    const HeldRun& R = S->run;
+    CkHead h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kCkMagic, sizeof(kCkMagic));
+    h.version = kCkVersion;
+    h.stats_bytes = (uint32_t)sizeof(pm_request_stats);
+    h.nreq = R.nreq;
+    h.done = R.done;
+    h.nwaiting = (int32_t)R.waiting.size();
+    h.nactive = (int32_t)R.act.size();
+    h.nresults = (int32_t)S->results.size();
+    h.npool = (int32_t)L.pools.size();
+    h.fingerprint = R.fingerprint;
+    h.next = R.next;
+    h.elapsed = R.elapsed;
+    h.pre = S->opt.pre;
+    h.post = S->opt.post;
+    h.bottom = S->opt.bottom;
+    h.max_iter = S->opt.max_iter;
+    h.sigma = S->opt.sigma;
+    std::memcpy(buf, &h, sizeof(h));
+    std::memcpy(buf + L.stats, R.stats.data(), sizeof(pm_request_stats) * R.stats.size());
+    int32_t* w = (int32_t*)(buf + L.waiting);
+    for (size_t k = 0; k < R.waiting.size(); k++) { w[k] = R.waiting[k]; }
+    CkActive* ca = (CkActive*)(buf + L.active);
+    size_t off = L.data;
+    for (size_t k = 0; k < R.act.size(); k++) {
+        const Active& a = R.act[k];
+        ca[k] = CkActive{a.idx, a.ns[0], a.iters, a.fresh ? 1 : 0, a.res0, a.r};
+        const int rc = level_linear(a.ns[0], a.h->phi64, buf + off, true, s);
+        if (rc) { return rc; }
+        off += 8 * (size_t)a.ns[0] * a.ns[0] * a.ns[0];
+    }
+    CkResult* cr = (CkResult*)(buf + L.results);
+    size_t k = 0;
+    for (const auto& kv : S->results) {
+        int n = 1;
+        while ((size_t)n * n * n < kv.second.size()) { n++; }
+        cr[k++] = CkResult{kv.first, n};
+        std::memcpy(buf + off, kv.second.data(), 8 * kv.second.size());
+        off += 8 * kv.second.size();
+    }
+    std::memcpy(buf + L.pool, L.pools.data(), sizeof(CkPool) * L.pools.size());
+    return PM_OK;
+}
+
+int ck_read(pm_server* S, const char* buf, size_t bytes, hipStream_t s)
+{
+    CkHead h;
+    PM_REQUIRE(bytes >= sizeof(h), "pm_server_restore: buffer too short for a checkpoint header");
+    std::memcpy(&h, buf, sizeof(h));
+    PM_REQUIRE(std::memcmp(h.magic, kCkMagic, sizeof(kCkMagic)) == 0 && h.version == kCkVersion &&
+                   h.stats_bytes == sizeof(pm_request_stats),
+               "pm_server_restore: not a server checkpoint of this version");
+    PM_REQUIRE(h.nreq >= 1 && h.done >= 0 && h.done <= h.nreq && h.nwaiting >= 0 && h.nactive >= 0 &&
+                   h.nresults >= 0 && h.npool >= 0 && h.nwaiting + h.nactive + h.done <= h.nreq &&
+                   h.next <= (uint64_t)h.nreq,
+               "pm_server_restore: inconsistent checkpoint header");
+    PM_REQUIRE(h.pre == S->opt.pre && h.post == S->opt.post && h.bottom == S->opt.bottom &&
+                   h.max_iter == S->opt.max_iter && h.sigma == S->opt.sigma,
+               "pm_server_restore: the checkpoint was written by a server with other sweep counts / max_iter / sigma");
+    PM_REQUIRE(h.nactive <= S->opt.max_active, "pm_server_restore: %d requests in flight, max_active is %d", h.nactive,
+               S->opt.max_active);
+    HeldRun R;
+    R.nreq = h.nreq;
+    R.done = h.done;
+    R.fingerprint = h.fingerprint;
+    R.next = (size_t)h.next;
+    R.elapsed = h.elapsed;
+    size_t off = sizeof(h);
+    auto take = [&](size_t b) -> const char* {
+        if (off + b > bytes) { return nullptr; }
+        const char* p = buf + off;
+        off += b;
+        return p;
+    };
+    const char* ps = take(sizeof(pm_request_stats) * (size_t)h.nreq);
+    const char* pw = ps ? take(sizeof(int32_t) * (size_t)h.nwaiting) : nullptr;
+    const char* pa = pw ? take(sizeof(CkActive) * (size_t)h.nactive) : nullptr;
+    const char* pr = pa ? take(sizeof(CkResult) * (size_t)h.nresults) : nullptr;
+    const char* pp = pr ? take(sizeof(CkPool) * (size_t)h.npool) : nullptr;
+    PM_REQUIRE(pp, "pm_server_restore: buffer too short for the checkpoint tables");
+    off = align8(off);
+    R.stats.resize(h.nreq);
+    std::memcpy(R.stats.data(), ps, sizeof(pm_request_stats) * (size_t)h.nreq);
+    for (int k = 0; k < h.nwaiting; k++) {
+        int32_t i;
+        std::memcpy(&i, pw + 4 * k, 4);
+        PM_REQUIRE(i >= 0 && i < h.nreq, "pm_server_restore: bad waiting entry");
+        R.waiting.push_back(i);
+    }
+    std::vector<CkActive> ca(h.nactive);
+    std::vector<CkResult> cr(h.nresults);
+    std::vector<CkPool> cp(h.npool);
+    std::memcpy(ca.data(), pa, sizeof(CkActive) * ca.size());
+    std::memcpy(cr.data(), pr, sizeof(CkResult) * cr.size());
+    std::memcpy(cp.data(), pp, sizeof(CkPool) * cp.size());
+    std::map<int, std::vector<double>> results;
+    int rc = PM_OK;
+    for (const CkActive& c : ca) {
+        if (rc) { break; }
+        if (!(c.idx >= 0 && c.idx < h.nreq && c.n >= 8 && c.n <= 1024 && c.n % 2 == 0 && c.iters >= 0)) {
+            set_error("pm_server_restore: bad in-flight entry");
+            rc = PM_ERR_INVALID;
+            break;
+        }
+        const size_t nb = 8 * (size_t)c.n * c.n * c.n;
+        const char* pd = take(nb);
+        if (!pd) {
+            set_error("pm_server_restore: buffer too short for the solution of request %d", c.idx);
+            rc = PM_ERR_INVALID;
+            break;
+        }
+        Hier* hh = nullptr;
+        rc = take_hier(S, c.n, &hh);
+        if (rc == PM_OK && !hh) {
+            set_error("pm_server_restore: the HBM budget has no room for request %d (n = %d)", c.idx, c.n);
+            rc = PM_ERR_NOMEM;
+        }
+        if (rc) { break; }
+        Active a;
+        a.idx = c.idx;
+        a.h = hh;
+        a.ns = level_sizes(c.n);
+        a.iters = c.iters;
+        a.res0 = c.res0;
+        a.r = c.r;
+        a.fresh = true;            // its first iteration recomputes the residual of the saved solution
+        a.restored = c.fresh == 0; // a request still fresh at the checkpoint starts over
+        R.act.push_back(a);
+        rc = level_linear(c.n, hh->phi64, (char*)pd, false, s);
+    }
+    for (const CkResult& c : cr) {
+        if (rc) { break; }
+        const size_t n3 = (size_t)c.n * c.n * c.n;
+        const char* pd = (c.idx >= 0 && c.idx < h.nreq && c.n >= 1 && c.n <= 1024) ? take(8 * n3) : nullptr;
+        if (!pd) {
+            set_error("pm_server_restore: bad or truncated kept result");
+            rc = PM_ERR_INVALID;
+            break;
+        }
+        std::vector<double>& v = results[c.idx];
+        v.resize(n3);
+        std::memcpy(v.data(), pd, 8 * n3);
+    }
+    if (rc) {
+        release_active(S, R.act);
+        return rc;
+    }
+    // the resident pool the run had: warm hierarchies, as far as the budget allows
+    for (const CkPool& c : cp) {
+        if (c.n < 8 || c.n > 1024 || c.n % 2 || c.count < 0) { continue; }
+        for (int k = (int)S->pool[c.n].size(); k < c.count && S->resident + hier_bytes(c.n) <= S->opt.hbm_budget; k++) {
+            Hier* hh = nullptr;
+            if (alloc_hier(c.n, &hh) != PM_OK) { break; }
+            S->resident += hh->bytes;
+            S->pool[c.n].push_back(hh);
+        }
+    }
+    R.held = true;
+    R.restored = true;
+    S->run = std::move(R);
+    S->results = std::move(results);
     return PM_OK;
 }
 
@@ -1182,15 +1562,10 @@ int pm_server_create(pm_server** out, const pm_server_options* o)
 int pm_server_destroy(pm_server* S)
 {
     if (!S) { return PM_OK; }
+    release_active(S, S->run.act);  // a held run's in-flight hierarchies
     for (auto& kv : S->pool)
         for (Hier* h : kv.second) { free_hier(h); }
-    for (auto& kv : S->plain) {
-        pm_mg_destroy(kv.second.mg);
-        pm_layout_destroy(kv.second.layout);
-        hipFree(kv.second.phi);
-        hipFree(kv.second.rhs);
-        hipFree(kv.second.tab);
-    }
+    for (auto& kv : S->plain) { free_plain(kv.second); }
     hipFree(S->d_lv);
     hipFree(S->d_rq);
     hipFree(S->d_items);
@@ -1227,7 +1602,10 @@ int pm_server_run(pm_server* S, int nreq, const pm_request* reqs, pm_request_sta
         PM_REQUIRE(S->opt.mode == 1 || (size_t)q.n * q.n * q.n < ((size_t)1 << 31),
                    "pm_server_run: request %d too large", i);
     }
-    S->results.clear();
+    if (!S->run.held) {
+        // a held run keeps the results of the requests it finished before the stop
+        S->results.clear();
+    }
     for (int i = 0; i < nreq; i++) {
         pm_request_stats& o = st[i];
         std::memset(&o, 0, sizeof(o));
@@ -1236,6 +1614,51 @@ int pm_server_run(pm_server* S, int nreq, const pm_request* reqs, pm_request_sta
     }
     const hipStream_t s = (hipStream_t)stream;
     return S->opt.mode == 0 ? run_batched(S, nreq, reqs, st, s) : run_plain(S, nreq, reqs, st, s);
+}
+
+int pm_server_set_stop(pm_server* S, int iterations)
+{
+    PM_REQUIRE(S && iterations >= 0, "pm_server_set_stop: bad argument");
+    if (S->opt.mode != 0) {
+        set_error("pm_server_set_stop: only the batched mode (0) stops between loop iterations");
+        return PM_ERR_UNSUPPORTED;
+    }
+    S->stop_after = iterations;
+    return PM_OK;
+}
+
+int pm_server_discard(pm_server* S)
+{
+    PM_REQUIRE(S, "pm_server_discard: null argument");
+    release_active(S, S->run.act);
+    S->run = HeldRun();
+    S->results.clear();
+    return PM_OK;
+}
+
+int pm_server_checkpoint_size(const pm_server* S, size_t* bytes)
+{
+    PM_REQUIRE(S && bytes, "pm_server_checkpoint_size: null argument");
+    PM_REQUIRE(S->run.held, "pm_server_checkpoint_size: no held run (stop one with pm_server_set_stop)");
+    *bytes = ck_layout(S).total;
+    return PM_OK;
+}
+
+int pm_server_checkpoint(pm_server* S, void* buffer, size_t bytes, pm_stream_t stream)
+{
+    PM_REQUIRE(S && buffer, "pm_server_checkpoint: null argument");
+    PM_REQUIRE(S->run.held, "pm_server_checkpoint: no held run (stop one with pm_server_set_stop)");
+    const CkLayout L = ck_layout(S);
+    PM_REQUIRE(bytes >= L.total, "pm_server_checkpoint: buffer of %zu bytes, %zu needed", bytes, L.total);
+    return ck_write(S, (char*)buffer, L, (hipStream_t)stream);
+}
+
+int pm_server_restore(pm_server* S, const void* buffer, size_t bytes, pm_stream_t stream)
+{
+    PM_REQUIRE(S && buffer, "pm_server_restore: null argument");
+    PM_REQUIRE(S->opt.mode == 0, "pm_server_restore: only a batched-mode (0) server resumes a checkpoint");
+    PM_REQUIRE(!S->run.held, "pm_server_restore: the server holds a run (pm_server_discard it first)");
+    return ck_read(S, (const char*)buffer, bytes, (hipStream_t)stream);
 }
 
 int pm_server_rhs(int n, int ngauss, uint64_t seed, double sigma, double* rhs_dev, pm_stream_t stream)

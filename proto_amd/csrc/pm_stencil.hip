@@ -36,20 +36,27 @@ static int detect_kind(const pm_stencil* s)
     for (int d = 0; d < 3; d++) {
         if (s->sr[d] != 1 || s->dr[d] != 1 || s->ds[d] != 0) { unit = false; }
     }
-    if (!unit || s->dim != 3) { return 0; }
+    if (!unit) { return 0; }
+    // kind 1: taps read from LDS by the generic z-march (k_zgen)
+    bool near = s->ncoef >= 1 && s->ncoef <= 27;
+    for (int j = 0; j < s->ncoef && near; j++)
+        for (int d = 0; d < 3; d++)
+            if (s->offs[3 * j + d] < -1 || s->offs[3 * j + d] > 1) { near = false; }
+    const int generic = near ? 1 : 0;
+    if (s->dim != 3) { return generic; }
     if (s->ncoef == 7) {
         for (int j = 0; j < 7; j++)
             for (int d = 0; d < 3; d++)
-                if (s->offs[3 * j + d] != lap7_off(j, d)) { return 0; }
+                if (s->offs[3 * j + d] != lap7_off(j, d)) { return generic; }
         return 7;
     }
     if (s->ncoef == 27) {
         for (int j = 0; j < 27; j++)
             for (int d = 0; d < 3; d++)
-                if (s->offs[3 * j + d] != lap27_off(j, d)) { return 0; }
+                if (s->offs[3 * j + d] != lap27_off(j, d)) { return generic; }
         return 27;
     }
-    return 0;
+    return generic;
 }
 
 // ------------------------------------------------------------ generic kernel
@@ -111,6 +118,8 @@ struct ZArgs {
     int ntx, nty, tz;
     T c[27];        // coefficients, pre-scaled (first operator)
     T c2[27];       // second operator (ZM_RESIDUAL2, always the 27-point shape)
+    int ntap;       // k_zgen: taps in the stencil's order, (dx+1) | (dy+1) << 8 | (dz+1) << 16
+    int tap[27];
 };
 
 template<int NT, typename T>
@@ -434,6 +443,108 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
     }
 }
 
+// Any unit-ratio stencil whose offsets lie in [-1,1]^3 (or [-1,1]^2 in 2-D),
+// in any order and with any coefficients (Laplacian_19, a Laplacian with
+// reordered terms, a user stencil): the tile and z walk of k_zmarch, with the
+// source planes staged through an LDS ring of 4 planes (one barrier per z
+// step) and every tap read from LDS at its offset, accumulated in the
+// stencil's order with one rounding per multiply and per add, as hostApply
+// does.  Loads are bounds-checked against the source box (a stencil without
+// z taps, e.g. 2-D, never reads the planes above and below).  PLANAR: every
+// tap has dz = 0 (2-D data, one plane): a single LDS plane.
+template<typename T, int MODE, int RY, int NT, int PLANAR>
+__global__ __launch_bounds__(256) void k_zgen(ZArgs<T> a)
+{
+    using G = ZGeom<RY>;
+    constexpr int NL = G::NL, LX = G::LX, NE = G::NE;
+    constexpr int NS = PLANAR ? 1 : 4;
+    __shared__ T sm[NS][G::NP];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = b % a.ntx;
+    b /= a.ntx;
+    const int ty = b % a.nty;
+    const int tzi = b / a.nty;
+    const int x0 = a.ibox.lo[0] + tx * G::TX, y0 = a.ibox.lo[1] + ty * G::TY;
+    const int zb = a.ibox.lo[2] + tzi * a.tz;
+    const int ze = min(zb + a.tz, a.ibox.hi[2] + 1);  // outputs z in [zb, ze)
+    const long sx = box_nx(a.sbox), sxy = sx * box_ny(a.sbox);
+    long eoff[NL];
+    bool eok[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const int e = tid + 256 * k;
+        const int row = e / LX, col = e - (e / LX) * LX;
+        const int gx = x0 - 1 + col, gy = y0 - 1 + row;
+        eok[k] = (e < NE) && gx >= a.sbox.lo[0] && gx <= a.sbox.hi[0] && gy >= a.sbox.lo[1] && gy <= a.sbox.hi[1];
+        eoff[k] = (long)(gx - a.sbox.lo[0]) + sx * (long)(gy - a.sbox.lo[1]);
+    }
+    T pre[NL];
+    auto load = [&](int z) {
+        const bool zin = z >= a.sbox.lo[2] && z <= a.sbox.hi[2];
+        const T* base = a.src + sxy * (long)(z - a.sbox.lo[2]);
+#pragma unroll
+        for (int k = 0; k < NL; k++) { pre[k] = (zin && eok[k]) ? base[eoff[k]] : (T)0; }
+    };
+    auto slot = [&](int z) { return PLANAR ? 0 : (z & 3); };
+    auto store = [&](int z) {
+        T* pl = sm[slot(z)];
+#pragma unroll
+        for (int k = 0; k < NL; k++) { pl[tid + 256 * k] = pre[k]; }
+    };
+    const int gx = x0 + lane;
+    const bool xok = gx <= a.ibox.hi[0];
+    const int lc = (w * RY + 1) * LX + lane + 1;  // LDS element of this thread's first row, centre
+    const Box3& ob = a.dbox;
+    auto compute = [&](int z) {
+        T acc[RY];
+        bool ok[RY];
+#pragma unroll
+        for (int o = 0; o < RY; o++) {
+            const int gy = y0 + w * RY + o;
+            ok[o] = xok && gy <= a.ibox.hi[1];
+            acc[o] = (MODE == ZM_ACCUM && ok[o]) ? a.dst[box_index(ob, gx, gy, z)] : (T)0;
+        }
+        for (int j = 0; j < a.ntap; j++) {
+            const int t = a.tap[j];
+            const int dz = (t >> 16) - 1;
+            const int off = (((t >> 8) & 255) - 1) * LX + (t & 255) - 1;
+            const T* p = sm[slot(z + dz)] + lc + off;
+            const T c = a.c[j];
+#pragma unroll
+            for (int o = 0; o < RY; o++) { acc[o] = acc[o] + c * p[o * LX]; }
+        }
+#pragma unroll
+        for (int o = 0; o < RY; o++) {
+            if (ok[o]) { put<NT>(&a.dst[box_index(ob, gx, y0 + w * RY + o, z)], acc[o]); }
+        }
+    };
+    if (PLANAR) {
+        for (int z = zb; z < ze; z++) {
+            load(z);
+            if (z > zb) { __syncthreads(); }  // the previous plane's taps are read
+            store(z);
+            __syncthreads();
+            compute(z);
+        }
+        return;
+    }
+    // ring of 4 planes: step z commits plane z + 1 into the slot of plane z - 3
+    // (last read at step z - 2, before the barrier of step z - 1)
+    load(zb - 1);
+    store(zb - 1);
+    load(zb);
+    store(zb);
+    load(zb + 1);
+    for (int z = zb; z < ze; z++) {
+        store(z + 1);
+        if (z + 1 < ze) { load(z + 2); }
+        __syncthreads();
+        compute(z);
+    }
+}
+
 // Deterministic second stage over per-block {amax, sum} groups of PER values
 // (one pair per operator): one workgroup per value q (blockIdx.x), a fixed
 // strided split and a fixed tree, so the value's bits do not depend on the
@@ -615,6 +726,51 @@ static int launch_zmarch(ZArgs<T>& a, const Box3& ibox, const ZCfg& c, hipStream
     return PM_OK;
 }
 
+// Generic z-march (kind 1): taps of the stencil in its order; PLANAR when no
+// tap leaves the plane.
+static bool zgen_planar(const pm_stencil* st)
+{
+    for (int j = 0; j < st->ncoef; j++)
+        if (st->offs[3 * j + 2] != 0) { return false; }
+    return true;
+}
+
+template<typename T, int MODE>
+using ZGKernel = void (*)(ZArgs<T>);
+
+template<typename T, int MODE>
+static ZGKernel<T, MODE> zgkernel(const ZCfg& c, bool planar)
+{
+#define PM_ZG(RY, NT)                                                                  \
+    if (c.ry == RY && c.nt == NT) {                                                    \
+        return planar ? k_zgen<T, MODE, RY, NT, 1> : k_zgen<T, MODE, RY, NT, 0>;       \
+    }
+    PM_ZG(1, 0) PM_ZG(1, 1) PM_ZG(2, 0) PM_ZG(2, 1) PM_ZG(4, 0) PM_ZG(4, 1)
+#undef PM_ZG
+    return nullptr;
+}
+
+template<typename T, int MODE>
+static int launch_zgen(const pm_stencil* st, ZArgs<T>& a, const Box3& ibox, const ZCfg& c, hipStream_t s)
+{
+    const bool planar = zgen_planar(st);
+    ZGKernel<T, MODE> k = zgkernel<T, MODE>(c, planar);
+    PM_REQUIRE(k != nullptr, "no generic z-march kernel for rows %d, nt %d", c.ry, c.nt);
+    ZPlan p = zplan(ibox, c);
+    a.ntx = p.ntx;
+    a.nty = p.nty;
+    a.tz = p.tz;
+    a.ibox = ibox;
+    a.ntap = st->ncoef;
+    for (int j = 0; j < st->ncoef; j++) {
+        const int* o = &st->offs[3 * j];
+        a.tap[j] = (o[0] + 1) | ((o[1] + 1) << 8) | ((o[2] + 1) << 16);
+    }
+    hipLaunchKernelGGL(k, dim3(p.blocks), dim3(256), 0, s, a);
+    PM_CHECK_LAUNCH();
+    return PM_OK;
+}
+
 template<typename T>
 static void scaled_coefs(const pm_stencil* st, double scale, T* out)
 {
@@ -656,6 +812,23 @@ static int apply_t(const pm_stencil* st, const void* src, const Box3& sbox, void
                 rc = replace ? launch_zmarch<T, 27, ZM_REPLACE>(a, ibox, cfg, s, &nb)
                              : launch_zmarch<T, 27, ZM_ACCUM>(a, ibox, cfg, s, &nb);
             }
+            if (rc) { return rc; }
+        }
+        return PM_OK;
+    }
+    if (st->kind == 1) {
+        const long ssz = box_size(sbox), dsz = box_size(dbox);
+        const ZCfg cfg = lookup_cfg(1, replace ? ZM_REPLACE : ZM_ACCUM, st->dtype, ibox);
+        for (int c = 0; c < ncomp; c++) {
+            ZArgs<T> a;
+            std::memset(&a, 0, sizeof(a));
+            a.src = (const T*)src + c * ssz;
+            a.dst = (T*)dst + c * dsz;
+            a.sbox = sbox;
+            a.dbox = dbox;
+            scaled_coefs<T>(st, scale, a.c);
+            const int rc = replace ? launch_zgen<T, ZM_REPLACE>(st, a, ibox, cfg, s)
+                                   : launch_zgen<T, ZM_ACCUM>(st, a, ibox, cfg, s);
             if (rc) { return rc; }
         }
         return PM_OK;
@@ -946,6 +1119,16 @@ int pm_stencil_kind(const pm_stencil* st, int* kind)
 {
     PM_REQUIRE(st && kind, "pm_stencil_kind: null argument");
     *kind = st->kind;
+    return PM_OK;
+}
+
+int pm_stencil_set_kind(pm_stencil* st, int kind)
+{
+    PM_REQUIRE(st, "pm_stencil_set_kind: null argument");
+    const int natural = detect_kind(st);
+    const bool ok = kind == 0 || kind == natural || (kind == 1 && (natural == 7 || natural == 27));
+    PM_REQUIRE(ok, "pm_stencil_set_kind: kind %d cannot serve this stencil (its own kind is %d)", kind, natural);
+    st->kind = kind;
     return PM_OK;
 }
 

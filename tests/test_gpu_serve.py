@@ -125,3 +125,64 @@ def test_not_converged_status():
         srv.result(0, 32)
     with pytest.raises(pa.NativeError, match="even"):
         srv.run([req(33, 4, 62)])
+
+
+def _ck_trace():
+    # all present at t = 0 with 3 slots: some in flight, some waiting at the stop
+    return [req(32, 5, 41), req(48, 9, 42), req(64, 12, 43), req(32, 20, 44), req(96, 7, 45), req(48, 3, 46)]
+
+
+def test_checkpoint_resume_same_bits_and_cycles():
+    """A run stopped after 3 loop iterations, written to a checkpoint, loaded
+    into a new server and resumed gives every request the bits, status and
+    V-cycle count of the uninterrupted run.  The in-flight solutions in the
+    checkpoint are LevelBoxData<double,1>::linearOut blocks of one-box levels
+    (n^3 values, x fastest)."""
+    trace = _ck_trace()
+    ref = pa.Server(mode=0, max_active=3)
+    st_ref = ref.run(trace)
+    phis = [ref.result(i, r["n"]) for i, r in enumerate(trace)]
+    a = pa.Server(mode=0, max_active=3)
+    a.set_stop(3)
+    with pytest.raises(pa.RunStopped) as stop:
+        a.run(trace)
+    held = stop.value.stats
+    say("stopped", [(s["status"], s["iters"]) for s in held])
+    assert any(s["status"] == -1 for s in held)
+    buf = a.checkpoint()
+    assert buf[:8] == b"PMSRVCK1"
+    del a
+    b = pa.Server(mode=0, max_active=3)
+    b.restore(buf)
+    st = b.run(trace)
+    for i, r in enumerate(trace):
+        say("resumed", i, st[i]["iters"], st_ref[i]["iters"])
+        assert st[i]["status"] == pa.SERVE_SOLVED and st[i]["iters"] == st_ref[i]["iters"], i
+        assert st[i]["rel_residual"] == st_ref[i]["rel_residual"], i
+        assert b.result(i, r["n"]).tobytes() == phis[i].tobytes(), i
+
+
+def test_stop_then_continue_in_place_and_bad_checkpoints():
+    """A held run continues in the same server with the next pm_server_run;
+    a held run refuses another request list; restore refuses a corrupt
+    buffer and a server with other sweep counts."""
+    trace = _ck_trace()
+    ref = pa.Server(mode=0, max_active=3)
+    st_ref = ref.run(trace)
+    a = pa.Server(mode=0, max_active=3)
+    a.set_stop(2)
+    with pytest.raises(pa.RunStopped):
+        a.run(trace)
+    buf = a.checkpoint()
+    with pytest.raises(pa.NativeError):
+        a.run(trace[:5])  # another list
+    a.set_stop(0)
+    st = a.run(trace)
+    assert [s["iters"] for s in st] == [s["iters"] for s in st_ref]
+    assert [s["rel_residual"] for s in st] == [s["rel_residual"] for s in st_ref]
+    with pytest.raises(pa.NativeError):
+        pa.Server(mode=0, max_active=3).restore(b"XXXXXXXX" + buf[8:])
+    with pytest.raises(pa.NativeError):
+        pa.Server(mode=0, max_active=3, pre=4).restore(buf)
+    with pytest.raises(pa.NativeError):
+        pa.Server(mode=0, max_active=3).restore(buf[:200])

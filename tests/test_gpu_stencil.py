@@ -43,10 +43,14 @@ def test_laplacian_kinds():
     for name, kind in (("Laplacian", 7), ("Laplacian_27", 27)):
         c, o, _, _, _ = util.lib_stencil(name)
         assert pa.Stencil(c, o).kind() == kind
+    # any other unit-ratio stencil within [-1,1]^3: the generic z-march (taps from LDS)
     c, o, _, _, _ = util.lib_stencil("Laplacian_19")
-    assert pa.Stencil(c, o).kind() == 0
+    assert pa.Stencil(c, o).kind() == 1
     c, o, _, _, _ = util.lib_stencil("Laplacian")
-    assert pa.Stencil(c[::-1].copy(), o[::-1].copy()).kind() == 0  # other order -> generic
+    assert pa.Stencil(c[::-1].copy(), o[::-1].copy()).kind() == 1  # other order
+    # refinement ratios or taps further out: the per-cell kernel
+    c, o, sr, dr, ds = util.lib_stencil("AvgDown", 2)
+    assert pa.Stencil(c, o, src_ratio=sr, dst_ratio=dr, dst_shift=ds).kind() == 0
 
 
 @pytest.mark.parametrize("name,dtype", [("Laplacian", pa.PM_F64), ("Laplacian_27", pa.PM_F64),
@@ -73,6 +77,66 @@ def test_random_boxes_vs_oracle(name, dtype):
         util.oracle_apply(c, o, src, slo, shi, ref, dlo, dhi, lo, hi, replace, scale)
         out = apply_gpu(st, src, slo, shi, dst0.copy(), dlo, dhi, lo, hi, replace, scale)
         assert out.tobytes() == ref.tobytes(), (name, trial)
+
+
+@pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
+def test_generic_zmarch_random_stencils(dtype):
+    """Kind-1 stencils (any taps in [-1,1]^DIM, any order, random coefficients):
+    3-D with and without z taps (the one-plane form), 2-D, two components,
+    replace and accumulate, scale != 1, boxes that are not tile multiples and
+    destinations larger than the index box; bitwise against the oracle."""
+    rng = np.random.default_rng(23)
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    allo = np.array([(x, y, z) for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1, 0, 1)], dtype=np.int32)
+    for trial in range(10):
+        dim = 2 if trial % 4 == 3 else 3
+        pool = allo[allo[:, 2] == 0] if (dim == 2 or trial % 4 == 2) else allo
+        k = int(rng.integers(1, len(pool) + 1))
+        o = pool[rng.permutation(len(pool))[:k]].copy()
+        c = rng.uniform(-2, 2, size=k).astype(npdt)
+        st = pa.Stencil(c, o, dtype, dim)
+        assert st.kind() == 1
+        nc = 2 if trial % 5 == 4 else 1
+        lo = rng.integers(-9, 9, size=3)
+        n = rng.integers(1, 80, size=3) if trial else np.array([131, 70, 21])
+        if dim == 2:
+            lo[2], n[2] = 0, 1
+        hi = lo + n - 1
+        g = np.array([1, 1, 1 if dim == 3 else 0])
+        slo, shi = lo - g, hi + g
+        dlo, dhi = lo - rng.integers(0, 2, size=3) * (g > 0), hi + rng.integers(0, 2, size=3) * (g > 0)
+        replace = trial % 3 != 1
+        scale = [1.0, 0.5, -3.0][trial % 3]
+        say("generic", dtype, "trial", trial, "dim", dim, "taps", k, "box", lo.tolist(), hi.tolist())
+        ns, nd = util.box_size(slo, shi), util.box_size(dlo, dhi)
+        src = util.uniform(700 + trial, 0, nc * ns).astype(npdt)
+        dst0 = util.uniform(700 + trial, 1, nc * nd).astype(npdt)
+        ref = dst0.copy()
+        for q in range(nc):
+            r = ref[q * nd:(q + 1) * nd].copy()
+            util.oracle_apply(c, o, src[q * ns:(q + 1) * ns].copy(), slo, shi, r, dlo, dhi, lo, hi, replace, scale)
+            ref[q * nd:(q + 1) * nd] = r
+        out = apply_gpu(st, src, slo, shi, dst0.copy(), dlo, dhi, lo, hi, replace, scale, ncomp=nc)
+        assert out.tobytes() == ref.tobytes(), trial
+
+
+@pytest.mark.parametrize("name", ["Laplacian", "Laplacian_27"])
+def test_kernel_families_agree(name):
+    """The 7/27-point z-march, the generic z-march and the per-cell kernel
+    give the same bits for the same stencil (the set_kind override)."""
+    c, o, _, _, _ = util.lib_stencil(name)
+    lo, hi = np.array([-3, 2, 5]), np.array([70, 41, 38])
+    slo, shi = lo - 1, hi + 1
+    src = util.uniform(91, 0, util.box_size(slo, shi))
+    dst0 = util.uniform(91, 1, util.box_size(lo, hi))
+    outs = {}
+    for kind in (7 if name == "Laplacian" else 27, 1, 0):
+        st = pa.Stencil(c, o).set_kind(kind)
+        assert st.kind() == kind
+        outs[kind] = apply_gpu(st, src, slo, shi, dst0.copy(), lo, hi, lo, hi, False, 4096.0).tobytes()
+    assert len(set(outs.values())) == 1
+    with pytest.raises(pa.NativeError):
+        pa.Stencil(c, o).set_kind(27 if name == "Laplacian" else 7)
 
 
 def test_multicomponent_and_generic_ratios():

@@ -167,7 +167,7 @@ def test_level_linear_out_in(n, bs, g):
 
 
 def test_stream_probe_kinds():
-    """The bandwidth probe: copy reproduces the source, write fills the
+    """The bandwidth probe: copies (grid-stride, chunked) reproduce the source, write fills the
     word pattern (w, w+1, w+2, w+3) of 16-byte vector w, read leaves memory
     alone; bad arguments are refused."""
     rng = np.random.default_rng(5)
@@ -182,7 +182,11 @@ def test_stream_probe_kinds():
     sink = zeros(4, np.uint32)
     pa.call("pm_stream_probe", 0, vp(sink), vp(src), a.nbytes, 4, None)
     assert not host(sink).any() and host(src).tobytes() == a.tobytes()
-    assert pa.lib().pm_stream_probe(3, vp(dst), vp(src), a.nbytes, 4, None) == -1
+    for kind in (3, 4):  # chunked copies: one 64 KB chunk and a 3-word tail
+        dst2 = zeros(a.size, np.uint32)
+        pa.call("pm_stream_probe", kind, vp(dst2), vp(src), a.nbytes, 4, None)
+        assert host(dst2).tobytes() == a.tobytes(), kind
+    assert pa.lib().pm_stream_probe(5, vp(dst), vp(src), a.nbytes, 4, None) == -1
     assert pa.lib().pm_stream_probe(2, vp(dst), vp(src), a.nbytes - 4, 4, None) == -1
     assert pa.lib().pm_stream_probe(2, vp(dst), vp(src), a.nbytes, 65, None) == -1
 
