@@ -146,6 +146,20 @@ int pm_stencil_residual2(const pm_stencil* st1, const pm_stencil* st2, const voi
                          const pm_box* r_box, double scale1, double scale2, void* red_out,
                          pm_stream_t stream);
 
+/* pm_stencil_residual2 with no device allocation: no counterpart in the
+ * reference.  It is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-workgroup partials live in a fixed 1 MiB device array of the library
+ * (one per GPU, held until the process ends) instead of the stream's scratch,
+ * so r_box may launch at most 32768 FP64 workgroups (65536 FP32): 512^3 with
+ * the default tiles launches 8192; larger boxes return PM_ERR_INVALID.  Same
+ * kernels, same bits as pm_stencil_residual2.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (two streams, two threads) race. */
+int pm_stencil_residual2_noalloc(const pm_stencil* st1, const pm_stencil* st2, const void* u,
+                                 const pm_box* u_box, const void* f, void* r1, void* r2,
+                                 const pm_box* r_box, double scale1, double scale2,
+                                 void* red_out, pm_stream_t stream);
+
 /* ------------------------------------------------------------- autotuner */
 /* Tile configuration of the z-marching 7/27-point kernels: rows of the
  * 64-wide tile per thread (1, 2, 4), planes prefetched (1, 2), non-temporal

@@ -116,6 +116,7 @@ SIGNATURES = {
     "pm_stencil_apply": (_i, [_vp, _vp, _BP, _vp, _BP, _BP, _i, _i, _d, _vp]),
     "pm_stencil_residual": (_i, [_vp, _vp, _BP, _vp, _vp, _BP, _d, _vp, _vp]),
     "pm_stencil_residual2": (_i, [_vp, _vp, _vp, _BP, _vp, _vp, _vp, _BP, _d, _d, _vp, _vp]),
+    "pm_stencil_residual2_noalloc": (_i, [_vp, _vp, _vp, _BP, _vp, _vp, _vp, _BP, _d, _d, _vp, _vp]),
     "pm_reduce": (_i, [_i, _i, _vp, _sz, _vp, _i, _vp]),
     "pm_reduce_box": (_i, [_i, _i, _vp, _BP, _BP, _i, _vp, _i, _vp]),
     "pm_reduce_init": (_i, [_i, _i, _vp, _vp]),

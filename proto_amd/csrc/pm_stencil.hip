@@ -884,10 +884,20 @@ static int residual_t(const pm_stencil* st, const void* u, const Box3& ubox, con
     return PM_OK;
 }
 
+// Partials of the two-operator fused residual: 4 values per workgroup.
+template<typename T>
+static size_t residual2_partials_bytes(const pm_stencil* st1, const pm_stencil* st2, const Box3& rbox)
+{
+    const ZCfg cfg = lookup_cfg(st1->kind * 100 + st2->kind, ZM_RESIDUAL2, st1->dtype, rbox);
+    return sizeof(T) * 4 * (size_t)zplan(rbox, cfg).blocks;
+}
+
+// ws: the partials buffer (residual2_partials_bytes), or null for the
+// stream's scratch
 template<typename T>
 static int residual2_t(const pm_stencil* st1, const pm_stencil* st2, const void* u, const Box3& ubox,
                        const void* f, void* r1, void* r2, const Box3& rbox, double scale1, double scale2,
-                       void* red_out, hipStream_t s)
+                       void* red_out, hipStream_t s, void* ws = nullptr)
 {
     ZArgs<T> a;
     std::memset(&a, 0, sizeof(a));
@@ -901,9 +911,11 @@ static int residual2_t(const pm_stencil* st1, const pm_stencil* st2, const void*
     scaled_coefs<T>(st2, scale2, a.c2);
     const ZCfg cfg = lookup_cfg(st1->kind * 100 + st2->kind, ZM_RESIDUAL2, st1->dtype, rbox);
     ZPlan p = zplan(rbox, cfg);
-    void* ws = nullptr;
-    int rc = workspace(s, sizeof(T) * 4 * (size_t)p.blocks, &ws);
-    if (rc) { return rc; }
+    int rc = PM_OK;
+    if (!ws) {
+        rc = workspace(s, sizeof(T) * 4 * (size_t)p.blocks, &ws);
+        if (rc) { return rc; }
+    }
     a.partials = (T*)ws;
     unsigned nb;
     rc = (st1->kind == 7) ? launch_zmarch<T, 7, ZM_RESIDUAL2>(a, rbox, cfg, s, &nb)
@@ -1268,6 +1280,79 @@ int pm_stencil_residual2(const pm_stencil* st1, const pm_stencil* st2, const voi
         return residual2_t<double>(st1, st2, u, ub, f, r1, r2, rb, scale1, scale2, red_out, S(stream));
     }
     return residual2_t<float>(st1, st2, u, ub, f, r1, r2, rb, scale1, scale2, red_out, S(stream));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ allocation-free residual
+// Fixed device scratch of pm_stencil_residual2_noalloc: the per-workgroup
+// partials of the fused kernel (4 values each), 1 MiB, enough for 32768 FP64
+// workgroups (512^3 with the default tiles launches 8192).  Each device holds
+// its own copy for the life of the process.  Host code never passes the name
+// `scratch` to a kernel (on the host it is only a stand-in for the symbol):
+// the device address comes from hipGetSymbolAddress, once per device.
+static constexpr size_t kScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace pm {
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices];
+
+// Device address of this device's scratch, looked up on the first call made
+// with the device current.
+static int scratch_address(void** out)
+{
+    int dev = 0;
+    PM_CHECK_HIP(hipGetDevice(&dev));
+    PM_REQUIRE(dev >= 0 && dev < kMaxDevices, "pm_stencil_residual2_noalloc: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("pm_stencil_residual2_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return PM_ERR_HIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return PM_OK;
+}
+
+}  // namespace pm
+
+extern "C" {
+
+int pm_stencil_residual2_noalloc(const pm_stencil* st1, const pm_stencil* st2, const void* u, const pm_box* u_box,
+                                 const void* f, void* r1, void* r2, const pm_box* r_box, double scale1,
+                                 double scale2, void* red_out, pm_stream_t stream)
+{
+    PM_REQUIRE(st1 && st2 && u && u_box && f && r1 && r2 && r_box, "pm_stencil_residual2_noalloc: null argument");
+    PM_REQUIRE(st1->dtype == st2->dtype, "pm_stencil_residual2_noalloc: dtype mismatch");
+    PM_REQUIRE((st1->kind == 7 || st1->kind == 27) && st2->kind == 27,
+               "pm_stencil_residual2_noalloc: needs a 7- or 27-point Laplacian shape and a 27-point one");
+    const Box3 ub = box3(u_box), rb = box3(r_box);
+    PM_REQUIRE(!box_empty(rb), "pm_stencil_residual2_noalloc: empty box");
+    PM_REQUIRE(covers(st1, ub, rb, rb) && covers(st2, ub, rb, rb),
+               "pm_stencil_residual2_noalloc: u does not cover the stencils' domain of r_box");
+    PM_REQUIRE(scale1 != 0.0 && scale2 != 0.0, "pm_stencil_residual2_noalloc: zero scale");
+    const bool f64 = st1->dtype == PM_F64;
+    const size_t need = f64 ? residual2_partials_bytes<double>(st1, st2, rb)
+                            : residual2_partials_bytes<float>(st1, st2, rb);
+    PM_REQUIRE(need <= kScratchBytes,
+               "pm_stencil_residual2_noalloc: the box needs %zu bytes of partials, the fixed scratch holds %zu; "
+               "use pm_stencil_residual2",
+               need, kScratchBytes);
+    void* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != PM_OK) { return rc; }
+    if (f64) {
+        return residual2_t<double>(st1, st2, u, ub, f, r1, r2, rb, scale1, scale2, red_out, S(stream), ws);
+    }
+    return residual2_t<float>(st1, st2, u, ub, f, r1, r2, rb, scale1, scale2, red_out, S(stream), ws);
 }
 
 }  // extern "C"
