@@ -471,7 +471,18 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
         ms1 = time_launches(one_pass, 5, stream)
         alg = np.dtype(t).itemsize * 3 * n ** 3  # phi read, rhs read, phi written
         sweeps[tname] = {"two_colour_ms": round(ms2, 4), "one_pass_ms": round(ms1, 4),
-                         "one_pass_GB/s": round(alg / (ms1 * 1e-3) / 1e9, 1)}
+                         "one_pass_GB/s": round(alg / (ms1 * 1e-3) / 1e9, 1),
+                         "one_pass_frac": round(alg / (ms1 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        # the one-pass sweep by kernel: LDS tiles, row registers with 4 / 8 rows
+        forms = {}
+        try:
+            for form, fname in ((1, "tile"), (2, "rows4"), (3, "rows8")):
+                pa.call("pm_level_sweep_form", form)
+                msf = time_launches(one_pass, 5, stream)
+                forms[fname] = {"ms": round(msf, 4), "frac": round(alg / (msf * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        finally:
+            pa.call("pm_level_sweep_form", 0)
+        sweeps[tname]["by_kernel"] = forms
         for d in (ph, tm, rh):
             d.free()
     out["fine_sweep"] = sweeps

@@ -249,10 +249,16 @@ int pm_level_gsrb(const pm_layout* layout, int dtype, void* phi, const int phi_g
  * out of place, with no exchange: phi_out (valid cells) = swept phi_in; cells
  * of other boxes are read from their owners.  Bitwise the exchange / red /
  * exchange / black sequence.  3-D, periodic layouts covering their domain;
- * PM_ERR_UNSUPPORTED otherwise (use pm_level_gsrb). */
+ * PM_ERR_UNSUPPORTED otherwise (use pm_level_gsrb).  Levels of boxes 64 cells
+ * wide (at most 8 across the domain) run the row-register kernel, others the
+ * LDS tile kernel; both give the same bits. */
 int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx,
                         pm_stream_t stream);
+/* Which kernel runs the one-pass sweep here and in the multigrid driver
+ * (process-wide; tests and benchmarks compare them): 0 automatic, 1 the LDS
+ * tile kernel, 2 / 3 the row-register kernel with 4 / 8 rows per thread. */
+int pm_level_sweep_form(int form);
 /* Damped point Jacobi of examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:99-123:
  * tmp = L(phi, 1/(4 DIM)) + (-(dx*dx)/(4 DIM)) rhs on all valid cells, then
  * phi += tmp.  tmp: level data without ghosts. */

@@ -403,6 +403,227 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
     write_out(z1 - 1);
 }
 
+// ------------------------------------------------------------ row-register sweep
+// x neighbours across the wave: lane l takes lane l - 1 (wave_shr:1) or l + 1
+// (wave_shl:1); the lane with no source (0 or 63) keeps `edge`.
+constexpr int kShr1 = 0x138, kShl1 = 0x130;
+template<int CTRL>
+__device__ __forceinline__ float lane_shift(float edge, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template<int CTRL>
+__device__ __forceinline__ double lane_shift(double edge, double v)
+{
+    const long long e = __double_as_longlong(edge), x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)e, (int)x, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One full red-black sweep of a periodic level of boxes 64 cells wide that
+// covers its domain, out of place, in one pass, the rows held in registers.
+// A workgroup is one wave per box column of the domain (at most 8): lane l of
+// wave w holds column 64 w + l, so the workgroup holds whole rows of the
+// domain.  It owns rows [Y0, Y0 + RY) of one box row and planes [Z0, Z1) of
+// one box layer; each thread keeps its column of rows [Y0 - 2, Y0 + RY + 2)
+// of the planes in use in registers and walks z.  Step k: the red cells of
+// plane k + 1 over the tile rows and one halo row on each side (the values
+// the neighbouring tiles compute for those cells: same inputs, same order),
+// then the black cells of plane k, which is written out.  x neighbours are
+// the adjacent lanes (DPP wave shifts); at a box edge, the adjacent wave's
+// edge lane, passed through LDS (the last wave wraps to the first): one
+// barrier per step.  Addresses are uniform over a wave: per plane and array
+// three row bases (the two rows above the tile, the tile, the two rows below,
+// which may lie in the neighbouring boxes) are scalar, and rows add
+// compile-time offsets (the ghost widths GP, GR, GO of phi in, rhs and phi
+// out are template arguments).  Arithmetic, order and colouring are k_gsrb's;
+// IN 0 / 1 and OUT as k_gsrb_fused.
+template<typename T, int RY, int IN, int OUT, int NT, int GP, int GR, int GO>
+__global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restrict__ pin, T* __restrict__ pout,
+                                                   const T* __restrict__ rhs, Lap<T> c, T lambda, int zc,
+                                                   double* __restrict__ pacc)
+{
+    constexpr int NR = RY + 4;  // phi rows held: Y0 - 2 .. Y0 + RY + 1
+    constexpr int NQ = RY + 2;  // rhs rows: Y0 - 1 .. Y0 + RY
+    constexpr long XP = 64 + 2 * GP, XR = 64 + 2 * GR, XO = 64 + 2 * GO;
+    __shared__ T eo[2][8][2][NQ];  // old values of a plane's edge lanes (0, 63), rows Y0 - 1 .. Y0 + RY
+    __shared__ T er[2][8][2][RY];  // red-updated edge values, rows Y0 .. Y0 + RY - 1
+    // every argument the loops use, as scalars (a reference to a by-value
+    // kernel argument would copy it to private memory)
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nw = L.nb[0], nb1 = L.nb[1], nb2 = L.nb[2];
+    const int bs1 = L.bs[1], bs2 = L.bs[2], dom1 = L.domain[1];
+    const int wl = w == 0 ? nw - 1 : w - 1, wr = w == nw - 1 ? 0 : w + 1;
+    const int nyt = dom1 / RY;
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const int Y0 = (int)(t % nyt) * RY, Z0 = (int)(t / nyt) * zc, Z1 = Z0 + zc;
+    const int by = Y0 / bs1, bz = Z0 / bs2, y0 = Y0 - by * bs1;
+    const T c0 = c.c[0], c1 = c.c[1], c2 = c.c[2], c3 = c.c[3], c4 = c.c[4], c5 = c.c[5], c6 = c.c[6];
+    // the two rows above the tile lie in the box above when the tile starts its
+    // box, the two below in the box below when it ends it
+    const int ytop = y0 == 0 ? -1 : 0, ybot = y0 + RY == bs1 ? 1 : 0;
+    const int* table = L.table;
+    auto box_at = [&](int ys, int zs) __attribute__((always_inline)) {
+        const int yb = (by + ys + nb1) % nb1, zb = (bz + zs + nb2) % nb2;
+        return table[w + nw * (yb + nb1 * zb)];
+    };
+    // boxes of the three row classes for the planes below / in / above the box layer
+    const int tm = box_at(ytop, -1), t0 = box_at(ytop, 0), tp = box_at(ytop, 1);
+    const int mm = box_at(0, -1), m0 = box_at(0, 0), mp = box_at(0, 1);
+    const int bm = box_at(ybot, -1), b0 = box_at(ybot, 0), bp = box_at(ybot, 1);
+    // row offsets of the classes within their box: row i sits at class base + X (i - 2)
+    const int yt = y0 == 0 ? bs1 : y0, yb = ybot ? -RY : y0;
+    // strides of the three arrays
+    const long XYP = XP * (bs1 + 2 * GP), VP = XYP * (bs2 + 2 * GP), OP = GP + XP * GP + XYP * GP;
+    const long XYR = XR * (bs1 + 2 * GR), VR = XYR * (bs2 + 2 * GR), OR = GR + XR * GR + XYR * GR;
+    const long XYO = XO * (bs1 + 2 * GO), VO = XYO * (bs2 + 2 * GO), OO = GO + XO * GO + XYO * GO;
+    // pointers to the first tile row (row i = 2) of each row class of plane Z:
+    // row i of the class is at that pointer + X (i - 2); all uniform
+#define PM_ROW_BASES(X, XY, V, O0, Z, PT, PM, PB)                                              \
+    {                                                                                          \
+        int it_ = t0, im_ = m0, ib_ = b0, zz_ = (Z) - bz * bs2;                                \
+        if (zz_ < 0) {                                                                         \
+            it_ = tm;                                                                          \
+            im_ = mm;                                                                          \
+            ib_ = bm;                                                                          \
+            zz_ += bs2;                                                                        \
+        } else if (zz_ >= bs2) {                                                               \
+            it_ = tp;                                                                          \
+            im_ = mp;                                                                          \
+            ib_ = bp;                                                                          \
+            zz_ -= bs2;                                                                        \
+        }                                                                                      \
+        const long zo_ = (O0) + (XY) * zz_;                                                    \
+        PT = (long)it_ * (V) + zo_ + (X) * yt;                                                 \
+        PM = (long)im_ * (V) + zo_ + (X) * y0;                                                 \
+        PB = (long)ib_ * (V) + zo_ + (X) * yb;                                                 \
+    }
+
+    T P[5][NR];  // phi planes by slot: plane Z0 - 2 + s + 5 m in slot s
+    T Q[5][NQ];  // rhs planes, same slots
+    auto loadP = [&](T (&d)[NR], int Z) {
+        if constexpr (IN == 1) {
+#pragma unroll
+            for (int i = 0; i < NR; i++) { d[i] = (T)0; }
+        } else {
+            long ot, om, ob;
+            PM_ROW_BASES(XP, XYP, VP, OP, Z, ot, om, ob)
+#pragma unroll
+            for (int i = 0; i < NR; i++) { d[i] = pin[(i < 2 ? ot : (i < RY + 2 ? om : ob)) + XP * (i - 2) + lane]; }
+        }
+    };
+    auto loadQ = [&](T (&d)[NQ], int Z) {
+        long ot, om, ob;
+        PM_ROW_BASES(XR, XYR, VR, OR, Z, ot, om, ob)
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int i = j + 1;
+            d[j] = rhs[(i < 2 ? ot : (i < RY + 2 ? om : ob)) + XR * (i - 2) + lane];
+        }
+    };
+    const bool edge = lane == 0 || lane == 63;
+    const int side = lane == 63 ? 1 : 0;
+    auto pub_old = [&](const T (&d)[NR], int Z) {
+        if (edge) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) { eo[Z & 1][w][side][j] = d[j + 1]; }
+        }
+    };
+    auto pub_red = [&](const T (&d)[NR], int Z) {
+        if (edge) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) { er[Z & 1][w][side][r] = d[r + 2]; }
+        }
+    };
+    // red (colour 0) where x + y + z is even; lane parity against the row's
+    const bool lodd = lane & 1;
+    auto is_red = [&](int i, int Z) __attribute__((always_inline)) { return lodd == (((Y0 - 2 + i + Z) & 1) != 0); };
+    auto relax = [&](T cur, T left, T right, T up, T down, T zp, T zm, T f) __attribute__((always_inline)) {
+        T s = 0;
+        s = s + c0 * cur;
+        s = s + c1 * right;
+        s = s + c2 * left;
+        s = s + c3 * up;
+        s = s + c4 * down;
+        s = s + c5 * zp;
+        s = s + c6 * zm;
+        s = s + f;
+        return cur + lambda * s;
+    };
+    // the step of plane k with slot S of plane k (k - (Z0 - 2) = S mod 5)
+    auto step = [&](auto sc, int k) __attribute__((always_inline)) {
+        constexpr int S = decltype(sc)::value;
+        constexpr int S0 = S, S1 = (S + 1) % 5, S2 = (S + 2) % 5, S3 = (S + 3) % 5, SM = (S + 4) % 5;
+        __syncthreads();
+        // the next planes (clamped to the chunk's last halo planes: a valid address)
+        loadP(P[S3], min(k + 3, Z1 + 1));
+        loadQ(Q[S2], min(k + 2, Z1));
+        // red cells of plane k + 1, rows Y0 - 1 .. Y0 + RY
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int i = j + 1;
+            const T cur = P[S1][i];
+            const T le = eo[(k + 1) & 1][wl][1][j], re = eo[(k + 1) & 1][wr][0][j];
+            const T nv = relax(cur, lane_shift<kShr1>(le, cur), lane_shift<kShl1>(re, cur), P[S1][i + 1],
+                               P[S1][i - 1], P[S2][i], P[S0][i], Q[S1][j]);
+            P[S1][i] = is_red(i, k + 1) ? nv : cur;
+        }
+        if (k >= Z0) {
+            // black cells of plane k, rows Y0 .. Y0 + RY - 1, then plane k out
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int i = r + 2;
+                const T cur = P[S0][i];
+                const T le = er[k & 1][wl][1][r], re = er[k & 1][wr][0][r];
+                const T nv = relax(cur, lane_shift<kShr1>(le, cur), lane_shift<kShl1>(re, cur), P[S0][i + 1],
+                                   P[S0][i - 1], P[S1][i], P[SM][i], Q[S0][r + 1]);
+                P[S0][i] = is_red(i, k) ? cur : nv;
+            }
+            // the tile rows of the output plane are the middle class
+            long ot, om, ob;
+            PM_ROW_BASES(XO, XYO, VO, OO, k, ot, om, ob)
+            (void)ot;
+            (void)ob;
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const long o = om + XO * r + lane;
+                if constexpr (OUT == 1) {
+                    pacc[o] = pacc[o] + (double)P[S0][r + 2];
+                } else if constexpr (NT) {
+                    __builtin_nontemporal_store(P[S0][r + 2], &pout[o]);
+                } else {
+                    pout[o] = P[S0][r + 2];
+                }
+            }
+        }
+        pub_red(P[S1], k + 1);
+        pub_old(P[S2], k + 2);
+    };
+    // planes Z0 - 2 .. Z0 (slots 0..2), rhs Z0 - 1 (slot 1); the first step
+    // loads plane Z0 + 1 and rhs Z0
+    loadP(P[0], Z0 - 2);
+    loadP(P[1], Z0 - 1);
+    loadP(P[2], Z0);
+    loadQ(Q[1], Z0 - 1);
+    pub_old(P[1], Z0 - 1);
+    int k = Z0 - 2;
+    for (;;) {
+        step(std::integral_constant<int, 0>{}, k);
+        if (++k >= Z1) { break; }
+        step(std::integral_constant<int, 1>{}, k);
+        if (++k >= Z1) { break; }
+        step(std::integral_constant<int, 2>{}, k);
+        if (++k >= Z1) { break; }
+        step(std::integral_constant<int, 3>{}, k);
+        if (++k >= Z1) { break; }
+        step(std::integral_constant<int, 4>{}, k);
+        if (++k >= Z1) { break; }
+    }
+#undef PM_ROW_BASES
+}
+
 // ------------------------------------------------------------ small levels
 // All n GSRB sweeps of a one-box periodic level of at most GS_MAX cells in one
 // workgroup: phi lives in LDS for the whole relaxation (periodic neighbours by
@@ -1435,6 +1656,95 @@ namespace pm {
         PM_SWEEP_LAUNCH(E, 2, 0); \
     }
 
+// Sweep kernel of the one-pass sweep (tests and benchmarks compare them): 0
+// automatic (the row-register sweep where the level qualifies), 1 the tile
+// sweep, 2 / 3 the row-register sweep with 4 / 8 rows per thread.
+static int g_sweep_form = 0;
+
+// the row-register sweep takes boxes 64 cells wide, at most 8 across the
+// domain, whose height the rows per thread divide
+static bool rows_ok(const pm_layout* L, int ry)
+{
+    return L->bs[0] == 64 && L->nb[0] <= 8 && L->bs[1] % ry == 0 && L->bs[1] >= 2 && L->bs[2] >= 2;
+}
+
+// planes per workgroup of the row sweep: a divisor of the box depth, small
+// enough for about 8192 waves over the level
+static int rows_zc(const pm_layout* L, int ry)
+{
+    int zc = L->bs[2];
+    while (zc % 2 == 0 && zc > 8 && (long)(L->domain[1] / ry) * (L->domain[2] / zc) * L->nb[0] < 8192) { zc /= 2; }
+    return zc;
+}
+
+// ghost layers (phi in, phi out) the row sweep is built for: the multigrid
+// driver's ping-pong phi (ghost 1) -> scratch (ghost 0) -> phi, rhs without ghosts
+static int rows_ghosts(const int in_ghost[3], const int out_ghost[3], const int rhs_ghost[3], int in_mode)
+{
+    const G3 gi = g3(in_ghost), go = g3(out_ghost), gq = g3(rhs_ghost);
+    bool in1 = true, in0 = true, out1 = true, out0 = true, rhs0 = true;
+    for (int k = 0; k < 3; k++) {
+        in1 = in1 && gi.g[k] == 1;
+        in0 = in0 && gi.g[k] == 0;
+        out1 = out1 && go.g[k] == 1;
+        out0 = out0 && go.g[k] == 0;
+        rhs0 = rhs0 && gq.g[k] == 0;
+    }
+    if (!rhs0) { return -1; }
+    if ((in1 || in_mode == 1) && out0) { return 10; }
+    if ((in0 || in_mode == 1) && out1) { return 1; }
+    return -1;
+}
+
+template<typename T, int RY>
+static void launch_rows(const pm_layout* L, int in_mode, int ghosts, const T* pi, T* po, const T* rhs,
+                        const Lap<T>& c, T lt, double* acc, hipStream_t stream)
+{
+    const int zc = rows_zc(L, RY);
+    const dim3 grid((unsigned)((L->domain[1] / RY) * (L->domain[2] / zc))), block((unsigned)(64 * L->nb[0]));
+    const LayoutDev D = ldev(L);
+    constexpr int NT = sizeof(T) == 8 ? 1 : 0;  // non-temporal write-out in FP64, as the tile sweep
+#define PM_ROWS(I, O, N, GP, GO) \
+    hipLaunchKernelGGL((k_gsrb_rows<T, RY, I, O, N, GP, 0, GO>), grid, block, 0, stream, D, pi, po, rhs, c, lt, zc, acc)
+    if (ghosts == 10) {  // phi (ghost 1) -> scratch (ghost 0)
+        if (in_mode == 0) {
+            PM_ROWS(0, 0, NT, 1, 0);
+        } else {
+            PM_ROWS(1, 0, NT, 1, 0);
+        }
+    } else {  // scratch (ghost 0) -> phi (ghost 1), or added into the FP64 solution
+        if (acc) {
+            PM_ROWS(0, 1, 0, 0, 1);
+        } else if (in_mode == 0) {
+            PM_ROWS(0, 0, NT, 0, 1);
+        } else {
+            PM_ROWS(1, 0, NT, 0, 1);
+        }
+    }
+#undef PM_ROWS
+}
+
+// the row-register sweep with the coefficients and weight of pm_level_gsrb
+template<typename T>
+static void sweep_rows(const pm_layout* L, int ry, int ghosts, const void* phi_in, void* phi_out, const void* rhs,
+                       double dx, int in_mode, hipStream_t stream, double* acc)
+{
+    double lam = -(dx * dx) / (4.0 * L->dim);
+    lam *= 2.0;
+    const double m = -1.0 / (dx * dx);
+    Lap<T> c = lap_coefs<T>(L->dim, m);
+    if (sizeof(T) == 8) {
+        for (int j = 0; j < 7; j++) { c.c[j] = (T)((j == 0 ? -2.0 * L->dim : 1.0) * m); }
+    }
+    const T* pi = (const T*)(in_mode == 1 ? phi_out : phi_in);
+    const T lt = (T)(1.0 * lam);
+    if (ry == 4) {
+        launch_rows<T, 4>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, stream);
+    } else {
+        launch_rows<T, 8>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, stream);
+    }
+}
+
 int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, int in_mode,
                         const pm_layout* C, const void* coarse, const int coarse_ghost[3], hipStream_t stream,
@@ -1469,6 +1779,23 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
             return PM_ERR_UNSUPPORTED;
         }
     }
+    for (int k = 0; k < 3; k++) { PM_REQUIRE(L->bs[k] >= 2, "pm_level_gsrb_sweep: boxes need 2+ cells per side"); }
+    {
+        // 4 rows per thread: 512^3 in 64^3 boxes swept in 0.389 ms (FP32) and
+        // 0.683 ms (FP64) against 0.411 / 1.61 ms with 8 rows (FP64 spills)
+        // and 0.501 / 0.724 ms with the tile sweep
+        const int ry = g_sweep_form == 3 ? 8 : 4;
+        const int ghosts = rows_ghosts(in_ghost, out_ghost, rhs_ghost, in_mode);
+        if (g_sweep_form != 1 && in_mode != 2 && ghosts > 0 && (!acc || ghosts == 1) && rows_ok(L, ry)) {
+            if (dtype == PM_F64) {
+                sweep_rows<double>(L, ry, ghosts, phi_in, phi_out, rhs, dx, in_mode, stream, acc);
+            } else {
+                sweep_rows<float>(L, ry, ghosts, phi_in, phi_out, rhs, dx, in_mode, stream, acc);
+            }
+            PM_CHECK_LAUNCH();
+            return PM_OK;
+        }
+    }
     const size_t esz = dtype == PM_F64 ? 8 : 4;
     // column tiles of at most 64 cells (boxes wider than 64 split evenly), so
     // that 8-row tiles fit the LDS however wide the box is
@@ -1487,7 +1814,6 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
         set_error("pm_level_gsrb_sweep: tiles %d cells wide do not fit the LDS ring", tw);
         return PM_ERR_UNSUPPORTED;
     }
-    for (int k = 0; k < 3; k++) { PM_REQUIRE(L->bs[k] >= 2, "pm_level_gsrb_sweep: boxes need 2+ cells per side"); }
     const int nyt = (L->bs[1] + rows - 1) / rows;
     int zc = L->bs[2];
     while (zc > 8 && (long)L->nboxes * nxt * nyt * ((L->bs[2] + zc - 1) / zc) < 2048) { zc = (zc + 1) / 2; }
@@ -1571,6 +1897,13 @@ int level_gsrb_small(const pm_layout* L, int dtype, void* phi, const int phi_gho
 }  // namespace pm
 
 extern "C" {
+
+int pm_level_sweep_form(int form)
+{
+    PM_REQUIRE(form >= 0 && form <= 3, "pm_level_sweep_form: form must be 0..3");
+    g_sweep_form = form;
+    return PM_OK;
+}
 
 int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx, pm_stream_t stream)

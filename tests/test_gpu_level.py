@@ -160,6 +160,69 @@ def test_gsrb_sweep_equals_two_colours(dt, n, bs):
     assert L.unpack(host(back), 1).tobytes() == L.unpack(host(ref), 1).tobytes()
 
 
+def _box_layout(domain, box):
+    """A periodic layout of `box`-sized boxes covering `domain`, boxes in z, y, x order."""
+    nb = [d // b for d, b in zip(domain, box)]
+    lows = np.array([(x * box[0], y * box[1], z * box[2]) for z in range(nb[2]) for y in range(nb[1])
+                     for x in range(nb[0])], dtype=np.int32)
+    h = ctypes.c_void_p()
+    pa.call("pm_layout_create", ctypes.byref(h), 3, pa.i3(domain), pa.i3(box), pa.i3([1, 1, 1]), len(lows),
+            lows.reshape(-1).ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return h, lows
+
+
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+@pytest.mark.parametrize("domain,box", [((128, 128, 128), (64, 64, 64)), ((64, 96, 64), (64, 32, 32)),
+                                        ((256, 64, 128), (64, 16, 64)), ((512, 64, 32), (64, 8, 16))])
+def test_row_sweep_equals_tile_sweep(dt, domain, box):
+    """The row-register sweep (4 and 8 rows per thread) gives the tile sweep's
+    bits: levels one to eight boxes wide, tiles whose halo rows and planes lie
+    in the neighbouring boxes, and several sweeps in a row."""
+    t = np.float64 if dt == pa.PM_F64 else np.float32
+    h, lows = _box_layout(domain, box)
+    nbox = len(lows)
+    e1 = int(np.prod([b + 2 for b in box]))
+    e0 = int(np.prod(box))
+    phi = util.uniform(41, 0, nbox * e1).astype(t)
+    rhs = util.uniform(41, 1, nbox * e0).astype(t)
+    dx = 1.0 / domain[0]
+    outs = {}
+    try:
+        for form in (1, 2, 3):
+            pa.call("pm_level_sweep_form", form)
+            a, tmp, rh = dev(phi), zeros(nbox * e0, t), dev(rhs)
+            say("row sweep", dt, domain, box, "form", form)
+            for _ in range(2):  # phi (ghost 1) -> scratch (ghost 0) -> phi, as the multigrid driver
+                pa.call("pm_level_gsrb_sweep", h, dt, vp(a), G1, vp(tmp), G0, vp(rh), G0, dx, None)
+                pa.call("pm_level_gsrb_sweep", h, dt, vp(tmp), G0, vp(a), G1, vp(rh), G0, dx, None)
+            pa.call("pm_level_gsrb_sweep", h, dt, vp(a), G1, vp(tmp), G0, vp(rh), G0, dx, None)
+            outs[form] = (host(a).tobytes(), host(tmp).tobytes())
+    finally:
+        pa.call("pm_level_sweep_form", 0)
+        pa.lib().pm_layout_destroy(h)
+    assert outs[2] == outs[1] and outs[3] == outs[1]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_row_and_tile_sweeps_same_solve(precision):
+    """Whole solves (zero initial guess folded into the first sweep, FP32 sweeps
+    adding into the FP64 solution) with the row-register sweep and with the
+    tile sweep: the same residual history and solution bits."""
+    n, bs, nlev = 128, 64, 4
+    rhs = util.gaussian_rhs(n, 5, 8)
+    res = {}
+    try:
+        for form in (1, 0):
+            pa.call("pm_level_sweep_form", form)
+            res[form] = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs)
+    finally:
+        pa.call("pm_level_sweep_form", 0)
+    (rc1, it1, h1, p1), (rc0, it0, h0, p0) = res[1], res[0]
+    assert rc1 == 0 and rc0 == 0 and it1 == it0
+    assert h1.tobytes() == h0.tobytes()
+    assert p1.tobytes() == p0.tobytes()
+
+
 def test_gsrb_sweep_unsupported_layouts():
     hole = Layout(32, 16, lows=[(0, 0, 0), (16, 0, 0)])
     buf = zeros(2 * 18 ** 3)
