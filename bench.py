@@ -47,7 +47,9 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tile configurations)")
     ap.add_argument("--no-mg", action="store_true", help="skip the config-3 multigrid measurement")
     ap.add_argument("--mg-n", type=int, default=512)
-    ap.add_argument("--mg-graph", type=int, default=1, help="replay each multigrid cycle as a captured graph")
+    ap.add_argument("--mg-graph", type=int, default=0,
+                    help="replay each multigrid cycle as a captured graph (the other form is timed beside it; "
+                         "the two measure within 1 %% of each other)")
     ap.add_argument("--serve-n", type=int, default=300, help="requests in the config-5 stream (0 = skip)")
     ap.add_argument("--no-batch", action="store_true", help="skip the config-4 batched-patch measurement")
     return ap.parse_args()
@@ -399,11 +401,15 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
     nphi = nb ** 3 * (bs + 2) ** 3
     out = {"workload": "levelpoisson%d_box%d_gsrb22" % (n, bs), "levels": nlev + 1}
     sols = {}
-    for name, prec, fused in (("fp64_two_colour", 0, 0), ("fp64", 0, 1), ("fp32_ir", 1, 1)):
+    forms = [("fp64_two_colour", 0, 0, use_graph), ("fp64", 0, 1, use_graph), ("fp32_ir", 1, 1, use_graph)]
+    # the same two solves launched eagerly (use_graph 0) or replayed as one captured graph per cycle
+    forms += [("fp64_graph%d" % (1 - use_graph), 0, 1, 1 - use_graph), ("fp32_ir_graph%d" % (1 - use_graph), 1, 1,
+                                                                          1 - use_graph)]
+    for name, prec, fused, graph in forms:
         opt = pa.MgOptions()
         pa.lib().pm_mg_default_options(ctypes.byref(opt))
         opt.relax, opt.pre, opt.post, opt.bottom, opt.fused = 0, 2, 2, 10, fused
-        opt.precision, opt.max_iter, opt.tol, opt.use_graph = prec, 40, 1e-10, use_graph
+        opt.precision, opt.max_iter, opt.tol, opt.use_graph = prec, 40, 1e-10, graph
         mg = ctypes.c_void_p()
         pa.call("pm_mg_create", ctypes.byref(mg), lay, nlev, 1.0 / n, ctypes.byref(opt))
         phi = DeviceArray.zeros(nphi)

@@ -438,11 +438,13 @@ __device__ __forceinline__ double lane_shift(double edge, double v)
 // which may lie in the neighbouring boxes) are scalar, and rows add
 // compile-time offsets (the ghost widths GP, GR, GO of phi in, rhs and phi
 // out are template arguments).  Arithmetic, order and colouring are k_gsrb's;
-// IN 0 / 1 and OUT as k_gsrb_fused.
+// IN (0 phi, 1 zero guess, 2 phi + the constant prolongation of a coarse level
+// with one ghost layer) and OUT as k_gsrb_fused.
 template<typename T, int RY, int IN, int OUT, int NT, int GP, int GR, int GO>
 __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restrict__ pin, T* __restrict__ pout,
                                                    const T* __restrict__ rhs, Lap<T> c, T lambda, int zc,
-                                                   double* __restrict__ pacc)
+                                                   double* __restrict__ pacc, LayoutDev C,
+                                                   const T* __restrict__ crs)
 {
     constexpr int NR = RY + 4;  // phi rows held: Y0 - 2 .. Y0 + RY + 1
     constexpr int NQ = RY + 2;  // rhs rows: Y0 - 1 .. Y0 + RY
@@ -501,6 +503,31 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
         PB = (long)ib_ * (V) + zo_ + (X) * yb;                                                 \
     }
 
+    // IN == 2: the coarse level (ghost 1, boxes on its box grid).  Fine row i of
+    // class (top / tile / bottom) over coarse row class base + CX ((i - 2) >> 1),
+    // column lane / 2 of the coarse box under this wave's box; the class base of
+    // plane Z adds CXY (Z wrapped / 2).  Every fine box's coarsened region lies
+    // in one coarse box.
+    long CX = 0, CXY = 0, ctm = 0, ct0 = 0, ctp = 0, cmm = 0, cm0 = 0, cmp = 0, cbm = 0, cb0 = 0, cbp = 0;
+    const int dom2 = L.domain[2];
+    if constexpr (IN == 2) {
+        const int cbs0 = C.bs[0], cbs1 = C.bs[1], cbs2 = C.bs[2], cnb0 = C.nb[0], cnb1 = C.nb[1];
+        const int* ctable = C.table;
+        const long cX = cbs0 + 2, cXY = cX * (cbs1 + 2), cvol = cXY * (cbs2 + 2), co0 = 1 + cX + cXY;
+        CX = cX;
+        CXY = cXY;
+        // coarse rows of the classes' first rows (row i = 2 position), as in the fine sweep
+        const int yct = (((Y0 - 2 + dom1) % dom1) >> 1) + 1, ycm = Y0 >> 1, ycb = (((Y0 + RY) % dom1) >> 1) - RY / 2;
+        auto cof = [&](int ys, int zs, int yc) {
+            const int ybx = (by + ys + nb1) % nb1, zbx = (bz + zs + nb2) % nb2;
+            const int cx = (w * 64 / 2) / cbs0, cy = (ybx * bs1 / 2) / cbs1, cz = (zbx * bs2 / 2) / cbs2;
+            const int cid = ctable[cx + cnb0 * (cy + cnb1 * cz)];
+            return (long)cid * cvol + co0 + (w * 32 - cx * cbs0) + cX * (long)(yc - cy * cbs1) - cXY * (long)(cz * cbs2);
+        };
+        ctm = cof(ytop, -1, yct), ct0 = cof(ytop, 0, yct), ctp = cof(ytop, 1, yct);
+        cmm = cof(0, -1, ycm), cm0 = cof(0, 0, ycm), cmp = cof(0, 1, ycm);
+        cbm = cof(ybot, -1, ycb), cb0 = cof(ybot, 0, ycb), cbp = cof(ybot, 1, ycb);
+    }
     T P[5][NR];  // phi planes by slot: plane Z0 - 2 + s + 5 m in slot s
     T Q[5][NQ];  // rhs planes, same slots
     auto loadP = [&](T (&d)[NR], int Z) {
@@ -512,6 +539,33 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
             PM_ROW_BASES(XP, XYP, VP, OP, Z, ot, om, ob)
 #pragma unroll
             for (int i = 0; i < NR; i++) { d[i] = pin[(i < 2 ? ot : (i < RY + 2 ? om : ob)) + XP * (i - 2) + lane]; }
+            if constexpr (IN == 2) {
+                // plus the constant prolongation of the coarse correction, rounded as
+                // k_prolong_const_box rounds it: the cells fineInterp would leave
+                const int zw = Z < 0 ? Z + dom2 : (Z >= dom2 ? Z - dom2 : Z);
+                const long zpart = CXY * (long)(zw >> 1);
+                // uniform branches, not selects (a select chain of these became a
+                // private-memory table)
+                long ct = ct0, cm = cm0, cb = cb0;
+                if (Z - bz * bs2 < 0) {
+                    ct = ctm;
+                    cm = cmm;
+                    cb = cbm;
+                } else if (Z - bz * bs2 >= bs2) {
+                    ct = ctp;
+                    cm = cmp;
+                    cb = cbp;
+                }
+                ct += zpart;
+                cm += zpart;
+                cb += zpart;
+                const T zero = 0;
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    const T v = crs[(i < 2 ? ct : (i < RY + 2 ? cm : cb)) + CX * ((i - 2) >> 1) + (lane >> 1)];
+                    d[i] = d[i] + (zero + v);
+                }
+            }
         }
     };
     auto loadQ = [&](T (&d)[NQ], int Z) {
@@ -1698,19 +1752,22 @@ static int rows_ghosts(const int in_ghost[3], const int out_ghost[3], const int 
 
 template<typename T, int RY>
 static void launch_rows(const pm_layout* L, int in_mode, int ghosts, const T* pi, T* po, const T* rhs,
-                        const Lap<T>& c, T lt, double* acc, hipStream_t stream)
+                        const Lap<T>& c, T lt, double* acc, const pm_layout* C, const T* crs, hipStream_t stream)
 {
     const int zc = rows_zc(L, RY);
     const dim3 grid((unsigned)((L->domain[1] / RY) * (L->domain[2] / zc))), block((unsigned)(64 * L->nb[0]));
-    const LayoutDev D = ldev(L);
+    const LayoutDev D = ldev(L), DC = in_mode == 2 ? ldev(C) : D;
     constexpr int NT = sizeof(T) == 8 ? 1 : 0;  // non-temporal write-out in FP64, as the tile sweep
-#define PM_ROWS(I, O, N, GP, GO) \
-    hipLaunchKernelGGL((k_gsrb_rows<T, RY, I, O, N, GP, 0, GO>), grid, block, 0, stream, D, pi, po, rhs, c, lt, zc, acc)
+#define PM_ROWS(I, O, N, GP, GO)                                                                                  \
+    hipLaunchKernelGGL((k_gsrb_rows<T, RY, I, O, N, GP, 0, GO>), grid, block, 0, stream, D, pi, po, rhs, c, lt, zc, \
+                       acc, DC, crs)
     if (ghosts == 10) {  // phi (ghost 1) -> scratch (ghost 0)
         if (in_mode == 0) {
             PM_ROWS(0, 0, NT, 1, 0);
-        } else {
+        } else if (in_mode == 1) {
             PM_ROWS(1, 0, NT, 1, 0);
+        } else {
+            PM_ROWS(2, 0, NT, 1, 0);
         }
     } else {  // scratch (ghost 0) -> phi (ghost 1), or added into the FP64 solution
         if (acc) {
@@ -1727,7 +1784,7 @@ static void launch_rows(const pm_layout* L, int in_mode, int ghosts, const T* pi
 // the row-register sweep with the coefficients and weight of pm_level_gsrb
 template<typename T>
 static void sweep_rows(const pm_layout* L, int ry, int ghosts, const void* phi_in, void* phi_out, const void* rhs,
-                       double dx, int in_mode, hipStream_t stream, double* acc)
+                       double dx, int in_mode, const pm_layout* C, const void* coarse, hipStream_t stream, double* acc)
 {
     double lam = -(dx * dx) / (4.0 * L->dim);
     lam *= 2.0;
@@ -1739,9 +1796,9 @@ static void sweep_rows(const pm_layout* L, int ry, int ghosts, const void* phi_i
     const T* pi = (const T*)(in_mode == 1 ? phi_out : phi_in);
     const T lt = (T)(1.0 * lam);
     if (ry == 4) {
-        launch_rows<T, 4>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, stream);
+        launch_rows<T, 4>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, C, (const T*)coarse, stream);
     } else {
-        launch_rows<T, 8>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, stream);
+        launch_rows<T, 8>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, C, (const T*)coarse, stream);
     }
 }
 
@@ -1786,11 +1843,17 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
         // and 0.501 / 0.724 ms with the tile sweep
         const int ry = g_sweep_form == 3 ? 8 : 4;
         const int ghosts = rows_ghosts(in_ghost, out_ghost, rhs_ghost, in_mode);
-        if (g_sweep_form != 1 && in_mode != 2 && ghosts > 0 && (!acc || ghosts == 1) && rows_ok(L, ry)) {
+        bool coarse_ok = in_mode != 2 || ghosts == 10;
+        if (in_mode == 2) {
+            // the prolongation input reads a coarse level with one ghost layer (checked box-aligned above)
+            const G3 gcc = g3(coarse_ghost);
+            for (int k = 0; k < 3; k++) { coarse_ok = coarse_ok && gcc.g[k] == 1; }
+        }
+        if (g_sweep_form != 1 && coarse_ok && ghosts > 0 && (!acc || ghosts == 1) && rows_ok(L, ry)) {
             if (dtype == PM_F64) {
-                sweep_rows<double>(L, ry, ghosts, phi_in, phi_out, rhs, dx, in_mode, stream, acc);
+                sweep_rows<double>(L, ry, ghosts, phi_in, phi_out, rhs, dx, in_mode, C, coarse, stream, acc);
             } else {
-                sweep_rows<float>(L, ry, ghosts, phi_in, phi_out, rhs, dx, in_mode, stream, acc);
+                sweep_rows<float>(L, ry, ghosts, phi_in, phi_out, rhs, dx, in_mode, C, coarse, stream, acc);
             }
             PM_CHECK_LAUNCH();
             return PM_OK;

@@ -204,11 +204,13 @@ def test_row_sweep_equals_tile_sweep(dt, domain, box):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_row_and_tile_sweeps_same_solve(precision):
-    """Whole solves (zero initial guess folded into the first sweep, FP32 sweeps
-    adding into the FP64 solution) with the row-register sweep and with the
-    tile sweep: the same residual history and solution bits."""
-    n, bs, nlev = 128, 64, 4
+@pytest.mark.parametrize("n,nlev", [(128, 4), (256, 5)])
+def test_row_and_tile_sweeps_same_solve(precision, n, nlev):
+    """Whole solves (zero initial guess and the coarse correction's prolongation
+    folded into sweeps, FP32 sweeps adding into the FP64 solution) with the
+    row-register sweep and with the tile sweep: the same residual history and
+    solution bits.  64^3 boxes: levels 2 and 4 boxes wide, then one box."""
+    bs = 64
     rhs = util.gaussian_rhs(n, 5, 8)
     res = {}
     try:
