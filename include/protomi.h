@@ -257,8 +257,12 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
                         pm_stream_t stream);
 /* Which kernel runs the one-pass sweep here and in the multigrid driver
  * (process-wide; tests and benchmarks compare them): 0 automatic, 1 the LDS
- * tile kernel, 2 / 3 the row-register kernel with 4 / 8 rows per thread. */
+ * tile kernel, 2 / 3 / 4 the row-register kernel with 4 / 8 / 2 rows per
+ * thread.  pm_level_sweep_tune: the row kernel's largest planes per workgroup
+ * (0 automatic) and non-temporal write-out (-1 automatic: FP64 only, 0 off,
+ * 1 on); every setting gives the same bits. */
 int pm_level_sweep_form(int form);
+int pm_level_sweep_tune(int planes_max, int nt_store);
 /* Damped point Jacobi of examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:99-123:
  * tmp = L(phi, 1/(4 DIM)) + (-(dx*dx)/(4 DIM)) rhs on all valid cells, then
  * phi += tmp.  tmp: level data without ghosts. */

@@ -179,6 +179,7 @@ SIGNATURES = {
     "pm_server_restore": (_i, [_vp, _vp, _sz, _vp]),
     "pm_level_gsrb_sweep": (_i, [_vp, _i, _vp, _ip, _vp, _ip, _vp, _ip, _d, _vp]),
     "pm_level_sweep_form": (_i, [_i]),
+    "pm_level_sweep_tune": (_i, [_i, _i]),
     "pm_fas_create": (_i, [ctypes.POINTER(_vp), _vp, _i, _d]),
     "pm_fas_destroy": (_i, [_vp]),
     "pm_fas_bytes": (_i, [_vp, ctypes.POINTER(_sz)]),

@@ -449,8 +449,10 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
     constexpr int NR = RY + 4;  // phi rows held: Y0 - 2 .. Y0 + RY + 1
     constexpr int NQ = RY + 2;  // rhs rows: Y0 - 1 .. Y0 + RY
     constexpr long XP = 64 + 2 * GP, XR = 64 + 2 * GR, XO = 64 + 2 * GO;
-    __shared__ T eo[2][8][2][NQ];  // old values of a plane's edge lanes (0, 63), rows Y0 - 1 .. Y0 + RY
-    __shared__ T er[2][8][2][RY];  // red-updated edge values, rows Y0 .. Y0 + RY - 1
+    using TO = typename std::conditional<OUT == 1, double, T>::type;  // what the output array holds
+    __shared__ T eo[2][8][2][NQ];   // old values of a plane's edge lanes (0, 63), rows Y0 - 1 .. Y0 + RY
+    __shared__ T er[2][8][2][RY];   // red-updated edge values, rows Y0 .. Y0 + RY - 1
+    __shared__ TO ef[2][8][2][GO == 1 ? RY : 1];  // final output values of the edge lanes (GO == 1)
     // every argument the loops use, as scalars (a reference to a by-value
     // kernel argument would copy it to private memory)
     const int lane = threadIdx.x & 63;
@@ -606,11 +608,36 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
         s = s + f;
         return cur + lambda * s;
     };
+    // GO == 1: after a plane is written, the x ghost cells of its rows receive
+    // the periodic images (the neighbouring boxes' edge values, through LDS: what
+    // an exchange would copy there), so that every 128-byte line of a row is
+    // written whole and the partial-line writes at the row ends merge in L2
+    auto ghost_cols = [&](int k) {
+        if constexpr (GO == 1) {
+            if (edge) {
+                long ot, om, ob;
+                PM_ROW_BASES(XO, XYO, VO, OO, k, ot, om, ob)
+                (void)ot;
+                (void)ob;
+#pragma unroll
+                for (int r = 0; r < RY; r++) {
+                    const long o = om + XO * r + (side ? 64 : -1);
+                    const TO v = side ? ef[k & 1][wr][0][r] : ef[k & 1][wl][1][r];
+                    if constexpr (OUT == 1) {
+                        pacc[o] = v;
+                    } else {
+                        pout[o] = v;
+                    }
+                }
+            }
+        }
+    };
     // the step of plane k with slot S of plane k (k - (Z0 - 2) = S mod 5)
     auto step = [&](auto sc, int k) __attribute__((always_inline)) {
         constexpr int S = decltype(sc)::value;
         constexpr int S0 = S, S1 = (S + 1) % 5, S2 = (S + 2) % 5, S3 = (S + 3) % 5, SM = (S + 4) % 5;
         __syncthreads();
+        if (k - 1 >= Z0) { ghost_cols(k - 1); }
         // the next planes (clamped to the chunk's last halo planes: a valid address)
         loadP(P[S3], min(k + 3, Z1 + 1));
         loadQ(Q[S2], min(k + 2, Z1));
@@ -644,11 +671,16 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
             for (int r = 0; r < RY; r++) {
                 const long o = om + XO * r + lane;
                 if constexpr (OUT == 1) {
-                    pacc[o] = pacc[o] + (double)P[S0][r + 2];
-                } else if constexpr (NT) {
-                    __builtin_nontemporal_store(P[S0][r + 2], &pout[o]);
+                    const double v = pacc[o] + (double)P[S0][r + 2];
+                    pacc[o] = v;
+                    if (GO == 1 && edge) { ef[k & 1][w][side][r] = v; }
                 } else {
-                    pout[o] = P[S0][r + 2];
+                    if constexpr (NT && GO == 0) {
+                        __builtin_nontemporal_store(P[S0][r + 2], &pout[o]);
+                    } else {
+                        pout[o] = P[S0][r + 2];
+                    }
+                    if (GO == 1 && edge) { ef[k & 1][w][side][r] = P[S0][r + 2]; }
                 }
             }
         }
@@ -675,6 +707,8 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
         step(std::integral_constant<int, 4>{}, k);
         if (++k >= Z1) { break; }
     }
+    __syncthreads();
+    ghost_cols(Z1 - 1);
 #undef PM_ROW_BASES
 }
 
@@ -1712,7 +1746,7 @@ namespace pm {
 
 // Sweep kernel of the one-pass sweep (tests and benchmarks compare them): 0
 // automatic (the row-register sweep where the level qualifies), 1 the tile
-// sweep, 2 / 3 the row-register sweep with 4 / 8 rows per thread.
+// sweep, 2 / 3 / 4 the row-register sweep with 4 / 8 / 2 rows per thread.
 static int g_sweep_form = 0;
 
 // the row-register sweep takes boxes 64 cells wide, at most 8 across the
@@ -1724,10 +1758,15 @@ static bool rows_ok(const pm_layout* L, int ry)
 
 // planes per workgroup of the row sweep: a divisor of the box depth, small
 // enough for about 8192 waves over the level
+// tuning overrides (pm_level_sweep_tune): largest planes per workgroup (0:
+// automatic), non-temporal write-out (-1: automatic, FP64 only)
+static int g_rows_zc_max = 0, g_rows_nt = -1;
+
 static int rows_zc(const pm_layout* L, int ry)
 {
     int zc = L->bs[2];
     while (zc % 2 == 0 && zc > 8 && (long)(L->domain[1] / ry) * (L->domain[2] / zc) * L->nb[0] < 8192) { zc /= 2; }
+    while (g_rows_zc_max > 0 && zc > g_rows_zc_max && zc % 2 == 0) { zc /= 2; }
     return zc;
 }
 
@@ -1757,27 +1796,38 @@ static void launch_rows(const pm_layout* L, int in_mode, int ghosts, const T* pi
     const int zc = rows_zc(L, RY);
     const dim3 grid((unsigned)((L->domain[1] / RY) * (L->domain[2] / zc))), block((unsigned)(64 * L->nb[0]));
     const LayoutDev D = ldev(L), DC = in_mode == 2 ? ldev(C) : D;
-    constexpr int NT = sizeof(T) == 8 ? 1 : 0;  // non-temporal write-out in FP64, as the tile sweep
+    // non-temporal write-out where the output has no ghost layer (512^3 FP32
+    // sweep 0.385 vs 0.392 ms, FP64 0.672 vs 0.691 ms); into ghost-1 arrays
+    // the stores stay cached, so that the row-end ghost cells written a step
+    // later complete the same lines in L2
+    const bool nt = g_rows_nt < 0 ? true : g_rows_nt == 1;
 #define PM_ROWS(I, O, N, GP, GO)                                                                                  \
     hipLaunchKernelGGL((k_gsrb_rows<T, RY, I, O, N, GP, 0, GO>), grid, block, 0, stream, D, pi, po, rhs, c, lt, zc, \
                        acc, DC, crs)
+#define PM_ROWS_NT(I, O, GP, GO) \
+    if (nt) {                    \
+        PM_ROWS(I, O, 1, GP, GO); \
+    } else {                     \
+        PM_ROWS(I, O, 0, GP, GO); \
+    }
     if (ghosts == 10) {  // phi (ghost 1) -> scratch (ghost 0)
         if (in_mode == 0) {
-            PM_ROWS(0, 0, NT, 1, 0);
+            PM_ROWS_NT(0, 0, 1, 0);
         } else if (in_mode == 1) {
-            PM_ROWS(1, 0, NT, 1, 0);
+            PM_ROWS_NT(1, 0, 1, 0);
         } else {
-            PM_ROWS(2, 0, NT, 1, 0);
+            PM_ROWS_NT(2, 0, 1, 0);
         }
     } else {  // scratch (ghost 0) -> phi (ghost 1), or added into the FP64 solution
         if (acc) {
             PM_ROWS(0, 1, 0, 0, 1);
         } else if (in_mode == 0) {
-            PM_ROWS(0, 0, NT, 0, 1);
+            PM_ROWS_NT(0, 0, 0, 1);
         } else {
-            PM_ROWS(1, 0, NT, 0, 1);
+            PM_ROWS_NT(1, 0, 0, 1);
         }
     }
+#undef PM_ROWS_NT
 #undef PM_ROWS
 }
 
@@ -1797,6 +1847,8 @@ static void sweep_rows(const pm_layout* L, int ry, int ghosts, const void* phi_i
     const T lt = (T)(1.0 * lam);
     if (ry == 4) {
         launch_rows<T, 4>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, C, (const T*)coarse, stream);
+    } else if (ry == 2) {
+        launch_rows<T, 2>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, C, (const T*)coarse, stream);
     } else {
         launch_rows<T, 8>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, C, (const T*)coarse, stream);
     }
@@ -1841,7 +1893,7 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
         // 4 rows per thread: 512^3 in 64^3 boxes swept in 0.389 ms (FP32) and
         // 0.683 ms (FP64) against 0.411 / 1.61 ms with 8 rows (FP64 spills)
         // and 0.501 / 0.724 ms with the tile sweep
-        const int ry = g_sweep_form == 3 ? 8 : 4;
+        const int ry = g_sweep_form == 3 ? 8 : (g_sweep_form == 4 ? 2 : 4);
         const int ghosts = rows_ghosts(in_ghost, out_ghost, rhs_ghost, in_mode);
         bool coarse_ok = in_mode != 2 || ghosts == 10;
         if (in_mode == 2) {
@@ -1963,8 +2015,16 @@ extern "C" {
 
 int pm_level_sweep_form(int form)
 {
-    PM_REQUIRE(form >= 0 && form <= 3, "pm_level_sweep_form: form must be 0..3");
+    PM_REQUIRE(form >= 0 && form <= 4, "pm_level_sweep_form: form must be 0..4");
     g_sweep_form = form;
+    return PM_OK;
+}
+
+int pm_level_sweep_tune(int planes_max, int nt_store)
+{
+    PM_REQUIRE(planes_max >= 0 && nt_store >= -1 && nt_store <= 1, "pm_level_sweep_tune: bad argument");
+    g_rows_zc_max = planes_max;
+    g_rows_nt = nt_store;
     return PM_OK;
 }
 

@@ -196,11 +196,20 @@ def test_row_sweep_equals_tile_sweep(dt, domain, box):
                 pa.call("pm_level_gsrb_sweep", h, dt, vp(a), G1, vp(tmp), G0, vp(rh), G0, dx, None)
                 pa.call("pm_level_gsrb_sweep", h, dt, vp(tmp), G0, vp(a), G1, vp(rh), G0, dx, None)
             pa.call("pm_level_gsrb_sweep", h, dt, vp(a), G1, vp(tmp), G0, vp(rh), G0, dx, None)
-            outs[form] = (host(a).tobytes(), host(tmp).tobytes())
+            outs[form] = (host(a).reshape(nbox, box[2] + 2, box[1] + 2, box[0] + 2), host(tmp))
+            if form == 1:  # the tile sweep's result with its ghost cells exchanged
+                pa.call("pm_level_exchange", h, dt, vp(a), G1, 1, None)
+                exch = host(a).reshape(nbox, box[2] + 2, box[1] + 2, box[0] + 2)
     finally:
         pa.call("pm_level_sweep_form", 0)
         pa.lib().pm_layout_destroy(h)
-    assert outs[2] == outs[1] and outs[3] == outs[1]
+    valid = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    xghost = (slice(None), slice(1, -1), slice(1, -1), [0, -1])
+    for form in (2, 3):
+        assert outs[form][1].tobytes() == outs[1][1].tobytes()
+        assert outs[form][0][valid].tobytes() == outs[1][0][valid].tobytes()
+        # the row sweep also writes the x ghost cells of ghost-1 outputs: the periodic images
+        assert outs[form][0][xghost].tobytes() == exch[xghost].tobytes()
 
 
 @pytest.mark.parametrize("precision", [0, 1])
