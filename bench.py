@@ -356,11 +356,17 @@ def config1_2d(pa, stream, reps=200):
     sb, db = pa.Box.of([-1, -1], [n, n]), pa.Box.of([0, 0], [n - 1, n - 1])
     sp = stream.h
 
-    def step():
+    def step_separate():
         pa.call("pm_stencil_apply", st.h, u.ptr, ctypes.byref(sb), dst.ptr, ctypes.byref(db), ctypes.byref(db), 1, 1,
                 1.0 / (h * h), sp)
         pa.call("pm_reduce", pa.RED_ABS, pa.PM_F64, dst.ptr, n * n, out.ptr, 0, sp)
 
+    def step():
+        # the apply and absMax of the cells it writes in one pass (+ a one-block finish)
+        pa.call("pm_stencil_apply_reduce", st.h, u.ptr, ctypes.byref(sb), dst.ptr, ctypes.byref(db), ctypes.byref(db),
+                1, 1.0 / (h * h), pa.RED_ABS, out.ptr, 0, sp)
+
+    us_separate = 1e3 * time_launches(step_separate, reps, stream)
     us_stream = 1e3 * time_launches(step, reps, stream)
     gr = ctypes.c_void_p()
     pa.call("pm_graph_begin", sp)
@@ -379,6 +385,7 @@ def config1_2d(pa, stream, reps=200):
         d.free()
     return {"workload": "laplacian2d_256_absmax", "us_per_step_stream": round(us_stream, 2),
             "us_per_step_graph": round(us_graph, 2), "us_latency_p50": round(1e6 * float(np.median(lat)), 2),
+            "us_per_step_apply_then_reduce": round(us_separate, 2),
             "absmax": amax.hex(), "bytes": 8 * (258 * 258 + 2 * 256 * 256)}
 
 
@@ -875,9 +882,19 @@ def main():
                 sweep["%s_bpc%d" % (kname, bpc)] = round(nb4 / (ms * 1e-3) / 1e9, 1)
         best = max(sweep, key=sweep.get)
         l7 = extra["apply_L7_f64"]["GB/s"]
+        # read-only streams over the same 4 GiB at four grid sizes: the fastest
+        # traffic HBM gives, which bounds any mix with writes
+        rsweep = {}
+        for bpc in (2, 4, 8, 16, 32):
+            ms = time_launches(lambda: pa.call("pm_stream_probe", 0, big2.ptr, big.ptr, nb4, bpc, sp), 10, stream)
+            rsweep["read_bpc%d" % bpc] = round(nb4 / (ms * 1e-3) / 1e9, 1)
+        rbest = max(rsweep, key=rsweep.get)
         extra["copy_sweep"] = {"GB/s": sweep, "best": best, "best_GB/s": sweep[best],
                                "best_frac": round(sweep[best] / PEAK_HBM_GBS, 4),
-                               "apply_L7_f64_vs_best_copy": round(l7 / sweep[best], 4)}
+                               "apply_L7_f64_vs_best_copy": round(l7 / sweep[best], 4),
+                               "read_only_GB/s": rsweep, "best_read": rbest, "best_read_GB/s": rsweep[rbest],
+                               "best_read_frac": round(rsweep[rbest] / PEAK_HBM_GBS, 4),
+                               "apply_L7_f64_vs_best_read": round(l7 / rsweep[rbest], 4)}
         big.free()
         big2.free()
         a.free()

@@ -128,6 +128,18 @@ int pm_stencil_apply(const pm_stencil* st, const void* src, const pm_box* src_bo
                      const pm_box* dst_box, const pm_box* idx_box, int ncomp, int replace,
                      double scale, pm_stream_t stream);
 
+/* Stencil::apply followed by a Reduction of the values it wrote, in one
+ * pass where the generic z-march serves the stencil (kind 1, 2-D included):
+ * config 1's  dst |= L(u, scale); dst.absMax()  as one kernel plus a
+ * one-block finish (Proto_StencilImplem.H:412-431, Proto_BoxDataImplem.H:
+ * 1375-1414).  out_dev receives op over the written cells (combined with its
+ * old value when combine != 0); fixed tree, so the bits repeat run to run.
+ * Other families run pm_stencil_apply then pm_reduce_box over the same cells
+ * (destination-refined stencils are refused: use the two calls).  ncomp 1. */
+int pm_stencil_apply_reduce(const pm_stencil* st, const void* src, const pm_box* src_box, void* dst,
+                            const pm_box* dst_box, const pm_box* idx_box, int replace, double scale, int op,
+                            void* out_dev, int combine, pm_stream_t stream);
+
 /* Fused form of   Lu |= L(u, scale);  r = forall(f - Lu);  r.absMax(); r.sum()
  * (config-2 program: Proto_StencilImplem.H:456-470, Proto_ForallImplem.H:195-213,
  * Proto_BoxDataImplem.H:1375-1414).  r, f share r_box; u covers the stencil's

@@ -114,6 +114,7 @@ SIGNATURES = {
     "pm_stencil_kind": (_i, [_vp, _ip]),
     "pm_stencil_set_kind": (_i, [_vp, _i]),
     "pm_stencil_apply": (_i, [_vp, _vp, _BP, _vp, _BP, _BP, _i, _i, _d, _vp]),
+    "pm_stencil_apply_reduce": (_i, [_vp, _vp, _BP, _vp, _BP, _BP, _i, _d, _i, _vp, _i, _vp]),
     "pm_stencil_residual": (_i, [_vp, _vp, _BP, _vp, _vp, _BP, _d, _vp, _vp]),
     "pm_stencil_residual2": (_i, [_vp, _vp, _vp, _BP, _vp, _vp, _vp, _BP, _d, _d, _vp, _vp]),
     "pm_stencil_residual2_noalloc": (_i, [_vp, _vp, _vp, _BP, _vp, _vp, _vp, _BP, _d, _d, _vp, _vp]),

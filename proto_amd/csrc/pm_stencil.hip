@@ -120,6 +120,7 @@ struct ZArgs {
     T c2[27];       // second operator (ZM_RESIDUAL2, always the 27-point shape)
     int ntap;       // k_zgen: taps in the stencil's order, (dx+1) | (dy+1) << 8 | (dz+1) << 16
     int tap[27];
+    int op;         // k_zgen with RED: reduction operator of the written values (partials per block)
 };
 
 template<int NT, typename T>
@@ -452,13 +453,15 @@ __global__ __launch_bounds__(256) void k_zmarch(ZArgs<T> a)
 // does.  Loads are bounds-checked against the source box (a stencil without
 // z taps, e.g. 2-D, never reads the planes above and below).  PLANAR: every
 // tap has dz = 0 (2-D data, one plane): a single LDS plane.
-template<typename T, int MODE, int RY, int NT, int PLANAR>
+template<typename T, int MODE, int RY, int NT, int PLANAR, int RED>
 __global__ __launch_bounds__(256) void k_zgen(ZArgs<T> a)
 {
     using G = ZGeom<RY>;
     constexpr int NL = G::NL, LX = G::LX, NE = G::NE;
     constexpr int NS = PLANAR ? 1 : 4;
     __shared__ T sm[NS][G::NP];
+    __shared__ T red[16];
+    T rv = RED ? red_init<T>(a.op) : (T)0;  // RED: this thread's fold of the values it writes
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     unsigned b = xcd_remap(blockIdx.x, gridDim.x);
@@ -517,7 +520,17 @@ __global__ __launch_bounds__(256) void k_zgen(ZArgs<T> a)
         }
 #pragma unroll
         for (int o = 0; o < RY; o++) {
-            if (ok[o]) { put<NT>(&a.dst[box_index(ob, gx, y0 + w * RY + o, z)], acc[o]); }
+            if (ok[o]) {
+                put<NT>(&a.dst[box_index(ob, gx, y0 + w * RY + o, z)], acc[o]);
+                if (RED) { rv = red_update(a.op, rv, acc[o]); }
+            }
+        }
+    };
+    // RED: one partial per block, a fixed tree (deterministic for the launch shape)
+    auto finish = [&]() {
+        if (RED) {
+            const T v = block_reduce(a.op, rv, red);
+            if (tid == 0) { a.partials[blockIdx.x] = v; }
         }
     };
     if (PLANAR) {
@@ -528,6 +541,7 @@ __global__ __launch_bounds__(256) void k_zgen(ZArgs<T> a)
             __syncthreads();
             compute(z);
         }
+        finish();
         return;
     }
     // ring of 4 planes: step z commits plane z + 1 into the slot of plane z - 3
@@ -543,6 +557,19 @@ __global__ __launch_bounds__(256) void k_zgen(ZArgs<T> a)
         __syncthreads();
         compute(z);
     }
+    finish();
+}
+
+// The one-block second stage of k_zgen's reductions (fixed strided split and tree).
+template<typename T>
+__global__ __launch_bounds__(1024) void k_finish_op(int op, const T* __restrict__ partials, int nb, T* out,
+                                                    int combine)
+{
+    __shared__ T red[16];
+    T v = red_init<T>(op);
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) { v = red_combine(op, v, partials[i]); }
+    v = block_reduce(op, v, red);
+    if (threadIdx.x == 0) { *out = combine ? red_combine(op, *out, v) : v; }
 }
 
 // Deterministic second stage over per-block {amax, sum} groups of PER values
@@ -739,22 +766,28 @@ template<typename T, int MODE>
 using ZGKernel = void (*)(ZArgs<T>);
 
 template<typename T, int MODE>
-static ZGKernel<T, MODE> zgkernel(const ZCfg& c, bool planar)
+static ZGKernel<T, MODE> zgkernel(const ZCfg& c, bool planar, bool red)
 {
-#define PM_ZG(RY, NT)                                                                  \
-    if (c.ry == RY && c.nt == NT) {                                                    \
-        return planar ? k_zgen<T, MODE, RY, NT, 1> : k_zgen<T, MODE, RY, NT, 0>;       \
+#define PM_ZG(RY, NT)                                                                                    \
+    if (c.ry == RY && c.nt == NT) {                                                                      \
+        if (red) { return planar ? k_zgen<T, MODE, RY, NT, 1, 1> : k_zgen<T, MODE, RY, NT, 0, 1>; }      \
+        return planar ? k_zgen<T, MODE, RY, NT, 1, 0> : k_zgen<T, MODE, RY, NT, 0, 0>;                   \
     }
     PM_ZG(1, 0) PM_ZG(1, 1) PM_ZG(2, 0) PM_ZG(2, 1) PM_ZG(4, 0) PM_ZG(4, 1)
 #undef PM_ZG
     return nullptr;
 }
 
+// Workgroups k_zgen launches for ibox with configuration c.
+static unsigned zgen_blocks(const Box3& ibox, const ZCfg& c) { return zplan(ibox, c).blocks; }
+
+// red: a.partials receives one partial of a.op per workgroup (zgen_blocks)
 template<typename T, int MODE>
-static int launch_zgen(const pm_stencil* st, ZArgs<T>& a, const Box3& ibox, const ZCfg& c, hipStream_t s)
+static int launch_zgen(const pm_stencil* st, ZArgs<T>& a, const Box3& ibox, const ZCfg& c, hipStream_t s,
+                       bool red = false)
 {
     const bool planar = zgen_planar(st);
-    ZGKernel<T, MODE> k = zgkernel<T, MODE>(c, planar);
+    ZGKernel<T, MODE> k = zgkernel<T, MODE>(c, planar, red);
     PM_REQUIRE(k != nullptr, "no generic z-march kernel for rows %d, nt %d", c.ry, c.nt);
     ZPlan p = zplan(ibox, c);
     a.ntx = p.ntx;
@@ -1158,6 +1191,62 @@ int pm_stencil_apply(const pm_stencil* st, const void* src, const pm_box* src_bo
                ib.lo[0], ib.lo[1], ib.lo[2], ib.hi[0], ib.hi[1], ib.hi[2]);
     if (st->dtype == PM_F64) { return apply_t<double>(st, src, sb, dst, db, ib, ncomp, replace, scale, S(stream)); }
     return apply_t<float>(st, src, sb, dst, db, ib, ncomp, replace, scale, S(stream));
+}
+
+int pm_stencil_apply_reduce(const pm_stencil* st, const void* src, const pm_box* src_box, void* dst,
+                            const pm_box* dst_box, const pm_box* idx_box, int replace, double scale, int op,
+                            void* out_dev, int combine, pm_stream_t stream)
+{
+    PM_REQUIRE(st && src_box && dst_box && idx_box && out_dev, "pm_stencil_apply_reduce: null argument");
+    PM_REQUIRE(op >= PM_RED_MAX && op <= PM_RED_SUMSQ, "pm_stencil_apply_reduce: bad op %d", op);
+    const Box3 sb = box3(src_box), db = box3(dst_box), ib = box3(idx_box);
+    PM_REQUIRE(!box_empty(ib) && st->ncoef > 0, "pm_stencil_apply_reduce: empty index box or stencil");
+    PM_REQUIRE(src && dst, "pm_stencil_apply_reduce: null data pointer");
+    PM_REQUIRE(covers(st, sb, db, ib), "pm_stencil_apply_reduce: index box reads or writes outside the boxes");
+    const hipStream_t s = S(stream);
+    const bool f64 = st->dtype == PM_F64;
+    const double sc_t = f64 ? scale : (double)(float)scale;
+    if (st->kind == 1 && sc_t != 0.0) {
+        // one pass: the generic z-march writes dst and a partial per workgroup
+        const ZCfg cfg = lookup_cfg(1, replace ? ZM_REPLACE : ZM_ACCUM, st->dtype, ib);
+        const unsigned nb = zgen_blocks(ib, cfg);
+        void* ws = nullptr;
+        int rc = workspace(s, (f64 ? 8 : 4) * (size_t)nb, &ws);
+        if (rc) { return rc; }
+        auto run = [&](auto tag) -> int {
+            using T = decltype(tag);
+            ZArgs<T> a;
+            std::memset(&a, 0, sizeof(a));
+            a.src = (const T*)src;
+            a.dst = (T*)dst;
+            a.sbox = sb;
+            a.dbox = db;
+            a.partials = (T*)ws;
+            a.op = op;
+            scaled_coefs<T>(st, scale, a.c);
+            int r2 = replace ? launch_zgen<T, ZM_REPLACE>(st, a, ib, cfg, s, true)
+                             : launch_zgen<T, ZM_ACCUM>(st, a, ib, cfg, s, true);
+            if (r2) { return r2; }
+            hipLaunchKernelGGL(k_finish_op<T>, dim3(1), dim3(1024), 0, s, op, (const T*)ws, (int)nb, (T*)out_dev,
+                               combine);
+            PM_CHECK_LAUNCH();
+            return PM_OK;
+        };
+        return f64 ? run(0.0) : run(0.0f);
+    }
+    // other kernel families: the apply, then the reduction over the cells it wrote
+    int rc = pm_stencil_apply(st, src, src_box, dst, dst_box, idx_box, 1, replace, scale, stream);
+    if (rc) { return rc; }
+    pm_box wr;
+    for (int d = 0; d < 3; d++) {
+        wr.lo[d] = ib.lo[d] * st->dr[d] + st->ds[d];
+        wr.hi[d] = ib.hi[d] * st->dr[d] + st->ds[d];
+    }
+    bool dense = true;
+    for (int d = 0; d < 3; d++) { dense = dense && (st->dr[d] == 1 || ib.lo[d] == ib.hi[d]); }
+    PM_REQUIRE(dense, "pm_stencil_apply_reduce: destination-refined stencils write a strided set; use "
+                      "pm_stencil_apply and pm_reduce");
+    return pm_reduce_box(op, st->dtype, dst, dst_box, &wr, 0, out_dev, combine, stream);
 }
 
 int pm_stencil_residual(const pm_stencil* st, const void* u, const pm_box* u_box, const void* f, void* r,

@@ -139,6 +139,44 @@ def test_kernel_families_agree(name):
         pa.Stencil(c, o).set_kind(27 if name == "Laplacian" else 7)
 
 
+@pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
+def test_apply_reduce(dtype):
+    """pm_stencil_apply_reduce: dst as pm_stencil_apply writes it; Max, Min
+    and Abs of the written cells exact, Sum within the stated bound; 3-D and
+    2-D generic stencils (one pass), a 7-point one (apply, then reduce)."""
+    rng = np.random.default_rng(5)
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    c2 = np.array([-4.0, 1, 1, 1, 1], dtype=npdt)
+    o2 = [[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0]]
+    c19, o19, _, _, _ = util.lib_stencil("Laplacian_19", dtype=npdt)
+    c7, o7, _, _, _ = util.lib_stencil("Laplacian", dtype=npdt)
+    cases = [(c2, o2, 2, [3, -4, 0], [130, 70, 0]), (c19, o19, 3, [-5, 2, 1], [60, 41, 37]),
+             (c7, o7, 3, [0, 0, 0], [47, 33, 29])]
+    for ci, (c, o, dim, lo, hi) in enumerate(cases):
+        st = pa.Stencil(c, o, dtype, dim)
+        lo, hi = np.array(lo), np.array(hi)
+        g = np.array([1, 1, 1 if dim == 3 else 0])
+        slo, shi = lo - g, hi + g
+        src = util.uniform(60 + ci, 0, util.box_size(slo, shi)).astype(npdt)
+        ref = np.zeros(util.box_size(lo, hi), dtype=npdt)
+        util.oracle_apply(c, o, src, slo, shi, ref, lo, hi, lo, hi, True, 256.0)
+        for op in (pa.RED_MAX, pa.RED_MIN, pa.RED_ABS, pa.RED_SUM):
+            s_d, d_d, out = dev(src), zeros(ref.size, npdt), zeros(1, npdt)
+            say("apply_reduce", dtype, ci, op)
+            pa.call("pm_stencil_apply_reduce", st.h, vp(s_d), ctypes.byref(box(slo, shi)), vp(d_d),
+                    ctypes.byref(box(lo, hi)), ctypes.byref(box(lo, hi)), 1, 256.0, op, vp(out), 0, None)
+            assert host(d_d).tobytes() == ref.tobytes(), (ci, op)
+            got = host(out)[0]
+            if op == pa.RED_MAX:
+                assert got == ref.max()
+            elif op == pa.RED_MIN:
+                assert got == ref.min()
+            elif op == pa.RED_ABS:
+                assert got == np.abs(ref).max()
+            else:
+                assert abs(float(got) - float(ref.astype(np.float64).sum())) <= _sum_bound(ref)
+
+
 def test_multicomponent_and_generic_ratios():
     # 3 components, AvgDown (srcRatio 2) and a destination-refined stencil
     c, o, sr, dr, ds = util.lib_stencil("AvgDown", 2)
