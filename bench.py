@@ -709,8 +709,9 @@ def main():
             rh = DeviceArray.zeros(nb ** 3 * bs ** 3, t)
             pa.call("pm_fill_uniform", dt, ph.ptr, nb ** 3 * (bs + 2) ** 3, 5, 0, -1.0, 1.0, sp)
             pa.call("pm_fill_uniform", dt, rh.ptr, nb ** 3 * bs ** 3, 5, 2, -1.0, 1.0, sp)
-            for _ in range(args.steps):
+            for _ in range(args.steps):  # phi (ghost 1) -> scratch (ghost 0) -> phi, as the solver alternates
                 pa.call("pm_level_gsrb_sweep", lay, dt, ph.ptr, g1, tm.ptr, g0, rh.ptr, g0, 1.0 / n, sp)
+                pa.call("pm_level_gsrb_sweep", lay, dt, tm.ptr, g0, ph.ptr, g1, rh.ptr, g0, 1.0 / n, sp)
             stream.sync()
             for d in (ph, tm, rh):
                 d.free()
