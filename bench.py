@@ -374,6 +374,20 @@ def config1_2d(pa, stream, reps=200):
     pa.call("pm_graph_end", sp, ctypes.byref(gr))
     us_graph = 1e3 * time_launches(lambda: pa.call("pm_graph_launch", gr, sp), reps, stream)
     pa.call("pm_graph_destroy", gr)
+    # the runtime's own floor: two one-element kernels per step, launched on the
+    # stream and replayed as a two-node graph
+    tiny = DeviceArray(1)
+
+    def two_tiny():
+        for k in range(2):
+            pa.call("pm_fill_uniform", pa.PM_F64, tiny.ptr, 1, 9, k, -1.0, 1.0, sp)
+    us_floor_stream = 1e3 * time_launches(two_tiny, reps, stream)
+    pa.call("pm_graph_begin", sp)
+    two_tiny()
+    pa.call("pm_graph_end", sp, ctypes.byref(gr))
+    us_floor_graph = 1e3 * time_launches(lambda: pa.call("pm_graph_launch", gr, sp), reps, stream)
+    pa.call("pm_graph_destroy", gr)
+    tiny.free()
     lat = []
     for _ in range(50):
         t0 = time.perf_counter()
@@ -386,6 +400,7 @@ def config1_2d(pa, stream, reps=200):
     return {"workload": "laplacian2d_256_absmax", "us_per_step_stream": round(us_stream, 2),
             "us_per_step_graph": round(us_graph, 2), "us_latency_p50": round(1e6 * float(np.median(lat)), 2),
             "us_per_step_apply_then_reduce": round(us_separate, 2),
+            "us_two_1elem_kernels_stream": round(us_floor_stream, 2), "us_two_1elem_kernels_graph": round(us_floor_graph, 2),
             "absmax": amax.hex(), "bytes": 8 * (258 * 258 + 2 * 256 * 256)}
 
 
@@ -807,12 +822,18 @@ def main():
         tune("residual_L7_f64", s7, None, 2)
         tune("residual_L27_f64", s27, None, 2)
         apply_bytes = bytes_u + bytes_f
-        for name, st in (("apply_L7_f64", s7), ("apply_L27_f64", s27)):
+        # Stencil<double>::Identity() on the same boxes runs the same z-march kernel
+        # with its one centre tap: the same halo'd planes staged, the same stores,
+        # so its rate is what the 7-point apply's own access pattern gives
+        sid = pa.Stencil(np.array([1.0]), [[0, 0, 0]])
+        tune("apply_identity_f64", sid, None, 0)
+        for name, st in (("apply_L7_f64", s7), ("apply_L27_f64", s27), ("apply_identity_f64", sid)):
             ms = time_launches(lambda: pa.call("pm_stencil_apply", st.h, u.ptr, ctypes.byref(ub), dst.ptr,
                                                ctypes.byref(rb), ctypes.byref(rb), 1, 1, sc, sp), 10, stream)
             gbs = apply_bytes / (ms * 1e-3) / 1e9
             extra[name] = {"ms": round(ms, 4), "GB/s": round(gbs, 1), "frac": round(gbs / PEAK_HBM_GBS, 4),
                            "bytes": apply_bytes}
+        extra["apply_L7_f64"]["vs_identity"] = round(extra["apply_identity_f64"]["ms"] / extra["apply_L7_f64"]["ms"], 4)
         # stencils outside the two fixed shapes: the generic z-march (taps from an
         # LDS plane ring, any order) against the per-cell kernel it replaced, at
         # the same 512^3 box: Laplacian_19, and the 7-point Laplacian served by

@@ -107,14 +107,16 @@ int pm_stencil_create(pm_stencil** out, int dtype, int dim, int ncoef, const voi
 int pm_stencil_destroy(pm_stencil* st);
 /* Which kernel family serves this stencil (for tests and tools): 7 / 27 the
  * 7- / 27-point z-march (Proto's Laplacian / Laplacian_27 in Proto's order,
- * 3-D), 1 the generic z-march (any other unit-ratio stencil whose offsets lie
- * in [-1,1]^DIM, 2-D included: taps read from an LDS plane ring in the
- * stencil's order), 0 the per-cell kernel (refinement ratios, shifts, wider
- * offsets). */
+ * 3-D), 2 the same z-march with its single centre tap (a 3-D stencil of one
+ * coefficient at the origin: Stencil<T>::Identity(), Proto_Stencil.H:620, and
+ * its multiples; it stages the same halo'd planes as the 7-point kernel), 1
+ * the generic z-march (any other unit-ratio stencil whose offsets lie in
+ * [-1,1]^DIM, 2-D included: taps read from an LDS plane ring in the stencil's
+ * order), 0 the per-cell kernel (refinement ratios, shifts, wider offsets). */
 int pm_stencil_kind(const pm_stencil* st, int* kind);
 /* Serve this stencil with another kernel family that can (tests and
- * benchmarks compare the families): 0 (any stencil), 1 (a 7/27-point one too)
- * or its own kind.  Every family gives the same bits. */
+ * benchmarks compare the families): 0 (any stencil), 1 (a 7/27-point or
+ * one-tap one too) or its own kind.  Every family gives the same bits. */
 int pm_stencil_set_kind(pm_stencil* st, int kind);
 
 /* Stencil::apply -> protoApply<DEVICE> -> sIndexer3::gpu
@@ -129,7 +131,8 @@ int pm_stencil_apply(const pm_stencil* st, const void* src, const pm_box* src_bo
                      double scale, pm_stream_t stream);
 
 /* Stencil::apply followed by a Reduction of the values it wrote, in one
- * pass where the generic z-march serves the stencil (kind 1, 2-D included):
+ * pass where the generic z-march serves the stencil (kinds 1 and 2, 2-D
+ * included):
  * config 1's  dst |= L(u, scale); dst.absMax()  as one kernel plus a
  * one-block finish (Proto_StencilImplem.H:412-431, Proto_BoxDataImplem.H:
  * 1375-1414).  out_dev receives op over the written cells (combined with its
@@ -181,7 +184,8 @@ typedef struct pm_tile_config { int rows; int prefetch; int nt_store; int zplane
  * of mode 3, else NULL), mode (0 apply |=, 1 apply +=, 2 fused residual,
  * 3 fused two-operator residual), dtype and index box on scratch data (the
  * source box src_box), keeps the fastest in the library's table -- later calls
- * with the same key use it -- and returns it with its time.  No counterpart in
+ * with the same key use it -- and returns it with its time.  The z-march
+ * kinds only (7 and 27 in every mode, 2 in modes 0 and 1).  No counterpart in
  * the reference, which has fixed launch shapes (Proto_StencilImplem.H:860-891). */
 int pm_stencil_autotune(const pm_stencil* st, const pm_stencil* st2, int mode, const pm_box* src_box,
                         const pm_box* idx_box, int reps, pm_tile_config* best, float* best_ms,

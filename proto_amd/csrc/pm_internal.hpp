@@ -149,9 +149,11 @@ __host__ __device__ constexpr int lap27_off(int j, int d)
     int v = (d == 0) ? (k % 3) : (d == 1 ? (k / 3) % 3 : k / 9);
     return v - 1;
 }
+// SHAPE 7 / 27: the two Laplacians; SHAPE 1: one tap at the origin
+// (Stencil<T>::Identity(), Proto_Stencil.H:620, and its multiples)
 template<int SHAPE> __host__ __device__ constexpr int shape_off(int j, int d)
 {
-    return SHAPE == 7 ? lap7_off(j, d) : lap27_off(j, d);
+    return SHAPE == 7 ? lap7_off(j, d) : (SHAPE == 27 ? lap27_off(j, d) : 0);
 }
 
 // XCD-aware block remap: MI355X dispatches workgroup i to XCD (i mod 8); this

@@ -44,6 +44,8 @@ static int detect_kind(const pm_stencil* s)
             if (s->offs[3 * j + d] < -1 || s->offs[3 * j + d] > 1) { near = false; }
     const int generic = near ? 1 : 0;
     if (s->dim != 3) { return generic; }
+    // kind 2: one tap at the origin, on the z-march kernel with SHAPE 1
+    if (s->ncoef == 1 && s->offs[0] == 0 && s->offs[1] == 0 && s->offs[2] == 0) { return 2; }
     if (s->ncoef == 7) {
         for (int j = 0; j < 7; j++)
             for (int d = 0; d < 3; d++)
@@ -149,7 +151,7 @@ __device__ __forceinline__ void zm_interior(const ZArgs<T>& a, T (*sm)[ZGeom<RY>
 {
     using G = ZGeom<RY>;
     constexpr int NL = G::NL, LX = G::LX;
-    constexpr int NC = (SHAPE == 7) ? 7 : 27;
+    constexpr int NC = SHAPE;
     constexpr bool RES = (MODE == ZM_RESIDUAL || MODE == ZM_RESIDUAL2);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long sx = box_nx(a.sbox), sxy = sx * box_ny(a.sbox);
@@ -266,7 +268,7 @@ __device__ __forceinline__ void zm_edge(const ZArgs<T>& a, T (*sm)[ZGeom<RY>::NP
 {
     using G = ZGeom<RY>;
     constexpr int NL = G::NL, LX = G::LX, NE = G::NE;
-    constexpr int NC = (SHAPE == 7) ? 7 : 27;
+    constexpr int NC = SHAPE;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long sx = box_nx(a.sbox), sxy = sx * box_ny(a.sbox);
     long eoff[NL];
@@ -815,6 +817,18 @@ static void scaled_coefs(const pm_stencil* st, double scale, T* out)
     }
 }
 
+// k_zmarch stages the index box grown by one cell on every side (planes z - 1
+// and z + 1 included, without bounds checks in z): the 7/27-point stencils'
+// coverage guarantees it; a one-tap stencil's does not, so kind 2 takes the
+// z-march only when the source box holds that halo.
+static bool zmarch_halo(const Box3& sbox, const Box3& ibox)
+{
+    for (int d = 0; d < 3; d++) {
+        if (sbox.lo[d] > ibox.lo[d] - 1 || sbox.hi[d] < ibox.hi[d] + 1) { return false; }
+    }
+    return true;
+}
+
 template<typename T>
 static int apply_t(const pm_stencil* st, const void* src, const Box3& sbox, void* dst, const Box3& dbox,
                    const Box3& ibox, int ncomp, int replace, double scale, hipStream_t s)
@@ -824,7 +838,7 @@ static int apply_t(const pm_stencil* st, const void* src, const Box3& sbox, void
         if (!replace) { return PM_OK; }
         return launch_generic<T>(st, src, sbox, dst, dbox, ibox, ncomp, 1, sc, 0, s);
     }
-    if ((st->kind == 7 || st->kind == 27) && ncomp >= 1) {
+    if ((st->kind == 7 || st->kind == 27 || (st->kind == 2 && zmarch_halo(sbox, ibox))) && ncomp >= 1) {
         const long ssz = box_size(sbox), dsz = box_size(dbox);
         for (int c = 0; c < ncomp; c++) {
             ZArgs<T> a;
@@ -841,6 +855,9 @@ static int apply_t(const pm_stencil* st, const void* src, const Box3& sbox, void
             if (st->kind == 7) {
                 rc = replace ? launch_zmarch<T, 7, ZM_REPLACE>(a, ibox, cfg, s, &nb)
                              : launch_zmarch<T, 7, ZM_ACCUM>(a, ibox, cfg, s, &nb);
+            } else if (st->kind == 2) {
+                rc = replace ? launch_zmarch<T, 1, ZM_REPLACE>(a, ibox, cfg, s, &nb)
+                             : launch_zmarch<T, 1, ZM_ACCUM>(a, ibox, cfg, s, &nb);
             } else {
                 rc = replace ? launch_zmarch<T, 27, ZM_REPLACE>(a, ibox, cfg, s, &nb)
                              : launch_zmarch<T, 27, ZM_ACCUM>(a, ibox, cfg, s, &nb);
@@ -849,7 +866,7 @@ static int apply_t(const pm_stencil* st, const void* src, const Box3& sbox, void
         }
         return PM_OK;
     }
-    if (st->kind == 1) {
+    if (st->kind == 1 || st->kind == 2) {  // kind 2 here: no halo around the index box
         const long ssz = box_size(sbox), dsz = box_size(dbox);
         const ZCfg cfg = lookup_cfg(1, replace ? ZM_REPLACE : ZM_ACCUM, st->dtype, ibox);
         for (int c = 0; c < ncomp; c++) {
@@ -977,6 +994,10 @@ static int run_cfg(const pm_stencil* st, const pm_stencil* st2, int mode, ZArgs<
 {
     unsigned nb;
     const bool k7 = st->kind == 7;
+    if (st->kind == 2) {  // modes 0 and 1 only (checked by the caller)
+        return mode == ZM_REPLACE ? launch_zmarch<T, 1, ZM_REPLACE>(a, ib, c, s, &nb)
+                                  : launch_zmarch<T, 1, ZM_ACCUM>(a, ib, c, s, &nb);
+    }
     switch (mode) {
         case ZM_REPLACE: return k7 ? launch_zmarch<T, 7, ZM_REPLACE>(a, ib, c, s, &nb) : launch_zmarch<T, 27, ZM_REPLACE>(a, ib, c, s, &nb);
         case ZM_ACCUM: return k7 ? launch_zmarch<T, 7, ZM_ACCUM>(a, ib, c, s, &nb) : launch_zmarch<T, 27, ZM_ACCUM>(a, ib, c, s, &nb);
@@ -1171,7 +1192,7 @@ int pm_stencil_set_kind(pm_stencil* st, int kind)
 {
     PM_REQUIRE(st, "pm_stencil_set_kind: null argument");
     const int natural = detect_kind(st);
-    const bool ok = kind == 0 || kind == natural || (kind == 1 && (natural == 7 || natural == 27));
+    const bool ok = kind == 0 || kind == natural || (kind == 1 && (natural == 7 || natural == 27 || natural == 2));
     PM_REQUIRE(ok, "pm_stencil_set_kind: kind %d cannot serve this stencil (its own kind is %d)", kind, natural);
     st->kind = kind;
     return PM_OK;
@@ -1206,7 +1227,7 @@ int pm_stencil_apply_reduce(const pm_stencil* st, const void* src, const pm_box*
     const hipStream_t s = S(stream);
     const bool f64 = st->dtype == PM_F64;
     const double sc_t = f64 ? scale : (double)(float)scale;
-    if (st->kind == 1 && sc_t != 0.0) {
+    if ((st->kind == 1 || st->kind == 2) && sc_t != 0.0) {
         // one pass: the generic z-march writes dst and a partial per workgroup
         const ZCfg cfg = lookup_cfg(1, replace ? ZM_REPLACE : ZM_ACCUM, st->dtype, ib);
         const unsigned nb = zgen_blocks(ib, cfg);
@@ -1270,12 +1291,15 @@ int pm_stencil_autotune(const pm_stencil* st, const pm_stencil* st2, int mode, c
 {
     PM_REQUIRE(st && src_box && idx_box && best && best_ms, "pm_stencil_autotune: null argument");
     PM_REQUIRE(mode >= 0 && mode <= 3, "pm_stencil_autotune: bad mode %d", mode);
-    PM_REQUIRE(st->kind == 7 || st->kind == 27, "pm_stencil_autotune: only the 7/27-point z-march kernels are tunable");
+    PM_REQUIRE(st->kind == 7 || st->kind == 27 || (st->kind == 2 && mode <= ZM_ACCUM),
+               "pm_stencil_autotune: only the z-march kernels (7/27-point; the one-tap stencil's apply) are tunable");
     PM_REQUIRE(mode != ZM_RESIDUAL2 || (st2 && st2->kind == 27 && st2->dtype == st->dtype),
                "pm_stencil_autotune: mode 3 needs a 27-point second operator of the same dtype");
     PM_REQUIRE(reps >= 1, "pm_stencil_autotune: reps must be >= 1");
     const Box3 sb = box3(src_box), ib = box3(idx_box);
     PM_REQUIRE(!box_empty(ib) && covers(st, sb, ib, ib), "pm_stencil_autotune: source box does not cover the index box");
+    PM_REQUIRE(st->kind != 2 || zmarch_halo(sb, ib),
+               "pm_stencil_autotune: the one-tap z-march needs a source box one cell wider than the index box");
     ZCfg c;
     int rc = st->dtype == PM_F64 ? autotune_t<double>(st, st2, mode, sb, ib, reps, &c, best_ms, S(stream))
                                  : autotune_t<float>(st, st2, mode, sb, ib, reps, &c, best_ms, S(stream));

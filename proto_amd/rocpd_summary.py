@@ -48,21 +48,24 @@ def calls(db, pattern, last=40):
 
 
 def groups(db, top=30):
-    """Total time per (kernel, grid) over the whole database -- one solve's
-    breakdown by kernel and level size when the profiled program runs one --
+    """Total time per (kernel, grid, workgroup) over the whole database -- one
+    solve's breakdown by kernel and level size when the profiled program runs
+    one (kernels that split z to fill the GPU launch the same thread count on
+    several levels; their workgroup of 64 x boxes-along-x tells those apart) --
     with calls and the mean per call, largest first."""
     c = sqlite3.connect(db)
     acc = {}
-    for name, dur, gx, gy, gz in c.execute("select name, duration, grid_x, grid_y, grid_z from kernels"):
-        k = (name.split("(")[0][:64], "%dx%dx%d" % (gx, gy, gz))
+    for name, dur, gx, gy, gz, wx in c.execute("select name, duration, grid_x, grid_y, grid_z, workgroup_x "
+                                               "from kernels"):
+        k = (name.split("(")[0][:64], "%dx%dx%d/%d" % (gx, gy, gz, wx))
         a = acc.setdefault(k, [0, 0.0])
         a[0] += 1
         a[1] += dur / 1e6
     tot = sum(v[1] for v in acc.values())
     out = ["total %.3f ms over %d dispatches" % (tot, sum(v[0] for v in acc.values())),
-           "%-64s %-18s %6s %10s %10s" % ("kernel", "grid (threads)", "calls", "total_ms", "us/call")]
+           "%-64s %-22s %6s %10s %10s" % ("kernel", "grid (threads)/wg", "calls", "total_ms", "us/call")]
     for (name, grid), (n, t) in sorted(acc.items(), key=lambda kv: -kv[1][1])[:top]:
-        out.append("%-64s %-18s %6d %10.3f %10.1f" % (name, grid, n, t, 1e3 * t / n))
+        out.append("%-64s %-22s %6d %10.3f %10.1f" % (name, grid, n, t, 1e3 * t / n))
     return "\n".join(out)
 
 

@@ -140,6 +140,42 @@ def test_kernel_families_agree(name):
 
 
 @pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
+def test_one_tap_stencil(dtype):
+    """Stencil<T>::Identity() and its multiples (kind 2: the z-march kernel with
+    its centre tap) against the oracle and the closed form dst = (replace ? 0 :
+    dst) + c * src, on boxes with edge tiles, replace and accumulate; the
+    generic z-march and the per-cell kernel give the same bits.  The last box
+    has no halo (g = 0): the z-march stages planes z - 1 and z + 1 unchecked, so
+    kind 2 must take the bounds-checked generic z-march there."""
+    npdt = np.float64 if dtype == pa.PM_F64 else np.float32
+    o = [[0, 0, 0]]
+    for ci, coef in enumerate((1.0, -0.75)):
+        c = np.array([coef], dtype=npdt)
+        assert pa.Stencil(c, o, dtype).kind() == 2
+        for trial, (lo, hi, g) in enumerate((([0, 0, 0], [511, 63, 40], 2), ([-3, 5, 1], [70, 44, 29], 1),
+                                             ([2, -7, 0], [2, 9, 17], 0))):
+            lo, hi = np.array(lo), np.array(hi)
+            slo, shi = lo - g, hi + g
+            src = util.uniform(70 + trial, ci, util.box_size(slo, shi)).astype(npdt)
+            dst0 = util.uniform(80 + trial, ci, util.box_size(lo, hi)).astype(npdt)
+            for replace in (True, False):
+                say("one-tap", dtype, coef, "box", lo.tolist(), hi.tolist(), "replace", replace)
+                ref = dst0.copy()
+                util.oracle_apply(c, o, src, slo, shi, ref, lo, hi, lo, hi, replace, 1.0)
+                s3 = src.reshape(util.box_shape(slo, shi))
+                inner = s3[g:s3.shape[0] - g, g:s3.shape[1] - g, g:s3.shape[2] - g].reshape(-1)
+                closed = ((np.zeros_like(dst0) if replace else dst0) + c[0] * inner).astype(npdt)
+                assert ref.tobytes() == closed.tobytes()
+                outs = set()
+                for kind in (2, 1, 0):
+                    st = pa.Stencil(c, o, dtype).set_kind(kind)
+                    out = apply_gpu(st, src, slo, shi, dst0.copy(), lo, hi, lo, hi, replace, 1.0)
+                    assert out.tobytes() == ref.tobytes(), (coef, trial, replace, kind)
+                    outs.add(out.tobytes())
+                assert len(outs) == 1
+
+
+@pytest.mark.parametrize("dtype", [pa.PM_F64, pa.PM_F32])
 def test_apply_reduce(dtype):
     """pm_stencil_apply_reduce: dst as pm_stencil_apply writes it; Max, Min
     and Abs of the written cells exact, Sum within the stated bound; 3-D and

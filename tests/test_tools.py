@@ -12,8 +12,10 @@ def _kernels_db(path, rows):
     c.execute("create table kernels (dispatch_id integer, name text, duration integer, grid_x integer, "
               "grid_y integer, grid_z integer, workgroup_x integer, vgpr_count integer, "
               "accum_vgpr_count integer, lds_size integer)")
-    for i, (name, dur, grid) in enumerate(rows):
-        c.execute("insert into kernels values (?,?,?,?,?,?,?,?,?,?)", (i, name, dur, grid[0], grid[1], grid[2], 256,
+    for i, row in enumerate(rows):
+        name, dur, grid = row[:3]
+        wg = row[3] if len(row) > 3 else 256
+        c.execute("insert into kernels values (?,?,?,?,?,?,?,?,?,?)", (i, name, dur, grid[0], grid[1], grid[2], wg,
                                                                        40, 0, 1024))
     c.commit()
     c.close()
@@ -43,13 +45,16 @@ def test_trace_and_calls(tmp_path):
 def test_groups_by_kernel_and_grid(tmp_path):
     db = str(tmp_path / "g.db")
     _kernels_db(db, [("void k<double>(int)", 2000, (1024, 512, 1)), ("void k<double>(int)", 4000, (1024, 512, 1)),
-                     ("void k<double>(int)", 500, (1024, 64, 1)), ("void j(float)", 1500, (64, 1, 1))])
+                     ("void k<double>(int)", 500, (1024, 64, 1)), ("void j(float)", 1500, (64, 1, 1)),
+                     ("void j(float)", 700, (64, 1, 1), 128)])
     out = rs.groups(db).splitlines()
-    assert out[0] == "total 0.008 ms over 4 dispatches"
+    assert out[0] == "total 0.009 ms over 5 dispatches"
     first = out[2].split()
-    assert first[:2] == ["void", "k<double>"] and first[2] == "1024x512x1" and first[3] == "2"
+    assert first[:2] == ["void", "k<double>"] and first[2] == "1024x512x1/256" and first[3] == "2"
     assert float(first[4]) == pytest.approx(0.006) and float(first[5]) == pytest.approx(3.0)
-    assert len(out) == 5
+    # the same grid with another workgroup size is another level: its own line
+    assert any(l.split()[:3] == ["void", "j", "64x1x1/128"] for l in out[2:])
+    assert len(out) == 6
 
 
 def test_pmc_calibrates_against_the_copy(tmp_path):
