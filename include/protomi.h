@@ -460,6 +460,21 @@ int pm_batch_apply(pm_batch* batch, const pm_stencil* st, const void* src, void*
  * ghosts of phi_out must equal those of phi_in (they are not written). */
 int pm_batch_gsrb(pm_batch* batch, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
                   pm_stream_t stream);
+/* Which kernel runs pm_batch_gsrb (tests and benchmarks compare them; every
+ * form gives the same bits): 0 automatic (FP64: the pair sweep where every box
+ * is even and 4..128 cells wide, the ghost even and the arrays pair-aligned,
+ * else the row sweep where every box is 2..64 cells wide; FP32 the row sweep
+ * first, then the pair sweep; else the LDS ring),
+ * 1 the LDS-ring kernel, 2 the row sweep (a wave per 64 / nx packed sub-tiles
+ * of 4 rows, rows in registers, one cell per lane), 3 the row sweep with
+ * 2-row sub-tiles, 4 the row sweep with cached stores, 5 the pair sweep (two
+ * cells per lane, 64 / (nx / 2) sub-tiles of 4 rows per wave; either dtype),
+ * 6 the pair sweep with 2-row sub-tiles.  waves: workgroup size of forms 0, 2
+ * and 5 in waves (0 automatic, 1, 2, 4 or 8; the waves of a workgroup take
+ * neighbouring sub-tiles of one box and walk z together); forms 3, 4 and 6 run
+ * one wave per workgroup.  No counterpart in the reference (one fixed kernel
+ * per box). */
+int pm_batch_sweep_form(pm_batch* batch, int form, int waves);
 
 /* ------------------------------------------------------ library multigrid */
 /* Proto's LevelMultigridSolver<BoxOp_Laplace> (include/utils/

@@ -164,6 +164,7 @@ SIGNATURES = {
     "pm_batch_autotune": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _i, _vp, _vp, _vp]),
     "pm_batch_apply": (_i, [_vp, _vp, _vp, _vp, _d, _vp]),
     "pm_batch_gsrb": (_i, [_vp, _i, _vp, _vp, _vp, _d, _vp]),
+    "pm_batch_sweep_form": (_i, [_vp, _i, _i]),
     "pm_mg_set_tolerance": (_i, [_vp, _d, _i]),
     "pm_server_default_options": (None, [ctypes.POINTER(ServerOptions)]),
     "pm_server_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(ServerOptions)]),
@@ -323,6 +324,14 @@ class Batch:
 
     def gsrb(self, dtype, phi_in, phi_out, rhs, dx, stream=None):
         call("pm_batch_gsrb", self.h, dtype, phi_in, phi_out, rhs, ctypes.c_double(dx), stream)
+
+    def sweep_form(self, form, waves=0):
+        """Kernel of gsrb(): 0 automatic, 1 LDS ring, 2 row sweep, 3 row sweep
+        with 2-row sub-tiles, 4 row sweep with cached stores, 5 pair sweep (two
+        cells per lane), 6 pair sweep with 2-row sub-tiles; waves per
+        workgroup for forms 0, 2 and 5 (0 automatic)."""
+        call("pm_batch_sweep_form", self.h, int(form), int(waves))
+        return self
 
     def ghosted_view(self, flat, b):
         """Box b of a flat ghosted array, shaped (z, y, x)."""

@@ -26,6 +26,7 @@
 //          (identical values to the neighbouring tile's) so that no tile
 //          waits for another.
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -43,6 +44,15 @@ struct pm_batch {
     int nitems_gsrb = 0, gsrb_rows = 0;
     unsigned* d_counter = nullptr;   // 2 x (queue head, blocks done) per operator
     pm_batch_config cfg{0, 4, 8, 4};
+    // the row-register sweep: every box 2..64 cells wide (rows_ok), its items
+    // built for rows_ry rows per sub-tile; sweep_form as pm_batch_sweep_form
+    bool rows_ok = false;
+    void* d_ritems = nullptr;        // RowItem[] (row sweep)
+    int nitems_rows = 0, rows_ry = 0, rows_wpb = 0, sweep_form = 0, sweep_waves = 0;
+    // the two-cells-per-lane row sweep: every box even and 4..128 wide, even ghost
+    bool pairs_ok = false;
+    void* d_pitems = nullptr;        // RowItem[] (pair sweep)
+    int nitems_pairs = 0, pairs_ry = 0, pairs_wpb = 0;
 };
 
 namespace pm {
@@ -370,6 +380,388 @@ __global__ __launch_bounds__(256) void k_batch_gsrb(const BatchBox* __restrict__
     queue_done(counter);
 }
 
+// ------------------------------------------------------------ batched GSRB, rows in registers
+// The same sweep (arithmetic, order and colouring of k_batch_gsrb) for boxes at
+// most 64 cells wide, with no LDS and no barrier.  A workgroup is one wave and
+// does one item: P = 64 / nx sub-tiles of one box packed side by side in the
+// wave (lanes [p nx, (p + 1) nx) hold sub-tile p, lane p nx + x its column x),
+// sub-tile p owning rows y0 + p RY + [0, RY) over the whole z extent.  Each
+// lane keeps its column of rows [Y0 - 2, Y0 + RY + 2) of the live planes in
+// registers and walks z as the level row sweep does: step k updates the red
+// cells of plane k + 1 (tile rows and one halo row each side, recomputed
+// rather than shared), then the black cells of plane k, and writes plane k.
+// x neighbours are the adjacent lanes (DPP wave shifts); a sub-tile's first
+// and last column take the fixed ghost value from an edge register instead
+// (each lane also loads its own edge neighbour, or itself, per row).  Ghost
+// rows and planes are loaded (clamped into the ghosted box) but never updated
+// or written.  Lanes past the last sub-tile shadow lane 0 and store nothing.
+struct RowItem {
+    int box, y0, nsub, pad;
+};
+
+// waves per workgroup of the row and pair sweeps unless set otherwise (config
+// 4, 2000 boxes: FP32 row sweep 0.397 ms with 4 waves, 0.416 with 1, 0.513
+// with 8; FP64 pair sweep 0.696 ms with 1, 0.722 with 2, 0.803 with 4)
+constexpr int kRowsWaves = 4, kPairsWaves = 1;
+
+template<typename T, int RY, int NT, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_batch_rows(const BatchBox* __restrict__ boxes,
+                                                   const RowItem* __restrict__ items, int g,
+                                                   const T* __restrict__ phi_in, T* __restrict__ phi_out,
+                                                   const T* __restrict__ rhs, BCoefs<T> c, T lambda)
+{
+    constexpr int NR = RY + 4;  // phi rows held: Y0 - 2 .. Y0 + RY + 1
+    constexpr int NQ = RY + 2;  // rhs and edge rows: Y0 - 1 .. Y0 + RY
+    const RowItem wi = items[blockIdx.x];
+    if (wi.box < 0) { return; }  // padding of a shorter XCD list
+    const BatchBox bx = boxes[wi.box];
+    const int nx = bx.nx, ny = bx.ny, nz = bx.nz;
+    // wave w of the workgroup holds sub-tiles w P .. w P + P - 1 of the item
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, npw = 64 / nx;
+    const int pl = lane / nx;
+    int p = w * npw + pl, x = lane - pl * nx;
+    const bool live = pl < npw && p < wi.nsub;
+    if (!live) {
+        p = 0;
+        x = 0;
+    }
+    const int Y0 = wi.y0 + p * RY;
+    const int Gx = nx + 2 * g;
+    const long Gxy = (long)Gx * (ny + 2 * g), Vxy = (long)nx * ny;
+    const T c0 = c.c[0], c1 = c.c[1], c2 = c.c[2], c3 = c.c[3], c4 = c.c[4], c5 = c.c[5], c6 = c.c[6];
+    // in-plane element offsets of this lane's rows, clamped to the rows the
+    // sweep reads (-1 .. ny: rows further out only neighbour ghost rows, which
+    // are not updated, so they are not fetched)
+    int ro[NR], qo[NQ];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int y = min(max(Y0 - 2 + i, -1), ny);
+        ro[i] = (x + g) + Gx * (y + g);
+    }
+#pragma unroll
+    for (int j = 0; j < NQ; j++) { qo[j] = x + nx * min(max(Y0 - 1 + j, 0), ny - 1); }
+    const bool xl = x == 0, xr = x == nx - 1;
+    const int ed = xl ? -1 : (xr ? 1 : 0);
+    // rows i this lane may update: inside the box and of a live sub-tile
+    unsigned upd = 0;
+#pragma unroll
+    for (int i = 1; i <= RY + 2; i++) {
+        const int y = Y0 - 2 + i;
+        upd |= (live && y >= 0 && y < ny) ? (1u << i) : 0u;
+    }
+    const int par = (x + Y0) & 1;  // red where (x + y + z) is even
+    auto is_red = [&](int i, int z) __attribute__((always_inline)) { return ((par + i + z) & 1) == 0; };
+    const T* pb = phi_in + bx.goff;
+    T* po = phi_out + bx.goff;
+    const T* qb = rhs + bx.voff;
+    T P[5][NR];  // phi planes by slot: plane z in slot (z + 1) mod 5
+    T Q[5][NQ];  // rhs planes, same slots
+    T E[5][NQ];  // edge values (x - 1 for the first column, x + 1 for the last)
+    auto loadP = [&](T (&d)[NR], int z) {
+        const T* b = pb + Gxy * (long)(min(z, nz) + g);
+#pragma unroll
+        for (int i = 0; i < NR; i++) { d[i] = b[ro[i]]; }
+    };
+    auto loadE = [&](T (&d)[NQ], int z) {
+        const T* b = pb + Gxy * (long)(min(z, nz) + g) + ed;
+#pragma unroll
+        for (int j = 0; j < NQ; j++) { d[j] = b[ro[j + 1]]; }
+    };
+    auto loadQ = [&](T (&d)[NQ], int z) {
+        const T* b = qb + Vxy * (long)min(z, nz - 1);
+#pragma unroll
+        for (int j = 0; j < NQ; j++) { d[j] = b[qo[j]]; }
+    };
+    auto relax = [&](T cur, T left, T right, T up, T down, T zp, T zm, T f) __attribute__((always_inline)) {
+        T s = 0;
+        s = s + c0 * cur;
+        s = s + c1 * right;
+        s = s + c2 * left;
+        s = s + c3 * up;
+        s = s + c4 * down;
+        s = s + c5 * zp;
+        s = s + c6 * zm;
+        s = s + f;
+        return cur + lambda * s;
+    };
+    auto left = [&](T v, T e) __attribute__((always_inline)) {
+        const T s = lane_shift<kShr1>(e, v);
+        return xl ? e : s;
+    };
+    auto right = [&](T v, T e) __attribute__((always_inline)) {
+        const T s = lane_shift<kShl1>(e, v);
+        return xr ? e : s;
+    };
+    auto step = [&](auto sc, int k) __attribute__((always_inline)) {
+        constexpr int S = decltype(sc)::value;
+        constexpr int S0 = S, S1 = (S + 1) % 5, S2 = (S + 2) % 5, S3 = (S + 3) % 5, SM = (S + 4) % 5;
+        if (WPB > 1) { __syncthreads(); }  // the item's waves walk z together
+        loadP(P[S3], k + 3);
+        loadQ(Q[S2], k + 2);
+        loadE(E[S2], k + 2);
+        if (k + 1 < nz) {  // red cells of plane k + 1, rows Y0 - 1 .. Y0 + RY
+#pragma unroll
+            for (int j = 0; j < NQ; j++) {
+                const int i = j + 1;
+                const T cur = P[S1][i], e = E[S1][j];
+                const T nv = relax(cur, left(cur, e), right(cur, e), P[S1][i + 1], P[S1][i - 1], P[S2][i], P[S0][i],
+                                   Q[S1][j]);
+                P[S1][i] = (((upd >> i) & 1u) && is_red(i, k + 1)) ? nv : cur;
+            }
+        }
+        if (k >= 0) {  // black cells of plane k, rows Y0 .. Y0 + RY - 1, then plane k out
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int i = r + 2;
+                const T cur = P[S0][i], e = E[S0][r + 1];
+                const T nv = relax(cur, left(cur, e), right(cur, e), P[S0][i + 1], P[S0][i - 1], P[S1][i], P[SM][i],
+                                   Q[S0][r + 1]);
+                P[S0][i] = (((upd >> i) & 1u) && !is_red(i, k)) ? nv : cur;
+            }
+            T* ob = po + Gxy * (long)(k + g);
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                if ((upd >> (r + 2)) & 1u) {
+                    if (NT) {
+                        __builtin_nontemporal_store(P[S0][r + 2], &ob[ro[r + 2]]);
+                    } else {
+                        ob[ro[r + 2]] = P[S0][r + 2];
+                    }
+                }
+            }
+        }
+    };
+    // planes -1, 0, 1 (slots 0..2), rhs and edges of plane 0 (slot 1)
+    loadP(P[0], -1);
+    loadP(P[1], 0);
+    loadP(P[2], 1);
+    loadQ(Q[1], 0);
+    loadE(E[1], 0);
+    int k = -1;
+    for (;;) {
+        step(std::integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(std::integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(std::integral_constant<int, 2>{}, k);
+        if (++k >= nz) { break; }
+        step(std::integral_constant<int, 3>{}, k);
+        if (++k >= nz) { break; }
+        step(std::integral_constant<int, 4>{}, k);
+        if (++k >= nz) { break; }
+    }
+}
+
+// Two cells per lane: lane q of a sub-row holds columns 2q (A) and 2q + 1 (B)
+// of its rows, loaded and stored as pairs, so a 64-lane wave covers 128
+// columns and each colour pass relaxes one register of the pair per row
+// instead of computing every lane and discarding half.  Sub-tile heights RY
+// are even and sub-tiles start on even rows, so which of A and B holds the
+// row's cell of a colour depends only on the row and the plane parity: the z
+// walk is unrolled ten times (five register slots by two parities) and every
+// colour choice is made at compile time.  Boxes 4..128 cells wide and even,
+// an even number of ghost layers (pairs aligned); P = 64 / (nx / 2) sub-tiles
+// per wave.  Arithmetic, order and colouring are k_batch_gsrb's.
+template<typename T>
+struct Vec2;
+template<>
+struct Vec2<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template<>
+struct Vec2<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template<typename T, int RY, int NT, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __restrict__ boxes,
+                                                    const RowItem* __restrict__ items, int g,
+                                                    const T* __restrict__ phi_in, T* __restrict__ phi_out,
+                                                    const T* __restrict__ rhs, BCoefs<T> c, T lambda)
+{
+    using V = typename Vec2<T>::type;
+    constexpr int NR = RY + 4;  // phi rows held: Y0 - 2 .. Y0 + RY + 1
+    constexpr int NQ = RY + 2;  // rhs and edge rows: Y0 - 1 .. Y0 + RY
+    static_assert(RY % 2 == 0, "sub-tiles start on even rows");
+    const RowItem wi = items[blockIdx.x];
+    if (wi.box < 0) { return; }  // padding of a shorter XCD list
+    const BatchBox bx = boxes[wi.box];
+    const int nx = bx.nx, ny = bx.ny, nz = bx.nz, h = nx >> 1;
+    // wave w of the workgroup holds sub-tiles w npw .. w npw + npw - 1 of the item
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, npw = 64 / h;
+    const int pl = lane / h;
+    int p = w * npw + pl, q = lane - pl * h;
+    const bool live = pl < npw && p < wi.nsub;
+    if (!live) {
+        p = 0;
+        q = 0;
+    }
+    const int Y0 = wi.y0 + p * RY;
+    const int Gx = nx + 2 * g;
+    const long Gxy = (long)Gx * (ny + 2 * g), Vxy = (long)nx * ny;
+    const T c0 = c.c[0], c1 = c.c[1], c2 = c.c[2], c3 = c.c[3], c4 = c.c[4], c5 = c.c[5], c6 = c.c[6];
+    // in-plane byte offsets of column 2q of this lane's rows, clamped to the
+    // rows the sweep reads (-1 .. ny); 32-bit and unsigned: the loads take a
+    // scalar plane base plus a 32-bit lane offset
+    unsigned ro[NR], qo[NQ];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int y = min(max(Y0 - 2 + i, -1), ny);
+        ro[i] = (unsigned)(((2 * q + g) + Gx * (y + g)) * (int)sizeof(T));
+    }
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+        qo[j] = (unsigned)((2 * q + nx * min(max(Y0 - 1 + j, 0), ny - 1)) * (int)sizeof(T));
+    }
+    const bool ql = q == 0, qr = q == h - 1;
+    // the edge value a lane needs: column -1 for the first pair, nx for the last
+    // (ro >= g sizeof(T) >= 2 sizeof(T), so the unsigned sum does not wrap)
+    const unsigned edu = ql ? (unsigned)(-(int)sizeof(T)) : (qr ? 2u * (unsigned)sizeof(T) : 0u);
+    unsigned upd = 0;
+#pragma unroll
+    for (int i = 1; i <= RY + 2; i++) {
+        const int y = Y0 - 2 + i;
+        upd |= (live && y >= 0 && y < ny) ? (1u << i) : 0u;
+    }
+    const T* pb = phi_in + bx.goff;
+    T* po = phi_out + bx.goff;
+    const T* qb = rhs + bx.voff;
+    T A[5][NR], B[5][NR];    // phi planes by slot (plane z in slot (z + 1) mod 5), columns 2q / 2q + 1
+    T QA[5][NQ], QB[5][NQ];  // rhs, same slots
+    T E[5][NQ];              // edge values
+    auto loadP = [&](T (&a)[NR], T (&b)[NR], int z) {
+        const char* base = reinterpret_cast<const char*>(pb + Gxy * (long)(min(z, nz) + g));
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const V v = *reinterpret_cast<const V*>(base + ro[i]);
+            a[i] = v.x;
+            b[i] = v.y;
+        }
+    };
+    auto loadE = [&](T (&d)[NQ], int z) {
+        const char* base = reinterpret_cast<const char*>(pb + Gxy * (long)(min(z, nz) + g));
+#pragma unroll
+        for (int j = 0; j < NQ; j++) { d[j] = *reinterpret_cast<const T*>(base + (ro[j + 1] + edu)); }
+    };
+    auto loadQ = [&](T (&a)[NQ], T (&b)[NQ], int z) {
+        const char* base = reinterpret_cast<const char*>(qb + Vxy * (long)min(z, nz - 1));
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const V v = *reinterpret_cast<const V*>(base + qo[j]);
+            a[j] = v.x;
+            b[j] = v.y;
+        }
+    };
+    auto relax = [&](T cur, T left, T right, T up, T down, T zp, T zm, T f) __attribute__((always_inline)) {
+        T s = 0;
+        s = s + c0 * cur;
+        s = s + c1 * right;
+        s = s + c2 * left;
+        s = s + c3 * up;
+        s = s + c4 * down;
+        s = s + c5 * zp;
+        s = s + c6 * zm;
+        s = s + f;
+        return cur + lambda * s;
+    };
+    // row i of a plane: relax its A cell (left neighbour: the previous lane's B,
+    // or the edge) or its B cell (right neighbour: the next lane's A, or the edge)
+    auto cellA = [&](T (&a)[NR], T (&b)[NR], const T (&ap)[NR], const T (&am)[NR], T f, T e, int i, bool ok)
+        __attribute__((always_inline)) {
+        const T cur = a[i];
+        const T sl = lane_shift<kShr1>(e, b[i]);
+        const T nv = relax(cur, ql ? e : sl, b[i], a[i + 1], a[i - 1], ap[i], am[i], f);
+        a[i] = ok ? nv : cur;
+    };
+    auto cellB = [&](T (&a)[NR], T (&b)[NR], const T (&bp)[NR], const T (&bm)[NR], T f, T e, int i, bool ok)
+        __attribute__((always_inline)) {
+        const T cur = b[i];
+        const T sr = lane_shift<kShl1>(e, a[i]);
+        const T nv = relax(cur, a[i], qr ? e : sr, b[i + 1], b[i - 1], bp[i], bm[i], f);
+        b[i] = ok ? nv : cur;
+    };
+    // step k, slot S of plane k, ZP = parity of plane k + 1: red cells of plane
+    // k + 1 (rows Y0 - 1 .. Y0 + RY), black cells of plane k, plane k out.  The
+    // red cell of row i of plane k + 1 is A when i + ZP is even; so is the black
+    // cell of row i of plane k (whose parity is 1 - ZP).
+    auto step = [&](auto sc, auto zc, int k) __attribute__((always_inline)) {
+        constexpr int S = decltype(sc)::value, ZP = decltype(zc)::value;
+        constexpr int S0 = S, S1 = (S + 1) % 5, S2 = (S + 2) % 5, S3 = (S + 3) % 5, SM = (S + 4) % 5;
+        if (WPB > 1) { __syncthreads(); }  // the item's waves walk z together
+        loadP(A[S3], B[S3], k + 3);
+        loadQ(QA[S2], QB[S2], k + 2);
+        loadE(E[S2], k + 2);
+        if (k + 1 < nz) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) {
+                const int i = j + 1;
+                const bool ok = (upd >> i) & 1u;
+                if (((i + ZP) & 1) == 0) {
+                    cellA(A[S1], B[S1], A[S2], A[S0], QA[S1][j], E[S1][j], i, ok);
+                } else {
+                    cellB(A[S1], B[S1], B[S2], B[S0], QB[S1][j], E[S1][j], i, ok);
+                }
+            }
+        }
+        if (k >= 0) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int i = r + 2;
+                const bool ok = (upd >> i) & 1u;
+                if (((i + ZP) & 1) == 0) {
+                    cellA(A[S0], B[S0], A[S1], A[SM], QA[S0][r + 1], E[S0][r + 1], i, ok);
+                } else {
+                    cellB(A[S0], B[S0], B[S1], B[SM], QB[S0][r + 1], E[S0][r + 1], i, ok);
+                }
+            }
+            char* ob = reinterpret_cast<char*>(po + Gxy * (long)(k + g));
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                if ((upd >> (r + 2)) & 1u) {
+                    V v;
+                    v.x = A[S0][r + 2];
+                    v.y = B[S0][r + 2];
+                    V* dst = reinterpret_cast<V*>(ob + ro[r + 2]);
+                    if (NT) {
+                        __builtin_nontemporal_store(v, dst);
+                    } else {
+                        *dst = v;
+                    }
+                }
+            }
+        }
+    };
+    using std::integral_constant;
+    loadP(A[0], B[0], -1);
+    loadP(A[1], B[1], 0);
+    loadP(A[2], B[2], 1);
+    loadQ(QA[1], QB[1], 0);
+    loadE(E[1], 0);
+    int k = -1;
+    for (;;) {
+        step(integral_constant<int, 0>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 1>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 2>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 3>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 4>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 0>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 1>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 2>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 3>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 4>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+    }
+}
+
 }  // namespace pm
 
 using namespace pm;
@@ -455,6 +847,56 @@ int build_gsrb_items(pm_batch* B, int rows)
     return upload(&B->d_gitems, items.data(), sizeof(BatchItem) * items.size());
 }
 
+// Row-sweep items: per box, strips of ry rows, 64 / (nx / cells per lane)
+// strips per wave, wpb waves per workgroup (one item each; the waves of an item
+// walk z together, so the halo rows they share are re-read while still
+// cached).  Workgroup w runs on XCD w mod 8, and
+// the strips of a box share halo rows, so every box goes whole to one XCD
+// (whose L2 then serves the re-read rows): boxes are spread over the 8 XCDs
+// by cell count (largest first, to the least loaded), each XCD's list takes
+// its boxes longest z extent first, and the lists are interleaved into the
+// launch order, shorter ones padded with empty items (box -1).
+int build_row_items(pm_batch* B, int ry, bool pairs, int wpb)
+{
+    constexpr int kXcd = 8;
+    std::vector<int> order(B->nboxes);
+    for (int b = 0; b < B->nboxes; b++) { order[b] = b; }
+    auto cells = [&](int b) { return (long)B->sizes[3 * b] * B->sizes[3 * b + 1] * B->sizes[3 * b + 2]; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    std::vector<std::vector<int>> xb(kXcd);
+    std::vector<long> load(kXcd, 0);
+    for (int b : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        xb[x].push_back(b);
+        load[x] += cells(b);
+    }
+    std::vector<std::vector<RowItem>> lists(kXcd);
+    size_t nmax = 0;
+    for (int x = 0; x < kXcd; x++) {
+        std::stable_sort(xb[x].begin(), xb[x].end(),
+                         [&](int a, int b) { return B->sizes[3 * a + 2] > B->sizes[3 * b + 2]; });
+        for (int b : xb[x]) {
+            const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1];
+            const int P = wpb * (64 / (pairs ? nx / 2 : nx)), nstrips = (ny + ry - 1) / ry;
+            for (int s = 0; s < nstrips; s += P) { lists[x].push_back(RowItem{b, s * ry, std::min(P, nstrips - s), 0}); }
+        }
+        nmax = std::max(nmax, lists[x].size());
+    }
+    std::vector<RowItem> items(nmax * kXcd, RowItem{-1, 0, 0, 0});
+    for (int x = 0; x < kXcd; x++)
+        for (size_t m = 0; m < lists[x].size(); m++) { items[m * kXcd + x] = lists[x][m]; }
+    if (pairs) {
+        B->nitems_pairs = (int)items.size();
+        B->pairs_ry = ry;
+        B->pairs_wpb = wpb;
+        return upload(&B->d_pitems, items.data(), sizeof(RowItem) * items.size());
+    }
+    B->nitems_rows = (int)items.size();
+    B->rows_ry = ry;
+    B->rows_wpb = wpb;
+    return upload(&B->d_ritems, items.data(), sizeof(RowItem) * items.size());
+}
+
 unsigned persistent_blocks(int blocks_per_cu)
 {
     int dev = 0, ncu = 256;
@@ -510,6 +952,15 @@ int pm_batch_create(pm_batch** out, int nboxes, const int* sizes_host, int ghost
     if (rc == PM_OK && hipMemset(B->d_counter, 0, 4 * sizeof(unsigned)) != hipSuccess) { rc = PM_ERR_HIP; }
     if (rc == PM_OK) { rc = build_apply_items(B, 0); }
     if (rc == PM_OK) { rc = build_gsrb_items(B, B->cfg.gsrb_rows); }
+    B->rows_ok = true;
+    B->pairs_ok = ghost % 2 == 0;
+    for (int b = 0; b < nboxes; b++) {
+        const int nx = B->sizes[3 * b];
+        B->rows_ok = B->rows_ok && nx >= 2 && nx <= 64;
+        B->pairs_ok = B->pairs_ok && nx >= 4 && nx <= 128 && nx % 2 == 0;
+    }
+    if (rc == PM_OK && B->rows_ok) { rc = build_row_items(B, 4, false, kRowsWaves); }
+    if (rc == PM_OK && B->pairs_ok) { rc = build_row_items(B, 4, true, kPairsWaves); }
     if (rc != PM_OK) {
         pm_batch_destroy(B);
         return rc;
@@ -524,6 +975,8 @@ int pm_batch_destroy(pm_batch* B)
     hipFree(B->d_boxes);
     hipFree(B->d_items);
     hipFree(B->d_gitems);
+    hipFree(B->d_ritems);
+    hipFree(B->d_pitems);
     hipFree(B->d_counter);
     delete B;
     return PM_OK;
@@ -605,12 +1058,120 @@ int pm_batch_apply(pm_batch* B, const pm_stencil* st, const void* src, void* dst
     return PM_OK;
 }
 
+int pm_batch_sweep_form(pm_batch* B, int form, int waves)
+{
+    PM_REQUIRE(B && form >= 0 && form <= 6, "pm_batch_sweep_form: bad form");
+    PM_REQUIRE(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8,
+               "pm_batch_sweep_form: waves per workgroup must be 0 (automatic), 1, 2, 4 or 8");
+    PM_REQUIRE(form < 2 || form > 4 || B->rows_ok, "pm_batch_sweep_form: the row sweep needs every box 2..64 cells wide");
+    PM_REQUIRE(form < 5 || B->pairs_ok,
+               "pm_batch_sweep_form: the pair sweep needs every box even and 4..128 cells wide, and an even ghost");
+    // 2-row sub-tiles and cached stores are measured with one wave per workgroup only
+    const bool one = form == 3 || form == 4 || form == 6;
+    PM_REQUIRE(!one || waves <= 1, "pm_batch_sweep_form: forms 3, 4 and 6 run one wave per workgroup");
+    const int rw = one ? 1 : (waves ? waves : kRowsWaves), pw = one ? 1 : (waves ? waves : kPairsWaves);
+    int rc = PM_OK;
+    const int ry = form == 3 ? 2 : 4;
+    if (B->rows_ok && (B->rows_ry != ry || B->rows_wpb != rw) && form != 1 && form < 5) {
+        rc = build_row_items(B, ry, false, rw);
+    }
+    const int pry = form == 6 ? 2 : 4;
+    if (rc == PM_OK && B->pairs_ok && (B->pairs_ry != pry || B->pairs_wpb != pw) && (form == 0 || form >= 5)) {
+        rc = build_row_items(B, pry, true, pw);
+    }
+    if (rc == PM_OK) {
+        B->sweep_form = form;
+        B->sweep_waves = waves;
+    }
+    return rc;
+}
+
 int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
                   pm_stream_t stream)
 {
     PM_REQUIRE(B && phi_in && phi_out && rhs, "pm_batch_gsrb: null argument");
     PM_REQUIRE(phi_in != phi_out, "pm_batch_gsrb: the sweep is out of place (phi_in != phi_out)");
     PM_REQUIRE(dtype == PM_F32 || dtype == PM_F64, "pm_batch_gsrb: bad dtype");
+    // automatic: FP64 the pair sweep (where the arrays are pair-aligned), FP32 the
+    // row sweep (config 4: FP64 pairs 0.696 ms against rows 0.720, FP32 rows
+    // 0.397 ms against pairs 0.432), each falling back on the other, then the LDS ring
+    const size_t al = 2 * (dtype == PM_F64 ? 8 : 4);
+    const bool aligned = (uintptr_t)phi_in % al == 0 && (uintptr_t)phi_out % al == 0 && (uintptr_t)rhs % al == 0;
+    const int sf = B->sweep_form;
+    if (sf >= 5) { PM_REQUIRE(aligned, "pm_batch_gsrb: the pair sweep needs arrays aligned to two elements"); }
+    const bool pairs_auto = B->pairs_ok && aligned && (dtype == PM_F64 || !B->rows_ok);
+    if (sf >= 5 || (sf == 0 && pairs_auto)) {
+        double lam = -(dx * dx) / (4.0 * 3);
+        lam *= 2.0;
+        const double m = -1.0 / (dx * dx);
+        auto go = [&](auto tag) {
+            using T = decltype(tag);
+            BCoefs<T> cc;
+            for (int j = 0; j < 7; j++) { cc.c[j] = (T)(j == 0 ? -6.0 : 1.0) * (T)m; }
+            const T lamT = (T)(1.0 * lam);
+            auto run = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3((unsigned)B->nitems_pairs), dim3(64 * B->pairs_wpb), 0, S(stream),
+                                   (const BatchBox*)B->d_boxes, (const RowItem*)B->d_pitems, B->ghost,
+                                   (const T*)phi_in, (T*)phi_out, (const T*)rhs, cc, lamT);
+            };
+            if (B->pairs_ry == 2) {
+                run(k_batch_rows2<T, 2, 1, 1>);
+            } else if (B->pairs_wpb == 2) {
+                run(k_batch_rows2<T, 4, 1, 2>);
+            } else if (B->pairs_wpb == 4) {
+                run(k_batch_rows2<T, 4, 1, 4>);
+            } else if (B->pairs_wpb == 8) {
+                run(k_batch_rows2<T, 4, 1, 8>);
+            } else {
+                run(k_batch_rows2<T, 4, 1, 1>);
+            }
+        };
+        if (dtype == PM_F64) {
+            go(0.0);
+        } else {
+            go(0.0f);
+        }
+        PM_CHECK_LAUNCH();
+        return PM_OK;
+    }
+    if (B->rows_ok && B->sweep_form != 1) {
+        // LaplaceOp::init: coefficients Laplacian()_j * (-1/(dx*dx)), lambda = 2 * (-(dx*dx)/(4 DIM))
+        double lam = -(dx * dx) / (4.0 * 3);
+        lam *= 2.0;
+        const double m = -1.0 / (dx * dx);
+        const int form = B->sweep_form;
+        auto go = [&](auto tag) {
+            using T = decltype(tag);
+            BCoefs<T> cc;
+            for (int j = 0; j < 7; j++) { cc.c[j] = (T)(j == 0 ? -6.0 : 1.0) * (T)m; }
+            const T lamT = (T)(1.0 * lam);
+            auto run = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3((unsigned)B->nitems_rows), dim3(64 * B->rows_wpb), 0, S(stream),
+                                   (const BatchBox*)B->d_boxes, (const RowItem*)B->d_ritems, B->ghost,
+                                   (const T*)phi_in, (T*)phi_out, (const T*)rhs, cc, lamT);
+            };
+            if (form == 3) {
+                run(k_batch_rows<T, 2, 1, 1>);
+            } else if (form == 4) {
+                run(k_batch_rows<T, 4, 0, 1>);
+            } else if (B->rows_wpb == 2) {
+                run(k_batch_rows<T, 4, 1, 2>);
+            } else if (B->rows_wpb == 4) {
+                run(k_batch_rows<T, 4, 1, 4>);
+            } else if (B->rows_wpb == 8) {
+                run(k_batch_rows<T, 4, 1, 8>);
+            } else {
+                run(k_batch_rows<T, 4, 1, 1>);
+            }
+        };
+        if (dtype == PM_F64) {
+            go(0.0);
+        } else {
+            go(0.0f);
+        }
+        PM_CHECK_LAUNCH();
+        return PM_OK;
+    }
     int maxnx = 0;
     for (int b = 0; b < B->nboxes; b++) { maxnx = std::max(maxnx, B->sizes[3 * b]); }
     PM_REQUIRE(maxnx <= 256, "pm_batch_gsrb: boxes wider than 256 cells are not supported");

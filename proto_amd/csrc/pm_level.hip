@@ -404,22 +404,6 @@ __global__ __launch_bounds__(256) void k_gsrb_fused(LayoutDev L, G3 gin, G3 gout
 }
 
 // ------------------------------------------------------------ row-register sweep
-// x neighbours across the wave: lane l takes lane l - 1 (wave_shr:1) or l + 1
-// (wave_shl:1); the lane with no source (0 or 63) keeps `edge`.
-constexpr int kShr1 = 0x138, kShl1 = 0x130;
-template<int CTRL>
-__device__ __forceinline__ float lane_shift(float edge, float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template<int CTRL>
-__device__ __forceinline__ double lane_shift(double edge, double v)
-{
-    const long long e = __double_as_longlong(edge), x = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp((int)e, (int)x, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(e >> 32), (int)(x >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 // One full red-black sweep of a periodic level of boxes 64 cells wide that
 // covers its domain, out of place, in one pass, the rows held in registers.

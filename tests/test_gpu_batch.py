@@ -112,6 +112,79 @@ def test_batch_gsrb_vs_oracle(dt):
             assert first_diff(B, got2, want2, False) is None, (rows, rep, first_diff(B, got2, want2, False))
 
 
+# every box 2..64 wide: the row-register sweep serves these (odd extents,
+# rows not a multiple of the sub-tile height, one- and two-plane boxes, one
+# to four sub-tiles per wave)
+SIZES_ROWS = [(16, 16, 16), (24, 24, 24), (33, 7, 5), (2, 3, 4), (64, 64, 8), (48, 32, 16), (7, 7, 7),
+              (17, 19, 23), (32, 32, 32), (64, 1, 2), (2, 50, 1), (63, 5, 3), (21, 4, 9), (40, 13, 2), (3, 2, 1)]
+
+
+@pytest.mark.parametrize("ghost", [1, 2])
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+def test_batch_row_sweep_vs_oracle(dt, ghost):
+    """The row sweep (forms 0 = automatic, 2, 3, 4) and the LDS-ring kernel
+    (form 1) against the oracle, box by box, bitwise, and a second sweep back."""
+    B = pa.Batch(SIZES_ROWS, ghost)
+    t = NPDT[dt]
+    phi, rhs = inputs(B, dt, 14 + ghost)
+    dx = 1.0 / 32
+    want1 = oracle_gsrb_batch(B, phi, rhs, dx)
+    want2 = oracle_gsrb_batch(B, want1, rhs, dx)
+    p_in, r_d = dev(phi), dev(rhs)
+    for form, waves in ((0, 0), (2, 1), (2, 2), (2, 8), (3, 0), (4, 0), (1, 0)):
+        B.sweep_form(form, waves)
+        say("batch row sweep", dt, "ghost", ghost, "form", form, "waves", waves)
+        p_out = dev(phi)
+        B.gsrb(dt, vp(p_in), vp(p_out), vp(r_d), dx)
+        got = host(p_out)
+        assert first_diff(B, got, want1, False) is None, (form, first_diff(B, got, want1, False))
+        p_back = dev(phi)
+        B.gsrb(dt, vp(p_out), vp(p_back), vp(r_d), dx)
+        assert first_diff(B, host(p_back), want2, False) is None, (form, first_diff(B, host(p_back), want2, False))
+    # a batch with a box wider than 64 cells has no row sweep
+    with pytest.raises(pa.NativeError, match="2..64"):
+        make_batch().sweep_form(2)
+
+
+# every box even and 4..128 wide: the pair sweep (two cells per lane) serves these
+SIZES_PAIRS = [(16, 16, 16), (24, 24, 24), (34, 7, 5), (4, 3, 4), (64, 64, 8), (48, 32, 16), (6, 7, 7),
+               (18, 19, 23), (32, 32, 32), (64, 1, 2), (4, 50, 1), (62, 5, 3), (22, 4, 9), (40, 13, 2), (4, 2, 1),
+               (128, 6, 3), (96, 9, 4)]
+
+
+@pytest.mark.parametrize("ghost", [2, 4])
+@pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
+def test_batch_pair_sweep_vs_oracle(dt, ghost):
+    """The pair sweep (forms 0 = automatic for FP32, 5, 6) and the LDS ring
+    against the oracle, box by box, bitwise, and a second sweep back."""
+    B = pa.Batch(SIZES_PAIRS, ghost)
+    t = NPDT[dt]
+    phi, rhs = inputs(B, dt, 30 + ghost)
+    dx = 1.0 / 48
+    want1 = oracle_gsrb_batch(B, phi, rhs, dx)
+    want2 = oracle_gsrb_batch(B, want1, rhs, dx)
+    p_in, r_d = dev(phi), dev(rhs)
+    for form, waves in ((0, 0), (5, 1), (5, 2), (5, 8), (6, 0), (1, 0)):
+        B.sweep_form(form, waves)
+        say("batch pair sweep", dt, "ghost", ghost, "form", form, "waves", waves)
+        p_out = dev(phi)
+        B.gsrb(dt, vp(p_in), vp(p_out), vp(r_d), dx)
+        got = host(p_out)
+        assert first_diff(B, got, want1, False) is None, (form, first_diff(B, got, want1, False))
+        p_back = dev(phi)
+        B.gsrb(dt, vp(p_out), vp(p_back), vp(r_d), dx)
+        assert first_diff(B, host(p_back), want2, False) is None, (form, first_diff(B, host(p_back), want2, False))
+    with pytest.raises(pa.NativeError, match="one wave"):
+        B.sweep_form(6, 4)
+    with pytest.raises(pa.NativeError, match="waves per workgroup"):
+        B.sweep_form(5, 3)
+    # odd widths or an odd ghost have no pair sweep
+    with pytest.raises(pa.NativeError, match="pair sweep"):
+        pa.Batch(SIZES_ROWS, 2).sweep_form(5)
+    with pytest.raises(pa.NativeError, match="pair sweep"):
+        pa.Batch(SIZES_PAIRS, 1).sweep_form(5)
+
+
 def test_batch_gsrb_limits():
     B = make_batch()
     phi, rhs = inputs(B, pa.PM_F64, 13)
