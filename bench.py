@@ -427,7 +427,8 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
     # the same two solves launched eagerly (use_graph 0) or replayed as one captured graph per cycle
     forms += [("fp64_graph%d" % (1 - use_graph), 0, 1, 1 - use_graph), ("fp32_ir_graph%d" % (1 - use_graph), 1, 1,
                                                                           1 - use_graph)]
-    for name, prec, fused, graph in forms:
+    for name, prec, fused, graph, *sweep_form in forms:
+        pa.call("pm_level_sweep_form", sweep_form[0] if sweep_form else 0)
         opt = pa.MgOptions()
         pa.lib().pm_mg_default_options(ctypes.byref(opt))
         opt.relax, opt.pre, opt.post, opt.bottom, opt.fused = 0, 2, 2, 10, fused
@@ -453,6 +454,7 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
         if name in ("fp64", "fp32_ir"):
             sols[name] = phi
         pa.call("pm_mg_destroy", mg)
+    pa.call("pm_level_sweep_form", 0)
     # Proto's library solver on the same problem: LevelMultigridSolver<BoxOp_Laplace>
     # (FAS, 4th-order flux-form operator, Jacobi), state with 2 ghost layers
     fas = ctypes.c_void_p()
@@ -495,6 +497,9 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
 
         def one_pass():
             pa.call("pm_level_gsrb_sweep", lay, dt, ph.ptr, g1, tm.ptr, g0, rh.ptr, g0, 1.0 / n, stream.h)
+
+        def one_pass_back():  # scratch (no ghosts) back into phi (one ghost layer), the solver's other half
+            pa.call("pm_level_gsrb_sweep", lay, dt, tm.ptr, g0, ph.ptr, g1, rh.ptr, g0, 1.0 / n, stream.h)
         ms2 = time_launches(two_colour, 5, stream)
         ms1 = time_launches(one_pass, 5, stream)
         alg = np.dtype(t).itemsize * 3 * n ** 3  # phi read, rhs read, phi written
@@ -507,7 +512,9 @@ def mg_config3(pa, stream, n, bs, nlev, reps=3, use_graph=1):
             for form, fname in ((1, "tile"), (2, "rows4"), (3, "rows8")):
                 pa.call("pm_level_sweep_form", form)
                 msf = time_launches(one_pass, 5, stream)
-                forms[fname] = {"ms": round(msf, 4), "frac": round(alg / (msf * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+                msb = time_launches(one_pass_back, 5, stream)
+                forms[fname] = {"ms": round(msf, 4), "frac": round(alg / (msf * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                "back_ms": round(msb, 4)}
         finally:
             pa.call("pm_level_sweep_form", 0)
         sweeps[tname]["by_kernel"] = forms
@@ -563,12 +570,13 @@ def batch_config4(pa, stream, dtype, reps=10, nboxes=2000, per_box=True):
     B.sweep_form(0)
     ms_a = time_launches(lambda: B.apply(st27, src.ptr, dst.ptr, 1.0, sp), reps, stream)
     ms_g = time_launches(lambda: B.gsrb(dtype, src.ptr, phi.ptr, rhs.ptr, dx, sp), reps, stream)
-    # the sweep by kernel: LDS ring, row sweep (4-row sub-tiles; 2-row; cached stores)
+    # the sweep by kernel: LDS ring, row sweep (4-row sub-tiles; 2-row; writing the rows' ghost cells), pair sweep
     forms = {}
     try:
         for form, waves, fname in ((1, 0, "lds_ring"), (2, 1, "rows4_w1"), (2, 4, "rows4_w4"), (2, 8, "rows4_w8"),
-                                   (3, 0, "rows2"), (4, 0, "rows4_cached"), (5, 1, "pairs4_w1"),
-                                   (5, 2, "pairs4_w2"), (5, 4, "pairs4_w4"), (5, 8, "pairs4_w8"), (6, 0, "pairs2")):
+                                   (3, 0, "rows2"), (4, 1, "rows4_ghostw_w1"), (4, 4, "rows4_ghostw_w4"),
+                                   (5, 1, "pairs4_w1"), (5, 2, "pairs4_w2"), (5, 4, "pairs4_w4"), (5, 8, "pairs4_w8"),
+                                   (6, 0, "pairs4_ghostw")):
             B.sweep_form(form, waves)
             msf = time_launches(lambda: B.gsrb(dtype, src.ptr, phi.ptr, rhs.ptr, dx, sp), reps, stream)
             forms[fname] = {"ms": round(msf, 4), "frac": round(gsrb_bytes / (msf * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}

@@ -275,8 +275,8 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
  * (process-wide; tests and benchmarks compare them): 0 automatic, 1 the LDS
  * tile kernel, 2 / 3 / 4 the row-register kernel with 4 / 8 / 2 rows per
  * thread.  pm_level_sweep_tune: the row kernel's largest planes per workgroup
- * (0 automatic) and non-temporal write-out (-1 automatic: FP64 only, 0 off,
- * 1 on); every setting gives the same bits. */
+ * (0 automatic) and non-temporal write-out into ghost-free outputs (-1
+ * automatic: on, 0 off, 1 on); every setting gives the same bits. */
 int pm_level_sweep_form(int form);
 int pm_level_sweep_tune(int planes_max, int nt_store);
 /* Damped point Jacobi of examples/_archived/LevelMultigrid/src/LevelMultigrid.cpp:99-123:
@@ -457,23 +457,28 @@ int pm_batch_apply(pm_batch* batch, const pm_stencil* st, const void* src, void*
 /* One red-black Gauss-Seidel sweep of the 7-point Laplacian on every box,
  * ghosts held fixed (LaplaceOp.H:95-123 without the exchange: the patches are
  * independent): phi_out (ghosted storage) receives the swept phi_in; the
- * ghosts of phi_out must equal those of phi_in (they are not written). */
+ * ghosts of phi_out must equal those of phi_in (some kernels write the x ghost
+ * cells of the rows they sweep, with phi_in's values; none writes any other
+ * ghost cell). */
 int pm_batch_gsrb(pm_batch* batch, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
                   pm_stream_t stream);
 /* Which kernel runs pm_batch_gsrb (tests and benchmarks compare them; every
- * form gives the same bits): 0 automatic (FP64: the pair sweep where every box
- * is even and 4..128 cells wide, the ghost even and the arrays pair-aligned,
- * else the row sweep where every box is 2..64 cells wide; FP32 the row sweep
- * first, then the pair sweep; else the LDS ring),
- * 1 the LDS-ring kernel, 2 the row sweep (a wave per 64 / nx packed sub-tiles
- * of 4 rows, rows in registers, one cell per lane), 3 the row sweep with
- * 2-row sub-tiles, 4 the row sweep with cached stores, 5 the pair sweep (two
- * cells per lane, 64 / (nx / 2) sub-tiles of 4 rows per wave; either dtype),
- * 6 the pair sweep with 2-row sub-tiles.  waves: workgroup size of forms 0, 2
- * and 5 in waves (0 automatic, 1, 2, 4 or 8; the waves of a workgroup take
- * neighbouring sub-tiles of one box and walk z together); forms 3, 4 and 6 run
- * one wave per workgroup.  No counterpart in the reference (one fixed kernel
- * per box). */
+ * form gives the same bits): 0 automatic (with 2 ghost layers form 6 where
+ * every box is even and 4..128 cells wide and the arrays pair-aligned; else
+ * FP64 the pair sweep (even ghost), then the row sweep where every box is 2..64
+ * cells wide; FP32 the row sweep first, then the pair sweep; else the LDS
+ * ring), 1 the LDS-ring kernel, 2
+ * the row sweep (a wave per 64 / nx packed sub-tiles of 4 rows, rows in
+ * registers, one cell per lane), 3 the row sweep with 2-row sub-tiles, 4 the
+ * row sweep that also writes the x ghost cells of its rows (ghost 1 or 2;
+ * phi_in's values) with cached stores, 5 the pair sweep (two cells per lane,
+ * 64 / (nx / 2) sub-tiles of 4 rows per wave; either dtype), 6 the pair sweep
+ * also writing the ghost pairs of its rows (ghost 2; phi_in's values) with
+ * cached stores, so that output rows fill whole cache lines.  waves: workgroup size of
+ * forms 0, 2, 4 and 5 in waves (0 automatic, 1, 2, 4 or 8; form 4: 1 or 4;
+ * the waves of a workgroup take neighbouring sub-tiles of one box and walk z
+ * together); forms 3 and 6 run one wave per workgroup.  No counterpart in the
+ * reference (one fixed kernel per box). */
 int pm_batch_sweep_form(pm_batch* batch, int form, int waves);
 
 /* ------------------------------------------------------ library multigrid */

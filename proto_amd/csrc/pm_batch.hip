@@ -404,7 +404,7 @@ struct RowItem {
 // with 8; FP64 pair sweep 0.696 ms with 1, 0.722 with 2, 0.803 with 4)
 constexpr int kRowsWaves = 4, kPairsWaves = 1;
 
-template<typename T, int RY, int NT, int WPB>
+template<typename T, int RY, int NT, int WPB, int PF, int GW>
 __global__ __launch_bounds__(64 * WPB) void k_batch_rows(const BatchBox* __restrict__ boxes,
                                                    const RowItem* __restrict__ items, int g,
                                                    const T* __restrict__ phi_in, T* __restrict__ phi_out,
@@ -454,9 +454,17 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows(const BatchBox* __restr
     const T* pb = phi_in + bx.goff;
     T* po = phi_out + bx.goff;
     const T* qb = rhs + bx.voff;
-    T P[5][NR];  // phi planes by slot: plane z in slot (z + 1) mod 5
-    T Q[5][NQ];  // rhs planes, same slots
-    T E[5][NQ];  // edge values (x - 1 for the first column, x + 1 for the last)
+    // planes are loaded PF steps ahead of their first use: a ring of 4 + PF slots
+    constexpr int R = 4 + PF;
+    T P[R][NR];  // phi planes by slot: plane z in slot (z + 1) mod R
+    T Q[R][NQ];  // rhs planes, same slots
+    T E[R][NQ];  // edge values (x - 1 for the first column, x + 1 for the last)
+    // GW: the x ghost cells of every written row are written too (with phi_in's
+    // values, which phi_out's ghosts hold already), so that rows fill whole
+    // cache lines; the second ghost column (x - 2, x + 2) of the output rows
+    // is loaded for it when g == 2 (other ghost widths write no ghost cells)
+    T G[R][GW ? RY : 1];
+    const bool gw = GW && g <= 2;
     auto loadP = [&](T (&d)[NR], int z) {
         const T* b = pb + Gxy * (long)(min(z, nz) + g);
 #pragma unroll
@@ -466,6 +474,13 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows(const BatchBox* __restr
         const T* b = pb + Gxy * (long)(min(z, nz) + g) + ed;
 #pragma unroll
         for (int j = 0; j < NQ; j++) { d[j] = b[ro[j + 1]]; }
+    };
+    auto loadG = [&](T (&d)[GW ? RY : 1], int z) {
+        if (GW && g == 2) {
+            const T* b = pb + Gxy * (long)(min(z, nz) + g) + 2 * ed;
+#pragma unroll
+            for (int r = 0; r < RY; r++) { d[r] = b[ro[r + 2]]; }
+        }
     };
     auto loadQ = [&](T (&d)[NQ], int z) {
         const T* b = qb + Vxy * (long)min(z, nz - 1);
@@ -494,11 +509,13 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows(const BatchBox* __restr
     };
     auto step = [&](auto sc, int k) __attribute__((always_inline)) {
         constexpr int S = decltype(sc)::value;
-        constexpr int S0 = S, S1 = (S + 1) % 5, S2 = (S + 2) % 5, S3 = (S + 3) % 5, SM = (S + 4) % 5;
+        constexpr int S0 = S, S1 = (S + 1) % R, S2 = (S + 2) % R, SM = (S + R - 1) % R;
+        constexpr int LP = (S + 2 + PF) % R, LQ = (S + 1 + PF) % R;
         if (WPB > 1) { __syncthreads(); }  // the item's waves walk z together
-        loadP(P[S3], k + 3);
-        loadQ(Q[S2], k + 2);
-        loadE(E[S2], k + 2);
+        loadP(P[LP], k + 2 + PF);
+        loadQ(Q[LQ], k + 1 + PF);
+        loadE(E[LQ], k + 1 + PF);
+        loadG(G[LQ], k + 1 + PF);
         if (k + 1 < nz) {  // red cells of plane k + 1, rows Y0 - 1 .. Y0 + RY
 #pragma unroll
             for (int j = 0; j < NQ; j++) {
@@ -527,28 +544,32 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows(const BatchBox* __restr
                     } else {
                         ob[ro[r + 2]] = P[S0][r + 2];
                     }
+                    if (GW && gw && ed != 0) {
+                        ob[ro[r + 2] + ed] = E[S0][r + 1];
+                        if (g == 2) { ob[ro[r + 2] + 2 * ed] = G[S0][r]; }
+                    }
                 }
             }
         }
     };
-    // planes -1, 0, 1 (slots 0..2), rhs and edges of plane 0 (slot 1)
-    loadP(P[0], -1);
-    loadP(P[1], 0);
-    loadP(P[2], 1);
-    loadQ(Q[1], 0);
-    loadE(E[1], 0);
+    // planes -1 .. PF (slots 0 .. PF + 1), rhs and edges of planes 0 .. PF - 1 (slots 1 .. PF)
+    static_for<0, PF + 2>([&](auto zc) { loadP(P[decltype(zc)::value], decltype(zc)::value - 1); });
+    static_for<0, PF>([&](auto zc) {
+        constexpr int z = decltype(zc)::value;
+        loadQ(Q[z + 1], z);
+        loadE(E[z + 1], z);
+        loadG(G[z + 1], z);
+    });
     int k = -1;
+    bool done = false;
     for (;;) {
-        step(std::integral_constant<int, 0>{}, k);
-        if (++k >= nz) { break; }
-        step(std::integral_constant<int, 1>{}, k);
-        if (++k >= nz) { break; }
-        step(std::integral_constant<int, 2>{}, k);
-        if (++k >= nz) { break; }
-        step(std::integral_constant<int, 3>{}, k);
-        if (++k >= nz) { break; }
-        step(std::integral_constant<int, 4>{}, k);
-        if (++k >= nz) { break; }
+        static_for<0, R>([&](auto sc) {
+            if (!done) {
+                step(sc, k);
+                done = ++k >= nz;
+            }
+        });
+        if (done) { break; }
     }
 }
 
@@ -573,7 +594,7 @@ struct Vec2<double> {
     typedef double type __attribute__((ext_vector_type(2)));
 };
 
-template<typename T, int RY, int NT, int WPB>
+template<typename T, int RY, int NT, int WPB, int GW>
 __global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __restrict__ boxes,
                                                     const RowItem* __restrict__ items, int g,
                                                     const T* __restrict__ phi_in, T* __restrict__ phi_out,
@@ -629,6 +650,13 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __rest
     T A[5][NR], B[5][NR];    // phi planes by slot (plane z in slot (z + 1) mod 5), columns 2q / 2q + 1
     T QA[5][NQ], QB[5][NQ];  // rhs, same slots
     T E[5][NQ];              // edge values
+    // GW (g == 2): the ghost pairs (x -2, -1) and (nx, nx + 1) of every written
+    // row are written too (phi_in's values, which phi_out's ghosts hold
+    // already), so that rows fill whole cache lines; the second ghost column
+    // of the output rows is loaded for it
+    T G[5][GW ? RY : 1];
+    const bool gw = GW && g == 2 && (ql || qr);
+    const unsigned gdu = ql ? (unsigned)(-2 * (int)sizeof(T)) : 3u * (unsigned)sizeof(T);
     auto loadP = [&](T (&a)[NR], T (&b)[NR], int z) {
         const char* base = reinterpret_cast<const char*>(pb + Gxy * (long)(min(z, nz) + g));
 #pragma unroll
@@ -642,6 +670,13 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __rest
         const char* base = reinterpret_cast<const char*>(pb + Gxy * (long)(min(z, nz) + g));
 #pragma unroll
         for (int j = 0; j < NQ; j++) { d[j] = *reinterpret_cast<const T*>(base + (ro[j + 1] + edu)); }
+    };
+    auto loadG = [&](T (&d)[GW ? RY : 1], int z) {
+        if (GW && g == 2) {
+            const char* base = reinterpret_cast<const char*>(pb + Gxy * (long)(min(z, nz) + g));
+#pragma unroll
+            for (int r = 0; r < RY; r++) { d[r] = *reinterpret_cast<const T*>(base + (ro[r + 2] + gdu)); }
+        }
     };
     auto loadQ = [&](T (&a)[NQ], T (&b)[NQ], int z) {
         const char* base = reinterpret_cast<const char*>(qb + Vxy * (long)min(z, nz - 1));
@@ -691,6 +726,7 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __rest
         loadP(A[S3], B[S3], k + 3);
         loadQ(QA[S2], QB[S2], k + 2);
         loadE(E[S2], k + 2);
+        loadG(G[S2], k + 2);
         if (k + 1 < nz) {
 #pragma unroll
             for (int j = 0; j < NQ; j++) {
@@ -727,6 +763,15 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __rest
                     } else {
                         *dst = v;
                     }
+                    if (GW && gw) {
+                        // the 32-bit offset is summed before it meets the pointer: the first
+                        // pair's offset steps back 2 elements and must wrap within 32 bits
+                        V gv;
+                        gv.x = ql ? G[S0][r] : E[S0][r + 1];
+                        gv.y = ql ? E[S0][r + 1] : G[S0][r];
+                        const unsigned go = ro[r + 2] + (ql ? (unsigned)(-2 * (int)sizeof(T)) : 2u * (unsigned)sizeof(T));
+                        *reinterpret_cast<V*>(ob + go) = gv;
+                    }
                 }
             }
         }
@@ -737,6 +782,7 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __rest
     loadP(A[2], B[2], 1);
     loadQ(QA[1], QB[1], 0);
     loadE(E[1], 0);
+    loadG(G[1], 0);
     int k = -1;
     for (;;) {
         step(integral_constant<int, 0>{}, integral_constant<int, 0>{}, k);
@@ -1066,16 +1112,17 @@ int pm_batch_sweep_form(pm_batch* B, int form, int waves)
     PM_REQUIRE(form < 2 || form > 4 || B->rows_ok, "pm_batch_sweep_form: the row sweep needs every box 2..64 cells wide");
     PM_REQUIRE(form < 5 || B->pairs_ok,
                "pm_batch_sweep_form: the pair sweep needs every box even and 4..128 cells wide, and an even ghost");
-    // 2-row sub-tiles and cached stores are measured with one wave per workgroup only
-    const bool one = form == 3 || form == 4 || form == 6;
-    PM_REQUIRE(!one || waves <= 1, "pm_batch_sweep_form: forms 3, 4 and 6 run one wave per workgroup");
+    // 2-row sub-tiles are measured with one wave per workgroup only
+    const bool one = form == 3 || form == 6;
+    PM_REQUIRE(!one || waves <= 1, "pm_batch_sweep_form: forms 3 and 6 run one wave per workgroup");
+    PM_REQUIRE(form != 4 || waves == 0 || waves == 1 || waves == 4, "pm_batch_sweep_form: form 4 runs 1 or 4 waves");
     const int rw = one ? 1 : (waves ? waves : kRowsWaves), pw = one ? 1 : (waves ? waves : kPairsWaves);
     int rc = PM_OK;
     const int ry = form == 3 ? 2 : 4;
     if (B->rows_ok && (B->rows_ry != ry || B->rows_wpb != rw) && form != 1 && form < 5) {
         rc = build_row_items(B, ry, false, rw);
     }
-    const int pry = form == 6 ? 2 : 4;
+    const int pry = 4;
     if (rc == PM_OK && B->pairs_ok && (B->pairs_ry != pry || B->pairs_wpb != pw) && (form == 0 || form >= 5)) {
         rc = build_row_items(B, pry, true, pw);
     }
@@ -1092,14 +1139,18 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
     PM_REQUIRE(B && phi_in && phi_out && rhs, "pm_batch_gsrb: null argument");
     PM_REQUIRE(phi_in != phi_out, "pm_batch_gsrb: the sweep is out of place (phi_in != phi_out)");
     PM_REQUIRE(dtype == PM_F32 || dtype == PM_F64, "pm_batch_gsrb: bad dtype");
-    // automatic: FP64 the pair sweep (where the arrays are pair-aligned), FP32 the
-    // row sweep (config 4: FP64 pairs 0.696 ms against rows 0.720, FP32 rows
-    // 0.397 ms against pairs 0.432), each falling back on the other, then the LDS ring
+    // automatic: with 2 ghost layers the pair sweep writing its rows' ghost pairs
+    // (config 4: FP64 0.615 ms, FP32 0.376 ms; without the ghost writes 0.698 /
+    // 0.435 ms, the row sweep 0.720 / 0.397 ms, the LDS ring 0.758 / 0.501 ms;
+    // with non-temporal stores 0.620 / 0.401 ms); otherwise FP64 the pair sweep
+    // and FP32 the row sweep, each falling back on the other, then the LDS ring.
+    // The pair sweep needs the arrays aligned to two elements.
     const size_t al = 2 * (dtype == PM_F64 ? 8 : 4);
     const bool aligned = (uintptr_t)phi_in % al == 0 && (uintptr_t)phi_out % al == 0 && (uintptr_t)rhs % al == 0;
     const int sf = B->sweep_form;
     if (sf >= 5) { PM_REQUIRE(aligned, "pm_batch_gsrb: the pair sweep needs arrays aligned to two elements"); }
-    const bool pairs_auto = B->pairs_ok && aligned && (dtype == PM_F64 || !B->rows_ok);
+    const bool pairs_auto = B->pairs_ok && aligned && (B->ghost == 2 || dtype == PM_F64 || !B->rows_ok);
+    const bool ghost_pairs = sf == 6 || (sf == 0 && B->ghost == 2);
     if (sf >= 5 || (sf == 0 && pairs_auto)) {
         double lam = -(dx * dx) / (4.0 * 3);
         lam *= 2.0;
@@ -1114,16 +1165,16 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
                                    (const BatchBox*)B->d_boxes, (const RowItem*)B->d_pitems, B->ghost,
                                    (const T*)phi_in, (T*)phi_out, (const T*)rhs, cc, lamT);
             };
-            if (B->pairs_ry == 2) {
-                run(k_batch_rows2<T, 2, 1, 1>);
+            if (ghost_pairs) {
+                run(k_batch_rows2<T, 4, 0, 1, 1>);
             } else if (B->pairs_wpb == 2) {
-                run(k_batch_rows2<T, 4, 1, 2>);
+                run(k_batch_rows2<T, 4, 1, 2, 0>);
             } else if (B->pairs_wpb == 4) {
-                run(k_batch_rows2<T, 4, 1, 4>);
+                run(k_batch_rows2<T, 4, 1, 4, 0>);
             } else if (B->pairs_wpb == 8) {
-                run(k_batch_rows2<T, 4, 1, 8>);
+                run(k_batch_rows2<T, 4, 1, 8, 0>);
             } else {
-                run(k_batch_rows2<T, 4, 1, 1>);
+                run(k_batch_rows2<T, 4, 1, 1, 0>);
             }
         };
         if (dtype == PM_F64) {
@@ -1151,17 +1202,21 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
                                    (const T*)phi_in, (T*)phi_out, (const T*)rhs, cc, lamT);
             };
             if (form == 3) {
-                run(k_batch_rows<T, 2, 1, 1>);
+                run(k_batch_rows<T, 2, 1, 1, 1, 0>);
             } else if (form == 4) {
-                run(k_batch_rows<T, 4, 0, 1>);
+                if (B->rows_wpb == 4) {
+                    run(k_batch_rows<T, 4, 0, 4, 1, 1>);
+                } else {
+                    run(k_batch_rows<T, 4, 0, 1, 1, 1>);
+                }
             } else if (B->rows_wpb == 2) {
-                run(k_batch_rows<T, 4, 1, 2>);
+                run(k_batch_rows<T, 4, 1, 2, 1, 0>);
             } else if (B->rows_wpb == 4) {
-                run(k_batch_rows<T, 4, 1, 4>);
+                run(k_batch_rows<T, 4, 1, 4, 1, 0>);
             } else if (B->rows_wpb == 8) {
-                run(k_batch_rows<T, 4, 1, 8>);
+                run(k_batch_rows<T, 4, 1, 8, 1, 0>);
             } else {
-                run(k_batch_rows<T, 4, 1, 1>);
+                run(k_batch_rows<T, 4, 1, 1, 1, 0>);
             }
         };
         if (dtype == PM_F64) {

@@ -1,6 +1,7 @@
 // Internal helpers shared by the protomi HIP sources (not part of the C ABI).
 #pragma once
 
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -154,6 +155,16 @@ __host__ __device__ constexpr int lap27_off(int j, int d)
 template<int SHAPE> __host__ __device__ constexpr int shape_off(int j, int d)
 {
     return SHAPE == 7 ? lap7_off(j, d) : (SHAPE == 27 ? lap27_off(j, d) : 0);
+}
+
+// Compile-time loop: f(integral_constant<int, I>) for I = B .. E - 1.
+template<int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
 }
 
 // x neighbours across a wave (row-register sweeps): lane l takes lane l - 1

@@ -131,7 +131,7 @@ def test_batch_row_sweep_vs_oracle(dt, ghost):
     want1 = oracle_gsrb_batch(B, phi, rhs, dx)
     want2 = oracle_gsrb_batch(B, want1, rhs, dx)
     p_in, r_d = dev(phi), dev(rhs)
-    for form, waves in ((0, 0), (2, 1), (2, 2), (2, 8), (3, 0), (4, 0), (1, 0)):
+    for form, waves in ((0, 0), (2, 1), (2, 2), (2, 8), (3, 0), (4, 1), (4, 4), (1, 0)):
         B.sweep_form(form, waves)
         say("batch row sweep", dt, "ghost", ghost, "form", form, "waves", waves)
         p_out = dev(phi)

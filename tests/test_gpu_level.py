@@ -228,10 +228,12 @@ def test_row_and_tile_sweeps_same_solve(precision, n, nlev):
             res[form] = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs)
     finally:
         pa.call("pm_level_sweep_form", 0)
-    (rc1, it1, h1, p1), (rc0, it0, h0, p0) = res[1], res[0]
-    assert rc1 == 0 and rc0 == 0 and it1 == it0
-    assert h1.tobytes() == h0.tobytes()
-    assert p1.tobytes() == p0.tobytes()
+    (rc1, it1, h1, p1) = res[1]
+    for form in (0,):
+        (rc0, it0, h0, p0) = res[form]
+        assert rc1 == 0 and rc0 == 0 and it1 == it0, form
+        assert h1.tobytes() == h0.tobytes(), form
+        assert p1.tobytes() == p0.tobytes(), form
 
 
 def test_gsrb_sweep_unsupported_layouts():
