@@ -390,6 +390,8 @@ typedef struct pm_mg_options {
                        bits; 3-D periodic levels) instead of exchange / red / exchange / black;
                        one-box levels of up to 4096 cells run all sweeps of a relaxation in
                        one launch (same bits) */
+    int inner;      /* precision 1: FP32 V-cycles per refinement step (the preconditioner; 0 or
+                       1 = one, at most 4); each starts from the previous one's correction */
 } pm_mg_options;
 void pm_mg_default_options(pm_mg_options* opt);
 /* nlevels = number of coarsenings (the numLevels argument of LevelMultigrid). */
@@ -540,6 +542,8 @@ typedef struct pm_server_options {
     int pre, post, bottom; /* GSRB sweeps (even counts) */
     int max_iter;       /* outer iterations before PM_SERVE_NOT_CONVERGED */
     double sigma;       /* Gaussian width */
+    int inner;          /* mode 0: FP32 V-cycles per refinement step (0 or 1 = one, at most 4),
+                           as pm_mg_options.inner */
 } pm_server_options;
 typedef struct pm_request {
     double arrival_s;   /* seconds after the start of pm_server_run */

@@ -54,7 +54,8 @@ class MgOptions(ctypes.Structure):
     """pm_mg_options."""
     _fields_ = [("relax", ctypes.c_int), ("pre", ctypes.c_int), ("post", ctypes.c_int), ("bottom", ctypes.c_int),
                 ("interp", ctypes.c_int), ("precision", ctypes.c_int), ("max_iter", ctypes.c_int),
-                ("tol", ctypes.c_double), ("use_graph", ctypes.c_int), ("fused", ctypes.c_int)]
+                ("tol", ctypes.c_double), ("use_graph", ctypes.c_int), ("fused", ctypes.c_int),
+                ("inner", ctypes.c_int)]
 
 
 _OP = ctypes.POINTER(MgOptions)
@@ -71,7 +72,7 @@ class BatchConfig(ctypes.Structure):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("mode", ctypes.c_int),
                 ("pre", ctypes.c_int), ("post", ctypes.c_int), ("bottom", ctypes.c_int), ("max_iter", ctypes.c_int),
-                ("sigma", ctypes.c_double)]
+                ("sigma", ctypes.c_double), ("inner", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -363,11 +364,11 @@ class Server:
     FP32-preconditioned FP64 refinement; mode 1: plain FP64, one at a time."""
 
     def __init__(self, mode=0, hbm_budget=200 << 30, max_active=64, max_iter=30, sigma=0.05, pre=2, post=2,
-                 bottom=10):
+                 bottom=10, inner=1):
         o = ServerOptions()
         lib().pm_server_default_options(ctypes.byref(o))
         o.mode, o.hbm_budget, o.max_active, o.max_iter, o.sigma = mode, hbm_budget, max_active, max_iter, sigma
-        o.pre, o.post, o.bottom = pre, post, bottom
+        o.pre, o.post, o.bottom, o.inner = pre, post, bottom, inner
         h = _vp()
         call("pm_server_create", ctypes.byref(h), ctypes.byref(o))
         self.h = h

@@ -247,13 +247,19 @@ static int residual32(pm_mg* mg, void* phi, const void* rhs, hipStream_t s)
 }
 
 // One step of the FP32-preconditioned refinement after the residual r32 of
-// level 0 is in place: e = 0, FP32 V-cycle, phi += e.
+// level 0 is in place: e = 0, `inner` FP32 V-cycles on A e = r32 (each from
+// the previous one's e), phi += e.
 static int ir_correct(pm_mg* mg, void* phi, hipStream_t s)
 {
     auto& L = mg->lv[0];
+    const int inner = mg->opt.inner > 1 ? mg->opt.inner : 1;
     // e = 0 folded into the first sweep, phi += e into the last where it can be
     bool folded = false;
-    MG_TRY(vcycle(mg, 0, PM_F32, L.phi32, L.rhs32, s, true, (double*)phi, &folded));
+    for (int c = 0; c < inner; c++) {
+        const bool last = c == inner - 1;
+        MG_TRY(vcycle(mg, 0, PM_F32, L.phi32, L.rhs32, s, c == 0, last ? (double*)phi : nullptr,
+                      last ? &folded : nullptr));
+    }
     if (!folded) { MG_TRY(pm_level_add_correction(L.layout, PM_F64, phi, mg->g1, PM_F32, L.phi32, mg->g1, s)); }
     return PM_OK;
 }
@@ -321,6 +327,7 @@ void pm_mg_default_options(pm_mg_options* o)
     o->tol = 1e-10;
     o->use_graph = 0;
     o->fused = 1;
+    o->inner = 1;
 }
 
 int pm_mg_create(pm_mg** out, const pm_layout* fine, int nlevels, double dx, const pm_mg_options* opt)
@@ -331,6 +338,7 @@ int pm_mg_create(pm_mg** out, const pm_layout* fine, int nlevels, double dx, con
     PM_REQUIRE(opt->relax == 0 || opt->relax == 1, "pm_mg_create: relax must be 0 (GSRB) or 1 (Jacobi)");
     PM_REQUIRE(opt->precision == 0 || opt->precision == 1, "pm_mg_create: precision must be 0 or 1");
     PM_REQUIRE(opt->interp == 0 || opt->interp == 1, "pm_mg_create: interp must be 0 or 1");
+    PM_REQUIRE(opt->inner >= 0 && opt->inner <= 4, "pm_mg_create: inner must be 0..4");
     PM_REQUIRE(opt->pre >= 0 && opt->post >= 0 && opt->bottom >= 0 && opt->max_iter >= 0, "pm_mg_create: bad counts");
     PM_REQUIRE(dx > 0, "pm_mg_create: dx must be positive");
     int dim, domain[3], bs[3], periodic[3], nboxes;

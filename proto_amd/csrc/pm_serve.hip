@@ -803,9 +803,17 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
     // levels below the finest of at most SMALL_CELLS cells relax in one launch
     // each (k_srv_small), all requests' such levels of one depth together
     auto small = [&](const Active& a, int d) { return d > 0 && (long)a.ns[d] * a.ns[d] * a.ns[d] <= SMALL_CELLS; };
-    // V-cycle of every non-fresh request (fresh ones only get their first residual)
+    // `inner` V-cycles of every non-fresh request (fresh ones only get their first
+    // residual): the first takes the zero initial guess on every level, the later
+    // ones start level 0 from the previous cycle's correction; only the last
+    // cycle's last sweep of level 0 adds e into the FP64 solution
+    const int inner = S->opt.inner > 1 ? S->opt.inner : 1;
+    for (int cyc = 0; cyc < inner; cyc++) {
+    const bool last_cycle = cyc == inner - 1;
     for (int d = 0; d <= maxd; d++) {
-        begin(5, 0, zero_in ? 1 : 0);
+        // level 0 of a later cycle reads its correction; coarser levels start from zero
+        const bool zin = zero_in && (d > 0 || cyc == 0);
+        begin(5, 0, zin ? 1 : 0);
         for (size_t r = 0; r < act.size(); r++) {
             const Active& a = act[r];
             const int bd = (int)a.ns.size() - 1;
@@ -816,7 +824,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
         end();
         const int nsw = std::max(pre, bot);
         for (int sw = 0; sw < nsw; sw++) {
-            begin(0, sw & 1, sw == 0 && zero_in ? 1 : 0);
+            begin(0, sw & 1, sw == 0 && zin ? 1 : 0);
             for (size_t r = 0; r < act.size(); r++) {
                 const Active& a = act[r];
                 const int bd = (int)a.ns.size() - 1;
@@ -856,7 +864,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
         for (int sw = 0; sw < post; sw++) {
             // the last sweep of level 0 adds e into the FP64 solution
             const int in = sw == 0 && fold_prolong ? 2 : 0;
-            begin(0, sw & 1, d == 0 && sw == post - 1 && in == 0 ? 3 : in);
+            begin(0, sw & 1, last_cycle && d == 0 && sw == post - 1 && in == 0 ? 3 : in);
             for (size_t r = 0; r < act.size(); r++) {
                 const Active& a = act[r];
                 if (a.fresh || d >= (int)a.ns.size() - 1 || small(a, d)) { continue; }
@@ -864,6 +872,7 @@ int build_plan(pm_server* S, std::vector<Active>& act, hipStream_t s)
             }
             end();
         }
+    }
     }
     // phi += e where no post sweep of level 0 added it (single-level requests,
     // or no post sweeps, or a post count of 1 whose only sweep takes the
@@ -1288,7 +1297,7 @@ int run_plain(pm_server* S, int nreq, const pm_request* reqs, pm_request_stats* 
 // layout.  The FP32 hierarchy is not saved: each V-cycle starts it from zero,
 // and the right-hand side is regenerated from the request.
 constexpr char kCkMagic[8] = {'P', 'M', 'S', 'R', 'V', 'C', 'K', '1'};
-constexpr uint32_t kCkVersion = 1;
+constexpr uint32_t kCkVersion = 2;
 
 struct CkHead {
     char magic[8];
@@ -1298,6 +1307,7 @@ struct CkHead {
     double elapsed;
     int32_t pre, post, bottom, max_iter;
     double sigma;
+    int32_t inner, pad;
 };
 struct CkActive {
     int32_t idx, n, iters, fresh;
@@ -1372,6 +1382,8 @@ int ck_write(pm_server* S, char* buf, const CkLayout& L, hipStream_t s)
     h.bottom = S->opt.bottom;
     h.max_iter = S->opt.max_iter;
     h.sigma = S->opt.sigma;
+    h.inner = S->opt.inner > 1 ? S->opt.inner : 1;
+    h.pad = 0;
     std::memcpy(buf, &h, sizeof(h));
     std::memcpy(buf + L.stats, R.stats.data(), sizeof(pm_request_stats) * R.stats.size());
     int32_t* w = (int32_t*)(buf + L.waiting);
@@ -1411,8 +1423,10 @@ int ck_read(pm_server* S, const char* buf, size_t bytes, hipStream_t s)
                    h.next <= (uint64_t)h.nreq,
                "pm_server_restore: inconsistent checkpoint header");
     PM_REQUIRE(h.pre == S->opt.pre && h.post == S->opt.post && h.bottom == S->opt.bottom &&
-                   h.max_iter == S->opt.max_iter && h.sigma == S->opt.sigma,
-               "pm_server_restore: the checkpoint was written by a server with other sweep counts / max_iter / sigma");
+                   h.max_iter == S->opt.max_iter && h.sigma == S->opt.sigma &&
+                   h.inner == (S->opt.inner > 1 ? S->opt.inner : 1),
+               "pm_server_restore: the checkpoint was written by a server with other sweep counts / max_iter / sigma "
+               "/ inner cycles");
     PM_REQUIRE(h.nactive <= S->opt.max_active, "pm_server_restore: %d requests in flight, max_active is %d", h.nactive,
                S->opt.max_active);
     HeldRun R;
@@ -1534,6 +1548,7 @@ void pm_server_default_options(pm_server_options* o)
     o->bottom = 10;
     o->max_iter = 30;
     o->sigma = 0.05;
+    o->inner = 1;
 }
 
 int pm_server_create(pm_server** out, const pm_server_options* o)
@@ -1546,6 +1561,7 @@ int pm_server_create(pm_server** out, const pm_server_options* o)
                    o->bottom % 2 == 0,
                "pm_server_create: sweep counts must be even");
     PM_REQUIRE(o->max_iter >= 1 && o->sigma > 0, "pm_server_create: bad max_iter / sigma");
+    PM_REQUIRE(o->inner >= 0 && o->inner <= 4, "pm_server_create: inner must be 0..4");
     pm_server* S = new pm_server();
     S->opt = *o;
     if (hipMalloc((void**)&S->d_norms, sizeof(unsigned long long) * o->max_active) != hipSuccess ||

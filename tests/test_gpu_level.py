@@ -104,12 +104,14 @@ def test_exchange_with_hole_leaves_unowned_ghosts():
 
 
 def _solve(n, bs, nlev, relax, linear, precision=0, maxiter=20, use_graph=False, tol=1e-10, rhs=None, stream=None,
-           fused=None, sweeps=(2, 2, 10)):
+           fused=None, sweeps=(2, 2, 10), inner=None):
     L = Layout(n, bs)
     opt = pa.MgOptions()
     pa.lib().pm_mg_default_options(ctypes.byref(opt))
     if fused is not None:
         opt.fused = int(fused)
+    if inner is not None:
+        opt.inner = int(inner)
     opt.relax, opt.pre, opt.post, opt.bottom, opt.interp = relax, sweeps[0], sweeps[1], sweeps[2], linear
     opt.precision, opt.max_iter, opt.tol, opt.use_graph = precision, maxiter, tol, int(use_graph)
     mg = ctypes.c_void_p()
@@ -131,6 +133,36 @@ def test_default_options_use_the_fused_sweep():
     opt = pa.MgOptions()
     pa.lib().pm_mg_default_options(ctypes.byref(opt))
     assert opt.fused == 1  # so every solve test below runs the one-pass sweep
+    assert opt.inner == 1  # one FP32 V-cycle per refinement step
+
+
+@pytest.mark.parametrize("inner", [2, 3])
+def test_refinement_inner_cycles(inner):
+    """Two or three FP32 V-cycles per refinement step (each from the previous
+    one's correction): the solve reaches the tolerance in fewer refinement
+    steps, each step cutting the FP64 residual by more than one cycle does;
+    the answer agrees with the plain FP64 solve to the refinement's bound;
+    inner 1 is the default bit for bit; graph replay gives the eager bits."""
+    n, bs, nlev = 64, 32, 3
+    rhs = util.gaussian_rhs(n, 6, 2)
+    rc1, it1, h1, p1 = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs)
+    rcd, itd, hd, pd = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs, inner=1)
+    assert rc1 == 0 and rcd == 0 and it1 == itd and h1.tobytes() == hd.tobytes() and p1.tobytes() == pd.tobytes()
+    rci, iti, hi, pi = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs, inner=inner)
+    say("inner", inner, "steps", iti, "vs", it1, "rel residual", hi[-1] / hi[0])
+    assert rci == 0 and iti < it1 and hi[-1] <= 1e-10 * hi[0]
+    assert np.all(hi[1:] / hi[:-1] < (h1[1:] / h1[:-1]).max())
+    from proto_amd.device import Stream
+    s = Stream()  # graphs are captured on a created stream, not the null stream
+    rcg, itg, hg, pg = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs, inner=inner, use_graph=True, stream=s.h)
+    assert rcg == 0, pa.lib().pm_last_error()
+    assert itg == iti and hg.tobytes() == hi.tobytes() and pg.tobytes() == pi.tobytes()
+    # both stop at a relative residual of 1e-10, so they differ from the FP64
+    # solve by what that residual leaves: as much as the one-cycle refinement does
+    rc64, it64, h64, p64 = _solve(n, bs, nlev, 0, 0, precision=0, rhs=rhs)
+    e1, ei = np.abs(p1 - p64).max(), np.abs(pi - p64).max()
+    say("inner", inner, "error vs FP64", ei, "one cycle", e1, "scale", np.abs(p64).max())
+    assert rc64 == 0 and ei < 1e-7 * np.abs(p64).max() and ei < 10 * e1
 
 
 @pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])

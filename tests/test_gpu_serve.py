@@ -28,7 +28,7 @@ def levels(n):
     return len(ns) - 1
 
 
-def mg_solve(n, ngauss, seed, tol, precision, sigma=0.05):
+def mg_solve(n, ngauss, seed, tol, precision, sigma=0.05, inner=1):
     """The multigrid driver on a one-box periodic layout; returns (phi n^3, iters)."""
     lay = ctypes.c_void_p()
     pa.call("pm_layout_create", ctypes.byref(lay), 3, pa.i3([n] * 3), pa.i3([n] * 3), pa.i3([1] * 3), 1,
@@ -39,6 +39,7 @@ def mg_solve(n, ngauss, seed, tol, precision, sigma=0.05):
     pa.lib().pm_mg_default_options(ctypes.byref(opt))
     opt.relax, opt.pre, opt.post, opt.bottom = 0, 2, 2, 10
     opt.precision, opt.max_iter, opt.tol, opt.use_graph = precision, 30, tol, 0
+    opt.inner = inner
     mg = ctypes.c_void_p()
     pa.call("pm_mg_create", ctypes.byref(mg), lay, levels(n), 1.0 / n, ctypes.byref(opt))
     phi = zeros((n + 2) ** 3)
@@ -66,6 +67,24 @@ def test_served_equals_driver_fp32_ir(n, ngauss, seed):
     phi_m, it = mg_solve(n, ngauss, seed, 1e-8, 1)
     assert it == st[0]["iters"]
     assert phi_s.tobytes() == phi_m.tobytes(), np.abs(phi_s - phi_m).max()
+
+
+@pytest.mark.parametrize("inner", [2, 4])
+def test_served_inner_cycles_equal_driver(inner):
+    """Several FP32 V-cycles per refinement step: the served answer and its
+    step count equal the driver's with the same inner count, alone and in a
+    mix of sizes (single-level and multi-level requests)."""
+    reqs = [req(64, 9, 51), req(32, 5, 52, 0.0005), req(96, 12, 53, 0.001), req(8, 3, 54, 0.0015)]
+    srv = pa.Server(mode=0, inner=inner)
+    st = srv.run(reqs)
+    for i, r in enumerate(reqs):
+        say("inner", inner, "n", r["n"], st[i])
+        assert st[i]["status"] == pa.SERVE_SOLVED and st[i]["rel_residual"] < 1e-8
+        phi_m, it = mg_solve(r["n"], r["ngauss"], r["seed"], 1e-8, 1, inner=inner)
+        assert it == st[i]["iters"]
+        assert srv.result(i, r["n"]).tobytes() == phi_m.tobytes()
+    one = pa.Server(mode=0).run([reqs[0]])
+    assert st[0]["iters"] < one[0]["iters"]
 
 
 def test_alone_equals_in_mix():
