@@ -797,6 +797,16 @@ def main():
         print(json.dumps(mg_one_solve(pa, stream, args.only)))
         return
 
+    if args.only in ("serve_batched", "serve_plain"):
+        # profiling helper: the config-5 stream (300 requests at 200/s) once
+        # through the batched FP32-IR server or the plain FP64 one
+        srv = pa.Server(mode=0 if args.only == "serve_batched" else 1, hbm_budget=200 << 30, max_active=64)
+        t0 = time.perf_counter()
+        st = srv.run(serve_trace(300), stream.h)
+        ok = sum(s_["status"] == pa.SERVE_SOLVED for s_ in st)
+        print(json.dumps({"only": args.only, "solved": ok, "wall_s": round(time.perf_counter() - t0, 3)}))
+        return
+
     if args.only == "batch":
         # profiling helper: the config-4 batched operators (both precisions)
         # and a plain 1 GiB device copy to calibrate the counters
