@@ -15,6 +15,12 @@
 
 namespace pm {
 
+// The z-marching level kernels split z into chunks until a launch has enough
+// workgroups; small levels, whose launches are bound by the latency of the z
+// walk, go down to chunks of kZcFloor planes.
+constexpr int kZcFloor = 2;
+
+
 // ------------------------------------------------------------ exchange
 template<typename T>
 __global__ __launch_bounds__(256) void k_exchange(LayoutDev L, G3 g, T* data, int ncomp)
@@ -1296,7 +1302,7 @@ static dim3 rz_grid(const pm_layout* L, int* zc)
 {
     const int nyt = (L->bs[1] + RZ_TY - 1) / RZ_TY;
     int z = L->bs[2];
-    while (z > 8 && (long)nyt * ((L->bs[2] + z - 1) / z) * L->nboxes < 2048) { z = (z + 1) / 2; }
+    while (z > kZcFloor && (long)nyt * ((L->bs[2] + z - 1) / z) * L->nboxes < 2048) { z = (z + 1) / 2; }
     *zc = z;
     return dim3((unsigned)(nyt * ((L->bs[2] + z - 1) / z)), (unsigned)L->nboxes);
 }
@@ -1748,7 +1754,7 @@ static int g_rows_zc_max = 0, g_rows_nt = -1;
 static int rows_zc(const pm_layout* L, int ry)
 {
     int zc = L->bs[2];
-    while (zc % 2 == 0 && zc > 8 && (long)(L->domain[1] / ry) * (L->domain[2] / zc) * L->nb[0] < 8192) { zc /= 2; }
+    while (zc % 2 == 0 && zc > kZcFloor && (long)(L->domain[1] / ry) * (L->domain[2] / zc) * L->nb[0] < 8192) { zc /= 2; }
     while (g_rows_zc_max > 0 && zc > g_rows_zc_max && zc % 2 == 0) { zc /= 2; }
     return zc;
 }
@@ -1918,7 +1924,7 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
     }
     const int nyt = (L->bs[1] + rows - 1) / rows;
     int zc = L->bs[2];
-    while (zc > 8 && (long)L->nboxes * nxt * nyt * ((L->bs[2] + zc - 1) / zc) < 2048) { zc = (zc + 1) / 2; }
+    while (zc > kZcFloor && (long)L->nboxes * nxt * nyt * ((L->bs[2] + zc - 1) / zc) < 2048) { zc = (zc + 1) / 2; }
     const G3 gi = g3(in_ghost), go = g3(out_ghost), gr = g3(rhs_ghost), gc = g3(coarse_ghost);
     const LayoutDev D = ldev(L);
     const LayoutDev DC = in_mode == 2 ? ldev(C) : D;
