@@ -19,6 +19,9 @@ namespace pm {
 // workgroups; small levels, whose launches are bound by the latency of the z
 // walk, go down to chunks of kZcFloor planes.
 constexpr int kZcFloor = 2;
+// waves the row sweep aims at per launch: 16 per CU (four per SIMD, what its
+// FP32 register count allows); fewer z chunks mean fewer warm-up planes
+constexpr long kRowsWaves = 4096;
 
 
 // ------------------------------------------------------------ exchange
@@ -1754,7 +1757,7 @@ static int g_rows_zc_max = 0, g_rows_nt = -1;
 static int rows_zc(const pm_layout* L, int ry)
 {
     int zc = L->bs[2];
-    while (zc % 2 == 0 && zc > kZcFloor && (long)(L->domain[1] / ry) * (L->domain[2] / zc) * L->nb[0] < 8192) { zc /= 2; }
+    while (zc % 2 == 0 && zc > kZcFloor && (long)(L->domain[1] / ry) * (L->domain[2] / zc) * L->nb[0] < kRowsWaves) { zc /= 2; }
     while (g_rows_zc_max > 0 && zc > g_rows_zc_max && zc % 2 == 0) { zc /= 2; }
     return zc;
 }
@@ -1883,11 +1886,13 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
     }
     for (int k = 0; k < 3; k++) { PM_REQUIRE(L->bs[k] >= 2, "pm_level_gsrb_sweep: boxes need 2+ cells per side"); }
     {
-        // 4 rows per thread: 512^3 in 64^3 boxes swept in 0.389 ms (FP32) and
-        // 0.683 ms (FP64) against 0.411 / 1.61 ms with 8 rows (FP64 spills)
-        // and 0.501 / 0.724 ms with the tile sweep
-        const int ry = g_sweep_form == 3 ? 8 : (g_sweep_form == 4 ? 2 : 4);
+        // 4 rows per thread: 512^3 in 64^3 boxes swept in 0.385 ms (FP32) and
+        // 0.682 ms (FP64) against 0.392 / 1.24 ms with 8 rows (FP64 spills) and
+        // 0.500 / 0.729 ms with the tile sweep.  Back into a ghost-1 phi FP32 is
+        // faster with 8 rows alone (0.367 against 0.390 ms), but not in the
+        // solve (FP32-IR 0.0318 against 0.0312 s), so 4 rows everywhere
         const int ghosts = rows_ghosts(in_ghost, out_ghost, rhs_ghost, in_mode);
+        const int ry = g_sweep_form == 3 ? 8 : (g_sweep_form == 4 ? 2 : 4);
         bool coarse_ok = in_mode != 2 || ghosts == 10;
         if (in_mode == 2) {
             // the prolongation input reads a coarse level with one ghost layer (checked box-aligned above)
