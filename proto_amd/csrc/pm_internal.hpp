@@ -66,6 +66,12 @@ int level_residual_nb(const pm_layout* L, int dtype, const void* phi, const int 
 int level_restrict_nb(const pm_layout* F, const pm_layout* C, int dtype, const void* phi, const int phi_ghost[3],
                       const void* rhs, const int rhs_ghost[3], double dx, void* resc, const int resc_ghost[3],
                       hipStream_t stream);
+// The multigrid driver's second pre-smoothing sweep (scratch, ghost 0 -> phi,
+// ghost 1) with the residual restriction of level_restrict_nb in the same pass;
+// PM_ERR_UNSUPPORTED where the level does not qualify (pm_level.hip).
+int level_sweep_restrict(const pm_layout* F, const pm_layout* C, int dtype, const void* phi_in,
+                         const int in_ghost[3], void* phi_out, const int out_ghost[3], const void* rhs,
+                         const int rhs_ghost[3], double dx, void* resc, const int resc_ghost[3], hipStream_t stream);
 // Kernel family (0 generic, 7, 27), dtype and coefficients of a stencil (pm_stencil.hip).
 int stencil_info(const pm_stencil* st, int* kind, int* dtype, std::vector<double>* coefs);
 

@@ -704,6 +704,268 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
 #undef PM_ROW_BASES
 }
 
+// The second pre-smoothing sweep of a V-cycle level with the residual
+// restriction that follows it (coarseResidual: r = f - L phi, averaged onto the
+// coarse level) in the same pass: k_gsrb_rows for input scratch (no ghosts) ->
+// phi (one ghost layer), plus, one plane behind the sweep, the residual of the
+// swept phi and its 2 x 2 x 2 averages written into the coarse right-hand side,
+// with k_residual_nb's arithmetic and order.  The residual of the tile's rows
+// needs the swept values one row above and below the tile and one plane beyond
+// the z chunk at each end, which neighbouring tiles own: the tile recomputes
+// them (red cells of rows Y0 - 2 .. Y0 + RY + 1, black cells of rows Y0 - 1 ..
+// Y0 + RY, planes Z0 - 1 .. Z1), so each thread holds RY + 6 rows and the ring
+// six planes.  Rows and planes outside the tile are never written.
+template<typename T, int RY>
+__global__ __launch_bounds__(512) void k_gsrb_rows_res(LayoutDev L, const T* __restrict__ pin, T* __restrict__ pout,
+                                                       const T* __restrict__ rhs, Lap<T> c, T lambda, int zc,
+                                                       Lap<T> cr, T avg, LayoutDev C, G3 gc, T* __restrict__ resc)
+{
+    constexpr int H = 3;                // halo rows held on each side
+    constexpr int NR = RY + 2 * H;      // phi rows: Y0 - 3 .. Y0 + RY + 2
+    constexpr int NQ = NR - 2;          // rhs rows and red rows: Y0 - 2 .. Y0 + RY + 1
+    constexpr int NB = RY + 2;          // black rows: Y0 - 1 .. Y0 + RY
+    constexpr long X = 64, XO = 66;     // rows of the ghost-free arrays / of phi
+    constexpr int R = 6;                // phi plane ring: k - 2 .. k + 3 at step k
+    constexpr int RQ = 3;               // rhs plane ring: k .. k + 2 at step k
+    __shared__ T eo[2][8][2][NQ];  // old values of a plane's edge lanes (0, 63), red rows
+    __shared__ T er[2][8][2][NB];  // red-updated edge values, black rows
+    __shared__ T ef[2][8][2][RY];  // final edge values, tile rows
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nw = L.nb[0], nb1 = L.nb[1], nb2 = L.nb[2];
+    const int bs1 = L.bs[1], bs2 = L.bs[2], dom1 = L.domain[1];
+    const int wl = w == 0 ? nw - 1 : w - 1, wr = w == nw - 1 ? 0 : w + 1;
+    const int nyt = dom1 / RY;
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const int Y0 = (int)(t % nyt) * RY, Z0 = (int)(t / nyt) * zc, Z1 = Z0 + zc;
+    const int by = Y0 / bs1, bz = Z0 / bs2, y0 = Y0 - by * bs1;
+    const T c0 = c.c[0], c1 = c.c[1], c2 = c.c[2], c3 = c.c[3], c4 = c.c[4], c5 = c.c[5], c6 = c.c[6];
+    const T r0 = cr.c[0], r1 = cr.c[1], r2 = cr.c[2], r3 = cr.c[3], r4 = cr.c[4], r5 = cr.c[5], r6 = cr.c[6];
+    // the three rows above the tile lie in the box above when the tile starts its
+    // box, the three below in the box below when it ends it (box heights are
+    // multiples of RY >= 4)
+    const int ytop = y0 == 0 ? -1 : 0, ybot = y0 + RY == bs1 ? 1 : 0;
+    const int* table = L.table;
+    auto box_at = [&](int ys, int zs) __attribute__((always_inline)) {
+        const int yb = (by + ys + nb1) % nb1, zb = (bz + zs + nb2) % nb2;
+        return table[w + nw * (yb + nb1 * zb)];
+    };
+    const int tm = box_at(ytop, -1), t0 = box_at(ytop, 0), tp = box_at(ytop, 1);
+    const int mm = box_at(0, -1), m0 = box_at(0, 0), mp = box_at(0, 1);
+    const int bm = box_at(ybot, -1), b0 = box_at(ybot, 0), bp = box_at(ybot, 1);
+    // row i (phi) sits at its class base + X (i - H)
+    const int yt = y0 == 0 ? bs1 : y0, yb = ybot ? -RY : y0;
+    const long XY = X * bs1, V = XY * bs2;
+    const long XYO = XO * (bs1 + 2), VO = XYO * (bs2 + 2), OO = 1 + XO + XYO;
+#define PM_ROW_BASES(XX, XXY, VV, O0, Z, PT, PM, PB)                                           \
+    {                                                                                          \
+        int it_ = t0, im_ = m0, ib_ = b0, zz_ = (Z) - bz * bs2;                                \
+        if (zz_ < 0) {                                                                         \
+            it_ = tm;                                                                          \
+            im_ = mm;                                                                          \
+            ib_ = bm;                                                                          \
+            zz_ += bs2;                                                                        \
+        } else if (zz_ >= bs2) {                                                               \
+            it_ = tp;                                                                          \
+            im_ = mp;                                                                          \
+            ib_ = bp;                                                                          \
+            zz_ -= bs2;                                                                        \
+        }                                                                                      \
+        const long zo_ = (O0) + (XXY) * zz_;                                                   \
+        PT = (long)it_ * (VV) + zo_ + (XX) * yt;                                               \
+        PM = (long)im_ * (VV) + zo_ + (XX) * y0;                                               \
+        PB = (long)ib_ * (VV) + zo_ + (XX) * yb;                                               \
+    }
+    T P[R][NR];   // phi planes by slot: plane Z0 - 3 + s + R m in slot s
+    T Q[RQ][NQ];  // rhs planes: plane Z0 - 3 + s + RQ m in slot s
+    T F[2][RY];   // the tile rows of rhs plane k, kept for the residual at step k + 1 (slot k - Z0 + 3 mod 2)
+    auto loadP = [&](T (&d)[NR], int Z) {
+        long ot, om, ob;
+        PM_ROW_BASES(X, XY, V, 0, Z, ot, om, ob)
+#pragma unroll
+        for (int i = 0; i < NR; i++) { d[i] = pin[(i < H ? ot : (i < RY + H ? om : ob)) + X * (i - H) + lane]; }
+    };
+    auto loadQ = [&](T (&d)[NQ], int Z) {
+        long ot, om, ob;
+        PM_ROW_BASES(X, XY, V, 0, Z, ot, om, ob)
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int i = j + 1;
+            d[j] = rhs[(i < H ? ot : (i < RY + H ? om : ob)) + X * (i - H) + lane];
+        }
+    };
+    const bool edge = lane == 0 || lane == 63;
+    const int side = lane == 63 ? 1 : 0;
+    auto pub_old = [&](const T (&d)[NR], int Z) {
+        if (edge) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) { eo[Z & 1][w][side][j] = d[j + 1]; }
+        }
+    };
+    auto pub_red = [&](const T (&d)[NR], int Z) {
+        if (edge) {
+#pragma unroll
+            for (int b = 0; b < NB; b++) { er[Z & 1][w][side][b] = d[b + H - 1]; }
+        }
+    };
+    const bool lodd = lane & 1;
+    auto is_red = [&](int i, int Z) __attribute__((always_inline)) { return lodd == (((Y0 - H + i + Z) & 1) != 0); };
+    auto relax = [&](T cur, T left, T right, T up, T down, T zp, T zm, T f) __attribute__((always_inline)) {
+        T s = 0;
+        s = s + c0 * cur;
+        s = s + c1 * right;
+        s = s + c2 * left;
+        s = s + c3 * up;
+        s = s + c4 * down;
+        s = s + c5 * zp;
+        s = s + c6 * zm;
+        s = s + f;
+        return cur + lambda * s;
+    };
+    // the x ghost cells of plane k's output rows: the periodic images (as k_gsrb_rows)
+    auto ghost_cols = [&](int k) {
+        if (edge) {
+            long ot, om, ob;
+            PM_ROW_BASES(XO, XYO, VO, OO, k, ot, om, ob)
+            (void)ot;
+            (void)ob;
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const long o = om + XO * r + (side ? 64 : -1);
+                pout[o] = side ? ef[k & 1][wr][0][r] : ef[k & 1][wl][1][r];
+            }
+        }
+    };
+    // the coarse level C (ghost gc): the tile's coarsened cells lie in one
+    // coarse box (box-aligned levels); the element of coarse row Y0 / 2 and
+    // plane 0 of the box's frame under this lane's even column, then rows add
+    // CX, planes CXY
+    long cbase, CX, CXY;
+    {
+        const int cbs0 = C.bs[0], cbs1 = C.bs[1], cbs2 = C.bs[2];
+        const int cx = (w * 32) / cbs0, cy = (Y0 >> 1) / cbs1, cz = (Z0 >> 1) / cbs2;
+        const int cid = C.table[cx + C.nb[0] * (cy + C.nb[1] * cz)];
+        const int g0 = gc.g[0], g1 = gc.g[1], g2 = gc.g[2];
+        CX = cbs0 + 2 * g0;
+        CXY = CX * (cbs1 + 2 * g1);
+        cbase = (long)cid * CXY * (cbs2 + 2 * g2) + g0 + (w * 32 - cx * cbs0 + (lane >> 1)) +
+                CX * (long)(g1 + (Y0 >> 1) - cy * cbs1) + CXY * (long)(g2 - cz * cbs2);
+    }
+    T acc[RY / 2];
+#pragma unroll
+    for (int q = 0; q < RY / 2; q++) { acc[q] = (T)0; }
+    // step k, slot S of plane k: red cells of plane k + 1, black cells of plane k
+    // (written when it is a tile plane), the residual of plane k - 1
+    auto step = [&](auto sc, int k) __attribute__((always_inline)) {
+        constexpr int S = decltype(sc)::value;
+        constexpr int S0 = S, S1 = (S + 1) % R, S2 = (S + 2) % R, S3 = (S + 3) % R;
+        constexpr int SM = (S + R - 1) % R, SMM = (S + R - 2) % R;
+        constexpr int Q0 = S % RQ, Q1 = (S + 1) % RQ, Q2 = (S + 2) % RQ, F0 = S & 1, FM = (S + 1) & 1;
+        __syncthreads();
+        if (k - 1 >= Z0 && k - 1 < Z1) { ghost_cols(k - 1); }
+        // the next planes (clamped to the chunk's last halo planes: a valid address)
+        loadP(P[S3], min(k + 3, Z1 + 2));
+        loadQ(Q[Q2], min(k + 2, Z1 + 1));  // into the slot of plane k - 1 (its tile rows kept in F)
+        // red cells of plane k + 1, rows Y0 - 2 .. Y0 + RY + 1
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int i = j + 1;
+            const T cur = P[S1][i];
+            const T le = eo[(k + 1) & 1][wl][1][j], re = eo[(k + 1) & 1][wr][0][j];
+            const T nv = relax(cur, lane_shift<kShr1>(le, cur), lane_shift<kShl1>(re, cur), P[S1][i + 1],
+                               P[S1][i - 1], P[S2][i], P[S0][i], Q[Q1][j]);
+            P[S1][i] = is_red(i, k + 1) ? nv : cur;
+        }
+        if (k >= Z0 - 1) {
+            // black cells of plane k, rows Y0 - 1 .. Y0 + RY
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int i = b + H - 1;
+                const T cur = P[S0][i];
+                const T le = er[k & 1][wl][1][b], re = er[k & 1][wr][0][b];
+                const T nv = relax(cur, lane_shift<kShr1>(le, cur), lane_shift<kShl1>(re, cur), P[S0][i + 1],
+                                   P[S0][i - 1], P[S1][i], P[SM][i], Q[Q0][i - 1]);
+                P[S0][i] = is_red(i, k) ? cur : nv;
+            }
+#pragma unroll
+            for (int r = 0; r < RY; r++) { F[F0][r] = Q[Q0][r + 2]; }
+            if (k >= Z0 && k < Z1) {
+                long ot, om, ob;
+                PM_ROW_BASES(XO, XYO, VO, OO, k, ot, om, ob)
+                (void)ot;
+                (void)ob;
+#pragma unroll
+                for (int r = 0; r < RY; r++) {
+                    pout[om + XO * r + lane] = P[S0][r + H];
+                    if (edge) { ef[k & 1][w][side][r] = P[S0][r + H]; }
+                }
+            }
+        }
+        if (k - 1 >= Z0 && k - 1 < Z1) {
+            // residual of plane kr = k - 1 (k_residual_nb's order), then its 2 x 2
+            // averages onto the sums of plane pair kr & ~1
+            const int kr = k - 1;
+            T res[RY];
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int i = r + H;
+                const T cur = P[SM][i];
+                const T le = ef[kr & 1][wl][1][r], re = ef[kr & 1][wr][0][r];
+                T sv = 0;
+                sv = sv + F[FM][r];
+                sv = sv + r0 * cur;
+                sv = sv + r1 * lane_shift<kShl1>(re, cur);
+                sv = sv + r2 * lane_shift<kShr1>(le, cur);
+                sv = sv + r3 * P[SM][i + 1];
+                sv = sv + r4 * P[SM][i - 1];
+                sv = sv + r5 * P[S0][i];
+                sv = sv + r6 * P[SMM][i];
+                res[r] = sv;
+            }
+            const bool first = (kr & 1) == 0;
+#pragma unroll
+            for (int q = 0; q < RY / 2; q++) {
+                // the right neighbour's value (lane 63's, which no lane writes, its own)
+                const T a0 = res[2 * q], a1 = lane_shift<kShl1>(res[2 * q], res[2 * q]);
+                const T b0v = res[2 * q + 1], b1v = lane_shift<kShl1>(res[2 * q + 1], res[2 * q + 1]);
+                T tq = first ? (T)0 : acc[q];
+                tq = tq + avg * a0;
+                tq = tq + avg * a1;
+                tq = tq + avg * b0v;
+                tq = tq + avg * b1v;
+                acc[q] = tq;
+            }
+            if (!first && !lodd) {
+#pragma unroll
+                for (int q = 0; q < RY / 2; q++) {
+                    resc[cbase + CX * q + CXY * (kr >> 1)] = acc[q];
+                }
+            }
+        }
+        pub_red(P[S1], k + 1);
+        pub_old(P[S2], k + 2);
+    };
+    // planes Z0 - 3 .. Z0 - 1 (slots 0..2), rhs Z0 - 2 (slot 1); the first step
+    // loads plane Z0 and rhs Z0 - 1
+    loadP(P[0], Z0 - 3);
+    loadP(P[1], Z0 - 2);
+    loadP(P[2], Z0 - 1);
+    loadQ(Q[1], Z0 - 2);
+    pub_old(P[1], Z0 - 2);
+    int k = Z0 - 3;
+    bool done = false;
+    for (;;) {
+        static_for<0, R>([&](auto sc) {
+            if (!done) {
+                step(sc, k);
+                done = ++k > Z1;
+            }
+        });
+        if (done) { break; }
+    }
+#undef PM_ROW_BASES
+}
+
 // ------------------------------------------------------------ small levels
 // All n GSRB sweeps of a one-box periodic level of at most GS_MAX cells in one
 // workgroup: phi lives in LDS for the whole relaxation (periodic neighbours by
@@ -1848,6 +2110,57 @@ static void sweep_rows(const pm_layout* L, int ry, int ghosts, const void* phi_i
     } else {
         launch_rows<T, 8, 1>(L, in_mode, ghosts, pi, (T*)phi_out, (const T*)rhs, c, lt, acc, C, (const T*)coarse, stream);
     }
+}
+
+// The multigrid driver's second pre-smoothing sweep, scratch (ghost 0) -> phi
+// (ghost 1), with the residual restriction onto the coarse right-hand side in
+// the same pass (k_gsrb_rows_res): the bits of pm_level_gsrb_sweep followed by
+// level_restrict_nb, one pass over the level fewer.  PM_ERR_UNSUPPORTED where
+// the level does not qualify or a sweep form other than the automatic one is
+// chosen.
+int level_sweep_restrict(const pm_layout* L, const pm_layout* C, int dtype, const void* phi_in,
+                         const int in_ghost[3], void* phi_out, const int out_ghost[3], const void* rhs,
+                         const int rhs_ghost[3], double dx, void* resc, const int resc_ghost[3], hipStream_t stream)
+{
+    PM_REQUIRE(L && C && phi_in && phi_out && rhs && resc && phi_in != phi_out, "pm_level_gsrb_sweep: bad argument");
+    constexpr int RY = 4;
+    auto unsupported = [](const char* why) {
+        set_error("sweep with residual restriction: %s", why);
+        return PM_ERR_UNSUPPORTED;
+    };
+    if (g_sweep_form != 0) { return unsupported("a sweep form is forced"); }
+    if (L->dim != 3 || rows_ghosts(in_ghost, out_ghost, rhs_ghost, 0) != 1 || !rows_ok(L, RY) || L->bs[1] < 4 ||
+        L->bs[2] < 4 || !box_aligned(L, C)) {
+        return unsupported("level shape");
+    }
+    for (int k = 0; k < 3; k++) {
+        if (!L->periodic[k] || C->domain[k] * 2 != L->domain[k]) { return unsupported("level shape"); }
+    }
+    const G3 gc = g3(resc_ghost);
+    const LayoutDev D = ldev(L), DC = ldev(C);
+    if (gvol(D, g3(out_ghost)) * L->nboxes >= (1L << 31) || gvol(DC, gc) * C->nboxes >= (1L << 31)) {
+        return unsupported("level size");
+    }
+    // FP32 only: the FP64 rings (phi 6 x 10 rows, rhs 3 x 8) exceed the 256
+    // registers a wave of a 512-thread workgroup has, and spill
+    if (dtype != PM_F32) { return unsupported("FP32 only"); }
+    const int zc = rows_zc(L, RY);
+    if (zc % 2) { return unsupported("odd z chunk"); }
+    // relaxation coefficients and weight of sweep_rows, residual ones of level_restrict_nb
+    double lam = -(dx * dx) / (4.0 * L->dim);
+    lam *= 2.0;
+    const double m = -1.0 / (dx * dx), hsqinv = 1. / (dx * dx);
+    const dim3 grid((unsigned)((L->domain[1] / RY) * (L->domain[2] / zc))), block((unsigned)(64 * L->nb[0]));
+    {
+        using T = float;
+        const Lap<T> c = lap_coefs<T>(L->dim, m);
+        Lap<T> cr;
+        for (int j = 0; j < 7; j++) { cr.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)(-hsqinv); }
+        hipLaunchKernelGGL((k_gsrb_rows_res<T, RY>), grid, block, 0, stream, D, (const T*)phi_in, (T*)phi_out,
+                           (const T*)rhs, c, (T)(1.0 * lam), zc, cr, (T)(1.0 / 8), DC, gc, (T*)resc);
+    }
+    PM_CHECK_LAUNCH();
+    return PM_OK;
 }
 
 int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const int in_ghost[3], void* phi_out,

@@ -99,9 +99,12 @@ static void mg_free(pm_mg* mg)
 // such an input directly; otherwise it is written first, as Proto does.
 // acc64 (FP32 level 0 of the refinement): the last sweep adds its result to
 // the FP64 solution instead of storing it (*folded = true), which is then the
-// correction phi += e.
+// correction phi += e.  crhs / restricted (pre-smoothing): where the level
+// qualifies, the last sweep also writes the residual restriction into the
+// coarse right-hand side crhs (*restricted = true).
 static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* tmp, int n, hipStream_t s,
-                 int in_mode = 0, const void* cphi = nullptr, double* acc64 = nullptr, bool* folded = nullptr)
+                 int in_mode = 0, const void* cphi = nullptr, double* acc64 = nullptr, bool* folded = nullptr,
+                 void* crhs = nullptr, bool* restricted = nullptr)
 {
     const auto& L = mg->lv[l];
     auto prepare = [&]() -> int {
@@ -141,11 +144,17 @@ static int relax(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, void* 
                 if (rc == PM_ERR_UNSUPPORTED) { break; }
                 if (rc != PM_OK) { return rc; }
             }
+            if (restricted && it + 2 == n) {
+                rc = level_sweep_restrict(L.layout, mg->lv[l + 1].layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0,
+                                          L.dx, crhs, mg->g0, s);
+                if (rc != PM_OK && rc != PM_ERR_UNSUPPORTED) { return rc; }
+                *restricted = rc == PM_OK;
+            }
             if (acc64 && it + 2 == n) {
                 MG_TRY(level_gsrb_sweep_in(L.layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0, L.dx, 0, nullptr,
                                            nullptr, nullptr, s, acc64));
                 *folded = true;
-            } else {
+            } else if (!(restricted && *restricted)) {
                 MG_TRY(pm_level_gsrb_sweep(L.layout, dtype, tmp, mg->g0, phi, mg->g1, rhs, mg->g0, L.dx, s));
             }
         }
@@ -181,11 +190,15 @@ static int vcycle(pm_mg* mg, int l, int dtype, void* phi, const void* rhs, hipSt
         auto& C = mg->lv[l + 1];
         void* cphi = f32 ? C.phi32 : C.phi;
         void* crhs = f32 ? C.rhs32 : C.rhs;
-        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, pre, s, zin));
         // coarseResidual (51-73): exchange, residual, AvgDown, copy to the coarse
-        // layout; where the level qualifies the halo is read from the
-        // neighbouring boxes instead of an exchange
-        const int rc = level_restrict_nb(L.layout, C.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, crhs, mg->g0, s);
+        // layout; where the level qualifies it is computed by the last
+        // pre-smoothing sweep, else the halo is read from the neighbouring boxes
+        // instead of an exchange
+        bool restricted = false;
+        MG_TRY(relax(mg, l, dtype, phi, rhs, tmp, pre, s, zin, nullptr, nullptr, nullptr, crhs, &restricted));
+        const int rc = restricted ? PM_OK
+                                  : level_restrict_nb(L.layout, C.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, crhs,
+                                                      mg->g0, s);
         if (rc == PM_ERR_UNSUPPORTED) {
             MG_TRY(pm_level_exchange(L.layout, dtype, phi, mg->g1, 1, s));
             MG_TRY(pm_level_restrict_residual(L.layout, C.layout, dtype, phi, mg->g1, rhs, mg->g0, L.dx, crhs, mg->g0,

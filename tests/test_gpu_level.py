@@ -136,14 +136,15 @@ def test_default_options_use_the_fused_sweep():
     assert opt.inner == 1  # one FP32 V-cycle per refinement step
 
 
+@pytest.mark.parametrize("n,bs,nlev", [(64, 32, 3), (128, 64, 4)])
 @pytest.mark.parametrize("inner", [2, 3])
-def test_refinement_inner_cycles(inner):
+def test_refinement_inner_cycles(inner, n, bs, nlev):
     """Two or three FP32 V-cycles per refinement step (each from the previous
     one's correction): the solve reaches the tolerance in fewer refinement
     steps, each step cutting the FP64 residual by more than one cycle does;
     the answer agrees with the plain FP64 solve to the refinement's bound;
-    inner 1 is the default bit for bit; graph replay gives the eager bits."""
-    n, bs, nlev = 64, 32, 3
+    inner 1 is the default bit for bit; graph replay gives the eager bits
+    (64-wide boxes: with the residual restriction in the pre-smoothing sweep)."""
     rhs = util.gaussian_rhs(n, 6, 2)
     rc1, it1, h1, p1 = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs)
     rcd, itd, hd, pd = _solve(n, bs, nlev, 0, 0, precision=1, rhs=rhs, inner=1)
@@ -158,11 +159,18 @@ def test_refinement_inner_cycles(inner):
     assert rcg == 0, pa.lib().pm_last_error()
     assert itg == iti and hg.tobytes() == hi.tobytes() and pg.tobytes() == pi.tobytes()
     # both stop at a relative residual of 1e-10, so they differ from the FP64
-    # solve by what that residual leaves: as much as the one-cycle refinement does
+    # solve by what that residual leaves: as much as the one-cycle refinement
+    # does.  The periodic problem fixes the solution up to a constant, which the
+    # solves leave at different values: errors are taken with the mean removed
     rc64, it64, h64, p64 = _solve(n, bs, nlev, 0, 0, precision=0, rhs=rhs)
-    e1, ei = np.abs(p1 - p64).max(), np.abs(pi - p64).max()
-    say("inner", inner, "error vs FP64", ei, "one cycle", e1, "scale", np.abs(p64).max())
-    assert rc64 == 0 and ei < 1e-7 * np.abs(p64).max() and ei < 10 * e1
+    scale = np.abs(p64).max()
+
+    def err(p):
+        return np.abs((p - p.mean()) - (p64 - p64.mean())).max()
+
+    e1, ei = err(p1), err(pi)
+    say("inner", inner, "error vs FP64 (mean removed)", ei, "one cycle", e1, "scale", scale)
+    assert rc64 == 0 and ei < 1e-7 * scale and ei < 10 * e1 + 1e-9 * scale
 
 
 @pytest.mark.parametrize("dt", [pa.PM_F64, pa.PM_F32])
@@ -250,18 +258,21 @@ def test_row_and_tile_sweeps_same_solve(precision, n, nlev):
     """Whole solves (zero initial guess and the coarse correction's prolongation
     folded into sweeps, FP32 sweeps adding into the FP64 solution) with the
     row-register sweep and with the tile sweep: the same residual history and
-    solution bits.  64^3 boxes: levels 2 and 4 boxes wide, then one box."""
+    solution bits.  64^3 boxes: levels 2 and 4 boxes wide, then one box.  The
+    automatic form (0) computes FP32 levels' residual restriction in the last
+    pre-smoothing sweep; form 2 (the same row sweep) and the tile sweep leave it
+    to the separate residual pass."""
     bs = 64
     rhs = util.gaussian_rhs(n, 5, 8)
     res = {}
     try:
-        for form in (1, 0):
+        for form in (1, 2, 0):
             pa.call("pm_level_sweep_form", form)
             res[form] = _solve(n, bs, nlev, 0, 0, precision=precision, rhs=rhs)
     finally:
         pa.call("pm_level_sweep_form", 0)
     (rc1, it1, h1, p1) = res[1]
-    for form in (0,):
+    for form in (2, 0):
         (rc0, it0, h0, p0) = res[form]
         assert rc1 == 0 and rc0 == 0 and it1 == it0, form
         assert h1.tobytes() == h0.tobytes(), form
