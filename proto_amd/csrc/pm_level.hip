@@ -2123,13 +2123,16 @@ int level_sweep_restrict(const pm_layout* L, const pm_layout* C, int dtype, cons
                          const int rhs_ghost[3], double dx, void* resc, const int resc_ghost[3], hipStream_t stream)
 {
     PM_REQUIRE(L && C && phi_in && phi_out && rhs && resc && phi_in != phi_out, "pm_level_gsrb_sweep: bad argument");
-    constexpr int RY = 4;
+    // 8 rows per thread where the boxes allow (512^3 in 64^3 boxes: 0.503 ms
+    // against 0.644 ms with 4 rows, whose halo rows are a larger share of the
+    // work and loads)
+    const int RY = L->bs[1] % 8 == 0 ? 8 : 4;
     auto unsupported = [](const char* why) {
         set_error("sweep with residual restriction: %s", why);
         return PM_ERR_UNSUPPORTED;
     };
     if (g_sweep_form != 0) { return unsupported("a sweep form is forced"); }
-    if (L->dim != 3 || rows_ghosts(in_ghost, out_ghost, rhs_ghost, 0) != 1 || !rows_ok(L, RY) || L->bs[1] < 4 ||
+    if (L->dim != 3 || rows_ghosts(in_ghost, out_ghost, rhs_ghost, 0) != 1 || !rows_ok(L, 4) || L->bs[1] < 4 ||
         L->bs[2] < 4 || !box_aligned(L, C)) {
         return unsupported("level shape");
     }
@@ -2156,8 +2159,13 @@ int level_sweep_restrict(const pm_layout* L, const pm_layout* C, int dtype, cons
         const Lap<T> c = lap_coefs<T>(L->dim, m);
         Lap<T> cr;
         for (int j = 0; j < 7; j++) { cr.c[j] = (T)(j == 0 ? -2.0 * L->dim : 1.0) * (T)(-hsqinv); }
-        hipLaunchKernelGGL((k_gsrb_rows_res<T, RY>), grid, block, 0, stream, D, (const T*)phi_in, (T*)phi_out,
-                           (const T*)rhs, c, (T)(1.0 * lam), zc, cr, (T)(1.0 / 8), DC, gc, (T*)resc);
+        if (RY == 8) {
+            hipLaunchKernelGGL((k_gsrb_rows_res<T, 8>), grid, block, 0, stream, D, (const T*)phi_in, (T*)phi_out,
+                               (const T*)rhs, c, (T)(1.0 * lam), zc, cr, (T)(1.0 / 8), DC, gc, (T*)resc);
+        } else {
+            hipLaunchKernelGGL((k_gsrb_rows_res<T, 4>), grid, block, 0, stream, D, (const T*)phi_in, (T*)phi_out,
+                               (const T*)rhs, c, (T)(1.0 * lam), zc, cr, (T)(1.0 / 8), DC, gc, (T*)resc);
+        }
     }
     PM_CHECK_LAUNCH();
     return PM_OK;
