@@ -704,6 +704,275 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
 #undef PM_ROW_BASES
 }
 
+// The one-pass sweep of k_gsrb_rows from phi with one ghost layer (IN 0), the
+// zero initial guess (IN 1) or phi plus the constant prolongation of a coarse
+// level with one ghost layer (IN 2) into a ghost-free scratch level, FP32, two
+// cells per lane: a wave holds two neighbouring boxes along x, lane l columns
+// 2 (l mod 32) (cell A) and 2 (l mod 32) + 1 (cell B) of box column 2 w + l / 32,
+// so each load and store moves 8 bytes a lane, as the FP64 sweep's do, for the
+// same instructions.  In a row of a plane one of A, B is red and the other
+// black, which the plane's parity fixes: the steps are unrolled over two turns
+// of the plane ring, so each row updates one named cell with no select.  A's
+// left neighbour is the previous lane's B, B's right neighbour the next lane's
+// A (DPP wave shifts; lanes 31 and 32 meet at the boundary between the wave's
+// two boxes); lanes 0 and 63 take theirs from the neighbouring waves through
+// LDS.  Arithmetic, order and colouring are k_gsrb's; z chunks start on even
+// planes.
+template<int RY, int IN, int NT>
+__global__ __launch_bounds__(256) void k_gsrb_rows2(LayoutDev L, const float* __restrict__ pin,
+                                                    float* __restrict__ pout, const float* __restrict__ rhs,
+                                                    Lap<float> c, float lambda, int zc, LayoutDev C,
+                                                    const float* __restrict__ crs)
+{
+    using T = float;
+    // a pair of cells; rows of phi with one ghost layer put pairs on 4-byte boundaries
+    typedef float V __attribute__((ext_vector_type(2), aligned(4)));
+    constexpr int NR = RY + 4;          // phi rows held: Y0 - 2 .. Y0 + RY + 1
+    constexpr int NQ = RY + 2;          // rhs rows: Y0 - 1 .. Y0 + RY
+    constexpr long X = 64, XP = 66;     // row pitch of the scratch and rhs / of phi
+    constexpr int R = 5;                // plane ring: k - 1 .. k + 3 at step k
+    __shared__ T eo[2][4][2][NQ];  // old values of lane 0's A (side 0) and lane 63's B (side 1)
+    __shared__ T er[2][4][2][RY];  // their red-updated values, tile rows
+    const int lane = threadIdx.x & 63, q = lane & 31, hb = lane >> 5;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nb0 = L.nb[0], nw = nb0 >> 1, nb1 = L.nb[1], nb2 = L.nb[2];
+    const int bs1 = L.bs[1], bs2 = L.bs[2], dom1 = L.domain[1];
+    const int wl = w == 0 ? nw - 1 : w - 1, wr = w == nw - 1 ? 0 : w + 1;
+    const int bxl = 2 * w + hb;  // this lane's box column
+    const int nyt = dom1 / RY;
+    const unsigned t = xcd_remap(blockIdx.x, gridDim.x);
+    const int Y0 = (int)(t % nyt) * RY, Z0 = (int)(t / nyt) * zc, Z1 = Z0 + zc;
+    const int by = Y0 / bs1, bz = Z0 / bs2, y0 = Y0 - by * bs1;
+    const T c0 = c.c[0], c1 = c.c[1], c2 = c.c[2], c3 = c.c[3], c4 = c.c[4], c5 = c.c[5], c6 = c.c[6];
+    const int ytop = y0 == 0 ? -1 : 0, ybot = y0 + RY == bs1 ? 1 : 0;
+    const int* table = L.table;
+    auto box_at = [&](int ys, int zs) __attribute__((always_inline)) {
+        const int yb = (by + ys + nb1) % nb1, zb = (bz + zs + nb2) % nb2;
+        return table[bxl + nb0 * (yb + nb1 * zb)];
+    };
+    // this lane's boxes of the three row classes for the planes below / in / above the box layer
+    const int tm = box_at(ytop, -1), t0 = box_at(ytop, 0), tp = box_at(ytop, 1);
+    const int mm = box_at(0, -1), m0 = box_at(0, 0), mp = box_at(0, 1);
+    const int bm = box_at(ybot, -1), b0 = box_at(ybot, 0), bp = box_at(ybot, 1);
+    const int yt = y0 == 0 ? bs1 : y0, yb = ybot ? -RY : y0;
+    const long XYP = XP * (bs1 + 2), VP = XYP * (bs2 + 2), OP = 1 + XP + XYP;
+    const long XY = X * bs1, VV = XY * bs2;
+    // element of column 2q of the first tile row (row i = 2) of each row class of
+    // plane Z in the lane's own boxes (the two boxes of a wave need not be stored
+    // side by side; with uniform addresses for boxes that are, the sweep measured
+    // 3 % slower); row i of the class adds X (i - 2)
+#define PM_ROW_BASES2(XX, XXY, VB, O0, Z, PT, PM, PB)                                          \
+    {                                                                                          \
+        int it_ = t0, im_ = m0, ib_ = b0, zz_ = (Z) - bz * bs2;                                \
+        if (zz_ < 0) {                                                                         \
+            it_ = tm;                                                                          \
+            im_ = mm;                                                                          \
+            ib_ = bm;                                                                          \
+            zz_ += bs2;                                                                        \
+        } else if (zz_ >= bs2) {                                                               \
+            it_ = tp;                                                                          \
+            im_ = mp;                                                                          \
+            ib_ = bp;                                                                          \
+            zz_ -= bs2;                                                                        \
+        }                                                                                      \
+        const long zo_ = (O0) + (XXY) * zz_ + 2 * q;                                           \
+        PT = (long)it_ * (VB) + zo_ + (XX) * yt;                                               \
+        PM = (long)im_ * (VB) + zo_ + (XX) * y0;                                               \
+        PB = (long)ib_ * (VB) + zo_ + (XX) * yb;                                               \
+    }
+    // IN == 2: coarse rows under the classes' first rows, as k_gsrb_rows, for
+    // this lane's box column (coarse column q of its 32-column block); 32-bit
+    // (coarse levels hold fewer than 2^31 elements)
+    long CX = 0, CXY = 0;
+    int ctm = 0, ct0 = 0, ctp = 0, cmm = 0, cm0 = 0, cmp = 0, cbm = 0, cb0 = 0, cbp = 0;
+    const int dom2 = L.domain[2];
+    if constexpr (IN == 2) {
+        const int cbs0 = C.bs[0], cbs1 = C.bs[1], cbs2 = C.bs[2], cnb0 = C.nb[0], cnb1 = C.nb[1];
+        const int* ctable = C.table;
+        const long cX = cbs0 + 2, cXY = cX * (cbs1 + 2), cvol = cXY * (cbs2 + 2), co0 = 1 + cX + cXY;
+        CX = cX;
+        CXY = cXY;
+        const int yct = (((Y0 - 2 + dom1) % dom1) >> 1) + 1, ycm = Y0 >> 1, ycb = (((Y0 + RY) % dom1) >> 1) - RY / 2;
+        auto cof = [&](int ys, int zs, int yc) {
+            const int ybx = (by + ys + nb1) % nb1, zbx = (bz + zs + nb2) % nb2;
+            const int cx = (bxl * 64 / 2) / cbs0, cy = (ybx * bs1 / 2) / cbs1, cz = (zbx * bs2 / 2) / cbs2;
+            const int cid = ctable[cx + cnb0 * (cy + cnb1 * cz)];
+            return (int)((long)cid * cvol + co0 + (bxl * 32 - cx * cbs0 + q) + cX * (long)(yc - cy * cbs1) -
+                         cXY * (long)(cz * cbs2));
+        };
+        ctm = cof(ytop, -1, yct), ct0 = cof(ytop, 0, yct), ctp = cof(ytop, 1, yct);
+        cmm = cof(0, -1, ycm), cm0 = cof(0, 0, ycm), cmp = cof(0, 1, ycm);
+        cbm = cof(ybot, -1, ycb), cb0 = cof(ybot, 0, ycb), cbp = cof(ybot, 1, ycb);
+    }
+    T A[R][NR], B[R][NR];    // phi planes by slot (plane Z0 - 2 + s + R m in slot s), cells A / B
+    T QA[R][NQ], QB[R][NQ];  // rhs planes, same slots
+    auto loadP = [&](T (&a)[NR], T (&b)[NR], int Z) {
+        if constexpr (IN == 1) {
+#pragma unroll
+            for (int i = 0; i < NR; i++) {
+                a[i] = (T)0;
+                b[i] = (T)0;
+            }
+        } else {
+            long ot, om, ob;
+            PM_ROW_BASES2(XP, XYP, VP, OP, Z, ot, om, ob)
+#pragma unroll
+            for (int i = 0; i < NR; i++) {
+                const V v = *reinterpret_cast<const V*>(pin + (i < 2 ? ot : (i < RY + 2 ? om : ob)) + XP * (i - 2));
+                a[i] = v.x;
+                b[i] = v.y;
+            }
+            if constexpr (IN == 2) {
+                // plus the constant prolongation, rounded as k_prolong_const_box
+                // rounds it; cells A and B lie over the same coarse cell
+                const int zw = Z < 0 ? Z + dom2 : (Z >= dom2 ? Z - dom2 : Z);
+                const long zpart = CXY * (long)(zw >> 1);
+                long ct = ct0, cm = cm0, cb = cb0;
+                if (Z - bz * bs2 < 0) {
+                    ct = ctm;
+                    cm = cmm;
+                    cb = cbm;
+                } else if (Z - bz * bs2 >= bs2) {
+                    ct = ctp;
+                    cm = cmp;
+                    cb = cbp;
+                }
+                ct += zpart;
+                cm += zpart;
+                cb += zpart;
+                const T zero = 0;
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    const T v = crs[(i < 2 ? ct : (i < RY + 2 ? cm : cb)) + CX * ((i - 2) >> 1)];
+                    a[i] = a[i] + (zero + v);
+                    b[i] = b[i] + (zero + v);
+                }
+            }
+        }
+    };
+    auto loadQ = [&](T (&a)[NQ], T (&b)[NQ], int Z) {
+        long ot, om, ob;
+        PM_ROW_BASES2(X, XY, VV, 0, Z, ot, om, ob)
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int i = j + 1;
+            const V v = *reinterpret_cast<const V*>(rhs + (i < 2 ? ot : (i < RY + 2 ? om : ob)) + X * (i - 2));
+            a[j] = v.x;
+            b[j] = v.y;
+        }
+    };
+    auto pub_old = [&](const T (&a)[NR], const T (&b)[NR], int Z) {
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) { eo[Z & 1][w][0][j] = a[j + 1]; }
+        } else if (lane == 63) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) { eo[Z & 1][w][1][j] = b[j + 1]; }
+        }
+    };
+    auto pub_red = [&](const T (&a)[NR], const T (&b)[NR], int Z) {
+        if (lane == 0) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) { er[Z & 1][w][0][r] = a[r + 2]; }
+        } else if (lane == 63) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) { er[Z & 1][w][1][r] = b[r + 2]; }
+        }
+    };
+    auto relax = [&](T cur, T left, T right, T up, T down, T zp, T zm, T f) __attribute__((always_inline)) {
+        T s = 0;
+        s = s + c0 * cur;
+        s = s + c1 * right;
+        s = s + c2 * left;
+        s = s + c3 * up;
+        s = s + c4 * down;
+        s = s + c5 * zp;
+        s = s + c6 * zm;
+        s = s + f;
+        return cur + lambda * s;
+    };
+    // row i of a plane: relax its A cell (left neighbour: the previous lane's B,
+    // lane 0 the left wave's lane 63 B, e) or its B cell (right neighbour: the
+    // next lane's A, lane 63 the right wave's lane 0 A, e)
+    auto cellA = [&](T (&a)[NR], const T (&b)[NR], const T (&ap)[NR], const T (&am)[NR], T f, T e, int i)
+        __attribute__((always_inline)) {
+        a[i] = relax(a[i], lane_shift<kShr1>(e, b[i]), b[i], a[i + 1], a[i - 1], ap[i], am[i], f);
+    };
+    auto cellB = [&](const T (&a)[NR], T (&b)[NR], const T (&bp)[NR], const T (&bm)[NR], T f, T e, int i)
+        __attribute__((always_inline)) {
+        b[i] = relax(b[i], a[i], lane_shift<kShl1>(e, a[i]), b[i + 1], b[i - 1], bp[i], bm[i], f);
+    };
+    // step k with slot S of plane k and ZP the parity of plane k + 1: red cells of
+    // plane k + 1 (rows Y0 - 1 .. Y0 + RY), black cells of plane k, plane k out.
+    // The red cell of row i of plane k + 1 is A when i + ZP is even (x of A and Y0
+    // even); so is the black cell of row i of plane k.
+    auto step = [&](auto sc, int k) __attribute__((always_inline)) {
+        constexpr int U = decltype(sc)::value;  // step within the two ring turns
+        constexpr int S = U % R, ZP = (U + 1) & 1;
+        constexpr int S0 = S, S1 = (S + 1) % R, S2 = (S + 2) % R, S3 = (S + 3) % R, SM = (S + R - 1) % R;
+        __syncthreads();
+        loadP(A[S3], B[S3], min(k + 3, Z1 + 1));
+        loadQ(QA[S2], QB[S2], min(k + 2, Z1));
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const int i = j + 1;
+            if (((i + ZP) & 1) == 0) {
+                cellA(A[S1], B[S1], A[S2], A[S0], QA[S1][j], eo[(k + 1) & 1][wl][1][j], i);
+            } else {
+                cellB(A[S1], B[S1], B[S2], B[S0], QB[S1][j], eo[(k + 1) & 1][wr][0][j], i);
+            }
+        }
+        if (k >= Z0) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int i = r + 2;
+                if (((i + ZP) & 1) == 0) {
+                    cellA(A[S0], B[S0], A[S1], A[SM], QA[S0][r + 1], er[k & 1][wl][1][r], i);
+                } else {
+                    cellB(A[S0], B[S0], B[S1], B[SM], QB[S0][r + 1], er[k & 1][wr][0][r], i);
+                }
+            }
+            long ot, om, ob;
+            PM_ROW_BASES2(X, XY, VV, 0, k, ot, om, ob)
+            (void)ot;
+            (void)ob;
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                V v;
+                v.x = A[S0][r + 2];
+                v.y = B[S0][r + 2];
+                V* o = reinterpret_cast<V*>(pout + om + X * r);
+                if constexpr (NT) {
+                    __builtin_nontemporal_store(v, o);
+                } else {
+                    *o = v;
+                }
+            }
+        }
+        pub_red(A[S1], B[S1], k + 1);
+        pub_old(A[S2], B[S2], k + 2);
+    };
+    // planes Z0 - 2 .. Z0 (slots 0 .. 2), rhs Z0 - 1 (slot 1); the first step
+    // loads plane Z0 + 1 and rhs Z0
+    loadP(A[0], B[0], Z0 - 2);
+    loadP(A[1], B[1], Z0 - 1);
+    loadP(A[2], B[2], min(Z0, Z1 + 1));
+    loadQ(QA[1], QB[1], Z0 - 1);
+    pub_old(A[1], B[1], Z0 - 1);
+    int k = Z0 - 2;
+    bool done = false;
+    for (;;) {
+        static_for<0, 2 * R>([&](auto sc) {
+            if (!done) {
+                step(sc, k);
+                done = ++k >= Z1;
+            }
+        });
+        if (done) { break; }
+    }
+#undef PM_ROW_BASES2
+}
+
 // The second pre-smoothing sweep of a V-cycle level with the residual
 // restriction that follows it (coarseResidual: r = f - L phi, averaged onto the
 // coarse level) in the same pass: k_gsrb_rows for input scratch (no ghosts) ->
@@ -2100,6 +2369,28 @@ static void sweep_rows(const pm_layout* L, int ry, int ghosts, const void* phi_i
     }
     const T* pi = (const T*)(in_mode == 1 ? phi_out : phi_in);
     const T lt = (T)(1.0 * lam);
+    if constexpr (sizeof(T) == 4) {
+        // FP32 into the scratch level: two cells per lane where the level is an
+        // even number of boxes wide (z chunks start on even planes)
+        const int zc = rows_zc(L, 4);
+        if (g_sweep_form == 0 && ry == 4 && ghosts == 10 && !acc && L->nb[0] % 2 == 0 && zc % 2 == 0) {
+            const dim3 grid((unsigned)((L->domain[1] / 4) * (L->domain[2] / zc))), block((unsigned)(32 * L->nb[0]));
+            const LayoutDev D = ldev(L), DC = in_mode == 2 ? ldev(C) : D;
+            const bool nt = g_rows_nt < 0 ? true : g_rows_nt == 1;
+#define PM_ROWS2(I, N)                                                                                          \
+    hipLaunchKernelGGL((k_gsrb_rows2<4, I, N>), grid, block, 0, stream, D, pi, (T*)phi_out, (const T*)rhs, c, lt, zc, \
+                       DC, (const T*)coarse)
+            if (in_mode == 0) {
+                if (nt) { PM_ROWS2(0, 1); } else { PM_ROWS2(0, 0); }
+            } else if (in_mode == 1) {
+                if (nt) { PM_ROWS2(1, 1); } else { PM_ROWS2(1, 0); }
+            } else {
+                if (nt) { PM_ROWS2(2, 1); } else { PM_ROWS2(2, 0); }
+            }
+#undef PM_ROWS2
+            return;
+        }
+    }
     // PF 1: planes loaded one step ahead (two steps ahead measured no faster:
     // 512^3 FP32 sweep 0.387 vs 0.390 ms, back into ghost 1 0.401 vs 0.393 ms,
     // FP64 0.692 vs 0.680 ms)

@@ -215,9 +215,11 @@ def _box_layout(domain, box):
 @pytest.mark.parametrize("domain,box", [((128, 128, 128), (64, 64, 64)), ((64, 96, 64), (64, 32, 32)),
                                         ((256, 64, 128), (64, 16, 64)), ((512, 64, 32), (64, 8, 16))])
 def test_row_sweep_equals_tile_sweep(dt, domain, box):
-    """The row-register sweep (4 and 8 rows per thread) gives the tile sweep's
-    bits: levels one to eight boxes wide, tiles whose halo rows and planes lie
-    in the neighbouring boxes, and several sweeps in a row."""
+    """The row-register sweep (4 and 8 rows per thread, and the automatic form:
+    FP32 sweeps into the scratch level of an even number of boxes across hold
+    two cells per lane) gives the tile sweep's bits: levels one to eight boxes
+    wide, tiles whose halo rows and planes lie in the neighbouring boxes, and
+    several sweeps in a row."""
     t = np.float64 if dt == pa.PM_F64 else np.float32
     h, lows = _box_layout(domain, box)
     nbox = len(lows)
@@ -228,7 +230,7 @@ def test_row_sweep_equals_tile_sweep(dt, domain, box):
     dx = 1.0 / domain[0]
     outs = {}
     try:
-        for form in (1, 2, 3):
+        for form in (1, 2, 3, 0):
             pa.call("pm_level_sweep_form", form)
             a, tmp, rh = dev(phi), zeros(nbox * e0, t), dev(rhs)
             say("row sweep", dt, domain, box, "form", form)
@@ -245,7 +247,7 @@ def test_row_sweep_equals_tile_sweep(dt, domain, box):
         pa.lib().pm_layout_destroy(h)
     valid = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
     xghost = (slice(None), slice(1, -1), slice(1, -1), [0, -1])
-    for form in (2, 3):
+    for form in (2, 3, 0):
         assert outs[form][1].tobytes() == outs[1][1].tobytes()
         assert outs[form][0][valid].tobytes() == outs[1][0][valid].tobytes()
         # the row sweep also writes the x ghost cells of ghost-1 outputs: the periodic images
