@@ -597,7 +597,9 @@ def batch_config4(pa, stream, dtype, reps=10, nboxes=2000, per_box=True):
         for form, waves, fname in ((1, 0, "lds_ring"), (2, 1, "rows4_w1"), (2, 4, "rows4_w4"), (2, 8, "rows4_w8"),
                                    (3, 0, "rows2"), (4, 1, "rows4_ghostw_w1"), (4, 4, "rows4_ghostw_w4"),
                                    (5, 1, "pairs4_w1"), (5, 2, "pairs4_w2"), (5, 4, "pairs4_w4"), (5, 8, "pairs4_w8"),
-                                   (6, 0, "pairs4_ghostw")):
+                                   (6, 0, "pairs4_ghostw"), (7, 0, "quads2"), (8, 0, "quads2_nt")):
+            if form >= 7 and dtype != pa.PM_F32:
+                continue  # the quad sweep is FP32
             B.sweep_form(form, waves)
             msf = time_launches(lambda: B.gsrb(dtype, src.ptr, phi.ptr, rhs.ptr, dx, sp), reps, stream)
             forms[fname] = {"ms": round(msf, 4), "frac": round(gsrb_bytes / (msf * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}

@@ -465,8 +465,10 @@ int pm_batch_apply(pm_batch* batch, const pm_stencil* st, const void* src, void*
 int pm_batch_gsrb(pm_batch* batch, int dtype, const void* phi_in, void* phi_out, const void* rhs, double dx,
                   pm_stream_t stream);
 /* Which kernel runs pm_batch_gsrb (tests and benchmarks compare them; every
- * form gives the same bits): 0 automatic (with 2 ghost layers form 6 where
- * every box is even and 4..128 cells wide and the arrays pair-aligned; else
+ * form gives the same bits): 0 automatic (FP32 with 2 ghost layers form 7
+ * where every box is 4..252 cells wide in multiples of 4, phi 16-byte and rhs
+ * 8-byte aligned; else with 2 ghost layers form 6 where every box is even and
+ * 4..128 cells wide and the arrays pair-aligned; else
  * FP64 the pair sweep (even ghost), then the row sweep where every box is 2..64
  * cells wide; FP32 the row sweep first, then the pair sweep; else the LDS
  * ring), 1 the LDS-ring kernel, 2
@@ -476,11 +478,15 @@ int pm_batch_gsrb(pm_batch* batch, int dtype, const void* phi_in, void* phi_out,
  * phi_in's values) with cached stores, 5 the pair sweep (two cells per lane,
  * 64 / (nx / 2) sub-tiles of 4 rows per wave; either dtype), 6 the pair sweep
  * also writing the ghost pairs of its rows (ghost 2; phi_in's values) with
- * cached stores, so that output rows fill whole cache lines.  waves: workgroup size of
- * forms 0, 2, 4 and 5 in waves (0 automatic, 1, 2, 4 or 8; form 4: 1 or 4;
- * the waves of a workgroup take neighbouring sub-tiles of one box and walk z
- * together); forms 3 and 6 run one wave per workgroup.  No counterpart in the
- * reference (one fixed kernel per box). */
+ * cached stores, so that output rows fill whole cache lines, 7 the quad sweep
+ * (FP32, ghost 2, boxes 4..252 wide in multiples of 4: four cells per lane, a
+ * lane holding a 16-byte quad of a row as stored, the rows' ghost columns
+ * included and written back with phi_in's values; 64 / (nx / 4 + 1) sub-tiles
+ * of 2 rows per wave), 8 the quad sweep with non-temporal stores.  waves:
+ * workgroup size of forms 0, 2, 4 and 5 in waves (0 automatic, 1, 2, 4 or 8;
+ * form 4: 1 or 4; the waves of a workgroup take neighbouring sub-tiles of one
+ * box and walk z together); forms 3, 6, 7 and 8 run one wave per workgroup.
+ * No counterpart in the reference (one fixed kernel per box). */
 int pm_batch_sweep_form(pm_batch* batch, int form, int waves);
 
 /* ------------------------------------------------------ library multigrid */

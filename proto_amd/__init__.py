@@ -330,7 +330,9 @@ class Batch:
         """Kernel of gsrb(): 0 automatic, 1 LDS ring, 2 row sweep, 3 row sweep
         with 2-row sub-tiles, 4 row sweep writing its rows' x ghost cells, 5
         pair sweep (two cells per lane), 6 pair sweep writing its rows' ghost
-        pairs; waves per workgroup for forms 0, 2, 4 and 5 (0 automatic)."""
+        pairs, 7 quad sweep (FP32, ghost 2, four cells per lane), 8 quad sweep
+        with non-temporal stores; waves per workgroup for forms 0, 2, 4 and 5
+        (0 automatic)."""
         call("pm_batch_sweep_form", self.h, int(form), int(waves))
         return self
 

@@ -53,6 +53,11 @@ struct pm_batch {
     bool pairs_ok = false;
     void* d_pitems = nullptr;        // RowItem[] (pair sweep)
     int nitems_pairs = 0, pairs_ry = 0, pairs_wpb = 0;
+    // the four-cells-per-lane FP32 row sweep: ghost 2, every box 4..252 wide in
+    // multiples of 4
+    bool quads_ok = false;
+    void* d_qitems = nullptr;        // RowItem[] (quad sweep, 2-row sub-tiles)
+    int nitems_quads = 0;
 };
 
 namespace pm {
@@ -808,6 +813,199 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_rows2(const BatchBox* __rest
     }
 }
 
+// The row sweep with four cells per lane, FP32, two ghost layers: lane q of a
+// sub-tile holds the 16-byte quad of elements 4q .. 4q + 3 of a row as stored,
+// columns 4q - 2 .. 4q + 1, so each load and store moves 16 bytes a lane, as
+// the FP64 pair sweep's do.  The first and last quads of a row hold its ghost
+// columns (-2, -1 and nx, nx + 1; nx a multiple of 4): they are not updated, serve
+// as the neighbours of columns 0 and nx - 1, and are written back with their
+// input values, so output rows are written whole.  h = nx / 4 + 1 lanes per row,
+// 64 / h sub-tiles of RY rows per wave.  The red cells of a row are elements 0
+// and 2 or 1 and 3, which the row's and plane's parity fix at compile time.
+// Element 0's left neighbour is the previous lane's element 3, element 3's right
+// neighbour the next lane's element 0 (DPP wave shifts; at a sub-tile's ends
+// those elements are ghost columns, never updated).  Arithmetic and order are
+// k_gsrb's.
+template<int RY, int NT>
+__global__ __launch_bounds__(64) void k_batch_rows4(const BatchBox* __restrict__ boxes,
+                                                   const RowItem* __restrict__ items, const float* __restrict__ phi_in,
+                                                   float* __restrict__ phi_out, const float* __restrict__ rhs,
+                                                   BCoefs<float> c, float lambda)
+{
+    using T = float;
+    typedef float V4 __attribute__((ext_vector_type(4)));
+    typedef float V2 __attribute__((ext_vector_type(2)));
+    constexpr int g = 2;
+    constexpr int NR = RY + 4;  // phi rows held: Y0 - 2 .. Y0 + RY + 1
+    constexpr int NQ = RY + 2;  // rhs rows: Y0 - 1 .. Y0 + RY
+    static_assert(RY % 2 == 0, "sub-tiles start on even rows");
+    const RowItem wi = items[blockIdx.x];
+    if (wi.box < 0) { return; }  // padding of a shorter XCD list
+    const BatchBox bx = boxes[wi.box];
+    const int nx = bx.nx, ny = bx.ny, nz = bx.nz, h = (nx >> 2) + 1;
+    const int lane = threadIdx.x & 63, npw = 64 / h;
+    const int pl = lane / h;
+    int p = pl, q = lane - pl * h;
+    const bool live = pl < npw && p < wi.nsub;
+    if (!live) {
+        p = 0;
+        q = 0;
+    }
+    const int Y0 = wi.y0 + p * RY;
+    const int Gx = nx + 2 * g;
+    const long Gxy = (long)Gx * (ny + 2 * g), Vxy = (long)nx * ny;
+    const T c0 = c.c[0], c1 = c.c[1], c2 = c.c[2], c3 = c.c[3], c4 = c.c[4], c5 = c.c[5], c6 = c.c[6];
+    // in-plane byte offsets of the lane's quad in its rows, clamped to the rows
+    // the sweep reads (-1 .. ny); rhs pairs of columns 4q - 2, 4q - 1 and 4q, 4q + 1,
+    // clamped into the row where they are ghost columns (whose cells are not updated)
+    unsigned ro[NR], qa[NQ], qb[NQ];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int y = min(max(Y0 - 2 + i, -1), ny);
+        ro[i] = (unsigned)((4 * q + Gx * (y + g)) * (int)sizeof(T));
+    }
+    const int ca = q == 0 ? 0 : 4 * q - 2, cb = q == h - 1 ? nx - 2 : 4 * q;
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+        const int y = min(max(Y0 - 1 + j, 0), ny - 1);
+        qa[j] = (unsigned)((ca + nx * y) * (int)sizeof(T));
+        qb[j] = (unsigned)((cb + nx * y) * (int)sizeof(T));
+    }
+    // elements 0, 1 are cells where q > 0, elements 2, 3 where q < h - 1
+    const bool lok = q > 0, hok = q < h - 1;
+    unsigned upd = 0;
+#pragma unroll
+    for (int i = 1; i <= RY + 2; i++) {
+        const int y = Y0 - 2 + i;
+        upd |= (live && y >= 0 && y < ny) ? (1u << i) : 0u;
+    }
+    const T* pb = phi_in + bx.goff;
+    T* po = phi_out + bx.goff;
+    const T* qbase = rhs + bx.voff;
+    V4 P[5][NR];  // phi planes by slot (plane z in slot (z + 1) mod 5)
+    V4 Q[5][NQ];  // rhs, same slots (elements 0..3: columns 4q - 2 .. 4q + 1)
+    auto loadP = [&](V4 (&d)[NR], int z) {
+        const char* base = reinterpret_cast<const char*>(pb + Gxy * (long)(min(z, nz) + g));
+#pragma unroll
+        for (int i = 0; i < NR; i++) { d[i] = *reinterpret_cast<const V4*>(base + ro[i]); }
+    };
+    auto loadQ = [&](V4 (&d)[NQ], int z) {
+        const char* base = reinterpret_cast<const char*>(qbase + Vxy * (long)min(z, nz - 1));
+#pragma unroll
+        for (int j = 0; j < NQ; j++) {
+            const V2 a = *reinterpret_cast<const V2*>(base + qa[j]);
+            const V2 b = *reinterpret_cast<const V2*>(base + qb[j]);
+            d[j] = V4{a.x, a.y, b.x, b.y};
+        }
+    };
+    auto relax = [&](T cur, T left, T right, T up, T down, T zp, T zm, T f) __attribute__((always_inline)) {
+        T s = 0;
+        s = s + c0 * cur;
+        s = s + c1 * right;
+        s = s + c2 * left;
+        s = s + c3 * up;
+        s = s + c4 * down;
+        s = s + c5 * zp;
+        s = s + c6 * zm;
+        s = s + f;
+        return cur + lambda * s;
+    };
+    // the cells of row i of a plane with elements E0 and E0 + 2 (E0 0 or 1)
+    auto cells = [&](auto ec, V4 (&d)[NR], const V4 (&dp)[NR], const V4 (&dm)[NR], const V4& f, int i, bool ok)
+        __attribute__((always_inline)) {
+        constexpr int E0 = decltype(ec)::value;
+        const V4 cur = d[i];
+        const bool okl = ok && lok, okh = ok && hok;
+        if constexpr (E0 == 0) {
+            const T l0 = lane_shift<kShr1>(cur.w, cur.w);
+            const T n0 = relax(cur.x, l0, cur.y, d[i + 1].x, d[i - 1].x, dp[i].x, dm[i].x, f.x);
+            const T n2 = relax(cur.z, cur.y, cur.w, d[i + 1].z, d[i - 1].z, dp[i].z, dm[i].z, f.z);
+            d[i].x = okl ? n0 : cur.x;
+            d[i].z = okh ? n2 : cur.z;
+        } else {
+            const T r3 = lane_shift<kShl1>(cur.x, cur.x);
+            const T n1 = relax(cur.y, cur.x, cur.z, d[i + 1].y, d[i - 1].y, dp[i].y, dm[i].y, f.y);
+            const T n3 = relax(cur.w, cur.z, r3, d[i + 1].w, d[i - 1].w, dp[i].w, dm[i].w, f.w);
+            d[i].y = okl ? n1 : cur.y;
+            d[i].w = okh ? n3 : cur.w;
+        }
+    };
+    // step k, slot S of plane k, ZP = parity of plane k + 1: red cells of plane
+    // k + 1 (rows Y0 - 1 .. Y0 + RY), black cells of plane k, plane k out.  The
+    // red cells of row i of plane k + 1 are elements 0 and 2 when i + ZP is even;
+    // so are the black cells of row i of plane k.
+    auto step = [&](auto sc, auto zc, int k) __attribute__((always_inline)) {
+        constexpr int S = decltype(sc)::value, ZP = decltype(zc)::value;
+        constexpr int S0 = S, S1 = (S + 1) % 5, S2 = (S + 2) % 5, S3 = (S + 3) % 5, SM = (S + 4) % 5;
+        loadP(P[S3], k + 3);
+        loadQ(Q[S2], k + 2);
+        if (k + 1 < nz) {
+#pragma unroll
+            for (int j = 0; j < NQ; j++) {
+                const int i = j + 1;
+                const bool ok = (upd >> i) & 1u;
+                if (((i + ZP) & 1) == 0) {
+                    cells(std::integral_constant<int, 0>{}, P[S1], P[S2], P[S0], Q[S1][j], i, ok);
+                } else {
+                    cells(std::integral_constant<int, 1>{}, P[S1], P[S2], P[S0], Q[S1][j], i, ok);
+                }
+            }
+        }
+        if (k >= 0) {
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                const int i = r + 2;
+                const bool ok = (upd >> i) & 1u;
+                if (((i + ZP) & 1) == 0) {
+                    cells(std::integral_constant<int, 0>{}, P[S0], P[S1], P[SM], Q[S0][r + 1], i, ok);
+                } else {
+                    cells(std::integral_constant<int, 1>{}, P[S0], P[S1], P[SM], Q[S0][r + 1], i, ok);
+                }
+            }
+            char* ob = reinterpret_cast<char*>(po + Gxy * (long)(k + g));
+#pragma unroll
+            for (int r = 0; r < RY; r++) {
+                if ((upd >> (r + 2)) & 1u) {
+                    V4* dst = reinterpret_cast<V4*>(ob + ro[r + 2]);
+                    if (NT) {
+                        __builtin_nontemporal_store(P[S0][r + 2], dst);
+                    } else {
+                        *dst = P[S0][r + 2];
+                    }
+                }
+            }
+        }
+    };
+    using std::integral_constant;
+    loadP(P[0], -1);
+    loadP(P[1], 0);
+    loadP(P[2], 1);
+    loadQ(Q[1], 0);
+    int k = -1;
+    for (;;) {
+        step(integral_constant<int, 0>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 1>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 2>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 3>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 4>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 0>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 1>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 2>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 3>{}, integral_constant<int, 0>{}, k);
+        if (++k >= nz) { break; }
+        step(integral_constant<int, 4>{}, integral_constant<int, 1>{}, k);
+        if (++k >= nz) { break; }
+    }
+}
+
 }  // namespace pm
 
 using namespace pm;
@@ -893,8 +1091,9 @@ int build_gsrb_items(pm_batch* B, int rows)
     return upload(&B->d_gitems, items.data(), sizeof(BatchItem) * items.size());
 }
 
-// Row-sweep items: per box, strips of ry rows, 64 / (nx / cells per lane)
-// strips per wave, wpb waves per workgroup (one item each; the waves of an item
+// Row-sweep items: per box, strips of ry rows, 64 / (lanes per row) strips per
+// wave (nx, nx / 2 lanes with one or two cells per lane; nx / 4 + 1 with four,
+// whose rows include the two ghost columns on each side), wpb waves per workgroup (one item each; the waves of an item
 // walk z together, so the halo rows they share are re-read while still
 // cached).  Workgroup w runs on XCD w mod 8, and
 // the strips of a box share halo rows, so every box goes whole to one XCD
@@ -902,7 +1101,7 @@ int build_gsrb_items(pm_batch* B, int rows)
 // by cell count (largest first, to the least loaded), each XCD's list takes
 // its boxes longest z extent first, and the lists are interleaved into the
 // launch order, shorter ones padded with empty items (box -1).
-int build_row_items(pm_batch* B, int ry, bool pairs, int wpb)
+int build_row_items(pm_batch* B, int ry, int cpl, int wpb)
 {
     constexpr int kXcd = 8;
     std::vector<int> order(B->nboxes);
@@ -923,7 +1122,8 @@ int build_row_items(pm_batch* B, int ry, bool pairs, int wpb)
                          [&](int a, int b) { return B->sizes[3 * a + 2] > B->sizes[3 * b + 2]; });
         for (int b : xb[x]) {
             const int nx = B->sizes[3 * b], ny = B->sizes[3 * b + 1];
-            const int P = wpb * (64 / (pairs ? nx / 2 : nx)), nstrips = (ny + ry - 1) / ry;
+            const int lanes = cpl == 4 ? nx / 4 + 1 : nx / cpl;
+            const int P = wpb * (64 / lanes), nstrips = (ny + ry - 1) / ry;
             for (int s = 0; s < nstrips; s += P) { lists[x].push_back(RowItem{b, s * ry, std::min(P, nstrips - s), 0}); }
         }
         nmax = std::max(nmax, lists[x].size());
@@ -931,7 +1131,11 @@ int build_row_items(pm_batch* B, int ry, bool pairs, int wpb)
     std::vector<RowItem> items(nmax * kXcd, RowItem{-1, 0, 0, 0});
     for (int x = 0; x < kXcd; x++)
         for (size_t m = 0; m < lists[x].size(); m++) { items[m * kXcd + x] = lists[x][m]; }
-    if (pairs) {
+    if (cpl == 4) {
+        B->nitems_quads = (int)items.size();
+        return upload(&B->d_qitems, items.data(), sizeof(RowItem) * items.size());
+    }
+    if (cpl == 2) {
         B->nitems_pairs = (int)items.size();
         B->pairs_ry = ry;
         B->pairs_wpb = wpb;
@@ -1000,13 +1204,16 @@ int pm_batch_create(pm_batch** out, int nboxes, const int* sizes_host, int ghost
     if (rc == PM_OK) { rc = build_gsrb_items(B, B->cfg.gsrb_rows); }
     B->rows_ok = true;
     B->pairs_ok = ghost % 2 == 0;
+    B->quads_ok = ghost == 2;
     for (int b = 0; b < nboxes; b++) {
         const int nx = B->sizes[3 * b];
         B->rows_ok = B->rows_ok && nx >= 2 && nx <= 64;
         B->pairs_ok = B->pairs_ok && nx >= 4 && nx <= 128 && nx % 2 == 0;
+        B->quads_ok = B->quads_ok && nx >= 4 && nx <= 252 && nx % 4 == 0;
     }
-    if (rc == PM_OK && B->rows_ok) { rc = build_row_items(B, 4, false, kRowsWaves); }
-    if (rc == PM_OK && B->pairs_ok) { rc = build_row_items(B, 4, true, kPairsWaves); }
+    if (rc == PM_OK && B->rows_ok) { rc = build_row_items(B, 4, 1, kRowsWaves); }
+    if (rc == PM_OK && B->pairs_ok) { rc = build_row_items(B, 4, 2, kPairsWaves); }
+    if (rc == PM_OK && B->quads_ok) { rc = build_row_items(B, 2, 4, 1); }
     if (rc != PM_OK) {
         pm_batch_destroy(B);
         return rc;
@@ -1023,6 +1230,7 @@ int pm_batch_destroy(pm_batch* B)
     hipFree(B->d_gitems);
     hipFree(B->d_ritems);
     hipFree(B->d_pitems);
+    hipFree(B->d_qitems);
     hipFree(B->d_counter);
     delete B;
     return PM_OK;
@@ -1106,12 +1314,16 @@ int pm_batch_apply(pm_batch* B, const pm_stencil* st, const void* src, void* dst
 
 int pm_batch_sweep_form(pm_batch* B, int form, int waves)
 {
-    PM_REQUIRE(B && form >= 0 && form <= 6, "pm_batch_sweep_form: bad form");
+    PM_REQUIRE(B && form >= 0 && form <= 8, "pm_batch_sweep_form: bad form");
     PM_REQUIRE(waves == 0 || waves == 1 || waves == 2 || waves == 4 || waves == 8,
                "pm_batch_sweep_form: waves per workgroup must be 0 (automatic), 1, 2, 4 or 8");
     PM_REQUIRE(form < 2 || form > 4 || B->rows_ok, "pm_batch_sweep_form: the row sweep needs every box 2..64 cells wide");
-    PM_REQUIRE(form < 5 || B->pairs_ok,
+    PM_REQUIRE(form < 5 || form > 6 || B->pairs_ok,
                "pm_batch_sweep_form: the pair sweep needs every box even and 4..128 cells wide, and an even ghost");
+    PM_REQUIRE(form < 7 || B->quads_ok,
+               "pm_batch_sweep_form: the quad sweep needs two ghost layers and every box 4..252 cells wide in "
+               "multiples of 4");
+    PM_REQUIRE(form < 7 || waves <= 1, "pm_batch_sweep_form: forms 7 and 8 run one wave per workgroup");
     // 2-row sub-tiles are measured with one wave per workgroup only
     const bool one = form == 3 || form == 6;
     PM_REQUIRE(!one || waves <= 1, "pm_batch_sweep_form: forms 3 and 6 run one wave per workgroup");
@@ -1120,11 +1332,11 @@ int pm_batch_sweep_form(pm_batch* B, int form, int waves)
     int rc = PM_OK;
     const int ry = form == 3 ? 2 : 4;
     if (B->rows_ok && (B->rows_ry != ry || B->rows_wpb != rw) && form != 1 && form < 5) {
-        rc = build_row_items(B, ry, false, rw);
+        rc = build_row_items(B, ry, 1, rw);
     }
     const int pry = 4;
-    if (rc == PM_OK && B->pairs_ok && (B->pairs_ry != pry || B->pairs_wpb != pw) && (form == 0 || form >= 5)) {
-        rc = build_row_items(B, pry, true, pw);
+    if (rc == PM_OK && B->pairs_ok && (B->pairs_ry != pry || B->pairs_wpb != pw) && (form == 0 || form == 5 || form == 6)) {
+        rc = build_row_items(B, pry, 2, pw);
     }
     if (rc == PM_OK) {
         B->sweep_form = form;
@@ -1148,6 +1360,34 @@ int pm_batch_gsrb(pm_batch* B, int dtype, const void* phi_in, void* phi_out, con
     const size_t al = 2 * (dtype == PM_F64 ? 8 : 4);
     const bool aligned = (uintptr_t)phi_in % al == 0 && (uintptr_t)phi_out % al == 0 && (uintptr_t)rhs % al == 0;
     const int sf = B->sweep_form;
+    // FP32 with two ghost layers: the quad sweep (rows of phi start on 16-byte
+    // quads with their ghost columns, rhs rows on 8-byte pairs)
+    const bool aligned16 = (uintptr_t)phi_in % 16 == 0 && (uintptr_t)phi_out % 16 == 0 && (uintptr_t)rhs % 8 == 0;
+    if (sf >= 7 || (sf == 0 && dtype == PM_F32 && B->quads_ok && aligned16)) {
+        PM_REQUIRE(dtype == PM_F32 && aligned16,
+                   "pm_batch_gsrb: the quad sweep is FP32, phi aligned to 16 bytes and rhs to 8");
+        double lam = -(dx * dx) / (4.0 * 3);
+        lam *= 2.0;
+        const double m = -1.0 / (dx * dx);
+        BCoefs<float> cc;
+        for (int j = 0; j < 7; j++) { cc.c[j] = (float)(j == 0 ? -6.0 : 1.0) * (float)m; }
+        const float lamT = (float)(1.0 * lam);
+        auto run = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3((unsigned)B->nitems_quads), dim3(64), 0, S(stream),
+                               (const BatchBox*)B->d_boxes, (const RowItem*)B->d_qitems, (const float*)phi_in,
+                               (float*)phi_out, (const float*)rhs, cc, lamT);
+        };
+        // 2-row sub-tiles (244 registers, two waves per SIMD): config 4 FP32
+        // 0.308 ms against 0.320 ms with 4 rows (which holds 376 registers, one
+        // wave per SIMD) and 0.375 ms for the pair sweep writing ghost pairs
+        if (sf == 8) {
+            run(k_batch_rows4<2, 1>);
+        } else {
+            run(k_batch_rows4<2, 0>);
+        }
+        PM_CHECK_LAUNCH();
+        return PM_OK;
+    }
     if (sf >= 5) { PM_REQUIRE(aligned, "pm_batch_gsrb: the pair sweep needs arrays aligned to two elements"); }
     const bool pairs_auto = B->pairs_ok && aligned && (B->ghost == 2 || dtype == PM_F64 || !B->rows_ok);
     const bool ghost_pairs = sf == 6 || (sf == 0 && B->ghost == 2);

@@ -185,6 +185,47 @@ def test_batch_pair_sweep_vs_oracle(dt, ghost):
         pa.Batch(SIZES_PAIRS, 1).sweep_form(5)
 
 
+# every box 4..252 wide in multiples of 4: the quad sweep (four cells per lane,
+# FP32, two ghost layers) serves these
+SIZES_QUADS = [(16, 16, 16), (24, 24, 24), (4, 3, 4), (64, 64, 8), (48, 32, 16), (20, 19, 23), (32, 32, 32), (64, 1, 2),
+               (4, 50, 1), (60, 5, 3), (8, 4, 9), (40, 13, 2), (4, 2, 1), (128, 6, 3), (96, 9, 4), (252, 3, 2)]
+
+
+def test_batch_quad_sweep_vs_oracle():
+    """The quad sweep (form 0 = automatic for FP32 with two ghost layers, 7 and 8)
+    against the oracle, box by box, bitwise, and a second sweep back; the rows'
+    ghost columns keep their input values.  FP64, odd multiples of 2 and other
+    ghost widths have no quad sweep."""
+    dt = pa.PM_F32
+    B = pa.Batch(SIZES_QUADS, 2)
+    phi, rhs = inputs(B, dt, 52)
+    dx = 1.0 / 40
+    want1 = oracle_gsrb_batch(B, phi, rhs, dx)
+    want2 = oracle_gsrb_batch(B, want1, rhs, dx)
+    p_in, r_d = dev(phi), dev(rhs)
+    for form in (0, 7, 8):
+        B.sweep_form(form)
+        say("batch quad sweep form", form)
+        p_out = dev(phi)
+        B.gsrb(dt, vp(p_in), vp(p_out), vp(r_d), dx)
+        got = host(p_out)
+        assert first_diff(B, got, want1, False) is None, (form, first_diff(B, got, want1, False))
+        p_back = dev(phi)
+        B.gsrb(dt, vp(p_out), vp(p_back), vp(r_d), dx)
+        assert first_diff(B, host(p_back), want2, False) is None, (form, first_diff(B, host(p_back), want2, False))
+    B.sweep_form(7)
+    f64 = pa.Batch(SIZES_QUADS, 2)
+    f64.sweep_form(7)
+    phi64, rhs64 = inputs(f64, pa.PM_F64, 53)
+    a64, b64, r64 = dev(phi64), dev(phi64), dev(rhs64)
+    with pytest.raises(pa.NativeError, match="quad sweep is FP32"):
+        f64.gsrb(pa.PM_F64, vp(a64), vp(b64), vp(r64), dx)
+    with pytest.raises(pa.NativeError, match="quad sweep"):
+        pa.Batch(SIZES_PAIRS, 2).sweep_form(7)
+    with pytest.raises(pa.NativeError, match="quad sweep"):
+        pa.Batch(SIZES_QUADS, 4).sweep_form(7)
+
+
 def test_batch_gsrb_limits():
     B = make_batch()
     phi, rhs = inputs(B, pa.PM_F64, 13)
