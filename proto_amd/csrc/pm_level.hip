@@ -2500,11 +2500,12 @@ int level_gsrb_sweep_in(const pm_layout* L, int dtype, const void* phi_in, const
     {
         // 4 rows per thread: 512^3 in 64^3 boxes swept in 0.385 ms (FP32) and
         // 0.682 ms (FP64) against 0.392 / 1.24 ms with 8 rows (FP64 spills) and
-        // 0.500 / 0.729 ms with the tile sweep.  Back into a ghost-1 phi FP32 is
-        // faster with 8 rows alone (0.367 against 0.390 ms), but not in the
-        // solve (FP32-IR 0.0318 against 0.0312 s), so 4 rows everywhere
+        // 0.500 / 0.729 ms with the tile sweep.  FP32 back into a ghost-1 phi
+        // (the plain store form) takes 8 rows: 0.367 against 0.390 ms alone, and
+        // the FP32-IR solve 0.0280-0.0281 against 0.0282-0.0283 s on one box
         const int ghosts = rows_ghosts(in_ghost, out_ghost, rhs_ghost, in_mode);
-        const int ry = g_sweep_form == 3 ? 8 : (g_sweep_form == 4 ? 2 : 4);
+        const bool back32 = g_sweep_form == 0 && dtype == PM_F32 && ghosts == 1 && !acc && rows_ok(L, 8);
+        const int ry = g_sweep_form == 3 || back32 ? 8 : (g_sweep_form == 4 ? 2 : 4);
         bool coarse_ok = in_mode != 2 || ghosts == 10;
         if (in_mode == 2) {
             // the prolongation input reads a coarse level with one ghost layer (checked box-aligned above)
