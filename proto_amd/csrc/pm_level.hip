@@ -718,7 +718,7 @@ __global__ __launch_bounds__(512) void k_gsrb_rows(LayoutDev L, const T* __restr
 // two boxes); lanes 0 and 63 take theirs from the neighbouring waves through
 // LDS.  Arithmetic, order and colouring are k_gsrb's; z chunks start on even
 // planes.
-template<int RY, int IN, int NT>
+template<int RY, int IN, int NT, int PF>
 __global__ __launch_bounds__(256) void k_gsrb_rows2(LayoutDev L, const float* __restrict__ pin,
                                                     float* __restrict__ pout, const float* __restrict__ rhs,
                                                     Lap<float> c, float lambda, int zc, LayoutDev C,
@@ -730,7 +730,7 @@ __global__ __launch_bounds__(256) void k_gsrb_rows2(LayoutDev L, const float* __
     constexpr int NR = RY + 4;          // phi rows held: Y0 - 2 .. Y0 + RY + 1
     constexpr int NQ = RY + 2;          // rhs rows: Y0 - 1 .. Y0 + RY
     constexpr long X = 64, XP = 66;     // row pitch of the scratch and rhs / of phi
-    constexpr int R = 5;                // plane ring: k - 1 .. k + 3 at step k
+    constexpr int R = 4 + PF;           // plane ring: k - 1 .. k + 2 + PF at step k (PF planes loaded ahead)
     __shared__ T eo[2][4][2][NQ];  // old values of lane 0's A (side 0) and lane 63's B (side 1)
     __shared__ T er[2][4][2][RY];  // their red-updated values, tile rows
     const int lane = threadIdx.x & 63, q = lane & 31, hb = lane >> 5;
@@ -909,10 +909,11 @@ __global__ __launch_bounds__(256) void k_gsrb_rows2(LayoutDev L, const float* __
     auto step = [&](auto sc, int k) __attribute__((always_inline)) {
         constexpr int U = decltype(sc)::value;  // step within the two ring turns
         constexpr int S = U % R, ZP = (U + 1) & 1;
-        constexpr int S0 = S, S1 = (S + 1) % R, S2 = (S + 2) % R, S3 = (S + 3) % R, SM = (S + R - 1) % R;
+        constexpr int S0 = S, S1 = (S + 1) % R, S2 = (S + 2) % R, SM = (S + R - 1) % R;
+        constexpr int LP = (S + 2 + PF) % R, LQ = (S + 1 + PF) % R;
         __syncthreads();
-        loadP(A[S3], B[S3], min(k + 3, Z1 + 1));
-        loadQ(QA[S2], QB[S2], min(k + 2, Z1));
+        loadP(A[LP], B[LP], min(k + 2 + PF, Z1 + 1));
+        loadQ(QA[LQ], QB[LQ], min(k + 1 + PF, Z1));
 #pragma unroll
         for (int j = 0; j < NQ; j++) {
             const int i = j + 1;
@@ -952,12 +953,16 @@ __global__ __launch_bounds__(256) void k_gsrb_rows2(LayoutDev L, const float* __
         pub_red(A[S1], B[S1], k + 1);
         pub_old(A[S2], B[S2], k + 2);
     };
-    // planes Z0 - 2 .. Z0 (slots 0 .. 2), rhs Z0 - 1 (slot 1); the first step
-    // loads plane Z0 + 1 and rhs Z0
-    loadP(A[0], B[0], Z0 - 2);
-    loadP(A[1], B[1], Z0 - 1);
-    loadP(A[2], B[2], min(Z0, Z1 + 1));
-    loadQ(QA[1], QB[1], Z0 - 1);
+    // planes Z0 - 2 .. Z0 - 1 + PF (slots 0 .. PF + 1), rhs Z0 - 1 .. Z0 - 2 + PF
+    // (slots 1 .. PF); the first step loads plane Z0 + PF and rhs Z0 - 1 + PF
+    static_for<0, PF + 2>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        loadP(A[s], B[s], min(Z0 - 2 + s, Z1 + 1));
+    });
+    static_for<1, PF + 1>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        loadQ(QA[s], QB[s], min(Z0 - 2 + s, Z1));
+    });
     pub_old(A[1], B[1], Z0 - 1);
     int k = Z0 - 2;
     bool done = false;
@@ -2377,8 +2382,9 @@ static void sweep_rows(const pm_layout* L, int ry, int ghosts, const void* phi_i
             const dim3 grid((unsigned)((L->domain[1] / 4) * (L->domain[2] / zc))), block((unsigned)(32 * L->nb[0]));
             const LayoutDev D = ldev(L), DC = in_mode == 2 ? ldev(C) : D;
             const bool nt = g_rows_nt < 0 ? true : g_rows_nt == 1;
+            // planes loaded one step ahead (two: 0.364-0.371 ms, the same)
 #define PM_ROWS2(I, N)                                                                                          \
-    hipLaunchKernelGGL((k_gsrb_rows2<4, I, N>), grid, block, 0, stream, D, pi, (T*)phi_out, (const T*)rhs, c, lt, zc, \
+    hipLaunchKernelGGL((k_gsrb_rows2<4, I, N, 1>), grid, block, 0, stream, D, pi, (T*)phi_out, (const T*)rhs, c, lt, zc, \
                        DC, (const T*)coarse)
             if (in_mode == 0) {
                 if (nt) { PM_ROWS2(0, 1); } else { PM_ROWS2(0, 0); }
