@@ -272,10 +272,15 @@ int pm_level_gsrb_sweep(const pm_layout* L, int dtype, const void* phi_in, const
                         const int out_ghost[3], const void* rhs, const int rhs_ghost[3], double dx,
                         pm_stream_t stream);
 /* Which kernel runs the one-pass sweep here and in the multigrid driver
- * (process-wide; tests and benchmarks compare them): 0 automatic, 1 the LDS
+ * (process-wide; tests and benchmarks compare them): 0 automatic (levels of
+ * 64-wide boxes: the row-register kernel; FP32 into a ghost-free output two
+ * cells per lane where the level is an even number of boxes across, FP32 back
+ * into a ghost-1 phi with 8 rows per thread; in the multigrid driver the FP32
+ * second pre-smoothing sweep also writes the residual restriction), 1 the LDS
  * tile kernel, 2 / 3 / 4 the row-register kernel with 4 / 8 / 2 rows per
- * thread.  pm_level_sweep_tune: the row kernel's largest planes per workgroup
- * (0 automatic) and non-temporal write-out into ghost-free outputs (-1
+ * thread (one cell per lane, restriction in its own pass).
+ * pm_level_sweep_tune: the row kernel's largest planes per workgroup (0
+ * automatic) and non-temporal write-out into ghost-free outputs (-1
  * automatic: on, 0 off, 1 on); every setting gives the same bits. */
 int pm_level_sweep_form(int form);
 int pm_level_sweep_tune(int planes_max, int nt_store);
