@@ -795,8 +795,16 @@ def main():
     if args.only in ("mg_fp64", "mg_ir", "mg_fas"):
         # profiling helper: ONE config-3 solve (512^3 in 64^3 boxes, no graph
         # replay): the fused FP64 V-cycles, the FP32-preconditioned refinement
-        # or Proto's library FAS solver
-        print(json.dumps(mg_one_solve(pa, stream, args.only)))
+        # or Proto's library FAS solver; then a plain 1 GiB device copy that
+        # calibrates the traffic counters of a --pmc run
+        out = mg_one_solve(pa, stream, args.only)
+        from proto_amd.device import DeviceArray
+        n_cal = 1 << 27
+        a_cal, b_cal = DeviceArray.zeros(n_cal), DeviceArray(n_cal)
+        pa.call("pm_memcpy", b_cal.ptr, a_cal.ptr, 8 * n_cal, 2, stream.h)
+        stream.sync()
+        out["copy_bytes"] = 2 * 8 * n_cal
+        print(json.dumps(out))
         return
 
     if args.only in ("serve_batched", "serve_plain"):
